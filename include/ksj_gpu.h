@@ -1,0 +1,170 @@
+/*
+ * ksj_gpu.h -- C ABI of the MI355X (gfx950) reconstruction library libksj_gpu.so.
+ *
+ * Plain pointers and sizes only.  Every d_* argument is device memory on the
+ * current HIP device; `stream` is a hipStream_t (NULL = the default stream).
+ * All work is enqueued on `stream`; a call returns when the work is enqueued,
+ * except where a function states that it synchronises.
+ *
+ * Complex data is interleaved float32 (re, im), i.e. the reference's cf32 /
+ * complex_float32 payload encoding.  Layouts follow the reference type registry
+ * (types/registry.json):
+ *   k-space (Cartesian)   [channel][ky][kx]        channel-major   (ksj.kspace-frame)
+ *   k-space (non-Cart.)   [channel][sample]        channel-major   (ksj.noncartesian-kspace-frame)
+ *   trajectory            [sample][ky, kx] float32 rad/pixel       (ksj.trajectory-frame)
+ *   coil images           [ky][kx][channel]        coil-last       (ksj.coil-image-frame)
+ *   image                 [ky][kx] float32                         (ksj.image-frame)
+ * Reference citations are relative to the KSpaceJet source tree.
+ *
+ * Every int-returning function returns KSJ_GPU_OK or an error code, and on error
+ * ksj_gpu_last_error() describes it (per calling thread).  No function throws.
+ */
+#ifndef KSJ_GPU_H_
+#define KSJ_GPU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KSJ_GPU_OK 0
+#define KSJ_GPU_INVALID_ARGUMENT 1
+#define KSJ_GPU_UNSUPPORTED 2
+#define KSJ_GPU_DEVICE_ERROR 3
+#define KSJ_GPU_RESOURCE_EXHAUSTED 4
+
+#define KSJ_GPU_LAYOUT_COIL_LAST 0  /* [ky][kx][channel], the reference's coil-image-frame */
+#define KSJ_GPU_LAYOUT_COIL_MAJOR 1 /* [channel][ky][kx] */
+
+#define KSJ_GPU_KERNEL_LINEAR 0        /* 2x2 bilinear (the reference radial route) */
+#define KSJ_GPU_KERNEL_KAISER_BESSEL 1
+
+#define KSJ_GPU_DENSITY_NONE 0
+#define KSJ_GPU_DENSITY_RAMP 1  /* hypot(k_row, k_col), radial_analytic_ramp */
+#define KSJ_GPU_DENSITY_GIVEN 2
+
+/* ------------------------------------------------------------------ library */
+
+/* Selects the HIP device for the calling thread and warms the runtime.
+ * No reference counterpart (the reference is CPU-only); the Provider calls it
+ * from execution_context_create with ksj_execution_context_descriptor.device_ordinal
+ * (libs/recon/kspacejet-provider-sdk/include/kspacejet/provider/provider.h:363-371). */
+int ksj_gpu_init(int device_ordinal);
+const char* ksj_gpu_last_error(void);
+/* Waits for `stream`. */
+int ksj_gpu_stream_synchronize(void* stream);
+
+/* ------------------------------------------------------------------ Cartesian */
+
+/* Multi-coil 2-D inverse FFT, 1/cols then 1/rows (Normalization::inverse).
+ * centred = 0: uncentred, as the reference
+ *   (providers/kspacejet-cartesian-recon/src/operators/cartesian_ifft2_coil_images.cpp:375-412
+ *    over ksj::fft::fft2_inplace_with_workspace,
+ *    libs/numerics/kspacejet-fft/include/kspacejet/fft/transforms.hpp:1003-1009);
+ * centred = 1: fftshift(ifft2(ifftshift(k))) per coil.
+ * Replaces the per-channel copy + fft2_inplace_with_workspace + strided coil-last store. */
+size_t ksj_gpu_ifft2_workspace_bytes(uint32_t channels, uint32_t rows, uint32_t cols);
+int ksj_gpu_ifft2_coil_images(const void* d_kspace, void* d_images, uint32_t channels, uint32_t rows, uint32_t cols,
+                              int centred, int layout, void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------ coil combine */
+
+/* out[p] = sqrt(sum_c |x[p][c]|^2) over coil-last images.  Replaces
+ * ksj::stats::root_sum_of_squares_across(CubeView, MatrixView, Dim::dim2)
+ * (libs/numerics/kspacejet-stats/include/kspacejet/stats/norms.hpp:207-219) as called by
+ * providers/kspacejet-coil-combine/src/operators/coil_combine_rss.cpp:308. */
+int ksj_gpu_rss(const void* d_coil_images, float* d_image, uint64_t pixels, uint32_t channels, void* stream);
+
+/* Sensitivity-weighted combine x = sum_c conj(S_c) I_c / sum_c |S_c|^2 (complex output).
+ * The reference only plans this operator
+ * (providers/interfaces/org.kspacejet.coil-combine/coil_combine_adaptive.json: coil_images + sensitivity_maps
+ *  -> ksj.complex-image-frame). */
+int ksj_gpu_sense_combine(const void* d_coil_images, const void* d_sensitivities, void* d_image, uint64_t pixels,
+                          uint32_t channels, void* stream);
+
+/* ------------------------------------------------------------------ non-Cartesian */
+
+/* Analytic ramp DCF hypot(k_row, k_col) in float.  Replaces ksj::nufft::radial_analytic_ramp_dcf2
+ * (libs/numerics/kspacejet-nufft/include/kspacejet/nufft/radial_gridding.hpp:33). */
+int ksj_gpu_ramp_dcf2(const float* d_traj, float* d_dcf, uint64_t samples, void* stream);
+
+/* Exact adjoint NUDFT x(r,c) = sum_s y_s exp(+i(w_r (r - r0) + w_c (c - c0))) per channel, float.
+ * Replaces the per-channel ksj::nufft::direct_nudft2_adjoint loop of
+ * providers/kspacejet-noncartesian-recon/src/operators/noncartesian_adjoint_reconstruct.cpp:346-368
+ * (libs/numerics/kspacejet-nufft/src/eigen/eigen_nufft_direct_nudft.cpp:77-97). */
+size_t ksj_gpu_nudft2_adjoint_workspace_bytes(uint32_t channels, uint32_t rows, uint32_t cols);
+int ksj_gpu_nudft2_adjoint(const float* d_traj, const void* d_samples, void* d_images, uint32_t channels,
+                           uint32_t samples, uint32_t rows, uint32_t cols, double row_origin, double col_origin,
+                           int layout, void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* Exact forward NUDFT in FP64 (complex128 in and out), for simulating k-space.
+ * Counterpart of ksj::nufft::direct_nudft2_forward
+ * (libs/numerics/kspacejet-nufft/src/eigen/eigen_nufft_direct_nudft.cpp:52-74). */
+int ksj_gpu_nudft2_forward_f64(const float* d_traj, const void* d_image, void* d_samples, uint32_t channels,
+                               uint32_t samples, uint32_t rows, uint32_t cols, double row_origin, double col_origin,
+                               void* stream);
+
+/* Golden-angle radial trajectory, FP64 on the device, rounded to float [S][ky, kx] rad/pixel:
+ * spoke p at angle p * pi / phi (111.246 deg), k_j = (j - M/2) / M cycles per pixel.  The
+ * reference receives trajectories from ISMRMRD files and has no generator. */
+int ksj_gpu_golden_angle2(float* d_traj, uint32_t spokes, uint32_t readout, void* stream);
+
+/* ------------------------------------------------------------------ gridding plans */
+
+typedef struct ksj_gpu_plan2_desc {
+  uint32_t struct_size;     /* sizeof(ksj_gpu_plan2_desc) */
+  uint32_t image_rows, image_cols;
+  uint32_t grid_rows, grid_cols;   /* oversampled grid (= image for the linear route) */
+  uint32_t kernel;                 /* KSJ_GPU_KERNEL_* */
+  uint32_t kernel_width;           /* 2 (linear) or 4/6/8 (Kaiser-Bessel) */
+  uint32_t density;                /* KSJ_GPU_DENSITY_* */
+  uint32_t centred;                /* 1: crop the centre of the oversampled grid; 0: unshifted (linear route) */
+  uint32_t samples;
+  uint32_t heavy_threshold;        /* 0 = default */
+  uint32_t reserved0;
+  double beta;                     /* <= 0: Beatty et al. 2005 value for the oversampling */
+  double row_origin, col_origin;   /* image origin; the reference uses ((R-1)/2, (C-1)/2) */
+} ksj_gpu_plan2_desc;
+
+typedef struct ksj_gpu_plan2_info {
+  uint32_t struct_size;
+  uint32_t heavy_cells;
+  uint64_t device_bytes;
+  double beta;
+} ksj_gpu_plan2_info;
+
+typedef struct ksj_gpu_plan2 ksj_gpu_plan2;
+
+/* A plan holds everything derived from one trajectory: FP64 grid coordinates,
+ * the stable sample-to-cell sort, density x origin-phase weights, kernel taps and
+ * deapodization.  The grid-coordinate rule is that of
+ * libs/numerics/kspacejet-nufft/src/eigen/eigen_radial_gridding.cpp:262-277; with
+ * kernel = LINEAR, grid = image and centred = 0 the plan reproduces
+ * ksj::nufft::radial_linear_gridding2_adjoint
+ * (libs/numerics/kspacejet-nufft/include/kspacejet/nufft/radial_gridding.hpp:59). */
+int ksj_gpu_plan2_create(const ksj_gpu_plan2_desc* desc, ksj_gpu_plan2** out_plan);
+/* Builds the tables for a trajectory (d_dcf read only for DENSITY_GIVEN).  Synchronises
+ * `stream`.  Fails with INVALID_ARGUMENT on a non-finite sample or one outside [-pi, pi]. */
+int ksj_gpu_plan2_set_trajectory(ksj_gpu_plan2* plan, const float* d_traj, const float* d_dcf, void* stream);
+int ksj_gpu_plan2_get_info(const ksj_gpu_plan2* plan, ksj_gpu_plan2_info* out_info);
+/* Copies the sort result: d_cell_start[grid_rows*grid_cols + 1] and d_perm[samples]
+ * (sorted position -> sample index), for bit-exact checks of the binning. */
+int ksj_gpu_plan2_export_bins(const ksj_gpu_plan2* plan, uint32_t* d_cell_start, uint32_t* d_perm, void* stream);
+void ksj_gpu_plan2_destroy(ksj_gpu_plan2* plan);
+
+/* Adjoint NUFFT of channel-major samples [C][S]: weight, gather onto the grid,
+ * pruned inverse 2-D FFT, deapodize and crop; writes coil images in `layout`
+ * (if d_images) and/or the RSS image (if d_rss).  For the KB plan this is the
+ * reconstruction that replaces the bilinear route of
+ * providers/kspacejet-noncartesian-recon/src/operators/radial_gridding_reconstruct.cpp:391-413. */
+size_t ksj_gpu_adjoint2_workspace_bytes(const ksj_gpu_plan2* plan, uint32_t channels);
+int ksj_gpu_adjoint2_execute(const ksj_gpu_plan2* plan, const void* d_kspace, uint32_t channels, void* d_images,
+                             int layout, float* d_rss, void* d_workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KSJ_GPU_H_ */

@@ -1,0 +1,102 @@
+"""ctypes binding of libksj_gpu.so (the C ABI declared in include/ksj_gpu.h).
+
+The library is required: importing an op whose library is missing raises, so no
+call can silently fall back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+GPU_LIB_PATH = os.path.join(PKG, "libksj_gpu.so")
+PROVIDER_LIB_PATH = os.path.join(PKG, "libksj_provider_gpu.so")
+
+OK = 0
+LAYOUT_COIL_LAST = 0
+LAYOUT_COIL_MAJOR = 1
+KERNEL_LINEAR = 0
+KERNEL_KAISER_BESSEL = 1
+DENSITY_NONE = 0
+DENSITY_RAMP = 1
+DENSITY_GIVEN = 2
+
+
+class KsjGpuError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"ksj_gpu error {code}: {message}")
+        self.code = code
+
+
+class Plan2Desc(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("image_rows", C.c_uint32), ("image_cols", C.c_uint32),
+        ("grid_rows", C.c_uint32), ("grid_cols", C.c_uint32),
+        ("kernel", C.c_uint32), ("kernel_width", C.c_uint32),
+        ("density", C.c_uint32), ("centred", C.c_uint32),
+        ("samples", C.c_uint32), ("heavy_threshold", C.c_uint32), ("reserved0", C.c_uint32),
+        ("beta", C.c_double), ("row_origin", C.c_double), ("col_origin", C.c_double),
+    ]
+
+
+class Plan2Info(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("heavy_cells", C.c_uint32), ("device_bytes", C.c_uint64),
+                ("beta", C.c_double)]
+
+
+_vp = C.c_void_p
+_u32 = C.c_uint32
+_u64 = C.c_uint64
+_sz = C.c_size_t
+_int = C.c_int
+_dbl = C.c_double
+
+# name -> (restype, argtypes); every symbol include/ksj_gpu.h declares.
+SIGNATURES = {
+    "ksj_gpu_init": (_int, [_int]),
+    "ksj_gpu_last_error": (C.c_char_p, []),
+    "ksj_gpu_stream_synchronize": (_int, [_vp]),
+    "ksj_gpu_ifft2_workspace_bytes": (_sz, [_u32, _u32, _u32]),
+    "ksj_gpu_ifft2_coil_images": (_int, [_vp, _vp, _u32, _u32, _u32, _int, _int, _vp, _sz, _vp]),
+    "ksj_gpu_rss": (_int, [_vp, _vp, _u64, _u32, _vp]),
+    "ksj_gpu_sense_combine": (_int, [_vp, _vp, _vp, _u64, _u32, _vp]),
+    "ksj_gpu_ramp_dcf2": (_int, [_vp, _vp, _u64, _vp]),
+    "ksj_gpu_nudft2_adjoint_workspace_bytes": (_sz, [_u32, _u32, _u32]),
+    "ksj_gpu_nudft2_adjoint": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _u32, _dbl, _dbl, _int, _vp, _sz, _vp]),
+    "ksj_gpu_nudft2_forward_f64": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _u32, _dbl, _dbl, _vp]),
+    "ksj_gpu_golden_angle2": (_int, [_vp, _u32, _u32, _vp]),
+    "ksj_gpu_plan2_create": (_int, [C.POINTER(Plan2Desc), C.POINTER(_vp)]),
+    "ksj_gpu_plan2_set_trajectory": (_int, [_vp, _vp, _vp, _vp]),
+    "ksj_gpu_plan2_get_info": (_int, [_vp, C.POINTER(Plan2Info)]),
+    "ksj_gpu_plan2_export_bins": (_int, [_vp, _vp, _vp, _vp]),
+    "ksj_gpu_plan2_destroy": (None, [_vp]),
+    "ksj_gpu_adjoint2_workspace_bytes": (_sz, [_vp, _u32]),
+    "ksj_gpu_adjoint2_execute": (_int, [_vp, _vp, _u32, _vp, _int, _vp, _vp, _sz, _vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libksj_gpu.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(GPU_LIB_PATH):
+            raise ImportError(f"{GPU_LIB_PATH} is missing; run `python -m kspacejet_amd.build`")
+        lib = C.CDLL(GPU_LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(code: int) -> None:
+    if code != OK:
+        raise KsjGpuError(code, load().ksj_gpu_last_error().decode(errors="replace"))
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args))

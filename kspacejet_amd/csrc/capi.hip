@@ -1,0 +1,227 @@
+// extern "C" boundary of libksj_gpu.so: argument checks, exception -> status mapping.
+#include "../../include/ksj_gpu.h"
+
+#include "combine.hpp"
+#include "fft.hpp"
+#include "grid.hpp"
+#include "nudft.hpp"
+#include "plan.hpp"
+#include "recon.hpp"
+
+#include <cstring>
+#include <new>
+#include <string>
+
+struct ksj_gpu_plan2 {
+  ksj::gpu::Plan2* impl;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& message) {
+  g_last_error = message;
+  return code;
+}
+
+template <typename F> int guarded(F&& f) {
+  try {
+    f();
+    return KSJ_GPU_OK;
+  } catch (const ksj::gpu::Error& e) {
+    const int code = e.code == 3 ? KSJ_GPU_DEVICE_ERROR : KSJ_GPU_INVALID_ARGUMENT;
+    return fail(code, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(KSJ_GPU_RESOURCE_EXHAUSTED, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(KSJ_GPU_DEVICE_ERROR, e.what());
+  } catch (...) {
+    return fail(KSJ_GPU_DEVICE_ERROR, "unknown error");
+  }
+}
+
+hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+}  // namespace
+
+extern "C" {
+
+int ksj_gpu_init(int device_ordinal) {
+  return guarded([&] {
+    int count = 0;
+    KSJ_HIP(hipGetDeviceCount(&count));
+    ksj::gpu::require(device_ordinal >= 0 && device_ordinal < count, "device ordinal out of range");
+    KSJ_HIP(hipSetDevice(device_ordinal));
+    KSJ_HIP(hipFree(nullptr));
+  });
+}
+
+const char* ksj_gpu_last_error(void) { return g_last_error.c_str(); }
+
+int ksj_gpu_stream_synchronize(void* stream) {
+  return guarded([&] { KSJ_HIP(hipStreamSynchronize(as_stream(stream))); });
+}
+
+size_t ksj_gpu_ifft2_workspace_bytes(uint32_t channels, uint32_t rows, uint32_t cols) {
+  return ksj::gpu::ifft2_workspace_bytes(channels, rows, cols);
+}
+
+int ksj_gpu_ifft2_coil_images(const void* d_kspace, void* d_images, uint32_t channels, uint32_t rows, uint32_t cols,
+                              int centred, int layout, void* d_workspace, size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(d_kspace && d_images && d_workspace, "null device pointer");
+    ksj::gpu::ifft2_coil_images(static_cast<const float2*>(d_kspace), static_cast<float2*>(d_images), channels, rows,
+                                cols, centred != 0, layout, d_workspace, workspace_bytes, as_stream(stream));
+  });
+}
+
+int ksj_gpu_rss(const void* d_coil_images, float* d_image, uint64_t pixels, uint32_t channels, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(d_coil_images && d_image, "null device pointer");
+    ksj::gpu::rss_coil_last(static_cast<const float2*>(d_coil_images), d_image, pixels, channels, as_stream(stream));
+  });
+}
+
+int ksj_gpu_sense_combine(const void* d_coil_images, const void* d_sensitivities, void* d_image, uint64_t pixels,
+                          uint32_t channels, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(d_coil_images && d_sensitivities && d_image, "null device pointer");
+    ksj::gpu::sense_combine(static_cast<const float2*>(d_coil_images), static_cast<const float2*>(d_sensitivities),
+                            static_cast<float2*>(d_image), pixels, channels, as_stream(stream));
+  });
+}
+
+int ksj_gpu_ramp_dcf2(const float* d_traj, float* d_dcf, uint64_t samples, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(d_traj && d_dcf, "null device pointer");
+    ksj::gpu::ramp_dcf2(d_traj, d_dcf, samples, as_stream(stream));
+  });
+}
+
+size_t ksj_gpu_nudft2_adjoint_workspace_bytes(uint32_t channels, uint32_t rows, uint32_t cols) {
+  return ksj::gpu::nudft2_adjoint_workspace_bytes(channels, rows, cols);
+}
+
+int ksj_gpu_nudft2_adjoint(const float* d_traj, const void* d_samples, void* d_images, uint32_t channels,
+                           uint32_t samples, uint32_t rows, uint32_t cols, double row_origin, double col_origin,
+                           int layout, void* d_workspace, size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(d_traj && d_samples && d_images, "null device pointer");
+    ksj::gpu::require(layout == KSJ_GPU_LAYOUT_COIL_MAJOR || layout == KSJ_GPU_LAYOUT_COIL_LAST, "unknown layout");
+    if (layout == KSJ_GPU_LAYOUT_COIL_MAJOR) {
+      ksj::gpu::nudft2_adjoint(d_traj, static_cast<const float2*>(d_samples), static_cast<float2*>(d_images), samples,
+                               channels, rows, cols, row_origin, col_origin, as_stream(stream));
+      return;
+    }
+    ksj::gpu::require(d_workspace && workspace_bytes >= ksj::gpu::nudft2_adjoint_workspace_bytes(channels, rows, cols),
+                      "direct adjoint workspace too small");
+    auto* tmp = static_cast<float2*>(d_workspace);
+    ksj::gpu::nudft2_adjoint(d_traj, static_cast<const float2*>(d_samples), tmp, samples, channels, rows, cols,
+                             row_origin, col_origin, as_stream(stream));
+    ksj::gpu::coil_major_to_last(tmp, static_cast<float2*>(d_images), nullptr, static_cast<uint64_t>(rows) * cols,
+                                 channels, as_stream(stream));
+  });
+}
+
+int ksj_gpu_nudft2_forward_f64(const float* d_traj, const void* d_image, void* d_samples, uint32_t channels,
+                               uint32_t samples, uint32_t rows, uint32_t cols, double row_origin, double col_origin,
+                               void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(d_traj && d_image && d_samples, "null device pointer");
+    ksj::gpu::nudft2_forward_f64(d_traj, static_cast<const double2*>(d_image), static_cast<double2*>(d_samples),
+                                 samples, channels, rows, cols, row_origin, col_origin, as_stream(stream));
+  });
+}
+
+int ksj_gpu_golden_angle2(float* d_traj, uint32_t spokes, uint32_t readout, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(d_traj != nullptr, "null device pointer");
+    ksj::gpu::golden_angle2(d_traj, spokes, readout, as_stream(stream));
+  });
+}
+
+int ksj_gpu_plan2_create(const ksj_gpu_plan2_desc* desc, ksj_gpu_plan2** out_plan) {
+  return guarded([&] {
+    ksj::gpu::require(desc && out_plan, "null argument");
+    ksj::gpu::require(desc->struct_size >= sizeof(ksj_gpu_plan2_desc), "plan descriptor too small");
+    *out_plan = nullptr;
+    ksj::gpu::Plan2Desc d;
+    d.image_rows = desc->image_rows;
+    d.image_cols = desc->image_cols;
+    d.grid_rows = desc->grid_rows;
+    d.grid_cols = desc->grid_cols;
+    d.width = desc->kernel_width;
+    d.kernel = desc->kernel;
+    d.beta = desc->beta;
+    d.row_origin = desc->row_origin;
+    d.col_origin = desc->col_origin;
+    d.density = desc->density;
+    d.centred = desc->centred;
+    d.samples = desc->samples;
+    d.heavy_threshold = desc->heavy_threshold == 0 ? 128u : desc->heavy_threshold;
+    ksj::gpu::require(d.density <= KSJ_GPU_DENSITY_GIVEN, "unknown density mode");
+    auto* handle = new ksj_gpu_plan2{nullptr};
+    try {
+      handle->impl = ksj::gpu::plan2_create(d);
+    } catch (...) {
+      delete handle;
+      throw;
+    }
+    *out_plan = handle;
+  });
+}
+
+int ksj_gpu_plan2_set_trajectory(ksj_gpu_plan2* plan, const float* d_traj, const float* d_dcf, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(plan && plan->impl && d_traj, "null argument");
+    ksj::gpu::plan2_set_trajectory(plan->impl, d_traj, d_dcf, as_stream(stream));
+  });
+}
+
+int ksj_gpu_plan2_get_info(const ksj_gpu_plan2* plan, ksj_gpu_plan2_info* out_info) {
+  return guarded([&] {
+    ksj::gpu::require(plan && plan->impl && out_info, "null argument");
+    ksj::gpu::require(out_info->struct_size >= sizeof(ksj_gpu_plan2_info), "info struct too small");
+    out_info->heavy_cells = plan->impl->heavy_count;
+    out_info->device_bytes = plan->impl->device_bytes;
+    out_info->beta = plan->impl->d.beta;
+  });
+}
+
+int ksj_gpu_plan2_export_bins(const ksj_gpu_plan2* plan, uint32_t* d_cell_start, uint32_t* d_perm, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(plan && plan->impl && plan->impl->has_trajectory, "plan has no trajectory");
+    const auto* p = plan->impl;
+    if (d_cell_start)
+      KSJ_HIP(hipMemcpyAsync(d_cell_start, p->cell_start, (p->nbins + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                             as_stream(stream)));
+    if (d_perm)
+      KSJ_HIP(hipMemcpyAsync(d_perm, p->src, static_cast<size_t>(p->d.samples) * sizeof(uint32_t),
+                             hipMemcpyDeviceToDevice, as_stream(stream)));
+  });
+}
+
+void ksj_gpu_plan2_destroy(ksj_gpu_plan2* plan) {
+  if (!plan) return;
+  ksj::gpu::plan2_destroy(plan->impl);
+  delete plan;
+}
+
+size_t ksj_gpu_adjoint2_workspace_bytes(const ksj_gpu_plan2* plan, uint32_t channels) {
+  if (!plan || !plan->impl) return 0;
+  return ksj::gpu::adjoint2_workspace_bytes(*plan->impl, channels);
+}
+
+int ksj_gpu_adjoint2_execute(const ksj_gpu_plan2* plan, const void* d_kspace, uint32_t channels, void* d_images,
+                             int layout, float* d_rss, void* d_workspace, size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(plan && plan->impl && d_kspace && d_workspace, "null argument");
+    ksj::gpu::require(d_images || d_rss, "adjoint needs an output");
+    ksj::gpu::adjoint2_execute(*plan->impl, static_cast<const float2*>(d_kspace), channels,
+                               static_cast<float2*>(d_images), layout, d_rss, d_workspace, workspace_bytes,
+                               as_stream(stream));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,22 @@
+// Coil combination and layout kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace ksj::gpu {
+
+// Root sum of squares over the contiguous coil axis of coil-last images [P][C]:
+// out[p] = sqrt(sum_c |x[p][c]|^2), accumulated in coil order in float
+// (reductions.hpp:973-983 then eigen_stats_common.hpp:376-381).
+void rss_coil_last(const float2* d_in, float* d_out, uint64_t pixels, uint32_t coils, hipStream_t stream);
+
+// Coil-major planes [C][P] -> optional coil-last [P][C] and optional RSS [P],
+// staged through LDS so both the reads and the writes are contiguous.
+void coil_major_to_last(const float2* d_in, float2* d_coil_last, float* d_rss, uint64_t pixels, uint32_t coils,
+                        hipStream_t stream);
+
+// x[p] = sum_c conj(S[p][c]) I[p][c] / sum_c |S[p][c]|^2, 0 where the denominator is 0.
+void sense_combine(const float2* d_images, const float2* d_sens, float2* d_out, uint64_t pixels, uint32_t coils,
+                   hipStream_t stream);
+
+}  // namespace ksj::gpu

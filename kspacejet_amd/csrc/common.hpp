@@ -1,0 +1,46 @@
+// Shared device/host helpers for the KSpaceJet MI355X (gfx950) kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace ksj::gpu {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 6.28318530717958647692;
+
+struct cf {
+  float x, y;
+};
+
+__host__ __device__ inline float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// Thrown by host code on a HIP error or a violated precondition; the C ABI
+// layer turns it into a status code and a last-error message.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error(3, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+inline void require(bool ok, const char* what) {
+  if (!ok) throw Error(1, what);
+}
+
+#define KSJ_HIP(call) ::ksj::gpu::check((call), #call)
+#define KSJ_LAUNCH_CHECK() ::ksj::gpu::check(hipGetLastError(), "kernel launch")
+
+inline unsigned ceil_div(uint64_t a, uint64_t b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+}  // namespace ksj::gpu

@@ -1,0 +1,341 @@
+// LDS Stockham FFT kernels for gfx950 (wave64).
+//
+// One workgroup transforms `lpb` lines of length N held in LDS.  Each line is
+// served by T threads; every pass reads its radix-R butterfly inputs into
+// registers, barriers, writes the autosorted outputs back in place and
+// barriers again (Stockham autosort: output is in natural order, no bit
+// reversal).  Twiddles come from an FP64-generated table staged in LDS.
+// Loads and stores are mapped so that consecutive threads touch consecutive
+// addresses in global memory for both contiguous rows and strided columns.
+#include "fft.hpp"
+
+#include <map>
+#include <mutex>
+#include <utility>
+
+namespace ksj::gpu {
+namespace {
+
+// ----------------------------------------------------------------------------- radix schedules
+// T = threads per line.  Every pass satisfies (N / R) % T == 0, so each thread
+// owns exactly N / (R T) butterflies in every pass.
+template <int N> struct Shape;
+#define KSJ_SHAPE(NN, TT, ...)                         \
+  template <> struct Shape<NN> {                       \
+    static constexpr int T = TT;                       \
+    static constexpr int R[] = {__VA_ARGS__};          \
+    static constexpr int P = sizeof(R) / sizeof(int);  \
+  };
+KSJ_SHAPE(64, 8, 8, 8)
+KSJ_SHAPE(128, 16, 8, 4, 4)
+KSJ_SHAPE(192, 16, 4, 4, 4, 3)
+KSJ_SHAPE(256, 32, 8, 8, 4)
+KSJ_SHAPE(320, 16, 4, 4, 4, 5)
+KSJ_SHAPE(384, 16, 4, 4, 8, 3)
+KSJ_SHAPE(512, 64, 8, 8, 8)
+KSJ_SHAPE(640, 32, 4, 4, 4, 2, 5)
+KSJ_SHAPE(768, 32, 8, 8, 4, 3)
+KSJ_SHAPE(1024, 64, 8, 8, 4, 4)
+#undef KSJ_SHAPE
+
+__device__ inline float2 mul_i(float2 z, bool inv) {  // z * (+i) for inverse, z * (-i) for forward
+  return inv ? make_float2(-z.y, z.x) : make_float2(z.y, -z.x);
+}
+
+template <int R, bool INV> struct Dft;
+
+template <bool INV> struct Dft<2, INV> {
+  __device__ static inline void run(float2* v) {
+    const float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <bool INV> struct Dft<4, INV> {
+  __device__ static inline void run(float2* v) {
+    const float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    const float2 b0 = cadd(v[1], v[3]), b1 = mul_i(csub(v[1], v[3]), INV);
+    v[0] = cadd(a0, b0);
+    v[2] = csub(a0, b0);
+    v[1] = cadd(a1, b1);
+    v[3] = csub(a1, b1);
+  }
+};
+
+template <bool INV> struct Dft<8, INV> {
+  __device__ static inline void run(float2* v) {
+    constexpr float h = 0.70710678118654752440f;
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4, INV>::run(e);
+    Dft<4, INV>::run(o);
+    const float s = INV ? 1.0f : -1.0f;
+    const float2 w1 = make_float2(h, s * h), w3 = make_float2(-h, s * h);
+    const float2 t0 = o[0], t1 = cmul(o[1], w1), t2 = mul_i(o[2], INV), t3 = cmul(o[3], w3);
+    v[0] = cadd(e[0], t0);
+    v[4] = csub(e[0], t0);
+    v[1] = cadd(e[1], t1);
+    v[5] = csub(e[1], t1);
+    v[2] = cadd(e[2], t2);
+    v[6] = csub(e[2], t2);
+    v[3] = cadd(e[3], t3);
+    v[7] = csub(e[3], t3);
+  }
+};
+
+template <bool INV> struct Dft<3, INV> {
+  __device__ static inline void run(float2* v) {
+    constexpr float r3 = 0.86602540378443864676f;
+    const float2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+    const float2 m = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+    const float2 n = mul_i(cscale(t2, r3), INV);
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(m, n);
+    v[2] = csub(m, n);
+  }
+};
+
+template <bool INV> struct Dft<5, INV> {
+  __device__ static inline void run(float2* v) {
+    constexpr float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+    constexpr float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+    const float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    const float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    const float2 a1 = make_float2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+    const float2 a2 = make_float2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+    const float2 b1 = mul_i(make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y), INV);
+    const float2 b2 = mul_i(make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y), INV);
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = cadd(a1, b1);
+    v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[3] = csub(a2, b2);
+  }
+};
+
+template <int N, bool INV, int P, int Ns>
+__device__ inline void passes(float2* line, const float2* tw, int t) {
+  if constexpr (P < Shape<N>::P) {
+    constexpr int T = Shape<N>::T;
+    constexpr int R = Shape<N>::R[P];
+    constexpr int B = N / R / T;
+    static_assert((N / R) % T == 0, "radix schedule must split evenly over the line's threads");
+    float2 v[B][R];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int j = t + b * T;
+      const int k = j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = line[j + r * (N / R)];
+      if constexpr (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[k * r * (N / (Ns * R))];
+          if (!INV) w.y = -w.y;
+          v[b][r] = cmul(v[b][r], w);
+        }
+      }
+      Dft<R, INV>::run(v[b]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int j = t + b * T;
+      const int k = j % Ns;
+      const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) line[base + r * Ns] = v[b][r];
+    }
+    __syncthreads();
+    passes<N, INV, P + 1, Ns * R>(line, tw, t);
+  }
+}
+
+__device__ inline uint64_t addr(const FftLines& L, uint64_t line, uint32_t e) {
+  const uint64_t p = line / L.lines, q = line % L.lines;
+  return p * L.plane_stride + q * L.line_stride + static_cast<uint64_t>(e) * L.elem_stride;
+}
+
+// Loads `lpb` lines into LDS (line stride ls), rotated by in_offset.
+__device__ inline void load_lines(const FftArgs& a, float2* buf, int ls, int n, int lpb, uint64_t first,
+                                  uint64_t total) {
+  const int count = lpb * n;
+  for (int idx = threadIdx.x; idx < count; idx += blockDim.x) {
+    int ly, e;
+    if (a.columns) {
+      ly = idx % lpb;
+      e = idx / lpb;
+    } else {
+      ly = idx / n;
+      e = idx % n;
+    }
+    const uint64_t line = first + ly;
+    float2 v = make_float2(0.f, 0.f);
+    if (line < total) {
+      uint32_t src = static_cast<uint32_t>(e) + a.in_offset;
+      if (src >= static_cast<uint32_t>(n)) src -= n;
+      v = a.in[addr(a.src, line, src)];
+    }
+    buf[ly * ls + e] = v;
+  }
+}
+
+__device__ inline void store_lines(const FftArgs& a, const float2* buf, int ls, int n, int lpb, uint64_t first,
+                                   uint64_t total) {
+  const int count = lpb * static_cast<int>(a.n_out);
+  for (int idx = threadIdx.x; idx < count; idx += blockDim.x) {
+    int ly, e;
+    if (a.columns) {
+      ly = idx % lpb;
+      e = idx / lpb;
+    } else {
+      ly = idx / a.n_out;
+      e = idx % a.n_out;
+    }
+    const uint64_t line = first + ly;
+    if (line >= total) continue;
+    uint32_t bin = static_cast<uint32_t>(e) + a.out_offset;
+    bin %= static_cast<uint32_t>(n);
+    float w = a.scale;
+    if (a.out_weight) w *= a.out_weight[e];
+    if (a.line_weight) w *= a.line_weight[line % a.src.lines];
+    a.out[addr(a.dst, line, e)] = cscale(buf[ly * ls + bin], w);
+  }
+}
+
+template <int N, bool INV>
+__global__ __launch_bounds__(1024) void fft_lds_kernel(FftArgs a, const float2* __restrict__ tw_global, int lpb) {
+  constexpr int T = Shape<N>::T;
+  constexpr int LS = N + 1;  // pad each LDS line by one element to spread banks across lines
+  extern __shared__ float2 smem[];
+  float2* tw = smem;
+  float2* buf = smem + N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = tw_global[i];
+  const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
+  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * lpb;
+  load_lines(a, buf, LS, N, lpb, first, total);
+  __syncthreads();
+  const int ly = threadIdx.x / T, t = threadIdx.x % T;
+  passes<N, INV, 0, 1>(buf + ly * LS, tw, t);
+  store_lines(a, buf, LS, N, lpb, first, total);
+}
+
+// Any length: direct DFT per output bin against the twiddle table (used for
+// lengths without a radix schedule, e.g. odd or small reference test sizes).
+__global__ __launch_bounds__(256) void dft_direct_kernel(FftArgs a, const float2* __restrict__ tw_global, int lpb) {
+  extern __shared__ float2 smem[];
+  const int n = static_cast<int>(a.n);
+  const int ls = n + 1;
+  float2* tw = smem;
+  float2* buf = smem + n;
+  float2* res = buf + lpb * ls;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = tw_global[i];
+  const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
+  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * lpb;
+  load_lines(a, buf, ls, n, lpb, first, total);
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < lpb * n; idx += blockDim.x) {
+    const int ly = idx / n, k = idx % n;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int j = 0; j < n; ++j) {
+      float2 w = tw[(static_cast<int64_t>(j) * k) % n];
+      if (!a.inverse) w.y = -w.y;
+      acc = cadd(acc, cmul(buf[ly * ls + j], w));
+    }
+    res[ly * ls + k] = acc;
+  }
+  __syncthreads();
+  store_lines(a, res, ls, n, lpb, first, total);
+}
+
+__global__ void twiddle_kernel(float2* tw, uint32_t n) {
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  double s, c;
+  sincospi(2.0 * static_cast<double>(m) / static_cast<double>(n), &s, &c);
+  tw[m] = make_float2(static_cast<float>(c), static_cast<float>(s));
+}
+
+template <int N>
+void launch_fast(const FftArgs& a, const float2* tw, hipStream_t stream, int lpb) {
+  constexpr int T = Shape<N>::T;
+  if (lpb <= 0) lpb = a.columns ? 16 : (T >= 64 ? 4 : 256 / T);
+  if (lpb * T > 1024) lpb = 1024 / T;
+  // Keep the LDS footprint within 64 KiB so at least two workgroups fit per CU.
+  const int lds_cap = static_cast<int>((65536 / sizeof(float2) - N) / (N + 1));
+  while (lpb > 1 && lpb > lds_cap) lpb /= 2;
+  const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
+  const size_t shmem = (static_cast<size_t>(N) + static_cast<size_t>(lpb) * (N + 1)) * sizeof(float2);
+  const unsigned grid = ceil_div(total, static_cast<uint64_t>(lpb));
+  if (a.inverse)
+    hipLaunchKernelGGL((fft_lds_kernel<N, true>), dim3(grid), dim3(lpb * T), shmem, stream, a, tw, lpb);
+  else
+    hipLaunchKernelGGL((fft_lds_kernel<N, false>), dim3(grid), dim3(lpb * T), shmem, stream, a, tw, lpb);
+  KSJ_LAUNCH_CHECK();
+}
+
+struct TwKey {
+  int device;
+  uint32_t n;
+  bool operator<(const TwKey& o) const { return device != o.device ? device < o.device : n < o.n; }
+};
+
+}  // namespace
+
+bool fft_fast_length(uint32_t n) {
+  switch (n) {
+    case 64: case 128: case 192: case 256: case 320: case 384: case 512: case 640: case 768: case 1024:
+      return true;
+    default:
+      return false;
+  }
+}
+
+const float2* fft_twiddles(uint32_t n, hipStream_t stream) {
+  static std::mutex mu;
+  static std::map<TwKey, float2*> cache;
+  int device = 0;
+  KSJ_HIP(hipGetDevice(&device));
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find({device, n});
+  if (it != cache.end()) return it->second;
+  float2* tw = nullptr;
+  KSJ_HIP(hipMalloc(&tw, sizeof(float2) * n));
+  hipLaunchKernelGGL(twiddle_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, stream, tw, n);
+  KSJ_LAUNCH_CHECK();
+  KSJ_HIP(hipStreamSynchronize(stream));
+  cache.emplace(TwKey{device, n}, tw);
+  return tw;
+}
+
+void fft_launch(const FftArgs& a, hipStream_t stream, int lpb) {
+  require(a.n >= 1 && a.n <= 4096, "FFT length must be in [1, 4096]");
+  require(a.n_out >= 1 && a.n_out <= a.n, "FFT pruned output count must be in [1, n]");
+  require(a.in_offset < a.n && a.out_offset < a.n, "FFT rotations must be below the length");
+  require(a.src.planes == a.dst.planes && a.src.lines == a.dst.lines, "FFT source/destination line counts differ");
+  const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
+  if (total == 0) return;
+  const float2* tw = fft_twiddles(a.n, stream);
+  switch (a.n) {
+    case 64: return launch_fast<64>(a, tw, stream, lpb);
+    case 128: return launch_fast<128>(a, tw, stream, lpb);
+    case 192: return launch_fast<192>(a, tw, stream, lpb);
+    case 256: return launch_fast<256>(a, tw, stream, lpb);
+    case 320: return launch_fast<320>(a, tw, stream, lpb);
+    case 384: return launch_fast<384>(a, tw, stream, lpb);
+    case 512: return launch_fast<512>(a, tw, stream, lpb);
+    case 640: return launch_fast<640>(a, tw, stream, lpb);
+    case 768: return launch_fast<768>(a, tw, stream, lpb);
+    case 1024: return launch_fast<1024>(a, tw, stream, lpb);
+    default: break;
+  }
+  int l = a.columns ? 8 : 1;
+  while (l > 1 && (static_cast<size_t>(a.n) + 2ull * l * (a.n + 1)) * sizeof(float2) > 60 * 1024) l /= 2;
+  const size_t shmem = (static_cast<size_t>(a.n) + 2ull * l * (a.n + 1)) * sizeof(float2);
+  require(shmem <= 160 * 1024, "direct DFT line does not fit in LDS");
+  hipLaunchKernelGGL(dft_direct_kernel, dim3(ceil_div(total, l)), dim3(256), shmem, stream, a, tw, l);
+  KSJ_LAUNCH_CHECK();
+}
+
+}  // namespace ksj::gpu

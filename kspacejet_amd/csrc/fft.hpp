@@ -1,0 +1,47 @@
+// LDS Stockham FFT for gfx950: batched 1-D transforms along either the contiguous
+// axis (rows) or a strided axis (columns) of a stack of 2-D planes, with fused
+// load rotation (ifftshift), pruned/rotated store (crop, fftshift), scaling and
+// separable output weights (deapodization).
+#pragma once
+
+#include "common.hpp"
+
+namespace ksj::gpu {
+
+// Addressing of one batched 1-D transform: line (plane p, index q), element e is at
+//   base + p * plane_stride + q * line_stride + e * elem_stride   (in complex elements).
+struct FftLines {
+  uint32_t planes = 1;
+  uint32_t lines = 1;           // lines per plane
+  uint64_t plane_stride = 0;
+  uint64_t line_stride = 0;
+  uint64_t elem_stride = 1;
+};
+
+struct FftArgs {
+  const float2* in = nullptr;
+  float2* out = nullptr;
+  FftLines src;
+  FftLines dst;                 // planes/lines must equal src's
+  uint32_t n = 0;               // transform length
+  uint32_t n_out = 0;           // stored outputs (<= n); output e' reads bin (e' + out_offset) mod n
+  uint32_t in_offset = 0;       // line[e] = in[(e + in_offset) mod n]
+  uint32_t out_offset = 0;
+  float scale = 1.0f;
+  const float* out_weight = nullptr;   // per stored element e' (length n_out) or null
+  const float* line_weight = nullptr;  // per line index q or null
+  int inverse = 1;              // +1: sum x e^{+2 pi i jk/n}; 0: forward (e^{-...})
+  int columns = 0;              // 0: lines are contiguous rows; 1: lines are strided columns
+};
+
+// Returns true when an LDS Stockham kernel is compiled for length n; other
+// lengths run through a direct DFT kernel.
+bool fft_fast_length(uint32_t n);
+
+// Device table of exp(+2 pi i m / n), m in [0, n), generated in FP64 and rounded
+// to float.  Cached per length for the life of the process (per device).
+const float2* fft_twiddles(uint32_t n, hipStream_t stream);
+
+void fft_launch(const FftArgs& args, hipStream_t stream, int lines_per_block = 0);
+
+}  // namespace ksj::gpu

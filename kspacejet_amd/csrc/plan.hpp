@@ -1,0 +1,82 @@
+// Trajectory plan for 2-D gridding: everything derived from a trajectory alone,
+// computed once on the device and kept resident so later scans on the same
+// trajectory reuse it.
+//
+//   coordinates  FP64 grid coordinate per sample, u = w * G / (2 pi) wrapped into
+//                [0, G) (the rule of eigen_radial_gridding.cpp:262-277), its cell
+//                and fraction; bin = cell_row * G_cols + cell_col.
+//   sort         stable LSD radix sort of samples by bin (ties keep sample order),
+//                cell_start[b] = first sorted position of bin b.
+//   weights      per-sample complex factor dcf * exp(+i(w_r s_r + w_c s_c)) with
+//                s = (centred ? N/2 : 0) - origin, evaluated in FP64.
+//   taps         per sorted sample W row taps and W column taps of the kernel
+//                (Kaiser-Bessel or linear), evaluated in FP64, stored as float.
+//   heavy cells  cells whose gather list exceeds heavy_threshold samples; these
+//                are computed by a workgroup each with a fixed-order reduction.
+//   deapod       1 / phi_hat per output row / column (Kaiser-Bessel only), FP64.
+#pragma once
+
+#include "common.hpp"
+
+namespace ksj::gpu {
+
+enum : uint32_t { kKernelLinear = 0, kKernelKaiserBessel = 1 };
+enum : uint32_t { kDensityNone = 0, kDensityRamp = 1, kDensityGiven = 2 };
+
+struct Plan2Desc {
+  uint32_t image_rows = 0, image_cols = 0;  // N
+  uint32_t grid_rows = 0, grid_cols = 0;    // G
+  uint32_t width = 4;                       // taps per axis (2 for linear)
+  uint32_t kernel = kKernelKaiserBessel;
+  double beta = 0.0;                        // <= 0: Beatty value for the oversampling
+  double row_origin = 0.0, col_origin = 0.0;
+  uint32_t density = kDensityRamp;
+  uint32_t centred = 1;                     // 1: image pixel r reads bin (r - N/2) mod G; 0: bin r
+  uint32_t samples = 0;
+  uint32_t heavy_threshold = 128;
+};
+
+struct Plan2 {
+  Plan2Desc d{};
+  int device = 0;
+  uint64_t nbins = 0;
+  int key_bits = 0;
+  // resident tables
+  float2* weight = nullptr;        // [S] original order
+  uint32_t* rank = nullptr;        // [S] original -> sorted position
+  uint32_t* src = nullptr;         // [S] sorted position -> original
+  uint32_t* bins = nullptr;        // [S] sample bins (original order), then sorted keys
+  uint32_t* cell_start = nullptr;  // [nbins + 1]
+  float* taps = nullptr;           // [S][2W] sorted: W row taps then W column taps
+  uint8_t* heavy_flag = nullptr;   // [nbins]
+  uint32_t* heavy_list = nullptr;  // [heavy_capacity]
+  uint32_t heavy_capacity = 0;
+  uint32_t heavy_count = 0;        // host copy, valid after set_trajectory
+  float* deapod_rows = nullptr;    // [N_r] or null
+  float* deapod_cols = nullptr;    // [N_c] or null
+  // scratch used only while building
+  double* frac = nullptr;          // [S][2]
+  uint32_t* sorted_keys = nullptr; // [S]
+  uint32_t* counters = nullptr;    // [4]: heavy count, error flags
+  void* sort_scratch = nullptr;
+  size_t device_bytes = 0;
+  bool has_trajectory = false;
+};
+
+Plan2* plan2_create(const Plan2Desc& desc);
+void plan2_destroy(Plan2* plan);
+// Recomputes every table from a float32 [S][2] radian-per-pixel trajectory on the
+// device.  dcf is read only when density == kDensityGiven.  Synchronises `stream`
+// once (to size the heavy-cell launch) and throws on a trajectory that cannot be
+// mapped (non-finite or outside [-pi, pi]).
+void plan2_set_trajectory(Plan2* plan, const float* d_traj, const float* d_dcf, hipStream_t stream);
+
+// Golden-angle radial trajectory (FP64 on the device, rounded to float):
+// spoke p at angle p * pi / phi, readout k_j = (j - M/2) / M cycles per pixel,
+// written as [S][2] (ky, kx) in radians per pixel.
+void golden_angle2(float* d_traj, uint32_t spokes, uint32_t readout, hipStream_t stream);
+
+// Analytic ramp DCF hypot(w_r, w_c) per sample in float (eigen_radial_gridding.cpp:279-295).
+void ramp_dcf2(const float* d_traj, float* d_dcf, uint64_t samples, hipStream_t stream);
+
+}  // namespace ksj::gpu

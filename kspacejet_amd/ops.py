@@ -1,0 +1,192 @@
+"""PyTorch-facing wrappers over the C ABI (include/ksj_gpu.h).
+
+torch supplies device memory and the current HIP stream only; every operation
+runs in libksj_gpu.so.  Inputs must already be on the GPU with the documented
+dtype and layout; nothing is converted or copied implicitly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+from ._lib import call, load
+
+LAYOUTS = {"coil_last": _lib.LAYOUT_COIL_LAST, "coil_major": _lib.LAYOUT_COIL_MAJOR}
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _ptr(t: torch.Tensor) -> int:
+    return t.data_ptr()
+
+
+def _need(t: torch.Tensor, dtype: torch.dtype, name: str) -> None:
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a GPU tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+def _workspace(nbytes: int, device) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+def init(device: int = 0) -> None:
+    call("ksj_gpu_init", device)
+
+
+def ifft2_coil_images(kspace: torch.Tensor, centred: bool = False, layout: str = "coil_last") -> torch.Tensor:
+    """[C, R, Cols] complex64 channel-major k-space -> coil images (1/R/Cols scaling)."""
+    _need(kspace, torch.complex64, "kspace")
+    ch, rows, cols = kspace.shape
+    shape = (rows, cols, ch) if layout == "coil_last" else (ch, rows, cols)
+    out = torch.empty(shape, dtype=torch.complex64, device=kspace.device)
+    ws = _workspace(load().ksj_gpu_ifft2_workspace_bytes(ch, rows, cols), kspace.device)
+    call("ksj_gpu_ifft2_coil_images", _ptr(kspace), _ptr(out), ch, rows, cols, int(centred), LAYOUTS[layout],
+         _ptr(ws), ws.numel(), _stream())
+    return out
+
+
+def rss(coil_images: torch.Tensor) -> torch.Tensor:
+    """[..., C] complex64 coil-last -> [...] float32 root sum of squares."""
+    _need(coil_images, torch.complex64, "coil_images")
+    ch = coil_images.shape[-1]
+    out = torch.empty(coil_images.shape[:-1], dtype=torch.float32, device=coil_images.device)
+    call("ksj_gpu_rss", _ptr(coil_images), _ptr(out), out.numel(), ch, _stream())
+    return out
+
+
+def sense_combine(coil_images: torch.Tensor, sensitivities: torch.Tensor) -> torch.Tensor:
+    _need(coil_images, torch.complex64, "coil_images")
+    _need(sensitivities, torch.complex64, "sensitivities")
+    if coil_images.shape != sensitivities.shape:
+        raise ValueError("coil images and sensitivities must have the same shape")
+    ch = coil_images.shape[-1]
+    out = torch.empty(coil_images.shape[:-1], dtype=torch.complex64, device=coil_images.device)
+    call("ksj_gpu_sense_combine", _ptr(coil_images), _ptr(sensitivities), _ptr(out), out.numel(), ch, _stream())
+    return out
+
+
+def ramp_dcf2(traj: torch.Tensor) -> torch.Tensor:
+    _need(traj, torch.float32, "traj")
+    out = torch.empty(traj.shape[0], dtype=torch.float32, device=traj.device)
+    call("ksj_gpu_ramp_dcf2", _ptr(traj), _ptr(out), traj.shape[0], _stream())
+    return out
+
+
+def nudft2_adjoint(traj: torch.Tensor, samples: torch.Tensor, rows: int, cols: int, row_origin: float,
+                   col_origin: float, layout: str = "coil_last") -> torch.Tensor:
+    """Exact adjoint NUDFT (float), samples [C, S] -> coil images."""
+    _need(traj, torch.float32, "traj")
+    _need(samples, torch.complex64, "samples")
+    ch, S = samples.shape
+    shape = (rows, cols, ch) if layout == "coil_last" else (ch, rows, cols)
+    out = torch.empty(shape, dtype=torch.complex64, device=samples.device)
+    ws = _workspace(load().ksj_gpu_nudft2_adjoint_workspace_bytes(ch, rows, cols), samples.device)
+    call("ksj_gpu_nudft2_adjoint", _ptr(traj), _ptr(samples), _ptr(out), ch, S, rows, cols, float(row_origin),
+         float(col_origin), LAYOUTS[layout], _ptr(ws), ws.numel(), _stream())
+    return out
+
+
+def nudft2_forward_f64(traj: torch.Tensor, image: torch.Tensor, row_origin: float, col_origin: float) -> torch.Tensor:
+    """Exact FP64 forward NUDFT: image [C, R, Cols] complex128 -> samples [C, S] complex128."""
+    _need(traj, torch.float32, "traj")
+    _need(image, torch.complex128, "image")
+    ch, rows, cols = image.shape
+    S = traj.shape[0]
+    out = torch.empty((ch, S), dtype=torch.complex128, device=image.device)
+    call("ksj_gpu_nudft2_forward_f64", _ptr(traj), _ptr(image), _ptr(out), ch, S, rows, cols, float(row_origin),
+         float(col_origin), _stream())
+    return out
+
+
+def golden_angle2(spokes: int, readout: int, device="cuda") -> torch.Tensor:
+    out = torch.empty((spokes * readout, 2), dtype=torch.float32, device=device)
+    call("ksj_gpu_golden_angle2", _ptr(out), spokes, readout, _stream())
+    return out
+
+
+class Plan2:
+    """Device-resident gridding plan for one trajectory (see ksj_gpu_plan2_* in include/ksj_gpu.h)."""
+
+    def __init__(self, image_rows: int, image_cols: int, samples: int, oversampling: float = 2.0,
+                 kernel: str = "kaiser_bessel", width: int = 4, density: str = "ramp", centred: bool = True,
+                 heavy_threshold: int = 0, beta: float = 0.0, row_origin: float | None = None,
+                 col_origin: float | None = None, grid_rows: int | None = None, grid_cols: int | None = None):
+        d = _lib.Plan2Desc()
+        d.struct_size = C.sizeof(_lib.Plan2Desc)
+        d.image_rows, d.image_cols = image_rows, image_cols
+        d.grid_rows = grid_rows if grid_rows is not None else int(round(image_rows * oversampling))
+        d.grid_cols = grid_cols if grid_cols is not None else int(round(image_cols * oversampling))
+        d.kernel = {"linear": _lib.KERNEL_LINEAR, "kaiser_bessel": _lib.KERNEL_KAISER_BESSEL}[kernel]
+        d.kernel_width = width
+        d.density = {"none": _lib.DENSITY_NONE, "ramp": _lib.DENSITY_RAMP, "given": _lib.DENSITY_GIVEN}[density]
+        d.centred = int(centred)
+        d.samples = samples
+        d.heavy_threshold = heavy_threshold
+        d.beta = beta
+        d.row_origin = (image_rows - 1) / 2.0 if row_origin is None else row_origin
+        d.col_origin = (image_cols - 1) / 2.0 if col_origin is None else col_origin
+        self.desc = d
+        self._handle = C.c_void_p()
+        call("ksj_gpu_plan2_create", C.byref(d), C.byref(self._handle))
+        self._lib = load()
+
+    def set_trajectory(self, traj: torch.Tensor, dcf: torch.Tensor | None = None) -> "Plan2":
+        _need(traj, torch.float32, "traj")
+        if traj.shape != (self.desc.samples, 2):
+            raise ValueError("trajectory shape does not match the plan")
+        call("ksj_gpu_plan2_set_trajectory", self._handle, _ptr(traj), 0 if dcf is None else _ptr(dcf), _stream())
+        return self
+
+    def info(self) -> dict:
+        i = _lib.Plan2Info()
+        i.struct_size = C.sizeof(_lib.Plan2Info)
+        call("ksj_gpu_plan2_get_info", self._handle, C.byref(i))
+        return {"heavy_cells": i.heavy_cells, "device_bytes": i.device_bytes, "beta": i.beta}
+
+    def export_bins(self, device="cuda"):
+        nb = self.desc.grid_rows * self.desc.grid_cols
+        start = torch.empty(nb + 1, dtype=torch.int32, device=device)
+        perm = torch.empty(self.desc.samples, dtype=torch.int32, device=device)
+        call("ksj_gpu_plan2_export_bins", self._handle, _ptr(start), _ptr(perm), _stream())
+        return start, perm
+
+    def workspace_bytes(self, channels: int) -> int:
+        return int(self._lib.ksj_gpu_adjoint2_workspace_bytes(self._handle, channels))
+
+    def adjoint(self, kspace: torch.Tensor, images: bool = True, rss: bool = False, layout: str = "coil_last",
+                workspace: torch.Tensor | None = None):
+        _need(kspace, torch.complex64, "kspace")
+        ch, S = kspace.shape
+        if S != self.desc.samples:
+            raise ValueError("sample count does not match the plan")
+        r, c = self.desc.image_rows, self.desc.image_cols
+        img = None
+        if images:
+            img = torch.empty((r, c, ch) if layout == "coil_last" else (ch, r, c), dtype=torch.complex64,
+                              device=kspace.device)
+        out_rss = torch.empty((r, c), dtype=torch.float32, device=kspace.device) if rss else None
+        need = self.workspace_bytes(ch)
+        if workspace is None or workspace.numel() < need:
+            workspace = _workspace(need, kspace.device)
+        call("ksj_gpu_adjoint2_execute", self._handle, _ptr(kspace), ch, 0 if img is None else _ptr(img),
+             LAYOUTS[layout], 0 if out_rss is None else _ptr(out_rss), _ptr(workspace), workspace.numel(), _stream())
+        return img, out_rss
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None) and self._handle.value:
+            self._lib.ksj_gpu_plan2_destroy(self._handle)
+            self._handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,403 @@
+/*
+ * FP64 CPU oracle for the KSpaceJet reconstruction hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: a plain restatement of the reference algorithms
+ * used to check the HIP path.  The product never links or calls it.
+ * Built with -ffp-contract=off so the FP64 binning arithmetic is the exact
+ * IEEE sequence the GPU binning kernel must reproduce.
+ */
+#include "ksj_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORA_PI 3.14159265358979323846
+#define ORA_TWO_PI 6.28318530717958647692
+
+/* ---------------------------------------------------------------- direct NUDFT */
+
+void ora_nudft2_forward(size_t rows, size_t cols, double row_origin, double col_origin, size_t samples,
+                        const double* traj, const double* image, double* out) {
+  /* eigen_nufft_direct_nudft.cpp:52-74: y_s = sum_{r,c} x(r,c) exp(-i(w_r (r-r0) + w_c (c-c0))) */
+  for (size_t s = 0; s < samples; ++s) {
+    const double wr = traj[2 * s], wc = traj[2 * s + 1];
+    double acc_re = 0.0, acc_im = 0.0;
+    for (size_t r = 0; r < rows; ++r) {
+      const double rp = (double)r - row_origin;
+      for (size_t c = 0; c < cols; ++c) {
+        const double cp = (double)c - col_origin;
+        const double phase = -(wr * rp + wc * cp);
+        const double cs = cos(phase), sn = sin(phase);
+        const double xr = image[2 * (r * cols + c)], xi = image[2 * (r * cols + c) + 1];
+        acc_re += xr * cs - xi * sn;
+        acc_im += xr * sn + xi * cs;
+      }
+    }
+    out[2 * s] = acc_re;
+    out[2 * s + 1] = acc_im;
+  }
+}
+
+void ora_nudft2_adjoint(size_t rows, size_t cols, double row_origin, double col_origin, size_t samples,
+                        const double* traj, const double* values, double* image) {
+  /* eigen_nufft_direct_nudft.cpp:77-97: x(r,c) = sum_s y_s exp(+i(w_r (r-r0) + w_c (c-c0))) */
+  for (size_t r = 0; r < rows; ++r) {
+    const double rp = (double)r - row_origin;
+    for (size_t c = 0; c < cols; ++c) {
+      const double cp = (double)c - col_origin;
+      double acc_re = 0.0, acc_im = 0.0;
+      for (size_t s = 0; s < samples; ++s) {
+        const double phase = traj[2 * s] * rp + traj[2 * s + 1] * cp;
+        const double cs = cos(phase), sn = sin(phase);
+        const double yr = values[2 * s], yi = values[2 * s + 1];
+        acc_re += yr * cs - yi * sn;
+        acc_im += yr * sn + yi * cs;
+      }
+      image[2 * (r * cols + c)] = acc_re;
+      image[2 * (r * cols + c) + 1] = acc_im;
+    }
+  }
+}
+
+void ora_ramp_dcf2(size_t samples, const double* traj, double* dcf) {
+  /* eigen_radial_gridding.cpp:279-295 */
+  for (size_t s = 0; s < samples; ++s) dcf[s] = hypot(traj[2 * s], traj[2 * s + 1]);
+}
+
+/* ---------------------------------------------------------------- FFT */
+
+static size_t smallest_factor(size_t n) {
+  static const size_t preferred[] = {4, 2, 3, 5};
+  for (size_t i = 0; i < 4; ++i)
+    if (n % preferred[i] == 0) return preferred[i];
+  for (size_t p = 7; p * p <= n; p += 2)
+    if (n % p == 0) return p;
+  return n;
+}
+
+void ora_fft1(size_t n, double* data, int sign) {
+  if (n <= 1) return;
+  double* a = data;
+  double* b = (double*)malloc(2 * n * sizeof(double));
+  double* v = (double*)malloc(2 * n * sizeof(double));
+  double* w = (double*)malloc(2 * n * sizeof(double));
+  /* roots of unity e^{sign 2 pi i m / n}; every twiddle below is one of them */
+  double* tw = (double*)malloc(2 * n * sizeof(double));
+  for (size_t m = 0; m < n; ++m) {
+    const double angle = sign * ORA_TWO_PI * (double)m / (double)n;
+    tw[2 * m] = cos(angle);
+    tw[2 * m + 1] = sin(angle);
+  }
+  size_t ns = 1;
+  size_t rest = n;
+  while (rest > 1) {
+    const size_t radix = smallest_factor(rest);
+    rest /= radix;
+    const size_t stride = n / radix;
+    const size_t tw_step = n / (ns * radix);
+    for (size_t j = 0; j < stride; ++j) {
+      const size_t k = j % ns;
+      for (size_t r = 0; r < radix; ++r) {
+        const size_t m = k * r * tw_step;
+        const double cs = tw[2 * m], sn = tw[2 * m + 1];
+        const double xr = a[2 * (j + r * stride)], xi = a[2 * (j + r * stride) + 1];
+        v[2 * r] = xr * cs - xi * sn;
+        v[2 * r + 1] = xr * sn + xi * cs;
+      }
+      for (size_t q = 0; q < radix; ++q) {
+        double sr = 0.0, si = 0.0;
+        for (size_t r = 0; r < radix; ++r) {
+          const size_t m = ((r * q) % radix) * stride;
+          const double cs = tw[2 * m], sn = tw[2 * m + 1];
+          sr += v[2 * r] * cs - v[2 * r + 1] * sn;
+          si += v[2 * r] * sn + v[2 * r + 1] * cs;
+        }
+        w[2 * q] = sr;
+        w[2 * q + 1] = si;
+      }
+      const size_t base = (j / ns) * ns * radix + k;
+      for (size_t q = 0; q < radix; ++q) {
+        b[2 * (base + q * ns)] = w[2 * q];
+        b[2 * (base + q * ns) + 1] = w[2 * q + 1];
+      }
+    }
+    double* t = a;
+    a = b;
+    b = t;
+    ns *= radix;
+  }
+  if (a != data) {
+    memcpy(data, a, 2 * n * sizeof(double));
+    free(a);
+  } else {
+    free(b);
+  }
+  free(v);
+  free(w);
+  free(tw);
+}
+
+/* Unscaled +i inverse DFT over rows then columns of a rows x cols interleaved matrix. */
+static void inverse_dft2_unscaled(size_t rows, size_t cols, double* data) {
+  for (size_t r = 0; r < rows; ++r) ora_fft1(cols, data + 2 * r * cols, +1);
+  double* line = (double*)malloc(2 * rows * sizeof(double));
+  for (size_t c = 0; c < cols; ++c) {
+    for (size_t r = 0; r < rows; ++r) {
+      line[2 * r] = data[2 * (r * cols + c)];
+      line[2 * r + 1] = data[2 * (r * cols + c) + 1];
+    }
+    ora_fft1(rows, line, +1);
+    for (size_t r = 0; r < rows; ++r) {
+      data[2 * (r * cols + c)] = line[2 * r];
+      data[2 * (r * cols + c) + 1] = line[2 * r + 1];
+    }
+  }
+  free(line);
+}
+
+static void circular_shift2(size_t rows, size_t cols, double* data, size_t shift_r, size_t shift_c) {
+  double* tmp = (double*)malloc(2 * rows * cols * sizeof(double));
+  for (size_t r = 0; r < rows; ++r)
+    for (size_t c = 0; c < cols; ++c) {
+      const size_t rr = (r + shift_r) % rows, cc = (c + shift_c) % cols;
+      tmp[2 * (rr * cols + cc)] = data[2 * (r * cols + c)];
+      tmp[2 * (rr * cols + cc) + 1] = data[2 * (r * cols + c) + 1];
+    }
+  memcpy(data, tmp, 2 * rows * cols * sizeof(double));
+  free(tmp);
+}
+
+void ora_ifft2(size_t rows, size_t cols, double* data, int centred) {
+  /* eigen_fft_transforms.cpp:148-205 with Normalization::inverse (:14-27): 1/cols then 1/rows */
+  if (centred) circular_shift2(rows, cols, data, rows - rows / 2, cols - cols / 2); /* ifftshift */
+  inverse_dft2_unscaled(rows, cols, data);
+  const double scale = 1.0 / ((double)rows * (double)cols);
+  for (size_t i = 0; i < 2 * rows * cols; ++i) data[i] *= scale;
+  if (centred) circular_shift2(rows, cols, data, rows / 2, cols / 2); /* fftshift */
+}
+
+/* ---------------------------------------------------------------- radial bilinear route */
+
+static int reference_grid_coordinate(double w, size_t extent, size_t* lower, double* fraction) {
+  /* eigen_radial_gridding.cpp:262-277 */
+  const double e = (double)extent;
+  double u = w * e / (2.0 * ORA_PI);
+  if (u < 0.0) u += e;
+  if (u >= e) u -= e;
+  if (!isfinite(u) || u < 0.0 || u >= e) return -1;
+  *lower = (size_t)floor(u);
+  *fraction = u - (double)*lower;
+  return 0;
+}
+
+int ora_radial_linear_gridding2(size_t rows, size_t cols, double row_origin, double col_origin, size_t samples,
+                                const double* values, const double* traj, const double* dcf, double* image) {
+  for (size_t s = 0; s < samples; ++s)
+    if (!(traj[2 * s] >= -ORA_PI && traj[2 * s] <= ORA_PI && traj[2 * s + 1] >= -ORA_PI && traj[2 * s + 1] <= ORA_PI))
+      return -1;
+  memset(image, 0, 2 * rows * cols * sizeof(double));
+  for (size_t s = 0; s < samples; ++s) {
+    const double wr = traj[2 * s], wc = traj[2 * s + 1];
+    size_t rl, cl;
+    double rf, cf;
+    if (reference_grid_coordinate(wr, rows, &rl, &rf) || reference_grid_coordinate(wc, cols, &cl, &cf)) return -1;
+    const size_t ru = (rl + 1) % rows, cu = (cl + 1) % cols;
+    const double phase = -(wr * row_origin + wc * col_origin);
+    const double pr = cos(phase), pi = sin(phase);
+    const double yr = values[2 * s] * dcf[s], yi = values[2 * s + 1] * dcf[s];
+    const double zr = yr * pr - yi * pi, zi = yr * pi + yi * pr;
+    const double wts[4] = {(1.0 - rf) * (1.0 - cf), (1.0 - rf) * cf, rf * (1.0 - cf), rf * cf};
+    const size_t idx[4] = {rl * cols + cl, rl * cols + cu, ru * cols + cl, ru * cols + cu};
+    for (int t = 0; t < 4; ++t) {
+      image[2 * idx[t]] += zr * wts[t];
+      image[2 * idx[t] + 1] += zi * wts[t];
+    }
+  }
+  inverse_dft2_unscaled(rows, cols, image);
+  return 0;
+}
+
+/* ---------------------------------------------------------------- coil combine */
+
+void ora_rss(size_t pixels, size_t coils, const double* coil_last, double* out) {
+  for (size_t p = 0; p < pixels; ++p) {
+    double acc = 0.0;
+    for (size_t c = 0; c < coils; ++c) {
+      const double re = coil_last[2 * (p * coils + c)], im = coil_last[2 * (p * coils + c) + 1];
+      acc += re * re + im * im;
+    }
+    out[p] = sqrt(acc);
+  }
+}
+
+void ora_sense_combine(size_t pixels, size_t coils, const double* images, const double* sens, double* out) {
+  for (size_t p = 0; p < pixels; ++p) {
+    double nr = 0.0, ni = 0.0, den = 0.0;
+    for (size_t c = 0; c < coils; ++c) {
+      const double ir = images[2 * (p * coils + c)], ii = images[2 * (p * coils + c) + 1];
+      const double sr = sens[2 * (p * coils + c)], si = sens[2 * (p * coils + c) + 1];
+      nr += sr * ir + si * ii; /* conj(S) * I */
+      ni += sr * ii - si * ir;
+      den += sr * sr + si * si;
+    }
+    out[2 * p] = den > 0.0 ? nr / den : 0.0;
+    out[2 * p + 1] = den > 0.0 ? ni / den : 0.0;
+  }
+}
+
+/* ---------------------------------------------------------------- Kaiser-Bessel */
+
+double ora_i0(double x) {
+  /* Power series sum_k ((x/2)^k / k!)^2, summed until terms stop changing the sum. */
+  const double q = 0.25 * x * x;
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 500; ++k) {
+    term *= q / ((double)k * (double)k);
+    sum += term;
+    if (term < sum * 1e-17) break;
+  }
+  return sum;
+}
+
+double ora_kb_beta(int width, double sigma) {
+  /* Beatty, Nishimura, Pauly, IEEE TMI 24(6) 2005, eq. 5 */
+  const double a = (double)width / sigma * (sigma - 0.5);
+  return ORA_PI * sqrt(a * a - 0.8);
+}
+
+double ora_kb_kernel(double distance, int width, double beta) {
+  const double half = 0.5 * (double)width;
+  if (!(fabs(distance) < half)) return 0.0;
+  const double t = distance / half;
+  return ora_i0(beta * sqrt(1.0 - t * t)) / ora_i0(beta);
+}
+
+double ora_kb_ft(double index, size_t grid, int width, double beta) {
+  /* Continuous FT of ora_kb_kernel at angular frequency 2 pi index / grid:
+   * (W / I0(beta)) sinh(z)/z with z^2 = beta^2 - (pi W index / grid)^2 (sin for z^2 < 0). */
+  const double a = ORA_PI * (double)width * index / (double)grid;
+  const double z2 = beta * beta - a * a;
+  double shape;
+  if (z2 > 0.0) {
+    const double z = sqrt(z2);
+    shape = sinh(z) / z;
+  } else if (z2 < 0.0) {
+    const double z = sqrt(-z2);
+    shape = sin(z) / z;
+  } else {
+    shape = 1.0;
+  }
+  return (double)width * shape / ora_i0(beta);
+}
+
+int ora_grid_coordinate(float w, uint32_t extent, uint32_t* cell, float* frac) {
+  const double e = (double)extent;
+  double u = ((double)w * e) / ORA_TWO_PI;
+  if (u < 0.0) u += e;
+  if (u >= e) u -= e;
+  if (!(u >= 0.0 && u < e)) return -1;
+  const double f = floor(u);
+  *cell = (uint32_t)f;
+  *frac = (float)(u - f);
+  return 0;
+}
+
+int ora_bin2(size_t samples, const float* traj, uint32_t grid_rows, uint32_t grid_cols, uint32_t* bins,
+             float* frac, uint32_t* cell_start, uint32_t* perm) {
+  const size_t nbins = (size_t)grid_rows * grid_cols;
+  memset(cell_start, 0, (nbins + 1) * sizeof(uint32_t));
+  for (size_t s = 0; s < samples; ++s) {
+    uint32_t cr, cc;
+    if (ora_grid_coordinate(traj[2 * s], grid_rows, &cr, &frac[2 * s]) ||
+        ora_grid_coordinate(traj[2 * s + 1], grid_cols, &cc, &frac[2 * s + 1]))
+      return -1;
+    bins[s] = cr * grid_cols + cc;
+    cell_start[bins[s] + 1] += 1;
+  }
+  for (size_t b = 0; b < nbins; ++b) cell_start[b + 1] += cell_start[b];
+  uint32_t* cursor = (uint32_t*)malloc(nbins * sizeof(uint32_t));
+  memcpy(cursor, cell_start, nbins * sizeof(uint32_t));
+  for (size_t s = 0; s < samples; ++s) perm[cursor[bins[s]]++] = (uint32_t)s;
+  free(cursor);
+  return 0;
+}
+
+int ora_kb_adjoint2(size_t n_rows, size_t n_cols, size_t g_rows, size_t g_cols, int width, double beta,
+                    double row_origin, double col_origin, size_t samples, const double* traj, const double* values,
+                    const double* dcf, double* image) {
+  double* grid = (double*)calloc(2 * g_rows * g_cols, sizeof(double));
+  const double shift_r = (double)(n_rows / 2) - row_origin;
+  const double shift_c = (double)(n_cols / 2) - col_origin;
+  const int lead = width / 2 - 1;
+  double* wr_tap = (double*)malloc((size_t)width * sizeof(double));
+  double* wc_tap = (double*)malloc((size_t)width * sizeof(double));
+  for (size_t s = 0; s < samples; ++s) {
+    const double wr = traj[2 * s], wc = traj[2 * s + 1];
+    const double weight = dcf ? dcf[s] : 1.0;
+    const double phase = wr * shift_r + wc * shift_c;
+    const double pr = cos(phase), pi = sin(phase);
+    const double yr = values[2 * s] * weight, yi = values[2 * s + 1] * weight;
+    const double zr = yr * pr - yi * pi, zi = yr * pi + yi * pr;
+    uint32_t cr, cc;
+    float fr, fc;
+    if (ora_grid_coordinate((float)wr, (uint32_t)g_rows, &cr, &fr) ||
+        ora_grid_coordinate((float)wc, (uint32_t)g_cols, &cc, &fc)) {
+      free(grid);
+      free(wr_tap);
+      free(wc_tap);
+      return -1;
+    }
+    /* The cell is taken from the float-rounded coordinate rule above (the GPU bins the
+     * float32 payload); the tap distances use the exact FP64 coordinate. */
+    double ur = wr * (double)g_rows / ORA_TWO_PI, uc = wc * (double)g_cols / ORA_TWO_PI;
+    if (ur < 0.0) ur += (double)g_rows;
+    if (ur >= (double)g_rows) ur -= (double)g_rows;
+    if (uc < 0.0) uc += (double)g_cols;
+    if (uc >= (double)g_cols) uc -= (double)g_cols;
+    const double dr0 = ur - (double)cr, dc0 = uc - (double)cc;
+    for (int t = 0; t < width; ++t) {
+      wr_tap[t] = ora_kb_kernel((double)(t - lead) - dr0, width, beta);
+      wc_tap[t] = ora_kb_kernel((double)(t - lead) - dc0, width, beta);
+    }
+    for (int ty = 0; ty < width; ++ty) {
+      const size_t mr = ((size_t)cr + g_rows + (size_t)ty - (size_t)lead) % g_rows;
+      for (int tx = 0; tx < width; ++tx) {
+        const size_t mc = ((size_t)cc + g_cols + (size_t)tx - (size_t)lead) % g_cols;
+        const double w = wr_tap[ty] * wc_tap[tx];
+        grid[2 * (mr * g_cols + mc)] += zr * w;
+        grid[2 * (mr * g_cols + mc) + 1] += zi * w;
+      }
+    }
+  }
+  inverse_dft2_unscaled(g_rows, g_cols, grid);
+  for (size_t r = 0; r < n_rows; ++r) {
+    const long jr = (long)r - (long)(n_rows / 2);
+    const size_t ir = (size_t)((jr + (long)g_rows) % (long)g_rows);
+    const double dr = ora_kb_ft((double)jr, g_rows, width, beta);
+    for (size_t c = 0; c < n_cols; ++c) {
+      const long jc = (long)c - (long)(n_cols / 2);
+      const size_t ic = (size_t)((jc + (long)g_cols) % (long)g_cols);
+      const double d = dr * ora_kb_ft((double)jc, g_cols, width, beta);
+      image[2 * (r * n_cols + c)] = grid[2 * (ir * g_cols + ic)] / d;
+      image[2 * (r * n_cols + c) + 1] = grid[2 * (ir * g_cols + ic) + 1] / d;
+    }
+  }
+  free(grid);
+  free(wr_tap);
+  free(wc_tap);
+  return 0;
+}
+
+void ora_golden_angle2(size_t spokes, size_t readout, double* traj) {
+  const double golden = ORA_PI * (sqrt(5.0) - 1.0) / 2.0; /* pi / phi = 111.246 degrees */
+  for (size_t p = 0; p < spokes; ++p) {
+    const double angle = (double)p * golden;
+    const double cs = cos(angle), sn = sin(angle);
+    for (size_t j = 0; j < readout; ++j) {
+      const double k = ORA_TWO_PI * ((double)j - (double)(readout / 2)) / (double)readout;
+      traj[2 * (p * readout + j)] = k * sn;
+      traj[2 * (p * readout + j) + 1] = k * cs;
+    }
+  }
+}

@@ -1,0 +1,105 @@
+/*
+ * FP64 CPU oracle for the KSpaceJet reconstruction hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (kspacejet_amd/) may link,
+ * load or call this library; it exists so tests/, the smoke check and the
+ * bench's CPU-baseline leg can check the HIP path against a plain restatement
+ * of the reference algorithms.
+ *
+ * Complex arrays are interleaved (re, im) doubles.  Trajectories are [S][2]
+ * rows of (row_frequency = ky, col_frequency = kx) in radians per pixel, the
+ * layout of the reference ksj.trajectory-frame payload.
+ *
+ * Each function cites the reference file:line it restates.  Paths are
+ * relative to the KSpaceJet source tree.
+ */
+#ifndef KSJ_ORACLE_H_
+#define KSJ_ORACLE_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Exact non-uniform DFTs.
+ * libs/numerics/kspacejet-nufft/src/eigen/eigen_nufft_direct_nudft.cpp:52-74 (forward, sign -i)
+ * and :77-97 (adjoint, sign +i, no normalisation, no density weights). */
+void ora_nudft2_forward(size_t rows, size_t cols, double row_origin, double col_origin, size_t samples,
+                        const double* traj, const double* image, double* out);
+void ora_nudft2_adjoint(size_t rows, size_t cols, double row_origin, double col_origin, size_t samples,
+                        const double* traj, const double* values, double* image);
+
+/* Analytic ramp density compensation hypot(k_row, k_col).
+ * libs/numerics/kspacejet-nufft/src/eigen/eigen_radial_gridding.cpp:279-295 */
+void ora_ramp_dcf2(size_t samples, const double* traj, double* dcf);
+
+/* Reference radial route: periodic 2x2 bilinear gridding on the unoversampled
+ * rows x cols grid with the origin phase, then an unscaled +i inverse DFT per axis.
+ * libs/numerics/kspacejet-nufft/src/eigen/eigen_radial_gridding.cpp:262-277 (grid coordinate),
+ * :297-363 (gridding and separable inverse transform).  Returns 0, or -1 on a
+ * trajectory outside [-pi, pi]. */
+int ora_radial_linear_gridding2(size_t rows, size_t cols, double row_origin, double col_origin, size_t samples,
+                                const double* values, const double* traj, const double* dcf, double* image);
+
+/* In-place unscaled DFT of length n.  sign = +1 computes sum x e^{+2 pi i jk/n}
+ * (the inverse direction), sign = -1 the forward direction.  Mixed-radix
+ * Stockham with direct prime-radix butterflies; any n >= 1. */
+void ora_fft1(size_t n, double* data, int sign);
+
+/* Inverse 2-D transform with Normalization::inverse (1/cols per row pass then
+ * 1/rows per column pass), uncentred, as
+ * libs/numerics/kspacejet-fft/src/eigen/eigen_fft_transforms.cpp:14-27,148-205.
+ * centred != 0 computes fftshift(ifft2(ifftshift(x))) instead. */
+void ora_ifft2(size_t rows, size_t cols, double* data, int centred);
+
+/* Root sum of squares over the contiguous (last) coil axis, accumulated in coil
+ * order, then sqrt.  libs/numerics/kspacejet-array/include/kspacejet/array/reductions.hpp:973-983,
+ * libs/numerics/kspacejet-stats/src/eigen/eigen_stats_common.hpp:376-381. */
+void ora_rss(size_t pixels, size_t coils, const double* coil_last, double* out);
+
+/* Sensitivity-weighted combine x = sum_c conj(S_c) I_c / sum_c |S_c|^2 (0 where the
+ * denominator is 0).  The reference only plans this interface
+ * (providers/interfaces/org.kspacejet.coil-combine/coil_combine_adaptive.json). */
+void ora_sense_combine(size_t pixels, size_t coils, const double* images, const double* sens, double* out);
+
+/* Kaiser-Bessel pieces (the reference has no KB gridding; Beatty et al. 2005). */
+double ora_i0(double x);
+double ora_kb_beta(int width, double sigma);
+double ora_kb_kernel(double distance, int width, double beta);
+double ora_kb_ft(double index, size_t grid, int width, double beta);
+
+/* Grid coordinate of a float32 radian-per-pixel frequency on a grid of `extent`
+ * cells: u = (double)w * extent / (2 pi), wrapped once into [0, extent); then the
+ * cell floor(u) and the fraction u - floor(u) rounded to float.  Same rule as
+ * libs/numerics/kspacejet-nufft/src/eigen/eigen_radial_gridding.cpp:262-277, in FP64.
+ * Returns -1 if the coordinate cannot be mapped. */
+int ora_grid_coordinate(float w, uint32_t extent, uint32_t* cell, float* frac);
+
+/* Bins every sample to cell index cell_row * grid_cols + cell_col and stably sorts
+ * by bin (ties keep sample order).  cell_start has grid_rows*grid_cols+1 entries.
+ * frac is [S][2] (row, col) in sample order.  Returns -1 on a bad coordinate. */
+int ora_bin2(size_t samples, const float* traj, uint32_t grid_rows, uint32_t grid_cols, uint32_t* bins,
+             float* frac, uint32_t* cell_start, uint32_t* perm);
+
+/* FP64 KB adjoint NUFFT of one coil:
+ *   c_s = value_s * dcf_s * exp(+i(w_r (N_r/2 - r0) + w_c (N_c/2 - c0)))
+ *   spread c_s with phi(d_r) phi(d_c) onto G_r x G_c (taps at cell-W/2+1 .. cell+W/2)
+ *   unscaled +i inverse DFT of the grid, crop indices j in [-N/2, N/2),
+ *   divide by phi_hat(j_r) phi_hat(j_c); output pixel r = j_r + N_r/2.
+ * This equals sum_s value_s dcf_s exp(+i(w_r (r - r0) + w_c (c - c0))) up to the KB
+ * approximation error.  dcf may be NULL (unit weights).  Returns -1 on a bad coordinate. */
+int ora_kb_adjoint2(size_t n_rows, size_t n_cols, size_t g_rows, size_t g_cols, int width, double beta,
+                    double row_origin, double col_origin, size_t samples, const double* traj, const double* values,
+                    const double* dcf, double* image);
+
+/* Golden-angle radial trajectory: spoke p at angle p * pi / phi (111.246 deg),
+ * readout k_j = (j - M/2) / M cycles per pixel, stored as [S][2] (ky, kx) radians per pixel. */
+void ora_golden_angle2(size_t spokes, size_t readout, double* traj);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,249 @@
+"""ctypes wrapper over ``libksj_oracle.so`` (FP64 restatement) and, where it was
+built, ``_ref/libksj_ref.so`` (the reference's own Eigen-free NUFFT sources).
+
+TEST INFRASTRUCTURE ONLY (see the package docstring).  Every function takes and
+returns numpy arrays; complex arrays are complex128 unless stated.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_ORACLE_SO = os.path.join(_HERE, "libksj_oracle.so")
+_REF_SO = os.path.join(_HERE, "_ref", "libksj_ref.so")
+
+_lib = None
+_ref = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_sz = C.c_size_t
+
+
+def build() -> None:
+    """Compile the oracle (and the reference shim when the source tree is present)."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "all"], check=True)
+    if os.path.isdir("/root/reference/libs/numerics/kspacejet-nufft"):
+        subprocess.run(["make", "-s", "-B", "-C", _HERE, "ref"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_ORACLE_SO):
+            subprocess.run(["make", "-s", "-C", _HERE, "all"], check=True)
+        L = C.CDLL(_ORACLE_SO)
+        L.ora_nudft2_forward.argtypes = [_sz, _sz, C.c_double, C.c_double, _sz, _dp, _dp, _dp]
+        L.ora_nudft2_adjoint.argtypes = [_sz, _sz, C.c_double, C.c_double, _sz, _dp, _dp, _dp]
+        L.ora_ramp_dcf2.argtypes = [_sz, _dp, _dp]
+        L.ora_radial_linear_gridding2.argtypes = [_sz, _sz, C.c_double, C.c_double, _sz, _dp, _dp, _dp, _dp]
+        L.ora_fft1.argtypes = [_sz, _dp, C.c_int]
+        L.ora_ifft2.argtypes = [_sz, _sz, _dp, C.c_int]
+        L.ora_rss.argtypes = [_sz, _sz, _dp, _dp]
+        L.ora_sense_combine.argtypes = [_sz, _sz, _dp, _dp, _dp]
+        L.ora_i0.argtypes = [C.c_double]
+        L.ora_i0.restype = C.c_double
+        L.ora_kb_beta.argtypes = [C.c_int, C.c_double]
+        L.ora_kb_beta.restype = C.c_double
+        L.ora_kb_kernel.argtypes = [C.c_double, C.c_int, C.c_double]
+        L.ora_kb_kernel.restype = C.c_double
+        L.ora_kb_ft.argtypes = [C.c_double, _sz, C.c_int, C.c_double]
+        L.ora_kb_ft.restype = C.c_double
+        L.ora_grid_coordinate.argtypes = [C.c_float, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]
+        L.ora_bin2.argtypes = [_sz, _fp, C.c_uint32, C.c_uint32, _up, _fp, _up, _up]
+        L.ora_kb_adjoint2.argtypes = [_sz, _sz, _sz, _sz, C.c_int, C.c_double, C.c_double, C.c_double, _sz,
+                                      _dp, _dp, _dp, _dp]
+        L.ora_golden_angle2.argtypes = [_sz, _sz, _dp]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(_REF_SO)
+
+
+def ref():
+    """The reference's compiled NUFFT sources (None where oracle/_ref was not built)."""
+    global _ref
+    if _ref is None and ref_available():
+        R = C.CDLL(_REF_SO)
+        R.ref_ramp_dcf2_f64.argtypes = [_sz, _dp, _dp]
+        R.ref_ramp_dcf2_f32.argtypes = [_sz, _fp, _fp]
+        R.ref_radial_linear_gridding2_f64.argtypes = [_sz, _sz, C.c_double, C.c_double, _sz, _dp, _dp, _dp, _dp]
+        R.ref_radial_linear_gridding2_f32.argtypes = [_sz, _sz, C.c_double, C.c_double, _sz, _fp, _fp, _fp, _fp]
+        R.ref_direct_adjoint2_f64.argtypes = [_sz, _sz, C.c_double, C.c_double, _sz, _dp, _dp, _dp]
+        R.ref_direct_forward2_f64.argtypes = [_sz, _sz, C.c_double, C.c_double, _sz, _dp, _dp, _dp]
+        R.ref_radial_route_rss_batch.argtypes = [_sz, _sz, _sz, _sz, _sz, _fp, _fp, _fp, C.c_uint]
+        _ref = R
+    return _ref
+
+
+def _c128(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.complex128))
+
+
+def _as_f64_pairs(x: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(_c128(x)).view(np.float64)
+
+
+def _traj64(traj) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(traj, dtype=np.float64).reshape(-1, 2))
+
+
+# --------------------------------------------------------------------------- direct NUDFT
+
+def nudft2_forward(image, traj, row_origin=0.0, col_origin=0.0) -> np.ndarray:
+    img = _c128(image)
+    t = _traj64(traj)
+    out = np.zeros(t.shape[0], np.complex128)
+    lib().ora_nudft2_forward(img.shape[0], img.shape[1], row_origin, col_origin, t.shape[0], t.reshape(-1),
+                             img.view(np.float64).reshape(-1), out.view(np.float64))
+    return out
+
+
+def nudft2_adjoint(values, traj, rows, cols, row_origin=0.0, col_origin=0.0) -> np.ndarray:
+    v = _c128(values).reshape(-1)
+    t = _traj64(traj)
+    out = np.zeros((rows, cols), np.complex128)
+    lib().ora_nudft2_adjoint(rows, cols, row_origin, col_origin, t.shape[0], t.reshape(-1), v.view(np.float64),
+                             out.view(np.float64).reshape(-1))
+    return out
+
+
+def ramp_dcf2(traj) -> np.ndarray:
+    t = _traj64(traj)
+    out = np.zeros(t.shape[0], np.float64)
+    lib().ora_ramp_dcf2(t.shape[0], t.reshape(-1), out)
+    return out
+
+
+def radial_linear_gridding2(values, traj, dcf, rows, cols, row_origin, col_origin) -> np.ndarray:
+    v = _c128(values).reshape(-1)
+    t = _traj64(traj)
+    d = np.ascontiguousarray(np.asarray(dcf, np.float64))
+    out = np.zeros((rows, cols), np.complex128)
+    rc = lib().ora_radial_linear_gridding2(rows, cols, row_origin, col_origin, t.shape[0], v.view(np.float64),
+                                           t.reshape(-1), d, out.view(np.float64).reshape(-1))
+    if rc != 0:
+        raise ValueError("trajectory outside [-pi, pi] or unmappable")
+    return out
+
+
+# --------------------------------------------------------------------------- FFT / combine
+
+def fft1(x, sign: int) -> np.ndarray:
+    y = _c128(x).copy()
+    lib().ora_fft1(y.shape[0], y.view(np.float64), int(sign))
+    return y
+
+
+def ifft2(x, centred: bool = False) -> np.ndarray:
+    y = _c128(x).copy()
+    lib().ora_ifft2(y.shape[0], y.shape[1], y.view(np.float64).reshape(-1), 1 if centred else 0)
+    return y
+
+
+def ifft2_coil_images(kspace, centred: bool = False) -> np.ndarray:
+    """[C, R, Cols] channel-major k-space -> [R, Cols, C] coil-last images."""
+    k = _c128(kspace)
+    return np.stack([ifft2(k[c], centred) for c in range(k.shape[0])], axis=-1)
+
+
+def rss(coil_last) -> np.ndarray:
+    x = _c128(coil_last)
+    coils = x.shape[-1]
+    pix = x.size // coils
+    out = np.zeros(pix, np.float64)
+    lib().ora_rss(pix, coils, x.view(np.float64).reshape(-1), out)
+    return out.reshape(x.shape[:-1])
+
+
+def sense_combine(coil_last, sens_last) -> np.ndarray:
+    x = _c128(coil_last)
+    s = _c128(sens_last)
+    coils = x.shape[-1]
+    pix = x.size // coils
+    out = np.zeros(pix, np.complex128)
+    lib().ora_sense_combine(pix, coils, x.view(np.float64).reshape(-1), s.view(np.float64).reshape(-1),
+                            out.view(np.float64))
+    return out.reshape(x.shape[:-1])
+
+
+# --------------------------------------------------------------------------- Kaiser-Bessel
+
+def i0(x: float) -> float:
+    return lib().ora_i0(float(x))
+
+
+def kb_beta(width: int, sigma: float) -> float:
+    return lib().ora_kb_beta(int(width), float(sigma))
+
+
+def kb_kernel(d: float, width: int, beta: float) -> float:
+    return lib().ora_kb_kernel(float(d), int(width), float(beta))
+
+
+def kb_ft(index: float, grid: int, width: int, beta: float) -> float:
+    return lib().ora_kb_ft(float(index), int(grid), int(width), float(beta))
+
+
+def grid_coordinate(w: float, extent: int):
+    cell = C.c_uint32(0)
+    frac = C.c_float(0)
+    rc = lib().ora_grid_coordinate(C.c_float(w), extent, C.byref(cell), C.byref(frac))
+    if rc != 0:
+        raise ValueError("coordinate cannot be mapped")
+    return cell.value, np.float32(frac.value)
+
+
+def bin2(traj_f32, grid_rows: int, grid_cols: int):
+    """Returns (bins[S] u32, frac[S,2] f32, cell_start[G_r*G_c+1] u32, perm[S] u32)."""
+    t = np.ascontiguousarray(np.asarray(traj_f32, np.float32).reshape(-1, 2))
+    S = t.shape[0]
+    bins = np.zeros(S, np.uint32)
+    frac = np.zeros((S, 2), np.float32)
+    start = np.zeros(grid_rows * grid_cols + 1, np.uint32)
+    perm = np.zeros(S, np.uint32)
+    rc = lib().ora_bin2(S, t.reshape(-1), grid_rows, grid_cols, bins, frac.reshape(-1), start, perm)
+    if rc != 0:
+        raise ValueError("trajectory cannot be mapped to the grid")
+    return bins, frac, start, perm
+
+
+def kb_adjoint2(values, traj, dcf, n_rows, n_cols, oversampling=2.0, width=4, beta=None,
+                row_origin=None, col_origin=None) -> np.ndarray:
+    g_rows = int(round(n_rows * oversampling))
+    g_cols = int(round(n_cols * oversampling))
+    if beta is None:
+        beta = kb_beta(width, oversampling)
+    if row_origin is None:
+        row_origin = (n_rows - 1) / 2.0
+    if col_origin is None:
+        col_origin = (n_cols - 1) / 2.0
+    v = _c128(values).reshape(-1)
+    t = _traj64(traj)
+    d = np.ones(t.shape[0], np.float64) if dcf is None else np.ascontiguousarray(np.asarray(dcf, np.float64))
+    out = np.zeros((n_rows, n_cols), np.complex128)
+    rc = lib().ora_kb_adjoint2(n_rows, n_cols, g_rows, g_cols, int(width), float(beta), float(row_origin),
+                               float(col_origin), t.shape[0], t.reshape(-1), v.view(np.float64), d,
+                               out.view(np.float64).reshape(-1))
+    if rc != 0:
+        raise ValueError("trajectory cannot be mapped to the grid")
+    return out
+
+
+def kb_adjoint2_coils(kspace, traj, dcf, n, oversampling=2.0, width=4):
+    """[C, S] channel-major samples -> [N, N, C] coil-last FP64 KB images."""
+    k = _c128(kspace)
+    return np.stack([kb_adjoint2(k[c], traj, dcf, n, n, oversampling, width) for c in range(k.shape[0])], axis=-1)
+
+
+def golden_angle2(spokes: int, readout: int) -> np.ndarray:
+    out = np.zeros((spokes * readout, 2), np.float64)
+    lib().ora_golden_angle2(spokes, readout, out.reshape(-1))
+    return out
