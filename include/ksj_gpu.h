@@ -122,7 +122,7 @@ typedef struct ksj_gpu_plan2_desc {
   uint32_t density;                /* KSJ_GPU_DENSITY_* */
   uint32_t centred;                /* 1: crop the centre of the oversampled grid; 0: unshifted (linear route) */
   uint32_t samples;
-  uint32_t heavy_threshold;        /* 0 = default */
+  uint32_t item_visits;            /* gather work-item size in visits, 0 = default (64) */
   uint32_t reserved0;
   double beta;                     /* <= 0: Beatty et al. 2005 value for the oversampling */
   double row_origin, col_origin;   /* image origin; the reference uses ((R-1)/2, (C-1)/2) */
@@ -130,8 +130,9 @@ typedef struct ksj_gpu_plan2_desc {
 
 typedef struct ksj_gpu_plan2_info {
   uint32_t struct_size;
-  uint32_t heavy_cells;
-  uint64_t device_bytes;
+  uint32_t items;          /* gather work items */
+  uint64_t visits;         /* (sample, 2x2 cell block) visits */
+  uint64_t device_bytes;   /* resident table bytes */
   double beta;
 } ksj_gpu_plan2_info;
 
@@ -162,6 +163,30 @@ void ksj_gpu_plan2_destroy(ksj_gpu_plan2* plan);
 size_t ksj_gpu_adjoint2_workspace_bytes(const ksj_gpu_plan2* plan, uint32_t channels);
 int ksj_gpu_adjoint2_execute(const ksj_gpu_plan2* plan, const void* d_kspace, uint32_t channels, void* d_images,
                              int layout, float* d_rss, void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* Batched adjoint of `count` scans that share the plan's trajectory: the scans are
+ * stacked along the coil axis (up to 64 coils per gridding pass), so one pass of
+ * gather/FFT/deapodize kernels serves several scans.  d_kspace[i] is scan i's
+ * [C][S] samples; d_images[i] / d_rss[i] its outputs (either array may be NULL,
+ * or hold NULL entries; every scan needs at least one output).  Each scan's result
+ * is the same as its single-scan ksj_gpu_adjoint2_execute result up to the float
+ * summation order of the shared passes.  No reference counterpart: the reference
+ * processes one frame per firing (radial_gridding_reconstruct.cpp:400-413). */
+size_t ksj_gpu_adjoint2_batch_workspace_bytes(const ksj_gpu_plan2* plan, uint32_t channels, uint32_t scans);
+int ksj_gpu_adjoint2_execute_batch(const ksj_gpu_plan2* plan, uint32_t count, const void* const* d_kspace,
+                                   uint32_t channels, void* const* d_images, int layout, float* const* d_rss,
+                                   void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------ stage timing */
+
+/* Per-stage timing with HIP events recorded on the stream each stage runs on
+ * (stages: grid_prep, grid_gather, fft_rows, fft_cols_deapod, coil_combine, ...).
+ * Off by default; no reference counterpart (the reference's kspacejet-performance
+ * spans are compiled out unless KSJ_PERFORMANCE_LEVEL is set).  profile_read
+ * synchronises the pending events of that stage. */
+int ksj_gpu_profile_enable(int on);
+int ksj_gpu_profile_reset(void);
+int ksj_gpu_profile_read(const char* stage, double* total_ms, uint64_t* count);
 
 #ifdef __cplusplus
 }

@@ -35,14 +35,14 @@ class Plan2Desc(C.Structure):
         ("grid_rows", C.c_uint32), ("grid_cols", C.c_uint32),
         ("kernel", C.c_uint32), ("kernel_width", C.c_uint32),
         ("density", C.c_uint32), ("centred", C.c_uint32),
-        ("samples", C.c_uint32), ("heavy_threshold", C.c_uint32), ("reserved0", C.c_uint32),
+        ("samples", C.c_uint32), ("item_visits", C.c_uint32), ("reserved0", C.c_uint32),
         ("beta", C.c_double), ("row_origin", C.c_double), ("col_origin", C.c_double),
     ]
 
 
 class Plan2Info(C.Structure):
-    _fields_ = [("struct_size", C.c_uint32), ("heavy_cells", C.c_uint32), ("device_bytes", C.c_uint64),
-                ("beta", C.c_double)]
+    _fields_ = [("struct_size", C.c_uint32), ("items", C.c_uint32), ("visits", C.c_uint64),
+                ("device_bytes", C.c_uint64), ("beta", C.c_double)]
 
 
 _vp = C.c_void_p
@@ -73,6 +73,12 @@ SIGNATURES = {
     "ksj_gpu_plan2_destroy": (None, [_vp]),
     "ksj_gpu_adjoint2_workspace_bytes": (_sz, [_vp, _u32]),
     "ksj_gpu_adjoint2_execute": (_int, [_vp, _vp, _u32, _vp, _int, _vp, _vp, _sz, _vp]),
+    "ksj_gpu_adjoint2_batch_workspace_bytes": (_sz, [_vp, _u32, _u32]),
+    "ksj_gpu_adjoint2_execute_batch": (_int, [_vp, _u32, C.POINTER(_vp), _u32, C.POINTER(_vp), _int,
+                                               C.POINTER(_vp), _vp, _sz, _vp]),
+    "ksj_gpu_profile_enable": (_int, [_int]),
+    "ksj_gpu_profile_reset": (_int, []),
+    "ksj_gpu_profile_read": (_int, [C.c_char_p, C.POINTER(_dbl), C.POINTER(_u64)]),
 }
 
 _lib = None

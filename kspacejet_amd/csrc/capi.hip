@@ -6,11 +6,13 @@
 #include "grid.hpp"
 #include "nudft.hpp"
 #include "plan.hpp"
+#include "prof.hpp"
 #include "recon.hpp"
 
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 struct ksj_gpu_plan2 {
   ksj::gpu::Plan2* impl;
@@ -159,7 +161,7 @@ int ksj_gpu_plan2_create(const ksj_gpu_plan2_desc* desc, ksj_gpu_plan2** out_pla
     d.density = desc->density;
     d.centred = desc->centred;
     d.samples = desc->samples;
-    d.heavy_threshold = desc->heavy_threshold == 0 ? 128u : desc->heavy_threshold;
+    d.item_visits = desc->item_visits == 0 ? 64u : desc->item_visits;
     ksj::gpu::require(d.density <= KSJ_GPU_DENSITY_GIVEN, "unknown density mode");
     auto* handle = new ksj_gpu_plan2{nullptr};
     try {
@@ -183,8 +185,10 @@ int ksj_gpu_plan2_get_info(const ksj_gpu_plan2* plan, ksj_gpu_plan2_info* out_in
   return guarded([&] {
     ksj::gpu::require(plan && plan->impl && out_info, "null argument");
     ksj::gpu::require(out_info->struct_size >= sizeof(ksj_gpu_plan2_info), "info struct too small");
-    out_info->heavy_cells = plan->impl->heavy_count;
-    out_info->device_bytes = plan->impl->device_bytes;
+    out_info->items = plan->impl->item_count;
+    out_info->visits = plan->impl->visit_count;
+    out_info->device_bytes = plan->impl->device_bytes + plan->impl->visit_capacity * (sizeof(uint32_t) + sizeof(float4)) +
+                             static_cast<uint64_t>(plan->impl->item_capacity) * sizeof(uint32_t);
     out_info->beta = plan->impl->d.beta;
   });
 }
@@ -210,7 +214,7 @@ void ksj_gpu_plan2_destroy(ksj_gpu_plan2* plan) {
 
 size_t ksj_gpu_adjoint2_workspace_bytes(const ksj_gpu_plan2* plan, uint32_t channels) {
   if (!plan || !plan->impl) return 0;
-  return ksj::gpu::adjoint2_workspace_bytes(*plan->impl, channels);
+  return ksj::gpu::adjoint2_workspace_bytes(*plan->impl, channels, 1);
 }
 
 int ksj_gpu_adjoint2_execute(const ksj_gpu_plan2* plan, const void* d_kspace, uint32_t channels, void* d_images,
@@ -218,9 +222,49 @@ int ksj_gpu_adjoint2_execute(const ksj_gpu_plan2* plan, const void* d_kspace, ui
   return guarded([&] {
     ksj::gpu::require(plan && plan->impl && d_kspace && d_workspace, "null argument");
     ksj::gpu::require(d_images || d_rss, "adjoint needs an output");
-    ksj::gpu::adjoint2_execute(*plan->impl, static_cast<const float2*>(d_kspace), channels,
-                               static_cast<float2*>(d_images), layout, d_rss, d_workspace, workspace_bytes,
+    ksj::gpu::AdjointScan scan;
+    scan.kspace = static_cast<const float2*>(d_kspace);
+    scan.images = static_cast<float2*>(d_images);
+    scan.rss = d_rss;
+    ksj::gpu::adjoint2_execute(*plan->impl, &scan, 1, channels, layout, d_workspace, workspace_bytes,
                                as_stream(stream));
+  });
+}
+
+size_t ksj_gpu_adjoint2_batch_workspace_bytes(const ksj_gpu_plan2* plan, uint32_t channels, uint32_t scans) {
+  if (!plan || !plan->impl) return 0;
+  return ksj::gpu::adjoint2_workspace_bytes(*plan->impl, channels, scans);
+}
+
+int ksj_gpu_adjoint2_execute_batch(const ksj_gpu_plan2* plan, uint32_t count, const void* const* d_kspace,
+                                   uint32_t channels, void* const* d_images, int layout, float* const* d_rss,
+                                   void* d_workspace, size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(plan && plan->impl && d_kspace && d_workspace && count >= 1, "null argument");
+    std::vector<ksj::gpu::AdjointScan> scans(count);
+    for (uint32_t i = 0; i < count; ++i) {
+      scans[i].kspace = static_cast<const float2*>(d_kspace[i]);
+      scans[i].images = d_images ? static_cast<float2*>(d_images[i]) : nullptr;
+      scans[i].rss = d_rss ? d_rss[i] : nullptr;
+      ksj::gpu::require(scans[i].images || scans[i].rss, "every scan needs an output");
+    }
+    ksj::gpu::adjoint2_execute(*plan->impl, scans.data(), count, channels, layout, d_workspace, workspace_bytes,
+                               as_stream(stream));
+  });
+}
+
+int ksj_gpu_profile_enable(int on) {
+  return guarded([&] { ksj::gpu::prof_enable(on != 0); });
+}
+
+int ksj_gpu_profile_reset(void) {
+  return guarded([&] { ksj::gpu::prof_reset(); });
+}
+
+int ksj_gpu_profile_read(const char* stage, double* total_ms, uint64_t* count) {
+  return guarded([&] {
+    ksj::gpu::require(stage && total_ms && count, "null argument");
+    ksj::gpu::prof_read(stage, total_ms, count);
   });
 }
 

@@ -57,6 +57,39 @@ __global__ __launch_bounds__(256) void major_to_last_kernel(const float2* __rest
   }
 }
 
+__global__ __launch_bounds__(256) void major_split_kernel(const float2* __restrict__ in, CoilGroupOutputs out,
+                                                          uint64_t pixels) {
+  extern __shared__ float2 tile[];  // [kTilePixels][total + 1]
+  const uint32_t total = out.groups * out.coils;
+  const uint64_t p0 = static_cast<uint64_t>(blockIdx.x) * kTilePixels;
+  const uint32_t stride = total + 1;
+  const uint32_t np = static_cast<uint32_t>(p0 + kTilePixels <= pixels ? kTilePixels : pixels - p0);
+  for (uint32_t i = threadIdx.x; i < np * total; i += blockDim.x) {
+    const uint32_t c = i / np, pl = i % np;
+    tile[pl * stride + c] = in[static_cast<uint64_t>(c) * pixels + p0 + pl];
+  }
+  __syncthreads();
+  for (uint32_t g = 0; g < out.groups; ++g) {
+    if (out.coil_last[g]) {
+      float2* dst = out.coil_last[g] + p0 * out.coils;
+      for (uint32_t i = threadIdx.x; i < np * out.coils; i += blockDim.x) {
+        const uint32_t pl = i / out.coils, c = i % out.coils;
+        dst[i] = tile[pl * stride + g * out.coils + c];
+      }
+    }
+  }
+  for (uint32_t i = threadIdx.x; i < out.groups * np; i += blockDim.x) {
+    const uint32_t g = i / np, pl = i % np;
+    if (!out.rss[g]) continue;
+    float acc = 0.f;
+    for (uint32_t c = 0; c < out.coils; ++c) {
+      const float2 v = tile[pl * stride + g * out.coils + c];
+      acc += v.x * v.x + v.y * v.y;
+    }
+    out.rss[g][p0 + pl] = sqrtf(acc);
+  }
+}
+
 __global__ __launch_bounds__(256) void sense_kernel(const float2* __restrict__ img, const float2* __restrict__ sens,
                                                     float2* __restrict__ out, uint64_t pixels, uint32_t coils) {
   const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -91,6 +124,16 @@ void coil_major_to_last(const float2* d_in, float2* d_coil_last, float* d_rss, u
   const size_t shmem = static_cast<size_t>(kTilePixels) * (coils + 1) * sizeof(float2);
   hipLaunchKernelGGL(major_to_last_kernel, dim3(ceil_div(pixels, kTilePixels)), dim3(256), shmem, stream, d_in,
                      d_coil_last, d_rss, pixels, coils);
+  KSJ_LAUNCH_CHECK();
+}
+
+void coil_major_split(const float2* d_in, const CoilGroupOutputs& out, uint64_t pixels, hipStream_t stream) {
+  if (pixels == 0 || out.groups == 0) return;
+  require(out.groups <= kMaxCoilGroups && out.coils >= 1 && out.groups * out.coils <= 64,
+          "coil split supports at most 64 stacked coils");
+  const size_t shmem = static_cast<size_t>(kTilePixels) * (out.groups * out.coils + 1) * sizeof(float2);
+  hipLaunchKernelGGL(major_split_kernel, dim3(ceil_div(pixels, kTilePixels)), dim3(256), shmem, stream, d_in, out,
+                     pixels);
   KSJ_LAUNCH_CHECK();
 }
 

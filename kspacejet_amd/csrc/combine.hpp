@@ -15,6 +15,20 @@ void rss_coil_last(const float2* d_in, float* d_out, uint64_t pixels, uint32_t c
 void coil_major_to_last(const float2* d_in, float2* d_coil_last, float* d_rss, uint64_t pixels, uint32_t coils,
                         hipStream_t stream);
 
+constexpr uint32_t kMaxCoilGroups = 64;
+
+// Outputs of coil_major_split: images stacked along the coil axis in groups of
+// `coils` planes; group g writes coil-last images and/or its RSS image (null skips).
+struct CoilGroupOutputs {
+  float2* coil_last[kMaxCoilGroups];
+  float* rss[kMaxCoilGroups];
+  uint32_t groups = 0;
+  uint32_t coils = 0;
+};
+
+// Coil-major planes [groups * coils][P] -> per group coil-last [P][coils] and/or RSS [P].
+void coil_major_split(const float2* d_in, const CoilGroupOutputs& out, uint64_t pixels, hipStream_t stream);
+
 // x[p] = sum_c conj(S[p][c]) I[p][c] / sum_c |S[p][c]|^2, 0 where the denominator is 0.
 void sense_combine(const float2* d_images, const float2* d_sens, float2* d_out, uint64_t pixels, uint32_t coils,
                    hipStream_t stream);

@@ -152,77 +152,100 @@ __device__ inline void passes(float2* line, const float2* tw, int t) {
   }
 }
 
-__device__ inline uint64_t addr(const FftLines& L, uint64_t line, uint32_t e) {
-  const uint64_t p = line / L.lines, q = line % L.lines;
-  return p * L.plane_stride + q * L.line_stride + static_cast<uint64_t>(e) * L.elem_stride;
+__device__ inline uint64_t line_base(const FftLines& L, uint64_t line) {
+  const uint64_t p = line / L.lines, q = line - p * L.lines;
+  return p * L.plane_stride + q * L.line_stride;
 }
 
-// Loads `lpb` lines into LDS (line stride ls), rotated by in_offset.
-__device__ inline void load_lines(const FftArgs& a, float2* buf, int ls, int n, int lpb, uint64_t first,
-                                  uint64_t total) {
-  const int count = lpb * n;
+// Per-line base offsets of the block's lines (computed once per line, not per element).
+__device__ inline void stage_bases(const FftArgs& a, uint64_t* src_base, uint64_t* dst_base, uint32_t* line_q,
+                                   int lpb, uint64_t first, uint64_t total) {
+  for (int ly = threadIdx.x; ly < lpb; ly += blockDim.x) {
+    const uint64_t line = first + ly;
+    src_base[ly] = line < total ? line_base(a.src, line) : ~0ull;
+    dst_base[ly] = line < total ? line_base(a.dst, line) : ~0ull;
+    line_q[ly] = static_cast<uint32_t>(line % a.src.lines);
+  }
+}
+
+// Loads `lpb` lines into LDS (line stride ls), rotated by in_offset.  lpb is a power of two.
+template <int N>
+__device__ inline void load_lines(const FftArgs& a, float2* buf, int ls, int lpb_log2, const uint64_t* src_base) {
+  const int lpb = 1 << lpb_log2;
+  const int count = lpb * N;
+  const uint32_t es = static_cast<uint32_t>(a.src.elem_stride);
   for (int idx = threadIdx.x; idx < count; idx += blockDim.x) {
     int ly, e;
-    if (a.columns) {
-      ly = idx % lpb;
-      e = idx / lpb;
+    if (a.load_lines_fast) {
+      ly = idx & (lpb - 1);
+      e = idx >> lpb_log2;
     } else {
-      ly = idx / n;
-      e = idx % n;
+      ly = idx / N;
+      e = idx - ly * N;
     }
-    const uint64_t line = first + ly;
+    const uint64_t base = src_base[ly];
     float2 v = make_float2(0.f, 0.f);
-    if (line < total) {
+    if (base != ~0ull) {
       uint32_t src = static_cast<uint32_t>(e) + a.in_offset;
-      if (src >= static_cast<uint32_t>(n)) src -= n;
-      v = a.in[addr(a.src, line, src)];
+      if (src >= static_cast<uint32_t>(N)) src -= N;
+      v = a.in[base + static_cast<uint64_t>(src * es)];
     }
     buf[ly * ls + e] = v;
   }
 }
 
-__device__ inline void store_lines(const FftArgs& a, const float2* buf, int ls, int n, int lpb, uint64_t first,
-                                   uint64_t total) {
-  const int count = lpb * static_cast<int>(a.n_out);
+template <int N>
+__device__ inline void store_lines(const FftArgs& a, const float2* buf, int ls, int lpb_log2, const uint32_t* line_q,
+                                   const uint64_t* dst_base) {
+  const int lpb = 1 << lpb_log2;
+  const int nout = static_cast<int>(a.n_out);
+  const int count = lpb * nout;
+  const uint32_t es = static_cast<uint32_t>(a.dst.elem_stride);
   for (int idx = threadIdx.x; idx < count; idx += blockDim.x) {
     int ly, e;
-    if (a.columns) {
-      ly = idx % lpb;
-      e = idx / lpb;
+    if (a.store_lines_fast) {
+      ly = idx & (lpb - 1);
+      e = idx >> lpb_log2;
     } else {
-      ly = idx / a.n_out;
-      e = idx % a.n_out;
+      ly = idx / nout;
+      e = idx - ly * nout;
     }
-    const uint64_t line = first + ly;
-    if (line >= total) continue;
+    const uint64_t base = dst_base[ly];
+    if (base == ~0ull) continue;
     uint32_t bin = static_cast<uint32_t>(e) + a.out_offset;
-    bin %= static_cast<uint32_t>(n);
+    if (bin >= static_cast<uint32_t>(N)) bin -= N;
     float w = a.scale;
     if (a.out_weight) w *= a.out_weight[e];
-    if (a.line_weight) w *= a.line_weight[line % a.src.lines];
-    a.out[addr(a.dst, line, e)] = cscale(buf[ly * ls + bin], w);
+    if (a.line_weight) w *= a.line_weight[line_q[ly]];
+    a.out[base + static_cast<uint64_t>(static_cast<uint32_t>(e) * es)] = cscale(buf[ly * ls + bin], w);
   }
 }
 
 template <int N, bool INV>
-__global__ __launch_bounds__(1024) void fft_lds_kernel(FftArgs a, const float2* __restrict__ tw_global, int lpb) {
+__global__ __launch_bounds__(1024) void fft_lds_kernel(FftArgs a, const float2* __restrict__ tw_global, int lpb_log2) {
   constexpr int T = Shape<N>::T;
   constexpr int LS = N + 1;  // pad each LDS line by one element to spread banks across lines
   extern __shared__ float2 smem[];
+  __shared__ uint64_t src_base[64], dst_base[64];
+  __shared__ uint32_t line_q[64];
+  const int lpb = 1 << lpb_log2;
   float2* tw = smem;
   float2* buf = smem + N;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = tw_global[i];
   const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
   const uint64_t first = static_cast<uint64_t>(blockIdx.x) * lpb;
-  load_lines(a, buf, LS, N, lpb, first, total);
+  stage_bases(a, src_base, dst_base, line_q, lpb, first, total);
+  __syncthreads();
+  load_lines<N>(a, buf, LS, lpb_log2, src_base);
   __syncthreads();
   const int ly = threadIdx.x / T, t = threadIdx.x % T;
   passes<N, INV, 0, 1>(buf + ly * LS, tw, t);
-  store_lines(a, buf, LS, N, lpb, first, total);
+  store_lines<N>(a, buf, LS, lpb_log2, line_q, dst_base);
 }
 
 // Any length: direct DFT per output bin against the twiddle table (used for
 // lengths without a radix schedule, e.g. odd or small reference test sizes).
+template <int DUMMY>
 __global__ __launch_bounds__(256) void dft_direct_kernel(FftArgs a, const float2* __restrict__ tw_global, int lpb) {
   extern __shared__ float2 smem[];
   const int n = static_cast<int>(a.n);
@@ -233,7 +256,17 @@ __global__ __launch_bounds__(256) void dft_direct_kernel(FftArgs a, const float2
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = tw_global[i];
   const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
   const uint64_t first = static_cast<uint64_t>(blockIdx.x) * lpb;
-  load_lines(a, buf, ls, n, lpb, first, total);
+  for (int idx = threadIdx.x; idx < lpb * n; idx += blockDim.x) {
+    const int ly = idx / n, e = idx % n;
+    const uint64_t line = first + ly;
+    float2 v = make_float2(0.f, 0.f);
+    if (line < total) {
+      uint32_t src = static_cast<uint32_t>(e) + a.in_offset;
+      if (src >= static_cast<uint32_t>(n)) src -= n;
+      v = a.in[line_base(a.src, line) + src * a.src.elem_stride];
+    }
+    buf[ly * ls + e] = v;
+  }
   __syncthreads();
   for (int idx = threadIdx.x; idx < lpb * n; idx += blockDim.x) {
     const int ly = idx / n, k = idx % n;
@@ -246,7 +279,17 @@ __global__ __launch_bounds__(256) void dft_direct_kernel(FftArgs a, const float2
     res[ly * ls + k] = acc;
   }
   __syncthreads();
-  store_lines(a, res, ls, n, lpb, first, total);
+  for (int idx = threadIdx.x; idx < lpb * static_cast<int>(a.n_out); idx += blockDim.x) {
+    const int ly = idx / a.n_out, e = idx % a.n_out;
+    const uint64_t line = first + ly;
+    if (line >= total) continue;
+    uint32_t bin = static_cast<uint32_t>(e) + a.out_offset;
+    if (bin >= static_cast<uint32_t>(n)) bin -= n;
+    float w = a.scale;
+    if (a.out_weight) w *= a.out_weight[e];
+    if (a.line_weight) w *= a.line_weight[line % a.src.lines];
+    a.out[line_base(a.dst, line) + static_cast<uint64_t>(e) * a.dst.elem_stride] = cscale(res[ly * ls + bin], w);
+  }
 }
 
 __global__ void twiddle_kernel(float2* tw, uint32_t n) {
@@ -257,21 +300,44 @@ __global__ void twiddle_kernel(float2* tw, uint32_t n) {
   tw[m] = make_float2(static_cast<float>(c), static_cast<float>(s));
 }
 
+int max_dynamic_lds() {
+  static int cached = -1;
+  if (cached < 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+      cached = v;
+    else
+      cached = 65536;
+  }
+  return cached;
+}
+
 template <int N>
 void launch_fast(const FftArgs& a, const float2* tw, hipStream_t stream, int lpb) {
   constexpr int T = Shape<N>::T;
-  if (lpb <= 0) lpb = a.columns ? 16 : (T >= 64 ? 4 : 256 / T);
-  if (lpb * T > 1024) lpb = 1024 / T;
-  // Keep the LDS footprint within 64 KiB so at least two workgroups fit per CU.
-  const int lds_cap = static_cast<int>((65536 / sizeof(float2) - N) / (N + 1));
+  if (lpb <= 0) lpb = (a.load_lines_fast || a.store_lines_fast) ? 16 : (T >= 64 ? 4 : 256 / T);
+  while (lpb * T > 1024) lpb /= 2;
+  // LDS: twiddles + lpb padded lines (plus 1 KiB of static base offsets).
+  const int lds_cap = static_cast<int>(((max_dynamic_lds() - 1024) / sizeof(float2) - N) / (N + 1));
   while (lpb > 1 && lpb > lds_cap) lpb /= 2;
+  int lpb_log2 = 0;
+  while ((1 << (lpb_log2 + 1)) <= lpb) ++lpb_log2;
+  lpb = 1 << lpb_log2;
   const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
   const size_t shmem = (static_cast<size_t>(N) + static_cast<size_t>(lpb) * (N + 1)) * sizeof(float2);
   const unsigned grid = ceil_div(total, static_cast<uint64_t>(lpb));
-  if (a.inverse)
-    hipLaunchKernelGGL((fft_lds_kernel<N, true>), dim3(grid), dim3(lpb * T), shmem, stream, a, tw, lpb);
-  else
-    hipLaunchKernelGGL((fft_lds_kernel<N, false>), dim3(grid), dim3(lpb * T), shmem, stream, a, tw, lpb);
+  if (a.inverse) {
+    if (shmem > 65536)
+      KSJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_lds_kernel<N, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+    hipLaunchKernelGGL((fft_lds_kernel<N, true>), dim3(grid), dim3(lpb * T), shmem, stream, a, tw, lpb_log2);
+  } else {
+    if (shmem > 65536)
+      KSJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_lds_kernel<N, false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+    hipLaunchKernelGGL((fft_lds_kernel<N, false>), dim3(grid), dim3(lpb * T), shmem, stream, a, tw, lpb_log2);
+  }
   KSJ_LAUNCH_CHECK();
 }
 
@@ -330,11 +396,11 @@ void fft_launch(const FftArgs& a, hipStream_t stream, int lpb) {
     case 1024: return launch_fast<1024>(a, tw, stream, lpb);
     default: break;
   }
-  int l = a.columns ? 8 : 1;
+  int l = (a.load_lines_fast || a.store_lines_fast) ? 8 : 1;
   while (l > 1 && (static_cast<size_t>(a.n) + 2ull * l * (a.n + 1)) * sizeof(float2) > 60 * 1024) l /= 2;
   const size_t shmem = (static_cast<size_t>(a.n) + 2ull * l * (a.n + 1)) * sizeof(float2);
   require(shmem <= 160 * 1024, "direct DFT line does not fit in LDS");
-  hipLaunchKernelGGL(dft_direct_kernel, dim3(ceil_div(total, l)), dim3(256), shmem, stream, a, tw, l);
+  hipLaunchKernelGGL(dft_direct_kernel<0>, dim3(ceil_div(total, l)), dim3(256), shmem, stream, a, tw, l);
   KSJ_LAUNCH_CHECK();
 }
 

@@ -31,7 +31,11 @@ struct FftArgs {
   const float* out_weight = nullptr;   // per stored element e' (length n_out) or null
   const float* line_weight = nullptr;  // per line index q or null
   int inverse = 1;              // +1: sum x e^{+2 pi i jk/n}; 0: forward (e^{-...})
-  int columns = 0;              // 0: lines are contiguous rows; 1: lines are strided columns
+  // Thread mapping of the global loads / stores: consecutive threads take
+  // consecutive elements of a line (0) or the same element of consecutive lines
+  // (1).  Pick the one whose addresses are contiguous in memory.
+  int load_lines_fast = 0;
+  int store_lines_fast = 0;
 };
 
 // Returns true when an LDS Stockham kernel is compiled for length n; other

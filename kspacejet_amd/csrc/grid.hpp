@@ -5,19 +5,30 @@
 
 namespace ksj::gpu {
 
-constexpr uint32_t kCoilGroup = 8;  // coils accumulated per thread in the gather
+// Padded coil count Cp of the sorted sample rows and of the grid: the next power
+// of two >= max(8, coils) (lanes are mapped to coils).
+uint32_t padded_coils(uint32_t coils);
 
-inline uint32_t padded_coils(uint32_t coils) { return (coils + kCoilGroup - 1) / kCoilGroup * kCoilGroup; }
+constexpr uint32_t kMaxStackedCoils = 64;
 
-// Applies the plan's per-sample weight and moves channel-major samples [C][S]
-// into sorted, coil-fastest rows out[k][Cp] (Cp = padded_coils(C), pad zeroed).
-void grid_prep(const Plan2& plan, const float2* d_kspace, uint32_t coils, float2* d_sorted, hipStream_t stream);
+// Coil planes of one gather pass: coil c reads samples planes[c][0..S).  Several
+// scans on the same trajectory are stacked along the coil axis this way.
+struct CoilPlanes {
+  const float2* plane[kMaxStackedCoils];
+  uint32_t coils = 0;
+};
 
-// Gathers every grid cell of all coils: grid[c][G_r][G_c] (plane-major).  Cells
-// with more than heavy_threshold contributing samples are reduced by one
-// workgroup each in a fixed order; all others by one thread per cell and coil
-// group.  Every cell is written (no prior clear needed) and the result is
+// Applies the plan's per-sample weight and moves the coil planes into sorted,
+// coil-fastest rows out[k][Cp] (pad zeroed).
+void grid_prep(const Plan2& plan, const CoilPlanes& planes, float2* d_sorted, hipStream_t stream);
+
+// Bytes of the per-item partial-sum buffer grid_gather needs (blocks with several items).
+size_t grid_partial_bytes(const Plan2& plan, uint32_t coils);
+
+// Gathers every grid cell of all coils into the coil-fastest grid [G_r][G_c][Cp].
+// Every cell is written (no prior clear needed) and the result is
 // bit-reproducible for a given plan.
-void grid_gather(const Plan2& plan, const float2* d_sorted, uint32_t coils, float2* d_grid, hipStream_t stream);
+void grid_gather(const Plan2& plan, const float2* d_sorted, uint32_t coils, float2* d_grid, float2* d_partial,
+                 hipStream_t stream);
 
 }  // namespace ksj::gpu

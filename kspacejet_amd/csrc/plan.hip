@@ -1,4 +1,4 @@
-// Trajectory plan kernels: FP64 coordinates, binning, taps, heavy cells, deapodization.
+// Trajectory plan kernels: FP64 coordinates, binning, taps, gather schedule, deapodization.
 #include "plan.hpp"
 #include "sort.hpp"
 
@@ -117,28 +117,102 @@ __global__ void taps_kernel(TapArgs a) {
   }
 }
 
-// Number of samples a cell gathers (sum over its W x W contributing bins).
-__global__ void heavy_kernel(const uint32_t* start, uint32_t grid_rows, uint32_t grid_cols, int width,
-                             uint32_t threshold, uint8_t* flag, uint32_t* list, uint32_t capacity,
-                             uint32_t* counter) {
-  const uint64_t m = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (m >= static_cast<uint64_t>(grid_rows) * grid_cols) return;
-  const int y = static_cast<int>(m / grid_cols), x = static_cast<int>(m % grid_cols);
-  const int lead = width / 2 - 1;
-  uint32_t total = 0;
-  for (int ty = 0; ty < width; ++ty) {
-    const int by = (y - ty + lead + static_cast<int>(grid_rows)) % static_cast<int>(grid_rows);
-    for (int tx = 0; tx < width; ++tx) {
-      const int bx = (x - tx + lead + static_cast<int>(grid_cols)) % static_cast<int>(grid_cols);
-      const uint64_t b = static_cast<uint64_t>(by) * grid_cols + bx;
-      total += start[b + 1] - start[b];
+// ----------------------------------------------------------------------------- gather schedule
+// Window of 2x2 block (BY, BX): bin rows 2BY - W/2 + r and columns 2BX - W/2 + c for
+// r, c in [0, W]; cell (2BY + dy, 2BX + dx) uses row tap dy - r + W - 1 and column
+// tap dx - c + W - 1 of a sample in window bin (r, c) (taps outside [0, W) weigh 0).
+struct ScheduleArgs {
+  const uint32_t* start;
+  const float* taps;  // [S][2W]
+  uint32_t grid_rows, grid_cols, block_cols;
+  uint64_t nblocks;
+  int width;
+  uint32_t* row_off;  // counts, then offsets
+  uint32_t* visit_k;
+  float4* visit_w;
+};
+
+__device__ inline uint64_t window_bin(const ScheduleArgs& a, uint64_t blk, int r, int c) {
+  const int W = a.width;
+  const int gr = static_cast<int>(a.grid_rows), gc = static_cast<int>(a.grid_cols);
+  int by = 2 * static_cast<int>(blk / a.block_cols) - W / 2 + r;
+  int bx = 2 * static_cast<int>(blk % a.block_cols) - W / 2 + c;
+  by = ((by % gr) + gr) % gr;
+  bx = ((bx % gc) + gc) % gc;
+  return static_cast<uint64_t>(by) * a.grid_cols + bx;
+}
+
+__global__ void visit_count_kernel(ScheduleArgs a) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t rows = a.nblocks * (a.width + 1);
+  if (i > rows) return;
+  if (i == rows) {
+    a.row_off[i] = 0;
+    return;
+  }
+  const uint64_t blk = i / (a.width + 1);
+  const int r = static_cast<int>(i % (a.width + 1));
+  uint32_t n = 0;
+  for (int c = 0; c <= a.width; ++c) {
+    const uint64_t b = window_bin(a, blk, r, c);
+    n += a.start[b + 1] - a.start[b];
+  }
+  a.row_off[i] = n;
+}
+
+__global__ void visit_fill_kernel(ScheduleArgs a) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= a.nblocks * (a.width + 1)) return;
+  const int W = a.width;
+  const uint64_t blk = i / (W + 1);
+  const int r = static_cast<int>(i % (W + 1));
+  uint32_t pos = a.row_off[i];
+  const int ty0 = 0 - r + W - 1, ty1 = 1 - r + W - 1;
+  for (int c = 0; c <= W; ++c) {
+    const uint64_t b = window_bin(a, blk, r, c);
+    const int tx0 = 0 - c + W - 1, tx1 = 1 - c + W - 1;
+    for (uint32_t k = a.start[b]; k < a.start[b + 1]; ++k) {
+      const float* t = a.taps + static_cast<uint64_t>(k) * (2 * W);
+      const float wy0 = (ty0 >= 0 && ty0 < W) ? t[ty0] : 0.f, wy1 = (ty1 >= 0 && ty1 < W) ? t[ty1] : 0.f;
+      const float wx0 = (tx0 >= 0 && tx0 < W) ? t[W + tx0] : 0.f, wx1 = (tx1 >= 0 && tx1 < W) ? t[W + tx1] : 0.f;
+      a.visit_k[pos] = k;
+      a.visit_w[pos] = make_float4(wy0 * wx0, wy0 * wx1, wy1 * wx0, wy1 * wx1);
+      ++pos;
     }
   }
-  const bool heavy = total > threshold;
-  flag[m] = heavy ? 1 : 0;
-  if (heavy) {
-    const uint32_t i = atomicAdd(counter, 1u);
-    if (i < capacity) list[i] = static_cast<uint32_t>(m);
+}
+
+__global__ void item_count_kernel(const uint32_t* row_off, uint64_t nblocks, int width, uint32_t item_visits,
+                                  uint32_t* first, uint32_t* multi, uint32_t* counter) {
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b > nblocks) return;
+  if (b == nblocks) {
+    first[b] = 0;
+    return;
+  }
+  const uint32_t v = row_off[(b + 1) * (width + 1)] - row_off[b * (width + 1)];
+  const uint32_t n = v == 0 ? 1u : (v + item_visits - 1) / item_visits;
+  first[b] = n;
+  if (n > 1) multi[atomicAdd(counter, 1u)] = static_cast<uint32_t>(b);
+}
+
+__global__ void slot_count_kernel(const uint32_t* first, uint64_t nblocks, uint32_t* slots) {
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b > nblocks) return;
+  const uint32_t n = b < nblocks ? first[b + 1] - first[b] : 0u;
+  slots[b] = n > 1 ? n : 0u;
+}
+
+__global__ void item_fill_kernel(const uint32_t* first, const uint32_t* slot, const uint32_t* row_off, int width,
+                                 uint32_t item_visits, uint64_t nblocks, uint4* info) {
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  const uint32_t vb = row_off[b * (width + 1)], ve = row_off[(b + 1) * (width + 1)];
+  const uint32_t n = first[b + 1] - first[b];
+  for (uint32_t j = 0; j < n; ++j) {
+    const uint32_t v0 = vb + j * item_visits;
+    const uint32_t v1 = v0 + item_visits < ve ? v0 + item_visits : ve;
+    info[first[b] + j] = make_uint4(static_cast<uint32_t>(b), v0, v1, n > 1 ? slot[b] + j : 0xFFFFFFFFu);
   }
 }
 
@@ -210,7 +284,7 @@ Plan2* plan2_create(const Plan2Desc& desc) {
   require(desc.centred == 1 || (desc.grid_rows == desc.image_rows && desc.grid_cols == desc.image_cols),
           "uncentred output requires an unoversampled grid");
   require(desc.samples >= 1, "plan needs at least one sample");
-  require(desc.heavy_threshold >= 1, "heavy threshold must be positive");
+  require(desc.item_visits >= 1 && desc.item_visits <= 4096, "item visits must be in [1, 4096]");
   auto* p = new Plan2();
   p->d = desc;
   if (p->d.kernel == kKernelKaiserBessel && p->d.beta <= 0.0) {
@@ -224,8 +298,10 @@ Plan2* plan2_create(const Plan2Desc& desc) {
     p->nbins = static_cast<uint64_t>(desc.grid_rows) * desc.grid_cols;
     p->key_bits = 1;
     while ((1ull << p->key_bits) < p->nbins) ++p->key_bits;
-    const uint64_t cap = std::min<uint64_t>(p->nbins, S * desc.width * desc.width / desc.heavy_threshold + 1);
-    p->heavy_capacity = static_cast<uint32_t>(cap);
+    p->block_rows = (desc.grid_rows + 1) / 2;
+    p->block_cols = (desc.grid_cols + 1) / 2;
+    p->nblocks = static_cast<uint64_t>(p->block_rows) * p->block_cols;
+    p->item_visits = desc.item_visits;
     size_t total = 0;
     p->weight = alloc<float2>(S, &total);
     p->rank = alloc<uint32_t>(S, &total);
@@ -233,12 +309,15 @@ Plan2* plan2_create(const Plan2Desc& desc) {
     p->bins = alloc<uint32_t>(S, &total);
     p->cell_start = alloc<uint32_t>(p->nbins + 1, &total);
     p->taps = alloc<float>(S * 2 * desc.width, &total);
-    p->heavy_flag = alloc<uint8_t>(p->nbins, &total);
-    p->heavy_list = alloc<uint32_t>(cap, &total);
+    p->visit_row_off = alloc<uint32_t>(p->nblocks * (desc.width + 1) + 1, &total);
+    p->block_first_item = alloc<uint32_t>(p->nblocks + 1, &total);
+    p->block_first_slot = alloc<uint32_t>(p->nblocks + 1, &total);
+    p->multi_blocks = alloc<uint32_t>(p->nblocks, &total);
     p->frac = alloc<double>(2 * S, &total);
     p->sorted_keys = alloc<uint32_t>(S, &total);
     p->counters = alloc<uint32_t>(4, &total);
-    p->sort_scratch = alloc<char>(radix_sort_scratch_bytes(S), &total);
+    const uint64_t scan_n = std::max<uint64_t>(S, p->nblocks * (desc.width + 1) + 1);
+    p->sort_scratch = alloc<char>(std::max(radix_sort_scratch_bytes(scan_n), scan_scratch_bytes(scan_n)), &total);
     if (p->d.kernel == kKernelKaiserBessel) {
       p->deapod_rows = alloc<float>(desc.image_rows, &total);
       p->deapod_cols = alloc<float>(desc.image_cols, &total);
@@ -261,8 +340,10 @@ Plan2* plan2_create(const Plan2Desc& desc) {
 
 void plan2_destroy(Plan2* p) {
   if (!p) return;
-  void* ptrs[] = {p->weight, p->rank, p->src, p->bins, p->cell_start, p->taps, p->heavy_flag, p->heavy_list,
-                  p->frac, p->sorted_keys, p->counters, p->sort_scratch, p->deapod_rows, p->deapod_cols};
+  void* ptrs[] = {p->weight,        p->rank,        p->src,          p->bins,         p->cell_start,
+                  p->taps,          p->visit_row_off, p->visit_k,    p->visit_w,      p->block_first_item,
+                  p->block_first_slot, p->item_info,    p->multi_blocks, p->frac,         p->sorted_keys,
+                  p->counters,      p->sort_scratch, p->deapod_rows, p->deapod_cols};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -305,17 +386,79 @@ void plan2_set_trajectory(Plan2* p, const float* d_traj, const float* d_dcf, hip
   ta.taps = p->taps;
   hipLaunchKernelGGL(taps_kernel, dim3(ceil_div(S, 256)), dim3(256), 0, stream, ta);
   KSJ_LAUNCH_CHECK();
-  hipLaunchKernelGGL(heavy_kernel, dim3(ceil_div(p->nbins, 256)), dim3(256), 0, stream, p->cell_start,
-                     p->d.grid_rows, p->d.grid_cols, static_cast<int>(p->d.width), p->d.heavy_threshold,
-                     p->heavy_flag, p->heavy_list, p->heavy_capacity, p->counters);
-  KSJ_LAUNCH_CHECK();
   uint32_t host_counters[4] = {0, 0, 0, 0};
   KSJ_HIP(hipMemcpyAsync(host_counters, p->counters, sizeof(host_counters), hipMemcpyDeviceToHost, stream));
   KSJ_HIP(hipStreamSynchronize(stream));
   if (host_counters[1] & kErrBadCoordinate)
     throw Error(1, "trajectory must be finite and within [-pi, pi] radians per pixel");
-  require(host_counters[0] <= p->heavy_capacity, "heavy-cell list overflow");
-  p->heavy_count = host_counters[0];
+
+  // Gather schedule: visits per (block, window row) -> offsets -> visit records.
+  ScheduleArgs sa{};
+  sa.start = p->cell_start;
+  sa.taps = p->taps;
+  sa.grid_rows = p->d.grid_rows;
+  sa.grid_cols = p->d.grid_cols;
+  sa.block_cols = p->block_cols;
+  sa.nblocks = p->nblocks;
+  sa.width = static_cast<int>(p->d.width);
+  sa.row_off = p->visit_row_off;
+  const uint64_t nrows = p->nblocks * (p->d.width + 1);
+  hipLaunchKernelGGL(visit_count_kernel, dim3(ceil_div(nrows + 1, 256)), dim3(256), 0, stream, sa);
+  KSJ_LAUNCH_CHECK();
+  exclusive_scan_u32(p->visit_row_off, p->visit_row_off, nrows + 1, p->sort_scratch, stream);
+  uint32_t visits = 0;
+  KSJ_HIP(hipMemcpyAsync(&visits, p->visit_row_off + nrows, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  KSJ_HIP(hipStreamSynchronize(stream));
+  if (visits > p->visit_capacity) {  // first trajectory on this plan, or a denser one
+    if (p->visit_k) KSJ_HIP(hipFree(p->visit_k));
+    if (p->visit_w) KSJ_HIP(hipFree(p->visit_w));
+    p->visit_k = nullptr;
+    p->visit_w = nullptr;
+    p->visit_capacity = visits;
+    KSJ_HIP(hipMalloc(&p->visit_k, std::max<size_t>(1, visits) * sizeof(uint32_t)));
+    KSJ_HIP(hipMalloc(&p->visit_w, std::max<size_t>(1, visits) * sizeof(float4)));
+  }
+  p->visit_count = visits;
+  sa.visit_k = p->visit_k;
+  sa.visit_w = p->visit_w;
+  hipLaunchKernelGGL(visit_fill_kernel, dim3(ceil_div(nrows, 256)), dim3(256), 0, stream, sa);
+  KSJ_LAUNCH_CHECK();
+
+  // Work items: ceil(visits / item_visits) per block (at least one, so every cell is written).
+  KSJ_HIP(hipMemsetAsync(p->counters, 0, sizeof(uint32_t), stream));
+  hipLaunchKernelGGL(item_count_kernel, dim3(ceil_div(p->nblocks + 1, 256)), dim3(256), 0, stream, p->visit_row_off,
+                     p->nblocks, static_cast<int>(p->d.width), p->item_visits, p->block_first_item, p->multi_blocks,
+                     p->counters);
+  KSJ_LAUNCH_CHECK();
+  exclusive_scan_u32(p->block_first_item, p->block_first_item, p->nblocks + 1, p->sort_scratch, stream);
+  uint32_t items = 0, multi = 0;
+  KSJ_HIP(hipMemcpyAsync(&items, p->block_first_item + p->nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  KSJ_HIP(hipMemcpyAsync(&multi, p->counters, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  KSJ_HIP(hipStreamSynchronize(stream));
+  if (items > p->item_capacity) {
+    if (p->item_info) KSJ_HIP(hipFree(p->item_info));
+    p->item_info = nullptr;
+    p->item_capacity = items;
+    KSJ_HIP(hipMalloc(&p->item_info, std::max<size_t>(1, items) * sizeof(uint4)));
+  }
+  p->item_count = items;
+  // Partial-sum slots: consecutive for the items of each multi-item block.
+  hipLaunchKernelGGL(slot_count_kernel, dim3(ceil_div(p->nblocks + 1, 256)), dim3(256), 0, stream,
+                     p->block_first_item, p->nblocks, p->block_first_slot);
+  KSJ_LAUNCH_CHECK();
+  exclusive_scan_u32(p->block_first_slot, p->block_first_slot, p->nblocks + 1, p->sort_scratch, stream);
+  uint32_t slots = 0;
+  KSJ_HIP(hipMemcpyAsync(&slots, p->block_first_slot + p->nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  hipLaunchKernelGGL(item_fill_kernel, dim3(ceil_div(p->nblocks, 256)), dim3(256), 0, stream, p->block_first_item,
+                     p->block_first_slot, p->visit_row_off, static_cast<int>(p->d.width), p->item_visits, p->nblocks,
+                     p->item_info);
+  KSJ_LAUNCH_CHECK();
+  KSJ_HIP(hipStreamSynchronize(stream));
+  p->slot_count = slots;
+  // The multi-item list is in append order; each of its blocks is summed on its
+  // own, so the order changes which thread sums a block, never the sum.
+  p->multi_count = multi;
+  KSJ_HIP(hipStreamSynchronize(stream));
   p->has_trajectory = true;
 }
 

@@ -11,8 +11,8 @@
 //                s = (centred ? N/2 : 0) - origin, evaluated in FP64.
 //   taps         per sorted sample W row taps and W column taps of the kernel
 //                (Kaiser-Bessel or linear), evaluated in FP64, stored as float.
-//   heavy cells  cells whose gather list exceeds heavy_threshold samples; these
-//                are computed by a workgroup each with a fixed-order reduction.
+//   schedule     per 2x2 cell block, the list of (sample, four weights) visits,
+//                cut into equal work items so dense k-space centres are split.
 //   deapod       1 / phi_hat per output row / column (Kaiser-Bessel only), FP64.
 #pragma once
 
@@ -33,7 +33,7 @@ struct Plan2Desc {
   uint32_t density = kDensityRamp;
   uint32_t centred = 1;                     // 1: image pixel r reads bin (r - N/2) mod G; 0: bin r
   uint32_t samples = 0;
-  uint32_t heavy_threshold = 128;
+  uint32_t item_visits = 64;                // visits per gather work item (one wave)
 };
 
 struct Plan2 {
@@ -48,16 +48,34 @@ struct Plan2 {
   uint32_t* bins = nullptr;        // [S] sample bins (original order), then sorted keys
   uint32_t* cell_start = nullptr;  // [nbins + 1]
   float* taps = nullptr;           // [S][2W] sorted: W row taps then W column taps
-  uint8_t* heavy_flag = nullptr;   // [nbins]
-  uint32_t* heavy_list = nullptr;  // [heavy_capacity]
-  uint32_t heavy_capacity = 0;
-  uint32_t heavy_count = 0;        // host copy, valid after set_trajectory
+  // Gather schedule.  Cells are taken in 2x2 blocks; a block's visits are the
+  // samples of its (W+1) x (W+1) contributing bins (rows then columns of that bin
+  // window, sorted order inside a bin), each with the four weights
+  // phi(row tap) * phi(column tap) of the block's cells (0 outside the footprint).
+  // Blocks are cut into work items of at most item_visits visits; a block with
+  // several items is summed item by item in item order.
+  uint32_t block_rows = 0, block_cols = 0;
+  uint64_t nblocks = 0;
+  uint32_t item_visits = 64;
+  uint32_t* visit_row_off = nullptr;  // [nblocks * (W+1) + 1] first visit of each (block, window row)
+  uint32_t* visit_k = nullptr;        // [visit_capacity] sorted sample position
+  float4* visit_w = nullptr;          // [visit_capacity] weights of cells (0,0) (0,1) (1,0) (1,1)
+  uint64_t visit_capacity = 0;
+  uint64_t visit_count = 0;           // host copy
+  uint32_t* block_first_item = nullptr;  // [nblocks + 1]
+  uint32_t* block_first_slot = nullptr;  // [nblocks + 1] first partial slot (multi-item blocks only)
+  uint4* item_info = nullptr;            // [item_capacity] {block, first visit, end visit, slot or ~0}
+  uint32_t item_capacity = 0;
+  uint32_t item_count = 0;               // host copy
+  uint32_t slot_count = 0;               // partial slots (items of multi-item blocks), host copy
+  uint32_t* multi_blocks = nullptr;      // [nblocks] blocks with more than one item
+  uint32_t multi_count = 0;              // host copy
   float* deapod_rows = nullptr;    // [N_r] or null
   float* deapod_cols = nullptr;    // [N_c] or null
   // scratch used only while building
   double* frac = nullptr;          // [S][2]
   uint32_t* sorted_keys = nullptr; // [S]
-  uint32_t* counters = nullptr;    // [4]: heavy count, error flags
+  uint32_t* counters = nullptr;    // [4]: multi-item block count, error flags
   void* sort_scratch = nullptr;
   size_t device_bytes = 0;
   bool has_trajectory = false;
@@ -67,7 +85,7 @@ Plan2* plan2_create(const Plan2Desc& desc);
 void plan2_destroy(Plan2* plan);
 // Recomputes every table from a float32 [S][2] radian-per-pixel trajectory on the
 // device.  dcf is read only when density == kDensityGiven.  Synchronises `stream`
-// once (to size the heavy-cell launch) and throws on a trajectory that cannot be
+// (the schedule is sized on the host) and throws on a trajectory that cannot be
 // mapped (non-finite or outside [-pi, pi]).
 void plan2_set_trajectory(Plan2* plan, const float* d_traj, const float* d_dcf, hipStream_t stream);
 

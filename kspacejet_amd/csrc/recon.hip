@@ -4,6 +4,9 @@
 #include "combine.hpp"
 #include "fft.hpp"
 #include "grid.hpp"
+#include "prof.hpp"
+
+#include <algorithm>
 
 namespace ksj::gpu {
 namespace {
@@ -55,7 +58,6 @@ void ifft2_coil_images(const float2* d_kspace, float2* d_images, uint32_t channe
   a.out_offset = centred ? (cols - cols / 2) % cols : 0;
   a.scale = 1.0f / static_cast<float>(cols);
   a.inverse = 1;
-  a.columns = 0;
   fft_launch(a, stream);
   FftArgs b;
   b.in = tmp;
@@ -66,71 +68,132 @@ void ifft2_coil_images(const float2* d_kspace, float2* d_images, uint32_t channe
   b.out_offset = centred ? (rows - rows / 2) % rows : 0;
   b.scale = 1.0f / static_cast<float>(rows);
   b.inverse = 1;
-  b.columns = 1;
+  b.load_lines_fast = b.store_lines_fast = 1;
   fft_launch(b, stream);
   if (layout == kLayoutCoilLast)
     coil_major_to_last(img, d_images, nullptr, static_cast<uint64_t>(rows) * cols, channels, stream);
 }
 
-size_t adjoint2_workspace_bytes(const Plan2& plan, uint32_t channels) {
-  const auto& d = plan.d;
-  const size_t cp = padded_coils(channels);
-  const size_t sorted = static_cast<size_t>(d.samples) * cp * sizeof(float2);
-  const size_t grid = static_cast<size_t>(channels) * d.grid_rows * d.grid_cols * sizeof(float2);
-  const size_t rowout = static_cast<size_t>(channels) * d.grid_rows * d.image_cols * sizeof(float2);
-  const size_t img = static_cast<size_t>(channels) * d.image_rows * d.image_cols * sizeof(float2);
-  return align256(sorted) + align256(grid) + align256(rowout) + align256(img);
+uint32_t adjoint2_scans_per_pass(uint32_t channels) {
+  return channels >= kMaxStackedCoils ? 1u : kMaxStackedCoils / channels;
 }
 
-void adjoint2_execute(const Plan2& plan, const float2* d_kspace, uint32_t channels, float2* d_images, int layout,
-                      float* d_rss, void* d_ws, size_t ws_bytes, hipStream_t stream) {
+size_t adjoint2_workspace_bytes(const Plan2& plan, uint32_t channels, uint32_t scans) {
+  const auto& d = plan.d;
+  const uint32_t per = std::min(std::max(scans, 1u), adjoint2_scans_per_pass(channels));
+  const uint32_t total = per * channels;
+  const size_t cp = padded_coils(total);
+  const size_t sorted = static_cast<size_t>(d.samples) * cp * sizeof(float2);
+  const size_t grid = cp * d.grid_rows * d.grid_cols * sizeof(float2);
+  const size_t rowout = static_cast<size_t>(total) * d.grid_rows * d.image_cols * sizeof(float2);
+  const size_t img = static_cast<size_t>(total) * d.image_rows * d.image_cols * sizeof(float2);
+  return align256(sorted) + align256(grid) + align256(rowout) + align256(img) +
+         align256(grid_partial_bytes(plan, total));
+}
+
+void adjoint2_execute(const Plan2& plan, const AdjointScan* scans, uint32_t count, uint32_t channels, int layout,
+                      void* d_ws, size_t ws_bytes, hipStream_t stream) {
   const auto& d = plan.d;
   require(plan.has_trajectory, "plan has no trajectory");
-  require(channels >= 1 && channels <= 64, "adjoint supports 1..64 channels");
+  require(channels >= 1 && channels <= kMaxStackedCoils, "adjoint supports 1..64 channels");
   require(layout == kLayoutCoilLast || layout == kLayoutCoilMajor, "unknown image layout");
-  require(ws_bytes >= adjoint2_workspace_bytes(plan, channels), "adjoint workspace too small");
-  char* p = static_cast<char*>(d_ws);
-  const size_t cp = padded_coils(channels);
-  auto* sorted = reinterpret_cast<float2*>(p);
-  p += align256(static_cast<size_t>(d.samples) * cp * sizeof(float2));
-  auto* grid = reinterpret_cast<float2*>(p);
-  p += align256(static_cast<size_t>(channels) * d.grid_rows * d.grid_cols * sizeof(float2));
-  auto* rowout = reinterpret_cast<float2*>(p);
-  p += align256(static_cast<size_t>(channels) * d.grid_rows * d.image_cols * sizeof(float2));
-  auto* img = reinterpret_cast<float2*>(p);
-
-  grid_prep(plan, d_kspace, channels, sorted, stream);
-  grid_gather(plan, sorted, channels, grid, stream);
-
-  FftArgs a;
-  a.in = grid;
-  a.out = rowout;
-  a.src = rows_of(channels, d.grid_rows, d.grid_cols, d.grid_cols);
-  a.dst = rows_of(channels, d.grid_rows, d.image_cols, d.image_cols);
-  a.n = d.grid_cols;
-  a.n_out = d.image_cols;
-  a.out_offset = d.centred ? (d.grid_cols - d.image_cols / 2) % d.grid_cols : 0;
-  a.inverse = 1;
-  a.columns = 0;
-  fft_launch(a, stream);
-
-  FftArgs b;
-  b.in = rowout;
-  b.out = (layout == kLayoutCoilMajor && d_images) ? d_images : img;
-  b.src = cols_of(channels, d.grid_rows, d.image_cols);
-  b.dst = cols_of(channels, d.image_rows, d.image_cols);
-  b.n = d.grid_rows;
-  b.n_out = d.image_rows;
-  b.out_offset = d.centred ? (d.grid_rows - d.image_rows / 2) % d.grid_rows : 0;
-  b.out_weight = plan.deapod_rows;
-  b.line_weight = plan.deapod_cols;
-  b.inverse = 1;
-  b.columns = 1;
-  fft_launch(b, stream);
-
+  require(count >= 1, "adjoint needs at least one scan");
+  require(ws_bytes >= adjoint2_workspace_bytes(plan, channels, count), "adjoint workspace too small");
+  const uint32_t per = adjoint2_scans_per_pass(channels);
   const uint64_t pixels = static_cast<uint64_t>(d.image_rows) * d.image_cols;
-  float2* last = (layout == kLayoutCoilLast) ? d_images : nullptr;
-  coil_major_to_last(b.out, last, d_rss, pixels, channels, stream);
+  for (uint32_t first = 0; first < count; first += per) {
+    const uint32_t n = std::min(per, count - first);
+    const uint32_t total = n * channels;
+    const size_t cp = padded_coils(total);
+    char* p = static_cast<char*>(d_ws);
+    auto* sorted = reinterpret_cast<float2*>(p);
+    p += align256(static_cast<size_t>(d.samples) * cp * sizeof(float2));
+    auto* grid = reinterpret_cast<float2*>(p);
+    p += align256(cp * d.grid_rows * d.grid_cols * sizeof(float2));
+    auto* rowout = reinterpret_cast<float2*>(p);
+    p += align256(static_cast<size_t>(total) * d.grid_rows * d.image_cols * sizeof(float2));
+    auto* img = reinterpret_cast<float2*>(p);
+    p += align256(static_cast<size_t>(total) * d.image_rows * d.image_cols * sizeof(float2));
+    auto* partial = reinterpret_cast<float2*>(p);
+
+    CoilPlanes planes;
+    planes.coils = total;
+    for (uint32_t i = 0; i < n; ++i) {
+      require(scans[first + i].kspace != nullptr, "null k-space");
+      for (uint32_t c = 0; c < channels; ++c)
+        planes.plane[i * channels + c] = scans[first + i].kspace + static_cast<uint64_t>(c) * d.samples;
+    }
+    {
+      ProfScope ps("grid_prep", stream);
+      grid_prep(plan, planes, sorted, stream);
+    }
+    {
+      ProfScope ps("grid_gather", stream);
+      grid_gather(plan, sorted, total, grid, partial, stream);
+    }
+
+    // Row pass: lines (grid row y, coil c) of the coil-fastest grid [G_r][G_c][Cp],
+    // loaded coil-fastest, stored as coil planes [C][G_r][N_c].
+    FftArgs a;
+    a.in = grid;
+    a.out = rowout;
+    a.src.planes = d.grid_rows;
+    a.src.lines = total;
+    a.src.plane_stride = static_cast<uint64_t>(d.grid_cols) * cp;
+    a.src.line_stride = 1;
+    a.src.elem_stride = cp;
+    a.dst.planes = d.grid_rows;
+    a.dst.lines = total;
+    a.dst.plane_stride = d.image_cols;
+    a.dst.line_stride = static_cast<uint64_t>(d.grid_rows) * d.image_cols;
+    a.dst.elem_stride = 1;
+    a.n = d.grid_cols;
+    a.n_out = d.image_cols;
+    a.out_offset = d.centred ? (d.grid_cols - d.image_cols / 2) % d.grid_cols : 0;
+    a.inverse = 1;
+    a.load_lines_fast = 1;
+    a.store_lines_fast = 0;
+    {
+      ProfScope ps("fft_rows", stream);
+      fft_launch(a, stream);
+    }
+
+    // Column pass with deapodization and crop -> coil planes [C][N_r][N_c].
+    const bool direct = (count == 1 && layout == kLayoutCoilMajor && scans[0].images != nullptr);
+    FftArgs b;
+    b.in = rowout;
+    b.out = direct ? scans[0].images : img;
+    b.src = cols_of(total, d.grid_rows, d.image_cols);
+    b.dst = cols_of(total, d.image_rows, d.image_cols);
+    b.n = d.grid_rows;
+    b.n_out = d.image_rows;
+    b.out_offset = d.centred ? (d.grid_rows - d.image_rows / 2) % d.grid_rows : 0;
+    b.out_weight = plan.deapod_rows;
+    b.line_weight = plan.deapod_cols;
+    b.inverse = 1;
+    b.load_lines_fast = b.store_lines_fast = 1;
+    {
+      ProfScope ps("fft_cols_deapod", stream);
+      fft_launch(b, stream);
+    }
+
+    ProfScope ps("coil_combine", stream);
+    CoilGroupOutputs out;
+    out.groups = n;
+    out.coils = channels;
+    bool any = false;
+    for (uint32_t i = 0; i < n; ++i) {
+      const AdjointScan& sc = scans[first + i];
+      out.coil_last[i] = layout == kLayoutCoilLast ? sc.images : nullptr;
+      out.rss[i] = sc.rss;
+      any = any || out.coil_last[i] || out.rss[i];
+      if (layout == kLayoutCoilMajor && sc.images && !direct)
+        KSJ_HIP(hipMemcpyAsync(sc.images, img + static_cast<uint64_t>(i) * channels * pixels,
+                               static_cast<size_t>(channels) * pixels * sizeof(float2), hipMemcpyDeviceToDevice,
+                               stream));
+    }
+    if (any) coil_major_split(b.out, out, pixels, stream);
+  }
 }
 
 size_t nudft2_adjoint_workspace_bytes(uint32_t channels, uint32_t rows, uint32_t cols) {

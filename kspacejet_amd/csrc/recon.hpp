@@ -15,10 +15,19 @@ size_t ifft2_workspace_bytes(uint32_t channels, uint32_t rows, uint32_t cols);
 void ifft2_coil_images(const float2* d_kspace, float2* d_images, uint32_t channels, uint32_t rows, uint32_t cols,
                        bool centred, int layout, void* d_ws, size_t ws_bytes, hipStream_t stream);
 
-size_t adjoint2_workspace_bytes(const Plan2& plan, uint32_t channels);
-// Channel-major samples [C][S] -> coil images (layout) and/or RSS image [N_r][N_c].
-void adjoint2_execute(const Plan2& plan, const float2* d_kspace, uint32_t channels, float2* d_images, int layout,
-                      float* d_rss, void* d_ws, size_t ws_bytes, hipStream_t stream);
+// One scan of an adjoint call: channel-major samples [C][S] in, coil images
+// (layout) and/or RSS image [N_r][N_c] out (null skips an output).
+struct AdjointScan {
+  const float2* kspace = nullptr;
+  float2* images = nullptr;
+  float* rss = nullptr;
+};
+
+// Scans on one plan are stacked along the coil axis, up to 64 coils per pass.
+uint32_t adjoint2_scans_per_pass(uint32_t channels);
+size_t adjoint2_workspace_bytes(const Plan2& plan, uint32_t channels, uint32_t scans = 1);
+void adjoint2_execute(const Plan2& plan, const AdjointScan* scans, uint32_t count, uint32_t channels, int layout,
+                      void* d_ws, size_t ws_bytes, hipStream_t stream);
 
 size_t nudft2_adjoint_workspace_bytes(uint32_t channels, uint32_t rows, uint32_t cols);
 

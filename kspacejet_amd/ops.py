@@ -117,7 +117,7 @@ class Plan2:
 
     def __init__(self, image_rows: int, image_cols: int, samples: int, oversampling: float = 2.0,
                  kernel: str = "kaiser_bessel", width: int = 4, density: str = "ramp", centred: bool = True,
-                 heavy_threshold: int = 0, beta: float = 0.0, row_origin: float | None = None,
+                 item_visits: int = 0, beta: float = 0.0, row_origin: float | None = None,
                  col_origin: float | None = None, grid_rows: int | None = None, grid_cols: int | None = None):
         d = _lib.Plan2Desc()
         d.struct_size = C.sizeof(_lib.Plan2Desc)
@@ -129,7 +129,7 @@ class Plan2:
         d.density = {"none": _lib.DENSITY_NONE, "ramp": _lib.DENSITY_RAMP, "given": _lib.DENSITY_GIVEN}[density]
         d.centred = int(centred)
         d.samples = samples
-        d.heavy_threshold = heavy_threshold
+        d.item_visits = item_visits
         d.beta = beta
         d.row_origin = (image_rows - 1) / 2.0 if row_origin is None else row_origin
         d.col_origin = (image_cols - 1) / 2.0 if col_origin is None else col_origin
@@ -149,7 +149,7 @@ class Plan2:
         i = _lib.Plan2Info()
         i.struct_size = C.sizeof(_lib.Plan2Info)
         call("ksj_gpu_plan2_get_info", self._handle, C.byref(i))
-        return {"heavy_cells": i.heavy_cells, "device_bytes": i.device_bytes, "beta": i.beta}
+        return {"items": i.items, "visits": i.visits, "device_bytes": i.device_bytes, "beta": i.beta}
 
     def export_bins(self, device="cuda"):
         nb = self.desc.grid_rows * self.desc.grid_cols
@@ -179,6 +179,34 @@ class Plan2:
         call("ksj_gpu_adjoint2_execute", self._handle, _ptr(kspace), ch, 0 if img is None else _ptr(img),
              LAYOUTS[layout], 0 if out_rss is None else _ptr(out_rss), _ptr(workspace), workspace.numel(), _stream())
         return img, out_rss
+
+    def batch_workspace_bytes(self, channels: int, scans: int) -> int:
+        return int(self._lib.ksj_gpu_adjoint2_batch_workspace_bytes(self._handle, channels, scans))
+
+    def adjoint_batch(self, kspaces, images: bool = False, rss: bool = True, layout: str = "coil_last",
+                      workspace: torch.Tensor | None = None):
+        """Scans sharing this plan's trajectory, stacked along the coil axis (up to 64 coils per pass)."""
+        n = len(kspaces)
+        ch = kspaces[0].shape[0]
+        for k in kspaces:
+            _need(k, torch.complex64, "kspace")
+            if tuple(k.shape) != (ch, self.desc.samples):
+                raise ValueError("every scan needs the same [C, S] shape")
+        r, c = self.desc.image_rows, self.desc.image_cols
+        dev = kspaces[0].device
+        imgs = [torch.empty((r, c, ch) if layout == "coil_last" else (ch, r, c), dtype=torch.complex64, device=dev)
+                for _ in range(n)] if images else None
+        rsss = [torch.empty((r, c), dtype=torch.float32, device=dev) for _ in range(n)] if rss else None
+        need = self.batch_workspace_bytes(ch, n)
+        if workspace is None or workspace.numel() < need:
+            workspace = _workspace(need, dev)
+        P = C.c_void_p * n
+        kp = P(*[_ptr(k) for k in kspaces])
+        ip = P(*[_ptr(t) for t in imgs]) if imgs else None
+        rp = P(*[_ptr(t) for t in rsss]) if rsss else None
+        call("ksj_gpu_adjoint2_execute_batch", self._handle, n, kp, ch, ip, LAYOUTS[layout], rp, _ptr(workspace),
+             workspace.numel(), _stream())
+        return imgs, rsss
 
     def close(self) -> None:
         if getattr(self, "_handle", None) and self._handle.value:

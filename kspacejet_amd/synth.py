@@ -104,7 +104,7 @@ def add_noise(k: np.ndarray, seed: int, rel: float = 1e-3) -> np.ndarray:
     return k + s * (g[0] + 1j * g[1])
 
 
-def cartesian_kspace2(n: int, coils: int, seed: int, centred: bool = True) -> np.ndarray:
+def cartesian_kspace2(n: int, coils: int, seed: int, centred: bool = True, noise: bool = True) -> np.ndarray:
     """[coils, n, n] complex64 k-space: FP64 FFT of phantom x coils (+ noise).  centred
     k-space is fftshift(fft2(ifftshift(x))), so the centred inverse recovers x."""
     x = coil_images2(n, coils, seed)
@@ -112,7 +112,7 @@ def cartesian_kspace2(n: int, coils: int, seed: int, centred: bool = True) -> np
         k = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(x, axes=(-2, -1))), axes=(-2, -1))
     else:
         k = np.fft.fft2(x)
-    return add_noise(k, seed).astype(np.complex64)
+    return (add_noise(k, seed) if noise else k).astype(np.complex64)
 
 
 def golden_angle2(spokes: int, readout: int) -> np.ndarray:

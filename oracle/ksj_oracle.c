@@ -193,8 +193,10 @@ static int reference_grid_coordinate(double w, size_t extent, size_t* lower, dou
 
 int ora_radial_linear_gridding2(size_t rows, size_t cols, double row_origin, double col_origin, size_t samples,
                                 const double* values, const double* traj, const double* dcf, double* image) {
+  /* The payload is float32, checked against float pi (radial_gridding_reconstruct.cpp:217-226). */
+  const double pi_f = (double)(float)ORA_PI;
   for (size_t s = 0; s < samples; ++s)
-    if (!(traj[2 * s] >= -ORA_PI && traj[2 * s] <= ORA_PI && traj[2 * s + 1] >= -ORA_PI && traj[2 * s + 1] <= ORA_PI))
+    if (!(traj[2 * s] >= -pi_f && traj[2 * s] <= pi_f && traj[2 * s + 1] >= -pi_f && traj[2 * s + 1] <= pi_f))
       return -1;
   memset(image, 0, 2 * rows * cols * sizeof(double));
   for (size_t s = 0; s < samples; ++s) {

@@ -79,6 +79,7 @@ def ref():
         R.ref_direct_adjoint2_f64.argtypes = [_sz, _sz, C.c_double, C.c_double, _sz, _dp, _dp, _dp]
         R.ref_direct_forward2_f64.argtypes = [_sz, _sz, C.c_double, C.c_double, _sz, _dp, _dp, _dp]
         R.ref_radial_route_rss_batch.argtypes = [_sz, _sz, _sz, _sz, _sz, _fp, _fp, _fp, C.c_uint]
+        R.ref_radial_route_rss_repeat.argtypes = [_sz, _sz, _sz, _sz, _sz, _fp, _fp, _fp, C.c_uint]
         _ref = R
     return _ref
 
@@ -246,4 +247,20 @@ def kb_adjoint2_coils(kspace, traj, dcf, n, oversampling=2.0, width=4):
 def golden_angle2(spokes: int, readout: int) -> np.ndarray:
     out = np.zeros((spokes * readout, 2), np.float64)
     lib().ora_golden_angle2(spokes, readout, out.reshape(-1))
+    return out
+
+
+def ref_radial_route_rss(values_f32, traj_f32, rows: int, cols: int, scans: int = 1, threads: int = 1) -> np.ndarray:
+    """The reference radial route (ramp DCF + bilinear gridding + radix-2 iFFT per channel, then RSS)
+    run `scans` times over one scan's data on `threads` host threads (compiled reference sources)."""
+    R = ref()
+    if R is None:
+        raise RuntimeError("oracle/_ref was not built")
+    v = np.ascontiguousarray(values_f32, np.complex64)
+    t = np.ascontiguousarray(traj_f32, np.float32).reshape(-1, 2)
+    out = np.zeros((rows, cols), np.float32)
+    rc = R.ref_radial_route_rss_repeat(scans, rows, cols, v.shape[0], v.shape[1], v.view(np.float32).reshape(-1),
+                                       t.reshape(-1), out.reshape(-1), threads)
+    if rc != 0:
+        raise ValueError("reference route rejected its input")
     return out

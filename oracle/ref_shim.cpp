@@ -150,4 +150,27 @@ int ref_radial_route_rss_batch(std::size_t scans, std::size_t rows, std::size_t 
   return 0;
 }
 
+// Timing form of the route above: `scans` repetitions of one scan's data, each
+// thread writing its own RSS image (the last one is copied to `rss`).
+int ref_radial_route_rss_repeat(std::size_t scans, std::size_t rows, std::size_t cols, std::size_t channels,
+                                std::size_t samples, const float* values, const float* traj, float* rss,
+                                unsigned threads) {
+  threads = std::max(1U, threads);
+  std::vector<std::vector<float>> out(threads, std::vector<float>(rows * cols));
+  std::vector<int> status(threads, 0);
+  auto work = [&](unsigned t) {
+    for (std::size_t scan = t; scan < scans; scan += threads)
+      if (ref_radial_route_rss_batch(1U, rows, cols, channels, samples, values, traj, out[t].data(), 1U) != 0)
+        status[t] = -1;
+  };
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < threads; ++t) pool.emplace_back(work, t);
+  work(0U);
+  for (auto& th : pool) th.join();
+  std::copy(out[0].begin(), out[0].end(), rss);
+  for (int s : status)
+    if (s != 0) return -1;
+  return 0;
+}
+
 }  // extern "C"

@@ -113,12 +113,12 @@ def test_kb_adjoint_small_matches_oracle(ops, width):
     assert rel_l2(rss.cpu().numpy(), O.rss(ref)) <= TOL
 
 
-@pytest.mark.parametrize("threshold", [1, 16, 4096])
-def test_kb_heavy_cell_split_is_equivalent(ops, threshold):
-    # Every cell heavy, some heavy, none heavy: all must match the oracle.
+@pytest.mark.parametrize("item_visits", [1, 7, 64, 4096])
+def test_kb_work_item_split_is_equivalent(ops, item_visits):
+    # Every block split into single-visit items, odd items, the default, no split: all match the oracle.
     n, coils = 48, 2
     traj, k = radial_scan(ops, n, coils, 55, 96, seed=3)
-    plan = ops.Plan2(n, n, traj.shape[0], heavy_threshold=threshold).set_trajectory(traj)
+    plan = ops.Plan2(n, n, traj.shape[0], item_visits=item_visits).set_trajectory(traj)
     img, _ = plan.adjoint(dev(k))
     ref = oracle_kb(k, traj.cpu().numpy(), n)
     assert rel_l2(img.cpu().numpy(), ref) <= TOL
@@ -151,7 +151,7 @@ def test_kb_adjoint_config2_full_size(ops):
 def test_kb_adjoint_is_deterministic(ops):
     n, coils = 64, 4
     traj, k = radial_scan(ops, n, coils, 51, 128, seed=9)
-    plan = ops.Plan2(n, n, traj.shape[0], heavy_threshold=8).set_trajectory(traj)
+    plan = ops.Plan2(n, n, traj.shape[0], item_visits=8).set_trajectory(traj)
     a, ra = plan.adjoint(dev(k), rss=True)
     b, rb = plan.adjoint(dev(k), rss=True)
     assert torch.equal(a, b) and torch.equal(ra, rb)
@@ -222,3 +222,22 @@ def test_direct_forward_f64_matches_oracle(ops):
     tt = np.array([[0, 0], [np.pi, 0]], np.float32)
     out = ops.nudft2_forward_f64(dev(tt), dev(img), 0.0, 0.0).cpu().numpy()[0]
     np.testing.assert_allclose(out, [10, -4], atol=1e-6)
+
+
+@pytest.mark.parametrize("coils,count", [(12, 5), (16, 9), (8, 3)])
+def test_batched_scans_equal_single_scans_bitwise(ops, coils, count):
+    # Stacking scans of one trajectory along the coil axis changes no per-coil
+    # arithmetic, so each scan's result is identical alone and in the batch.
+    n = 64
+    traj = ops.golden_angle2(41, 2 * n)
+    plan = ops.Plan2(n, n, traj.shape[0]).set_trajectory(traj)
+    scans = []
+    for i in range(count):
+        img = dev(synth.coil_images2(n, coils, 100 + i))
+        scans.append(ops.nudft2_forward_f64(traj, img, (n - 1) / 2, (n - 1) / 2).to(torch.complex64))
+    imgs, rsss = plan.adjoint_batch(scans, images=True, rss=True)
+    for i, k in enumerate(scans):
+        one_img, one_rss = plan.adjoint(k, images=True, rss=True)
+        assert torch.equal(imgs[i], one_img) and torch.equal(rsss[i], one_rss)
+    ref = oracle_kb(scans[-1].cpu().numpy(), traj.cpu().numpy(), n)
+    assert rel_l2(imgs[-1].cpu().numpy(), ref) <= TOL
