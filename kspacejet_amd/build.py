@@ -4,8 +4,9 @@
     python -m kspacejet_amd.build --changed  # only sources newer than their objects
 
 Outputs (git-ignored, shipped with the tree):
-    kspacejet_amd/libksj_gpu.so           HIP kernels + the C ABI of include/ksj_gpu.h
-    kspacejet_amd/libksj_provider_gpu.so  KSpaceJet Provider module (ksj_provider_query)
+    kspacejet_amd/libksj_gpu.so                    HIP kernels + the C ABI of include/ksj_gpu.h
+    kspacejet_amd/libksj_provider_<module>.so      one KSpaceJet Provider module per
+                                                   csrc/provider/mod_<module>.cpp (ksj_provider_query)
 """
 from __future__ import annotations
 
@@ -29,11 +30,13 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{os.path.join(ROOT, 'include')}",
 ]
-CXX_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-Wall", "-D__HIP_PLATFORM_AMD__", f"-I{os.path.join(ROOT, 'include')}",
+CXX_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-Wall", "-fvisibility=hidden", "-D__HIP_PLATFORM_AMD__", f"-I{os.path.join(ROOT, 'include')}",
              "-I/opt/rocm/include"]
 
 GPU_LIB = os.path.join(PKG, "libksj_gpu.so")
-PROVIDER_LIB = os.path.join(PKG, "libksj_provider_gpu.so")
+
+def provider_lib(module: str) -> str:
+    return os.path.join(PKG, f"libksj_provider_{module}.so")
 
 
 def _run(cmd):
@@ -57,7 +60,7 @@ def build(changed_only: bool = False, jobs: int = 8) -> None:
     hip_srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     host_srcs = sorted(glob.glob(os.path.join(CSRC, "*.cpp")))
     prov_srcs = sorted(glob.glob(os.path.join(CSRC, "provider", "*.cpp")))
-    prov_headers = glob.glob(os.path.join(CSRC, "provider", "*.h*"))
+    prov_headers = glob.glob(os.path.join(CSRC, "provider", "*.h*")) + glob.glob(os.path.join(CSRC, "provider", "*.inc"))
     tasks = []
     for src in hip_srcs:
         obj = os.path.join(OBJ, os.path.basename(src) + ".o")
@@ -78,11 +81,20 @@ def build(changed_only: bool = False, jobs: int = 8) -> None:
     gpu_objs = [os.path.join(OBJ, os.path.basename(s) + ".o") for s in hip_srcs + host_srcs]
     if not changed_only or _needs(GPU_LIB, gpu_objs):
         _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", GPU_LIB, *gpu_objs, "-lpthread"])
-    prov_objs = [os.path.join(OBJ, "provider_" + os.path.basename(s) + ".o") for s in prov_srcs]
-    if prov_objs and (not changed_only or _needs(PROVIDER_LIB, prov_objs + [GPU_LIB])):
-        _run(["g++", "-shared", "-fPIC", "-o", PROVIDER_LIB, *prov_objs, f"-L{PKG}", "-lksj_gpu",
-              "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib",
-              "-Wl,--no-undefined"])
+    # Every module links the shared provider core plus its own mod_<module>.cpp,
+    # exports only ksj_provider_query, and finds libksj_gpu.so beside itself.
+    core = [s for s in prov_srcs if not os.path.basename(s).startswith("mod_")]
+    core_objs = [os.path.join(OBJ, "provider_" + os.path.basename(s) + ".o") for s in core]
+    for src in prov_srcs:
+        name = os.path.basename(src)
+        if not name.startswith("mod_"):
+            continue
+        lib = provider_lib(name[len("mod_"):-len(".cpp")])
+        objs = core_objs + [os.path.join(OBJ, "provider_" + name + ".o")]
+        if not changed_only or _needs(lib, objs + [GPU_LIB]):
+            _run(["g++", "-shared", "-fPIC", "-o", lib, *objs, f"-L{PKG}", "-lksj_gpu", "-Wl,-rpath,$ORIGIN",
+                  "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", "-Wl,--no-undefined",
+                  f"-Wl,--version-script={os.path.join(CSRC, 'provider', 'exports.map')}"])
 
 
 def main(argv=None) -> int:
@@ -91,7 +103,8 @@ def main(argv=None) -> int:
     ap.add_argument("-j", "--jobs", type=int, default=8)
     args = ap.parse_args(argv)
     build(changed_only=args.changed, jobs=args.jobs)
-    print(f"built {GPU_LIB}" + (f" and {PROVIDER_LIB}" if os.path.exists(PROVIDER_LIB) else ""))
+    mods = sorted(glob.glob(os.path.join(PKG, "libksj_provider_*.so")))
+    print(f"built {GPU_LIB}" + "".join(f"\n      {m}" for m in mods))
     return 0
 
 

@@ -1,0 +1,21 @@
+// Operator specifications of the provider modules.
+#pragma once
+
+#include "operator.hpp"
+
+namespace ksj::prov {
+
+// org.kspacejet.cartesian-recon
+extern const OpSpec kCartesianIfft2CoilImages;  // cartesian_ifft2_coil_images
+// org.kspacejet.coil-combine
+extern const OpSpec kCoilCombineRss;            // coil_combine_rss
+// org.kspacejet.noncartesian-recon
+extern const OpSpec kNoncartesianAdjoint;       // noncartesian_adjoint_reconstruct
+extern const OpSpec kRadialGridding;            // radial_gridding_reconstruct
+// org.kspacejet.gpu-recon (no reference implementation)
+extern const OpSpec kKbGridding;                // kb_gridding_reconstruct
+extern const OpSpec kKbGriddingRss;             // kb_gridding_reconstruct_rss
+extern const OpSpec kCartesianIfft2Rss;         // cartesian_ifft2_rss
+extern const OpSpec kCoilCombineSensitivity;    // coil_combine_sensitivity
+
+}  // namespace ksj::prov
