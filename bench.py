@@ -66,7 +66,27 @@ def algorithmic_bytes(stage: str) -> float:
         return 8.0 * G * N * COILS + 8.0 * N * N * COILS
     if stage == "coil_combine":
         return 8.0 * N * N * COILS + 4.0 * N * N
+    if stage == "grid_reduce":
+        return 0.0  # partial sums of blocks cut into several items: no compulsory bytes
     raise KeyError(stage)
+
+
+TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "radial256_c16_kb4_hbm_traffic.txt")
+
+
+def measured_traffic(scans_per_launch: int):
+    """HBM bytes per launch of the batched gather kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes of this workload (already scaled by the copy-kernel
+    calibration), or None when that summary is absent or was taken at another stacking."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch"):
+                _, key, value = line.split()
+                if key == f"gather_item_kernel<64,8>@{scans_per_launch}scans":
+                    return float(value)
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(scan_np, traj_np, seconds_hint: float):
@@ -99,6 +119,8 @@ def main(argv=None) -> int:
     ap.add_argument("--distinct", type=int, default=4, help="distinct seeded scans behind the batch")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="launch parameter for ksj_gpu_tune_set (repeatable), e.g. gather_map=0")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device-to-device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args(argv)
@@ -113,6 +135,10 @@ def main(argv=None) -> int:
 
     ops.init(0)
     lib = load()
+    for kv in args.tune:
+        key, value = kv.split("=", 1)
+        if lib.ksj_gpu_tune_set(key.encode(), int(value)) != 0:
+            raise SystemExit(f"--tune {kv}: {lib.ksj_gpu_last_error().decode()}")
     traj = ops.golden_angle2(SPOKES, READOUT)
     S = SPOKES * READOUT
     log("generating synthetic scans ...")
@@ -154,7 +180,7 @@ def main(argv=None) -> int:
         lib.ksj_gpu_profile_reset()
         timed(fn, steps)
         out = {}
-        for name in ("grid_prep", "grid_gather", "fft_rows", "fft_cols_deapod", "coil_combine"):
+        for name in ("grid_prep", "grid_gather", "grid_reduce", "fft_rows", "fft_cols_deapod", "coil_combine"):
             tot, cnt = C.c_double(0), C.c_uint64(0)
             lib.ksj_gpu_profile_read(name.encode(), C.byref(tot), C.byref(cnt))
             out[name] = {"avg_us": 1e3 * tot.value / max(cnt.value, 1), "launches": cnt.value}
@@ -177,17 +203,17 @@ def main(argv=None) -> int:
     stages_plain = stage_times(run_plain, max(1, args.steps // 2))
     stages = stage_times(run_batched, max(1, args.steps // 2))
     scans_per_launch = min(per_pass, args.batch)
-    dominant = max(stages, key=lambda s: stages[s]["avg_us"])
+    dominant = max(stages, key=lambda s: stages[s]["avg_us"] * stages[s]["launches"])
     g = stages["grid_gather"]
     per_launch_bytes = algorithmic_bytes("grid_gather") * scans_per_launch
     achieved = per_launch_bytes / (g["avg_us"] * 1e-6) / 1e9
     roofline = {"kernel": "grid_gather", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(scans_per_launch),
                 "scans_per_launch": scans_per_launch, "algorithmic_bytes_per_launch": per_launch_bytes,
                 "avg_us": round(g["avg_us"], 2)}
     stage_report = {k: {"avg_us": round(v["avg_us"], 2),
-                        "gbs": round(algorithmic_bytes(k) * scans_per_launch / (v["avg_us"] * 1e-6) / 1e9, 1)}
-                    for k, v in stages.items()}
+                        "gbs": round(algorithmic_bytes(k) * scans_per_launch / max(v["avg_us"] * 1e-6, 1e-12) / 1e9, 1)}
+                    for k, v in stages.items() if v["launches"]}
     plain_report = {k: round(v["avg_us"], 2) for k, v in stages_plain.items()}
     cpu = None
     if not args.no_cpu_baseline:

@@ -177,10 +177,23 @@ int ksj_gpu_adjoint2_execute_batch(const ksj_gpu_plan2* plan, uint32_t count, co
                                    uint32_t channels, void* const* d_images, int layout, float* const* d_rss,
                                    void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------ tuning */
+
+/* Process-wide launch parameters chosen by the autotuner (kspacejet_amd/autotune.py).
+ * No reference counterpart.  Keys and legal values:
+ *   "gather_map"           0 = work items in launch order, 1 = one contiguous item range per XCD
+ *   "item_visits"          1..4096, default gather work-item size of plans created afterwards
+ *   "fft_lines_per_block"  0 = per-length default, else 1..64 (power of two) lines per FFT block
+ *   "gather_batch"         4, 8 or 16 gather visits per batch (sample-row loads in flight per lane)
+ * Every setting gives bit-identical results, except item_visits, which sets the
+ * float summation order of the gather.  Not thread-safe against concurrent launches. */
+int ksj_gpu_tune_set(const char* key, int64_t value);
+int ksj_gpu_tune_get(const char* key, int64_t* value);
+
 /* ------------------------------------------------------------------ stage timing */
 
 /* Per-stage timing with HIP events recorded on the stream each stage runs on
- * (stages: grid_prep, grid_gather, fft_rows, fft_cols_deapod, coil_combine, ...).
+ * (stages: grid_prep, grid_gather, grid_reduce, fft_rows, fft_cols_deapod, coil_combine).
  * Off by default; no reference counterpart (the reference's kspacejet-performance
  * spans are compiled out unless KSJ_PERFORMANCE_LEVEL is set).  profile_read
  * synchronises the pending events of that stage. */

@@ -76,6 +76,8 @@ SIGNATURES = {
     "ksj_gpu_adjoint2_batch_workspace_bytes": (_sz, [_vp, _u32, _u32]),
     "ksj_gpu_adjoint2_execute_batch": (_int, [_vp, _u32, C.POINTER(_vp), _u32, C.POINTER(_vp), _int,
                                                C.POINTER(_vp), _vp, _sz, _vp]),
+    "ksj_gpu_tune_set": (_int, [C.c_char_p, C.c_int64]),
+    "ksj_gpu_tune_get": (_int, [C.c_char_p, C.POINTER(C.c_int64)]),
     "ksj_gpu_profile_enable": (_int, [_int]),
     "ksj_gpu_profile_reset": (_int, []),
     "ksj_gpu_profile_read": (_int, [C.c_char_p, C.POINTER(_dbl), C.POINTER(_u64)]),

@@ -161,7 +161,8 @@ int ksj_gpu_plan2_create(const ksj_gpu_plan2_desc* desc, ksj_gpu_plan2** out_pla
     d.density = desc->density;
     d.centred = desc->centred;
     d.samples = desc->samples;
-    d.item_visits = desc->item_visits == 0 ? 64u : desc->item_visits;
+    d.item_visits = desc->item_visits == 0 ? static_cast<uint32_t>(ksj::gpu::tune_get(ksj::gpu::kTuneItemVisits))
+                                           : desc->item_visits;
     ksj::gpu::require(d.density <= KSJ_GPU_DENSITY_GIVEN, "unknown density mode");
     auto* handle = new ksj_gpu_plan2{nullptr};
     try {
@@ -250,6 +251,22 @@ int ksj_gpu_adjoint2_execute_batch(const ksj_gpu_plan2* plan, uint32_t count, co
     }
     ksj::gpu::adjoint2_execute(*plan->impl, scans.data(), count, channels, layout, d_workspace, workspace_bytes,
                                as_stream(stream));
+  });
+}
+
+int ksj_gpu_tune_set(const char* key, int64_t value) {
+  return guarded([&] {
+    const int k = ksj::gpu::tune_key(key);
+    ksj::gpu::require(k >= 0, "unknown tuning key");
+    ksj::gpu::tune_set(static_cast<ksj::gpu::TuneKey>(k), value);
+  });
+}
+
+int ksj_gpu_tune_get(const char* key, int64_t* value) {
+  return guarded([&] {
+    const int k = ksj::gpu::tune_key(key);
+    ksj::gpu::require(k >= 0 && value != nullptr, "unknown tuning key or null output");
+    *value = ksj::gpu::tune_get(static_cast<ksj::gpu::TuneKey>(k));
   });
 }
 

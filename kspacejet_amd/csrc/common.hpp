@@ -43,4 +43,18 @@ inline void require(bool ok, const char* what) {
 
 inline unsigned ceil_div(uint64_t a, uint64_t b) { return static_cast<unsigned>((a + b - 1) / b); }
 
+// Process-wide launch parameters the autotuner may change (tune.hip).  Each
+// value has a fixed set of legal settings; every setting gives bit-identical
+// results except where a parameter's comment says otherwise.
+enum TuneKey : int {
+  kTuneGatherMap = 0,     // 0: work items in launch order; 1: one contiguous item range per XCD
+  kTuneItemVisits,        // default gather work-item size of new plans (visits; summation order changes)
+  kTuneFftLinesPerBlock,  // 0: per-length default; else lines per LDS FFT block (power of two)
+  kTuneGatherBatch,       // gather visits per batch (sample-row loads in flight per lane): 4, 8 or 16
+  kTuneKeys
+};
+int64_t tune_get(TuneKey key);
+void tune_set(TuneKey key, int64_t value);  // throws Error on an illegal value
+int tune_key(const char* name);             // -1 for an unknown name
+
 }  // namespace ksj::gpu

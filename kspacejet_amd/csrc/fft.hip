@@ -316,6 +316,7 @@ int max_dynamic_lds() {
 template <int N>
 void launch_fast(const FftArgs& a, const float2* tw, hipStream_t stream, int lpb) {
   constexpr int T = Shape<N>::T;
+  if (lpb <= 0) lpb = static_cast<int>(tune_get(kTuneFftLinesPerBlock));
   if (lpb <= 0) lpb = (a.load_lines_fast || a.store_lines_fast) ? 16 : (T >= 64 ? 4 : 256 / T);
   while (lpb * T > 1024) lpb /= 2;
   // LDS: twiddles + lpb padded lines (plus 1 KiB of static base offsets).

@@ -11,43 +11,67 @@
 //         coil-fastest grid [G_r][G_c][Cp].
 #include "grid.hpp"
 
+#include <utility>
+
 namespace ksj::gpu {
 namespace {
 
 // ----------------------------------------------------------------------------- prep
-constexpr int kPrepSamples = 64;
+constexpr int kPrepSamples = 64;  // one sample per lane of a wave
+constexpr int kPrepWaves = 4;
 
 struct PrepArgs {
   CoilPlanes planes;     // coil c -> [S] samples
   const float2* weight;  // [S]
   const uint32_t* rank;  // [S]
   float2* sorted;        // [S][Cp]
-  uint32_t samples, coils, cp;
+  uint32_t samples, coils;
 };
 
-__global__ __launch_bounds__(256) void prep_kernel(PrepArgs a) {
-  extern __shared__ float2 tile[];  // [kPrepSamples][cp + 1]; the pad spreads a coil column over the banks
-  const uint32_t ts = a.cp + 1;
+// Block = 64 consecutive samples; wave w reads coils w, w+4, ... (one coalesced
+// 512-byte read per coil plane, all issued before the first wait), weights them
+// and transposes through LDS; then every sorted row of CP complex values is
+// written as CP/2 16-byte pieces at the sample's rank.
+template <int CP>
+__global__ __launch_bounds__(kPrepSamples * kPrepWaves) void prep_kernel(PrepArgs a) {
+  constexpr int kTs = CP + 1;               // padded row: a coil column spreads over the banks
+  constexpr int kPerWave = CP / kPrepWaves;  // coils read by one wave
+  __shared__ float2 tile[kPrepSamples * kTs];
+  __shared__ uint32_t rank_s[kPrepSamples];
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t s0 = blockIdx.x * kPrepSamples;
-  const uint32_t ns = a.samples - s0 < static_cast<uint32_t>(kPrepSamples) ? a.samples - s0 : kPrepSamples;
-  for (uint32_t i = threadIdx.x; i < kPrepSamples * a.cp; i += blockDim.x) {
-    const uint32_t c = i / kPrepSamples, sl = i % kPrepSamples;
-    float2 v = make_float2(0.f, 0.f);
-    if (c < a.coils && sl < ns) v = cmul(a.planes.plane[c][s0 + sl], a.weight[s0 + sl]);
-    tile[sl * ts + c] = v;
+  const uint32_t s = s0 + lane;
+  const bool live = s < a.samples;
+  const float2 w = live ? a.weight[s] : make_float2(0.f, 0.f);
+  const uint32_t r = live ? a.rank[s] : 0u;
+  float2 v[kPerWave];
+#pragma unroll
+  for (int j = 0; j < kPerWave; ++j) {
+    const uint32_t c = wave + kPrepWaves * j;  // wave-uniform: the plane pointer is a scalar load
+    v[j] = (c < a.coils && live) ? a.planes.plane[c][s] : make_float2(0.f, 0.f);
   }
+  if (wave == 0) rank_s[lane] = r;  // after the sample loads are issued
+#pragma unroll
+  for (int j = 0; j < kPerWave; ++j) tile[lane * kTs + wave + kPrepWaves * j] = cmul(v[j], w);
   __syncthreads();
-  // Each row of cp complex values is written by cp/2 consecutive lanes (16 B each).
-  const uint32_t pieces = a.cp / 2;
-  for (uint32_t i = threadIdx.x; i < ns * pieces; i += blockDim.x) {
-    const uint32_t sl = i / pieces, pc = i % pieces;
-    const float2 lo = tile[sl * ts + 2 * pc], hi = tile[sl * ts + 2 * pc + 1];
-    float4* dst = reinterpret_cast<float4*>(a.sorted + static_cast<uint64_t>(a.rank[s0 + sl]) * a.cp);
-    dst[pc] = make_float4(lo.x, lo.y, hi.x, hi.y);
+  const uint32_t ns = a.samples - s0 < static_cast<uint32_t>(kPrepSamples) ? a.samples - s0 : kPrepSamples;
+  constexpr int kPieces = CP / 2;  // float4 pieces per sorted row
+  constexpr int kIters = kPrepSamples * kPieces / (kPrepSamples * kPrepWaves);
+#pragma unroll
+  for (int it = 0; it < kIters; ++it) {
+    const uint32_t i = threadIdx.x + it * kPrepSamples * kPrepWaves;
+    const uint32_t sl = i / kPieces, pc = i % kPieces;
+    if (sl < ns) {
+      const float2 lo = tile[sl * kTs + 2 * pc], hi = tile[sl * kTs + 2 * pc + 1];
+      float4* dst = reinterpret_cast<float4*>(a.sorted + static_cast<uint64_t>(rank_s[sl]) * CP);
+      dst[pc] = make_float4(lo.x, lo.y, hi.x, hi.y);
+    }
   }
 }
 
 // ----------------------------------------------------------------------------- gather
+constexpr uint32_t kXcds = 8;  // MI355X: workgroups are dispatched round-robin over 8 XCDs
 struct GatherArgs {
   const float2* sorted;             // [S][Cp]
   const uint32_t* visit_k;          // [visits]
@@ -58,6 +82,7 @@ struct GatherArgs {
   const uint32_t* multi_blocks;     // [multi_count]
   float2* partial;                  // [slots][4][Cp]
   float2* grid;                     // [G_r][G_c][Cp]
+  uint32_t xcd_chunk, wg_per_xcd;   // workgroup -> item range mapping
   uint32_t grid_rows, grid_cols, block_cols, cp, item_count, multi_count;
 };
 
@@ -73,85 +98,93 @@ __device__ inline void fma4(float2 (&acc)[4], float4 w, float2 d) {
 }
 
 // One sub-wave of SW lanes (lane = coil) per work item, 64/SW items per wave.  The
-// sub-wave walks its item's visits in batches of B: lane j < B loads the sample
-// position of visit j (one contiguous load), the positions are shared with
-// cross-lane shuffles, then all B sample-row loads (SW consecutive coils each) are
-// issued before the first use, so B loads are in flight per lane.  Weights are
-// broadcast loads independent of the positions.  Sums run in visit order.
-// One batch of B visits; `kj` holds (on lanes c < B) the sample positions of the
-// batch, loaded one batch ahead by the caller.
+// sub-wave walks its item's visits in batches of B and issues all B sample-row
+// loads (SW consecutive coils each) before the first use.  The next batch's
+// positions are loaded right after this batch's are distributed, so their latency
+// overlaps this batch's row loads.  Sums run in visit order.
+//
+// Per-visit metadata is loaded once per 16-lane row and distributed inside the row:
+// lane r of every row loads position r and weight floats 16j + r of the batch, and
+// DPP row_newbcast (broadcast of lane K of each 16-lane row) hands them to every lane.
+// A broadcast float4 load per visit would move 1 KiB through the texture units for
+// 16 bytes of data -- twice the bytes of the sample row itself.  Sub-waves of 8 lanes
+// (two per row) fall back to cross-lane shuffles.
+template <int B>
+__device__ inline uint32_t load_positions(const GatherArgs& a, uint32_t v, uint32_t end, int r) {
+  return (r < B && v + r < end) ? a.visit_k[v + r] : 0u;
+}
+
+template <int K>
+__device__ inline uint32_t row_bcast(uint32_t x) {
+  return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(x), 0x150 + K, 0xF, 0xF, false));
+}
+
+// Weight component F (= 4 * visit + cell) of a batch, held by lane F % 16 in load F / 16.
+template <int F, int NL>
+__device__ inline float weight_comp(const uint32_t (&wl)[NL]) {
+  return __uint_as_float(row_bcast<F % 16>(wl[F / 16]));
+}
+
 template <int SW, int B, bool TAIL>
-__device__ inline void gather_batch(const GatherArgs& a, float2 (&acc)[4], uint32_t kj, uint32_t v, uint32_t n,
+__device__ inline void gather_batch(const GatherArgs& a, float2 (&acc)[4], uint32_t& kj, uint32_t v, uint32_t end,
                                     int sw, int c) {
   const float2* rows = a.sorted + c;
+  const uint32_t n = end - v;
   uint32_t k[B];
   float4 w[B];
+  if constexpr (SW >= 16) {
+    constexpr int NL = (4 * B + 15) / 16;  // weight loads of 16 floats per batch
+    const int r = c & 15;
+    const float* wf = reinterpret_cast<const float*>(a.visit_w);
+    uint32_t wl[NL];
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+      const uint64_t idx = 4ull * v + 16u * j + r;
+      wl[j] = (!TAIL || idx < 4ull * end) ? __float_as_uint(wf[idx]) : 0u;
+    }
+    [&]<int... I>(std::integer_sequence<int, I...>) {
+      ((k[I] = row_bcast<I>(kj)), ...);
+    }(std::make_integer_sequence<int, B>{});
+    if (!TAIL) kj = load_positions<B>(a, v + B, end, r);
+    [&]<int... I>(std::integer_sequence<int, I...>) {
+      ((w[I] = make_float4(weight_comp<4 * I>(wl), weight_comp<4 * I + 1>(wl), weight_comp<4 * I + 2>(wl),
+                           weight_comp<4 * I + 3>(wl))),
+       ...);
+    }(std::make_integer_sequence<int, B>{});
+  } else {
+#pragma unroll
+    for (int i = 0; i < B; ++i) k[i] = __shfl(kj, sw * SW + i, 64);
+    if (!TAIL) kj = load_positions<B>(a, v + B, end, c);
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+      w[i] = (!TAIL || static_cast<uint32_t>(i) < n) ? a.visit_w[v + i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
   float2 d[B];
 #pragma unroll
-  for (int i = 0; i < B; ++i) {
-    k[i] = __shfl(kj, sw * SW + i, 64);
-    w[i] = (!TAIL || static_cast<uint32_t>(i) < n) ? a.visit_w[v + i] : make_float4(0.f, 0.f, 0.f, 0.f);
-  }
-#pragma unroll
-  for (int i = 0; i < B; ++i) d[i] = (!TAIL || static_cast<uint32_t>(i) < n) ? rows[k[i] * a.cp] : make_float2(0.f, 0.f);
+  for (int i = 0; i < B; ++i)
+    d[i] = (!TAIL || static_cast<uint32_t>(i) < n) ? rows[k[i] * a.cp] : make_float2(0.f, 0.f);
 #pragma unroll
   for (int i = 0; i < B; ++i) fma4(acc, w[i], d[i]);
 }
 
-template <int B>
-__device__ inline uint32_t load_positions(const GatherArgs& a, uint32_t v, uint32_t end, int c) {
-  return (c < B && v + c < end) ? a.visit_k[v + c] : 0u;
-}
-
-// One sub-wave of SW lanes (lane = coil) per work item, 64/SW items per wave.  The
-// sub-wave walks its item's visits in batches of B: lane j < B loads the sample
-// position of visit j (one contiguous load), the positions are shared with
-// cross-lane shuffles, then all B sample-row loads (SW consecutive coils each) are
-// issued before the first use, so B loads are in flight per lane.  Weights are
-// broadcast loads independent of the positions.  Sums run in visit order.
-template <int SW, int B, bool TAIL>
-__device__ inline void gather_batch(const GatherArgs& a, float2 (&acc)[4], uint32_t v, uint32_t n, int sw, int c) {
-  const float2* rows = a.sorted + c;
-  const uint32_t kj = (c < B && (!TAIL || static_cast<uint32_t>(c) < n)) ? a.visit_k[v + c] : 0u;
-  uint32_t k[B];
-  float4 w[B];
-  float2 d[B];
-#pragma unroll
-  for (int i = 0; i < B; ++i) {
-    k[i] = __shfl(kj, sw * SW + i, 64);
-    w[i] = (!TAIL || static_cast<uint32_t>(i) < n) ? a.visit_w[v + i] : make_float4(0.f, 0.f, 0.f, 0.f);
-  }
-#pragma unroll
-  for (int i = 0; i < B; ++i) d[i] = (!TAIL || static_cast<uint32_t>(i) < n) ? rows[k[i] * a.cp] : make_float2(0.f, 0.f);
-#pragma unroll
-  for (int i = 0; i < B; ++i) fma4(acc, w[i], d[i]);
-}
-
-// One sub-wave of SW lanes (lane = coil) per work item, 64/SW items per wave.  The
-// sub-wave walks its item's visits in batches of B: lane j < B loads the sample
-// position of visit j (one contiguous load), the positions are shared with
-// cross-lane shuffles, then all B sample-row loads (SW consecutive coils each) are
-// issued before the first use, so B loads are in flight per lane.  Weights are
-// broadcast loads independent of the positions.  Sums run in visit order.
 template <int SW, int B>
 __global__ __launch_bounds__(256) void gather_item_kernel(GatherArgs a) {
   constexpr int NSW = 64 / SW;
   const int lane = threadIdx.x & 63, sw = lane / SW, c = lane % SW;
-  const uint32_t item = (blockIdx.x * 4 + (threadIdx.x >> 6)) * NSW + sw;
+  // With xcd_chunk, workgroup w (dispatched to XCD w % 8) takes the w/8-th group
+  // of its XCD's contiguous item range, so neighbouring 2x2 blocks -- which read
+  // the same sample rows -- share one XCD's L2.
+  const uint32_t wg = a.xcd_chunk ? (blockIdx.x % kXcds) * a.wg_per_xcd + blockIdx.x / kXcds : blockIdx.x;
+  const uint32_t item = (wg * 4 + (threadIdx.x >> 6)) * NSW + sw;
   const bool live = item < a.item_count;
   const uint4 info = live ? a.item_info[item] : make_uint4(0, 0, 0, 0);
   float2 acc[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) acc[q] = make_float2(0.f, 0.f);
-  // Software pipeline: the positions of batch i+1 are in flight while batch i's rows load.
   uint32_t v = info.y;
-  uint32_t kj = load_positions<B>(a, v, info.z, c);
-  for (; v + B <= info.z; v += B) {
-    const uint32_t kn = load_positions<B>(a, v + B, info.z, c);
-    gather_batch<SW, B, false>(a, acc, kj, v, B, sw, c);
-    kj = kn;
-  }
-  if (v < info.z) gather_batch<SW, B, true>(a, acc, kj, v, info.z - v, sw, c);
+  uint32_t kj = load_positions<B>(a, v, info.z, SW >= 16 ? (c & 15) : c);
+  for (; v + B <= info.z; v += B) gather_batch<SW, B, false>(a, acc, kj, v, info.z, sw, c);
+  if (v < info.z) gather_batch<SW, B, true>(a, acc, kj, v, info.z, sw, c);
   if (!live) return;
   const uint32_t by = info.x / a.block_cols, bx = info.x % a.block_cols;
 #pragma unroll
@@ -195,13 +228,18 @@ template <int SW>
 void launch_gather(GatherArgs a, hipStream_t stream) {
   constexpr int NSW = 64 / SW;
   const uint32_t waves = ceil_div(a.item_count, NSW);
-  hipLaunchKernelGGL((gather_item_kernel<SW, 8>), dim3(ceil_div(waves, 4)), dim3(256), 0, stream, a);
-  KSJ_LAUNCH_CHECK();
-  if (a.multi_count > 0) {
-    hipLaunchKernelGGL(multi_reduce_kernel, dim3(ceil_div(static_cast<uint64_t>(a.multi_count) * 4 * a.cp, 256)),
-                       dim3(256), 0, stream, a);
-    KSJ_LAUNCH_CHECK();
+  uint32_t wgs = ceil_div(waves, 4);
+  a.xcd_chunk = tune_get(kTuneGatherMap) == 1 ? 1u : 0u;
+  if (a.xcd_chunk) {
+    a.wg_per_xcd = ceil_div(wgs, kXcds);
+    wgs = a.wg_per_xcd * kXcds;  // the padding workgroups find no item and exit
   }
+  switch (tune_get(kTuneGatherBatch)) {
+    case 4: hipLaunchKernelGGL((gather_item_kernel<SW, 4>), dim3(wgs), dim3(256), 0, stream, a); break;
+    case 16: hipLaunchKernelGGL((gather_item_kernel<SW, 16>), dim3(wgs), dim3(256), 0, stream, a); break;
+    default: hipLaunchKernelGGL((gather_item_kernel<SW, 8>), dim3(wgs), dim3(256), 0, stream, a); break;
+  }
+  KSJ_LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -219,24 +257,26 @@ size_t grid_partial_bytes(const Plan2& plan, uint32_t coils) {
 void grid_prep(const Plan2& plan, const CoilPlanes& planes, float2* d_sorted, hipStream_t stream) {
   require(plan.has_trajectory, "plan has no trajectory");
   require(planes.coils >= 1 && planes.coils <= kMaxStackedCoils, "gridding takes 1..64 stacked coils");
-  const uint32_t coils = planes.coils;
   PrepArgs a{};
   a.planes = planes;
   a.weight = plan.weight;
   a.rank = plan.rank;
   a.sorted = d_sorted;
   a.samples = plan.d.samples;
-  a.coils = coils;
-  a.cp = padded_coils(coils);
-  const size_t shmem = static_cast<size_t>(kPrepSamples) * (a.cp + 1) * sizeof(float2);
-  hipLaunchKernelGGL(prep_kernel, dim3(ceil_div(a.samples, kPrepSamples)), dim3(256), shmem, stream, a);
+  a.coils = planes.coils;
+  const dim3 grid(ceil_div(a.samples, kPrepSamples)), block(kPrepSamples * kPrepWaves);
+  switch (padded_coils(planes.coils)) {
+    case 8: hipLaunchKernelGGL(prep_kernel<8>, grid, block, 0, stream, a); break;
+    case 16: hipLaunchKernelGGL(prep_kernel<16>, grid, block, 0, stream, a); break;
+    case 32: hipLaunchKernelGGL(prep_kernel<32>, grid, block, 0, stream, a); break;
+    default: hipLaunchKernelGGL(prep_kernel<64>, grid, block, 0, stream, a); break;
+  }
   KSJ_LAUNCH_CHECK();
 }
 
-void grid_gather(const Plan2& plan, const float2* d_sorted, uint32_t coils, float2* d_grid, float2* d_partial,
-                 hipStream_t stream) {
-  require(plan.has_trajectory, "plan has no trajectory");
-  require(plan.multi_count == 0 || d_partial != nullptr, "multi-item blocks need a partial-sum buffer");
+namespace {
+GatherArgs gather_args(const Plan2& plan, const float2* d_sorted, uint32_t coils, float2* d_grid,
+                       float2* d_partial) {
   GatherArgs a{};
   a.sorted = d_sorted;
   a.visit_k = plan.visit_k;
@@ -253,6 +293,23 @@ void grid_gather(const Plan2& plan, const float2* d_sorted, uint32_t coils, floa
   a.cp = padded_coils(coils);
   a.item_count = plan.item_count;
   a.multi_count = plan.multi_count;
+  return a;
+}
+}  // namespace
+
+void grid_reduce(const Plan2& plan, uint32_t coils, float2* d_grid, const float2* d_partial, hipStream_t stream) {
+  if (plan.multi_count == 0) return;
+  const GatherArgs a = gather_args(plan, nullptr, coils, d_grid, const_cast<float2*>(d_partial));
+  hipLaunchKernelGGL(multi_reduce_kernel, dim3(ceil_div(static_cast<uint64_t>(a.multi_count) * 4 * a.cp, 256)),
+                     dim3(256), 0, stream, a);
+  KSJ_LAUNCH_CHECK();
+}
+
+void grid_gather(const Plan2& plan, const float2* d_sorted, uint32_t coils, float2* d_grid, float2* d_partial,
+                 hipStream_t stream) {
+  require(plan.has_trajectory, "plan has no trajectory");
+  require(plan.multi_count == 0 || d_partial != nullptr, "multi-item blocks need a partial-sum buffer");
+  const GatherArgs a = gather_args(plan, d_sorted, coils, d_grid, d_partial);
   switch (a.cp) {
     case 8: return launch_gather<8>(a, stream);
     case 16: return launch_gather<16>(a, stream);

@@ -30,5 +30,8 @@ size_t grid_partial_bytes(const Plan2& plan, uint32_t coils);
 // bit-reproducible for a given plan.
 void grid_gather(const Plan2& plan, const float2* d_sorted, uint32_t coils, float2* d_grid, float2* d_partial,
                  hipStream_t stream);
+// Sums the partial results of blocks cut into several work items into the grid
+// (item order, so the sum is deterministic); a no-op when no block was cut.
+void grid_reduce(const Plan2& plan, uint32_t coils, float2* d_grid, const float2* d_partial, hipStream_t stream);
 
 }  // namespace ksj::gpu

@@ -131,6 +131,10 @@ void adjoint2_execute(const Plan2& plan, const AdjointScan* scans, uint32_t coun
       ProfScope ps("grid_gather", stream);
       grid_gather(plan, sorted, total, grid, partial, stream);
     }
+    if (plan.multi_count > 0) {
+      ProfScope ps("grid_reduce", stream);
+      grid_reduce(plan, total, grid, partial, stream);
+    }
 
     // Row pass: lines (grid row y, coil c) of the coil-fastest grid [G_r][G_c][Cp],
     // loaded coil-fastest, stored as coil planes [C][G_r][N_c].

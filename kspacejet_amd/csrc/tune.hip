@@ -1,0 +1,42 @@
+// Process-wide launch parameters (see TuneKey in common.hpp).
+#include "common.hpp"
+
+#include <atomic>
+#include <cstring>
+
+namespace ksj::gpu {
+
+namespace {
+
+struct Param {
+  const char* name;
+  int64_t value;
+  bool (*legal)(int64_t);
+};
+
+bool is_pow2_or_zero(int64_t v) { return v == 0 || (v > 0 && v <= 64 && (v & (v - 1)) == 0); }
+
+Param g_params[kTuneKeys] = {
+    {"gather_map", 0, [](int64_t v) { return v == 0 || v == 1; }},
+    {"item_visits", 64, [](int64_t v) { return v >= 1 && v <= 4096; }},
+    {"fft_lines_per_block", 0, &is_pow2_or_zero},
+    {"gather_batch", 8, [](int64_t v) { return v == 4 || v == 8 || v == 16; }},
+};
+
+}  // namespace
+
+int64_t tune_get(TuneKey key) { return g_params[key].value; }
+
+void tune_set(TuneKey key, int64_t value) {
+  require(key >= 0 && key < kTuneKeys, "unknown tuning key");
+  require(g_params[key].legal(value), "illegal tuning value");
+  g_params[key].value = value;
+}
+
+int tune_key(const char* name) {
+  for (int k = 0; k < kTuneKeys; ++k)
+    if (name != nullptr && std::strcmp(name, g_params[k].name) == 0) return k;
+  return -1;
+}
+
+}  // namespace ksj::gpu

@@ -41,6 +41,17 @@ def init(device: int = 0) -> None:
     call("ksj_gpu_init", device)
 
 
+def tune_set(key: str, value: int) -> None:
+    """Sets a process-wide launch parameter (see ksj_gpu_tune_set)."""
+    call("ksj_gpu_tune_set", key.encode(), int(value))
+
+
+def tune_get(key: str) -> int:
+    v = C.c_int64(0)
+    call("ksj_gpu_tune_get", key.encode(), C.byref(v))
+    return int(v.value)
+
+
 def ifft2_coil_images(kspace: torch.Tensor, centred: bool = False, layout: str = "coil_last") -> torch.Tensor:
     """[C, R, Cols] complex64 channel-major k-space -> coil images (1/R/Cols scaling)."""
     _need(kspace, torch.complex64, "kspace")

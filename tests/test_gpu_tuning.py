@@ -1,0 +1,81 @@
+"""Launch parameters the autotuner may set change speed, not results: every
+legal setting of gather_map, gather_batch and fft_lines_per_block reproduces the
+default result bit for bit (item_visits changes the float summation order and is
+checked against the oracle instead)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from kspacejet_amd import synth
+from tests.helpers import rel_l2
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ops():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from kspacejet_amd import ops as K
+    K.init(0)
+    return K
+
+
+@pytest.fixture
+def tune(ops):
+    """Sets tuning keys for one test and restores the previous values afterwards."""
+    saved = {}
+
+    def set_(key, value):
+        if key not in saved:
+            saved[key] = ops.tune_get(key)
+        ops.tune_set(key, value)
+
+    yield set_
+    for key, value in saved.items():
+        ops.tune_set(key, value)
+
+
+def scan(ops, n=96, coils=12, spokes=61, readout=192, seed=3):
+    traj = ops.golden_angle2(spokes, readout)
+    img = torch.from_numpy(synth.coil_images2(n, coils, seed)).cuda()
+    k = ops.nudft2_forward_f64(traj, img, (n - 1) / 2, (n - 1) / 2).cpu().numpy()
+    return traj, synth.add_noise(k, seed).astype(np.complex64)
+
+
+def recon(ops, traj, k, n, item_visits=0):
+    plan = ops.Plan2(n, n, traj.shape[0], item_visits=item_visits).set_trajectory(traj)
+    img, rss = plan.adjoint(torch.from_numpy(k).cuda(), rss=True)
+    torch.cuda.synchronize()
+    return img.cpu().numpy(), rss.cpu().numpy()
+
+
+def test_launch_parameters_do_not_change_results(ops, tune):
+    n = 96
+    traj, k = scan(ops)
+    base_img, base_rss = recon(ops, traj, k, n)
+    for key, values in (("gather_map", (0, 1)), ("gather_batch", (4, 8, 16)), ("fft_lines_per_block", (0, 4, 16))):
+        for v in values:
+            tune(key, v)
+            img, rss = recon(ops, traj, k, n)
+            assert np.array_equal(img, base_img) and np.array_equal(rss, base_rss), (key, v)
+        tune(key, values[0])
+
+
+@pytest.mark.parametrize("item_visits", [16, 256])
+def test_item_size_keeps_oracle_parity(ops, tune, item_visits):
+    n = 64
+    traj, k = scan(ops, n=n, coils=4, spokes=41, readout=128, seed=5)
+    tune("item_visits", item_visits)
+    img, _ = recon(ops, traj, k, n)
+    t = traj.cpu().numpy().astype(np.float64)
+    ref = O.kb_adjoint2_coils(k, t, O.ramp_dcf2(t), n)
+    assert rel_l2(img, ref) <= 1e-5
+
+
+def test_illegal_values_are_rejected(ops):
+    with pytest.raises(RuntimeError):
+        ops.tune_set("gather_batch", 5)
+    with pytest.raises(RuntimeError):
+        ops.tune_set("no_such_key", 1)
