@@ -6,9 +6,20 @@ Workload (the headline configuration): 2-D golden-angle radial adjoint NUFFT of 
 Ram-Lak ramp density compensation, Kaiser-Bessel width 4 with Beatty beta on a
 512x512 grid (sigma 2), FP32 gridding/FFT, deapodize, crop, root-sum-of-squares.
 
-A step is one pass of the hot path over one batch of `--batch` scans that are
-already resident in HBM (k-space samples + trajectory); the output of every scan
-is its RSS image.  `value` is images per second over the timed steps.
+A step is one burst of `--batch` scans, already resident in HBM (k-space and
+trajectory), queued on the continuous-batching server (ksj_gpu_serve_*); the K
+bursts are queued back to back and the server is drained once at the end, so the
+timed region is the server running at saturation.  Every scan's output is its RSS
+image.  `value` is served images per second over the timed steps.  Beside it the line reports:
+  * latency: p50/p99 submit-to-completion latency of an open-loop Poisson stream of
+    the same scans at `--load` x the measured served rate;
+  * plain: the same scans through the per-scan operator (ksj_gpu_adjoint2_execute,
+    one synchronous call per scan), images/s and ms per scan;
+  * stages / roofline: per-kernel HIP-event timings of the batched pass (one pass
+    lane, so kernels do not overlap) and the gather kernel against HBM bandwidth;
+  * mixed: a mixed 2-D stream (radial scans of the headline shape and Cartesian
+    scans of 128/256/384 matrix with 8-32 coils), served vs plain;
+  * cpu_baseline: the reference's own radial route compiled from its sources.
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1]
 
@@ -19,6 +30,8 @@ noise (sigma = 1e-3 max|k|), stored complex64.
 from __future__ import annotations
 
 import argparse
+import ctypes as C
+import hashlib
 import json
 import os
 import sys
@@ -31,8 +44,9 @@ sys.path.insert(0, HERE)
 
 METRIC = "2D radial recon images/s + p50/p99 scan latency, batched-served vs plain, 1 GPU"
 HBM_PEAK_GBS = 8000.0
-
 N, COILS, SPOKES, READOUT, WIDTH, SIGMA = 256, 16, 402, 512, 4, 2.0
+STAGES = ("grid_prep", "grid_gather", "grid_reduce", "fft_rows", "fft_cols_deapod", "coil_combine")
+TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "radial256_c16_kb4_hbm_traffic.txt")
 
 
 def log(*a):
@@ -53,25 +67,17 @@ def make_scans(ops, torch, traj, count, distinct):
 
 
 def algorithmic_bytes(stage: str) -> float:
-    """Compulsory HBM bytes of one launch per image (cfg figures of the design notes)."""
+    """Compulsory HBM bytes of one stage per image of the headline workload."""
     S = SPOKES * READOUT
     G = int(N * SIGMA)
-    if stage == "grid_gather":
-        return 8.0 * S * COILS + 12.0 * S + 8.0 * G * G * COILS
-    if stage == "grid_prep":
-        return 16.0 * S * COILS + 12.0 * S
-    if stage == "fft_rows":
-        return 8.0 * G * G * COILS + 8.0 * G * N * COILS
-    if stage == "fft_cols_deapod":
-        return 8.0 * G * N * COILS + 8.0 * N * N * COILS
-    if stage == "coil_combine":
-        return 8.0 * N * N * COILS + 4.0 * N * N
-    if stage == "grid_reduce":
-        return 0.0  # partial sums of blocks cut into several items: no compulsory bytes
-    raise KeyError(stage)
-
-
-TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "radial256_c16_kb4_hbm_traffic.txt")
+    return {
+        "grid_gather": 8.0 * S * COILS + 12.0 * S + 8.0 * G * G * COILS,
+        "grid_prep": 16.0 * S * COILS + 12.0 * S,
+        "grid_reduce": 0.0,  # partial sums of blocks cut into several items: no compulsory bytes
+        "fft_rows": 8.0 * G * G * COILS + 8.0 * G * N * COILS,
+        "fft_cols_deapod": 8.0 * G * N * COILS + 8.0 * N * N * COILS,
+        "coil_combine": 8.0 * N * N * COILS + 4.0 * N * N,
+    }[stage]
 
 
 def measured_traffic(scans_per_launch: int):
@@ -106,8 +112,12 @@ def cpu_baseline(scan_np, traj_np, seconds_hint: float):
     return {"value": scans / dt, "unit": "images/s", "cores": threads, "kind": "reference",
             "sample": f"{scans} scans of the 256x256x16-coil 402x512 radial shape through the reference route "
                       f"(ramp DCF + bilinear gridding on 256x256 + radix-2 inverse FFT per coil, then RSS) "
-                      f"compiled from the reference's eigen_radial_gridding.cpp; sample-count envelope lifted; "
-                      f"{dt:.1f} s wall"}
+                      f"compiled from the reference's eigen_radial_gridding.cpp, scan-parallel on {threads} "
+                      f"threads; sample-count envelope lifted; {dt:.1f} s wall"}
+
+
+def percentile(xs, q):
+    return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
 
 
 def main(argv=None) -> int:
@@ -115,8 +125,13 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--batch", type=int, default=32, help="scans per step")
     ap.add_argument("--distinct", type=int, default=4, help="distinct seeded scans behind the batch")
+    ap.add_argument("--lanes", type=int, default=3, help="server lanes (batches in flight)")
+    ap.add_argument("--max-batch", type=int, default=16, help="scans stacked into one server launch")
+    ap.add_argument("--load", type=float, default=0.7, help="Poisson arrival rate as a fraction of served rate")
+    ap.add_argument("--latency-requests", type=int, default=300)
+    ap.add_argument("--mixed-requests", type=int, default=48, help="0 skips the mixed-stream section")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
@@ -130,7 +145,7 @@ def main(argv=None) -> int:
         return 2
 
     import torch
-    from kspacejet_amd import ops
+    from kspacejet_amd import ops, _lib
     from kspacejet_amd._lib import load
 
     ops.init(0)
@@ -141,25 +156,87 @@ def main(argv=None) -> int:
             raise SystemExit(f"--tune {kv}: {lib.ksj_gpu_last_error().decode()}")
     traj = ops.golden_angle2(SPOKES, READOUT)
     S = SPOKES * READOUT
+    traj_key = int.from_bytes(hashlib.sha256(traj.cpu().numpy().tobytes()).digest()[:8], "little") | 1
     log("generating synthetic scans ...")
     scans, scan0 = make_scans(ops, torch, traj, args.batch, min(args.distinct, args.batch))
+    outs = [torch.empty((N, N), dtype=torch.float32, device="cuda") for _ in range(args.batch)]
+    stream = lambda: torch.cuda.current_stream().cuda_stream
+
+    # ---------------------------------------------------------------- served
+    server = ops.Server(lanes=args.lanes, max_batch_scans=args.max_batch)
+    Req = _lib.ServeRequest * args.batch
+    reqs = Req(*[_lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_RADIAL2D, N, N, COILS, S, WIDTH,
+                                   _lib.DENSITY_RAMP, 0, 0, traj_key, traj.data_ptr(), k.data_ptr(), None,
+                                   o.data_ptr()) for k, o in zip(scans, outs)])
+    tickets = (C.c_uint64 * args.batch)()
+
+    def served_step():
+        """Queues one burst; the server works on it while the next burst is queued."""
+        rc = lib.ksj_gpu_serve_submit_batch(server._handle, args.batch, reqs, tickets)
+        if rc != 0:
+            raise RuntimeError(lib.ksj_gpu_last_error().decode())
+        return list(tickets)
+
+    log("served bursts ...")
+    for _ in range(args.warmup):
+        for t in served_step():
+            server.wait(t)
+    torch.cuda.synchronize()
+    step_tickets = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step_tickets += served_step()
+    server.drain()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    burst_lat = [server.wait(t) for t in step_tickets]
+    images = args.batch * args.steps
+    value = images / elapsed
+
+    # Open-loop Poisson stream at a fraction of the served rate: per-scan latency.
+    log("open-loop latency stream ...")
+    rate = args.load * value
+    rng = np.random.default_rng(7)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, args.latency_requests))
+    one = _lib.ServeRequest * 1
+    stream_tickets = []
+    t_start = time.perf_counter()
+    for i, t_arr in enumerate(arrivals):
+        while time.perf_counter() - t_start < t_arr:
+            pass
+        r = one(reqs[i % args.batch])
+        t = C.c_uint64(0)
+        if lib.ksj_gpu_serve_submit_batch(server._handle, 1, r, C.byref(t)) != 0:
+            raise RuntimeError(lib.ksj_gpu_last_error().decode())
+        stream_tickets.append(t.value)
+    stream_lat = [server.wait(t) for t in stream_tickets]
+    served_stats = server.stats()
+
+    # ---------------------------------------------------------------- plain
+    log("plain per-scan operator ...")
     plan = ops.Plan2(N, N, S, oversampling=SIGMA, width=WIDTH).set_trajectory(traj)
     ws = torch.empty(plan.workspace_bytes(COILS), dtype=torch.uint8, device="cuda")
-    outs = [torch.empty((N, N), dtype=torch.float32, device="cuda") for _ in range(args.batch)]
 
-    import ctypes as C
-
-    stream = lambda: torch.cuda.current_stream().cuda_stream
-    per_pass = max(1, 64 // COILS)  # scans stacked per gridding pass (64 coil lanes)
-    ws_batch = torch.empty(plan.batch_workspace_bytes(COILS, args.batch), dtype=torch.uint8, device="cuda")
-    P = C.c_void_p * args.batch
-    k_ptrs = P(*[k.data_ptr() for k in scans])
-    r_ptrs = P(*[o.data_ptr() for o in outs])
-
-    def run_plain():
+    def plain_step():
         for k, out in zip(scans, outs):
             lib.ksj_gpu_adjoint2_execute(plan._handle, k.data_ptr(), COILS, None, 0, out.data_ptr(),
                                          ws.data_ptr(), ws.numel(), stream())
+            torch.cuda.synchronize()  # one synchronous call per scan, as a per-firing operator runs
+
+    for _ in range(args.warmup):
+        plain_step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plain_step()
+    plain_value = images / (time.perf_counter() - t0)
+
+    # ---------------------------------------------------------------- stages and roofline
+    P = C.c_void_p * args.batch
+    k_ptrs = P(*[k.data_ptr() for k in scans])
+    r_ptrs = P(*[o.data_ptr() for o in outs])
+    lanes_saved = ops.tune_get("pass_lanes")
+    ops.tune_set("pass_lanes", 1)  # kernels back to back: per-kernel durations without overlap
+    ws_batch = torch.empty(plan.batch_workspace_bytes(COILS, args.batch), dtype=torch.uint8, device="cuda")
 
     def run_batched():
         rc = lib.ksj_gpu_adjoint2_execute_batch(plan._handle, args.batch, k_ptrs, COILS, None, 0, r_ptrs,
@@ -167,25 +244,41 @@ def main(argv=None) -> int:
         if rc != 0:
             raise RuntimeError(lib.ksj_gpu_last_error().decode())
 
-    def timed(fn, steps):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            fn()
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0
+    run_batched()
+    lib.ksj_gpu_profile_enable(1)
+    lib.ksj_gpu_profile_reset()
+    torch.cuda.synchronize()
+    for _ in range(max(1, args.steps // 2)):
+        run_batched()
+    torch.cuda.synchronize()
+    stages = {}
+    for name in STAGES:
+        tot, cnt = C.c_double(0), C.c_uint64(0)
+        lib.ksj_gpu_profile_read(name.encode(), C.byref(tot), C.byref(cnt))
+        stages[name] = {"avg_us": 1e3 * tot.value / max(cnt.value, 1), "launches": cnt.value}
+    lib.ksj_gpu_profile_enable(0)
+    ops.tune_set("pass_lanes", lanes_saved)
+    del ws_batch
+    per_pass = max(1, 64 // COILS)
+    scans_per_launch = min(per_pass, args.batch)
+    g = stages["grid_gather"]
+    per_launch_bytes = algorithmic_bytes("grid_gather") * scans_per_launch
+    achieved = per_launch_bytes / (g["avg_us"] * 1e-6) / 1e9
+    roofline = {"kernel": "gather_item_kernel<64,8> (grid_gather)", "bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(scans_per_launch), "scans_per_launch": scans_per_launch,
+                "algorithmic_bytes_per_launch": per_launch_bytes, "avg_us": round(g["avg_us"], 2)}
+    stage_report = {k: {"avg_us": round(v["avg_us"], 2),
+                        "gbs": round(algorithmic_bytes(k) * scans_per_launch / max(v["avg_us"] * 1e-6, 1e-12) / 1e9,
+                                     1)}
+                    for k, v in stages.items() if v["launches"]}
 
-    def stage_times(fn, steps):
-        lib.ksj_gpu_profile_enable(1)
-        lib.ksj_gpu_profile_reset()
-        timed(fn, steps)
-        out = {}
-        for name in ("grid_prep", "grid_gather", "grid_reduce", "fft_rows", "fft_cols_deapod", "coil_combine"):
-            tot, cnt = C.c_double(0), C.c_uint64(0)
-            lib.ksj_gpu_profile_read(name.encode(), C.byref(tot), C.byref(cnt))
-            out[name] = {"avg_us": 1e3 * tot.value / max(cnt.value, 1), "launches": cnt.value}
-        lib.ksj_gpu_profile_enable(0)
-        return out
+    # ---------------------------------------------------------------- mixed stream
+    mixed = None
+    if args.mixed_requests > 0:
+        log("mixed stream ...")
+        mixed = run_mixed(torch, lib, _lib, server, plan, scans, traj, traj_key, args.mixed_requests, stream)
+    server.close()
 
     if args.copy_probe:
         src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
@@ -193,28 +286,7 @@ def main(argv=None) -> int:
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    images = args.batch * args.steps
-    for _ in range(args.warmup):
-        run_plain()
-        run_batched()
-    plain_value = images / timed(run_plain, args.steps)
-    elapsed = timed(run_batched, args.steps)
-    value = images / elapsed
-    stages_plain = stage_times(run_plain, max(1, args.steps // 2))
-    stages = stage_times(run_batched, max(1, args.steps // 2))
-    scans_per_launch = min(per_pass, args.batch)
-    dominant = max(stages, key=lambda s: stages[s]["avg_us"] * stages[s]["launches"])
-    g = stages["grid_gather"]
-    per_launch_bytes = algorithmic_bytes("grid_gather") * scans_per_launch
-    achieved = per_launch_bytes / (g["avg_us"] * 1e-6) / 1e9
-    roofline = {"kernel": "grid_gather", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(scans_per_launch),
-                "scans_per_launch": scans_per_launch, "algorithmic_bytes_per_launch": per_launch_bytes,
-                "avg_us": round(g["avg_us"], 2)}
-    stage_report = {k: {"avg_us": round(v["avg_us"], 2),
-                        "gbs": round(algorithmic_bytes(k) * scans_per_launch / max(v["avg_us"] * 1e-6, 1e-12) / 1e9, 1)}
-                    for k, v in stages.items() if v["launches"]}
-    plain_report = {k: round(v["avg_us"], 2) for k, v in stages_plain.items()}
+
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU reference route ...")
@@ -224,16 +296,97 @@ def main(argv=None) -> int:
         "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic: seeded 10-ellipse phantom x 16 Gaussian coils, exact FP64 NUDFT k-space + noise, complex64",
-        "config": {"workload": "2-D golden-angle radial adjoint NUFFT + RSS, batched (scans on one trajectory "
-                               "stacked along the coil axis), device-resident",
+        "config": {"workload": "2-D golden-angle radial adjoint NUFFT + RSS, served: bursts of device-resident scans "
+                               "through the continuous-batching server (same-trajectory scans stacked along the coil "
+                               "axis, resident plan)",
                    "image": N, "coils": COILS, "spokes": SPOKES, "readout": READOUT, "kernel": "kaiser_bessel",
-                   "kernel_width": WIDTH, "oversampling": SIGMA, "grid": int(N * SIGMA), "batch": args.batch},
-        "roofline": roofline, "dominant_stage": dominant, "stages": stage_report, "cpu_baseline": cpu,
-        "plain_images_per_s": round(plain_value, 1), "batched_over_plain": round(value / plain_value, 2),
-        "plain_stage_us": plain_report,
+                   "kernel_width": WIDTH, "oversampling": SIGMA, "grid": int(N * SIGMA), "batch": args.batch,
+                   "server_lanes": args.lanes, "server_max_batch": args.max_batch},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "latency": {"p50_ms": round(percentile(stream_lat, 50), 3), "p99_ms": round(percentile(stream_lat, 99), 3),
+                    "arrival_rate_per_s": round(rate, 1), "requests": len(stream_lat),
+                    "process": "open-loop Poisson arrivals, one scan per request",
+                    "saturated_p50_ms": round(percentile(burst_lat, 50), 3),
+                    "saturated_p99_ms": round(percentile(burst_lat, 99), 3)},
+        "plain": {"images_per_s": round(plain_value, 1), "ms_per_scan": round(1e3 / plain_value, 3),
+                  "operator": "ksj_gpu_adjoint2_execute, one synchronous call per scan, plan resident"},
+        "served_over_plain": round(value / plain_value, 2),
+        "stages": stage_report,
+        "served_stats": served_stats,
+        "mixed": mixed,
     }
     print(json.dumps(line))
     return 0
+
+
+def run_mixed(torch, lib, _lib, server, plan, radial_scans, traj, traj_key, count, stream):
+    """Mixed 2-D stream: two thirds radial scans of the headline shape, one third Cartesian
+    scans (matrix 128/256/384, 8-32 coils, centred), served vs one synchronous call each."""
+    from kspacejet_amd import synth
+    rng = np.random.default_rng(11)
+    S = SPOKES * READOUT
+    cart_shapes = [(int(rng.choice([128, 256, 384])), int(rng.integers(8, 33))) for _ in range(6)]
+    cart = [torch.from_numpy(synth.cartesian_kspace2(n, c, synth.scan_seed(5, i)).astype(np.complex64)).cuda()
+            for i, (n, c) in enumerate(cart_shapes)]
+    kinds = rng.permutation(["radial"] * (2 * count // 3) + ["cartesian"] * (count - 2 * count // 3))
+    reqs, jobs = [], []
+    for i, kind in enumerate(kinds):
+        if kind == "radial":
+            k = radial_scans[i % len(radial_scans)]
+            o = torch.empty((N, N), dtype=torch.float32, device="cuda")
+            reqs.append(_lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_RADIAL2D, N, N, COILS, S, WIDTH,
+                                          _lib.DENSITY_RAMP, 0, 0, traj_key, traj.data_ptr(), k.data_ptr(), None,
+                                          o.data_ptr()))
+        else:
+            k = cart[i % len(cart)]
+            c, n, _ = k.shape
+            o = torch.empty((n, n), dtype=torch.float32, device="cuda")
+            reqs.append(_lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_CARTESIAN2D, n, n, c, 0, 0, 0, 1, 0,
+                                          0, None, k.data_ptr(), None, o.data_ptr()))
+        jobs.append((kind, k, o))
+    arr = (_lib.ServeRequest * count)(*reqs)
+    tickets = (C.c_uint64 * count)()
+
+    def served():
+        server.hold(True)
+        if lib.ksj_gpu_serve_submit_batch(server._handle, count, arr, tickets) != 0:
+            raise RuntimeError(lib.ksj_gpu_last_error().decode())
+        server.hold(False)
+        server.drain()
+        return [server.wait(t) for t in tickets]
+
+    ws = torch.empty(max(plan.workspace_bytes(COILS),
+                         max(lib.ksj_gpu_ifft2_workspace_bytes(c, n, n) for n, c in cart_shapes)),
+                     dtype=torch.uint8, device="cuda")
+    coil_buf = torch.empty(max(n * n * c for n, c in cart_shapes), dtype=torch.complex64, device="cuda")
+
+    def plain():
+        for kind, k, o in jobs:
+            if kind == "radial":
+                lib.ksj_gpu_adjoint2_execute(plan._handle, k.data_ptr(), COILS, None, 0, o.data_ptr(), ws.data_ptr(),
+                                             ws.numel(), stream())
+            else:
+                c, n, _ = k.shape
+                lib.ksj_gpu_ifft2_coil_images(k.data_ptr(), coil_buf.data_ptr(), c, n, n, 1, 0, ws.data_ptr(),
+                                              ws.numel(), stream())
+                lib.ksj_gpu_rss(coil_buf.data_ptr(), o.data_ptr(), n * n, c, stream())
+            torch.cuda.synchronize()
+
+    served()
+    plain()
+    t0 = time.perf_counter()
+    lat = served()
+    t_served = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plain()
+    t_plain = time.perf_counter() - t0
+    return {"requests": count, "radial": int((kinds == "radial").sum()), "cartesian": int((kinds == "cartesian").sum()),
+            "cartesian_shapes": [f"{n}x{n}x{c}" for n, c in cart_shapes],
+            "served_images_per_s": round(count / t_served, 1), "plain_images_per_s": round(count / t_plain, 1),
+            "served_over_plain": round(t_plain / t_served, 2), "burst_p50_ms": round(percentile(lat, 50), 3),
+            "burst_p99_ms": round(percentile(lat, 99), 3),
+            "note": "3-D radial requests of the reference stream are not part of this build"}
 
 
 if __name__ == "__main__":

@@ -177,6 +177,76 @@ int ksj_gpu_adjoint2_execute_batch(const ksj_gpu_plan2* plan, uint32_t count, co
                                    uint32_t channels, void* const* d_images, int layout, float* const* d_rss,
                                    void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------ serving */
+
+/* A continuous-batching server on one GPU.  No reference counterpart: the reference
+ * runs one frame per synchronous firing (providers/kspacejet-noncartesian-recon/src/
+ * operators/radial_gridding_reconstruct.cpp:228-437); this is the served form of the
+ * same operators for a stream of scans whose inputs already sit in HBM.
+ *
+ * A scheduler thread admits requests in arrival order: a batch is launched when one of
+ * `lanes` streams is free and its workspace fits the HBM budget next to the resident
+ * plans.  Radial requests that share a plan (same geometry and trajectory key) are
+ * stacked along the coil axis into one launch of up to max_batch_scans scans;
+ * Cartesian requests of one shape share a launch.  Plans stay resident per
+ * (geometry, trajectory key), least recently used evicted first.  Requests that arrive
+ * while others run join the next free lane; batches on different lanes overlap.  Each
+ * result equals the result of the same request run alone. */
+#define KSJ_GPU_SERVE_RADIAL2D 0     /* KB adjoint NUFFT, grid 2N, ramp or no density weights */
+#define KSJ_GPU_SERVE_CARTESIAN2D 1  /* inverse FFT (Normalization::inverse), centred or not */
+
+typedef struct ksj_gpu_serve_config {
+  uint32_t struct_size;
+  int32_t device;
+  uint64_t hbm_budget_bytes;   /* 0 = 80% of the device's free memory */
+  uint32_t lanes;              /* batches in flight, 1..8 (0 = 2) */
+  uint32_t max_batch_scans;    /* scans per radial launch, 1..64 (0 = 8) */
+  uint32_t plan_cache_entries; /* 0 = 16 */
+  uint32_t reserved0;
+} ksj_gpu_serve_config;
+
+typedef struct ksj_gpu_serve_request {
+  uint32_t struct_size;
+  uint32_t kind;                 /* KSJ_GPU_SERVE_* */
+  uint32_t image_rows, image_cols, channels;
+  uint32_t samples;              /* radial */
+  uint32_t kernel_width;         /* radial: 4, 6 or 8 */
+  uint32_t density;              /* radial: KSJ_GPU_DENSITY_NONE or _RAMP */
+  uint32_t centred;              /* Cartesian */
+  uint32_t reserved0;
+  uint64_t trajectory_key;       /* radial: names the trajectory bytes; 0 = the server hashes them */
+  const float* d_traj;           /* radial: [S][ky,kx] rad/px */
+  const void* d_kspace;          /* [C][S] (radial) or [C][R][Cols] (Cartesian) cf32 */
+  void* d_images;                /* coil-last [R][Cols][C] cf32, or NULL */
+  float* d_rss;                  /* [R][Cols] float, or NULL */
+} ksj_gpu_serve_request;
+
+typedef struct ksj_gpu_serve_stats {
+  uint32_t struct_size, reserved0;
+  uint64_t submitted, completed, failed, batches;
+  uint64_t plan_builds, plan_hits, plan_evictions;
+  uint64_t peak_device_bytes, plan_bytes, max_batch_seen;
+} ksj_gpu_serve_stats;
+
+typedef struct ksj_gpu_server ksj_gpu_server;
+
+int ksj_gpu_serve_create(const ksj_gpu_serve_config* config, ksj_gpu_server** out_server);
+/* Queues a request (device buffers must stay valid until it completes); *ticket > 0. */
+int ksj_gpu_serve_submit(ksj_gpu_server* server, const ksj_gpu_serve_request* request, uint64_t* ticket);
+/* Queues `count` requests at once (all validated first; none is queued on an error). */
+int ksj_gpu_serve_submit_batch(ksj_gpu_server* server, uint32_t count, const ksj_gpu_serve_request* requests,
+                               uint64_t* tickets);
+/* Blocks until the ticket completes; returns its status, and the submit-to-completion
+ * latency in milliseconds.  Each ticket is waited on once. */
+int ksj_gpu_serve_wait(ksj_gpu_server* server, uint64_t ticket, double* latency_ms);
+/* Blocks until every submitted request has completed. */
+int ksj_gpu_serve_drain(ksj_gpu_server* server);
+/* on != 0: launch nothing new until called again with 0 (requests queue up meanwhile
+ * and are batched when released).  Requests still queued at destroy are dropped. */
+int ksj_gpu_serve_hold(ksj_gpu_server* server, int on);
+int ksj_gpu_serve_get_stats(ksj_gpu_server* server, ksj_gpu_serve_stats* out_stats);
+void ksj_gpu_serve_destroy(ksj_gpu_server* server);
+
 /* ------------------------------------------------------------------ tuning */
 
 /* Process-wide launch parameters chosen by the autotuner (kspacejet_amd/autotune.py).
@@ -185,6 +255,8 @@ int ksj_gpu_adjoint2_execute_batch(const ksj_gpu_plan2* plan, uint32_t count, co
  *   "item_visits"          1..4096, default gather work-item size of plans created afterwards
  *   "fft_lines_per_block"  0 = per-length default, else 1..64 (power of two) lines per FFT block
  *   "gather_batch"         4, 8 or 16 gather visits per batch (sample-row loads in flight per lane)
+ *   "pass_lanes"           1..3 streams over which the passes of a batched adjoint overlap (the
+ *                          batch workspace holds one pass workspace per lane)
  * Every setting gives bit-identical results, except item_visits, which sets the
  * float summation order of the gather.  Not thread-safe against concurrent launches. */
 int ksj_gpu_tune_set(const char* key, int64_t value);

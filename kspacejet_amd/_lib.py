@@ -10,7 +10,6 @@ import os
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 GPU_LIB_PATH = os.path.join(PKG, "libksj_gpu.so")
-PROVIDER_LIB_PATH = os.path.join(PKG, "libksj_provider_gpu.so")
 
 OK = 0
 LAYOUT_COIL_LAST = 0
@@ -45,6 +44,29 @@ class Plan2Info(C.Structure):
                 ("device_bytes", C.c_uint64), ("beta", C.c_double)]
 
 
+class ServeConfig(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("device", C.c_int32), ("hbm_budget_bytes", C.c_uint64),
+                ("lanes", C.c_uint32), ("max_batch_scans", C.c_uint32), ("plan_cache_entries", C.c_uint32),
+                ("reserved0", C.c_uint32)]
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("kind", C.c_uint32), ("image_rows", C.c_uint32),
+                ("image_cols", C.c_uint32), ("channels", C.c_uint32), ("samples", C.c_uint32),
+                ("kernel_width", C.c_uint32), ("density", C.c_uint32), ("centred", C.c_uint32),
+                ("reserved0", C.c_uint32), ("trajectory_key", C.c_uint64), ("d_traj", C.c_void_p),
+                ("d_kspace", C.c_void_p), ("d_images", C.c_void_p), ("d_rss", C.c_void_p)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("reserved0", C.c_uint32)] + [
+        (n, C.c_uint64) for n in ("submitted", "completed", "failed", "batches", "plan_builds", "plan_hits",
+                                  "plan_evictions", "peak_device_bytes", "plan_bytes", "max_batch_seen")]
+
+
+SERVE_RADIAL2D = 0
+SERVE_CARTESIAN2D = 1
+
 _vp = C.c_void_p
 _u32 = C.c_uint32
 _u64 = C.c_uint64
@@ -76,6 +98,14 @@ SIGNATURES = {
     "ksj_gpu_adjoint2_batch_workspace_bytes": (_sz, [_vp, _u32, _u32]),
     "ksj_gpu_adjoint2_execute_batch": (_int, [_vp, _u32, C.POINTER(_vp), _u32, C.POINTER(_vp), _int,
                                                C.POINTER(_vp), _vp, _sz, _vp]),
+    "ksj_gpu_serve_create": (_int, [C.POINTER(ServeConfig), C.POINTER(_vp)]),
+    "ksj_gpu_serve_submit": (_int, [_vp, C.POINTER(ServeRequest), C.POINTER(_u64)]),
+    "ksj_gpu_serve_submit_batch": (_int, [_vp, _u32, C.POINTER(ServeRequest), C.POINTER(_u64)]),
+    "ksj_gpu_serve_wait": (_int, [_vp, _u64, C.POINTER(_dbl)]),
+    "ksj_gpu_serve_drain": (_int, [_vp]),
+    "ksj_gpu_serve_hold": (_int, [_vp, _int]),
+    "ksj_gpu_serve_get_stats": (_int, [_vp, C.POINTER(ServeStats)]),
+    "ksj_gpu_serve_destroy": (None, [_vp]),
     "ksj_gpu_tune_set": (_int, [C.c_char_p, C.c_int64]),
     "ksj_gpu_tune_get": (_int, [C.c_char_p, C.POINTER(C.c_int64)]),
     "ksj_gpu_profile_enable": (_int, [_int]),

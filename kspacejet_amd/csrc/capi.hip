@@ -8,6 +8,7 @@
 #include "plan.hpp"
 #include "prof.hpp"
 #include "recon.hpp"
+#include "serve.hpp"
 
 #include <cstring>
 #include <new>
@@ -16,6 +17,10 @@
 
 struct ksj_gpu_plan2 {
   ksj::gpu::Plan2* impl;
+};
+
+struct ksj_gpu_server {
+  ksj::gpu::Server* impl;
 };
 
 namespace {
@@ -252,6 +257,119 @@ int ksj_gpu_adjoint2_execute_batch(const ksj_gpu_plan2* plan, uint32_t count, co
     ksj::gpu::adjoint2_execute(*plan->impl, scans.data(), count, channels, layout, d_workspace, workspace_bytes,
                                as_stream(stream));
   });
+}
+
+int ksj_gpu_serve_create(const ksj_gpu_serve_config* config, ksj_gpu_server** out_server) {
+  return guarded([&] {
+    ksj::gpu::require(config && out_server, "null argument");
+    ksj::gpu::require(config->struct_size >= sizeof(ksj_gpu_serve_config), "config struct too small");
+    ksj::gpu::ServeConfig c;
+    c.device = config->device;
+    c.hbm_budget_bytes = config->hbm_budget_bytes;
+    c.lanes = config->lanes ? config->lanes : 2;
+    c.max_batch_scans = config->max_batch_scans ? config->max_batch_scans : 8;
+    c.plan_cache_entries = config->plan_cache_entries ? config->plan_cache_entries : 16;
+    auto* h = new ksj_gpu_server{nullptr};
+    try {
+      h->impl = new ksj::gpu::Server(c);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out_server = h;
+  });
+}
+
+}  // extern "C"
+
+namespace {
+ksj::gpu::ServeRequest to_request(const ksj_gpu_serve_request* request) {
+  ksj::gpu::require(request->struct_size >= sizeof(ksj_gpu_serve_request), "request struct too small");
+  ksj::gpu::ServeRequest r;
+  r.kind = request->kind;
+  r.image_rows = request->image_rows;
+  r.image_cols = request->image_cols;
+  r.channels = request->channels;
+  r.samples = request->samples;
+  r.kernel_width = request->kernel_width;
+  r.density = request->density;
+  r.centred = request->centred;
+  r.trajectory_key = request->trajectory_key;
+  r.d_traj = request->d_traj;
+  r.d_kspace = static_cast<const float2*>(request->d_kspace);
+  r.d_images = static_cast<float2*>(request->d_images);
+  r.d_rss = request->d_rss;
+  return r;
+}
+}  // namespace
+
+extern "C" {
+
+int ksj_gpu_serve_submit(ksj_gpu_server* server, const ksj_gpu_serve_request* request, uint64_t* ticket) {
+  return guarded([&] {
+    ksj::gpu::require(server && server->impl && request && ticket, "null argument");
+    *ticket = server->impl->submit(to_request(request));
+  });
+}
+
+int ksj_gpu_serve_submit_batch(ksj_gpu_server* server, uint32_t count, const ksj_gpu_serve_request* requests,
+                               uint64_t* tickets) {
+  return guarded([&] {
+    ksj::gpu::require(server && server->impl && requests && tickets && count >= 1, "null argument");
+    std::vector<ksj::gpu::ServeRequest> rs;
+    rs.reserve(count);
+    for (uint32_t i = 0; i < count; ++i) rs.push_back(to_request(&requests[i]));
+    server->impl->submit_many(rs.data(), count, tickets);
+  });
+}
+
+int ksj_gpu_serve_wait(ksj_gpu_server* server, uint64_t ticket, double* latency_ms) {
+  int status = KSJ_GPU_OK;
+  const int rc = guarded([&] {
+    ksj::gpu::require(server && server->impl, "null argument");
+    std::string error;
+    status = server->impl->wait(ticket, latency_ms, &error);
+    if (status != KSJ_GPU_OK) g_last_error = error;
+  });
+  return rc != KSJ_GPU_OK ? rc : status;
+}
+
+int ksj_gpu_serve_drain(ksj_gpu_server* server) {
+  return guarded([&] {
+    ksj::gpu::require(server && server->impl, "null argument");
+    server->impl->drain();
+  });
+}
+
+int ksj_gpu_serve_hold(ksj_gpu_server* server, int on) {
+  return guarded([&] {
+    ksj::gpu::require(server && server->impl, "null argument");
+    server->impl->hold(on != 0);
+  });
+}
+
+int ksj_gpu_serve_get_stats(ksj_gpu_server* server, ksj_gpu_serve_stats* out) {
+  return guarded([&] {
+    ksj::gpu::require(server && server->impl && out, "null argument");
+    ksj::gpu::require(out->struct_size >= sizeof(ksj_gpu_serve_stats), "stats struct too small");
+    const ksj::gpu::ServeStats s = server->impl->stats();
+    out->submitted = s.submitted;
+    out->completed = s.completed;
+    out->failed = s.failed;
+    out->batches = s.batches;
+    out->plan_builds = s.plan_builds;
+    out->plan_hits = s.plan_hits;
+    out->plan_evictions = s.plan_evictions;
+    out->peak_device_bytes = s.peak_workspace_bytes;
+    out->plan_bytes = s.plan_bytes;
+    out->max_batch_seen = s.max_batch_seen;
+  });
+}
+
+void ksj_gpu_serve_destroy(ksj_gpu_server* server) {
+  if (!server) return;
+  delete server->impl;
+  delete server;
 }
 
 int ksj_gpu_tune_set(const char* key, int64_t value) {

@@ -51,6 +51,7 @@ enum TuneKey : int {
   kTuneItemVisits,        // default gather work-item size of new plans (visits; summation order changes)
   kTuneFftLinesPerBlock,  // 0: per-length default; else lines per LDS FFT block (power of two)
   kTuneGatherBatch,       // gather visits per batch (sample-row loads in flight per lane): 4, 8 or 16
+  kTunePassLanes,         // streams over which the passes of a batched adjoint overlap: 1..3
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);

@@ -1,6 +1,10 @@
 // Pipeline orchestration (host side; every step is enqueued on one stream).
 #include "recon.hpp"
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "combine.hpp"
 #include "fft.hpp"
 #include "grid.hpp"
@@ -78,7 +82,50 @@ uint32_t adjoint2_scans_per_pass(uint32_t channels) {
   return channels >= kMaxStackedCoils ? 1u : kMaxStackedCoils / channels;
 }
 
+namespace {
+
+// Streams over which the passes of one batched call overlap (pass p runs on lane
+// p % lanes), kept per (device, caller stream) and created at first use.  They are
+// ordinary default-priority streams, joined to the caller's stream with events.
+struct Lanes {
+  hipStream_t stream[3] = {};
+  hipEvent_t join[3] = {};
+  hipEvent_t fork = nullptr;
+};
+
+Lanes& lanes_for(hipStream_t caller) {
+  static std::mutex mu;
+  static std::map<std::pair<int, hipStream_t>, Lanes> pool;
+  int dev = 0;
+  KSJ_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  Lanes& l = pool[{dev, caller}];
+  if (l.fork == nullptr) {
+    for (int i = 0; i < 3; ++i) {
+      KSJ_HIP(hipStreamCreateWithFlags(&l.stream[i], hipStreamNonBlocking));
+      KSJ_HIP(hipEventCreateWithFlags(&l.join[i], hipEventDisableTiming));
+    }
+    KSJ_HIP(hipEventCreateWithFlags(&l.fork, hipEventDisableTiming));
+  }
+  return l;
+}
+
+uint32_t pass_count(uint32_t channels, uint32_t scans) {
+  const uint32_t per = adjoint2_scans_per_pass(channels);
+  return (std::max(scans, 1u) + per - 1) / per;
+}
+
+uint32_t lane_count(uint32_t channels, uint32_t scans) {
+  return static_cast<uint32_t>(std::min<int64_t>(tune_get(kTunePassLanes), pass_count(channels, scans)));
+}
+
+}  // namespace
+
 size_t adjoint2_workspace_bytes(const Plan2& plan, uint32_t channels, uint32_t scans) {
+  return adjoint2_pass_workspace_bytes(plan, channels, scans) * lane_count(channels, scans);
+}
+
+size_t adjoint2_pass_workspace_bytes(const Plan2& plan, uint32_t channels, uint32_t scans) {
   const auto& d = plan.d;
   const uint32_t per = std::min(std::max(scans, 1u), adjoint2_scans_per_pass(channels));
   const uint32_t total = per * channels;
@@ -101,11 +148,21 @@ void adjoint2_execute(const Plan2& plan, const AdjointScan* scans, uint32_t coun
   require(ws_bytes >= adjoint2_workspace_bytes(plan, channels, count), "adjoint workspace too small");
   const uint32_t per = adjoint2_scans_per_pass(channels);
   const uint64_t pixels = static_cast<uint64_t>(d.image_rows) * d.image_cols;
-  for (uint32_t first = 0; first < count; first += per) {
+  const uint32_t nlanes = lane_count(channels, count);
+  const size_t lane_ws = adjoint2_pass_workspace_bytes(plan, channels, count);
+  Lanes* lanes = nullptr;
+  const hipStream_t caller = stream;
+  if (nlanes > 1) {
+    lanes = &lanes_for(caller);
+    KSJ_HIP(hipEventRecord(lanes->fork, caller));
+    for (uint32_t i = 0; i < nlanes; ++i) KSJ_HIP(hipStreamWaitEvent(lanes->stream[i], lanes->fork, 0));
+  }
+  for (uint32_t first = 0, pass = 0; first < count; first += per, ++pass) {
     const uint32_t n = std::min(per, count - first);
     const uint32_t total = n * channels;
     const size_t cp = padded_coils(total);
-    char* p = static_cast<char*>(d_ws);
+    if (lanes) stream = lanes->stream[pass % nlanes];
+    char* p = static_cast<char*>(d_ws) + (pass % nlanes) * lane_ws;
     auto* sorted = reinterpret_cast<float2*>(p);
     p += align256(static_cast<size_t>(d.samples) * cp * sizeof(float2));
     auto* grid = reinterpret_cast<float2*>(p);
@@ -197,6 +254,12 @@ void adjoint2_execute(const Plan2& plan, const AdjointScan* scans, uint32_t coun
                                stream));
     }
     if (any) coil_major_split(b.out, out, pixels, stream);
+  }
+  if (lanes) {
+    for (uint32_t i = 0; i < nlanes; ++i) {
+      KSJ_HIP(hipEventRecord(lanes->join[i], lanes->stream[i]));
+      KSJ_HIP(hipStreamWaitEvent(caller, lanes->join[i], 0));
+    }
   }
 }
 

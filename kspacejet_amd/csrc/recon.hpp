@@ -25,7 +25,9 @@ struct AdjointScan {
 
 // Scans on one plan are stacked along the coil axis, up to 64 coils per pass.
 uint32_t adjoint2_scans_per_pass(uint32_t channels);
+// Workspace of a whole call: one pass workspace per overlapping pass lane.
 size_t adjoint2_workspace_bytes(const Plan2& plan, uint32_t channels, uint32_t scans = 1);
+size_t adjoint2_pass_workspace_bytes(const Plan2& plan, uint32_t channels, uint32_t scans = 1);
 void adjoint2_execute(const Plan2& plan, const AdjointScan* scans, uint32_t count, uint32_t channels, int layout,
                       void* d_ws, size_t ws_bytes, hipStream_t stream);
 

@@ -1,0 +1,379 @@
+#include "serve.hpp"
+
+#include "combine.hpp"
+#include "grid.hpp"
+#include "recon.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace ksj::gpu {
+
+namespace {
+
+constexpr int kHashThreads = 1024;
+
+// FNV-1a over 32-bit words: each thread hashes one contiguous chunk, thread 0
+// folds the chunk hashes in order, so the value is fixed for given bytes.
+__global__ __launch_bounds__(kHashThreads) void hash_kernel(const uint32_t* words, uint64_t n, uint64_t* out) {
+  __shared__ uint64_t part[kHashThreads];
+  const uint64_t per = (n + kHashThreads - 1) / kHashThreads;
+  const uint64_t b = threadIdx.x * per, e = b + per < n ? b + per : n;
+  uint64_t h = 1469598103934665603ull;
+  for (uint64_t i = b; i < e; ++i) h = (h ^ words[i]) * 1099511628211ull;
+  part[threadIdx.x] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t f = 1469598103934665603ull ^ n;
+    for (int t = 0; t < kHashThreads; ++t) f = (f ^ part[t]) * 1099511628211ull;
+    *out = f;
+  }
+}
+
+uint64_t mix(uint64_t h, uint64_t v) {
+  h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+  return h;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+}
+
+}  // namespace
+
+uint64_t trajectory_hash(const float* d_traj, uint32_t samples, hipStream_t stream) {
+  uint64_t* d_out = nullptr;
+  uint64_t h = 0;
+  KSJ_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_out), sizeof(uint64_t), stream));
+  hipLaunchKernelGGL(hash_kernel, dim3(1), dim3(kHashThreads), 0, stream, reinterpret_cast<const uint32_t*>(d_traj),
+                     static_cast<uint64_t>(samples) * 2, d_out);
+  KSJ_LAUNCH_CHECK();
+  KSJ_HIP(hipMemcpyAsync(&h, d_out, sizeof(h), hipMemcpyDeviceToHost, stream));
+  KSJ_HIP(hipFreeAsync(d_out, stream));
+  KSJ_HIP(hipStreamSynchronize(stream));
+  return h;
+}
+
+uint64_t Server::key_of(const ServeRequest& r) {
+  uint64_t h = mix(0, r.kind);
+  for (uint64_t v : {uint64_t{r.image_rows}, uint64_t{r.image_cols}, uint64_t{r.samples}, uint64_t{r.kernel_width},
+                     uint64_t{r.density}, uint64_t{r.centred}, r.trajectory_key})
+    h = mix(h, v);
+  return h;
+}
+
+Server::Server(const ServeConfig& cfg) : cfg_(cfg) {
+  require(cfg_.lanes >= 1 && cfg_.lanes <= 8, "server lanes must be in [1, 8]");
+  require(cfg_.max_batch_scans >= 1 && cfg_.max_batch_scans <= 64, "max batch scans must be in [1, 64]");
+  require(cfg_.plan_cache_entries >= 1, "the plan cache needs at least one entry");
+  KSJ_HIP(hipSetDevice(cfg_.device));
+  if (cfg_.hbm_budget_bytes == 0) {
+    size_t free_b = 0, total_b = 0;
+    KSJ_HIP(hipMemGetInfo(&free_b, &total_b));
+    cfg_.hbm_budget_bytes = free_b / 5 * 4;
+  }
+  lanes_.resize(cfg_.lanes);
+  for (Lane& l : lanes_) {
+    KSJ_HIP(hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking));
+    KSJ_HIP(hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
+  }
+  thread_ = std::thread([this] { run(); });
+}
+
+Server::~Server() {
+  {
+    std::lock_guard<std::mutex> lock(mu_);
+    stop_ = true;
+  }
+  cv_work_.notify_all();
+  if (thread_.joinable()) thread_.join();
+  (void)hipSetDevice(cfg_.device);
+  for (Lane& l : lanes_) {
+    if (l.stream) (void)hipStreamSynchronize(l.stream);
+    if (l.ws) (void)hipFree(l.ws);
+    if (l.done) (void)hipEventDestroy(l.done);
+    if (l.stream) (void)hipStreamDestroy(l.stream);
+  }
+  plans_.clear();
+}
+
+Server::Pending Server::prepare(const ServeRequest& r) {
+  require(r.kind == kServeRadial2D || r.kind == kServeCartesian2D, "unknown request kind");
+  require(r.channels >= 1 && r.channels <= 64, "request channels must be in [1, 64]");
+  require(r.image_rows >= 2 && r.image_cols >= 2 && r.image_rows <= 4096 && r.image_cols <= 4096,
+          "request image dimensions out of range");
+  require(r.d_kspace != nullptr && (r.d_images != nullptr || r.d_rss != nullptr), "request needs k-space and an output");
+  Pending p{};
+  p.req = r;
+  if (r.kind == kServeRadial2D) {
+    require(r.samples >= 1 && r.d_traj != nullptr, "radial request needs a trajectory");
+    require(r.kernel_width == 4 || r.kernel_width == 6 || r.kernel_width == 8, "kernel width must be 4, 6 or 8");
+    require(r.density == kDensityNone || r.density == kDensityRamp, "density must be none or ramp");
+    if (p.req.trajectory_key == 0) {
+      KSJ_HIP(hipSetDevice(cfg_.device));
+      hipStream_t s = nullptr;
+      KSJ_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      p.req.trajectory_key = trajectory_hash(r.d_traj, r.samples, s) | 1ull;
+      KSJ_HIP(hipStreamDestroy(s));
+    }
+  } else {
+    p.req.samples = 0;
+    p.req.trajectory_key = 0;
+  }
+  p.plan_key = key_of(p.req);
+  return p;
+}
+
+uint64_t Server::submit(const ServeRequest& r) {
+  uint64_t t = 0;
+  submit_many(&r, 1, &t);
+  return t;
+}
+
+void Server::submit_many(const ServeRequest* r, uint32_t count, uint64_t* tickets) {
+  std::vector<Pending> ps;
+  ps.reserve(count);
+  for (uint32_t i = 0; i < count; ++i) ps.push_back(prepare(r[i]));  // all validated before any is queued
+  {
+    std::lock_guard<std::mutex> lock(mu_);
+    const auto now = std::chrono::steady_clock::now();
+    for (uint32_t i = 0; i < count; ++i) {
+      ps[i].ticket = next_ticket_++;
+      ps[i].submitted = now;
+      tickets[i] = ps[i].ticket;
+      pending_.push_back(ps[i]);
+    }
+    stats_.submitted += count;
+    inflight_ += count;
+  }
+  cv_work_.notify_one();
+}
+
+int Server::wait(uint64_t ticket, double* latency_ms, std::string* error) {
+  std::unique_lock<std::mutex> lock(mu_);
+  require(ticket > 0 && ticket < next_ticket_, "unknown ticket");
+  cv_done_.wait(lock, [&] { return done_.count(ticket) != 0; });
+  const Done d = done_[ticket];
+  done_.erase(ticket);
+  if (latency_ms) *latency_ms = d.latency_ms;
+  if (error) *error = d.error;
+  return d.status;
+}
+
+void Server::drain() {
+  std::unique_lock<std::mutex> lock(mu_);
+  cv_done_.wait(lock, [&] { return inflight_ == 0; });
+}
+
+void Server::hold(bool on) {
+  {
+    std::lock_guard<std::mutex> lock(mu_);
+    hold_ = on;
+  }
+  cv_work_.notify_all();
+}
+
+ServeStats Server::stats() {
+  std::lock_guard<std::mutex> lock(mu_);
+  ServeStats s = stats_;
+  s.plan_bytes = 0;
+  for (auto& kv : plans_) s.plan_bytes += kv.second.bytes;
+  return s;
+}
+
+uint64_t Server::used_bytes() const {
+  uint64_t b = 0;
+  for (const Lane& l : lanes_) b += l.ws_bytes;
+  for (const auto& kv : plans_) b += kv.second.bytes;
+  return b;
+}
+
+bool Server::evict_one() {
+  // The least recently used plan that no batch in flight uses.
+  for (auto it = lru_.rbegin(); it != lru_.rend(); ++it) {
+    auto pe = plans_.find(*it);
+    if (pe != plans_.end() && pe->second.users == 0) {
+      plans_.erase(pe);  // the entry's deleter frees the plan's device tables
+      lru_.erase(std::next(it).base());
+      ++stats_.plan_evictions;
+      return true;
+    }
+  }
+  return false;
+}
+
+void Server::evict_until(uint64_t need_bytes) {
+  while (used_bytes() + need_bytes > cfg_.hbm_budget_bytes && evict_one()) {
+  }
+}
+
+Plan2* Server::plan_for(const Pending& p, hipStream_t stream) {
+  auto it = plans_.find(p.plan_key);
+  if (it != plans_.end()) {
+    ++stats_.plan_hits;
+    lru_.remove(p.plan_key);
+    lru_.push_front(p.plan_key);
+    return it->second.plan.get();
+  }
+  while (plans_.size() >= cfg_.plan_cache_entries && evict_one()) {
+  }
+  const ServeRequest& r = p.req;
+  Plan2Desc d;
+  d.image_rows = r.image_rows;
+  d.image_cols = r.image_cols;
+  d.grid_rows = 2 * r.image_rows;
+  d.grid_cols = 2 * r.image_cols;
+  d.width = r.kernel_width;
+  d.kernel = kKernelKaiserBessel;
+  d.row_origin = (r.image_rows - 1) / 2.0;
+  d.col_origin = (r.image_cols - 1) / 2.0;
+  d.density = r.density;
+  d.centred = 1;
+  d.samples = r.samples;
+  d.item_visits = static_cast<uint32_t>(tune_get(kTuneItemVisits));
+  PlanPtr plan(plan2_create(d), &plan2_destroy);
+  plan2_set_trajectory(plan.get(), r.d_traj, nullptr, stream);
+  PlanEntry e;
+  e.bytes = plan->device_bytes;
+  e.plan = std::move(plan);
+  Plan2* raw = e.plan.get();
+  plans_.emplace(p.plan_key, std::move(e));
+  lru_.push_front(p.plan_key);
+  ++stats_.plan_builds;
+  return raw;
+}
+
+// Takes the oldest pending request and every later one that can share its launch,
+// sizes the workspace against the HBM budget, and enqueues the batch on `lane`.
+bool Server::launch_next(Lane& lane) {
+  if (pending_.empty()) return false;
+  const Pending head = pending_.front();
+  const ServeRequest& r = head.req;
+  std::vector<Pending> batch;
+  const uint32_t limit = r.kind == kServeRadial2D ? cfg_.max_batch_scans : cfg_.max_batch_scans;
+  for (auto it = pending_.begin(); it != pending_.end() && batch.size() < limit;) {
+    const ServeRequest& q = it->req;
+    const bool same = it->plan_key == head.plan_key && q.channels == r.channels;
+    if (same) {
+      batch.push_back(*it);
+      it = pending_.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  KSJ_HIP(hipSetDevice(cfg_.device));
+  size_t need = 0;
+  Plan2* plan = nullptr;
+  try {
+    if (r.kind == kServeRadial2D) {
+      plan = plan_for(head, lane.stream);
+      plans_[head.plan_key].users++;  // pinned against eviction from here on; released by complete()
+      lane.plan_key = head.plan_key;
+      need = adjoint2_workspace_bytes(*plan, r.channels, static_cast<uint32_t>(batch.size()));
+    } else {
+      need = ifft2_workspace_bytes(r.channels, r.image_rows, r.image_cols) +
+             256 + static_cast<size_t>(r.image_rows) * r.image_cols * r.channels * sizeof(float2);
+    }
+    if (need > lane.ws_bytes) {
+      // Admission against the budget: drop idle plans first, then give up on this lane's growth.
+      evict_until(need - lane.ws_bytes);
+      require(used_bytes() - lane.ws_bytes + need <= cfg_.hbm_budget_bytes,
+              "request does not fit in the HBM budget of the server");  // status 4 (resource exhausted)
+      if (lane.ws) {
+        KSJ_HIP(hipStreamSynchronize(lane.stream));
+        KSJ_HIP(hipFree(lane.ws));
+        lane.ws = nullptr;
+        lane.ws_bytes = 0;
+      }
+      KSJ_HIP(hipMalloc(&lane.ws, need));
+      lane.ws_bytes = need;
+      stats_.peak_workspace_bytes = std::max<uint64_t>(stats_.peak_workspace_bytes, used_bytes());
+    }
+    if (r.kind == kServeRadial2D) {
+      std::vector<AdjointScan> scans(batch.size());
+      for (size_t i = 0; i < batch.size(); ++i)
+        scans[i] = AdjointScan{batch[i].req.d_kspace, batch[i].req.d_images, batch[i].req.d_rss};
+      adjoint2_execute(*plan, scans.data(), static_cast<uint32_t>(scans.size()), r.channels, kLayoutCoilLast, lane.ws,
+                       lane.ws_bytes, lane.stream);
+    } else {
+      const size_t fft_ws = ifft2_workspace_bytes(r.channels, r.image_rows, r.image_cols);
+      auto* coils = reinterpret_cast<float2*>(static_cast<char*>(lane.ws) + ((fft_ws + 255) / 256) * 256);
+      for (const Pending& p : batch) {
+        float2* images = p.req.d_images ? p.req.d_images : coils;
+        ifft2_coil_images(p.req.d_kspace, images, r.channels, r.image_rows, r.image_cols, p.req.centred != 0,
+                          kLayoutCoilLast, lane.ws, fft_ws, lane.stream);
+        if (p.req.d_rss)
+          rss_coil_last(images, p.req.d_rss, static_cast<uint64_t>(r.image_rows) * r.image_cols, r.channels,
+                        lane.stream);
+      }
+    }
+    KSJ_HIP(hipEventRecord(lane.done, lane.stream));
+  } catch (const Error& e) {
+    lane.batch = std::move(batch);
+    complete(lane, std::strstr(e.what(), "HBM budget") ? 4 : (e.code == 3 ? 3 : 1), e.what());
+    return true;
+  } catch (const std::exception& e) {
+    lane.batch = std::move(batch);
+    complete(lane, 3, e.what());
+    return true;
+  }
+  lane.batch = std::move(batch);
+  lane.busy = true;
+  ++stats_.batches;
+  stats_.max_batch_seen = std::max<uint64_t>(stats_.max_batch_seen, lane.batch.size());
+  return true;
+}
+
+void Server::complete(Lane& lane, int status, const std::string& error) {
+  for (const Pending& p : lane.batch) {
+    done_[p.ticket] = Done{status, ms_since(p.submitted), error};
+    if (status == 0)
+      ++stats_.completed;
+    else
+      ++stats_.failed;
+    --inflight_;
+  }
+  if (lane.plan_key) {
+    auto it = plans_.find(lane.plan_key);
+    if (it != plans_.end() && it->second.users > 0) it->second.users--;
+  }
+  lane.batch.clear();
+  lane.plan_key = 0;
+  lane.busy = false;
+  cv_done_.notify_all();
+}
+
+void Server::run() {
+  (void)hipSetDevice(cfg_.device);
+  std::unique_lock<std::mutex> lock(mu_);
+  while (true) {
+    bool busy = false;
+    for (Lane& l : lanes_) {
+      if (l.busy) {
+        const hipError_t q = hipEventQuery(l.done);
+        if (q == hipSuccess) {
+          complete(l, 0, std::string());
+        } else if (q != hipErrorNotReady) {
+          complete(l, 3, std::string("device error: ") + hipGetErrorString(q));
+        } else {
+          busy = true;
+        }
+      }
+    }
+    bool launched = false;
+    for (Lane& l : lanes_)
+      if (!l.busy && !pending_.empty() && !hold_) launched = launch_next(l) || launched;
+    if (stop_ && (pending_.empty() || hold_) && !busy && !launched) break;
+    if (launched) continue;
+    if (busy) {
+      // Work in flight: poll its completion events at a fine grain.
+      lock.unlock();
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+      lock.lock();
+    } else {
+      cv_work_.wait(lock, [&] { return stop_ || (!pending_.empty() && !hold_); });
+    }
+  }
+}
+
+}  // namespace ksj::gpu

@@ -1,0 +1,143 @@
+// Continuous-batching reconstruction server on one GPU.
+//
+// Requests (2-D radial KB adjoint or 2-D Cartesian inverse FFT, inputs and outputs
+// in HBM) are queued by submit() and run by one scheduler thread:
+//  * admission: a request's batch is launched only when a lane (stream) is free and
+//    the HBM budget covers its workspace next to the resident plans;
+//  * plans (FP64 coordinates, bin sort, visit schedule, KB and deapodization tables)
+//    stay resident per (geometry, trajectory key), least recently used evicted first;
+//  * batching: pending radial requests with the same plan are stacked along the coil
+//    axis into one launch (up to max_batch_scans), Cartesian requests of one shape
+//    share a launch; requests arriving while batches run join the next free lane;
+//  * lanes: up to `lanes` batches are in flight on separate default-priority streams,
+//    so their kernels overlap on the device.
+// Every request's result equals its result when run alone (each coil lane's
+// arithmetic is independent of what it is stacked with).
+#pragma once
+
+#include "plan.hpp"
+
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <list>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace ksj::gpu {
+
+enum ServeKind : uint32_t { kServeRadial2D = 0, kServeCartesian2D = 1 };
+
+struct ServeConfig {
+  int device = 0;
+  uint64_t hbm_budget_bytes = 0;  // 0: 80% of the device's free memory at creation
+  uint32_t lanes = 2;             // batches in flight at once (streams)
+  uint32_t max_batch_scans = 8;   // scans stacked into one radial launch
+  uint32_t plan_cache_entries = 16;
+};
+
+struct ServeRequest {
+  uint32_t kind = kServeRadial2D;
+  uint32_t image_rows = 0, image_cols = 0, channels = 0, samples = 0;
+  uint32_t kernel_width = 4;
+  uint32_t density = 1;  // KSJ_GPU_DENSITY_NONE / _RAMP
+  uint32_t centred = 0;  // Cartesian only
+  uint64_t trajectory_key = 0;  // radial: identifies the trajectory bytes; 0 = hash them
+  const float* d_traj = nullptr;
+  const float2* d_kspace = nullptr;
+  float2* d_images = nullptr;  // coil-last, may be null
+  float* d_rss = nullptr;      // may be null
+};
+
+struct ServeStats {
+  uint64_t submitted = 0, completed = 0, failed = 0;
+  uint64_t batches = 0, plan_builds = 0, plan_hits = 0, plan_evictions = 0;
+  uint64_t peak_workspace_bytes = 0, plan_bytes = 0;
+  uint64_t max_batch_seen = 0;
+};
+
+class Server {
+ public:
+  explicit Server(const ServeConfig& cfg);
+  ~Server();
+  Server(const Server&) = delete;
+  Server& operator=(const Server&) = delete;
+
+  // Queues a request; returns its ticket (> 0).  Throws Error on an invalid request.
+  uint64_t submit(const ServeRequest& r);
+  // Queues several requests atomically (the scheduler sees all of them at once).
+  void submit_many(const ServeRequest* r, uint32_t count, uint64_t* tickets);
+  // Blocks until the ticket completes; returns 0 or an error code (ksj_gpu status)
+  // and the submit-to-completion latency in ms.  A ticket can be waited on once.
+  int wait(uint64_t ticket, double* latency_ms, std::string* error);
+  // Blocks until every submitted request has completed.
+  void drain();
+  // While held, nothing new is launched (requests queue up and are batched on release).
+  void hold(bool on);
+  ServeStats stats();
+
+ private:
+  struct Pending {
+    uint64_t ticket;
+    ServeRequest req;
+    uint64_t plan_key;
+    std::chrono::steady_clock::time_point submitted;
+  };
+  struct Done {
+    int status;
+    double latency_ms;
+    std::string error;
+  };
+  using PlanPtr = std::unique_ptr<Plan2, void (*)(Plan2*)>;
+  struct PlanEntry {
+    PlanPtr plan{nullptr, &plan2_destroy};
+    uint64_t bytes = 0;
+    uint32_t users = 0;  // batches in flight that use it
+  };
+  struct Lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    bool busy = false;
+    std::vector<Pending> batch;
+    uint64_t plan_key = 0;
+  };
+
+  Pending prepare(const ServeRequest& r);  // validates and keys a request (hashes a trajectory if needed)
+  void run();
+  bool launch_next(Lane& lane);  // under mu_; returns false when nothing could be launched
+  void complete(Lane& lane, int status, const std::string& error);
+  Plan2* plan_for(const Pending& p, hipStream_t stream);
+  bool evict_one();
+  void evict_until(uint64_t need_bytes);
+  uint64_t used_bytes() const;
+  static uint64_t key_of(const ServeRequest& r);
+
+  ServeConfig cfg_;
+  std::mutex mu_;
+  std::condition_variable cv_work_, cv_done_;
+  std::deque<Pending> pending_;
+  std::unordered_map<uint64_t, Done> done_;
+  std::vector<Lane> lanes_;
+  std::map<uint64_t, PlanEntry> plans_;
+  std::list<uint64_t> lru_;  // front = most recently used
+  ServeStats stats_;
+  uint64_t next_ticket_ = 1;
+  uint64_t inflight_ = 0;
+  bool stop_ = false;
+  bool hold_ = false;
+  std::thread thread_;
+};
+
+// 64-bit hash of a device trajectory (FNV-1a over its bytes, computed on the device
+// and read back); synchronises `stream`.
+uint64_t trajectory_hash(const float* d_traj, uint32_t samples, hipStream_t stream);
+
+}  // namespace ksj::gpu

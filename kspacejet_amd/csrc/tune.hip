@@ -21,6 +21,7 @@ Param g_params[kTuneKeys] = {
     {"item_visits", 64, [](int64_t v) { return v >= 1 && v <= 4096; }},
     {"fft_lines_per_block", 0, &is_pow2_or_zero},
     {"gather_batch", 8, [](int64_t v) { return v == 4 || v == 8 || v == 16; }},
+    {"pass_lanes", 2, [](int64_t v) { return v >= 1 && v <= 3; }},
 };
 
 }  // namespace

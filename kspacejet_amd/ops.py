@@ -229,3 +229,77 @@ class Plan2:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Continuous-batching server (ksj_gpu_serve_*): submit device-resident scans,
+    wait for tickets.  Buffers passed to submit() must stay alive until their
+    ticket completes; the server keeps references to them until then."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, lanes: int = 2, max_batch_scans: int = 8,
+                 plan_cache_entries: int = 16):
+        self._lib = load()
+        cfg = _lib.ServeConfig(C.sizeof(_lib.ServeConfig), device, hbm_budget_bytes, lanes, max_batch_scans,
+                               plan_cache_entries, 0)
+        self._handle = C.c_void_p()
+        call("ksj_gpu_serve_create", C.byref(cfg), C.byref(self._handle))
+        self._live = {}
+
+    def submit_radial(self, kspace: torch.Tensor, traj: torch.Tensor, n: int, images: torch.Tensor | None = None,
+                      rss: torch.Tensor | None = None, width: int = 4, density: str = "ramp",
+                      trajectory_key: int = 0) -> int:
+        _need(kspace, torch.complex64, "kspace")
+        _need(traj, torch.float32, "traj")
+        ch, S = kspace.shape
+        r = _lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_RADIAL2D, n, n, ch, S, width,
+                              _lib.DENSITY_RAMP if density == "ramp" else _lib.DENSITY_NONE, 0, 0, trajectory_key,
+                              _ptr(traj), _ptr(kspace), _ptr(images) if images is not None else None,
+                              _ptr(rss) if rss is not None else None)
+        return self._submit(r, (kspace, traj, images, rss))
+
+    def submit_cartesian(self, kspace: torch.Tensor, images: torch.Tensor | None = None,
+                         rss: torch.Tensor | None = None, centred: bool = False) -> int:
+        _need(kspace, torch.complex64, "kspace")
+        ch, rows, cols = kspace.shape
+        r = _lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_CARTESIAN2D, rows, cols, ch, 0, 0, 0,
+                              int(centred), 0, 0, None, _ptr(kspace), _ptr(images) if images is not None else None,
+                              _ptr(rss) if rss is not None else None)
+        return self._submit(r, (kspace, images, rss))
+
+    def _submit(self, req, keep) -> int:
+        # Work the caller queued on torch's stream (e.g. filling k-space) must land first.
+        torch.cuda.current_stream().synchronize()
+        t = C.c_uint64(0)
+        call("ksj_gpu_serve_submit", self._handle, C.byref(req), C.byref(t))
+        self._live[t.value] = keep
+        return t.value
+
+    def wait(self, ticket: int) -> float:
+        """Blocks until the ticket completes; returns its latency in ms (raises on a failed request)."""
+        lat = C.c_double(0)
+        rc = self._lib.ksj_gpu_serve_wait(self._handle, ticket, C.byref(lat))
+        self._live.pop(ticket, None)
+        _lib.check(rc)
+        return float(lat.value)
+
+    def drain(self) -> None:
+        call("ksj_gpu_serve_drain", self._handle)
+
+    def hold(self, on: bool) -> None:
+        """While held the server launches nothing new; queued requests batch on release."""
+        call("ksj_gpu_serve_hold", self._handle, int(bool(on)))
+
+    def stats(self) -> dict:
+        s = _lib.ServeStats()
+        s.struct_size = C.sizeof(_lib.ServeStats)
+        call("ksj_gpu_serve_get_stats", self._handle, C.byref(s))
+        return {name: int(getattr(s, name)) for name, _ in _lib.ServeStats._fields_[2:]}
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None) and self._handle.value:
+            self._lib.ksj_gpu_serve_destroy(self._handle)
+            self._handle = C.c_void_p()
+            self._live.clear()
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,143 @@
+"""The continuous-batching server: every request's result is the same alone and in a
+mixed stream (bit for bit), plans are built once per trajectory and reused, the HBM
+budget turns an oversized request into an error without stopping the stream, and
+served results keep parity with the FP64 oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+from kspacejet_amd import synth
+from tests.helpers import rel_l2
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ops():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from kspacejet_amd import ops as K
+    K.init(0)
+    return K
+
+
+def radial(ops, n, coils, spokes, readout, seed):
+    traj = ops.golden_angle2(spokes, readout)
+    img = torch.from_numpy(synth.coil_images2(n, coils, seed)).cuda()
+    k = ops.nudft2_forward_f64(traj, img, (n - 1) / 2, (n - 1) / 2).cpu().numpy()
+    return traj, torch.from_numpy(synth.add_noise(k, seed).astype(np.complex64)).cuda()
+
+
+def cartesian(n, coils, seed, centred):
+    return torch.from_numpy(synth.cartesian_kspace2(n, coils, seed, centred=centred).astype(np.complex64)).cuda()
+
+
+def make_stream(ops):
+    """A mixed stream: two radial trajectories (several scans each, one with coil
+    images and RSS) and Cartesian scans of three shapes, centred and not."""
+    ta, _ = radial(ops, 64, 8, 33, 128, 1)
+    tb, _ = radial(ops, 96, 12, 41, 192, 2)
+    reqs = []
+    for i in range(5):
+        _, k = radial(ops, 64, 8, 33, 128, 10 + i)
+        reqs.append(("radial", dict(kspace=k, traj=ta, n=64), i % 2 == 0))
+    for i in range(3):
+        _, k = radial(ops, 96, 12, 41, 192, 20 + i)
+        reqs.append(("radial", dict(kspace=k, traj=tb, n=96), True))
+    for i, (n, c, centred) in enumerate([(64, 4, False), (96, 8, True), (128, 16, False), (64, 4, True)]):
+        reqs.append(("cartesian", dict(kspace=cartesian(n, c, 30 + i, centred), centred=centred), i % 2 == 1))
+    return reqs
+
+
+def submit(server, kind, args, want_images):
+    k = args["kspace"]
+    if kind == "radial":
+        n = args["n"]
+        img = torch.empty((n, n, k.shape[0]), dtype=torch.complex64, device="cuda") if want_images else None
+        rss = torch.empty((n, n), dtype=torch.float32, device="cuda")
+        t = server.submit_radial(k, args["traj"], n, images=img, rss=rss)
+    else:
+        c, r, cols = k.shape
+        img = torch.empty((r, cols, c), dtype=torch.complex64, device="cuda") if want_images else None
+        rss = torch.empty((r, cols), dtype=torch.float32, device="cuda")
+        t = server.submit_cartesian(k, images=img, rss=rss, centred=args["centred"])
+    return t, img, rss
+
+
+def test_results_are_identical_alone_and_in_the_mix(ops):
+    reqs = make_stream(ops)
+    server = ops.Server(lanes=2, max_batch_scans=8)
+    alone = []
+    for kind, args, want in reqs:
+        t, img, rss = submit(server, kind, args, want)
+        server.wait(t)
+        alone.append((None if img is None else img.cpu().numpy(), rss.cpu().numpy()))
+    order = np.random.default_rng(0).permutation(len(reqs))
+    pending = {}
+    server.hold(True)  # the whole shuffled stream queues up, so same-trajectory scans get stacked
+    for i in order[: len(order) // 2]:
+        kind, args, want = reqs[i]
+        pending[i] = submit(server, kind, args, want)
+    server.hold(False)
+    for i in order[len(order) // 2:]:  # the rest arrives while the first half runs
+        kind, args, want = reqs[i]
+        pending[i] = submit(server, kind, args, want)
+    lat = {i: server.wait(t) for i, (t, _, _) in pending.items()}
+    for i, (t, img, rss) in pending.items():
+        assert np.array_equal(rss.cpu().numpy(), alone[i][1]), i
+        if img is not None:
+            assert np.array_equal(img.cpu().numpy(), alone[i][0]), i
+        assert lat[i] > 0
+    st = server.stats()
+    assert st["completed"] == 2 * len(reqs) and st["failed"] == 0
+    assert st["max_batch_seen"] >= 2  # same-trajectory scans were stacked in the mix
+    server.close()
+
+
+def test_plans_are_built_once_per_trajectory_and_hashing_finds_equal_bytes(ops):
+    traj, k = radial(ops, 64, 4, 21, 128, 3)
+    traj_copy = traj.clone()  # same bytes, different buffer, no key: the server hashes both
+    server = ops.Server(lanes=1)
+    outs = []
+    for tr in (traj, traj_copy, traj, traj_copy):
+        rss = torch.empty((64, 64), dtype=torch.float32, device="cuda")
+        server.wait(server.submit_radial(k, tr, 64, rss=rss))
+        outs.append(rss.cpu().numpy())
+    st = server.stats()
+    assert st["plan_builds"] == 1 and st["plan_hits"] == 3
+    assert all(np.array_equal(o, outs[0]) for o in outs)
+    server.close()
+
+
+def test_served_radial_matches_the_oracle(ops):
+    n, coils = 64, 6
+    traj, k = radial(ops, n, coils, 37, 128, 4)
+    server = ops.Server()
+    img = torch.empty((n, n, coils), dtype=torch.complex64, device="cuda")
+    rss = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    server.wait(server.submit_radial(k, traj, n, images=img, rss=rss, width=6))
+    t = traj.cpu().numpy().astype(np.float64)
+    ref = O.kb_adjoint2_coils(k.cpu().numpy(), t, O.ramp_dcf2(t), n, width=6)
+    assert rel_l2(img.cpu().numpy(), ref) <= 1e-5
+    assert rel_l2(rss.cpu().numpy(), O.rss(ref)) <= 1e-5
+    server.close()
+
+
+def test_budget_rejects_an_oversized_request_and_keeps_serving(ops):
+    from kspacejet_amd._lib import KsjGpuError
+    server = ops.Server(hbm_budget_bytes=8 << 20, lanes=1)  # 8 MiB: a 64x64x4 Cartesian scan fits
+    big = cartesian(512, 16, 5, False)                        # needs ~70 MiB of workspace
+    small = cartesian(64, 4, 6, False)
+    r_big = torch.empty((512, 512), dtype=torch.float32, device="cuda")
+    r_small = torch.empty((64, 64), dtype=torch.float32, device="cuda")
+    t_big = server.submit_cartesian(big, rss=r_big)
+    t_small = server.submit_cartesian(small, rss=r_small)
+    with pytest.raises(KsjGpuError) as e:
+        server.wait(t_big)
+    assert e.value.code == 4
+    server.wait(t_small)
+    assert rel_l2(r_small.cpu().numpy(), O.rss(O.ifft2_coil_images(small.cpu().numpy()))) <= 1e-5
+    st = server.stats()
+    assert st["failed"] == 1 and st["completed"] == 1
+    server.close()

@@ -63,6 +63,24 @@ def test_launch_parameters_do_not_change_results(ops, tune):
         tune(key, values[0])
 
 
+def test_pass_lanes_do_not_change_batched_results(ops, tune):
+    """Passes of a batched call overlap on 1, 2 or 3 streams with identical results,
+    including a batch whose last pass is partial (9 scans of 12 coils: 5 per pass)."""
+    n = 64
+    traj, k = scan(ops, n=n, coils=12, spokes=41, readout=128, seed=9)
+    ks = [torch.from_numpy(np.roll(k, i, axis=1).copy()).cuda() for i in range(9)]
+    plan = ops.Plan2(n, n, traj.shape[0]).set_trajectory(traj)
+    results = []
+    for lanes in (1, 2, 3):
+        tune("pass_lanes", lanes)
+        imgs, rss = plan.adjoint_batch(ks, images=True, rss=True)
+        torch.cuda.synchronize()
+        results.append(([i.cpu().numpy() for i in imgs], [r.cpu().numpy() for r in rss]))
+    for imgs, rss in results[1:]:
+        assert all(np.array_equal(a, b) for a, b in zip(imgs, results[0][0]))
+        assert all(np.array_equal(a, b) for a, b in zip(rss, results[0][1]))
+
+
 @pytest.mark.parametrize("item_visits", [16, 256])
 def test_item_size_keeps_oracle_parity(ops, tune, item_visits):
     n = 64
