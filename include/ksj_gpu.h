@@ -251,12 +251,16 @@ void ksj_gpu_serve_destroy(ksj_gpu_server* server);
 
 /* Process-wide launch parameters chosen by the autotuner (kspacejet_amd/autotune.py).
  * No reference counterpart.  Keys and legal values:
- *   "gather_map"           0 = work items in launch order, 1 = one contiguous item range per XCD
+ *   "gather_map"           0 = work items in launch order, 1 = one contiguous item range per XCD,
+ *                          2 = contiguous ranges of equal cost (visits) per XCD
  *   "item_visits"          1..4096, default gather work-item size of plans created afterwards
  *   "fft_lines_per_block"  0 = per-length default, else 1..64 (power of two) lines per FFT block
  *   "gather_batch"         4, 8 or 16 gather visits per batch (sample-row loads in flight per lane)
  *   "pass_lanes"           1..3 streams over which the passes of a batched adjoint overlap (the
  *                          batch workspace holds one pass workspace per lane)
+ *   "multi_lanes"          1..8 streams over which the items of ksj_gpu_adjoint2_execute_multi spread
+ *   "gather_coils_per_lane" 1 or 2 coils per gather lane (one 8- or 16-byte load per visit) from 32
+ *                          stacked coils up
  * Every setting gives bit-identical results, except item_visits, which sets the
  * float summation order of the gather.  Not thread-safe against concurrent launches. */
 int ksj_gpu_tune_set(const char* key, int64_t value);

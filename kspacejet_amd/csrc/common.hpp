@@ -47,11 +47,13 @@ inline unsigned ceil_div(uint64_t a, uint64_t b) { return static_cast<unsigned>(
 // value has a fixed set of legal settings; every setting gives bit-identical
 // results except where a parameter's comment says otherwise.
 enum TuneKey : int {
-  kTuneGatherMap = 0,     // 0: work items in launch order; 1: one contiguous item range per XCD
+  kTuneGatherMap = 0,     // 0: items in launch order; 1: equal item ranges per XCD; 2: equal-cost ranges
   kTuneItemVisits,        // default gather work-item size of new plans (visits; summation order changes)
   kTuneFftLinesPerBlock,  // 0: per-length default; else lines per LDS FFT block (power of two)
   kTuneGatherBatch,       // gather visits per batch (sample-row loads in flight per lane): 4, 8 or 16
   kTunePassLanes,         // streams over which the passes of a batched adjoint overlap: 1..3
+  kTuneMultiLanes,        // streams over which the items of a multi-plan adjoint are spread: 1..8
+  kTuneGatherCoilsPerLane,  // coils each gather lane loads per visit for 32+ coils: 1 or 2
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);

@@ -38,6 +38,16 @@ KSJ_SHAPE(768, 32, 8, 8, 4, 3)
 KSJ_SHAPE(1024, 64, 8, 8, 4, 4)
 #undef KSJ_SHAPE
 
+// LDS element index of line element i: one pad element per 16 spreads the strided
+// Stockham writes (stride R in the first pass, R * Ns blocks in the next) over the
+// banks instead of mapping a whole wave onto a few.
+__device__ constexpr int pad(int i) { return i + (i >> 4); }
+// LDS stride between lines: padded length plus one (odd) so lines start on different banks.
+template <int N>
+constexpr int line_stride() {
+  return N + N / 16 + 1;
+}
+
 __device__ inline float2 mul_i(float2 z, bool inv) {  // z * (+i) for inverse, z * (-i) for forward
   return inv ? make_float2(-z.y, z.x) : make_float2(z.y, -z.x);
 }
@@ -127,7 +137,7 @@ __device__ inline void passes(float2* line, const float2* tw, int t) {
       const int j = t + b * T;
       const int k = j % Ns;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = line[j + r * (N / R)];
+      for (int r = 0; r < R; ++r) v[b][r] = line[pad(j + r * (N / R))];
       if constexpr (Ns > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
@@ -145,7 +155,7 @@ __device__ inline void passes(float2* line, const float2* tw, int t) {
       const int k = j % Ns;
       const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) line[base + r * Ns] = v[b][r];
+      for (int r = 0; r < R; ++r) line[pad(base + r * Ns)] = v[b][r];
     }
     __syncthreads();
     passes<N, INV, P + 1, Ns * R>(line, tw, t);
@@ -157,24 +167,20 @@ __device__ inline uint64_t line_base(const FftLines& L, uint64_t line) {
   return p * L.plane_stride + q * L.line_stride;
 }
 
-// Per-line base offsets of the block's lines (computed once per line, not per element).
-__device__ inline void stage_bases(const FftArgs& a, uint64_t* src_base, uint64_t* dst_base, uint32_t* line_q,
-                                   int lpb, uint64_t first, uint64_t total) {
-  for (int ly = threadIdx.x; ly < lpb; ly += blockDim.x) {
-    const uint64_t line = first + ly;
-    src_base[ly] = line < total ? line_base(a.src, line) : ~0ull;
-    dst_base[ly] = line < total ? line_base(a.dst, line) : ~0ull;
-    line_q[ly] = static_cast<uint32_t>(line % a.src.lines);
-  }
-}
-
-// Loads `lpb` lines into LDS (line stride ls), rotated by in_offset.  lpb is a power of two.
+// Loads `lpb` lines into LDS (line stride ls), rotated by in_offset.  lpb is a power
+// of two and the block has lpb * T threads, so every thread moves exactly N / T
+// elements: all of its global loads are issued before the first LDS write, so the
+// loads of one thread overlap instead of paying one memory latency each.
 template <int N>
 __device__ inline void load_lines(const FftArgs& a, float2* buf, int ls, int lpb_log2, const uint64_t* src_base) {
+  constexpr int E = N / Shape<N>::T;
   const int lpb = 1 << lpb_log2;
-  const int count = lpb * N;
   const uint32_t es = static_cast<uint32_t>(a.src.elem_stride);
-  for (int idx = threadIdx.x; idx < count; idx += blockDim.x) {
+  float2 v[E];
+  int dst[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int idx = threadIdx.x + i * blockDim.x;
     int ly, e;
     if (a.load_lines_fast) {
       ly = idx & (lpb - 1);
@@ -184,24 +190,34 @@ __device__ inline void load_lines(const FftArgs& a, float2* buf, int ls, int lpb
       e = idx - ly * N;
     }
     const uint64_t base = src_base[ly];
-    float2 v = make_float2(0.f, 0.f);
-    if (base != ~0ull) {
-      uint32_t src = static_cast<uint32_t>(e) + a.in_offset;
-      if (src >= static_cast<uint32_t>(N)) src -= N;
-      v = a.in[base + static_cast<uint64_t>(src * es)];
-    }
-    buf[ly * ls + e] = v;
+    const bool live = base != ~0ull;
+    uint32_t src = static_cast<uint32_t>(e) + a.in_offset;
+    if (src >= static_cast<uint32_t>(N)) src -= N;
+    // Unconditional load, so no branch separates the loads and all of them are in
+    // flight together: a line past the end transforms a copy of line 0, and its
+    // outputs are never stored.
+    v[i] = a.in[(live ? base : 0ull) + static_cast<uint64_t>(src * es)];
+    dst[i] = ly * ls + pad(e);
   }
+#pragma unroll
+  for (int i = 0; i < E; ++i) buf[dst[i]] = v[i];
 }
 
+// Stores the n_out kept outputs of every line, scaled by a.scale x out_weight[e] x
+// line_weight[line]; the weights were staged in LDS (ow, lw), so the stores issue
+// back to back.
 template <int N>
-__device__ inline void store_lines(const FftArgs& a, const float2* buf, int ls, int lpb_log2, const uint32_t* line_q,
-                                   const uint64_t* dst_base) {
+__device__ inline void store_lines(const FftArgs& a, const float2* buf, int ls, int lpb_log2, const float* ow,
+                                   const float* lw, const uint64_t* dst_base) {
+  constexpr int E = N / Shape<N>::T;
   const int lpb = 1 << lpb_log2;
   const int nout = static_cast<int>(a.n_out);
   const int count = lpb * nout;
   const uint32_t es = static_cast<uint32_t>(a.dst.elem_stride);
-  for (int idx = threadIdx.x; idx < count; idx += blockDim.x) {
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int idx = threadIdx.x + i * blockDim.x;
+    if (idx >= count) break;
     int ly, e;
     if (a.store_lines_fast) {
       ly = idx & (lpb - 1);
@@ -214,33 +230,39 @@ __device__ inline void store_lines(const FftArgs& a, const float2* buf, int ls, 
     if (base == ~0ull) continue;
     uint32_t bin = static_cast<uint32_t>(e) + a.out_offset;
     if (bin >= static_cast<uint32_t>(N)) bin -= N;
-    float w = a.scale;
-    if (a.out_weight) w *= a.out_weight[e];
-    if (a.line_weight) w *= a.line_weight[line_q[ly]];
-    a.out[base + static_cast<uint64_t>(static_cast<uint32_t>(e) * es)] = cscale(buf[ly * ls + bin], w);
+    a.out[base + static_cast<uint64_t>(static_cast<uint32_t>(e) * es)] =
+        cscale(buf[ly * ls + pad(static_cast<int>(bin))], ow[e] * lw[ly]);
   }
 }
 
 template <int N, bool INV>
 __global__ __launch_bounds__(1024) void fft_lds_kernel(FftArgs a, const float2* __restrict__ tw_global, int lpb_log2) {
   constexpr int T = Shape<N>::T;
-  constexpr int LS = N + 1;  // pad each LDS line by one element to spread banks across lines
+  constexpr int LS = line_stride<N>();
   extern __shared__ float2 smem[];
   __shared__ uint64_t src_base[64], dst_base[64];
-  __shared__ uint32_t line_q[64];
+  __shared__ float ow[N], lw[64];  // output weights (scale x out_weight) and per-line weights
   const int lpb = 1 << lpb_log2;
   float2* tw = smem;
   float2* buf = smem + N;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = tw_global[i];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    tw[i] = tw_global[i];
+    ow[i] = (i < static_cast<int>(a.n_out) && a.out_weight) ? a.scale * a.out_weight[i] : a.scale;
+  }
   const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
   const uint64_t first = static_cast<uint64_t>(blockIdx.x) * lpb;
-  stage_bases(a, src_base, dst_base, line_q, lpb, first, total);
+  for (int ly = threadIdx.x; ly < lpb; ly += blockDim.x) {
+    const uint64_t line = first + ly;
+    src_base[ly] = line < total ? line_base(a.src, line) : ~0ull;
+    dst_base[ly] = line < total ? line_base(a.dst, line) : ~0ull;
+    lw[ly] = a.line_weight ? a.line_weight[line % a.src.lines] : 1.0f;
+  }
   __syncthreads();
   load_lines<N>(a, buf, LS, lpb_log2, src_base);
   __syncthreads();
   const int ly = threadIdx.x / T, t = threadIdx.x % T;
   passes<N, INV, 0, 1>(buf + ly * LS, tw, t);
-  store_lines<N>(a, buf, LS, lpb_log2, line_q, dst_base);
+  store_lines<N>(a, buf, LS, lpb_log2, ow, lw, dst_base);
 }
 
 // Any length: direct DFT per output bin against the twiddle table (used for
@@ -319,14 +341,16 @@ void launch_fast(const FftArgs& a, const float2* tw, hipStream_t stream, int lpb
   if (lpb <= 0) lpb = static_cast<int>(tune_get(kTuneFftLinesPerBlock));
   if (lpb <= 0) lpb = (a.load_lines_fast || a.store_lines_fast) ? 16 : (T >= 64 ? 4 : 256 / T);
   while (lpb * T > 1024) lpb /= 2;
-  // LDS: twiddles + lpb padded lines (plus 1 KiB of static base offsets).
-  const int lds_cap = static_cast<int>(((max_dynamic_lds() - 1024) / sizeof(float2) - N) / (N + 1));
+  // LDS: twiddles + lpb padded lines, next to the static per-block tables
+  // (line bases, output and line weights).
+  constexpr int kStatic = 64 * 2 * 8 + N * 4 + 64 * 4;
+  const int lds_cap = static_cast<int>(((max_dynamic_lds() - kStatic) / sizeof(float2) - N) / line_stride<N>());
   while (lpb > 1 && lpb > lds_cap) lpb /= 2;
   int lpb_log2 = 0;
   while ((1 << (lpb_log2 + 1)) <= lpb) ++lpb_log2;
   lpb = 1 << lpb_log2;
   const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
-  const size_t shmem = (static_cast<size_t>(N) + static_cast<size_t>(lpb) * (N + 1)) * sizeof(float2);
+  const size_t shmem = (static_cast<size_t>(N) + static_cast<size_t>(lpb) * line_stride<N>()) * sizeof(float2);
   const unsigned grid = ceil_div(total, static_cast<uint64_t>(lpb));
   if (a.inverse) {
     if (shmem > 65536)

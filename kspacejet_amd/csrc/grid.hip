@@ -82,7 +82,8 @@ struct GatherArgs {
   const uint32_t* multi_blocks;     // [multi_count]
   float2* partial;                  // [slots][4][Cp]
   float2* grid;                     // [G_r][G_c][Cp]
-  uint32_t xcd_chunk, wg_per_xcd;   // workgroup -> item range mapping
+  const uint32_t* xcd_first;        // [9] cost-balanced XCD item ranges (gather_map 2)
+  uint32_t xcd_chunk, wg_per_xcd;   // workgroup -> item range mapping (gather_map)
   uint32_t grid_rows, grid_cols, block_cols, cp, item_count, multi_count;
 };
 
@@ -125,30 +126,49 @@ __device__ inline float weight_comp(const uint32_t (&wl)[NL]) {
   return __uint_as_float(row_bcast<F % 16>(wl[F / 16]));
 }
 
-template <int SW, int B, bool TAIL>
-__device__ inline void gather_batch(const GatherArgs& a, float2 (&acc)[4], uint32_t& kj, uint32_t v, uint32_t end,
-                                    int sw, int c) {
-  const float2* rows = a.sorted + c;
+// CPL coils per lane: a lane reads CPL consecutive coils of a sample row with one
+// 8*CPL-byte load (16-byte loads move twice the data per texture-data cycle of
+// 8-byte ones), so an item of Cp coils takes SW = Cp / CPL lanes.
+template <int CPL>
+struct CoilVec;
+template <>
+struct CoilVec<1> {
+  using T = float2;
+  __device__ static void split(const T& v, float2 (&o)[1]) { o[0] = v; }
+};
+template <>
+struct CoilVec<2> {
+  using T = float4;
+  __device__ static void split(const T& v, float2 (&o)[2]) {
+    o[0] = make_float2(v.x, v.y);
+    o[1] = make_float2(v.z, v.w);
+  }
+};
+
+template <int SW, int B, int CPL, bool TAIL>
+__device__ inline void gather_batch(const GatherArgs& a, float2 (&acc)[4][CPL], uint32_t& kj, uint32_t v,
+                                    uint32_t end, int sw, int c) {
+  using V = typename CoilVec<CPL>::T;
+  const V* rows = reinterpret_cast<const V*>(a.sorted) + c;
+  const uint32_t rstride = a.cp / CPL;  // row length in V elements
   const uint32_t n = end - v;
   uint32_t k[B];
   float4 w[B];
   if constexpr (SW >= 16) {
-    constexpr int NL = (4 * B + 15) / 16;  // weight loads of 16 floats per batch
+    // Lane r of every 16-lane row loads visit r % B's weights (one float4) and, for
+    // r < B, its position; DPP row broadcasts hand them to every lane of the row.
     const int r = c & 15;
-    const float* wf = reinterpret_cast<const float*>(a.visit_w);
-    uint32_t wl[NL];
-#pragma unroll
-    for (int j = 0; j < NL; ++j) {
-      const uint64_t idx = 4ull * v + 16u * j + r;
-      wl[j] = (!TAIL || idx < 4ull * end) ? __float_as_uint(wf[idx]) : 0u;
-    }
+    const uint32_t vi = v + (r % B);
+    const float4 wl = (!TAIL || vi < end) ? a.visit_w[vi] : make_float4(0.f, 0.f, 0.f, 0.f);
     [&]<int... I>(std::integer_sequence<int, I...>) {
       ((k[I] = row_bcast<I>(kj)), ...);
     }(std::make_integer_sequence<int, B>{});
     if (!TAIL) kj = load_positions<B>(a, v + B, end, r);
     [&]<int... I>(std::integer_sequence<int, I...>) {
-      ((w[I] = make_float4(weight_comp<4 * I>(wl), weight_comp<4 * I + 1>(wl), weight_comp<4 * I + 2>(wl),
-                           weight_comp<4 * I + 3>(wl))),
+      ((w[I] = make_float4(__uint_as_float(row_bcast<I>(__float_as_uint(wl.x))),
+                           __uint_as_float(row_bcast<I>(__float_as_uint(wl.y))),
+                           __uint_as_float(row_bcast<I>(__float_as_uint(wl.z))),
+                           __uint_as_float(row_bcast<I>(__float_as_uint(wl.w))))),
        ...);
     }(std::make_integer_sequence<int, B>{});
   } else {
@@ -159,44 +179,91 @@ __device__ inline void gather_batch(const GatherArgs& a, float2 (&acc)[4], uint3
     for (int i = 0; i < B; ++i)
       w[i] = (!TAIL || static_cast<uint32_t>(i) < n) ? a.visit_w[v + i] : make_float4(0.f, 0.f, 0.f, 0.f);
   }
-  float2 d[B];
+  V d[B];
 #pragma unroll
-  for (int i = 0; i < B; ++i)
-    d[i] = (!TAIL || static_cast<uint32_t>(i) < n) ? rows[k[i] * a.cp] : make_float2(0.f, 0.f);
+  for (int i = 0; i < B; ++i) {
+    if (!TAIL || static_cast<uint32_t>(i) < n) {
+      d[i] = rows[static_cast<uint64_t>(k[i]) * rstride];
+    } else {
+      d[i] = V{};
+    }
+  }
 #pragma unroll
-  for (int i = 0; i < B; ++i) fma4(acc, w[i], d[i]);
+  for (int i = 0; i < B; ++i) {
+    float2 e[CPL];
+    CoilVec<CPL>::split(d[i], e);
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      float2 q[4] = {acc[0][j], acc[1][j], acc[2][j], acc[3][j]};
+      fma4(q, w[i], e[j]);
+      acc[0][j] = q[0];
+      acc[1][j] = q[1];
+      acc[2][j] = q[2];
+      acc[3][j] = q[3];
+    }
+  }
 }
 
-template <int SW, int B>
-__global__ __launch_bounds__(256) void gather_item_kernel(GatherArgs a) {
-  constexpr int NSW = 64 / SW;
-  const int lane = threadIdx.x & 63, sw = lane / SW, c = lane % SW;
-  // With xcd_chunk, workgroup w (dispatched to XCD w % 8) takes the w/8-th group
-  // of its XCD's contiguous item range, so neighbouring 2x2 blocks -- which read
-  // the same sample rows -- share one XCD's L2.
-  const uint32_t wg = a.xcd_chunk ? (blockIdx.x % kXcds) * a.wg_per_xcd + blockIdx.x / kXcds : blockIdx.x;
-  const uint32_t item = (wg * 4 + (threadIdx.x >> 6)) * NSW + sw;
-  const bool live = item < a.item_count;
+// One work item: gather its visits into four accumulators per coil and write its
+// cells (or its partial-sum slot).  `live` is false for lanes past the item range.
+template <int SW, int B, int CPL>
+__device__ inline void gather_one_item(const GatherArgs& a, uint32_t item, bool live, int sw, int c) {
   const uint4 info = live ? a.item_info[item] : make_uint4(0, 0, 0, 0);
-  float2 acc[4];
+  float2 acc[4][CPL];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) acc[q] = make_float2(0.f, 0.f);
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) acc[q][j] = make_float2(0.f, 0.f);
   uint32_t v = info.y;
   uint32_t kj = load_positions<B>(a, v, info.z, SW >= 16 ? (c & 15) : c);
-  for (; v + B <= info.z; v += B) gather_batch<SW, B, false>(a, acc, kj, v, info.z, sw, c);
-  if (v < info.z) gather_batch<SW, B, true>(a, acc, kj, v, info.z, sw, c);
+  for (; v + B <= info.z; v += B) gather_batch<SW, B, CPL, false>(a, acc, kj, v, info.z, sw, c);
+  if (v < info.z) gather_batch<SW, B, CPL, true>(a, acc, kj, v, info.z, sw, c);
   if (!live) return;
+  using V = typename CoilVec<CPL>::T;
   const uint32_t by = info.x / a.block_cols, bx = info.x % a.block_cols;
+  const uint32_t rstride = a.cp / CPL;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
+    V out;
+    if constexpr (CPL == 1) {
+      out = acc[q][0];
+    } else {
+      out = make_float4(acc[q][0].x, acc[q][0].y, acc[q][1].x, acc[q][1].y);
+    }
     if (info.w == 0xFFFFFFFFu) {
       const uint32_t y = 2 * by + q / 2, x = 2 * bx + q % 2;
       if (y < a.grid_rows && x < a.grid_cols)
-        a.grid[(static_cast<uint64_t>(y) * a.grid_cols + x) * a.cp + c] = acc[q];
+        reinterpret_cast<V*>(a.grid)[(static_cast<uint64_t>(y) * a.grid_cols + x) * rstride + c] = out;
     } else {
-      a.partial[(static_cast<uint64_t>(info.w) * 4 + q) * a.cp + c] = acc[q];
+      reinterpret_cast<V*>(a.partial)[(static_cast<uint64_t>(info.w) * 4 + q) * rstride + c] = out;
     }
   }
+}
+
+// Item -> wave mappings (tuning key gather_map):
+//  0: items in launch order, one group of 64/SW items per wave;
+//  1: workgroup w (dispatched to XCD w % 8) takes the w/8-th group of an equal share
+//     of items, so neighbouring 2x2 blocks -- which read the same sample rows --
+//     share one XCD's L2;
+//  2: as 1, with the eight ranges cut at equal cost (visits + a fixed cost per item)
+//     by the plan, each XCD's workgroups striding through their range.
+template <int SW, int B, int CPL>
+__global__ __launch_bounds__(256) void gather_item_kernel(GatherArgs a) {
+  constexpr int NSW = 64 / SW;
+  const int lane = threadIdx.x & 63, sw = lane / SW, c = lane % SW;
+  const uint32_t wave = threadIdx.x >> 6;
+  if (a.xcd_chunk == 2) {
+    const uint32_t x = blockIdx.x % kXcds, local = blockIdx.x / kXcds;
+    const uint32_t end = a.xcd_first[x + 1];
+    for (uint32_t base = a.xcd_first[x] + (local * 4 + wave) * NSW; base < end; base += a.wg_per_xcd * 4 * NSW) {
+      const uint32_t item = base + sw;
+      gather_one_item<SW, B, CPL>(a, item, item < end, sw, c);
+    }
+    return;
+  }
+  const uint32_t wg = a.xcd_chunk ? (blockIdx.x % kXcds) * a.wg_per_xcd + blockIdx.x / kXcds : blockIdx.x;
+  const uint32_t item = (wg * 4 + wave) * NSW + sw;
+  gather_one_item<SW, B, CPL>(a, item, item < a.item_count, sw, c);
 }
 
 // Blocks with several items: thread per (block, cell, coil) sums the item partials in item order.
@@ -224,20 +291,20 @@ __global__ __launch_bounds__(256) void multi_reduce_kernel(GatherArgs a) {
   if (y < a.grid_rows && x < a.grid_cols) a.grid[(static_cast<uint64_t>(y) * a.grid_cols + x) * a.cp + c] = s;
 }
 
-template <int SW>
+template <int SW, int CPL>
 void launch_gather(GatherArgs a, hipStream_t stream) {
   constexpr int NSW = 64 / SW;
   const uint32_t waves = ceil_div(a.item_count, NSW);
   uint32_t wgs = ceil_div(waves, 4);
-  a.xcd_chunk = tune_get(kTuneGatherMap) == 1 ? 1u : 0u;
+  a.xcd_chunk = static_cast<uint32_t>(tune_get(kTuneGatherMap));
   if (a.xcd_chunk) {
     a.wg_per_xcd = ceil_div(wgs, kXcds);
     wgs = a.wg_per_xcd * kXcds;  // the padding workgroups find no item and exit
   }
   switch (tune_get(kTuneGatherBatch)) {
-    case 4: hipLaunchKernelGGL((gather_item_kernel<SW, 4>), dim3(wgs), dim3(256), 0, stream, a); break;
-    case 16: hipLaunchKernelGGL((gather_item_kernel<SW, 16>), dim3(wgs), dim3(256), 0, stream, a); break;
-    default: hipLaunchKernelGGL((gather_item_kernel<SW, 8>), dim3(wgs), dim3(256), 0, stream, a); break;
+    case 4: hipLaunchKernelGGL((gather_item_kernel<SW, 4, CPL>), dim3(wgs), dim3(256), 0, stream, a); break;
+    case 16: hipLaunchKernelGGL((gather_item_kernel<SW, 16, CPL>), dim3(wgs), dim3(256), 0, stream, a); break;
+    default: hipLaunchKernelGGL((gather_item_kernel<SW, 8, CPL>), dim3(wgs), dim3(256), 0, stream, a); break;
   }
   KSJ_LAUNCH_CHECK();
 }
@@ -293,6 +360,7 @@ GatherArgs gather_args(const Plan2& plan, const float2* d_sorted, uint32_t coils
   a.cp = padded_coils(coils);
   a.item_count = plan.item_count;
   a.multi_count = plan.multi_count;
+  a.xcd_first = plan.xcd_first;
   return a;
 }
 }  // namespace
@@ -310,11 +378,14 @@ void grid_gather(const Plan2& plan, const float2* d_sorted, uint32_t coils, floa
   require(plan.has_trajectory, "plan has no trajectory");
   require(plan.multi_count == 0 || d_partial != nullptr, "multi-item blocks need a partial-sum buffer");
   const GatherArgs a = gather_args(plan, d_sorted, coils, d_grid, d_partial);
+  // Two coils per lane from 32 coils up (sub-waves of 16 or 32 lanes keep the DPP
+  // row broadcasts); one coil per lane below (tuning key gather_coils_per_lane).
+  const bool two = tune_get(kTuneGatherCoilsPerLane) == 2;
   switch (a.cp) {
-    case 8: return launch_gather<8>(a, stream);
-    case 16: return launch_gather<16>(a, stream);
-    case 32: return launch_gather<32>(a, stream);
-    case 64: return launch_gather<64>(a, stream);
+    case 8: return launch_gather<8, 1>(a, stream);
+    case 16: return launch_gather<16, 1>(a, stream);
+    case 32: return two ? launch_gather<16, 2>(a, stream) : launch_gather<32, 1>(a, stream);
+    case 64: return two ? launch_gather<32, 2>(a, stream) : launch_gather<64, 1>(a, stream);
     default: throw Error(1, "gridding supports at most 64 coils");
   }
 }

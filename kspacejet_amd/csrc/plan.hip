@@ -203,6 +203,37 @@ __global__ void slot_count_kernel(const uint32_t* first, uint64_t nblocks, uint3
   slots[b] = n > 1 ? n : 0u;
 }
 
+// Cost of an item: its visits plus a fixed cost for loading its header and writing
+// its cells.  cost[items] = 0 so the exclusive scan's last entry is the total.
+constexpr uint32_t kItemFixedCost = 16;
+
+__global__ void item_cost_kernel(const uint4* info, uint32_t items, uint32_t* cost) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < items) cost[i] = info[i].z - info[i].y + kItemFixedCost;
+  if (i == items) cost[i] = 0;
+}
+
+// first[x] = smallest item whose cost prefix reaches x/8 of the total (first[0] = 0,
+// first[8] = items): eight contiguous ranges of about equal cost.
+__global__ void xcd_split_kernel(const uint32_t* prefix, uint32_t items, uint32_t* first) {
+  const uint32_t x = threadIdx.x;
+  if (x > kPlanXcds) return;
+  if (x == 0 || x == kPlanXcds) {
+    first[x] = x == 0 ? 0u : items;
+    return;
+  }
+  const uint64_t target = static_cast<uint64_t>(prefix[items]) * x / kPlanXcds;
+  uint32_t lo = 0, hi = items;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (prefix[mid] >= target)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  first[x] = lo;
+}
+
 __global__ void item_fill_kernel(const uint32_t* first, const uint32_t* slot, const uint32_t* row_off, int width,
                                  uint32_t item_visits, uint64_t nblocks, uint4* info) {
   const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -313,6 +344,7 @@ Plan2* plan2_create(const Plan2Desc& desc) {
     p->block_first_item = alloc<uint32_t>(p->nblocks + 1, &total);
     p->block_first_slot = alloc<uint32_t>(p->nblocks + 1, &total);
     p->multi_blocks = alloc<uint32_t>(p->nblocks, &total);
+    p->xcd_first = alloc<uint32_t>(kPlanXcds + 1, &total);
     p->frac = alloc<double>(2 * S, &total);
     p->sorted_keys = alloc<uint32_t>(S, &total);
     p->counters = alloc<uint32_t>(4, &total);
@@ -343,7 +375,8 @@ void plan2_destroy(Plan2* p) {
   void* ptrs[] = {p->weight,        p->rank,        p->src,          p->bins,         p->cell_start,
                   p->taps,          p->visit_row_off, p->visit_k,    p->visit_w,      p->block_first_item,
                   p->block_first_slot, p->item_info,    p->multi_blocks, p->frac,         p->sorted_keys,
-                  p->counters,      p->sort_scratch, p->deapod_rows, p->deapod_cols};
+                  p->counters,      p->sort_scratch, p->deapod_rows, p->deapod_cols, p->item_cost,
+                  p->item_scratch,  p->xcd_first};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -435,11 +468,16 @@ void plan2_set_trajectory(Plan2* p, const float* d_traj, const float* d_dcf, hip
   KSJ_HIP(hipMemcpyAsync(&items, p->block_first_item + p->nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
   KSJ_HIP(hipMemcpyAsync(&multi, p->counters, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
   KSJ_HIP(hipStreamSynchronize(stream));
-  if (items > p->item_capacity) {
-    if (p->item_info) KSJ_HIP(hipFree(p->item_info));
-    p->item_info = nullptr;
-    p->item_capacity = items;
-    KSJ_HIP(hipMalloc(&p->item_info, std::max<size_t>(1, items) * sizeof(uint4)));
+  if (items > p->item_capacity || p->item_cost == nullptr) {
+    for (void** q : {reinterpret_cast<void**>(&p->item_info), reinterpret_cast<void**>(&p->item_cost),
+                     &p->item_scratch}) {
+      if (*q) KSJ_HIP(hipFree(*q));
+      *q = nullptr;
+    }
+    p->item_capacity = std::max(items, p->item_capacity);
+    KSJ_HIP(hipMalloc(&p->item_info, std::max<size_t>(1, p->item_capacity) * sizeof(uint4)));
+    KSJ_HIP(hipMalloc(&p->item_cost, (static_cast<size_t>(p->item_capacity) + 1) * sizeof(uint32_t)));
+    KSJ_HIP(hipMalloc(&p->item_scratch, std::max<size_t>(256, scan_scratch_bytes(p->item_capacity + 1ull))));
   }
   p->item_count = items;
   // Partial-sum slots: consecutive for the items of each multi-item block.
@@ -452,6 +490,12 @@ void plan2_set_trajectory(Plan2* p, const float* d_traj, const float* d_dcf, hip
   hipLaunchKernelGGL(item_fill_kernel, dim3(ceil_div(p->nblocks, 256)), dim3(256), 0, stream, p->block_first_item,
                      p->block_first_slot, p->visit_row_off, static_cast<int>(p->d.width), p->item_visits, p->nblocks,
                      p->item_info);
+  KSJ_LAUNCH_CHECK();
+  hipLaunchKernelGGL(item_cost_kernel, dim3(ceil_div(items + 1ull, 256)), dim3(256), 0, stream, p->item_info, items,
+                     p->item_cost);
+  KSJ_LAUNCH_CHECK();
+  exclusive_scan_u32(p->item_cost, p->item_cost, items + 1ull, p->item_scratch, stream);
+  hipLaunchKernelGGL(xcd_split_kernel, dim3(1), dim3(64), 0, stream, p->item_cost, items, p->xcd_first);
   KSJ_LAUNCH_CHECK();
   KSJ_HIP(hipStreamSynchronize(stream));
   p->slot_count = slots;

@@ -21,6 +21,8 @@
 namespace ksj::gpu {
 
 enum : uint32_t { kKernelLinear = 0, kKernelKaiserBessel = 1 };
+// XCD count of the MI355X: the gather can give each XCD a contiguous, cost-balanced item range.
+constexpr uint32_t kPlanXcds = 8;
 enum : uint32_t { kDensityNone = 0, kDensityRamp = 1, kDensityGiven = 2 };
 
 struct Plan2Desc {
@@ -68,6 +70,9 @@ struct Plan2 {
   uint32_t item_capacity = 0;
   uint32_t item_count = 0;               // host copy
   uint32_t slot_count = 0;               // partial slots (items of multi-item blocks), host copy
+  uint32_t* item_cost = nullptr;         // [item_capacity + 1] prefix sums of item costs (visits + fixed)
+  void* item_scratch = nullptr;          // scan scratch for item_cost
+  uint32_t* xcd_first = nullptr;         // [kPlanXcds + 1] first item of each XCD's contiguous range
   uint32_t* multi_blocks = nullptr;      // [nblocks] blocks with more than one item
   uint32_t multi_count = 0;              // host copy
   float* deapod_rows = nullptr;    // [N_r] or null

@@ -3,7 +3,9 @@
 
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <utility>
+#include <vector>
 
 #include "combine.hpp"
 #include "fft.hpp"
@@ -88,20 +90,23 @@ namespace {
 // p % lanes), kept per (device, caller stream) and created at first use.  They are
 // ordinary default-priority streams, joined to the caller's stream with events.
 struct Lanes {
-  hipStream_t stream[3] = {};
-  hipEvent_t join[3] = {};
+  hipStream_t stream[8] = {};
+  hipEvent_t join[8] = {};
   hipEvent_t fork = nullptr;
 };
 
-Lanes& lanes_for(hipStream_t caller) {
+// kind 0: pass lanes of one batched call; kind 1: item lanes of a multi-plan call
+// (a multi-plan item runs a batched call of one scan on its lane, so the two pools
+// must differ).
+Lanes& lane_pool(hipStream_t caller, int kind) {
   static std::mutex mu;
-  static std::map<std::pair<int, hipStream_t>, Lanes> pool;
+  static std::map<std::tuple<int, hipStream_t, int>, Lanes> pool;
   int dev = 0;
   KSJ_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lock(mu);
-  Lanes& l = pool[{dev, caller}];
+  Lanes& l = pool[{dev, caller, kind}];
   if (l.fork == nullptr) {
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < 8; ++i) {
       KSJ_HIP(hipStreamCreateWithFlags(&l.stream[i], hipStreamNonBlocking));
       KSJ_HIP(hipEventCreateWithFlags(&l.join[i], hipEventDisableTiming));
     }
@@ -109,6 +114,9 @@ Lanes& lanes_for(hipStream_t caller) {
   }
   return l;
 }
+
+Lanes& lanes_for(hipStream_t caller) { return lane_pool(caller, 0); }
+Lanes& multi_lanes_for(hipStream_t caller) { return lane_pool(caller, 1); }
 
 uint32_t pass_count(uint32_t channels, uint32_t scans) {
   const uint32_t per = adjoint2_scans_per_pass(channels);
@@ -260,6 +268,73 @@ void adjoint2_execute(const Plan2& plan, const AdjointScan* scans, uint32_t coun
       KSJ_HIP(hipEventRecord(lanes->join[i], lanes->stream[i]));
       KSJ_HIP(hipStreamWaitEvent(caller, lanes->join[i], 0));
     }
+  }
+}
+
+namespace {
+
+struct MultiLayout {
+  uint32_t lanes = 1;
+  std::vector<uint32_t> order;   // items, largest first
+  std::vector<size_t> lane_ws;   // workspace bytes of each lane (its largest item)
+};
+
+MultiLayout multi_layout(const MultiItem* items, uint32_t count) {
+  MultiLayout m;
+  m.lanes = static_cast<uint32_t>(std::min<int64_t>(tune_get(kTuneMultiLanes), std::max(count, 1u)));
+  m.order.resize(count);
+  std::vector<size_t> ws(count);
+  for (uint32_t i = 0; i < count; ++i) {
+    require(items[i].plan != nullptr && items[i].plan->has_trajectory, "multi item without a ready plan");
+    ws[i] = adjoint2_workspace_bytes(*items[i].plan, items[i].channels, 1);
+    m.order[i] = i;
+  }
+  std::stable_sort(m.order.begin(), m.order.end(), [&](uint32_t a, uint32_t b) { return ws[a] > ws[b]; });
+  m.lane_ws.assign(m.lanes, 0);
+  for (uint32_t j = 0; j < count; ++j) m.lane_ws[j % m.lanes] = std::max(m.lane_ws[j % m.lanes], ws[m.order[j]]);
+  return m;
+}
+
+}  // namespace
+
+size_t adjoint2_multi_workspace_bytes(const MultiItem* items, uint32_t count) {
+  const MultiLayout m = multi_layout(items, count);
+  size_t total = 0;
+  for (size_t b : m.lane_ws) total += align256(b);
+  return total;
+}
+
+void adjoint2_execute_multi(const MultiItem* items, uint32_t count, int layout, void* d_ws, size_t ws_bytes,
+                            hipStream_t stream) {
+  require(count >= 1, "multi adjoint needs at least one item");
+  const MultiLayout m = multi_layout(items, count);
+  size_t total = 0;
+  for (size_t b : m.lane_ws) total += align256(b);
+  require(ws_bytes >= total, "multi adjoint workspace too small");
+  if (m.lanes == 1) {
+    for (uint32_t j = 0; j < count; ++j) {
+      const MultiItem& it = items[m.order[j]];
+      adjoint2_execute(*it.plan, &it.scan, 1, it.channels, layout, d_ws, m.lane_ws[0], stream);
+    }
+    return;
+  }
+  Lanes& lanes = multi_lanes_for(stream);
+  KSJ_HIP(hipEventRecord(lanes.fork, stream));
+  std::vector<char*> base(m.lanes);
+  char* p = static_cast<char*>(d_ws);
+  for (uint32_t l = 0; l < m.lanes; ++l) {
+    KSJ_HIP(hipStreamWaitEvent(lanes.stream[l], lanes.fork, 0));
+    base[l] = p;
+    p += align256(m.lane_ws[l]);
+  }
+  for (uint32_t j = 0; j < count; ++j) {
+    const uint32_t l = j % m.lanes;
+    const MultiItem& it = items[m.order[j]];
+    adjoint2_execute(*it.plan, &it.scan, 1, it.channels, layout, base[l], m.lane_ws[l], lanes.stream[l]);
+  }
+  for (uint32_t l = 0; l < m.lanes; ++l) {
+    KSJ_HIP(hipEventRecord(lanes.join[l], lanes.stream[l]));
+    KSJ_HIP(hipStreamWaitEvent(stream, lanes.join[l], 0));
   }
 }
 

@@ -31,6 +31,21 @@ size_t adjoint2_pass_workspace_bytes(const Plan2& plan, uint32_t channels, uint3
 void adjoint2_execute(const Plan2& plan, const AdjointScan* scans, uint32_t count, uint32_t channels, int layout,
                       void* d_ws, size_t ws_bytes, hipStream_t stream);
 
+// One item of a multi-plan call: its own plan (geometry and trajectory), channels and outputs.
+struct MultiItem {
+  const Plan2* plan = nullptr;
+  uint32_t channels = 0;
+  AdjointScan scan;
+};
+
+// Items of different geometry, coil count and trajectory: each item's pipeline runs on
+// one of `multi_lanes` streams (largest items first, round-robin), the lanes joined to
+// `stream`.  Enqueue-only (no host synchronisation, no allocation once the FFT twiddles
+// of every grid length exist), so the call can be captured into a HIP graph.
+size_t adjoint2_multi_workspace_bytes(const MultiItem* items, uint32_t count);
+void adjoint2_execute_multi(const MultiItem* items, uint32_t count, int layout, void* d_ws, size_t ws_bytes,
+                            hipStream_t stream);
+
 size_t nudft2_adjoint_workspace_bytes(uint32_t channels, uint32_t rows, uint32_t cols);
 
 }  // namespace ksj::gpu

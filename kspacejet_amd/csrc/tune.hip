@@ -17,11 +17,13 @@ struct Param {
 bool is_pow2_or_zero(int64_t v) { return v == 0 || (v > 0 && v <= 64 && (v & (v - 1)) == 0); }
 
 Param g_params[kTuneKeys] = {
-    {"gather_map", 0, [](int64_t v) { return v == 0 || v == 1; }},
+    {"gather_map", 0, [](int64_t v) { return v >= 0 && v <= 2; }},
     {"item_visits", 64, [](int64_t v) { return v >= 1 && v <= 4096; }},
     {"fft_lines_per_block", 0, &is_pow2_or_zero},
     {"gather_batch", 8, [](int64_t v) { return v == 4 || v == 8 || v == 16; }},
     {"pass_lanes", 2, [](int64_t v) { return v >= 1 && v <= 3; }},
+    {"multi_lanes", 4, [](int64_t v) { return v >= 1 && v <= 8; }},
+    {"gather_coils_per_lane", 1, [](int64_t v) { return v == 1 || v == 2; }},
 };
 
 }  // namespace

@@ -37,7 +37,7 @@ def tune(ops):
         ops.tune_set(key, value)
 
 
-def scan(ops, n=96, coils=12, spokes=61, readout=192, seed=3):
+def scan(ops, n=96, coils=40, spokes=61, readout=192, seed=3):
     traj = ops.golden_angle2(spokes, readout)
     img = torch.from_numpy(synth.coil_images2(n, coils, seed)).cuda()
     k = ops.nudft2_forward_f64(traj, img, (n - 1) / 2, (n - 1) / 2).cpu().numpy()
@@ -55,7 +55,8 @@ def test_launch_parameters_do_not_change_results(ops, tune):
     n = 96
     traj, k = scan(ops)
     base_img, base_rss = recon(ops, traj, k, n)
-    for key, values in (("gather_map", (0, 1)), ("gather_batch", (4, 8, 16)), ("fft_lines_per_block", (0, 4, 16))):
+    for key, values in (("gather_map", (0, 1, 2)), ("gather_batch", (4, 8, 16)), ("fft_lines_per_block", (0, 4, 16)),
+                        ("gather_coils_per_lane", (1, 2))):
         for v in values:
             tune(key, v)
             img, rss = recon(ops, traj, k, n)
