@@ -132,6 +132,8 @@ def main(argv=None) -> int:
     ap.add_argument("--load", type=float, default=0.7, help="Poisson arrival rate as a fraction of served rate")
     ap.add_argument("--latency-requests", type=int, default=300)
     ap.add_argument("--mixed-requests", type=int, default=48, help="0 skips the mixed-stream section")
+    ap.add_argument("--uneven-items", type=int, default=64, help="items of the uneven batch; 0 skips that section")
+    ap.add_argument("--tune-seconds", type=float, default=20.0, help="autotuner budget for the uneven batch")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
@@ -279,6 +281,11 @@ def main(argv=None) -> int:
         log("mixed stream ...")
         mixed = run_mixed(torch, lib, _lib, server, plan, scans, traj, traj_key, args.mixed_requests, stream)
     server.close()
+    del scans, outs, ws
+    uneven = None
+    if args.uneven_items > 0:
+        log("uneven batch ...")
+        uneven = run_uneven(torch, ops, lib, args.uneven_items, args.steps, args.warmup, args.tune_seconds)
 
     if args.copy_probe:
         src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
@@ -315,9 +322,107 @@ def main(argv=None) -> int:
         "stages": stage_report,
         "served_stats": served_stats,
         "mixed": mixed,
+        "uneven": uneven,
     }
     print(json.dumps(line))
     return 0
+
+
+def run_uneven(torch, ops, lib, count, steps, warmup, tune_seconds):
+    """Uneven radial batch: `count` items drawn from seeds (matrix 128/192/256/320, 8-32
+    coils, 101-603 golden-angle spokes, readout 2 x matrix, KB width 4, sigma 2, ramp
+    density), every item's RSS image per step.  Three forms over the same resident data:
+    plain (one synchronous ksj_gpu_adjoint2_execute per item), one multi-plan call
+    (ksj_gpu_adjoint2_execute_multi) after autotuning its launch keys, and that call
+    captured once in a HIP graph and replayed."""
+    from kspacejet_amd import autotune, synth
+    shapes = synth.uneven_items2(count)
+    t0 = time.perf_counter()
+    plans, ks, outs = [], [], []
+    for i, (m, c, spokes) in enumerate(shapes):
+        seed = synth.scan_seed(3, i)
+        traj = ops.golden_angle2(spokes, 2 * m)
+        img = torch.from_numpy(synth.coil_images2(m, c, seed)).cuda()
+        k = ops.nudft2_forward_f64(traj, img, (m - 1) / 2, (m - 1) / 2).cpu().numpy()
+        ks.append(torch.from_numpy(synth.add_noise(k, seed).astype(np.complex64)).cuda())
+        plans.append(ops.Plan2(m, m, traj.shape[0], oversampling=SIGMA, width=WIDTH).set_trajectory(traj))
+        outs.append(torch.empty((m, m), dtype=torch.float32, device="cuda"))
+        del img, traj
+    torch.cuda.synchronize()
+    log(f"  {count} items generated in {time.perf_counter() - t0:.1f} s")
+    channels = [k.shape[0] for k in ks]
+    stream = torch.cuda.Stream()
+    sid = stream.cuda_stream
+
+    # plain: one synchronous call per item
+    ws1 = torch.empty(max(p.workspace_bytes(c) for p, c in zip(plans, channels)), dtype=torch.uint8, device="cuda")
+
+    def plain():
+        for p, k, c, o in zip(plans, ks, channels, outs):
+            rc = lib.ksj_gpu_adjoint2_execute(p._handle, k.data_ptr(), c, None, 0, o.data_ptr(), ws1.data_ptr(),
+                                              ws1.numel(), sid)
+            if rc != 0:
+                raise RuntimeError(lib.ksj_gpu_last_error().decode())
+            stream.synchronize()
+
+    for _ in range(warmup):
+        plain()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plain()
+    plain_s = (time.perf_counter() - t0) / steps
+    del ws1
+
+    # one multi-plan call; workspace sized for the largest lane counts the tuner tries
+    saved = {k: ops.tune_get(k) for k in ("multi_lanes", "pass_lanes")}
+    ops.tune_set("multi_lanes", max(autotune.EXACT_KEYS["multi_lanes"]))
+    ops.tune_set("pass_lanes", max(autotune.EXACT_KEYS["pass_lanes"]))
+    ws = torch.empty(ops.adjoint_multi_workspace_bytes(plans, channels), dtype=torch.uint8, device="cuda")
+    for k, v in saved.items():
+        ops.tune_set(k, v)
+
+    def multi():
+        with torch.cuda.stream(stream):
+            ops.adjoint_multi(plans, ks, rss=outs, workspace=ws)
+
+    def timed(run):
+        for _ in range(warmup):
+            run()
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run()
+        stream.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    untuned_s = timed(multi)
+    with torch.cuda.stream(stream):
+        tuned = autotune.autotune(multi, f"uneven{count}", budget_s=tune_seconds)
+    multi_s = timed(multi)
+
+    # the same call captured once in a HIP graph
+    graph = ops.Graph(stream)
+    with graph:
+        multi()
+    graph_s = timed(graph.replay)
+    graph.close()
+    res = {
+        "items": count, "shapes": "matrix {128,192,256,320} x coils 8-32 x spokes 101-603, readout 2 x matrix",
+        "matrix_counts": {str(m): sum(1 for s in shapes if s[0] == m) for m in synth.UNEVEN_MATRICES},
+        "coil_images": int(sum(channels)), "samples": int(sum(c * k.shape[1] for c, k in zip(channels, ks))),
+        "plain_images_per_s": round(count / plain_s, 1),
+        "multi_untuned_images_per_s": round(count / untuned_s, 1),
+        "multi_images_per_s": round(count / multi_s, 1),
+        "graph_images_per_s": round(count / graph_s, 1),
+        "graph_over_plain": round(plain_s / graph_s, 2),
+        "ms_per_batch": {"plain": round(1e3 * plain_s, 3), "multi": round(1e3 * multi_s, 3),
+                         "graph": round(1e3 * graph_s, 3)},
+        "tuned": tuned["settings"], "tune_trials": len(tuned["trials"]),
+        "workspace_bytes": ws.numel(),
+    }
+    for p in plans:
+        p.close()
+    return res
 
 
 def run_mixed(torch, lib, _lib, server, plan, radial_scans, traj, traj_key, count, stream):

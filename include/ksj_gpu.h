@@ -177,6 +177,30 @@ int ksj_gpu_adjoint2_execute_batch(const ksj_gpu_plan2* plan, uint32_t count, co
                                    uint32_t channels, void* const* d_images, int layout, float* const* d_rss,
                                    void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* Uneven batch: `count` scans, each with its own plan (geometry, kernel, trajectory)
+ * and channel count.  Every item's pipeline runs on one of "multi_lanes" streams
+ * (largest items first, round-robin), joined back to `stream`.  The call only
+ * enqueues work (no host synchronisation, no allocation once every grid length has
+ * been run once), so it can be captured with ksj_gpu_graph_begin/end and replayed.
+ * Item i writes d_images[i] (layout) and/or d_rss[i]; either array may be NULL or
+ * hold NULL entries.  No reference counterpart: the reference runs one frame per firing. */
+size_t ksj_gpu_adjoint2_multi_workspace_bytes(uint32_t count, const ksj_gpu_plan2* const* plans,
+                                              const uint32_t* channels);
+int ksj_gpu_adjoint2_execute_multi(uint32_t count, const ksj_gpu_plan2* const* plans, const void* const* d_kspace,
+                                   const uint32_t* channels, void* const* d_images, int layout, float* const* d_rss,
+                                   void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------ HIP graphs */
+
+/* Captures the work enqueued on `stream` (and on streams it forks to through events)
+ * between begin and end into an executable graph.  `stream` must not be the null
+ * stream.  Replaying the graph repeats the captured launches with the same pointers. */
+typedef struct ksj_gpu_graph ksj_gpu_graph;
+int ksj_gpu_graph_begin(void* stream);
+int ksj_gpu_graph_end(void* stream, ksj_gpu_graph** out_graph);
+int ksj_gpu_graph_launch(ksj_gpu_graph* graph, void* stream);
+void ksj_gpu_graph_destroy(ksj_gpu_graph* graph);
+
 /* ------------------------------------------------------------------ serving */
 
 /* A continuous-batching server on one GPU.  No reference counterpart: the reference

@@ -23,6 +23,11 @@ struct ksj_gpu_server {
   ksj::gpu::Server* impl;
 };
 
+struct ksj_gpu_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -257,6 +262,90 @@ int ksj_gpu_adjoint2_execute_batch(const ksj_gpu_plan2* plan, uint32_t count, co
     ksj::gpu::adjoint2_execute(*plan->impl, scans.data(), count, channels, layout, d_workspace, workspace_bytes,
                                as_stream(stream));
   });
+}
+
+}  // extern "C"
+
+namespace {
+std::vector<ksj::gpu::MultiItem> multi_items(uint32_t count, const ksj_gpu_plan2* const* plans,
+                                             const uint32_t* channels, const void* const* d_kspace,
+                                             void* const* d_images, float* const* d_rss) {
+  ksj::gpu::require(count >= 1 && plans && channels, "multi adjoint needs plans and channels");
+  std::vector<ksj::gpu::MultiItem> items(count);
+  for (uint32_t i = 0; i < count; ++i) {
+    ksj::gpu::require(plans[i] && plans[i]->impl, "null plan in a multi adjoint");
+    ksj::gpu::require(channels[i] >= 1 && channels[i] <= 64, "multi adjoint supports 1..64 channels per item");
+    items[i].plan = plans[i]->impl;
+    items[i].channels = channels[i];
+    if (d_kspace) {
+      ksj::gpu::require(d_kspace[i] != nullptr, "null k-space in a multi adjoint");
+      items[i].scan.kspace = static_cast<const float2*>(d_kspace[i]);
+      items[i].scan.images = d_images ? static_cast<float2*>(d_images[i]) : nullptr;
+      items[i].scan.rss = d_rss ? d_rss[i] : nullptr;
+      ksj::gpu::require(items[i].scan.images || items[i].scan.rss, "every multi item needs an output");
+    }
+  }
+  return items;
+}
+}  // namespace
+
+extern "C" {
+
+size_t ksj_gpu_adjoint2_multi_workspace_bytes(uint32_t count, const ksj_gpu_plan2* const* plans,
+                                              const uint32_t* channels) {
+  size_t out = 0;
+  const int rc = guarded([&] {
+    const auto items = multi_items(count, plans, channels, nullptr, nullptr, nullptr);
+    out = ksj::gpu::adjoint2_multi_workspace_bytes(items.data(), count);
+  });
+  return rc == KSJ_GPU_OK ? out : 0;
+}
+
+int ksj_gpu_adjoint2_execute_multi(uint32_t count, const ksj_gpu_plan2* const* plans, const void* const* d_kspace,
+                                   const uint32_t* channels, void* const* d_images, int layout, float* const* d_rss,
+                                   void* d_workspace, size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(d_kspace && d_workspace, "null argument");
+    const auto items = multi_items(count, plans, channels, d_kspace, d_images, d_rss);
+    ksj::gpu::adjoint2_execute_multi(items.data(), count, layout, d_workspace, workspace_bytes, as_stream(stream));
+  });
+}
+
+int ksj_gpu_graph_begin(void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(stream != nullptr, "graph capture needs a stream other than the null stream");
+    KSJ_HIP(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal));
+  });
+}
+
+int ksj_gpu_graph_end(void* stream, ksj_gpu_graph** out_graph) {
+  return guarded([&] {
+    ksj::gpu::require(stream != nullptr && out_graph != nullptr, "null argument");
+    auto* g = new ksj_gpu_graph{};
+    try {
+      KSJ_HIP(hipStreamEndCapture(as_stream(stream), &g->graph));
+      KSJ_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    } catch (...) {
+      if (g->graph) (void)hipGraphDestroy(g->graph);
+      delete g;
+      throw;
+    }
+    *out_graph = g;
+  });
+}
+
+int ksj_gpu_graph_launch(ksj_gpu_graph* graph, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(graph && graph->exec, "null graph");
+    KSJ_HIP(hipGraphLaunch(graph->exec, as_stream(stream)));
+  });
+}
+
+void ksj_gpu_graph_destroy(ksj_gpu_graph* graph) {
+  if (!graph) return;
+  if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+  if (graph->graph) (void)hipGraphDestroy(graph->graph);
+  delete graph;
 }
 
 int ksj_gpu_serve_create(const ksj_gpu_serve_config* config, ksj_gpu_server** out_server) {

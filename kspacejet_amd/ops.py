@@ -303,3 +303,71 @@ class Server:
 
     def __del__(self):
         self.close()
+
+
+def adjoint_multi_workspace_bytes(plans, channels) -> int:
+    n = len(plans)
+    P = C.c_void_p * n
+    U = C.c_uint32 * n
+    return int(load().ksj_gpu_adjoint2_multi_workspace_bytes(n, P(*[p._handle.value for p in plans]),
+                                                             U(*channels)))
+
+
+def adjoint_multi(plans, kspaces, rss=None, images=None, workspace: torch.Tensor | None = None,
+                  layout: str = "coil_last") -> None:
+    """Uneven batch (ksj_gpu_adjoint2_execute_multi): item i = plans[i] over kspaces[i] ([C_i, S_i])
+    into rss[i] ([N, N] float32) and/or images[i] (coil-last).  Outputs and the workspace are
+    supplied by the caller, so the call allocates nothing and can be captured into a Graph."""
+    n = len(plans)
+    if len(kspaces) != n:
+        raise ValueError("one k-space per plan")
+    for k in kspaces:
+        _need(k, torch.complex64, "kspace")
+    channels = [int(k.shape[0]) for k in kspaces]
+    need = adjoint_multi_workspace_bytes(plans, channels)
+    if workspace is None or workspace.numel() < need:
+        raise ValueError(f"adjoint_multi needs a workspace of {need} bytes")
+    P = C.c_void_p * n
+    U = C.c_uint32 * n
+    ip = P(*[_ptr(t) if t is not None else None for t in images]) if images is not None else None
+    rp = P(*[_ptr(t) if t is not None else None for t in rss]) if rss is not None else None
+    call("ksj_gpu_adjoint2_execute_multi", n, P(*[p._handle.value for p in plans]), P(*[_ptr(k) for k in kspaces]),
+         U(*channels), ip, LAYOUTS[layout], rp, _ptr(workspace), workspace.numel(), _stream())
+
+
+class Graph:
+    """HIP graph of the work enqueued on a torch stream inside `with Graph(stream) as g:`
+    (ksj_gpu_graph_*).  Nothing may allocate while capturing.  g.replay() relaunches it
+    on the same stream."""
+
+    def __init__(self, stream: torch.cuda.Stream):
+        if stream.cuda_stream == 0:
+            raise ValueError("capture needs a stream other than the null stream")
+        self.stream = stream
+        self._lib = load()
+        self._handle = C.c_void_p()
+        self._ctx = None
+
+    def __enter__(self):
+        self._ctx = torch.cuda.stream(self.stream)
+        self._ctx.__enter__()
+        call("ksj_gpu_graph_begin", self.stream.cuda_stream)
+        return self
+
+    def __exit__(self, *exc):
+        try:
+            call("ksj_gpu_graph_end", self.stream.cuda_stream, C.byref(self._handle))
+        finally:
+            self._ctx.__exit__(*exc)
+        return False
+
+    def replay(self) -> None:
+        call("ksj_gpu_graph_launch", self._handle, self.stream.cuda_stream)
+
+    def close(self) -> None:
+        if self._handle.value:
+            self._lib.ksj_gpu_graph_destroy(self._handle)
+            self._handle = C.c_void_p()
+
+    def __del__(self):
+        self.close()

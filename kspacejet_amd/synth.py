@@ -124,3 +124,17 @@ def golden_angle2(spokes: int, readout: int) -> np.ndarray:
     ky = (k[None, :] * np.sin(ang)[:, None]).reshape(-1)
     kx = (k[None, :] * np.cos(ang)[:, None]).reshape(-1)
     return np.stack([ky, kx], axis=1)
+
+
+UNEVEN_MATRICES = (128, 192, 256, 320)
+
+
+def uneven_items2(count: int, config: int = 3) -> list[tuple[int, int, int]]:
+    """Shapes of an uneven radial batch: item i draws, from seed scan_seed(config, i),
+    a matrix from {128, 192, 256, 320}, coils uniform in 8..32 and golden-angle spokes
+    uniform in 101..603 (readout = 2 x matrix).  Returns [(matrix, coils, spokes)]."""
+    out = []
+    for i in range(count):
+        u = uniform(scan_seed(config, i), 3, offset=1 << 24)
+        out.append((UNEVEN_MATRICES[int(u[0] * 4)], 8 + int(u[1] * 25), 101 + int(u[2] * 503)))
+    return out
