@@ -54,6 +54,9 @@ enum TuneKey : int {
   kTunePassLanes,         // streams over which the passes of a batched adjoint overlap: 1..3
   kTuneMultiLanes,        // streams over which the items of a multi-plan adjoint are spread: 1..8
   kTuneGatherCoilsPerLane,  // coils each gather lane loads per visit for 32+ coils: 1 or 2
+  kTuneGatherScheme,      // schedule of new plans: 0 = 2x2 blocks, 1 = 8x8 LDS tiles (summation order changes)
+  kTuneTileItemSamples,   // window samples per tile work item of new plans (summation order changes)
+  kTuneTileWidth,         // tile columns of new plans: 4 (16 x 4 tiles) or 8 (8 x 8) (summation order changes)
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);

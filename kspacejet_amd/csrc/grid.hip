@@ -3,12 +3,15 @@
 // prep:   k-space [C][S] x per-sample weight -> sorted rows [S][Cp] (coil fastest);
 //         staged through LDS so reads are coalesced per coil plane and each
 //         sorted row is written as whole cache lines.
-// gather: the plan's schedule of 2x2 cell blocks, cut into work items of at most
+// gather (blocks): the plan's schedule of 2x2 cell blocks, cut into work items of at most
 //         item_visits visits (so the dense k-space centre is spread over many
 //         waves); one sub-wave per item with lanes mapped to coils, accumulating
 //         the four cells in registers; blocks with several items are summed item
 //         by item by a second kernel.  Every cell is written exactly once into the
 //         coil-fastest grid [G_r][G_c][Cp].
+// gather (tiles): one workgroup per work item of a 64-cell tile; the tile's
+//         window samples are staged in LDS and every cell sums its footprint
+//         from there (below).
 #include "grid.hpp"
 
 #include <utility>
@@ -291,6 +294,283 @@ __global__ __launch_bounds__(256) void multi_reduce_kernel(GatherArgs a) {
   if (y < a.grid_rows && x < a.grid_cols) a.grid[(static_cast<uint64_t>(y) * a.grid_cols + x) * a.cp + c] = s;
 }
 
+// ----------------------------------------------------------------------------- tile gather
+// One workgroup per tile work item; a tile is 64 cells, TY = 64 / TX rows of TX
+// columns.  The item's window samples are staged in LDS kTileChunk at a time:
+// their sorted rows (coalesced float4 copies) and, per sample, the 64 weights
+// phi(row tap) * phi(column tap) of the tile's cells (0 outside the sample's
+// footprint).  Wave w owns tile rows [TY/4 w, TY/4 (w+1)) (16 cells, one register
+// accumulator each) and sweeps the window rows that reach them: every sample of
+// such a row updates all TX cells of each owned tile row it reaches, so a window
+// row is one contiguous run of the window list with no per-cell bookkeeping.
+// Narrow tiles waste fewer multiply-adds on zero weights (a sample reaches W of
+// the TX + W - 1 window columns) at the price of a wider window halo.  Lanes are
+// coils; with fewer than 64 coils the 64 / Cp lane groups take every (64 / Cp)-th
+// sample of a run and are summed at the end in a fixed butterfly order, so
+// results are deterministic.
+// Window samples staged at once: 64, or 32 for 64 coils (so four workgroups fit a CU's LDS).
+template <int CP>
+constexpr int tile_chunk() {
+  return CP >= 64 ? 32 : 64;
+}
+constexpr int kWgtStride = 68;  // floats per staged sample: 16-byte rows whose groups fall in different banks
+
+struct TileArgs {
+  const float2* sorted;        // [S][Cp]
+  const uint32_t* cell_start;  // [nbins + 1]
+  const float* taps;           // [S][2W]
+  const uint4* item_info;      // {tile, first, end window-list index, slot or ~0}
+  const uint32_t* tile_first_item;
+  const uint32_t* tile_first_slot;
+  const uint32_t* multi_tiles;
+  float2* partial;             // [slots][64][Cp]
+  float2* grid;                // [G_r][G_c][Cp]
+  uint32_t grid_rows, grid_cols, tile_cols, tile_w, cp, multi_count;
+};
+
+__device__ inline uint32_t wrap_bin(int v, int n) { return static_cast<uint32_t>(((v % n) + n) % n); }
+
+template <int W>
+__device__ inline float pick(const float (&v)[W], int i) {
+  float r = 0.f;
+#pragma unroll
+  for (int u = 0; u < W; ++u) r = i == u ? v[u] : r;
+  return r;
+}
+
+template <int CP, int W, int TX>
+__global__ __launch_bounds__(256) void tile_gather_kernel(TileArgs a) {
+  constexpr int TY = static_cast<int>(kTileCells) / TX;
+  constexpr int RW = TY / 4;                            // tile rows per wave
+  constexpr int RY = TY + W - 1, RX = TX + W - 1;       // window rows, columns
+  constexpr int NB = RY * RX;
+  constexpr int G = 64 / CP;
+  constexpr int kTileChunk = tile_chunk<CP>();
+  constexpr int PIECES = CP / 2;                        // float4 per sample row
+  constexpr int ROW_LOADS = kTileChunk * PIECES / 256;  // per thread and chunk
+  constexpr int U = RW > 2 ? 2 : 4;                     // run samples per lane group loaded before use
+  static_assert(NB < 256, "window bins are one per thread");
+  static_assert(ROW_LOADS >= 1, "at least one row piece per thread");
+  static_assert(TX % 4 == 0 && RW * TX == 16, "16 cells per wave, weights read as float4");
+  __shared__ uint32_t pos[NB + 1];
+  __shared__ uint32_t gst[NB];
+  __shared__ uint32_t kidx[kTileChunk];
+  __shared__ uint32_t kbin[kTileChunk];
+  __shared__ uint32_t wave_sum[4];
+  __shared__ __align__(16) float wgt[kTileChunk * kWgtStride];
+  __shared__ float4 rows[kTileChunk * PIECES];
+
+  const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const uint4 info = a.item_info[blockIdx.x];
+  const int gr = static_cast<int>(a.grid_rows), gc = static_cast<int>(a.grid_cols);
+  const int y0 = static_cast<int>(info.x / a.tile_cols) * TY, x0 = static_cast<int>(info.x % a.tile_cols) * TX;
+
+  // Window bins: first sorted sample and count; exclusive scan of the counts.
+  uint32_t cnt = 0;
+  if (t < static_cast<uint32_t>(NB)) {
+    const int r = static_cast<int>(t) / RX, c = static_cast<int>(t) % RX;
+    const uint64_t b = static_cast<uint64_t>(wrap_bin(y0 - W / 2 + r, gr)) * a.grid_cols + wrap_bin(x0 - W / 2 + c, gc);
+    const uint32_t s0 = a.cell_start[b];
+    cnt = a.cell_start[b + 1] - s0;
+    gst[t] = s0;
+  }
+  uint32_t inc = cnt;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t v = __shfl_up(inc, d, 64);
+    if (lane >= static_cast<uint32_t>(d)) inc += v;
+  }
+  if (lane == 63) wave_sum[wave] = inc;
+  __syncthreads();
+  uint32_t off = 0;
+  for (uint32_t w = 0; w < wave; ++w) off += wave_sum[w];
+  const uint32_t my_pos = off + inc - cnt;  // this thread's bin's first window-list index
+  if (t <= static_cast<uint32_t>(NB)) pos[t] = my_pos;  // pos[NB] = window total
+  __syncthreads();
+
+  const uint32_t end = info.z < pos[NB] ? info.z : pos[NB];
+  const uint32_t coil = lane % CP, g = lane / CP;
+  const int dyw = RW * static_cast<int>(wave);
+  float2 acc[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc[q] = make_float2(0.f, 0.f);
+
+  for (uint32_t c0 = info.y; c0 < end; c0 += kTileChunk) {
+    const uint32_t n = end - c0 < static_cast<uint32_t>(kTileChunk) ? end - c0 : kTileChunk;
+    // Each window bin writes the sorted positions of its samples in this chunk.
+    if (t < static_cast<uint32_t>(NB) && cnt > 0) {
+      const uint32_t lo = my_pos > c0 ? my_pos : c0, hi = my_pos + cnt < c0 + n ? my_pos + cnt : c0 + n;
+      for (uint32_t j = lo; j < hi; ++j) {
+        kidx[j - c0] = gst[t] + (j - my_pos);
+        kbin[j - c0] = t;
+      }
+    }
+    __syncthreads();
+    // Rows: all float4 loads in flight before the LDS writes.
+    float4 v[ROW_LOADS];
+#pragma unroll
+    for (int i = 0; i < ROW_LOADS; ++i) {
+      const uint32_t f = t + 256u * i, s = f / PIECES;
+      const uint32_t k = kidx[s < n ? s : 0];  // slots past the chunk copy its sample 0: never summed
+      v[i] = reinterpret_cast<const float4*>(a.sorted)[static_cast<uint64_t>(k) * PIECES + f % PIECES];
+    }
+    if (lane < n) {  // weights of sample `lane` (< kTileChunk) for this wave's tile rows
+      const uint32_t b = kbin[lane];
+      const int r = static_cast<int>(b) / RX, c = static_cast<int>(b) % RX;
+      const float* tp = a.taps + static_cast<uint64_t>(kidx[lane]) * (2 * W);
+      float ty[W], tx[W];
+#pragma unroll
+      for (int u = 0; u < W; ++u) {
+        ty[u] = tp[u];
+        tx[u] = tp[W + u];
+      }
+      float wx[TX];
+#pragma unroll
+      for (int dx = 0; dx < TX; ++dx) wx[dx] = pick<W>(tx, dx - c + W - 1);
+#pragma unroll
+      for (int e = 0; e < RW; ++e) {
+        const float wy = pick<W>(ty, dyw + e - r + W - 1);
+        float4* dst = reinterpret_cast<float4*>(wgt + lane * kWgtStride + (dyw + e) * TX);
+#pragma unroll
+        for (int x4 = 0; x4 < TX / 4; ++x4)
+          dst[x4] = make_float4(wy * wx[4 * x4], wy * wx[4 * x4 + 1], wy * wx[4 * x4 + 2], wy * wx[4 * x4 + 3]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < ROW_LOADS; ++i) rows[t + 256u * i] = v[i];
+    __syncthreads();
+    const float2* lrows = reinterpret_cast<const float2*>(rows);
+    // Window row dyw + i reaches owned tile row dyw + e when 0 <= i - e < W.
+#pragma unroll
+    for (int i = 0; i < RW + W - 1; ++i) {
+      const int r = dyw + i;
+      uint32_t jb = __builtin_amdgcn_readfirstlane(pos[r * RX]), je = __builtin_amdgcn_readfirstlane(pos[r * RX + RX]);
+      jb = jb > c0 ? jb - c0 : 0u;
+      je = je > c0 ? je - c0 : 0u;
+      je = je < n ? je : n;
+      for (uint32_t j = jb; j < je; j += U * G) {
+        float2 d[U];
+        float4 w[U][RW][TX / 4];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t l = j + u * G + g;
+          const uint32_t lc = l < je ? l : je - 1;
+          d[u] = lrows[lc * CP + coil];
+          const float4* wr = reinterpret_cast<const float4*>(wgt + lc * kWgtStride + dyw * TX);
+#pragma unroll
+          for (int e = 0; e < RW; ++e)
+            if (i - e >= 0 && i - e < W)
+#pragma unroll
+              for (int x4 = 0; x4 < TX / 4; ++x4) w[u][e][x4] = wr[e * (TX / 4) + x4];
+          d[u] = l < je ? d[u] : make_float2(0.f, 0.f);  // a slot past the run adds zero
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int e = 0; e < RW; ++e)
+            if (i - e >= 0 && i - e < W)
+#pragma unroll
+              for (int x4 = 0; x4 < TX / 4; ++x4) {
+                const float wv[4] = {w[u][e][x4].x, w[u][e][x4].y, w[u][e][x4].z, w[u][e][x4].w};
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                  float2& acc_c = acc[e * TX + 4 * x4 + k];
+                  acc_c.x += wv[k] * d[u].x;
+                  acc_c.y += wv[k] * d[u].y;
+                }
+              }
+      }
+    }
+    __syncthreads();
+  }
+  if constexpr (G > 1) {
+#pragma unroll
+    for (int m = CP; m < 64; m <<= 1)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        acc[q].x += __shfl_xor(acc[q].x, m, 64);
+        acc[q].y += __shfl_xor(acc[q].y, m, 64);
+      }
+  }
+  if (g != 0) return;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int dy = dyw + q / TX, dx = q % TX;
+    if (info.w == 0xFFFFFFFFu) {
+      const int y = y0 + dy, x = x0 + dx;
+      if (y < gr && x < gc) a.grid[(static_cast<uint64_t>(y) * a.grid_cols + x) * CP + coil] = acc[q];
+    } else {
+      a.partial[(static_cast<uint64_t>(info.w) * kTileCells + dy * TX + dx) * CP + coil] = acc[q];
+    }
+  }
+}
+
+// Tiles with several items: thread per (tile, cell, coil) sums the item partials in item order.
+__global__ __launch_bounds__(256) void tile_reduce_kernel(TileArgs a) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<uint64_t>(a.multi_count) * kTileCells * a.cp) return;
+  const uint32_t c = static_cast<uint32_t>(i % a.cp);
+  const uint32_t cell = static_cast<uint32_t>((i / a.cp) % kTileCells);
+  const uint32_t tile = a.multi_tiles[i / (static_cast<uint64_t>(kTileCells) * a.cp)];
+  const uint32_t s0 = a.tile_first_slot[tile];
+  const uint32_t n = a.tile_first_item[tile + 1] - a.tile_first_item[tile];
+  const float2* p = a.partial + (static_cast<uint64_t>(s0) * kTileCells + cell) * a.cp + c;
+  const uint64_t stride = static_cast<uint64_t>(kTileCells) * a.cp;
+  float2 s = make_float2(0.f, 0.f);
+  uint32_t j = 0;
+  for (; j + 8 <= n; j += 8) {
+    float2 t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = p[(j + u) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s = cadd(s, t[u]);
+  }
+  for (; j < n; ++j) s = cadd(s, p[j * stride]);
+  const uint32_t th = kTileCells / a.tile_w;
+  const uint32_t y = (tile / a.tile_cols) * th + cell / a.tile_w;
+  const uint32_t x = (tile % a.tile_cols) * a.tile_w + cell % a.tile_w;
+  if (y < a.grid_rows && x < a.grid_cols) a.grid[(static_cast<uint64_t>(y) * a.grid_cols + x) * a.cp + c] = s;
+}
+
+TileArgs tile_args(const Plan2& plan, const float2* d_sorted, uint32_t coils, float2* d_grid, float2* d_partial) {
+  TileArgs a{};
+  a.sorted = d_sorted;
+  a.cell_start = plan.cell_start;
+  a.taps = plan.taps;
+  a.item_info = plan.item_info;
+  a.tile_first_item = plan.block_first_item;
+  a.tile_first_slot = plan.block_first_slot;
+  a.multi_tiles = plan.multi_blocks;
+  a.partial = d_partial;
+  a.grid = d_grid;
+  a.grid_rows = plan.d.grid_rows;
+  a.grid_cols = plan.d.grid_cols;
+  a.tile_cols = plan.tile_cols;
+  a.tile_w = plan.tile_width;
+  a.cp = padded_coils(coils);
+  a.multi_count = plan.multi_count;
+  return a;
+}
+
+template <int CP, int TX>
+void launch_tile_gather(const TileArgs& a, uint32_t width, uint32_t items, hipStream_t stream) {
+  const dim3 grid(items), block(256);
+  switch (width) {
+    case 2: hipLaunchKernelGGL((tile_gather_kernel<CP, 2, TX>), grid, block, 0, stream, a); break;
+    case 4: hipLaunchKernelGGL((tile_gather_kernel<CP, 4, TX>), grid, block, 0, stream, a); break;
+    case 6: hipLaunchKernelGGL((tile_gather_kernel<CP, 6, TX>), grid, block, 0, stream, a); break;
+    case 8: hipLaunchKernelGGL((tile_gather_kernel<CP, 8, TX>), grid, block, 0, stream, a); break;
+    default: throw Error(1, "kernel width must be 2, 4, 6 or 8");
+  }
+  KSJ_LAUNCH_CHECK();
+}
+
+template <int CP>
+void launch_tile_gather(const TileArgs& a, uint32_t width, uint32_t items, hipStream_t stream) {
+  if (a.tile_w == 4) return launch_tile_gather<CP, 4>(a, width, items, stream);
+  return launch_tile_gather<CP, 8>(a, width, items, stream);
+}
+
 template <int SW, int CPL>
 void launch_gather(GatherArgs a, hipStream_t stream) {
   constexpr int NSW = 64 / SW;
@@ -318,7 +598,8 @@ uint32_t padded_coils(uint32_t coils) {
 }
 
 size_t grid_partial_bytes(const Plan2& plan, uint32_t coils) {
-  return static_cast<size_t>(plan.slot_count) * 4 * padded_coils(coils) * sizeof(float2);
+  const size_t cells = plan.scheme == kSchemeTiles ? kTileCells : 4;
+  return static_cast<size_t>(plan.slot_count) * cells * padded_coils(coils) * sizeof(float2);
 }
 
 void grid_prep(const Plan2& plan, const CoilPlanes& planes, float2* d_sorted, hipStream_t stream) {
@@ -367,6 +648,14 @@ GatherArgs gather_args(const Plan2& plan, const float2* d_sorted, uint32_t coils
 
 void grid_reduce(const Plan2& plan, uint32_t coils, float2* d_grid, const float2* d_partial, hipStream_t stream) {
   if (plan.multi_count == 0) return;
+  if (plan.scheme == kSchemeTiles) {
+    const TileArgs t = tile_args(plan, nullptr, coils, d_grid, const_cast<float2*>(d_partial));
+    hipLaunchKernelGGL(tile_reduce_kernel,
+                       dim3(ceil_div(static_cast<uint64_t>(t.multi_count) * kTileCells * t.cp, 256)), dim3(256), 0,
+                       stream, t);
+    KSJ_LAUNCH_CHECK();
+    return;
+  }
   const GatherArgs a = gather_args(plan, nullptr, coils, d_grid, const_cast<float2*>(d_partial));
   hipLaunchKernelGGL(multi_reduce_kernel, dim3(ceil_div(static_cast<uint64_t>(a.multi_count) * 4 * a.cp, 256)),
                      dim3(256), 0, stream, a);
@@ -377,6 +666,16 @@ void grid_gather(const Plan2& plan, const float2* d_sorted, uint32_t coils, floa
                  hipStream_t stream) {
   require(plan.has_trajectory, "plan has no trajectory");
   require(plan.multi_count == 0 || d_partial != nullptr, "multi-item blocks need a partial-sum buffer");
+  if (plan.scheme == kSchemeTiles) {
+    const TileArgs t = tile_args(plan, d_sorted, coils, d_grid, d_partial);
+    switch (t.cp) {
+      case 8: return launch_tile_gather<8>(t, plan.d.width, plan.item_count, stream);
+      case 16: return launch_tile_gather<16>(t, plan.d.width, plan.item_count, stream);
+      case 32: return launch_tile_gather<32>(t, plan.d.width, plan.item_count, stream);
+      case 64: return launch_tile_gather<64>(t, plan.d.width, plan.item_count, stream);
+      default: throw Error(1, "gridding supports at most 64 coils");
+    }
+  }
   const GatherArgs a = gather_args(plan, d_sorted, coils, d_grid, d_partial);
   // Two coils per lane from 32 coils up (sub-waves of 16 or 32 lanes keep the DPP
   // row broadcasts); one coil per lane below (tuning key gather_coils_per_lane).

@@ -182,6 +182,58 @@ __global__ void visit_fill_kernel(ScheduleArgs a) {
   }
 }
 
+// ----------------------------------------------------------------------------- tile schedule
+// Tile t's window: bin rows h ty - W/2 + r and columns w tx - W/2 + c (wrapped),
+// r in [0, h + W - 1), c in [0, w + W - 1).  Work items cut its window list into runs of item_samples.
+struct TileSchedArgs {
+  const uint32_t* start;
+  uint32_t grid_rows, grid_cols, tile_cols, ntiles, item_samples;
+  int width, tile_w, tile_h;
+  uint32_t* tile_samples;
+  uint32_t* first;    // items per tile, then (scanned) first item
+  uint32_t* multi;    // tiles with more than one item
+  uint32_t* counter;
+};
+
+__device__ inline uint32_t wrap_index(int v, int n) { return static_cast<uint32_t>(((v % n) + n) % n); }
+
+__global__ void tile_count_kernel(TileSchedArgs a) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > a.ntiles) return;
+  if (t == a.ntiles) {
+    a.first[t] = 0;
+    return;
+  }
+  const int W = a.width, RY = a.tile_h + W - 1, RX = a.tile_w + W - 1;
+  const int y0 = static_cast<int>(t / a.tile_cols) * a.tile_h - W / 2;
+  const int x0 = static_cast<int>(t % a.tile_cols) * a.tile_w - W / 2;
+  uint32_t n = 0;
+  for (int r = 0; r < RY; ++r) {
+    const uint64_t row = wrap_index(y0 + r, static_cast<int>(a.grid_rows));
+    for (int c = 0; c < RX; ++c) {
+      const uint64_t b = row * a.grid_cols + wrap_index(x0 + c, static_cast<int>(a.grid_cols));
+      n += a.start[b + 1] - a.start[b];
+    }
+  }
+  a.tile_samples[t] = n;
+  const uint32_t items = n == 0 ? 1u : (n + a.item_samples - 1) / a.item_samples;
+  a.first[t] = items;
+  if (items > 1) a.multi[atomicAdd(a.counter, 1u)] = t;
+}
+
+__global__ void tile_item_fill_kernel(const uint32_t* first, const uint32_t* slot, const uint32_t* tile_samples,
+                                      uint32_t ntiles, uint32_t item_samples, uint4* info) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const uint32_t n = tile_samples[t];
+  const uint32_t items = first[t + 1] - first[t];
+  for (uint32_t j = 0; j < items; ++j) {
+    const uint32_t b = j * item_samples;
+    const uint32_t e = b + item_samples < n ? b + item_samples : n;
+    info[first[t] + j] = make_uint4(t, b, e, items > 1 ? slot[t] + j : 0xFFFFFFFFu);
+  }
+}
+
 __global__ void item_count_kernel(const uint32_t* row_off, uint64_t nblocks, int width, uint32_t item_visits,
                                   uint32_t* first, uint32_t* multi, uint32_t* counter) {
   const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -303,6 +355,73 @@ template <typename T> T* alloc(size_t count, size_t* total) {
   return p;
 }
 
+// Grows the item arrays (info, cost prefix and its scan scratch) to hold `items`.
+void reserve_items(Plan2* p, uint32_t items) {
+  if (items <= p->item_capacity && p->item_cost != nullptr) return;
+  for (void** q : {reinterpret_cast<void**>(&p->item_info), reinterpret_cast<void**>(&p->item_cost),
+                   &p->item_scratch}) {
+    if (*q) KSJ_HIP(hipFree(*q));
+    *q = nullptr;
+  }
+  p->item_capacity = std::max(items, p->item_capacity);
+  KSJ_HIP(hipMalloc(&p->item_info, std::max<size_t>(1, p->item_capacity) * sizeof(uint4)));
+  KSJ_HIP(hipMalloc(&p->item_cost, (static_cast<size_t>(p->item_capacity) + 1) * sizeof(uint32_t)));
+  KSJ_HIP(hipMalloc(&p->item_scratch, std::max<size_t>(256, scan_scratch_bytes(p->item_capacity + 1ull))));
+}
+
+// Cost prefix of the items and the eight equal-cost XCD ranges.
+void xcd_ranges(Plan2* p, hipStream_t stream) {
+  const uint32_t items = p->item_count;
+  hipLaunchKernelGGL(item_cost_kernel, dim3(ceil_div(items + 1ull, 256)), dim3(256), 0, stream, p->item_info, items,
+                     p->item_cost);
+  KSJ_LAUNCH_CHECK();
+  exclusive_scan_u32(p->item_cost, p->item_cost, items + 1ull, p->item_scratch, stream);
+  hipLaunchKernelGGL(xcd_split_kernel, dim3(1), dim3(64), 0, stream, p->item_cost, items, p->xcd_first);
+  KSJ_LAUNCH_CHECK();
+}
+
+// Tile schedule: window-list length per tile -> items per tile -> item records.
+void tile_schedule(Plan2* p, hipStream_t stream) {
+  TileSchedArgs ta{};
+  ta.start = p->cell_start;
+  ta.grid_rows = p->d.grid_rows;
+  ta.grid_cols = p->d.grid_cols;
+  ta.tile_cols = p->tile_cols;
+  ta.ntiles = p->ntiles;
+  ta.item_samples = p->item_samples;
+  ta.width = static_cast<int>(p->d.width);
+  ta.tile_w = static_cast<int>(p->tile_width);
+  ta.tile_h = static_cast<int>(p->tile_height);
+  ta.tile_samples = p->tile_samples;
+  ta.first = p->block_first_item;
+  ta.multi = p->multi_blocks;
+  ta.counter = p->counters;
+  KSJ_HIP(hipMemsetAsync(p->counters, 0, sizeof(uint32_t), stream));
+  hipLaunchKernelGGL(tile_count_kernel, dim3(ceil_div(p->ntiles + 1, 256)), dim3(256), 0, stream, ta);
+  KSJ_LAUNCH_CHECK();
+  exclusive_scan_u32(p->block_first_item, p->block_first_item, p->ntiles + 1ull, p->sort_scratch, stream);
+  uint32_t items = 0, multi = 0;
+  KSJ_HIP(hipMemcpyAsync(&items, p->block_first_item + p->ntiles, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  KSJ_HIP(hipMemcpyAsync(&multi, p->counters, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  KSJ_HIP(hipStreamSynchronize(stream));
+  reserve_items(p, items);
+  p->item_count = items;
+  hipLaunchKernelGGL(slot_count_kernel, dim3(ceil_div(p->ntiles + 1, 256)), dim3(256), 0, stream,
+                     p->block_first_item, static_cast<uint64_t>(p->ntiles), p->block_first_slot);
+  KSJ_LAUNCH_CHECK();
+  exclusive_scan_u32(p->block_first_slot, p->block_first_slot, p->ntiles + 1ull, p->sort_scratch, stream);
+  uint32_t slots = 0;
+  KSJ_HIP(hipMemcpyAsync(&slots, p->block_first_slot + p->ntiles, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  hipLaunchKernelGGL(tile_item_fill_kernel, dim3(ceil_div(p->ntiles, 256)), dim3(256), 0, stream,
+                     p->block_first_item, p->block_first_slot, p->tile_samples, p->ntiles, p->item_samples,
+                     p->item_info);
+  KSJ_LAUNCH_CHECK();
+  xcd_ranges(p, stream);
+  KSJ_HIP(hipStreamSynchronize(stream));
+  p->slot_count = slots;
+  p->multi_count = multi;
+}
+
 }  // namespace
 
 Plan2* plan2_create(const Plan2Desc& desc) {
@@ -318,6 +437,11 @@ Plan2* plan2_create(const Plan2Desc& desc) {
   require(desc.item_visits >= 1 && desc.item_visits <= 4096, "item visits must be in [1, 4096]");
   auto* p = new Plan2();
   p->d = desc;
+  if (p->d.scheme == kSchemeDefault) p->d.scheme = static_cast<uint32_t>(tune_get(kTuneGatherScheme));
+  if (p->d.item_samples == 0) p->d.item_samples = static_cast<uint32_t>(tune_get(kTuneTileItemSamples));
+  if (p->d.tile_width == 0) p->d.tile_width = static_cast<uint32_t>(tune_get(kTuneTileWidth));
+  require(p->d.tile_width == 4 || p->d.tile_width == 8, "tile width must be 4 or 8");
+  require(p->d.scheme == kSchemeBlocks || p->d.scheme == kSchemeTiles, "unknown gather schedule");
   if (p->d.kernel == kKernelKaiserBessel && p->d.beta <= 0.0) {
     const double sigma = static_cast<double>(desc.grid_rows) / desc.image_rows;
     const double a = static_cast<double>(desc.width) / sigma * (sigma - 0.5);
@@ -333,6 +457,13 @@ Plan2* plan2_create(const Plan2Desc& desc) {
     p->block_cols = (desc.grid_cols + 1) / 2;
     p->nblocks = static_cast<uint64_t>(p->block_rows) * p->block_cols;
     p->item_visits = desc.item_visits;
+    p->scheme = p->d.scheme;
+    p->item_samples = p->d.item_samples;
+    p->tile_width = p->d.tile_width;
+    p->tile_height = kTileCells / p->tile_width;
+    p->tile_rows = ceil_div(desc.grid_rows, p->tile_height);
+    p->tile_cols = ceil_div(desc.grid_cols, p->tile_width);
+    p->ntiles = p->tile_rows * p->tile_cols;
     size_t total = 0;
     p->weight = alloc<float2>(S, &total);
     p->rank = alloc<uint32_t>(S, &total);
@@ -340,7 +471,10 @@ Plan2* plan2_create(const Plan2Desc& desc) {
     p->bins = alloc<uint32_t>(S, &total);
     p->cell_start = alloc<uint32_t>(p->nbins + 1, &total);
     p->taps = alloc<float>(S * 2 * desc.width, &total);
-    p->visit_row_off = alloc<uint32_t>(p->nblocks * (desc.width + 1) + 1, &total);
+    if (p->scheme == kSchemeBlocks)
+      p->visit_row_off = alloc<uint32_t>(p->nblocks * (desc.width + 1) + 1, &total);
+    else
+      p->tile_samples = alloc<uint32_t>(p->ntiles, &total);
     p->block_first_item = alloc<uint32_t>(p->nblocks + 1, &total);
     p->block_first_slot = alloc<uint32_t>(p->nblocks + 1, &total);
     p->multi_blocks = alloc<uint32_t>(p->nblocks, &total);
@@ -376,7 +510,7 @@ void plan2_destroy(Plan2* p) {
                   p->taps,          p->visit_row_off, p->visit_k,    p->visit_w,      p->block_first_item,
                   p->block_first_slot, p->item_info,    p->multi_blocks, p->frac,         p->sorted_keys,
                   p->counters,      p->sort_scratch, p->deapod_rows, p->deapod_cols, p->item_cost,
-                  p->item_scratch,  p->xcd_first};
+                  p->item_scratch,  p->xcd_first,   p->tile_samples};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -425,6 +559,11 @@ void plan2_set_trajectory(Plan2* p, const float* d_traj, const float* d_dcf, hip
   if (host_counters[1] & kErrBadCoordinate)
     throw Error(1, "trajectory must be finite and within [-pi, pi] radians per pixel");
 
+  if (p->scheme == kSchemeTiles) {
+    tile_schedule(p, stream);
+    p->has_trajectory = true;
+    return;
+  }
   // Gather schedule: visits per (block, window row) -> offsets -> visit records.
   ScheduleArgs sa{};
   sa.start = p->cell_start;
@@ -468,17 +607,7 @@ void plan2_set_trajectory(Plan2* p, const float* d_traj, const float* d_dcf, hip
   KSJ_HIP(hipMemcpyAsync(&items, p->block_first_item + p->nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
   KSJ_HIP(hipMemcpyAsync(&multi, p->counters, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
   KSJ_HIP(hipStreamSynchronize(stream));
-  if (items > p->item_capacity || p->item_cost == nullptr) {
-    for (void** q : {reinterpret_cast<void**>(&p->item_info), reinterpret_cast<void**>(&p->item_cost),
-                     &p->item_scratch}) {
-      if (*q) KSJ_HIP(hipFree(*q));
-      *q = nullptr;
-    }
-    p->item_capacity = std::max(items, p->item_capacity);
-    KSJ_HIP(hipMalloc(&p->item_info, std::max<size_t>(1, p->item_capacity) * sizeof(uint4)));
-    KSJ_HIP(hipMalloc(&p->item_cost, (static_cast<size_t>(p->item_capacity) + 1) * sizeof(uint32_t)));
-    KSJ_HIP(hipMalloc(&p->item_scratch, std::max<size_t>(256, scan_scratch_bytes(p->item_capacity + 1ull))));
-  }
+  reserve_items(p, items);
   p->item_count = items;
   // Partial-sum slots: consecutive for the items of each multi-item block.
   hipLaunchKernelGGL(slot_count_kernel, dim3(ceil_div(p->nblocks + 1, 256)), dim3(256), 0, stream,
@@ -491,12 +620,7 @@ void plan2_set_trajectory(Plan2* p, const float* d_traj, const float* d_dcf, hip
                      p->block_first_slot, p->visit_row_off, static_cast<int>(p->d.width), p->item_visits, p->nblocks,
                      p->item_info);
   KSJ_LAUNCH_CHECK();
-  hipLaunchKernelGGL(item_cost_kernel, dim3(ceil_div(items + 1ull, 256)), dim3(256), 0, stream, p->item_info, items,
-                     p->item_cost);
-  KSJ_LAUNCH_CHECK();
-  exclusive_scan_u32(p->item_cost, p->item_cost, items + 1ull, p->item_scratch, stream);
-  hipLaunchKernelGGL(xcd_split_kernel, dim3(1), dim3(64), 0, stream, p->item_cost, items, p->xcd_first);
-  KSJ_LAUNCH_CHECK();
+  xcd_ranges(p, stream);
   KSJ_HIP(hipStreamSynchronize(stream));
   p->slot_count = slots;
   // The multi-item list is in append order; each of its blocks is summed on its

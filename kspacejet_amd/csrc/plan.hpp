@@ -11,8 +11,10 @@
 //                s = (centred ? N/2 : 0) - origin, evaluated in FP64.
 //   taps         per sorted sample W row taps and W column taps of the kernel
 //                (Kaiser-Bessel or linear), evaluated in FP64, stored as float.
-//   schedule     per 2x2 cell block, the list of (sample, four weights) visits,
-//                cut into equal work items so dense k-space centres are split.
+//   schedule     either (blocks) per 2x2 cell block the list of (sample, four
+//                weights) visits, or (tiles) per 8x8 cell tile the length of its
+//                window sample list; both cut into bounded work items so dense
+//                k-space centres are split over many workgroups.
 //   deapod       1 / phi_hat per output row / column (Kaiser-Bessel only), FP64.
 #pragma once
 
@@ -24,6 +26,11 @@ enum : uint32_t { kKernelLinear = 0, kKernelKaiserBessel = 1 };
 // XCD count of the MI355X: the gather can give each XCD a contiguous, cost-balanced item range.
 constexpr uint32_t kPlanXcds = 8;
 enum : uint32_t { kDensityNone = 0, kDensityRamp = 1, kDensityGiven = 2 };
+// Gather schedules: 2x2 blocks with per-visit weights read from HBM, or 64-cell
+// tiles (64/w rows x w columns) whose window samples are staged in LDS.
+// kSchemeDefault takes tuning key gather_scheme.
+enum : uint32_t { kSchemeBlocks = 0, kSchemeTiles = 1, kSchemeDefault = 0xFFFFFFFFu };
+constexpr uint32_t kTileCells = 64;
 
 struct Plan2Desc {
   uint32_t image_rows = 0, image_cols = 0;  // N
@@ -35,7 +42,10 @@ struct Plan2Desc {
   uint32_t density = kDensityRamp;
   uint32_t centred = 1;                     // 1: image pixel r reads bin (r - N/2) mod G; 0: bin r
   uint32_t samples = 0;
-  uint32_t item_visits = 64;                // visits per gather work item (one wave)
+  uint32_t item_visits = 64;                // blocks: visits per gather work item (one wave)
+  uint32_t scheme = kSchemeDefault;
+  uint32_t item_samples = 0;                // tiles: window samples per work item (0: tuning key)
+  uint32_t tile_width = 0;                  // tiles: 4 or 8 columns (0: tuning key)
 };
 
 struct Plan2 {
@@ -75,6 +85,16 @@ struct Plan2 {
   uint32_t* xcd_first = nullptr;         // [kPlanXcds + 1] first item of each XCD's contiguous range
   uint32_t* multi_blocks = nullptr;      // [nblocks] blocks with more than one item
   uint32_t multi_count = 0;              // host copy
+  // Tile schedule (scheme == kSchemeTiles).  Tile (ty, tx) covers cells
+  // [h ty, h ty + h) x [w tx, w tx + w) with w = tile_width, h = 64 / w; its window
+  // is the (h+W-1) x (w+W-1) bins whose samples reach those cells, rows then
+  // columns, sorted order inside a bin.  Work item = {tile, first, end
+  // window-list index, slot or ~0}, at most item_samples long; block_first_item /
+  // block_first_slot / multi_blocks are indexed by tile.
+  uint32_t scheme = kSchemeBlocks;
+  uint32_t tile_width = 8, tile_height = 8;
+  uint32_t tile_rows = 0, tile_cols = 0, ntiles = 0, item_samples = 0;
+  uint32_t* tile_samples = nullptr;      // [ntiles] window-list length
   float* deapod_rows = nullptr;    // [N_r] or null
   float* deapod_cols = nullptr;    // [N_c] or null
   // scratch used only while building

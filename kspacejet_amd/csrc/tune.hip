@@ -24,6 +24,9 @@ Param g_params[kTuneKeys] = {
     {"pass_lanes", 2, [](int64_t v) { return v >= 1 && v <= 3; }},
     {"multi_lanes", 4, [](int64_t v) { return v >= 1 && v <= 8; }},
     {"gather_coils_per_lane", 1, [](int64_t v) { return v == 1 || v == 2; }},
+    {"gather_scheme", 0, [](int64_t v) { return v == 0 || v == 1; }},
+    {"tile_item_samples", 256, [](int64_t v) { return v >= 1 && v <= 65536; }},
+    {"tile_width", 4, [](int64_t v) { return v == 4 || v == 8; }},
 };
 
 }  // namespace
