@@ -57,6 +57,7 @@ enum TuneKey : int {
   kTuneGatherScheme,      // schedule of new plans: 0 = 2x2 blocks, 1 = 8x8 LDS tiles (summation order changes)
   kTuneTileItemSamples,   // window samples per tile work item of new plans (summation order changes)
   kTuneTileWidth,         // tile columns of new plans: 4 (16 x 4 tiles) or 8 (8 x 8) (summation order changes)
+  kTuneBlockGroup,        // 2x2-block group side of new plans: 1, 2 or 4 (work-item order; results unchanged)
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);

@@ -87,7 +87,7 @@ struct GatherArgs {
   float2* grid;                     // [G_r][G_c][Cp]
   const uint32_t* xcd_first;        // [9] cost-balanced XCD item ranges (gather_map 2)
   uint32_t xcd_chunk, wg_per_xcd;   // workgroup -> item range mapping (gather_map)
-  uint32_t grid_rows, grid_cols, block_cols, cp, item_count, multi_count;
+  uint32_t grid_rows, grid_cols, block_cols, block_group, cp, item_count, multi_count;
 };
 
 __device__ inline void fma4(float2 (&acc)[4], float4 w, float2 d) {
@@ -104,18 +104,29 @@ __device__ inline void fma4(float2 (&acc)[4], float4 w, float2 d) {
 // One sub-wave of SW lanes (lane = coil) per work item, 64/SW items per wave.  The
 // sub-wave walks its item's visits in batches of B and issues all B sample-row
 // loads (SW consecutive coils each) before the first use.  The next batch's
-// positions are loaded right after this batch's are distributed, so their latency
-// overlaps this batch's row loads.  Sums run in visit order.
+// positions and weights are loaded right after this batch's are distributed, so
+// their latency overlaps this batch's row loads and the row loads never wait for
+// metadata.  Sums run in visit order.
 //
 // Per-visit metadata is loaded once per 16-lane row and distributed inside the row:
-// lane r of every row loads position r and weight floats 16j + r of the batch, and
-// DPP row_newbcast (broadcast of lane K of each 16-lane row) hands them to every lane.
+// lane r of every row loads position r and the weights (float4) of visit r % B of
+// the batch, and DPP row_newbcast (broadcast of lane K of each 16-lane row) hands
+// them to every lane.
 // A broadcast float4 load per visit would move 1 KiB through the texture units for
 // 16 bytes of data -- twice the bytes of the sample row itself.  Sub-waves of 8 lanes
 // (two per row) fall back to cross-lane shuffles.
+// Visit index v + r clamped into the item (an empty item reads visit 0 of the
+// plan's arrays, which hold at least one entry): the metadata loads are
+// unconditional, and values past the item are never used -- their rows are not
+// loaded and their sample values are zero.
+__device__ inline uint32_t clamp_visit(uint32_t v, uint32_t end) {
+  const uint32_t last = end > 0 ? end - 1 : 0;
+  return v < last ? v : last;
+}
+
 template <int B>
 __device__ inline uint32_t load_positions(const GatherArgs& a, uint32_t v, uint32_t end, int r) {
-  return (r < B && v + r < end) ? a.visit_k[v + r] : 0u;
+  return a.visit_k[clamp_visit(v + static_cast<uint32_t>(r), end)];
 }
 
 template <int K>
@@ -148,9 +159,16 @@ struct CoilVec<2> {
   }
 };
 
+// Weights of the batch starting at visit v, as lane r of every 16-lane row holds
+// them (visit v + r % B).
+template <int B>
+__device__ inline float4 load_weights(const GatherArgs& a, uint32_t v, uint32_t end, int r) {
+  return a.visit_w[clamp_visit(v + static_cast<uint32_t>(r % B), end)];
+}
+
 template <int SW, int B, int CPL, bool TAIL>
-__device__ inline void gather_batch(const GatherArgs& a, float2 (&acc)[4][CPL], uint32_t& kj, uint32_t v,
-                                    uint32_t end, int sw, int c) {
+__device__ inline void gather_batch(const GatherArgs& a, float2 (&acc)[4][CPL], uint32_t& kj, float4& wl,
+                                    uint32_t v, uint32_t end, int sw, int c) {
   using V = typename CoilVec<CPL>::T;
   const V* rows = reinterpret_cast<const V*>(a.sorted) + c;
   const uint32_t rstride = a.cp / CPL;  // row length in V elements
@@ -158,15 +176,13 @@ __device__ inline void gather_batch(const GatherArgs& a, float2 (&acc)[4][CPL], 
   uint32_t k[B];
   float4 w[B];
   if constexpr (SW >= 16) {
-    // Lane r of every 16-lane row loads visit r % B's weights (one float4) and, for
-    // r < B, its position; DPP row broadcasts hand them to every lane of the row.
+    // Lane r of every 16-lane row holds visit r % B's weights (one float4) and, for
+    // r < B, its position, both loaded one batch ahead; DPP row broadcasts hand
+    // them to every lane of the row, then the next batch's are loaded.
     const int r = c & 15;
-    const uint32_t vi = v + (r % B);
-    const float4 wl = (!TAIL || vi < end) ? a.visit_w[vi] : make_float4(0.f, 0.f, 0.f, 0.f);
     [&]<int... I>(std::integer_sequence<int, I...>) {
       ((k[I] = row_bcast<I>(kj)), ...);
     }(std::make_integer_sequence<int, B>{});
-    if (!TAIL) kj = load_positions<B>(a, v + B, end, r);
     [&]<int... I>(std::integer_sequence<int, I...>) {
       ((w[I] = make_float4(__uint_as_float(row_bcast<I>(__float_as_uint(wl.x))),
                            __uint_as_float(row_bcast<I>(__float_as_uint(wl.y))),
@@ -174,6 +190,10 @@ __device__ inline void gather_batch(const GatherArgs& a, float2 (&acc)[4][CPL], 
                            __uint_as_float(row_bcast<I>(__float_as_uint(wl.w))))),
        ...);
     }(std::make_integer_sequence<int, B>{});
+    if (!TAIL) {
+      kj = load_positions<B>(a, v + B, end, r);
+      wl = load_weights<B>(a, v + B, end, r);
+    }
   } else {
 #pragma unroll
     for (int i = 0; i < B; ++i) k[i] = __shfl(kj, sw * SW + i, 64);
@@ -219,11 +239,13 @@ __device__ inline void gather_one_item(const GatherArgs& a, uint32_t item, bool 
     for (int j = 0; j < CPL; ++j) acc[q][j] = make_float2(0.f, 0.f);
   uint32_t v = info.y;
   uint32_t kj = load_positions<B>(a, v, info.z, SW >= 16 ? (c & 15) : c);
-  for (; v + B <= info.z; v += B) gather_batch<SW, B, CPL, false>(a, acc, kj, v, info.z, sw, c);
-  if (v < info.z) gather_batch<SW, B, CPL, true>(a, acc, kj, v, info.z, sw, c);
+  float4 wl = SW >= 16 ? load_weights<B>(a, v, info.z, c & 15) : make_float4(0.f, 0.f, 0.f, 0.f);
+  for (; v + B <= info.z; v += B) gather_batch<SW, B, CPL, false>(a, acc, kj, wl, v, info.z, sw, c);
+  if (v < info.z) gather_batch<SW, B, CPL, true>(a, acc, kj, wl, v, info.z, sw, c);
   if (!live) return;
   using V = typename CoilVec<CPL>::T;
-  const uint32_t by = info.x / a.block_cols, bx = info.x % a.block_cols;
+  uint32_t by, bx;
+  block_coords(info.x, a.block_cols, a.block_group, &by, &bx);
   const uint32_t rstride = a.cp / CPL;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -290,7 +312,9 @@ __global__ __launch_bounds__(256) void multi_reduce_kernel(GatherArgs a) {
     for (int u = 0; u < 8; ++u) s = cadd(s, t[u]);
   }
   for (; j < n; ++j) s = cadd(s, p[j * stride]);
-  const uint32_t y = 2 * (b / a.block_cols) + q / 2, x = 2 * (b % a.block_cols) + q % 2;
+  uint32_t by, bx;
+  block_coords(b, a.block_cols, a.block_group, &by, &bx);
+  const uint32_t y = 2 * by + q / 2, x = 2 * bx + q % 2;
   if (y < a.grid_rows && x < a.grid_cols) a.grid[(static_cast<uint64_t>(y) * a.grid_cols + x) * a.cp + c] = s;
 }
 
@@ -638,6 +662,7 @@ GatherArgs gather_args(const Plan2& plan, const float2* d_sorted, uint32_t coils
   a.grid_rows = plan.d.grid_rows;
   a.grid_cols = plan.d.grid_cols;
   a.block_cols = plan.block_cols;
+  a.block_group = plan.block_group;
   a.cp = padded_coils(coils);
   a.item_count = plan.item_count;
   a.multi_count = plan.multi_count;

@@ -124,7 +124,7 @@ __global__ void taps_kernel(TapArgs a) {
 struct ScheduleArgs {
   const uint32_t* start;
   const float* taps;  // [S][2W]
-  uint32_t grid_rows, grid_cols, block_cols;
+  uint32_t grid_rows, grid_cols, block_cols, block_group;
   uint64_t nblocks;
   int width;
   uint32_t* row_off;  // counts, then offsets
@@ -132,11 +132,20 @@ struct ScheduleArgs {
   float4* visit_w;
 };
 
+// False for the padding blocks past the grid (no visits).
+__device__ inline bool block_in_grid(const ScheduleArgs& a, uint64_t blk) {
+  uint32_t by, bx;
+  block_coords(blk, a.block_cols, a.block_group, &by, &bx);
+  return 2 * by < a.grid_rows && 2 * bx < a.grid_cols;
+}
+
 __device__ inline uint64_t window_bin(const ScheduleArgs& a, uint64_t blk, int r, int c) {
   const int W = a.width;
   const int gr = static_cast<int>(a.grid_rows), gc = static_cast<int>(a.grid_cols);
-  int by = 2 * static_cast<int>(blk / a.block_cols) - W / 2 + r;
-  int bx = 2 * static_cast<int>(blk % a.block_cols) - W / 2 + c;
+  uint32_t byu, bxu;
+  block_coords(blk, a.block_cols, a.block_group, &byu, &bxu);
+  int by = 2 * static_cast<int>(byu) - W / 2 + r;
+  int bx = 2 * static_cast<int>(bxu) - W / 2 + c;
   by = ((by % gr) + gr) % gr;
   bx = ((bx % gc) + gc) % gc;
   return static_cast<uint64_t>(by) * a.grid_cols + bx;
@@ -153,9 +162,11 @@ __global__ void visit_count_kernel(ScheduleArgs a) {
   const uint64_t blk = i / (a.width + 1);
   const int r = static_cast<int>(i % (a.width + 1));
   uint32_t n = 0;
-  for (int c = 0; c <= a.width; ++c) {
-    const uint64_t b = window_bin(a, blk, r, c);
-    n += a.start[b + 1] - a.start[b];
+  if (block_in_grid(a, blk)) {
+    for (int c = 0; c <= a.width; ++c) {
+      const uint64_t b = window_bin(a, blk, r, c);
+      n += a.start[b + 1] - a.start[b];
+    }
   }
   a.row_off[i] = n;
 }
@@ -166,6 +177,7 @@ __global__ void visit_fill_kernel(ScheduleArgs a) {
   const int W = a.width;
   const uint64_t blk = i / (W + 1);
   const int r = static_cast<int>(i % (W + 1));
+  if (!block_in_grid(a, blk)) return;
   uint32_t pos = a.row_off[i];
   const int ty0 = 0 - r + W - 1, ty1 = 1 - r + W - 1;
   for (int c = 0; c <= W; ++c) {
@@ -440,6 +452,8 @@ Plan2* plan2_create(const Plan2Desc& desc) {
   if (p->d.scheme == kSchemeDefault) p->d.scheme = static_cast<uint32_t>(tune_get(kTuneGatherScheme));
   if (p->d.item_samples == 0) p->d.item_samples = static_cast<uint32_t>(tune_get(kTuneTileItemSamples));
   if (p->d.tile_width == 0) p->d.tile_width = static_cast<uint32_t>(tune_get(kTuneTileWidth));
+  if (p->d.block_group == 0) p->d.block_group = static_cast<uint32_t>(tune_get(kTuneBlockGroup));
+  require(p->d.block_group == 1 || p->d.block_group == 2 || p->d.block_group == 4, "block group must be 1, 2 or 4");
   require(p->d.tile_width == 4 || p->d.tile_width == 8, "tile width must be 4 or 8");
   require(p->d.scheme == kSchemeBlocks || p->d.scheme == kSchemeTiles, "unknown gather schedule");
   if (p->d.kernel == kKernelKaiserBessel && p->d.beta <= 0.0) {
@@ -453,8 +467,9 @@ Plan2* plan2_create(const Plan2Desc& desc) {
     p->nbins = static_cast<uint64_t>(desc.grid_rows) * desc.grid_cols;
     p->key_bits = 1;
     while ((1ull << p->key_bits) < p->nbins) ++p->key_bits;
-    p->block_rows = (desc.grid_rows + 1) / 2;
-    p->block_cols = (desc.grid_cols + 1) / 2;
+    p->block_group = p->d.block_group;
+    p->block_rows = ceil_div(ceil_div(desc.grid_rows, 2), p->block_group) * p->block_group;
+    p->block_cols = ceil_div(ceil_div(desc.grid_cols, 2), p->block_group) * p->block_group;
     p->nblocks = static_cast<uint64_t>(p->block_rows) * p->block_cols;
     p->item_visits = desc.item_visits;
     p->scheme = p->d.scheme;
@@ -571,6 +586,7 @@ void plan2_set_trajectory(Plan2* p, const float* d_traj, const float* d_dcf, hip
   sa.grid_rows = p->d.grid_rows;
   sa.grid_cols = p->d.grid_cols;
   sa.block_cols = p->block_cols;
+  sa.block_group = p->block_group;
   sa.nblocks = p->nblocks;
   sa.width = static_cast<int>(p->d.width);
   sa.row_off = p->visit_row_off;

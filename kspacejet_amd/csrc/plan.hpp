@@ -32,6 +32,20 @@ enum : uint32_t { kDensityNone = 0, kDensityRamp = 1, kDensityGiven = 2 };
 enum : uint32_t { kSchemeBlocks = 0, kSchemeTiles = 1, kSchemeDefault = 0xFFFFFFFFu };
 constexpr uint32_t kTileCells = 64;
 
+// Block numbering: blocks come in groups of g x g (row-major inside a group,
+// groups row-major), so consecutive work items -- which one workgroup takes
+// together -- cover a square of neighbouring blocks whose windows share sample
+// rows in the CU's L1.  block_rows / block_cols are padded to whole groups; the
+// blocks past the grid get no visits and write nothing.  g = 1 is plain row-major.
+__host__ __device__ inline void block_coords(uint64_t b, uint32_t block_cols, uint32_t g, uint32_t* by,
+                                             uint32_t* bx) {
+  const uint64_t q = b / (g * g);
+  const uint32_t s = static_cast<uint32_t>(b % (g * g));
+  const uint32_t groups_per_row = block_cols / g;
+  *by = g * static_cast<uint32_t>(q / groups_per_row) + s / g;
+  *bx = g * static_cast<uint32_t>(q % groups_per_row) + s % g;
+}
+
 struct Plan2Desc {
   uint32_t image_rows = 0, image_cols = 0;  // N
   uint32_t grid_rows = 0, grid_cols = 0;    // G
@@ -46,6 +60,7 @@ struct Plan2Desc {
   uint32_t scheme = kSchemeDefault;
   uint32_t item_samples = 0;                // tiles: window samples per work item (0: tuning key)
   uint32_t tile_width = 0;                  // tiles: 4 or 8 columns (0: tuning key)
+  uint32_t block_group = 0;                 // blocks: group side 1, 2 or 4 (0: tuning key)
 };
 
 struct Plan2 {
@@ -66,7 +81,7 @@ struct Plan2 {
   // phi(row tap) * phi(column tap) of the block's cells (0 outside the footprint).
   // Blocks are cut into work items of at most item_visits visits; a block with
   // several items is summed item by item in item order.
-  uint32_t block_rows = 0, block_cols = 0;
+  uint32_t block_rows = 0, block_cols = 0, block_group = 1;  // rows / columns padded to whole groups
   uint64_t nblocks = 0;
   uint32_t item_visits = 64;
   uint32_t* visit_row_off = nullptr;  // [nblocks * (W+1) + 1] first visit of each (block, window row)

@@ -56,12 +56,26 @@ def test_launch_parameters_do_not_change_results(ops, tune):
     traj, k = scan(ops)
     base_img, base_rss = recon(ops, traj, k, n)
     for key, values in (("gather_map", (0, 1, 2)), ("gather_batch", (4, 8, 16)), ("fft_lines_per_block", (0, 4, 16)),
-                        ("gather_coils_per_lane", (1, 2))):
+                        ("gather_coils_per_lane", (1, 2)), ("block_group", (1, 2, 4))):
         for v in values:
             tune(key, v)
             img, rss = recon(ops, traj, k, n)
             assert np.array_equal(img, base_img) and np.array_equal(rss, base_rss), (key, v)
         tune(key, values[0])
+
+
+def test_block_groups_pad_grids_without_changing_results(ops, tune):
+    """Block groups pad the block rows and columns (a 100-point grid has 50 blocks a
+    side, 52 with groups of 4); the padding blocks write nothing and every cell is
+    summed as with row-major blocks."""
+    n = 50
+    traj, k = scan(ops, n=n, coils=5, spokes=37, readout=100, seed=4)
+    results = []
+    for g in (1, 2, 4):
+        tune("block_group", g)
+        results.append(recon(ops, traj, k, n))
+    for img, rss in results[1:]:
+        assert np.array_equal(img, results[0][0]) and np.array_equal(rss, results[0][1])
 
 
 def test_pass_lanes_do_not_change_batched_results(ops, tune):
