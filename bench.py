@@ -14,7 +14,12 @@ image.  `value` is served images per second over the timed steps.  Beside it the
   * latency: p50/p99 submit-to-completion latency of an open-loop Poisson stream of
     the same scans at `--load` x the measured served rate;
   * plain: the same scans through the per-scan operator (ksj_gpu_adjoint2_execute,
-    one synchronous call per scan), images/s and ms per scan;
+    one synchronous call per scan, data resident in HBM), images/s and ms per scan;
+  * plain_provider: the drop-in Provider operator (gpu-recon kb_gridding_reconstruct_rss
+    through ksj_provider_query) fed host-pageable payloads as the reference's host
+    feeds them, one synchronous firing per scan (PCIe staging included);
+  * uneven: 64 seeded items of mixed matrix / coil / spoke counts, plain per item vs
+    one autotuned multi-plan call vs that call replayed as a captured HIP graph;
   * stages / roofline: per-kernel HIP-event timings of the batched pass (one pass
     lane, so kernels do not overlap) and the gather kernel against HBM bandwidth;
   * mixed: a mixed 2-D stream (radial scans of the headline shape and Cartesian
@@ -133,6 +138,11 @@ def main(argv=None) -> int:
     ap.add_argument("--latency-requests", type=int, default=300)
     ap.add_argument("--mixed-requests", type=int, default=48, help="0 skips the mixed-stream section")
     ap.add_argument("--uneven-items", type=int, default=64, help="items of the uneven batch; 0 skips that section")
+    ap.add_argument("--volume-coils", type=int, default=20,
+                    help="coils of the 3-D kooshball volume (256^3, 16384 x 512 spokes); 0 skips that section")
+    ap.add_argument("--volume-coils-per-pass", type=int, default=8)
+    ap.add_argument("--provider-firings", type=int, default=40,
+                    help="firings of the drop-in Provider operator (host-fed plain form); 0 skips it")
     ap.add_argument("--tune-seconds", type=float, default=20.0, help="autotuner budget for the uneven batch")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -282,6 +292,15 @@ def main(argv=None) -> int:
         mixed = run_mixed(torch, lib, _lib, server, plan, scans, traj, traj_key, args.mixed_requests, stream)
     server.close()
     del scans, outs, ws
+    provider = None
+    if args.provider_firings > 0:
+        log("drop-in Provider operator ...")
+        provider = run_provider_plain(scan0, traj.cpu().numpy(), args.provider_firings, args.warmup)
+    volume = None
+    if args.volume_coils > 0:
+        log("3-D kooshball volume ...")
+        volume = run_volume(torch, ops, lib, args.volume_coils, args.volume_coils_per_pass, max(1, args.steps // 4),
+                            1)
     uneven = None
     if args.uneven_items > 0:
         log("uneven batch ...")
@@ -319,13 +338,116 @@ def main(argv=None) -> int:
         "plain": {"images_per_s": round(plain_value, 1), "ms_per_scan": round(1e3 / plain_value, 3),
                   "operator": "ksj_gpu_adjoint2_execute, one synchronous call per scan, plan resident"},
         "served_over_plain": round(value / plain_value, 2),
+        "plain_provider": provider,
+        "served_over_plain_provider": round(value / provider["images_per_s"], 2) if provider else None,
         "stages": stage_report,
         "served_stats": served_stats,
         "mixed": mixed,
         "uneven": uneven,
+        "volume": volume,
     }
     print(json.dumps(line))
     return 0
+
+
+def run_volume(torch, ops, lib, coils, coils_per_pass, steps, warmup):
+    """3-D radial (kooshball) adjoint of one 256^3 volume: 16384 golden-means spokes x
+    512 samples, KB width 4 on a 512^3 grid (sigma 2), |k|^2 density, coils in passes
+    of `coils_per_pass`, RSS volume out.  The k-space is seeded complex Gaussian noise
+    of the workload's shape (the adjoint's cost does not depend on the values; parity
+    is tested on small volumes against the FP64 oracle)."""
+    n, spokes, readout = 256, 16384, 512
+    traj = ops.golden_means3(spokes, readout)
+    S = traj.shape[0]
+    gen = torch.Generator(device="cuda").manual_seed(0x4B53504A ^ (4 << 32))
+    k = torch.randn((coils, S), dtype=torch.complex64, device="cuda", generator=gen)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan = ops.Plan3((n, n, n), S, width=WIDTH).set_trajectory(traj)
+    torch.cuda.synchronize()
+    build_ms = 1e3 * (time.perf_counter() - t0)
+    info = plan.info()
+    ws = torch.empty(plan.workspace_bytes(coils, coils_per_pass), dtype=torch.uint8, device="cuda")
+    rss = torch.empty((n, n, n), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def run():
+        rc = lib.ksj_gpu_adjoint3_execute(plan._handle, k.data_ptr(), coils, coils_per_pass, None, rss.data_ptr(),
+                                          ws.data_ptr(), ws.numel(), stream)
+        if rc != 0:
+            raise RuntimeError(lib.ksj_gpu_last_error().decode())
+
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    lib.ksj_gpu_profile_enable(1)
+    lib.ksj_gpu_profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    stages = {}
+    for name in ("grid3_prep", "grid3_gather", "grid3_reduce", "fft3_x", "fft3_y_deapod", "fft3_z_deapod",
+                 "coil_combine3"):
+        tot, cnt = C.c_double(0), C.c_uint64(0)
+        lib.ksj_gpu_profile_read(name.encode(), C.byref(tot), C.byref(cnt))
+        if cnt.value:
+            stages[name] = round(tot.value / steps, 3)  # ms per volume (the profile totals are in ms)
+    lib.ksj_gpu_profile_enable(0)
+    G = 2 * n
+    passes = -(-coils // coils_per_pass)
+    # Algorithmic bytes of the gather per volume: samples in, sorted coordinates and
+    # weights once per pass, every grid cell of every coil written once.
+    gather_bytes = 8 * S * coils + 12 * S * passes + 8 * G ** 3 * coils
+    gather_ms = stages.get("grid3_gather")
+    res = {"volumes_per_s": round(1.0 / dt, 3), "ms_per_volume": round(1e3 * dt, 2), "plan_build_ms": round(build_ms, 1),
+           "shape": [n, n, n], "grid": [G, G, G], "coils": coils, "coils_per_pass": coils_per_pass,
+           "spokes": spokes, "readout": readout, "samples": S, "kernel_width": WIDTH, "density": "|k|^2",
+           "plan_items": info["items"], "plan_visits": info["visits"], "plan_device_bytes": info["device_bytes"],
+           "workspace_bytes": ws.numel(), "stages_ms_per_volume": stages,
+           "gather_gbs": round(gather_bytes / (gather_ms * 1e-3) / 1e9, 1) if gather_ms else None,
+           "data": "seeded complex Gaussian k-space of the workload's shape"}
+    plan.close()
+    del ws, rss, k
+    return res
+
+
+def run_provider_plain(scan_np, traj_np, firings, warmup):
+    """The drop-in Provider operator (module gpu-recon, kb_gridding_reconstruct_rss,
+    reached through ksj_provider_query): one synchronous firing per scan with
+    host-pageable, 64-byte aligned k-space and trajectory payloads, as the
+    reference's host supplies them.  The operator stages them to the device through
+    its pinned bounce buffers, reuses its resident plan (same trajectory bytes),
+    runs the KB pipeline and copies the RSS image into the host output grant."""
+    from kspacejet_amd import provider_host as H
+    S = traj_np.shape[0]
+    cfg = (f'{{"channels":{COILS},"density_compensation":"radial_analytic_ramp","image_cols":{N},"image_rows":{N},'
+           f'"kernel_width":{WIDTH},"oversampling":2,"sample_count":{S},"trajectory_units":"radians_per_pixel"}}')
+    module = H.ProviderModule("gpu_recon")
+    st, op, msg = module.create("kb_gridding_reconstruct_rss", cfg)
+    if st != H.OK or op.open(0) != H.OK:
+        raise RuntimeError(f"provider operator: {msg}")
+    k = H.Item(H.aligned_copy(np.asarray(scan_np, np.complex64)), "ksj.noncartesian-kspace-frame")
+    t = H.Item(H.aligned_copy(np.asarray(traj_np, np.float32)), "ksj.trajectory-frame")
+    lease = H.Lease([H.Batch(0, [k]), H.Batch(1, [t])], H.OutputSpec(N * N * 4, "ksj.image-frame", np.float32))
+
+    def fire():
+        st, r, msg = op.process(lease)
+        if st != H.OK or r.sealed_output_count != 1:
+            raise RuntimeError(f"provider firing failed: {msg}")
+
+    for _ in range(max(1, warmup)):
+        fire()
+    t0 = time.perf_counter()
+    for _ in range(firings):
+        fire()
+    dt = (time.perf_counter() - t0) / firings
+    op.close()
+    return {"images_per_s": round(1.0 / dt, 1), "ms_per_firing": round(1e3 * dt, 3), "firings": firings,
+            "operator": "org.kspacejet.gpu-recon kb_gridding_reconstruct_rss via ksj_provider_query, host-pageable "
+                        "payloads staged by the operator, one synchronous firing per scan",
+            "host_bytes_per_firing": int(k.data.nbytes + t.data.nbytes + N * N * 4)}
 
 
 def run_uneven(torch, ops, lib, count, steps, warmup, tune_seconds):

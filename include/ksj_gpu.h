@@ -190,6 +190,60 @@ int ksj_gpu_adjoint2_execute_multi(uint32_t count, const ksj_gpu_plan2* const* p
                                    const uint32_t* channels, void* const* d_images, int layout, float* const* d_rss,
                                    void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------ 3-D (kooshball) */
+
+/* 3-D KB gridding adjoint.  No reference counterpart: the reference reconstructs 2-D
+ * frames only (ksj::nufft::Grid2D, include/kspacejet/nufft/types.hpp:9-14); these
+ * extend its 2-D definitions axis by axis.  Volumes are [z][y][x]; trajectories
+ * [S][3] float32 in (z, y, x) order, radians per pixel; origin per axis (the 2-D
+ * providers' rule is ((N-1)/2) per axis). */
+#define KSJ_GPU_DENSITY_RAMP_SQUARED 3 /* |k|^2, the 3-D radial density compensation */
+
+typedef struct ksj_gpu_plan3_desc {
+  uint32_t struct_size;          /* sizeof(ksj_gpu_plan3_desc) */
+  uint32_t image_z, image_y, image_x;
+  uint32_t grid_z, grid_y, grid_x; /* oversampled grid, each <= 2048, fewer than 2^32 points */
+  uint32_t kernel_width;         /* 2, 4, 6 or 8 (Kaiser-Bessel) */
+  uint32_t density;              /* KSJ_GPU_DENSITY_NONE, _RAMP_SQUARED or _GIVEN */
+  uint32_t samples;
+  uint32_t item_visits;          /* gather work-item size in visits, 0 = default (64) */
+  uint32_t reserved0;
+  double beta;                   /* <= 0: Beatty et al. 2005 value for the oversampling */
+  double origin_z, origin_y, origin_x;
+} ksj_gpu_plan3_desc;
+
+typedef struct ksj_gpu_plan3_info {
+  uint32_t struct_size;
+  uint32_t items;          /* gather work items */
+  uint64_t visits;         /* (sample, 2x2x2 cell block) visits */
+  uint64_t device_bytes;   /* resident table bytes */
+  double beta;
+} ksj_gpu_plan3_info;
+
+typedef struct ksj_gpu_plan3 ksj_gpu_plan3;
+
+int ksj_gpu_plan3_create(const ksj_gpu_plan3_desc* desc, ksj_gpu_plan3** out_plan);
+/* Builds every table from d_traj ([S][3] float32); d_dcf is read only for DENSITY_GIVEN.
+ * Synchronises `stream`.  Rejects non-finite coordinates and |w| > pi. */
+int ksj_gpu_plan3_set_trajectory(ksj_gpu_plan3* plan, const float* d_traj, const float* d_dcf, void* stream);
+int ksj_gpu_plan3_get_info(const ksj_gpu_plan3* plan, ksj_gpu_plan3_info* out_info);
+/* d_cell_start[grid_z*grid_y*grid_x + 1] and d_perm[samples], for bit-exact checks of the binning. */
+int ksj_gpu_plan3_export_bins(const ksj_gpu_plan3* plan, uint32_t* d_cell_start, uint32_t* d_perm, void* stream);
+void ksj_gpu_plan3_destroy(ksj_gpu_plan3* plan);
+
+/* Channel-major samples [C][S] -> coil images coil-last [z][y][x][C] (if d_images)
+ * and/or the RSS volume (if d_rss), coils taken coils_per_pass (1..64) at a time:
+ * one pass's grid holds G_z*G_y*G_x*padded(coils_per_pass) complex values. */
+size_t ksj_gpu_adjoint3_workspace_bytes(const ksj_gpu_plan3* plan, uint32_t channels, uint32_t coils_per_pass);
+int ksj_gpu_adjoint3_execute(const ksj_gpu_plan3* plan, const void* d_kspace, uint32_t channels,
+                             uint32_t coils_per_pass, void* d_images, float* d_rss, void* d_workspace,
+                             size_t workspace_bytes, void* stream);
+
+/* Golden-means kooshball [spokes*readout][3]: spoke p along (c, sqrt(1-c^2) sin a,
+ * sqrt(1-c^2) cos a), c = frac(0.4656 p), a = 2 pi frac(0.6823 p); readout
+ * k_j = (j - M/2)/M cycles per pixel; FP64 on the device, rounded to float. */
+int ksj_gpu_golden_means3(float* d_traj, uint32_t spokes, uint32_t readout, void* stream);
+
 /* ------------------------------------------------------------------ HIP graphs */
 
 /* Captures the work enqueued on `stream` (and on streams it forks to through events)

@@ -19,6 +19,7 @@ KERNEL_KAISER_BESSEL = 1
 DENSITY_NONE = 0
 DENSITY_RAMP = 1
 DENSITY_GIVEN = 2
+DENSITY_RAMP_SQUARED = 3
 
 
 class KsjGpuError(RuntimeError):
@@ -40,6 +41,22 @@ class Plan2Desc(C.Structure):
 
 
 class Plan2Info(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("items", C.c_uint32), ("visits", C.c_uint64),
+                ("device_bytes", C.c_uint64), ("beta", C.c_double)]
+
+
+class Plan3Desc(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("image_z", C.c_uint32), ("image_y", C.c_uint32), ("image_x", C.c_uint32),
+        ("grid_z", C.c_uint32), ("grid_y", C.c_uint32), ("grid_x", C.c_uint32),
+        ("kernel_width", C.c_uint32), ("density", C.c_uint32), ("samples", C.c_uint32),
+        ("item_visits", C.c_uint32), ("reserved0", C.c_uint32),
+        ("beta", C.c_double), ("origin_z", C.c_double), ("origin_y", C.c_double), ("origin_x", C.c_double),
+    ]
+
+
+class Plan3Info(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("items", C.c_uint32), ("visits", C.c_uint64),
                 ("device_bytes", C.c_uint64), ("beta", C.c_double)]
 
@@ -95,6 +112,14 @@ SIGNATURES = {
     "ksj_gpu_plan2_destroy": (None, [_vp]),
     "ksj_gpu_adjoint2_workspace_bytes": (_sz, [_vp, _u32]),
     "ksj_gpu_adjoint2_execute": (_int, [_vp, _vp, _u32, _vp, _int, _vp, _vp, _sz, _vp]),
+    "ksj_gpu_plan3_create": (_int, [C.POINTER(Plan3Desc), C.POINTER(_vp)]),
+    "ksj_gpu_plan3_set_trajectory": (_int, [_vp, _vp, _vp, _vp]),
+    "ksj_gpu_plan3_get_info": (_int, [_vp, C.POINTER(Plan3Info)]),
+    "ksj_gpu_plan3_export_bins": (_int, [_vp, _vp, _vp, _vp]),
+    "ksj_gpu_plan3_destroy": (None, [_vp]),
+    "ksj_gpu_adjoint3_workspace_bytes": (_sz, [_vp, _u32, _u32]),
+    "ksj_gpu_adjoint3_execute": (_int, [_vp, _vp, _u32, _u32, _vp, _vp, _vp, _sz, _vp]),
+    "ksj_gpu_golden_means3": (_int, [_vp, _u32, _u32, _vp]),
     "ksj_gpu_adjoint2_batch_workspace_bytes": (_sz, [_vp, _u32, _u32]),
     "ksj_gpu_adjoint2_execute_batch": (_int, [_vp, _u32, C.POINTER(_vp), _u32, C.POINTER(_vp), _int,
                                                C.POINTER(_vp), _vp, _sz, _vp]),

@@ -6,8 +6,10 @@
 #include "grid.hpp"
 #include "nudft.hpp"
 #include "plan.hpp"
+#include "plan3.hpp"
 #include "prof.hpp"
 #include "recon.hpp"
+#include "recon3.hpp"
 #include "serve.hpp"
 
 #include <cstring>
@@ -17,6 +19,10 @@
 
 struct ksj_gpu_plan2 {
   ksj::gpu::Plan2* impl;
+};
+
+struct ksj_gpu_plan3 {
+  ksj::gpu::Plan3* impl;
 };
 
 struct ksj_gpu_server {
@@ -239,6 +245,102 @@ int ksj_gpu_adjoint2_execute(const ksj_gpu_plan2* plan, const void* d_kspace, ui
     scan.rss = d_rss;
     ksj::gpu::adjoint2_execute(*plan->impl, &scan, 1, channels, layout, d_workspace, workspace_bytes,
                                as_stream(stream));
+  });
+}
+
+int ksj_gpu_plan3_create(const ksj_gpu_plan3_desc* desc, ksj_gpu_plan3** out_plan) {
+  return guarded([&] {
+    ksj::gpu::require(desc && out_plan, "null argument");
+    ksj::gpu::require(desc->struct_size >= sizeof(ksj_gpu_plan3_desc), "plan descriptor too small");
+    ksj::gpu::Plan3Desc d;
+    d.image[0] = desc->image_z;
+    d.image[1] = desc->image_y;
+    d.image[2] = desc->image_x;
+    d.grid[0] = desc->grid_z;
+    d.grid[1] = desc->grid_y;
+    d.grid[2] = desc->grid_x;
+    d.width = desc->kernel_width;
+    d.density = desc->density;
+    d.samples = desc->samples;
+    d.item_visits = desc->item_visits == 0 ? 64u : desc->item_visits;
+    d.beta = desc->beta;
+    d.origin[0] = desc->origin_z;
+    d.origin[1] = desc->origin_y;
+    d.origin[2] = desc->origin_x;
+    auto* handle = new ksj_gpu_plan3{nullptr};
+    try {
+      handle->impl = ksj::gpu::plan3_create(d);
+    } catch (...) {
+      delete handle;
+      throw;
+    }
+    *out_plan = handle;
+  });
+}
+
+int ksj_gpu_plan3_set_trajectory(ksj_gpu_plan3* plan, const float* d_traj, const float* d_dcf, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(plan && plan->impl, "null plan");
+    ksj::gpu::plan3_set_trajectory(plan->impl, d_traj, d_dcf, as_stream(stream));
+  });
+}
+
+int ksj_gpu_plan3_get_info(const ksj_gpu_plan3* plan, ksj_gpu_plan3_info* out_info) {
+  return guarded([&] {
+    ksj::gpu::require(plan && plan->impl && out_info, "null argument");
+    ksj::gpu::require(out_info->struct_size >= sizeof(ksj_gpu_plan3_info), "info struct too small");
+    const auto* p = plan->impl;
+    out_info->items = p->item_count;
+    out_info->visits = p->visit_count;
+    out_info->device_bytes = p->device_bytes + p->visit_capacity * (sizeof(uint32_t) + 2 * sizeof(float4)) +
+                             static_cast<uint64_t>(p->item_capacity) * sizeof(uint4);
+    out_info->beta = p->d.beta;
+  });
+}
+
+int ksj_gpu_plan3_export_bins(const ksj_gpu_plan3* plan, uint32_t* d_cell_start, uint32_t* d_perm, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(plan && plan->impl && plan->impl->has_trajectory, "plan has no trajectory");
+    const auto* p = plan->impl;
+    if (d_cell_start)
+      KSJ_HIP(hipMemcpyAsync(d_cell_start, p->cell_start, (p->nbins + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                             as_stream(stream)));
+    if (d_perm)
+      KSJ_HIP(hipMemcpyAsync(d_perm, p->src, static_cast<size_t>(p->d.samples) * sizeof(uint32_t),
+                             hipMemcpyDeviceToDevice, as_stream(stream)));
+  });
+}
+
+void ksj_gpu_plan3_destroy(ksj_gpu_plan3* plan) {
+  if (!plan) return;
+  ksj::gpu::plan3_destroy(plan->impl);
+  delete plan;
+}
+
+size_t ksj_gpu_adjoint3_workspace_bytes(const ksj_gpu_plan3* plan, uint32_t channels, uint32_t coils_per_pass) {
+  size_t bytes = 0;
+  const int rc = guarded([&] {
+    ksj::gpu::require(plan && plan->impl, "null plan");
+    bytes = ksj::gpu::adjoint3_workspace_bytes(*plan->impl, channels, coils_per_pass);
+  });
+  return rc == KSJ_GPU_OK ? bytes : 0;
+}
+
+int ksj_gpu_adjoint3_execute(const ksj_gpu_plan3* plan, const void* d_kspace, uint32_t channels,
+                             uint32_t coils_per_pass, void* d_images, float* d_rss, void* d_workspace,
+                             size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(plan && plan->impl && d_kspace && d_workspace, "null argument");
+    ksj::gpu::adjoint3_execute(*plan->impl, static_cast<const float2*>(d_kspace), channels, coils_per_pass,
+                               static_cast<float2*>(d_images), d_rss, d_workspace, workspace_bytes,
+                               as_stream(stream));
+  });
+}
+
+int ksj_gpu_golden_means3(float* d_traj, uint32_t spokes, uint32_t readout, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(d_traj != nullptr, "null trajectory");
+    ksj::gpu::golden_means3(d_traj, spokes, readout, as_stream(stream));
   });
 }
 

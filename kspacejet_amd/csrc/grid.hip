@@ -628,13 +628,18 @@ size_t grid_partial_bytes(const Plan2& plan, uint32_t coils) {
 
 void grid_prep(const Plan2& plan, const CoilPlanes& planes, float2* d_sorted, hipStream_t stream) {
   require(plan.has_trajectory, "plan has no trajectory");
+  grid_prep_rows(plan.weight, plan.rank, plan.d.samples, planes, d_sorted, stream);
+}
+
+void grid_prep_rows(const float2* d_weight, const uint32_t* d_rank, uint32_t samples, const CoilPlanes& planes,
+                    float2* d_sorted, hipStream_t stream) {
   require(planes.coils >= 1 && planes.coils <= kMaxStackedCoils, "gridding takes 1..64 stacked coils");
   PrepArgs a{};
   a.planes = planes;
-  a.weight = plan.weight;
-  a.rank = plan.rank;
+  a.weight = d_weight;
+  a.rank = d_rank;
   a.sorted = d_sorted;
-  a.samples = plan.d.samples;
+  a.samples = samples;
   a.coils = planes.coils;
   const dim3 grid(ceil_div(a.samples, kPrepSamples)), block(kPrepSamples * kPrepWaves);
   switch (padded_coils(planes.coils)) {

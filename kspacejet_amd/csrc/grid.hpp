@@ -21,6 +21,9 @@ struct CoilPlanes {
 // Applies the plan's per-sample weight and moves the coil planes into sorted,
 // coil-fastest rows out[k][Cp] (pad zeroed).
 void grid_prep(const Plan2& plan, const CoilPlanes& planes, float2* d_sorted, hipStream_t stream);
+// The same from a plan's per-sample weights and sorted ranks (any dimension).
+void grid_prep_rows(const float2* d_weight, const uint32_t* d_rank, uint32_t samples, const CoilPlanes& planes,
+                    float2* d_sorted, hipStream_t stream);
 
 // Bytes of the per-item partial-sum buffer grid_gather needs (blocks with several items).
 size_t grid_partial_bytes(const Plan2& plan, uint32_t coils);

@@ -1,4 +1,5 @@
 // Trajectory plan kernels: FP64 coordinates, binning, taps, gather schedule, deapodization.
+#include "kb.hpp"
 #include "plan.hpp"
 #include "sort.hpp"
 
@@ -7,46 +8,6 @@
 
 namespace ksj::gpu {
 namespace {
-
-constexpr uint32_t kErrBadCoordinate = 1;
-
-// Modified Bessel function I0 by its power series (FP64), the same definition as
-// the oracle's; beta <= ~40 converges in < 60 terms.
-__device__ inline double bessel_i0(double x) {
-  const double q = 0.25 * x * x;
-  double term = 1.0, sum = 1.0;
-  for (int k = 1; k < 500; ++k) {
-    term *= q / (static_cast<double>(k) * static_cast<double>(k));
-    sum += term;
-    if (term < sum * 1e-17) break;
-  }
-  return sum;
-}
-
-__device__ inline double kernel_value(double d, uint32_t kernel, int width, double beta, double i0_beta) {
-  if (kernel == kKernelLinear) {
-    const double a = fabs(d);
-    return a < 1.0 ? 1.0 - a : 0.0;
-  }
-  const double half = 0.5 * width;
-  if (!(fabs(d) < half)) return 0.0;
-  const double t = d / half;
-  return bessel_i0(beta * sqrt(1.0 - t * t)) / i0_beta;
-}
-
-// FP64 grid coordinate; the IEEE sequence must match the CPU binning exactly.
-__device__ inline bool grid_coordinate(float w, uint32_t extent, uint32_t* cell, double* frac) {
-#pragma clang fp contract(off)
-  const double e = static_cast<double>(extent);
-  double u = (static_cast<double>(w) * e) / kTwoPi;
-  if (u < 0.0) u += e;
-  if (u >= e) u -= e;
-  if (!(u >= 0.0 && u < e)) return false;
-  const double f = floor(u);
-  *cell = static_cast<uint32_t>(f);
-  *frac = u - f;
-  return true;
-}
 
 struct CoordArgs {
   const float* traj;
@@ -348,16 +309,6 @@ __global__ void ramp_dcf_kernel(const float* traj, float* dcf, uint64_t samples)
   dcf[s] = hypotf(traj[2 * s], traj[2 * s + 1]);
 }
 
-double host_i0(double x) {
-  const double q = 0.25 * x * x;
-  double term = 1.0, sum = 1.0;
-  for (int k = 1; k < 500; ++k) {
-    term *= q / (static_cast<double>(k) * static_cast<double>(k));
-    sum += term;
-    if (term < sum * 1e-17) break;
-  }
-  return sum;
-}
 
 template <typename T> T* alloc(size_t count, size_t* total) {
   T* p = nullptr;
@@ -644,6 +595,19 @@ void plan2_set_trajectory(Plan2* p, const float* d_traj, const float* d_dcf, hip
   p->multi_count = multi;
   KSJ_HIP(hipStreamSynchronize(stream));
   p->has_trajectory = true;
+}
+
+void deapod_table(float* d_table, uint32_t n, uint32_t grid, int width, double beta, hipStream_t stream) {
+  hipLaunchKernelGGL(deapod_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, stream, d_table, n, grid, width, beta,
+                     host_i0(beta));
+  KSJ_LAUNCH_CHECK();
+}
+
+void cell_starts(const uint32_t* d_sorted_keys, uint32_t samples, uint64_t nbins, uint32_t* d_start,
+                 hipStream_t stream) {
+  hipLaunchKernelGGL(cell_start_kernel, dim3(ceil_div(static_cast<uint64_t>(samples) + 1, 256)), dim3(256), 0, stream,
+                     d_sorted_keys, samples, nbins, d_start);
+  KSJ_LAUNCH_CHECK();
 }
 
 void golden_angle2(float* d_traj, uint32_t spokes, uint32_t readout, hipStream_t stream) {

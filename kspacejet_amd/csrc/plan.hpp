@@ -129,6 +129,13 @@ void plan2_destroy(Plan2* plan);
 // mapped (non-finite or outside [-pi, pi]).
 void plan2_set_trajectory(Plan2* plan, const float* d_traj, const float* d_dcf, hipStream_t stream);
 
+// 1 / phi_hat(j) for output index r (j = r - n/2) of an n-point crop of a grid-point
+// axis: the continuous FT of the width-W Kaiser-Bessel kernel at 2 pi j / grid, FP64.
+void deapod_table(float* d_table, uint32_t n, uint32_t grid, int width, double beta, hipStream_t stream);
+// start[b] = first position of bin b in the sorted keys (start[nbins] = samples).
+void cell_starts(const uint32_t* d_sorted_keys, uint32_t samples, uint64_t nbins, uint32_t* d_start,
+                 hipStream_t stream);
+
 // Golden-angle radial trajectory (FP64 on the device, rounded to float):
 // spoke p at angle p * pi / phi, readout k_j = (j - M/2) / M cycles per pixel,
 // written as [S][2] (ky, kx) in radians per pixel.

@@ -1,0 +1,80 @@
+// Trajectory plan for 3-D gridding (radial "kooshball" and any other 3-D
+// trajectory): the 2-D plan's tables with a third axis, computed once on the
+// device and kept resident.
+//
+//   coordinates  FP64 grid coordinate per axis, u = w * G / (2 pi) wrapped into
+//                [0, G), its cell and fraction; bin = (cz * G_y + cy) * G_x + cx.
+//   sort         stable LSD radix sort of samples by bin, cell_start[b].
+//   weights      per-sample complex factor dcf * exp(+i w . s), s = N/2 - origin
+//                per axis, FP64; dcf = 1, |w|^2 or given.
+//   taps         per sorted sample W taps per axis (z, y, x), FP64 rounded to float.
+//   schedule     per 2x2x2 cell block, the (sample, eight weights) visits of its
+//                (W+1)^3 window bins, cut into work items of at most item_visits.
+//   deapod       1 / phi_hat per output index of each axis, FP64.
+// Volumes are [z][y][x]; trajectories [S][3] in (z, y, x) order, radians per pixel.
+#pragma once
+
+#include "plan.hpp"
+
+namespace ksj::gpu {
+
+enum : uint32_t { kDensityRampSquared = 3 };  // |w|^2, the 3-D radial density compensation
+
+struct Plan3Desc {
+  uint32_t image[3] = {0, 0, 0};  // N_z, N_y, N_x
+  uint32_t grid[3] = {0, 0, 0};   // G_z, G_y, G_x
+  uint32_t width = 4;
+  double beta = 0.0;              // <= 0: Beatty value for the oversampling
+  double origin[3] = {0.0, 0.0, 0.0};
+  uint32_t density = kDensityRampSquared;  // kDensityNone, kDensityRampSquared or kDensityGiven
+  uint32_t samples = 0;
+  uint32_t item_visits = 64;
+};
+
+struct Plan3 {
+  Plan3Desc d{};
+  int device = 0;
+  uint64_t nbins = 0;
+  int key_bits = 0;
+  float2* weight = nullptr;        // [S] original order
+  uint32_t* rank = nullptr;        // [S] original -> sorted position
+  uint32_t* src = nullptr;         // [S] sorted -> original
+  uint32_t* bins = nullptr;        // [S] bins (original order)
+  uint32_t* cell_start = nullptr;  // [nbins + 1]
+  float* taps = nullptr;           // [S][3W] sorted: z, y, x taps
+  // Schedule over 2x2x2 blocks (block (bz, by, bx) covers cells 2b .. 2b+1 per axis).
+  uint32_t blocks[3] = {0, 0, 0};
+  uint64_t nblocks = 0;
+  uint32_t* block_visit_off = nullptr;  // [nblocks + 1]
+  uint32_t* visit_k = nullptr;          // [visit_capacity]
+  float4* visit_w = nullptr;            // [2 * visit_capacity]: cells (cz, cy, cx) in cz*4 + cy*2 + cx order
+  uint64_t visit_capacity = 0, visit_count = 0;
+  uint32_t* block_first_item = nullptr;  // [nblocks + 1]
+  uint32_t* block_first_slot = nullptr;  // [nblocks + 1]
+  uint4* item_info = nullptr;            // {block, first visit, end visit, slot or ~0}
+  uint32_t item_capacity = 0, item_count = 0, slot_count = 0;
+  uint32_t* multi_blocks = nullptr;      // [nblocks]
+  uint32_t multi_count = 0;
+  float* deapod[3] = {nullptr, nullptr, nullptr};
+  // build scratch
+  double* frac = nullptr;          // [S][3]
+  uint32_t* sorted_keys = nullptr; // [S]
+  uint32_t* counters = nullptr;    // [4]
+  void* sort_scratch = nullptr;
+  size_t device_bytes = 0;
+  bool has_trajectory = false;
+};
+
+Plan3* plan3_create(const Plan3Desc& desc);
+void plan3_destroy(Plan3* plan);
+// Recomputes every table from a float32 [S][3] trajectory on the device (dcf read
+// only for kDensityGiven); synchronises `stream`; throws on a non-finite coordinate
+// or one outside [-pi, pi].
+void plan3_set_trajectory(Plan3* plan, const float* d_traj, const float* d_dcf, hipStream_t stream);
+
+// Golden-means kooshball (FP64 on the device, rounded to float): spoke p along
+// (c, sqrt(1-c^2) sin a, sqrt(1-c^2) cos a), c = frac(p g1), a = 2 pi frac(p g2),
+// g1 = 0.4656, g2 = 0.6823; k_j = (j - M/2)/M cycles per pixel; [S][3] (z, y, x).
+void golden_means3(float* d_traj, uint32_t spokes, uint32_t readout, hipStream_t stream);
+
+}  // namespace ksj::gpu

@@ -1,0 +1,350 @@
+// 3-D KB adjoint NUFFT on one GPU: coils in passes of up to coils_per_pass, each
+// pass prep -> 2x2x2-block gather (no float atomics) -> multi-item reduce ->
+// pruned 3-D inverse FFT (x over the full grid, y over the kept x columns, z over
+// the kept (y, x) lines) with deapodization and crop fused into the y and z passes
+// -> RSS accumulated across passes in channel order and/or coil-last images.
+#include "fft.hpp"
+#include "grid.hpp"
+#include "plan3.hpp"
+#include "prof.hpp"
+#include "recon3.hpp"
+
+#include <algorithm>
+
+namespace ksj::gpu {
+namespace {
+
+struct Gather3Args {
+  const float2* sorted;        // [S][Cp]
+  const uint32_t* visit_k;
+  const float4* visit_w;       // two per visit
+  const uint4* item_info;
+  const uint32_t* block_first_item;
+  const uint32_t* block_first_slot;
+  const uint32_t* multi_blocks;
+  float2* partial;             // [slots][8][Cp]
+  float2* grid;                // [G_z][G_y][G_x][Cp]
+  uint32_t g[3], blocks[3];
+  uint32_t cp, item_count, multi_count;
+};
+
+__device__ inline uint32_t last_visit(uint32_t v, uint32_t end) {
+  const uint32_t last = end > 0 ? end - 1 : 0;
+  return v < last ? v : last;
+}
+
+__device__ inline void block_cells(const Gather3Args& a, uint32_t b, uint32_t* z, uint32_t* y, uint32_t* x) {
+  *x = 2 * (b % a.blocks[2]);
+  const uint32_t r = b / a.blocks[2];
+  *y = 2 * (r % a.blocks[1]);
+  *z = 2 * (r / a.blocks[1]);
+}
+
+// One sub-wave of CP lanes (lane = coil) per work item, 64 / CP items per wave.
+// Visits are taken B at a time: the batch's sample rows are all loaded before the
+// first use, and the next batch's positions and weights (clamped into the item,
+// values past it multiply zero) are loaded while this batch's rows are in flight.
+// Sums run in visit order; the eight cells are written once (or to the item's
+// partial slot).
+template <int CP, int B>
+__global__ __launch_bounds__(256) void gather3_kernel(Gather3Args a) {
+  constexpr int NSW = 64 / CP;
+  const int lane = threadIdx.x & 63, sw = lane / CP, c = lane % CP;
+  const uint32_t item = (blockIdx.x * 4 + (threadIdx.x >> 6)) * NSW + sw;
+  const bool live = item < a.item_count;
+  const uint4 info = live ? a.item_info[item] : make_uint4(0, 0, 0, 0);
+  float2 acc[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) acc[q] = make_float2(0.f, 0.f);
+  uint32_t k[B];
+  float4 w0[B], w1[B];
+#pragma unroll
+  for (int i = 0; i < B; ++i) {
+    const uint32_t vi = last_visit(info.y + i, info.z);
+    k[i] = a.visit_k[vi];
+    w0[i] = a.visit_w[2 * static_cast<uint64_t>(vi)];
+    w1[i] = a.visit_w[2 * static_cast<uint64_t>(vi) + 1];
+  }
+  for (uint32_t v = info.y; v < info.z; v += B) {
+    const uint32_t n = info.z - v;
+    float2 d[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+      d[i] = static_cast<uint32_t>(i) < n ? a.sorted[static_cast<uint64_t>(k[i]) * CP + c] : make_float2(0.f, 0.f);
+    uint32_t kn[B];
+    float4 w0n[B], w1n[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      const uint32_t vi = last_visit(v + B + i, info.z);
+      kn[i] = a.visit_k[vi];
+      w0n[i] = a.visit_w[2 * static_cast<uint64_t>(vi)];
+      w1n[i] = a.visit_w[2 * static_cast<uint64_t>(vi) + 1];
+    }
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      const float w[8] = {w0[i].x, w0[i].y, w0[i].z, w0[i].w, w1[i].x, w1[i].y, w1[i].z, w1[i].w};
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        acc[q].x += w[q] * d[i].x;
+        acc[q].y += w[q] * d[i].y;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      k[i] = kn[i];
+      w0[i] = w0n[i];
+      w1[i] = w1n[i];
+    }
+  }
+  if (!live) return;
+  uint32_t z0, y0, x0;
+  block_cells(a, info.x, &z0, &y0, &x0);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    if (info.w == 0xFFFFFFFFu) {
+      const uint32_t z = z0 + q / 4, y = y0 + (q / 2) % 2, x = x0 + q % 2;
+      if (z < a.g[0] && y < a.g[1] && x < a.g[2])
+        a.grid[((static_cast<uint64_t>(z) * a.g[1] + y) * a.g[2] + x) * CP + c] = acc[q];
+    } else {
+      a.partial[(static_cast<uint64_t>(info.w) * 8 + q) * CP + c] = acc[q];
+    }
+  }
+}
+
+// Blocks with several items: thread per (block, cell, coil) sums the partials in item order.
+__global__ __launch_bounds__(256) void reduce3_kernel(Gather3Args a) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<uint64_t>(a.multi_count) * 8 * a.cp) return;
+  const uint32_t c = static_cast<uint32_t>(i % a.cp);
+  const uint32_t q = static_cast<uint32_t>((i / a.cp) % 8);
+  const uint32_t b = a.multi_blocks[i / (8ull * a.cp)];
+  const uint32_t s0 = a.block_first_slot[b];
+  const uint32_t n = a.block_first_item[b + 1] - a.block_first_item[b];
+  const float2* p = a.partial + (static_cast<uint64_t>(s0) * 8 + q) * a.cp + c;
+  float2 s = make_float2(0.f, 0.f);
+  for (uint32_t j = 0; j < n; ++j) s = cadd(s, p[static_cast<uint64_t>(j) * 8 * a.cp]);
+  uint32_t z0, y0, x0;
+  block_cells(a, b, &z0, &y0, &x0);
+  const uint32_t z = z0 + q / 4, y = y0 + (q / 2) % 2, x = x0 + q % 2;
+  if (z < a.g[0] && y < a.g[1] && x < a.g[2]) a.grid[((static_cast<uint64_t>(z) * a.g[1] + y) * a.g[2] + x) * a.cp + c] = s;
+}
+
+// sumsq[p] (+)= sum over the pass's coils of |img[c][p]|^2 in channel order; on the
+// last pass rss[p] = sqrt(sumsq[p]).
+__global__ void rss3_kernel(const float2* img, uint32_t coils, uint64_t pixels, float* sumsq, float* rss, int first,
+                            int last) {
+  const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= pixels) return;
+  float s = first ? 0.f : sumsq[p];
+  for (uint32_t c = 0; c < coils; ++c) {
+    const float2 v = img[static_cast<uint64_t>(c) * pixels + p];
+    s += v.x * v.x + v.y * v.y;
+  }
+  if (last)
+    rss[p] = sqrtf(s);
+  else
+    sumsq[p] = s;
+}
+
+// Coil-major pass images [coils][P] -> channels c0 .. c0 + coils of coil-last [P][C].
+__global__ void coil_last3_kernel(const float2* img, uint32_t coils, uint64_t pixels, float2* out, uint32_t c0,
+                                  uint32_t channels) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= pixels * coils) return;
+  const uint64_t p = i / coils;
+  const uint32_t c = static_cast<uint32_t>(i % coils);
+  out[p * channels + c0 + c] = img[static_cast<uint64_t>(c) * pixels + p];
+}
+
+template <int CP>
+void launch_gather3(const Gather3Args& a, hipStream_t stream) {
+  constexpr int NSW = 64 / CP;
+  const uint32_t wgs = std::max<uint32_t>(1, ceil_div(ceil_div(a.item_count, NSW), 4));
+  hipLaunchKernelGGL((gather3_kernel<CP, 4>), dim3(wgs), dim3(256), 0, stream, a);
+  KSJ_LAUNCH_CHECK();
+}
+
+struct Layout3 {
+  size_t sorted, grid, partial, xout, yout, zout, sumsq, total;
+};
+
+size_t align256_3(size_t n) { return (n + 255) & ~static_cast<size_t>(255); }
+
+Layout3 layout3(const Plan3& p, uint32_t channels, uint32_t coils_per_pass) {
+  const Plan3Desc& d = p.d;
+  const uint32_t cpp = std::min(channels, coils_per_pass);
+  const size_t cp = padded_coils(cpp);
+  const size_t G = static_cast<size_t>(d.grid[0]) * d.grid[1] * d.grid[2];
+  const size_t P = static_cast<size_t>(d.image[0]) * d.image[1] * d.image[2];
+  Layout3 l{};
+  l.sorted = align256_3(static_cast<size_t>(d.samples) * cp * sizeof(float2));
+  l.grid = align256_3(G * cp * sizeof(float2));
+  l.partial = align256_3(std::max<size_t>(1, static_cast<size_t>(p.slot_count) * 8 * cp) * sizeof(float2));
+  l.xout = align256_3(cpp * static_cast<size_t>(d.grid[0]) * d.grid[1] * d.image[2] * sizeof(float2));
+  l.yout = align256_3(cpp * static_cast<size_t>(d.grid[0]) * d.image[1] * d.image[2] * sizeof(float2));
+  l.zout = align256_3(cpp * P * sizeof(float2));
+  l.sumsq = align256_3(P * sizeof(float));
+  l.total = l.sorted + l.grid + l.partial + l.xout + l.yout + l.zout + l.sumsq;
+  return l;
+}
+
+}  // namespace
+
+size_t adjoint3_workspace_bytes(const Plan3& plan, uint32_t channels, uint32_t coils_per_pass) {
+  require(channels >= 1 && coils_per_pass >= 1 && coils_per_pass <= kMaxStackedCoils,
+          "3-D adjoint needs channels >= 1 and 1..64 coils per pass");
+  require(plan.has_trajectory, "plan has no trajectory");
+  return layout3(plan, channels, coils_per_pass).total;
+}
+
+void adjoint3_execute(const Plan3& plan, const float2* d_kspace, uint32_t channels, uint32_t coils_per_pass,
+                      float2* d_images, float* d_rss, void* d_workspace, size_t workspace_bytes, hipStream_t stream) {
+  require(d_kspace != nullptr, "k-space must be non-null");
+  require(d_images != nullptr || d_rss != nullptr, "3-D adjoint needs an image or an RSS output");
+  const Layout3 l = layout3(plan, channels, coils_per_pass);
+  require(adjoint3_workspace_bytes(plan, channels, coils_per_pass) == l.total && d_workspace != nullptr &&
+            workspace_bytes >= l.total,
+          "3-D adjoint workspace is too small");
+  const Plan3Desc& d = plan.d;
+  const uint64_t P = static_cast<uint64_t>(d.image[0]) * d.image[1] * d.image[2];
+  char* w = static_cast<char*>(d_workspace);
+  auto* sorted = reinterpret_cast<float2*>(w);
+  auto* grid = reinterpret_cast<float2*>(w + l.sorted);
+  auto* partial = reinterpret_cast<float2*>(w + l.sorted + l.grid);
+  auto* xout = reinterpret_cast<float2*>(w + l.sorted + l.grid + l.partial);
+  auto* yout = reinterpret_cast<float2*>(w + l.sorted + l.grid + l.partial + l.xout);
+  auto* zout = reinterpret_cast<float2*>(w + l.sorted + l.grid + l.partial + l.xout + l.yout);
+  auto* sumsq = reinterpret_cast<float*>(w + l.sorted + l.grid + l.partial + l.xout + l.yout + l.zout);
+  const uint32_t cpp = std::min(channels, coils_per_pass);
+  for (uint32_t c0 = 0; c0 < channels; c0 += cpp) {
+    const uint32_t pc = std::min(cpp, channels - c0);
+    const uint32_t cp = padded_coils(pc);
+    CoilPlanes planes;
+    planes.coils = pc;
+    for (uint32_t c = 0; c < pc; ++c) planes.plane[c] = d_kspace + static_cast<uint64_t>(c0 + c) * d.samples;
+    {
+      ProfScope ps("grid3_prep", stream);
+      grid_prep_rows(plan.weight, plan.rank, d.samples, planes, sorted, stream);
+    }
+    Gather3Args a{};
+    a.sorted = sorted;
+    a.visit_k = plan.visit_k;
+    a.visit_w = plan.visit_w;
+    a.item_info = plan.item_info;
+    a.block_first_item = plan.block_first_item;
+    a.block_first_slot = plan.block_first_slot;
+    a.multi_blocks = plan.multi_blocks;
+    a.partial = partial;
+    a.grid = grid;
+    for (int i = 0; i < 3; ++i) {
+      a.g[i] = d.grid[i];
+      a.blocks[i] = plan.blocks[i];
+    }
+    a.cp = cp;
+    a.item_count = plan.item_count;
+    a.multi_count = plan.multi_count;
+    {
+      ProfScope ps("grid3_gather", stream);
+      switch (cp) {
+        case 8: launch_gather3<8>(a, stream); break;
+        case 16: launch_gather3<16>(a, stream); break;
+        case 32: launch_gather3<32>(a, stream); break;
+        default: launch_gather3<64>(a, stream); break;
+      }
+    }
+    if (plan.multi_count > 0) {
+      ProfScope ps("grid3_reduce", stream);
+      hipLaunchKernelGGL(reduce3_kernel, dim3(ceil_div(static_cast<uint64_t>(plan.multi_count) * 8 * cp, 256)),
+                         dim3(256), 0, stream, a);
+      KSJ_LAUNCH_CHECK();
+    }
+    const uint32_t Gz = d.grid[0], Gy = d.grid[1], Gx = d.grid[2];
+    const uint32_t Nz = d.image[0], Ny = d.image[1], Nx = d.image[2];
+    // x: lines (z, y, coil) of the coil-fastest grid -> [pc][Gz][Gy][Nx], crop to the centred Nx.
+    FftArgs fx;
+    fx.in = grid;
+    fx.out = xout;
+    fx.src.planes = Gz * Gy;
+    fx.src.lines = pc;
+    fx.src.plane_stride = static_cast<uint64_t>(Gx) * cp;
+    fx.src.line_stride = 1;
+    fx.src.elem_stride = cp;
+    fx.dst.planes = Gz * Gy;
+    fx.dst.lines = pc;
+    fx.dst.plane_stride = Nx;
+    fx.dst.line_stride = static_cast<uint64_t>(Gz) * Gy * Nx;
+    fx.dst.elem_stride = 1;
+    fx.n = Gx;
+    fx.n_out = Nx;
+    fx.out_offset = (Gx - Nx / 2) % Gx;
+    fx.inverse = 1;
+    fx.load_lines_fast = 1;
+    fx.store_lines_fast = 0;
+    {
+      ProfScope ps("fft3_x", stream);
+      fft_launch(fx, stream);
+    }
+    // y: lines (coil, z, x) -> [pc][Gz][Ny][Nx]; deapodization of y (per output) and x (per line).
+    FftArgs fy;
+    fy.in = xout;
+    fy.out = yout;
+    fy.src.planes = pc * Gz;
+    fy.src.lines = Nx;
+    fy.src.plane_stride = static_cast<uint64_t>(Gy) * Nx;
+    fy.src.line_stride = 1;
+    fy.src.elem_stride = Nx;
+    fy.dst.planes = pc * Gz;
+    fy.dst.lines = Nx;
+    fy.dst.plane_stride = static_cast<uint64_t>(Ny) * Nx;
+    fy.dst.line_stride = 1;
+    fy.dst.elem_stride = Nx;
+    fy.n = Gy;
+    fy.n_out = Ny;
+    fy.out_offset = (Gy - Ny / 2) % Gy;
+    fy.out_weight = plan.deapod[1];
+    fy.line_weight = plan.deapod[2];
+    fy.inverse = 1;
+    fy.load_lines_fast = fy.store_lines_fast = 1;
+    {
+      ProfScope ps("fft3_y_deapod", stream);
+      fft_launch(fy, stream);
+    }
+    // z: lines (coil, y, x) -> [pc][Nz][Ny][Nx]; deapodization of z.
+    FftArgs fz;
+    fz.in = yout;
+    fz.out = zout;
+    fz.src.planes = pc;
+    fz.src.lines = Ny * Nx;
+    fz.src.plane_stride = static_cast<uint64_t>(Gz) * Ny * Nx;
+    fz.src.line_stride = 1;
+    fz.src.elem_stride = static_cast<uint64_t>(Ny) * Nx;
+    fz.dst.planes = pc;
+    fz.dst.lines = Ny * Nx;
+    fz.dst.plane_stride = P;
+    fz.dst.line_stride = 1;
+    fz.dst.elem_stride = static_cast<uint64_t>(Ny) * Nx;
+    fz.n = Gz;
+    fz.n_out = Nz;
+    fz.out_offset = (Gz - Nz / 2) % Gz;
+    fz.out_weight = plan.deapod[0];
+    fz.inverse = 1;
+    fz.load_lines_fast = fz.store_lines_fast = 1;
+    {
+      ProfScope ps("fft3_z_deapod", stream);
+      fft_launch(fz, stream);
+    }
+    ProfScope ps("coil_combine3", stream);
+    if (d_rss) {
+      hipLaunchKernelGGL(rss3_kernel, dim3(ceil_div(P, 256)), dim3(256), 0, stream, zout, pc, P, sumsq, d_rss,
+                         c0 == 0 ? 1 : 0, c0 + pc >= channels ? 1 : 0);
+      KSJ_LAUNCH_CHECK();
+    }
+    if (d_images) {
+      hipLaunchKernelGGL(coil_last3_kernel, dim3(ceil_div(P * pc, 256)), dim3(256), 0, stream, zout, pc, P, d_images,
+                         c0, channels);
+      KSJ_LAUNCH_CHECK();
+    }
+  }
+}
+
+}  // namespace ksj::gpu

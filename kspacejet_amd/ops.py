@@ -305,6 +305,90 @@ class Server:
         self.close()
 
 
+def golden_means3(spokes: int, readout: int, device="cuda") -> torch.Tensor:
+    """Golden-means kooshball [spokes * readout, 3] (z, y, x) in radians per pixel."""
+    out = torch.empty((spokes * readout, 3), dtype=torch.float32, device=device)
+    call("ksj_gpu_golden_means3", _ptr(out), spokes, readout, _stream())
+    return out
+
+
+class Plan3:
+    """Device-resident 3-D gridding plan for one trajectory (ksj_gpu_plan3_* in include/ksj_gpu.h).
+    shape = (N_z, N_y, N_x); trajectories are [S, 3] float32 (z, y, x) radians per pixel."""
+
+    def __init__(self, shape, samples: int, oversampling: float = 2.0, width: int = 4, density: str = "ramp_squared",
+                 item_visits: int = 0, beta: float = 0.0, origin=None, grid=None):
+        d = _lib.Plan3Desc()
+        d.struct_size = C.sizeof(_lib.Plan3Desc)
+        d.image_z, d.image_y, d.image_x = shape
+        g = grid if grid is not None else tuple(int(round(n * oversampling)) for n in shape)
+        d.grid_z, d.grid_y, d.grid_x = g
+        d.kernel_width = width
+        d.density = {"none": _lib.DENSITY_NONE, "ramp_squared": _lib.DENSITY_RAMP_SQUARED,
+                     "given": _lib.DENSITY_GIVEN}[density]
+        d.samples = samples
+        d.item_visits = item_visits
+        d.beta = beta
+        o = tuple((n - 1) / 2.0 for n in shape) if origin is None else origin
+        d.origin_z, d.origin_y, d.origin_x = o
+        self.desc = d
+        self.shape = tuple(shape)
+        self._handle = C.c_void_p()
+        call("ksj_gpu_plan3_create", C.byref(d), C.byref(self._handle))
+        self._lib = load()
+
+    def set_trajectory(self, traj: torch.Tensor, dcf: torch.Tensor | None = None) -> "Plan3":
+        _need(traj, torch.float32, "traj")
+        if traj.shape != (self.desc.samples, 3):
+            raise ValueError("trajectory shape does not match the plan")
+        call("ksj_gpu_plan3_set_trajectory", self._handle, _ptr(traj), 0 if dcf is None else _ptr(dcf), _stream())
+        return self
+
+    def info(self) -> dict:
+        i = _lib.Plan3Info()
+        i.struct_size = C.sizeof(_lib.Plan3Info)
+        call("ksj_gpu_plan3_get_info", self._handle, C.byref(i))
+        return {"items": i.items, "visits": i.visits, "device_bytes": i.device_bytes, "beta": i.beta}
+
+    def export_bins(self, device="cuda"):
+        nb = self.desc.grid_z * self.desc.grid_y * self.desc.grid_x
+        start = torch.empty(nb + 1, dtype=torch.int32, device=device)
+        perm = torch.empty(self.desc.samples, dtype=torch.int32, device=device)
+        call("ksj_gpu_plan3_export_bins", self._handle, _ptr(start), _ptr(perm), _stream())
+        return start, perm
+
+    def workspace_bytes(self, channels: int, coils_per_pass: int = 8) -> int:
+        return int(self._lib.ksj_gpu_adjoint3_workspace_bytes(self._handle, channels, coils_per_pass))
+
+    def adjoint(self, kspace: torch.Tensor, images: bool = True, rss: bool = False, coils_per_pass: int = 8,
+                workspace: torch.Tensor | None = None):
+        """[C, S] complex64 -> (coil-last [N_z, N_y, N_x, C] complex64 or None, RSS [N_z, N_y, N_x] or None)."""
+        _need(kspace, torch.complex64, "kspace")
+        ch, S = kspace.shape
+        if S != self.desc.samples:
+            raise ValueError("sample count does not match the plan")
+        img = torch.empty((*self.shape, ch), dtype=torch.complex64, device=kspace.device) if images else None
+        out_rss = torch.empty(self.shape, dtype=torch.float32, device=kspace.device) if rss else None
+        need = self.workspace_bytes(ch, coils_per_pass)
+        if workspace is None or workspace.numel() < need:
+            workspace = _workspace(need, kspace.device)
+        call("ksj_gpu_adjoint3_execute", self._handle, _ptr(kspace), ch, coils_per_pass,
+             0 if img is None else _ptr(img), 0 if out_rss is None else _ptr(out_rss), _ptr(workspace),
+             workspace.numel(), _stream())
+        return img, out_rss
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None) and self._handle.value:
+            self._lib.ksj_gpu_plan3_destroy(self._handle)
+            self._handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def adjoint_multi_workspace_bytes(plans, channels) -> int:
     n = len(plans)
     P = C.c_void_p * n

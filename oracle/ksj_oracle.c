@@ -403,3 +403,185 @@ void ora_golden_angle2(size_t spokes, size_t readout, double* traj) {
     }
   }
 }
+
+/* ---------------------------------------------------------------- 3-D (kooshball) */
+/* The reference has no 3-D path; these extend its 2-D definitions axis by axis:
+ * coordinates [S][3] in (z, y, x) order, radians per pixel, adjoint sign +i
+ * (eigen_nufft_direct_nudft.cpp:77-97 with a third axis), origins per axis. */
+
+void ora_nudft3_adjoint(size_t nz, size_t ny, size_t nx, double oz, double oy, double ox, size_t samples,
+                        const double* traj, const double* values, double* image) {
+  for (size_t z = 0; z < nz; ++z)
+    for (size_t y = 0; y < ny; ++y)
+      for (size_t x = 0; x < nx; ++x) {
+        const double pz = (double)z - oz, py = (double)y - oy, px = (double)x - ox;
+        double acc_re = 0.0, acc_im = 0.0;
+        for (size_t s = 0; s < samples; ++s) {
+          const double phase = traj[3 * s] * pz + traj[3 * s + 1] * py + traj[3 * s + 2] * px;
+          const double cs = cos(phase), sn = sin(phase);
+          const double yr = values[2 * s], yi = values[2 * s + 1];
+          acc_re += yr * cs - yi * sn;
+          acc_im += yr * sn + yi * cs;
+        }
+        image[2 * ((z * ny + y) * nx + x)] = acc_re;
+        image[2 * ((z * ny + y) * nx + x) + 1] = acc_im;
+      }
+}
+
+void ora_ramp_sq_dcf3(size_t samples, const double* traj, double* dcf) {
+  /* |k|^2 in (radians per pixel)^2: the density of a 3-D radial trajectory falls as 1/|k|^2 */
+  for (size_t s = 0; s < samples; ++s)
+    dcf[s] = traj[3 * s] * traj[3 * s] + traj[3 * s + 1] * traj[3 * s + 1] + traj[3 * s + 2] * traj[3 * s + 2];
+}
+
+int ora_bin3(size_t samples, const float* traj, uint32_t gz, uint32_t gy, uint32_t gx, uint32_t* bins,
+             uint32_t* cell_start, uint32_t* perm) {
+  const size_t nbins = (size_t)gz * gy * gx;
+  memset(cell_start, 0, (nbins + 1) * sizeof(uint32_t));
+  for (size_t s = 0; s < samples; ++s) {
+    uint32_t cz, cy, cx;
+    float f;
+    if (ora_grid_coordinate(traj[3 * s], gz, &cz, &f) || ora_grid_coordinate(traj[3 * s + 1], gy, &cy, &f) ||
+        ora_grid_coordinate(traj[3 * s + 2], gx, &cx, &f))
+      return -1;
+    bins[s] = (cz * gy + cy) * gx + cx;
+    cell_start[bins[s] + 1] += 1;
+  }
+  for (size_t b = 0; b < nbins; ++b) cell_start[b + 1] += cell_start[b];
+  uint32_t* cursor = (uint32_t*)malloc(nbins * sizeof(uint32_t));
+  memcpy(cursor, cell_start, nbins * sizeof(uint32_t));
+  for (size_t s = 0; s < samples; ++s) perm[cursor[bins[s]]++] = (uint32_t)s;
+  free(cursor);
+  return 0;
+}
+
+/* Unscaled +i inverse DFT along each axis of a gz x gy x gx interleaved volume. */
+static void inverse_dft3_unscaled(size_t gz, size_t gy, size_t gx, double* data) {
+  const size_t nmax = gz > gy ? (gz > gx ? gz : gx) : (gy > gx ? gy : gx);
+  double* line = (double*)malloc(2 * nmax * sizeof(double));
+  for (size_t z = 0; z < gz; ++z)
+    for (size_t y = 0; y < gy; ++y) ora_fft1(gx, data + 2 * (z * gy + y) * gx, +1);
+  for (size_t z = 0; z < gz; ++z)
+    for (size_t x = 0; x < gx; ++x) {
+      for (size_t y = 0; y < gy; ++y) {
+        line[2 * y] = data[2 * ((z * gy + y) * gx + x)];
+        line[2 * y + 1] = data[2 * ((z * gy + y) * gx + x) + 1];
+      }
+      ora_fft1(gy, line, +1);
+      for (size_t y = 0; y < gy; ++y) {
+        data[2 * ((z * gy + y) * gx + x)] = line[2 * y];
+        data[2 * ((z * gy + y) * gx + x) + 1] = line[2 * y + 1];
+      }
+    }
+  for (size_t y = 0; y < gy; ++y)
+    for (size_t x = 0; x < gx; ++x) {
+      for (size_t z = 0; z < gz; ++z) {
+        line[2 * z] = data[2 * ((z * gy + y) * gx + x)];
+        line[2 * z + 1] = data[2 * ((z * gy + y) * gx + x) + 1];
+      }
+      ora_fft1(gz, line, +1);
+      for (size_t z = 0; z < gz; ++z) {
+        data[2 * ((z * gy + y) * gx + x)] = line[2 * z];
+        data[2 * ((z * gy + y) * gx + x) + 1] = line[2 * z + 1];
+      }
+    }
+  free(line);
+}
+
+/* Grid coordinate of an FP64 frequency on an axis of `extent` points (same wrap as the binning). */
+static double wrapped_coordinate(double w, size_t extent) {
+  double u = w * (double)extent / ORA_TWO_PI;
+  if (u < 0.0) u += (double)extent;
+  if (u >= (double)extent) u -= (double)extent;
+  return u;
+}
+
+int ora_kb_adjoint3(size_t nz, size_t ny, size_t nx, size_t gz, size_t gy, size_t gx, int width, double beta,
+                    double oz, double oy, double ox, size_t samples, const double* traj, const double* values,
+                    const double* dcf, double* image) {
+  /* KB gridding onto the (gz, gy, gx) grid, unscaled inverse DFT, crop of the centred
+   * n-cube (output index i reads grid frequency i - n/2), division by the separable
+   * KB transform: the 2-D route of ora_kb_adjoint2 with a third axis. */
+  double* grid = (double*)calloc(2 * gz * gy * gx, sizeof(double));
+  const double sz = (double)(nz / 2) - oz, sy = (double)(ny / 2) - oy, sx = (double)(nx / 2) - ox;
+  const int lead = width / 2 - 1;
+  double tz[16], ty[16], tx[16];
+  for (size_t s = 0; s < samples; ++s) {
+    const double wz = traj[3 * s], wy = traj[3 * s + 1], wx = traj[3 * s + 2];
+    const double weight = dcf ? dcf[s] : 1.0;
+    const double phase = wz * sz + wy * sy + wx * sx;
+    const double pr = cos(phase), pi = sin(phase);
+    const double yr = values[2 * s] * weight, yi = values[2 * s + 1] * weight;
+    const double zr = yr * pr - yi * pi, zi = yr * pi + yi * pr;
+    uint32_t cz, cy, cx;
+    float f;
+    if (ora_grid_coordinate((float)wz, (uint32_t)gz, &cz, &f) || ora_grid_coordinate((float)wy, (uint32_t)gy, &cy, &f) ||
+        ora_grid_coordinate((float)wx, (uint32_t)gx, &cx, &f)) {
+      free(grid);
+      return -1;
+    }
+    const double dz = wrapped_coordinate(wz, gz) - (double)cz, dy = wrapped_coordinate(wy, gy) - (double)cy;
+    const double dx = wrapped_coordinate(wx, gx) - (double)cx;
+    for (int t = 0; t < width; ++t) {
+      tz[t] = ora_kb_kernel((double)(t - lead) - dz, width, beta);
+      ty[t] = ora_kb_kernel((double)(t - lead) - dy, width, beta);
+      tx[t] = ora_kb_kernel((double)(t - lead) - dx, width, beta);
+    }
+    for (int a = 0; a < width; ++a) {
+      const size_t mz = ((size_t)cz + gz + (size_t)a - (size_t)lead) % gz;
+      for (int b = 0; b < width; ++b) {
+        const size_t my = ((size_t)cy + gy + (size_t)b - (size_t)lead) % gy;
+        const double wzy = tz[a] * ty[b];
+        for (int c = 0; c < width; ++c) {
+          const size_t mx = ((size_t)cx + gx + (size_t)c - (size_t)lead) % gx;
+          const double w = wzy * tx[c];
+          const size_t i = (mz * gy + my) * gx + mx;
+          grid[2 * i] += zr * w;
+          grid[2 * i + 1] += zi * w;
+        }
+      }
+    }
+  }
+  inverse_dft3_unscaled(gz, gy, gx, grid);
+  for (size_t z = 0; z < nz; ++z) {
+    const long jz = (long)z - (long)(nz / 2);
+    const size_t iz = (size_t)((jz + (long)gz) % (long)gz);
+    const double dz = ora_kb_ft((double)jz, gz, width, beta);
+    for (size_t y = 0; y < ny; ++y) {
+      const long jy = (long)y - (long)(ny / 2);
+      const size_t iy = (size_t)((jy + (long)gy) % (long)gy);
+      const double dzy = dz * ora_kb_ft((double)jy, gy, width, beta);
+      for (size_t x = 0; x < nx; ++x) {
+        const long jx = (long)x - (long)(nx / 2);
+        const size_t ix = (size_t)((jx + (long)gx) % (long)gx);
+        const double d = dzy * ora_kb_ft((double)jx, gx, width, beta);
+        const size_t o = (z * ny + y) * nx + x, i = (iz * gy + iy) * gx + ix;
+        image[2 * o] = grid[2 * i] / d;
+        image[2 * o + 1] = grid[2 * i + 1] / d;
+      }
+    }
+  }
+  free(grid);
+  return 0;
+}
+
+void ora_golden_means3(size_t spokes, size_t readout, double* traj) {
+  /* 3-D radial "kooshball" with multidimensional golden means (Chan et al., MRM 61
+   * 2009): spoke p has polar cosine frac(p g1) and azimuth 2 pi frac(p g2), with
+   * g1 = 0.4656, g2 = 0.6823 the golden means of the 3x3 Fibonacci recurrence; each
+   * spoke runs through the centre, k_j = (j - M/2)/M cycles per pixel, written
+   * (kz, ky, kx) in radians per pixel. */
+  const double g1 = 0.465571231876768, g2 = 0.682327803828019;
+  for (size_t p = 0; p < spokes; ++p) {
+    const double cz = fmod((double)p * g1, 1.0);
+    const double az = ORA_TWO_PI * fmod((double)p * g2, 1.0);
+    const double sz = sqrt(1.0 - cz * cz);
+    const double dz = cz, dy = sz * sin(az), dx = sz * cos(az);
+    for (size_t j = 0; j < readout; ++j) {
+      const double k = ORA_TWO_PI * ((double)j - (double)(readout / 2)) / (double)readout;
+      traj[3 * (p * readout + j)] = k * dz;
+      traj[3 * (p * readout + j) + 1] = k * dy;
+      traj[3 * (p * readout + j) + 2] = k * dx;
+    }
+  }
+}

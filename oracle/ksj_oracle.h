@@ -98,6 +98,24 @@ int ora_kb_adjoint2(size_t n_rows, size_t n_cols, size_t g_rows, size_t g_cols, 
  * readout k_j = (j - M/2) / M cycles per pixel, stored as [S][2] (ky, kx) radians per pixel. */
 void ora_golden_angle2(size_t spokes, size_t readout, double* traj);
 
+/* ---- 3-D: coordinates [S][3] in (z, y, x) order, radians per pixel; images [z][y][x]. */
+
+/* x(z,y,x) = sum_s y_s exp(+i(w_z (z-oz) + w_y (y-oy) + w_x (x-ox))), no normalisation. */
+void ora_nudft3_adjoint(size_t nz, size_t ny, size_t nx, double oz, double oy, double ox, size_t samples,
+                        const double* traj, const double* values, double* image);
+/* |k|^2 per sample. */
+void ora_ramp_sq_dcf3(size_t samples, const double* traj, double* dcf);
+/* Bin (cz * gy + cy) * gx + cx per sample from the float coordinates, stable counting sort. */
+int ora_bin3(size_t samples, const float* traj, uint32_t gz, uint32_t gy, uint32_t gx, uint32_t* bins,
+             uint32_t* cell_start, uint32_t* perm);
+/* KB gridding adjoint on a (gz, gy, gx) grid, centred crop to (nz, ny, nx), deapodized. */
+int ora_kb_adjoint3(size_t nz, size_t ny, size_t nx, size_t gz, size_t gy, size_t gx, int width, double beta,
+                    double oz, double oy, double ox, size_t samples, const double* traj, const double* values,
+                    const double* dcf, double* image);
+/* Golden-means kooshball: spoke p along (frac(p g1), sqrt(1-c^2) sin a, sqrt(1-c^2) cos a),
+ * a = 2 pi frac(p g2), g1 = 0.4656, g2 = 0.6823; readout as ora_golden_angle2; [S][3]. */
+void ora_golden_means3(size_t spokes, size_t readout, double* traj);
+
 #ifdef __cplusplus
 }
 #endif

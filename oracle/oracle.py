@@ -59,6 +59,12 @@ def lib():
         L.ora_kb_adjoint2.argtypes = [_sz, _sz, _sz, _sz, C.c_int, C.c_double, C.c_double, C.c_double, _sz,
                                       _dp, _dp, _dp, _dp]
         L.ora_golden_angle2.argtypes = [_sz, _sz, _dp]
+        L.ora_nudft3_adjoint.argtypes = [_sz, _sz, _sz, C.c_double, C.c_double, C.c_double, _sz, _dp, _dp, _dp]
+        L.ora_ramp_sq_dcf3.argtypes = [_sz, _dp, _dp]
+        L.ora_bin3.argtypes = [_sz, _fp, C.c_uint32, C.c_uint32, C.c_uint32, _up, _up, _up]
+        L.ora_kb_adjoint3.argtypes = [_sz, _sz, _sz, _sz, _sz, _sz, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, _sz, _dp, _dp, _dp, _dp]
+        L.ora_golden_means3.argtypes = [_sz, _sz, _dp]
         _lib = L
     return _lib
 
@@ -263,4 +269,74 @@ def ref_radial_route_rss(values_f32, traj_f32, rows: int, cols: int, scans: int 
                                        t.reshape(-1), out.reshape(-1), threads)
     if rc != 0:
         raise ValueError("reference route rejected its input")
+    return out
+
+
+# ---------------------------------------------------------------- 3-D (kooshball)
+# Coordinates [S][3] in (z, y, x) order, radians per pixel; volumes [z][y][x].  The
+# reference has no 3-D route: these extend its 2-D definitions axis by axis
+# (eigen_nufft_direct_nudft.cpp:77-97, eigen_radial_gridding.cpp:262-277).
+
+def _traj3(traj) -> np.ndarray:
+    t = np.ascontiguousarray(np.asarray(traj, np.float64))
+    if t.ndim != 2 or t.shape[1] != 3:
+        raise ValueError("3-D trajectory must be [S, 3]")
+    return t
+
+
+def nudft3_adjoint(values, traj, shape, origin=None) -> np.ndarray:
+    """Exact FP64 adjoint sum_s y_s exp(+i w . (r - origin)), no DCF, no normalisation."""
+    nz, ny, nx = shape
+    if origin is None:
+        origin = ((nz - 1) / 2.0, (ny - 1) / 2.0, (nx - 1) / 2.0)
+    v = _c128(values).reshape(-1)
+    t = _traj3(traj)
+    out = np.zeros(shape, np.complex128)
+    lib().ora_nudft3_adjoint(nz, ny, nx, *map(float, origin), t.shape[0], t.reshape(-1), v.view(np.float64),
+                             out.view(np.float64).reshape(-1))
+    return out
+
+
+def ramp_sq_dcf3(traj) -> np.ndarray:
+    t = _traj3(traj)
+    out = np.zeros(t.shape[0], np.float64)
+    lib().ora_ramp_sq_dcf3(t.shape[0], t.reshape(-1), out)
+    return out
+
+
+def bin3(traj_f32, grid):
+    """FP64 binning of float32 (z, y, x) coordinates: (bins, cell_start, perm)."""
+    t = np.ascontiguousarray(np.asarray(traj_f32, np.float32))
+    S = t.shape[0]
+    gz, gy, gx = grid
+    bins = np.zeros(S, np.uint32)
+    start = np.zeros(gz * gy * gx + 1, np.uint32)
+    perm = np.zeros(S, np.uint32)
+    if lib().ora_bin3(S, t.reshape(-1), gz, gy, gx, bins, start, perm) != 0:
+        raise ValueError("trajectory cannot be mapped to the grid")
+    return bins, start, perm
+
+
+def kb_adjoint3(values, traj, dcf, shape, oversampling=2.0, width=4, beta=None, origin=None) -> np.ndarray:
+    """FP64 KB gridding adjoint of one coil onto a sigma-oversampled grid, centred crop, deapodized."""
+    nz, ny, nx = shape
+    gz, gy, gx = (int(round(n * oversampling)) for n in shape)
+    if beta is None:
+        beta = kb_beta(width, oversampling)
+    if origin is None:
+        origin = ((nz - 1) / 2.0, (ny - 1) / 2.0, (nx - 1) / 2.0)
+    v = _c128(values).reshape(-1)
+    t = _traj3(traj)
+    d = np.ones(t.shape[0], np.float64) if dcf is None else np.ascontiguousarray(np.asarray(dcf, np.float64))
+    out = np.zeros(shape, np.complex128)
+    rc = lib().ora_kb_adjoint3(nz, ny, nx, gz, gy, gx, int(width), float(beta), *map(float, origin), t.shape[0],
+                               t.reshape(-1), v.view(np.float64), d, out.view(np.float64).reshape(-1))
+    if rc != 0:
+        raise ValueError("trajectory cannot be mapped to the grid")
+    return out
+
+
+def golden_means3(spokes: int, readout: int) -> np.ndarray:
+    out = np.zeros((spokes * readout, 3), np.float64)
+    lib().ora_golden_means3(spokes, readout, out.reshape(-1))
     return out

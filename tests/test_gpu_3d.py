@@ -1,0 +1,100 @@
+"""3-D KB adjoint (kooshball): the golden-means trajectory, bit-exact binning, and
+the volume reconstruction against the FP64 oracle (rel-L2 <= 1e-5) for unequal
+axes, every kernel width, multi-pass coil batches and work items cut short so
+blocks are summed from several partial slots; coil batching and repeat runs give
+bit-identical results."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.helpers import per_coil, rel_l2
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def ops():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from kspacejet_amd import ops as K
+    K.init(0)
+    return K
+
+
+def kspace(coils, samples, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal((coils, samples)) + 1j * rng.standard_normal((coils, samples))).astype(np.complex64)
+
+
+def oracle_volume(k, traj, shape, width, density):
+    t = traj.cpu().numpy().astype(np.float64)
+    dcf = O.ramp_sq_dcf3(t) if density == "ramp_squared" else None
+    return np.stack(per_coil(lambda c: O.kb_adjoint3(k[c], t, dcf, shape, width=width), k.shape[0]), axis=-1)
+
+
+def test_golden_means3_matches_the_oracle(ops):
+    spokes, readout = 301, 64
+    got = ops.golden_means3(spokes, readout).cpu().numpy().astype(np.float64)
+    ref = O.golden_means3(spokes, readout)
+    assert np.max(np.abs(got - ref)) <= 4e-7  # FP64 on both sides, rounded to float once
+
+
+@pytest.mark.parametrize("kind", ["kooshball", "random"])
+def test_binning_is_bit_exact(ops, kind):
+    shape = (12, 10, 14)
+    if kind == "kooshball":
+        traj = ops.golden_means3(97, 24)
+    else:
+        rng = np.random.default_rng(3)
+        traj = torch.from_numpy(rng.uniform(-np.pi, np.pi, (5000, 3)).astype(np.float32)).cuda()
+    plan = ops.Plan3(shape, traj.shape[0]).set_trajectory(traj)
+    start, perm = plan.export_bins()
+    _, rstart, rperm = O.bin3(traj.cpu().numpy(), tuple(2 * n for n in shape))
+    assert np.array_equal(start.cpu().numpy().astype(np.uint32), rstart)
+    assert np.array_equal(perm.cpu().numpy().astype(np.uint32), rperm)
+    plan.close()
+
+
+@pytest.mark.parametrize("shape,coils,spokes,readout,width,density,cpp,item_visits", [
+    ((16, 16, 16), 3, 120, 32, 4, "ramp_squared", 8, 0),
+    ((12, 14, 10), 12, 90, 28, 6, "none", 5, 0),       # unequal axes, passes of 5, 5, 2 coils
+    ((10, 10, 10), 20, 80, 20, 8, "ramp_squared", 8, 7),  # short items: most blocks summed from partial slots
+    ((14, 14, 14), 9, 100, 28, 2, "ramp_squared", 16, 0),
+])
+def test_volume_matches_the_oracle(ops, shape, coils, spokes, readout, width, density, cpp, item_visits):
+    traj = ops.golden_means3(spokes, readout)
+    k = kspace(coils, traj.shape[0], sum(shape) + coils)
+    plan = ops.Plan3(shape, traj.shape[0], width=width, density=density, item_visits=item_visits).set_trajectory(traj)
+    img, rss = plan.adjoint(torch.from_numpy(k).cuda(), rss=True, coils_per_pass=cpp)
+    torch.cuda.synchronize()
+    ref = oracle_volume(k, traj, shape, width, density)
+    assert rel_l2(img.cpu().numpy(), ref) <= TOL
+    assert rel_l2(rss.cpu().numpy(), np.sqrt(np.sum(np.abs(ref) ** 2, axis=-1))) <= TOL
+    plan.close()
+
+
+def test_coil_batching_and_repeats_are_bit_identical(ops):
+    shape = (12, 12, 12)
+    traj = ops.golden_means3(110, 24)
+    k = torch.from_numpy(kspace(13, traj.shape[0], 9)).cuda()
+    plan = ops.Plan3(shape, traj.shape[0]).set_trajectory(traj)
+    runs = [plan.adjoint(k, rss=True, coils_per_pass=c) for c in (13, 4, 13)]
+    torch.cuda.synchronize()
+    for img, rss in runs[1:]:
+        assert torch.equal(img, runs[0][0]) and torch.equal(rss, runs[0][1])
+    plan.close()
+
+
+def test_random_trajectory_to_the_grid_edge(ops):
+    shape = (8, 9, 10)
+    rng = np.random.default_rng(11)
+    traj = torch.from_numpy(rng.uniform(-np.pi, np.pi, (3000, 3)).astype(np.float32)).cuda()
+    k = kspace(4, 3000, 12)
+    plan = ops.Plan3(shape, 3000, density="none").set_trajectory(traj)
+    img, _ = plan.adjoint(torch.from_numpy(k).cuda())
+    torch.cuda.synchronize()
+    assert rel_l2(img.cpu().numpy(), oracle_volume(k, traj, shape, 4, "none")) <= TOL
+    plan.close()
