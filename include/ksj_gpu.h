@@ -190,6 +190,19 @@ int ksj_gpu_adjoint2_execute_multi(uint32_t count, const ksj_gpu_plan2* const* p
                                    const uint32_t* channels, void* const* d_images, int layout, float* const* d_rss,
                                    void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* Route trajectory normalisation: raw ISMRMRD-order [S][kx, ky] -> the Provider payload
+ * order [S][ky, kx] in rad/px, each axis normalised with its own encoded dimension, float
+ * arithmetic as the reference.  Replaces normalize_radial_coordinate(_pair) of
+ * libs/recon/kspacejet-recon-runtime/src/noncartesian_rss_hdf5.cpp:776-834 (same units,
+ * same inclusive Nyquist bounds).  Returns KSJ_GPU_INVALID_ARGUMENT with the first failing
+ * coordinate's reason (sample order, kx before ky).  Synchronises `stream`. */
+#define KSJ_GPU_TRAJ_UNITS_UNSPECIFIED 0
+#define KSJ_GPU_TRAJ_UNITS_CYCLES_PER_FOV 1
+#define KSJ_GPU_TRAJ_UNITS_RADIANS_PER_PIXEL 2
+#define KSJ_GPU_TRAJ_UNITS_ENCODED_MATRIX_INDEX 3
+int ksj_gpu_normalize_trajectory2(const float* d_raw, uint64_t samples, int units, uint32_t encoded_rows,
+                                  uint32_t encoded_cols, float* d_out, void* stream);
+
 /* ------------------------------------------------------------------ 3-D (kooshball) */
 
 /* 3-D KB gridding adjoint.  No reference counterpart: the reference reconstructs 2-D

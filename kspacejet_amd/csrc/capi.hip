@@ -337,6 +337,14 @@ int ksj_gpu_adjoint3_execute(const ksj_gpu_plan3* plan, const void* d_kspace, ui
   });
 }
 
+int ksj_gpu_normalize_trajectory2(const float* d_raw, uint64_t samples, int units, uint32_t encoded_rows,
+                                  uint32_t encoded_cols, float* d_out, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(samples == 0 || (d_raw && d_out), "null trajectory");
+    ksj::gpu::normalize_trajectory2(d_raw, samples, units, encoded_rows, encoded_cols, d_out, as_stream(stream));
+  });
+}
+
 int ksj_gpu_golden_means3(float* d_traj, uint32_t spokes, uint32_t readout, void* stream) {
   return guarded([&] {
     ksj::gpu::require(d_traj != nullptr, "null trajectory");

@@ -303,6 +303,46 @@ __global__ void golden_angle_kernel(float* traj, uint32_t spokes, uint32_t reado
   traj[2 * i + 1] = static_cast<float>(k * cs);
 }
 
+// Route trajectory normalisation (noncartesian_rss_hdf5.cpp:776-834), float arithmetic
+// as there: raw [S][kx, ky] -> [S][ky, kx] rad/px.  A failure records
+// ((2 s + axis) << 8 | code) with atomicMin, so the host reports the first failing
+// coordinate in the reference's order (kx before ky, sample by sample).
+struct NormArgs {
+  const float* raw;
+  float* out;
+  uint64_t samples;
+  int units;
+  float limit_col, limit_row;   // Nyquist bound per axis in input units
+  float scale_col, scale_row;   // multiplier to rad/px (float, computed on the host)
+  unsigned long long* first_error;
+};
+
+__device__ inline int normalize_one(float c, int units, float limit, float scale, float* out) {
+  if (!isfinite(c)) return 1;
+  if (units < 1 || units > 3) return 3;
+  if (fabsf(c) > limit) return 2;
+  *out = units == 2 ? c : c * scale;
+  return 0;
+}
+
+__global__ void normalize_kernel(NormArgs a) {
+  const uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= a.samples) return;
+  float col = 0.f, row = 0.f;
+  int rc = normalize_one(a.raw[2 * s], a.units, a.limit_col, a.scale_col, &col);
+  uint64_t axis = 0;
+  if (rc == 0) {
+    rc = normalize_one(a.raw[2 * s + 1], a.units, a.limit_row, a.scale_row, &row);
+    axis = 1;
+  }
+  if (rc != 0) {
+    atomicMin(a.first_error, static_cast<unsigned long long>(((2 * s + axis) << 8) | static_cast<uint64_t>(rc)));
+    return;
+  }
+  a.out[2 * s] = row;
+  a.out[2 * s + 1] = col;
+}
+
 __global__ void ramp_dcf_kernel(const float* traj, float* dcf, uint64_t samples) {
   const uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= samples) return;
@@ -615,6 +655,54 @@ void golden_angle2(float* d_traj, uint32_t spokes, uint32_t readout, hipStream_t
   if (n == 0) return;
   hipLaunchKernelGGL(golden_angle_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, stream, d_traj, spokes, readout);
   KSJ_LAUNCH_CHECK();
+}
+
+void normalize_trajectory2(const float* d_raw, uint64_t samples, int units, uint32_t encoded_rows,
+                           uint32_t encoded_cols, float* d_out, hipStream_t stream) {
+  require(units >= 1 && units <= 3, "trajectory units are unspecified");
+  require(units != 3 || (encoded_rows >= 2 && encoded_cols >= 2),
+          "encoded_matrix_index requires encoded dimensions >= 2");
+  if (samples == 0) return;
+  const float pi_f = 3.14159265358979323846f;
+  NormArgs a{};
+  a.raw = d_raw;
+  a.out = d_out;
+  a.samples = samples;
+  a.units = units;
+  if (units == 1) {
+    a.limit_col = a.limit_row = 0.5f;
+    a.scale_col = a.scale_row = 2.0f * pi_f;
+  } else if (units == 2) {
+    a.limit_col = a.limit_row = pi_f;
+    a.scale_col = a.scale_row = 1.0f;
+  } else {
+    a.limit_col = static_cast<float>(encoded_cols) / 2.0f;
+    a.limit_row = static_cast<float>(encoded_rows) / 2.0f;
+    a.scale_col = 2.0f * pi_f / static_cast<float>(encoded_cols);
+    a.scale_row = 2.0f * pi_f / static_cast<float>(encoded_rows);
+  }
+  unsigned long long* d_err = nullptr;
+  KSJ_HIP(hipMalloc(reinterpret_cast<void**>(&d_err), sizeof(unsigned long long)));
+  a.first_error = d_err;
+  unsigned long long host_err = ~0ull;
+  try {
+    KSJ_HIP(hipMemcpyAsync(d_err, &host_err, sizeof(host_err), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(normalize_kernel, dim3(ceil_div(samples, 256)), dim3(256), 0, stream, a);
+    KSJ_LAUNCH_CHECK();
+    KSJ_HIP(hipMemcpyAsync(&host_err, d_err, sizeof(host_err), hipMemcpyDeviceToHost, stream));
+    KSJ_HIP(hipStreamSynchronize(stream));
+  } catch (...) {
+    (void)hipFree(d_err);
+    throw;
+  }
+  KSJ_HIP(hipFree(d_err));
+  if (host_err != ~0ull) {
+    switch (host_err & 0xFF) {
+      case 1: throw Error(1, "trajectory coordinate must be finite");
+      case 2: throw Error(1, "trajectory coordinate is outside the inclusive Nyquist interval of its units");
+      default: throw Error(1, "trajectory units are unspecified");
+    }
+  }
 }
 
 void ramp_dcf2(const float* d_traj, float* d_dcf, uint64_t samples, hipStream_t stream) {

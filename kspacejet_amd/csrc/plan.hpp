@@ -141,6 +141,13 @@ void cell_starts(const uint32_t* d_sorted_keys, uint32_t samples, uint64_t nbins
 // written as [S][2] (ky, kx) in radians per pixel.
 void golden_angle2(float* d_traj, uint32_t spokes, uint32_t readout, hipStream_t stream);
 
+// Route trajectory normalisation (noncartesian_rss_hdf5.cpp:776-834): raw [S][kx, ky] in
+// units 1 = cycles/FOV, 2 = rad/px, 3 = encoded matrix index -> [S][ky, kx] rad/px, float
+// arithmetic as the reference.  Synchronises `stream`; throws Error(1) naming the first
+// failing coordinate's reason (non-finite, outside the Nyquist interval, unspecified units).
+void normalize_trajectory2(const float* d_raw, uint64_t samples, int units, uint32_t encoded_rows,
+                           uint32_t encoded_cols, float* d_out, hipStream_t stream);
+
 // Analytic ramp DCF hypot(w_r, w_c) per sample in float (eigen_radial_gridding.cpp:279-295).
 void ramp_dcf2(const float* d_traj, float* d_dcf, uint64_t samples, hipStream_t stream);
 

@@ -305,6 +305,18 @@ class Server:
         self.close()
 
 
+TRAJ_UNITS = {"unspecified": 0, "cycles_per_fov": 1, "radians_per_pixel": 2, "encoded_matrix_index": 3}
+
+
+def normalize_trajectory2(raw: torch.Tensor, units: str, encoded_rows: int = 0, encoded_cols: int = 0) -> torch.Tensor:
+    """Raw [S, 2] (kx, ky) in `units` -> [S, 2] (ky, kx) radians per pixel (ksj_gpu_normalize_trajectory2)."""
+    _need(raw, torch.float32, "raw trajectory")
+    out = torch.empty_like(raw)
+    call("ksj_gpu_normalize_trajectory2", _ptr(raw), raw.shape[0], TRAJ_UNITS[units], encoded_rows, encoded_cols,
+         _ptr(out), _stream())
+    return out
+
+
 def golden_means3(spokes: int, readout: int, device="cuda") -> torch.Tensor:
     """Golden-means kooshball [spokes * readout, 3] (z, y, x) in radians per pixel."""
     out = torch.empty((spokes * readout, 3), dtype=torch.float32, device=device)

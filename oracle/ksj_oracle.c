@@ -585,3 +585,48 @@ void ora_golden_means3(size_t spokes, size_t readout, double* traj) {
     }
   }
 }
+
+/* ---------------------------------------------------------------- trajectory units */
+
+static int normalize_coordinate(float c, int units, uint32_t encoded, float* out) {
+  /* noncartesian_rss_hdf5.cpp:776-808, float arithmetic as there. */
+  const float pi_f = 3.14159265358979323846f;
+  if (!isfinite(c)) return 1;
+  switch (units) {
+    case 1: /* cycles_per_fov */
+      if (fabsf(c) > 0.5f) return 2;
+      *out = c * (2.0f * pi_f);
+      return 0;
+    case 2: /* radians_per_pixel */
+      if (fabsf(c) > pi_f) return 2;
+      *out = c;
+      return 0;
+    case 3: { /* encoded_matrix_index */
+      if (encoded < 2) return 4;
+      const float half = (float)encoded / 2.0f;
+      if (fabsf(c) > half) return 2;
+      *out = c * (2.0f * pi_f / (float)encoded);
+      return 0;
+    }
+    default:
+      return 3;
+  }
+}
+
+int ora_normalize_trajectory2(size_t samples, const float* raw, int units, uint32_t encoded_rows,
+                              uint32_t encoded_cols, float* out) {
+  /* noncartesian_rss_hdf5.cpp:815-834: raw [kx, ky] per sample, each axis normalised with
+   * its own encoded dimension (kx first), written [row = ky, column = kx].  Returns 0 or
+   * the code of the first failure: 1 non-finite, 2 outside the Nyquist interval,
+   * 3 units unspecified, 4 encoded dimension < 2. */
+  for (size_t s = 0; s < samples; ++s) {
+    float col, row;
+    int rc = normalize_coordinate(raw[2 * s], units, encoded_cols, &col);
+    if (rc) return rc;
+    rc = normalize_coordinate(raw[2 * s + 1], units, encoded_rows, &row);
+    if (rc) return rc;
+    out[2 * s] = row;
+    out[2 * s + 1] = col;
+  }
+  return 0;
+}

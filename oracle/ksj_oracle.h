@@ -98,6 +98,13 @@ int ora_kb_adjoint2(size_t n_rows, size_t n_cols, size_t g_rows, size_t g_cols, 
  * readout k_j = (j - M/2) / M cycles per pixel, stored as [S][2] (ky, kx) radians per pixel. */
 void ora_golden_angle2(size_t spokes, size_t readout, double* traj);
 
+/* Route trajectory normalisation (noncartesian_rss_hdf5.cpp:776-834): raw [S][kx, ky] in
+ * units 1 = cycles/FOV, 2 = rad/px, 3 = encoded matrix index (0 = unspecified) ->
+ * [S][ky, kx] rad/px.  Returns 0 or the first failure: 1 non-finite, 2 outside the
+ * Nyquist interval, 3 unspecified units, 4 encoded dimension < 2. */
+int ora_normalize_trajectory2(size_t samples, const float* raw, int units, uint32_t encoded_rows,
+                              uint32_t encoded_cols, float* out);
+
 /* ---- 3-D: coordinates [S][3] in (z, y, x) order, radians per pixel; images [z][y][x]. */
 
 /* x(z,y,x) = sum_s y_s exp(+i(w_z (z-oz) + w_y (y-oy) + w_x (x-ox))), no normalisation. */

@@ -65,6 +65,7 @@ def lib():
         L.ora_kb_adjoint3.argtypes = [_sz, _sz, _sz, _sz, _sz, _sz, C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_double, _sz, _dp, _dp, _dp, _dp]
         L.ora_golden_means3.argtypes = [_sz, _sz, _dp]
+        L.ora_normalize_trajectory2.argtypes = [_sz, _fp, C.c_int, C.c_uint32, C.c_uint32, _fp]
         _lib = L
     return _lib
 
@@ -340,3 +341,18 @@ def golden_means3(spokes: int, readout: int) -> np.ndarray:
     out = np.zeros((spokes * readout, 3), np.float64)
     lib().ora_golden_means3(spokes, readout, out.reshape(-1))
     return out
+
+
+# ---------------------------------------------------------------- trajectory units
+TRAJ_UNITS = {"unspecified": 0, "cycles_per_fov": 1, "radians_per_pixel": 2, "encoded_matrix_index": 3}
+
+
+def normalize_trajectory2(raw_kx_ky, units: str, encoded_rows: int, encoded_cols: int):
+    """Route normalisation (noncartesian_rss_hdf5.cpp:776-834): ([S, 2] (ky, kx) float32 rad/px, 0)
+    or (None, code) with the first failure's code (1 non-finite, 2 outside Nyquist, 3 unspecified,
+    4 encoded dimension < 2)."""
+    raw = np.ascontiguousarray(np.asarray(raw_kx_ky, np.float32))
+    out = np.zeros_like(raw)
+    rc = lib().ora_normalize_trajectory2(raw.shape[0], raw.reshape(-1), TRAJ_UNITS[units], encoded_rows,
+                                         encoded_cols, out.reshape(-1))
+    return (out, 0) if rc == 0 else (None, rc)

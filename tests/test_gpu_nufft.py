@@ -241,3 +241,33 @@ def test_batched_scans_equal_single_scans_bitwise(ops, coils, count):
         assert torch.equal(imgs[i], one_img) and torch.equal(rsss[i], one_rss)
     ref = oracle_kb(scans[-1].cpu().numpy(), traj.cpu().numpy(), n)
     assert rel_l2(imgs[-1].cpu().numpy(), ref) <= TOL
+
+
+# ---------------------------------------------------------------- trajectory units (a13)
+
+@pytest.mark.parametrize("units,lo,hi,enc", [("cycles_per_fov", -0.5, 0.5, (2, 2)),
+                                             ("radians_per_pixel", -3.14159, 3.14159, (2, 2)),
+                                             ("encoded_matrix_index", -40.0, 40.0, (96, 80))])
+def test_trajectory_normalisation_is_bit_exact(ops, units, lo, hi, enc):
+    rng = np.random.default_rng(4)
+    raw = rng.uniform(lo, hi, (100000, 2)).astype(np.float32)
+    raw[:3] = [[lo, hi], [0.0, 0.0], [hi, lo]]  # the inclusive bounds
+    got = ops.normalize_trajectory2(torch.from_numpy(raw).cuda(), units, *enc).cpu().numpy()
+    ref, rc = O.normalize_trajectory2(raw, units, *enc)
+    assert rc == 0 and np.array_equal(got, ref)
+
+
+def test_trajectory_normalisation_reports_the_first_failure(ops):
+    from kspacejet_amd._lib import KsjGpuError
+    raw = np.zeros((1000, 2), np.float32)
+    raw[700, 0] = np.nan       # later sample, non-finite
+    raw[300, 1] = 0.75         # first failure: sample 300's ky outside the cycles/FOV interval
+    with pytest.raises(KsjGpuError, match="Nyquist"):
+        ops.normalize_trajectory2(torch.from_numpy(raw).cuda(), "cycles_per_fov", 2, 2)
+    raw[300, 1] = 0.0
+    with pytest.raises(KsjGpuError, match="finite"):
+        ops.normalize_trajectory2(torch.from_numpy(raw).cuda(), "cycles_per_fov", 2, 2)
+    with pytest.raises(KsjGpuError, match="unspecified"):
+        ops.normalize_trajectory2(torch.from_numpy(np.zeros((4, 2), np.float32)).cuda(), "unspecified", 2, 2)
+    with pytest.raises(KsjGpuError, match="encoded"):
+        ops.normalize_trajectory2(torch.from_numpy(np.zeros((4, 2), np.float32)).cuda(), "encoded_matrix_index", 1, 2)
