@@ -22,8 +22,9 @@ image.  `value` is served images per second over the timed steps.  Beside it the
     one autotuned multi-plan call vs that call replayed as a captured HIP graph;
   * stages / roofline: per-kernel HIP-event timings of the batched pass (one pass
     lane, so kernels do not overlap) and the gather kernel against HBM bandwidth;
-  * mixed: a mixed 2-D stream (radial scans of the headline shape and Cartesian
-    scans of 128/256/384 matrix with 8-32 coils), served vs plain;
+  * mixed: the reference's mixed stream (60 % radial scans of the headline shape, 30 %
+    Cartesian scans of 128/256/384 matrix with 8-32 coils, 10 % 3-D kooshball volumes at
+    half the 3-D spoke count), served vs plain;
   * cpu_baseline: the reference's own radial route compiled from its sources.
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1]
@@ -289,7 +290,10 @@ def main(argv=None) -> int:
     mixed = None
     if args.mixed_requests > 0:
         log("mixed stream ...")
-        mixed = run_mixed(torch, lib, _lib, server, plan, scans, traj, traj_key, args.mixed_requests, stream)
+        mixed = {"with_volumes": run_mixed(torch, lib, _lib, server, plan, scans, traj, traj_key, args.mixed_requests,
+                                           stream, 0.1),
+                 "2d_only": run_mixed(torch, lib, _lib, server, plan, scans, traj, traj_key, args.mixed_requests,
+                                      stream, 0.0)}
     server.close()
     del scans, outs, ws
     provider = None
@@ -547,16 +551,28 @@ def run_uneven(torch, ops, lib, count, steps, warmup, tune_seconds):
     return res
 
 
-def run_mixed(torch, lib, _lib, server, plan, radial_scans, traj, traj_key, count, stream):
-    """Mixed 2-D stream: two thirds radial scans of the headline shape, one third Cartesian
-    scans (matrix 128/256/384, 8-32 coils, centred), served vs one synchronous call each."""
-    from kspacejet_amd import synth
+def run_mixed(torch, lib, _lib, server, plan, radial_scans, traj, traj_key, count, stream, volume_fraction=0.1):
+    """Mixed request stream in the reference's proportions: 60 % radial scans of the
+    headline shape, 30 % Cartesian scans (matrix 128/256/384, 8-32 coils, centred), 10 %
+    3-D kooshball volumes at half the 3-D spoke count (256^3, 20 coils, 8192 x 512
+    golden-means spokes, |k|^2 density).  Served (queued at once on the server) vs one
+    synchronous plain call per request with resident plans; images/s counts every
+    request, 2-D image or 3-D volume."""
+    from kspacejet_amd import ops, synth
     rng = np.random.default_rng(11)
     S = SPOKES * READOUT
     cart_shapes = [(int(rng.choice([128, 256, 384])), int(rng.integers(8, 33))) for _ in range(6)]
     cart = [torch.from_numpy(synth.cartesian_kspace2(n, c, synth.scan_seed(5, i)).astype(np.complex64)).cuda()
             for i, (n, c) in enumerate(cart_shapes)]
-    kinds = rng.permutation(["radial"] * (2 * count // 3) + ["cartesian"] * (count - 2 * count // 3))
+    n3 = round(volume_fraction * count)
+    nc = round(0.3 * count)
+    kinds = rng.permutation(["radial"] * (count - n3 - nc) + ["cartesian"] * nc + ["volume"] * n3)
+    vol_n, vol_coils, vol_spokes = 256, 20, 8192
+    traj3 = ops.golden_means3(vol_spokes, READOUT)
+    S3 = traj3.shape[0]
+    gen = torch.Generator(device="cuda").manual_seed(0x4B53504A ^ (5 << 32))
+    k3 = torch.randn((vol_coils, S3), dtype=torch.complex64, device="cuda", generator=gen)
+    traj3_key = int.from_bytes(hashlib.sha256(traj3.cpu().numpy().tobytes()).digest()[:8], "little") | 1
     reqs, jobs = [], []
     for i, kind in enumerate(kinds):
         if kind == "radial":
@@ -565,12 +581,18 @@ def run_mixed(torch, lib, _lib, server, plan, radial_scans, traj, traj_key, coun
             reqs.append(_lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_RADIAL2D, N, N, COILS, S, WIDTH,
                                           _lib.DENSITY_RAMP, 0, 0, traj_key, traj.data_ptr(), k.data_ptr(), None,
                                           o.data_ptr()))
-        else:
+        elif kind == "cartesian":
             k = cart[i % len(cart)]
             c, n, _ = k.shape
             o = torch.empty((n, n), dtype=torch.float32, device="cuda")
             reqs.append(_lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_CARTESIAN2D, n, n, c, 0, 0, 0, 1, 0,
                                           0, None, k.data_ptr(), None, o.data_ptr()))
+        else:
+            k = k3
+            o = torch.empty((vol_n, vol_n, vol_n), dtype=torch.float32, device="cuda")
+            reqs.append(_lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_RADIAL3D, vol_n, vol_n, vol_coils,
+                                          S3, WIDTH, _lib.DENSITY_RAMP_SQUARED, 0, vol_n, traj3_key,
+                                          traj3.data_ptr(), k.data_ptr(), None, o.data_ptr()))
         jobs.append((kind, k, o))
     arr = (_lib.ServeRequest * count)(*reqs)
     tickets = (C.c_uint64 * count)()
@@ -583,6 +605,8 @@ def run_mixed(torch, lib, _lib, server, plan, radial_scans, traj, traj_key, coun
         server.drain()
         return [server.wait(t) for t in tickets]
 
+    plan3 = ops.Plan3((vol_n, vol_n, vol_n), S3, width=WIDTH).set_trajectory(traj3) if n3 else None
+    ws3 = torch.empty(plan3.workspace_bytes(vol_coils, 8) if n3 else 1, dtype=torch.uint8, device="cuda")
     ws = torch.empty(max(plan.workspace_bytes(COILS),
                          max(lib.ksj_gpu_ifft2_workspace_bytes(c, n, n) for n, c in cart_shapes)),
                      dtype=torch.uint8, device="cuda")
@@ -593,11 +617,14 @@ def run_mixed(torch, lib, _lib, server, plan, radial_scans, traj, traj_key, coun
             if kind == "radial":
                 lib.ksj_gpu_adjoint2_execute(plan._handle, k.data_ptr(), COILS, None, 0, o.data_ptr(), ws.data_ptr(),
                                              ws.numel(), stream())
-            else:
+            elif kind == "cartesian":
                 c, n, _ = k.shape
                 lib.ksj_gpu_ifft2_coil_images(k.data_ptr(), coil_buf.data_ptr(), c, n, n, 1, 0, ws.data_ptr(),
                                               ws.numel(), stream())
                 lib.ksj_gpu_rss(coil_buf.data_ptr(), o.data_ptr(), n * n, c, stream())
+            else:
+                lib.ksj_gpu_adjoint3_execute(plan3._handle, k.data_ptr(), vol_coils, 8, None, o.data_ptr(),
+                                             ws3.data_ptr(), ws3.numel(), stream())
             torch.cuda.synchronize()
 
     served()
@@ -608,12 +635,16 @@ def run_mixed(torch, lib, _lib, server, plan, radial_scans, traj, traj_key, coun
     t0 = time.perf_counter()
     plain()
     t_plain = time.perf_counter() - t0
+    if plan3 is not None:
+        plan3.close()
+    del ws3, k3
     return {"requests": count, "radial": int((kinds == "radial").sum()), "cartesian": int((kinds == "cartesian").sum()),
+            "volumes": int((kinds == "volume").sum()),
             "cartesian_shapes": [f"{n}x{n}x{c}" for n, c in cart_shapes],
-            "served_images_per_s": round(count / t_served, 1), "plain_images_per_s": round(count / t_plain, 1),
+            "volume_shape": f"{vol_n}^3 x {vol_coils} coils, {vol_spokes}x{READOUT} spokes",
+            "served_requests_per_s": round(count / t_served, 1), "plain_requests_per_s": round(count / t_plain, 1),
             "served_over_plain": round(t_plain / t_served, 2), "burst_p50_ms": round(percentile(lat, 50), 3),
-            "burst_p99_ms": round(percentile(lat, 99), 3),
-            "note": "3-D radial requests of the reference stream are not part of this build"}
+            "burst_p99_ms": round(percentile(lat, 99), 3)}
 
 
 if __name__ == "__main__":

@@ -285,6 +285,8 @@ void ksj_gpu_graph_destroy(ksj_gpu_graph* graph);
  * result equals the result of the same request run alone. */
 #define KSJ_GPU_SERVE_RADIAL2D 0     /* KB adjoint NUFFT, grid 2N, ramp or no density weights */
 #define KSJ_GPU_SERVE_CARTESIAN2D 1  /* inverse FFT (Normalization::inverse), centred or not */
+#define KSJ_GPU_SERVE_RADIAL3D 2     /* 3-D KB adjoint of a volume, grid 2N per axis, |k|^2 or no
+                                        density weights; runs alone, coils in passes of 8 */
 
 typedef struct ksj_gpu_serve_config {
   uint32_t struct_size;
@@ -304,9 +306,9 @@ typedef struct ksj_gpu_serve_request {
   uint32_t kernel_width;         /* radial: 4, 6 or 8 */
   uint32_t density;              /* radial: KSJ_GPU_DENSITY_NONE or _RAMP */
   uint32_t centred;              /* Cartesian */
-  uint32_t reserved0;
+  uint32_t image_depth;          /* 3-D: N_z (image_rows = N_y, image_cols = N_x); 0 otherwise */
   uint64_t trajectory_key;       /* radial: names the trajectory bytes; 0 = the server hashes them */
-  const float* d_traj;           /* radial: [S][ky,kx] rad/px */
+  const float* d_traj;           /* radial: [S][ky,kx] rad/px; 3-D: [S][kz,ky,kx] */
   const void* d_kspace;          /* [C][S] (radial) or [C][R][Cols] (Cartesian) cf32 */
   void* d_images;                /* coil-last [R][Cols][C] cf32, or NULL */
   float* d_rss;                  /* [R][Cols] float, or NULL */

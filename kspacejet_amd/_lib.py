@@ -71,7 +71,7 @@ class ServeRequest(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("kind", C.c_uint32), ("image_rows", C.c_uint32),
                 ("image_cols", C.c_uint32), ("channels", C.c_uint32), ("samples", C.c_uint32),
                 ("kernel_width", C.c_uint32), ("density", C.c_uint32), ("centred", C.c_uint32),
-                ("reserved0", C.c_uint32), ("trajectory_key", C.c_uint64), ("d_traj", C.c_void_p),
+                ("image_depth", C.c_uint32), ("trajectory_key", C.c_uint64), ("d_traj", C.c_void_p),
                 ("d_kspace", C.c_void_p), ("d_images", C.c_void_p), ("d_rss", C.c_void_p)]
 
 
@@ -83,6 +83,7 @@ class ServeStats(C.Structure):
 
 SERVE_RADIAL2D = 0
 SERVE_CARTESIAN2D = 1
+SERVE_RADIAL3D = 2
 
 _vp = C.c_void_p
 _u32 = C.c_uint32

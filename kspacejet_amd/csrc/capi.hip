@@ -493,6 +493,7 @@ ksj::gpu::ServeRequest to_request(const ksj_gpu_serve_request* request) {
   r.kernel_width = request->kernel_width;
   r.density = request->density;
   r.centred = request->centred;
+  r.image_depth = request->image_depth;
   r.trajectory_key = request->trajectory_key;
   r.d_traj = request->d_traj;
   r.d_kspace = static_cast<const float2*>(request->d_kspace);

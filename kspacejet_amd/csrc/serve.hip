@@ -3,6 +3,7 @@
 #include "combine.hpp"
 #include "grid.hpp"
 #include "recon.hpp"
+#include "recon3.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -41,12 +42,12 @@ double ms_since(std::chrono::steady_clock::time_point t) {
 
 }  // namespace
 
-uint64_t trajectory_hash(const float* d_traj, uint32_t samples, hipStream_t stream) {
+uint64_t trajectory_hash(const float* d_traj, uint64_t values, hipStream_t stream) {
   uint64_t* d_out = nullptr;
   uint64_t h = 0;
   KSJ_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_out), sizeof(uint64_t), stream));
   hipLaunchKernelGGL(hash_kernel, dim3(1), dim3(kHashThreads), 0, stream, reinterpret_cast<const uint32_t*>(d_traj),
-                     static_cast<uint64_t>(samples) * 2, d_out);
+                     values, d_out);
   KSJ_LAUNCH_CHECK();
   KSJ_HIP(hipMemcpyAsync(&h, d_out, sizeof(h), hipMemcpyDeviceToHost, stream));
   KSJ_HIP(hipFreeAsync(d_out, stream));
@@ -56,8 +57,8 @@ uint64_t trajectory_hash(const float* d_traj, uint32_t samples, hipStream_t stre
 
 uint64_t Server::key_of(const ServeRequest& r) {
   uint64_t h = mix(0, r.kind);
-  for (uint64_t v : {uint64_t{r.image_rows}, uint64_t{r.image_cols}, uint64_t{r.samples}, uint64_t{r.kernel_width},
-                     uint64_t{r.density}, uint64_t{r.centred}, r.trajectory_key})
+  for (uint64_t v : {uint64_t{r.image_rows}, uint64_t{r.image_cols}, uint64_t{r.image_depth}, uint64_t{r.samples},
+                     uint64_t{r.kernel_width}, uint64_t{r.density}, uint64_t{r.centred}, r.trajectory_key})
     h = mix(h, v);
   return h;
 }
@@ -98,26 +99,34 @@ Server::~Server() {
 }
 
 Server::Pending Server::prepare(const ServeRequest& r) {
-  require(r.kind == kServeRadial2D || r.kind == kServeCartesian2D, "unknown request kind");
-  require(r.channels >= 1 && r.channels <= 64, "request channels must be in [1, 64]");
+  require(r.kind == kServeRadial2D || r.kind == kServeCartesian2D || r.kind == kServeRadial3D,
+          "unknown request kind");
+  require(r.channels >= 1 && (r.kind == kServeRadial3D || r.channels <= 64),
+          "request channels must be in [1, 64] (3-D volumes: any count, taken in passes)");
   require(r.image_rows >= 2 && r.image_cols >= 2 && r.image_rows <= 4096 && r.image_cols <= 4096,
           "request image dimensions out of range");
   require(r.d_kspace != nullptr && (r.d_images != nullptr || r.d_rss != nullptr), "request needs k-space and an output");
   Pending p{};
   p.req = r;
-  if (r.kind == kServeRadial2D) {
+  if (r.kind == kServeRadial2D || r.kind == kServeRadial3D) {
+    const bool three = r.kind == kServeRadial3D;
     require(r.samples >= 1 && r.d_traj != nullptr, "radial request needs a trajectory");
     require(r.kernel_width == 4 || r.kernel_width == 6 || r.kernel_width == 8, "kernel width must be 4, 6 or 8");
-    require(r.density == kDensityNone || r.density == kDensityRamp, "density must be none or ramp");
+    require(three ? (r.density == kDensityNone || r.density == kDensityRampSquared)
+                  : (r.density == kDensityNone || r.density == kDensityRamp),
+            "density must be none or ramp (2-D) / ramp squared (3-D)");
+    require(!three || (r.image_depth >= 2 && r.image_depth <= 1024), "3-D request needs an image depth in [2, 1024]");
+    if (!three) p.req.image_depth = 0;
     if (p.req.trajectory_key == 0) {
       KSJ_HIP(hipSetDevice(cfg_.device));
       hipStream_t s = nullptr;
       KSJ_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-      p.req.trajectory_key = trajectory_hash(r.d_traj, r.samples, s) | 1ull;
+      p.req.trajectory_key = trajectory_hash(r.d_traj, static_cast<uint64_t>(r.samples) * (three ? 3 : 2), s) | 1ull;
       KSJ_HIP(hipStreamDestroy(s));
     }
   } else {
     p.req.samples = 0;
+    p.req.image_depth = 0;
     p.req.trajectory_key = 0;
   }
   p.plan_key = key_of(p.req);
@@ -207,14 +216,46 @@ void Server::evict_until(uint64_t need_bytes) {
   }
 }
 
-Plan2* Server::plan_for(const Pending& p, hipStream_t stream) {
-  auto it = plans_.find(p.plan_key);
-  if (it != plans_.end()) {
-    ++stats_.plan_hits;
-    lru_.remove(p.plan_key);
-    lru_.push_front(p.plan_key);
-    return it->second.plan.get();
+Server::PlanEntry* Server::cached_plan(uint64_t key) {
+  auto it = plans_.find(key);
+  if (it == plans_.end()) return nullptr;
+  ++stats_.plan_hits;
+  lru_.remove(key);
+  lru_.push_front(key);
+  return &it->second;
+}
+
+Plan3* Server::plan3_for(const Pending& p, hipStream_t stream) {
+  if (PlanEntry* e = cached_plan(p.plan_key)) return e->plan3.get();
+  while (plans_.size() >= cfg_.plan_cache_entries && evict_one()) {
   }
+  const ServeRequest& r = p.req;
+  Plan3Desc d;
+  const uint32_t n[3] = {r.image_depth, r.image_rows, r.image_cols};
+  for (int i = 0; i < 3; ++i) {
+    d.image[i] = n[i];
+    d.grid[i] = 2 * n[i];
+    d.origin[i] = (n[i] - 1) / 2.0;
+  }
+  d.width = r.kernel_width;
+  d.density = r.density;
+  d.samples = r.samples;
+  d.item_visits = static_cast<uint32_t>(tune_get(kTuneItemVisits));
+  Plan3Ptr plan(plan3_create(d), &plan3_destroy);
+  plan3_set_trajectory(plan.get(), r.d_traj, nullptr, stream);
+  PlanEntry e;
+  e.bytes = plan->device_bytes + plan->visit_capacity * (sizeof(uint32_t) + 2 * sizeof(float4)) +
+            static_cast<uint64_t>(plan->item_capacity) * sizeof(uint4);
+  e.plan3 = std::move(plan);
+  Plan3* raw = e.plan3.get();
+  plans_.emplace(p.plan_key, std::move(e));
+  lru_.push_front(p.plan_key);
+  ++stats_.plan_builds;
+  return raw;
+}
+
+Plan2* Server::plan_for(const Pending& p, hipStream_t stream) {
+  if (PlanEntry* e = cached_plan(p.plan_key)) return e->plan.get();
   while (plans_.size() >= cfg_.plan_cache_entries && evict_one()) {
   }
   const ServeRequest& r = p.req;
@@ -250,7 +291,7 @@ bool Server::launch_next(Lane& lane) {
   const Pending head = pending_.front();
   const ServeRequest& r = head.req;
   std::vector<Pending> batch;
-  const uint32_t limit = r.kind == kServeRadial2D ? cfg_.max_batch_scans : cfg_.max_batch_scans;
+  const uint32_t limit = r.kind == kServeRadial3D ? 1u : cfg_.max_batch_scans;
   for (auto it = pending_.begin(); it != pending_.end() && batch.size() < limit;) {
     const ServeRequest& q = it->req;
     const bool same = it->plan_key == head.plan_key && q.channels == r.channels;
@@ -264,8 +305,14 @@ bool Server::launch_next(Lane& lane) {
   KSJ_HIP(hipSetDevice(cfg_.device));
   size_t need = 0;
   Plan2* plan = nullptr;
+  Plan3* plan3 = nullptr;
   try {
-    if (r.kind == kServeRadial2D) {
+    if (r.kind == kServeRadial3D) {
+      plan3 = plan3_for(head, lane.stream);
+      plans_[head.plan_key].users++;
+      lane.plan_key = head.plan_key;
+      need = adjoint3_workspace_bytes(*plan3, r.channels, kServeCoilsPerPass3D);
+    } else if (r.kind == kServeRadial2D) {
       plan = plan_for(head, lane.stream);
       plans_[head.plan_key].users++;  // pinned against eviction from here on; released by complete()
       lane.plan_key = head.plan_key;
@@ -289,7 +336,10 @@ bool Server::launch_next(Lane& lane) {
       lane.ws_bytes = need;
       stats_.peak_workspace_bytes = std::max<uint64_t>(stats_.peak_workspace_bytes, used_bytes());
     }
-    if (r.kind == kServeRadial2D) {
+    if (r.kind == kServeRadial3D) {
+      adjoint3_execute(*plan3, r.d_kspace, r.channels, kServeCoilsPerPass3D, r.d_images, r.d_rss, lane.ws,
+                       lane.ws_bytes, lane.stream);
+    } else if (r.kind == kServeRadial2D) {
       std::vector<AdjointScan> scans(batch.size());
       for (size_t i = 0; i < batch.size(); ++i)
         scans[i] = AdjointScan{batch[i].req.d_kspace, batch[i].req.d_images, batch[i].req.d_rss};

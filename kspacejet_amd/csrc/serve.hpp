@@ -1,7 +1,8 @@
 // Continuous-batching reconstruction server on one GPU.
 //
-// Requests (2-D radial KB adjoint or 2-D Cartesian inverse FFT, inputs and outputs
-// in HBM) are queued by submit() and run by one scheduler thread:
+// Requests (2-D radial KB adjoint, 3-D radial KB adjoint of a volume, or 2-D
+// Cartesian inverse FFT, inputs and outputs in HBM) are queued by submit() and run by
+// one scheduler thread:
 //  * admission: a request's batch is launched only when a lane (stream) is free and
 //    the HBM budget covers its workspace next to the resident plans;
 //  * plans (FP64 coordinates, bin sort, visit schedule, KB and deapodization tables)
@@ -16,6 +17,7 @@
 #pragma once
 
 #include "plan.hpp"
+#include "plan3.hpp"
 
 #include <chrono>
 #include <condition_variable>
@@ -32,7 +34,8 @@
 
 namespace ksj::gpu {
 
-enum ServeKind : uint32_t { kServeRadial2D = 0, kServeCartesian2D = 1 };
+enum ServeKind : uint32_t { kServeRadial2D = 0, kServeCartesian2D = 1, kServeRadial3D = 2 };
+constexpr uint32_t kServeCoilsPerPass3D = 8;  // a 512^3 grid of 8 coils is 8.6 GB
 
 struct ServeConfig {
   int device = 0;
@@ -45,11 +48,12 @@ struct ServeConfig {
 struct ServeRequest {
   uint32_t kind = kServeRadial2D;
   uint32_t image_rows = 0, image_cols = 0, channels = 0, samples = 0;
+  uint32_t image_depth = 0;  // 3-D: N_z (image_rows = N_y, image_cols = N_x)
   uint32_t kernel_width = 4;
   uint32_t density = 1;  // KSJ_GPU_DENSITY_NONE / _RAMP
   uint32_t centred = 0;  // Cartesian only
   uint64_t trajectory_key = 0;  // radial: identifies the trajectory bytes; 0 = hash them
-  const float* d_traj = nullptr;
+  const float* d_traj = nullptr;  // radial: [S][2] (ky, kx) or 3-D [S][3] (kz, ky, kx)
   const float2* d_kspace = nullptr;
   float2* d_images = nullptr;  // coil-last, may be null
   float* d_rss = nullptr;      // may be null
@@ -95,8 +99,10 @@ class Server {
     std::string error;
   };
   using PlanPtr = std::unique_ptr<Plan2, void (*)(Plan2*)>;
+  using Plan3Ptr = std::unique_ptr<Plan3, void (*)(Plan3*)>;
   struct PlanEntry {
     PlanPtr plan{nullptr, &plan2_destroy};
+    Plan3Ptr plan3{nullptr, &plan3_destroy};
     uint64_t bytes = 0;
     uint32_t users = 0;  // batches in flight that use it
   };
@@ -115,6 +121,8 @@ class Server {
   bool launch_next(Lane& lane);  // under mu_; returns false when nothing could be launched
   void complete(Lane& lane, int status, const std::string& error);
   Plan2* plan_for(const Pending& p, hipStream_t stream);
+  Plan3* plan3_for(const Pending& p, hipStream_t stream);
+  PlanEntry* cached_plan(uint64_t key);
   bool evict_one();
   void evict_until(uint64_t need_bytes);
   uint64_t used_bytes() const;
@@ -136,8 +144,8 @@ class Server {
   std::thread thread_;
 };
 
-// 64-bit hash of a device trajectory (FNV-1a over its bytes, computed on the device
-// and read back); synchronises `stream`.
-uint64_t trajectory_hash(const float* d_traj, uint32_t samples, hipStream_t stream);
+// 64-bit hash of a device trajectory of `values` floats (FNV-1a over its bytes,
+// computed on the device and read back); synchronises `stream`.
+uint64_t trajectory_hash(const float* d_traj, uint64_t values, hipStream_t stream);
 
 }  // namespace ksj::gpu

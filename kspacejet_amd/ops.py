@@ -257,6 +257,21 @@ class Server:
                               _ptr(rss) if rss is not None else None)
         return self._submit(r, (kspace, traj, images, rss))
 
+    def submit_radial3d(self, kspace: torch.Tensor, traj: torch.Tensor, shape, images: torch.Tensor | None = None,
+                        rss: torch.Tensor | None = None, width: int = 4, density: str = "ramp_squared",
+                        trajectory_key: int = 0) -> int:
+        """A 3-D volume: kspace [C, S], traj [S, 3] (z, y, x) rad/px, shape (N_z, N_y, N_x);
+        images coil-last [N_z, N_y, N_x, C] and/or rss [N_z, N_y, N_x]."""
+        _need(kspace, torch.complex64, "kspace")
+        _need(traj, torch.float32, "traj")
+        ch, S = kspace.shape
+        nz, ny, nx = shape
+        dens = _lib.DENSITY_RAMP_SQUARED if density == "ramp_squared" else _lib.DENSITY_NONE
+        r = _lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_RADIAL3D, ny, nx, ch, S, width, dens, 0, nz,
+                              trajectory_key, _ptr(traj), _ptr(kspace), _ptr(images) if images is not None else None,
+                              _ptr(rss) if rss is not None else None)
+        return self._submit(r, (kspace, traj, images, rss))
+
     def submit_cartesian(self, kspace: torch.Tensor, images: torch.Tensor | None = None,
                          rss: torch.Tensor | None = None, centred: bool = False) -> int:
         _need(kspace, torch.complex64, "kspace")
