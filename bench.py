@@ -523,7 +523,7 @@ def run_uneven(torch, ops, lib, count, steps, warmup, tune_seconds):
 
     untuned_s = timed(multi)
     with torch.cuda.stream(stream):
-        tuned = autotune.autotune(multi, f"uneven{count}", budget_s=tune_seconds)
+        tuned = autotune.autotune(multi, f"uneven{count}", budget_s=tune_seconds, rounding=True)
     multi_s = timed(multi)
 
     # the same call captured once in a HIP graph

@@ -3,9 +3,10 @@
 It times a caller-supplied workload -- a function that enqueues GPU work on torch's
 current stream -- with HIP events, and walks the tuning keys one at a time
 (coordinate descent): each key is set to each of its candidate values with the
-others held at their best so far, and keeps the fastest.  Every key here leaves the
-results bit-identical (item_visits, which changes the summation order, is tuned only
-when asked for), so tuning never trades accuracy for speed.  Chosen settings are kept
+others held at their best so far, and keeps the fastest.  The default keys leave the
+results bit-identical; the FFT radix schedule (rounding only) and item_visits
+(summation order of new plans) are tuned only when asked for, and every value of
+them is tested against the FP64 oracle, so tuning never trades accuracy for speed.  Chosen settings are kept
 per workload signature in a JSON file when a path is given, so a later process can
 apply them without re-measuring.
 
@@ -31,7 +32,11 @@ EXACT_KEYS: Dict[str, tuple] = {
     "pass_lanes": (2, 1, 3),
     "multi_lanes": (4, 2, 6, 8),
 }
-# Changes the float summation order of the gather (results stay within tolerance).
+# Change rounding only: every FFT radix schedule is an exact factorisation of the DFT
+# (tested against the FP64 oracle at every length); tuned when `rounding=True`.
+ROUNDING_KEYS: Dict[str, tuple] = {"fft_schedule": (0, 1, 2)}
+# Changes the float summation order of the gather of new plans (results stay within
+# tolerance); needs `rebuild`.
 ORDER_KEYS: Dict[str, tuple] = {"item_visits": (64, 32, 128)}
 
 
@@ -53,8 +58,9 @@ def time_ms(run: Callable[[], None], repeats: int = 5) -> float:
 
 def autotune(run: Callable[[], None], signature: str, keys: Optional[Dict[str, Iterable[int]]] = None,
              repeats: int = 5, budget_s: float = 30.0, cache_path: Optional[str] = None,
-             rebuild: Optional[Callable[[], None]] = None) -> Dict[str, object]:
-    """Tunes `keys` (default: every bit-exact key) for `run` and leaves the best values set.
+             rebuild: Optional[Callable[[], None]] = None, rounding: bool = False) -> Dict[str, object]:
+    """Tunes `keys` (default: every bit-exact key, plus the FFT radix schedule when
+    `rounding`) for `run` and leaves the best values set.
 
     `rebuild` is called after a key that only affects new plans changes (item_visits),
     so the workload picks it up.  Returns {"settings": {...}, "ms": best time,
@@ -67,6 +73,8 @@ def autotune(run: Callable[[], None], signature: str, keys: Optional[Dict[str, I
                 ops.tune_set(k, v)
             return dict(cache[signature], cached=True)
     keys = dict(EXACT_KEYS if keys is None else keys)
+    if rounding:
+        keys.update(ROUNDING_KEYS)
     t_end = time.perf_counter() + budget_s
     best = {k: ops.tune_get(k) for k in keys}
     best_ms = baseline = time_ms(run, repeats)

@@ -58,6 +58,7 @@ enum TuneKey : int {
   kTuneTileItemSamples,   // window samples per tile work item of new plans (summation order changes)
   kTuneTileWidth,         // tile columns of new plans: 4 (16 x 4 tiles) or 8 (8 x 8) (summation order changes)
   kTuneBlockGroup,        // 2x2-block group side of new plans: 1, 2 or 4 (work-item order; results unchanged)
+  kTuneFftSchedule,       // FFT radix schedule per length: 0, 1 (half the threads) or 2 (radix-4 heavy); rounding changes
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);

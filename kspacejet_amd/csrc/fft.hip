@@ -18,25 +18,58 @@ namespace {
 
 // ----------------------------------------------------------------------------- radix schedules
 // T = threads per line.  Every pass satisfies (N / R) % T == 0, so each thread
-// owns exactly N / (R T) butterflies in every pass.
-template <int N> struct Shape;
-#define KSJ_SHAPE(NN, TT, ...)                         \
-  template <> struct Shape<NN> {                       \
-    static constexpr int T = TT;                       \
-    static constexpr int R[] = {__VA_ARGS__};          \
-    static constexpr int P = sizeof(R) / sizeof(int);  \
+// owns exactly N / (R T) butterflies in every pass.  Three schedules per length
+// (tuning key fft_schedule): 0 the default, 1 the same radices on half the threads
+// (twice the butterflies per thread), 2 a radix-4-heavy factorisation.  They differ
+// in rounding only (each is an exact factorisation of the DFT).
+template <int N, int V> struct Shape;
+#define KSJ_SHAPE(NN, VV, TT, ...)                                          \
+  template <> struct Shape<NN, VV> {                                        \
+    static constexpr int T = TT;                                            \
+    static constexpr int R[] = {__VA_ARGS__};                               \
+    static constexpr int P = sizeof(R) / sizeof(int);                       \
   };
-KSJ_SHAPE(64, 8, 8, 8)
-KSJ_SHAPE(128, 16, 8, 4, 4)
-KSJ_SHAPE(192, 16, 4, 4, 4, 3)
-KSJ_SHAPE(256, 32, 8, 8, 4)
-KSJ_SHAPE(320, 16, 4, 4, 4, 5)
-KSJ_SHAPE(384, 16, 4, 4, 8, 3)
-KSJ_SHAPE(512, 64, 8, 8, 8)
-KSJ_SHAPE(640, 32, 4, 4, 4, 2, 5)
-KSJ_SHAPE(768, 32, 8, 8, 4, 3)
-KSJ_SHAPE(1024, 64, 8, 8, 4, 4)
+KSJ_SHAPE(64, 0, 8, 8, 8)
+KSJ_SHAPE(64, 1, 4, 8, 8)
+KSJ_SHAPE(64, 2, 16, 4, 4, 4)
+KSJ_SHAPE(128, 0, 16, 8, 4, 4)
+KSJ_SHAPE(128, 1, 8, 8, 4, 4)
+KSJ_SHAPE(128, 2, 32, 4, 4, 4, 2)
+KSJ_SHAPE(192, 0, 16, 4, 4, 4, 3)
+KSJ_SHAPE(192, 1, 8, 4, 4, 4, 3)
+KSJ_SHAPE(192, 2, 16, 4, 4, 3, 4)
+KSJ_SHAPE(256, 0, 32, 8, 8, 4)
+KSJ_SHAPE(256, 1, 16, 8, 8, 4)
+KSJ_SHAPE(256, 2, 64, 4, 4, 4, 4)
+KSJ_SHAPE(320, 0, 16, 4, 4, 4, 5)
+KSJ_SHAPE(320, 1, 8, 4, 4, 4, 5)
+KSJ_SHAPE(320, 2, 16, 5, 4, 4, 4)
+KSJ_SHAPE(384, 0, 16, 4, 4, 8, 3)
+KSJ_SHAPE(384, 1, 8, 4, 4, 8, 3)
+KSJ_SHAPE(384, 2, 16, 8, 4, 4, 3)
+KSJ_SHAPE(512, 0, 64, 8, 8, 8)
+KSJ_SHAPE(512, 1, 32, 8, 8, 8)
+KSJ_SHAPE(512, 2, 64, 8, 4, 4, 4)
+KSJ_SHAPE(640, 0, 32, 4, 4, 4, 2, 5)
+KSJ_SHAPE(640, 1, 16, 4, 4, 4, 2, 5)
+KSJ_SHAPE(640, 2, 16, 8, 4, 4, 5)
+KSJ_SHAPE(768, 0, 32, 8, 8, 4, 3)
+KSJ_SHAPE(768, 1, 16, 8, 8, 4, 3)
+KSJ_SHAPE(768, 2, 64, 4, 4, 4, 4, 3)
+KSJ_SHAPE(1024, 0, 64, 8, 8, 4, 4)
+KSJ_SHAPE(1024, 1, 32, 8, 8, 4, 4)
+KSJ_SHAPE(1024, 2, 64, 4, 4, 4, 4, 4)
 #undef KSJ_SHAPE
+
+template <int N, int V>
+constexpr bool shape_ok() {
+  int prod = 1;
+  for (int i = 0; i < Shape<N, V>::P; ++i) {
+    prod *= Shape<N, V>::R[i];
+    if ((N / Shape<N, V>::R[i]) % Shape<N, V>::T != 0) return false;
+  }
+  return prod == N;
+}
 
 // LDS element index of line element i: one pad element per 16 spreads the strided
 // Stockham writes (stride R in the first pass, R * Ns blocks in the next) over the
@@ -124,11 +157,11 @@ template <bool INV> struct Dft<5, INV> {
   }
 };
 
-template <int N, bool INV, int P, int Ns>
+template <int N, int V, bool INV, int P, int Ns>
 __device__ inline void passes(float2* line, const float2* tw, int t) {
-  if constexpr (P < Shape<N>::P) {
-    constexpr int T = Shape<N>::T;
-    constexpr int R = Shape<N>::R[P];
+  if constexpr (P < Shape<N, V>::P) {
+    constexpr int T = Shape<N, V>::T;
+    constexpr int R = Shape<N, V>::R[P];
     constexpr int B = N / R / T;
     static_assert((N / R) % T == 0, "radix schedule must split evenly over the line's threads");
     float2 v[B][R];
@@ -158,7 +191,7 @@ __device__ inline void passes(float2* line, const float2* tw, int t) {
       for (int r = 0; r < R; ++r) line[pad(base + r * Ns)] = v[b][r];
     }
     __syncthreads();
-    passes<N, INV, P + 1, Ns * R>(line, tw, t);
+    passes<N, V, INV, P + 1, Ns * R>(line, tw, t);
   }
 }
 
@@ -171,9 +204,9 @@ __device__ inline uint64_t line_base(const FftLines& L, uint64_t line) {
 // of two and the block has lpb * T threads, so every thread moves exactly N / T
 // elements: all of its global loads are issued before the first LDS write, so the
 // loads of one thread overlap instead of paying one memory latency each.
-template <int N>
+template <int N, int V>
 __device__ inline void load_lines(const FftArgs& a, float2* buf, int ls, int lpb_log2, const uint64_t* src_base) {
-  constexpr int E = N / Shape<N>::T;
+  constexpr int E = N / Shape<N, V>::T;
   const int lpb = 1 << lpb_log2;
   const uint32_t es = static_cast<uint32_t>(a.src.elem_stride);
   float2 v[E];
@@ -206,10 +239,10 @@ __device__ inline void load_lines(const FftArgs& a, float2* buf, int ls, int lpb
 // Stores the n_out kept outputs of every line, scaled by a.scale x out_weight[e] x
 // line_weight[line]; the weights were staged in LDS (ow, lw), so the stores issue
 // back to back.
-template <int N>
+template <int N, int V>
 __device__ inline void store_lines(const FftArgs& a, const float2* buf, int ls, int lpb_log2, const float* ow,
                                    const float* lw, const uint64_t* dst_base) {
-  constexpr int E = N / Shape<N>::T;
+  constexpr int E = N / Shape<N, V>::T;
   const int lpb = 1 << lpb_log2;
   const int nout = static_cast<int>(a.n_out);
   const int count = lpb * nout;
@@ -235,9 +268,10 @@ __device__ inline void store_lines(const FftArgs& a, const float2* buf, int ls, 
   }
 }
 
-template <int N, bool INV>
+template <int N, int V, bool INV>
 __global__ __launch_bounds__(1024) void fft_lds_kernel(FftArgs a, const float2* __restrict__ tw_global, int lpb_log2) {
-  constexpr int T = Shape<N>::T;
+  static_assert(shape_ok<N, V>(), "radix schedule must multiply to N and split evenly over the line's threads");
+  constexpr int T = Shape<N, V>::T;
   constexpr int LS = line_stride<N>();
   extern __shared__ float2 smem[];
   __shared__ uint64_t src_base[64], dst_base[64];
@@ -258,11 +292,11 @@ __global__ __launch_bounds__(1024) void fft_lds_kernel(FftArgs a, const float2* 
     lw[ly] = a.line_weight ? a.line_weight[line % a.src.lines] : 1.0f;
   }
   __syncthreads();
-  load_lines<N>(a, buf, LS, lpb_log2, src_base);
+  load_lines<N, V>(a, buf, LS, lpb_log2, src_base);
   __syncthreads();
   const int ly = threadIdx.x / T, t = threadIdx.x % T;
-  passes<N, INV, 0, 1>(buf + ly * LS, tw, t);
-  store_lines<N>(a, buf, LS, lpb_log2, ow, lw, dst_base);
+  passes<N, V, INV, 0, 1>(buf + ly * LS, tw, t);
+  store_lines<N, V>(a, buf, LS, lpb_log2, ow, lw, dst_base);
 }
 
 // Any length: direct DFT per output bin against the twiddle table (used for
@@ -335,9 +369,9 @@ int max_dynamic_lds() {
   return cached;
 }
 
-template <int N>
+template <int N, int V>
 void launch_fast(const FftArgs& a, const float2* tw, hipStream_t stream, int lpb) {
-  constexpr int T = Shape<N>::T;
+  constexpr int T = Shape<N, V>::T;
   if (lpb <= 0) lpb = static_cast<int>(tune_get(kTuneFftLinesPerBlock));
   if (lpb <= 0) lpb = (a.load_lines_fast || a.store_lines_fast) ? 16 : (T >= 64 ? 4 : 256 / T);
   while (lpb * T > 1024) lpb /= 2;
@@ -354,16 +388,25 @@ void launch_fast(const FftArgs& a, const float2* tw, hipStream_t stream, int lpb
   const unsigned grid = ceil_div(total, static_cast<uint64_t>(lpb));
   if (a.inverse) {
     if (shmem > 65536)
-      KSJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_lds_kernel<N, true>),
+      KSJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_lds_kernel<N, V, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
-    hipLaunchKernelGGL((fft_lds_kernel<N, true>), dim3(grid), dim3(lpb * T), shmem, stream, a, tw, lpb_log2);
+    hipLaunchKernelGGL((fft_lds_kernel<N, V, true>), dim3(grid), dim3(lpb * T), shmem, stream, a, tw, lpb_log2);
   } else {
     if (shmem > 65536)
-      KSJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_lds_kernel<N, false>),
+      KSJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_lds_kernel<N, V, false>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
-    hipLaunchKernelGGL((fft_lds_kernel<N, false>), dim3(grid), dim3(lpb * T), shmem, stream, a, tw, lpb_log2);
+    hipLaunchKernelGGL((fft_lds_kernel<N, V, false>), dim3(grid), dim3(lpb * T), shmem, stream, a, tw, lpb_log2);
   }
   KSJ_LAUNCH_CHECK();
+}
+
+template <int N>
+void launch_fast(const FftArgs& a, const float2* tw, hipStream_t stream, int lpb) {
+  switch (tune_get(kTuneFftSchedule)) {
+    case 1: return launch_fast<N, 1>(a, tw, stream, lpb);
+    case 2: return launch_fast<N, 2>(a, tw, stream, lpb);
+    default: return launch_fast<N, 0>(a, tw, stream, lpb);
+  }
 }
 
 struct TwKey {
