@@ -178,10 +178,16 @@ int ksj_gpu_adjoint2_execute_batch(const ksj_gpu_plan2* plan, uint32_t count, co
                                    void* d_workspace, size_t workspace_bytes, void* stream);
 
 /* Uneven batch: `count` scans, each with its own plan (geometry, kernel, trajectory)
- * and channel count.  Every item's pipeline runs on one of "multi_lanes" streams
- * (largest items first, round-robin), joined back to `stream`.  The call only
- * enqueues work (no host synchronisation, no allocation once every grid length has
- * been run once), so it can be captured with ksj_gpu_graph_begin/end and replayed.
+ * and channel count.  Two forms with bit-identical results (tuning key "multi_fused"):
+ *   0: every item's pipeline runs on one of "multi_lanes" streams (largest items first,
+ *      round-robin), joined back to `stream`;
+ *   1: fused, for up to 64 items of the 2x2-block schedule with <= 64 channels: each
+ *      stage is one launch per shape class (padded coil count for gridding, length for
+ *      each FFT pass) over all items, with per-item argument tables written into the
+ *      workspace by small argument-carrying kernels.
+ * The call only enqueues work (no host synchronisation, no allocation once every grid
+ * length has been run once), so it can be captured with ksj_gpu_graph_begin/end and
+ * replayed.  The workspace size covers both forms.
  * Item i writes d_images[i] (layout) and/or d_rss[i]; either array may be NULL or
  * hold NULL entries.  No reference counterpart: the reference runs one frame per firing. */
 size_t ksj_gpu_adjoint2_multi_workspace_bytes(uint32_t count, const ksj_gpu_plan2* const* plans,
@@ -354,8 +360,16 @@ void ksj_gpu_serve_destroy(ksj_gpu_server* server);
  *   "multi_lanes"          1..8 streams over which the items of ksj_gpu_adjoint2_execute_multi spread
  *   "gather_coils_per_lane" 1 or 2 coils per gather lane (one 8- or 16-byte load per visit) from 32
  *                          stacked coils up
- * Every setting gives bit-identical results, except item_visits, which sets the
- * float summation order of the gather.  Not thread-safe against concurrent launches. */
+ *   "block_group"          1, 2 or 4: side of the groups of 2x2 blocks numbered together (new plans)
+ *   "multi_fused"          0 or 1: form of ksj_gpu_adjoint2_execute_multi (see there)
+ *   "fft_schedule"         0, 1 or 2: radix schedule of every LDS FFT length (rounding only)
+ *   "gather_scheme"        0 = 2x2 blocks, 1 = 64-cell LDS tiles (new plans; summation order)
+ *   "tile_width"           4 or 8 columns per tile (new plans; summation order)
+ *   "tile_item_samples"    window samples per tile work item (new plans; summation order)
+ * Every setting gives bit-identical results, except item_visits, the tile keys and
+ * gather_scheme (float summation order of the gather) and fft_schedule (FFT
+ * rounding); each of those is tested against the FP64 oracle.  Not thread-safe
+ * against concurrent launches. */
 int ksj_gpu_tune_set(const char* key, int64_t value);
 int ksj_gpu_tune_get(const char* key, int64_t* value);
 

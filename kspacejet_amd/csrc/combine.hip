@@ -1,6 +1,10 @@
 // Coil combination kernels.  All HBM-bound: one read of the coil data, one write.
 #include "combine.hpp"
 
+#include "fused.hpp"
+
+#include <algorithm>
+
 namespace ksj::gpu {
 namespace {
 
@@ -90,6 +94,47 @@ __global__ __launch_bounds__(256) void major_split_kernel(const float2* __restri
   }
 }
 
+// Fused items: one coil group each (coil-major [coils][pixels] -> coil-last and/or
+// RSS), the arithmetic of major_split_kernel with one group.
+struct SplitItemArgs {
+  const float2* in;
+  float2* coil_last;
+  float* rss;
+  uint64_t pixels;
+  uint32_t coils, pad;
+};
+
+__global__ __launch_bounds__(256) void split_fused_kernel(const FusedTable<SplitItemArgs>* t) {
+  extern __shared__ float2 tile[];  // [kTilePixels][coils + 1]
+  const uint32_t item = fused_item(t, blockIdx.x);
+  const SplitItemArgs a = fused_args(t, item);
+  const uint64_t p0 = (blockIdx.x - fused_first(t, item)) * kTilePixels;
+  const uint32_t stride = a.coils + 1;
+  const uint32_t np = static_cast<uint32_t>(p0 + kTilePixels <= a.pixels ? kTilePixels : a.pixels - p0);
+  for (uint32_t i = threadIdx.x; i < np * a.coils; i += blockDim.x) {
+    const uint32_t c = i / np, pl = i % np;
+    tile[pl * stride + c] = a.in[static_cast<uint64_t>(c) * a.pixels + p0 + pl];
+  }
+  __syncthreads();
+  if (a.coil_last) {
+    float2* dst = a.coil_last + p0 * a.coils;
+    for (uint32_t i = threadIdx.x; i < np * a.coils; i += blockDim.x) {
+      const uint32_t pl = i / a.coils, c = i % a.coils;
+      dst[i] = tile[pl * stride + c];
+    }
+  }
+  if (a.rss) {
+    for (uint32_t pl = threadIdx.x; pl < np; pl += blockDim.x) {
+      float acc = 0.f;
+      for (uint32_t c = 0; c < a.coils; ++c) {
+        const float2 v = tile[pl * stride + c];
+        acc += v.x * v.x + v.y * v.y;
+      }
+      a.rss[p0 + pl] = sqrtf(acc);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void sense_kernel(const float2* __restrict__ img, const float2* __restrict__ sens,
                                                     float2* __restrict__ out, uint64_t pixels, uint32_t coils) {
   const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -143,6 +188,33 @@ void sense_combine(const float2* d_images, const float2* d_sens, float2* d_out, 
   require(coils >= 1, "sensitivity combine needs at least one coil");
   hipLaunchKernelGGL(sense_kernel, dim3(ceil_div(pixels, 256)), dim3(256), 0, stream, d_images, d_sens, d_out, pixels,
                      coils);
+  KSJ_LAUNCH_CHECK();
+}
+
+size_t coil_split_fused_table_bytes() { return fused_table_bytes<SplitItemArgs>(); }
+
+void coil_split_fused(const float2* const* d_in, float2* const* d_coil_last, float* const* d_rss,
+                      const uint64_t* pixels, const uint32_t* coils, uint32_t count, void* d_table,
+                      hipStream_t stream) {
+  FusedTableBuilder<SplitItemArgs> b;
+  uint32_t max_coils = 1;
+  for (uint32_t i = 0; i < count; ++i) {
+    if (!d_coil_last[i] && !d_rss[i]) continue;
+    require(coils[i] >= 1 && coils[i] <= 64, "coil split supports 1..64 coils");
+    SplitItemArgs a{};
+    a.in = d_in[i];
+    a.coil_last = d_coil_last[i];
+    a.rss = d_rss[i];
+    a.pixels = pixels[i];
+    a.coils = coils[i];
+    b.add(a, ceil_div(pixels[i], static_cast<uint64_t>(kTilePixels)));
+    max_coils = std::max(max_coils, coils[i]);
+  }
+  if (b.t.count == 0) return;
+  b.upload(d_table, stream);
+  const size_t shmem = static_cast<size_t>(kTilePixels) * (max_coils + 1) * sizeof(float2);
+  hipLaunchKernelGGL(split_fused_kernel, dim3(static_cast<uint32_t>(b.blocks())), dim3(256), shmem, stream,
+                     static_cast<const FusedTable<SplitItemArgs>*>(d_table));
   KSJ_LAUNCH_CHECK();
 }
 

@@ -29,6 +29,14 @@ struct CoilGroupOutputs {
 // Coil-major planes [groups * coils][P] -> per group coil-last [P][coils] and/or RSS [P].
 void coil_major_split(const float2* d_in, const CoilGroupOutputs& out, uint64_t pixels, hipStream_t stream);
 
+// Items of an uneven batch, one launch: item i's coil-major [coils[i]][pixels[i]]
+// planes -> coil-last and/or RSS (either may be null), as coil_major_split with one
+// group.  The argument table goes to d_table (coil_split_fused_table_bytes() long).
+size_t coil_split_fused_table_bytes();
+void coil_split_fused(const float2* const* d_in, float2* const* d_coil_last, float* const* d_rss,
+                      const uint64_t* pixels, const uint32_t* coils, uint32_t count, void* d_table,
+                      hipStream_t stream);
+
 // x[p] = sum_c conj(S[p][c]) I[p][c] / sum_c |S[p][c]|^2, 0 where the denominator is 0.
 void sense_combine(const float2* d_images, const float2* d_sens, float2* d_out, uint64_t pixels, uint32_t coils,
                    hipStream_t stream);

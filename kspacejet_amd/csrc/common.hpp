@@ -59,6 +59,7 @@ enum TuneKey : int {
   kTuneTileWidth,         // tile columns of new plans: 4 (16 x 4 tiles) or 8 (8 x 8) (summation order changes)
   kTuneBlockGroup,        // 2x2-block group side of new plans: 1, 2 or 4 (work-item order; results unchanged)
   kTuneFftSchedule,       // FFT radix schedule per length: 0, 1 (half the threads) or 2 (radix-4 heavy); rounding changes
+  kTuneMultiFused,        // uneven batches: 1 = one launch per stage and shape class, 0 = items over lanes
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);

@@ -9,7 +9,10 @@
 // addresses in global memory for both contiguous rows and strided columns.
 #include "fft.hpp"
 
+#include "fused.hpp"
+
 #include <map>
+#include <vector>
 #include <mutex>
 #include <utility>
 
@@ -269,7 +272,8 @@ __device__ inline void store_lines(const FftArgs& a, const float2* buf, int ls, 
 }
 
 template <int N, int V, bool INV>
-__global__ __launch_bounds__(1024) void fft_lds_kernel(FftArgs a, const float2* __restrict__ tw_global, int lpb_log2) {
+__device__ inline void fft_lds_block(const FftArgs& a, const float2* __restrict__ tw_global, int lpb_log2,
+                                     uint64_t block) {
   static_assert(shape_ok<N, V>(), "radix schedule must multiply to N and split evenly over the line's threads");
   constexpr int T = Shape<N, V>::T;
   constexpr int LS = line_stride<N>();
@@ -284,7 +288,7 @@ __global__ __launch_bounds__(1024) void fft_lds_kernel(FftArgs a, const float2* 
     ow[i] = (i < static_cast<int>(a.n_out) && a.out_weight) ? a.scale * a.out_weight[i] : a.scale;
   }
   const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
-  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * lpb;
+  const uint64_t first = block * lpb;
   for (int ly = threadIdx.x; ly < lpb; ly += blockDim.x) {
     const uint64_t line = first + ly;
     src_base[ly] = line < total ? line_base(a.src, line) : ~0ull;
@@ -297,6 +301,20 @@ __global__ __launch_bounds__(1024) void fft_lds_kernel(FftArgs a, const float2* 
   const int ly = threadIdx.x / T, t = threadIdx.x % T;
   passes<N, V, INV, 0, 1>(buf + ly * LS, tw, t);
   store_lines<N, V>(a, buf, LS, lpb_log2, ow, lw, dst_base);
+}
+
+template <int N, int V, bool INV>
+__global__ __launch_bounds__(1024) void fft_lds_kernel(FftArgs a, const float2* __restrict__ tw_global, int lpb_log2) {
+  fft_lds_block<N, V, INV>(a, tw_global, lpb_log2, blockIdx.x);
+}
+
+// Fused items of one length: the table's blocks run fft_lds_block of their item.
+template <int N, int V, bool INV>
+__global__ __launch_bounds__(1024) void fft_fused_kernel(const FusedTable<FftArgs>* t,
+                                                         const float2* __restrict__ tw_global, int lpb_log2) {
+  const uint32_t item = fused_item(t, blockIdx.x);
+  const FftArgs a = fused_args(t, item);
+  fft_lds_block<N, V, INV>(a, tw_global, lpb_log2, blockIdx.x - fused_first(t, item));
 }
 
 // Any length: direct DFT per output bin against the twiddle table (used for
@@ -369,8 +387,10 @@ int max_dynamic_lds() {
   return cached;
 }
 
+// Lines per block (log2) of a launch: the tuning key or a per-mapping default, capped
+// by 1024 threads and by the LDS.
 template <int N, int V>
-void launch_fast(const FftArgs& a, const float2* tw, hipStream_t stream, int lpb) {
+int lines_per_block_log2(const FftArgs& a, int lpb) {
   constexpr int T = Shape<N, V>::T;
   if (lpb <= 0) lpb = static_cast<int>(tune_get(kTuneFftLinesPerBlock));
   if (lpb <= 0) lpb = (a.load_lines_fast || a.store_lines_fast) ? 16 : (T >= 64 ? 4 : 256 / T);
@@ -382,9 +402,21 @@ void launch_fast(const FftArgs& a, const float2* tw, hipStream_t stream, int lpb
   while (lpb > 1 && lpb > lds_cap) lpb /= 2;
   int lpb_log2 = 0;
   while ((1 << (lpb_log2 + 1)) <= lpb) ++lpb_log2;
+  return lpb_log2;
+}
+
+template <int N, int V>
+size_t fft_shmem(int lpb_log2) {
+  return (static_cast<size_t>(N) + (static_cast<size_t>(1) << lpb_log2) * line_stride<N>()) * sizeof(float2);
+}
+
+template <int N, int V>
+void launch_fast(const FftArgs& a, const float2* tw, hipStream_t stream, int lpb) {
+  constexpr int T = Shape<N, V>::T;
+  const int lpb_log2 = lines_per_block_log2<N, V>(a, lpb);
   lpb = 1 << lpb_log2;
   const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
-  const size_t shmem = (static_cast<size_t>(N) + static_cast<size_t>(lpb) * line_stride<N>()) * sizeof(float2);
+  const size_t shmem = fft_shmem<N, V>(lpb_log2);
   const unsigned grid = ceil_div(total, static_cast<uint64_t>(lpb));
   if (a.inverse) {
     if (shmem > 65536)
@@ -398,6 +430,45 @@ void launch_fast(const FftArgs& a, const float2* tw, hipStream_t stream, int lpb
     hipLaunchKernelGGL((fft_lds_kernel<N, V, false>), dim3(grid), dim3(lpb * T), shmem, stream, a, tw, lpb_log2);
   }
   KSJ_LAUNCH_CHECK();
+}
+
+template <int N, int V>
+void launch_fused(const FftArgs* const* items, uint32_t count, const float2* tw, void* d_table, hipStream_t stream) {
+  constexpr int T = Shape<N, V>::T;
+  const int lpb_log2 = lines_per_block_log2<N, V>(*items[0], 0);
+  const uint64_t lpb = 1ull << lpb_log2;
+  FusedTableBuilder<FftArgs> b;
+  for (uint32_t i = 0; i < count; ++i) {
+    require(items[i]->inverse == items[0]->inverse && items[i]->load_lines_fast == items[0]->load_lines_fast &&
+              items[i]->store_lines_fast == items[0]->store_lines_fast,
+            "fused FFT items must share direction and thread mapping");
+    b.add(*items[i], ceil_div(static_cast<uint64_t>(items[i]->src.planes) * items[i]->src.lines, lpb));
+  }
+  b.upload(d_table, stream);
+  const auto* t = static_cast<const FusedTable<FftArgs>*>(d_table);
+  const size_t shmem = fft_shmem<N, V>(lpb_log2);
+  const dim3 grid(static_cast<uint32_t>(b.blocks())), block(static_cast<uint32_t>(lpb * T));
+  if (items[0]->inverse) {
+    if (shmem > 65536)
+      KSJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_fused_kernel<N, V, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+    hipLaunchKernelGGL((fft_fused_kernel<N, V, true>), grid, block, shmem, stream, t, tw, lpb_log2);
+  } else {
+    if (shmem > 65536)
+      KSJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_fused_kernel<N, V, false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+    hipLaunchKernelGGL((fft_fused_kernel<N, V, false>), grid, block, shmem, stream, t, tw, lpb_log2);
+  }
+  KSJ_LAUNCH_CHECK();
+}
+
+template <int N>
+void launch_fused(const FftArgs* const* items, uint32_t count, const float2* tw, void* d_table, hipStream_t stream) {
+  switch (tune_get(kTuneFftSchedule)) {
+    case 1: return launch_fused<N, 1>(items, count, tw, d_table, stream);
+    case 2: return launch_fused<N, 2>(items, count, tw, d_table, stream);
+    default: return launch_fused<N, 0>(items, count, tw, d_table, stream);
+  }
 }
 
 template <int N>
@@ -470,6 +541,47 @@ void fft_launch(const FftArgs& a, hipStream_t stream, int lpb) {
   require(shmem <= 160 * 1024, "direct DFT line does not fit in LDS");
   hipLaunchKernelGGL(dft_direct_kernel<0>, dim3(ceil_div(total, l)), dim3(256), shmem, stream, a, tw, l);
   KSJ_LAUNCH_CHECK();
+}
+
+size_t fft_fused_table_bytes(uint32_t count) { return static_cast<size_t>(count) * fused_table_bytes<FftArgs>(); }
+
+void fft_launch_fused(const FftArgs* args, uint32_t count, void* d_tables, hipStream_t stream) {
+  require(count <= kMaxFusedItems, "fused FFT takes at most 64 items");
+  std::vector<uint32_t> done(count, 0);
+  char* tab = static_cast<char*>(d_tables);
+  for (uint32_t i = 0; i < count; ++i) {
+    if (done[i]) continue;
+    const FftArgs& a = args[i];
+    require(a.n >= 1 && a.n <= 4096 && a.n_out >= 1 && a.n_out <= a.n && a.in_offset < a.n && a.out_offset < a.n &&
+              a.src.planes == a.dst.planes && a.src.lines == a.dst.lines,
+            "invalid fused FFT item");
+    std::vector<const FftArgs*> cls;
+    for (uint32_t j = i; j < count; ++j)
+      if (!done[j] && args[j].n == a.n) {
+        done[j] = 1;
+        if (static_cast<uint64_t>(args[j].src.planes) * args[j].src.lines > 0) cls.push_back(&args[j]);
+      }
+    if (cls.empty()) continue;
+    if (!fft_fast_length(a.n)) {  // direct-DFT lengths: one launch per item
+      for (const FftArgs* p : cls) fft_launch(*p, stream);
+      continue;
+    }
+    const float2* tw = fft_twiddles(a.n, stream);
+    const uint32_t nc = static_cast<uint32_t>(cls.size());
+    switch (a.n) {
+      case 64: launch_fused<64>(cls.data(), nc, tw, tab, stream); break;
+      case 128: launch_fused<128>(cls.data(), nc, tw, tab, stream); break;
+      case 192: launch_fused<192>(cls.data(), nc, tw, tab, stream); break;
+      case 256: launch_fused<256>(cls.data(), nc, tw, tab, stream); break;
+      case 320: launch_fused<320>(cls.data(), nc, tw, tab, stream); break;
+      case 384: launch_fused<384>(cls.data(), nc, tw, tab, stream); break;
+      case 512: launch_fused<512>(cls.data(), nc, tw, tab, stream); break;
+      case 640: launch_fused<640>(cls.data(), nc, tw, tab, stream); break;
+      case 768: launch_fused<768>(cls.data(), nc, tw, tab, stream); break;
+      default: launch_fused<1024>(cls.data(), nc, tw, tab, stream); break;
+    }
+    tab += fused_table_bytes<FftArgs>();
+  }
 }
 
 }  // namespace ksj::gpu

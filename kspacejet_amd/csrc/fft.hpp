@@ -48,4 +48,11 @@ const float2* fft_twiddles(uint32_t n, hipStream_t stream);
 
 void fft_launch(const FftArgs& args, hipStream_t stream, int lines_per_block = 0);
 
+// Transforms of up to 64 items of an uneven batch (same direction and thread
+// mapping): one launch per LDS length, each item's blocks running that length's
+// kernel on its own arguments (direct-DFT lengths are launched per item).  Tables
+// go to d_tables, fft_fused_table_bytes(count) long.
+size_t fft_fused_table_bytes(uint32_t count);
+void fft_launch_fused(const FftArgs* args, uint32_t count, void* d_tables, hipStream_t stream);
+
 }  // namespace ksj::gpu

@@ -14,6 +14,8 @@
 //         from there (below).
 #include "grid.hpp"
 
+#include "fused.hpp"
+
 #include <utility>
 
 namespace ksj::gpu {
@@ -35,30 +37,39 @@ struct PrepArgs {
 // 512-byte read per coil plane, all issued before the first wait), weights them
 // and transposes through LDS; then every sorted row of CP complex values is
 // written as CP/2 16-byte pieces at the sample's rank.
-template <int CP>
-__global__ __launch_bounds__(kPrepSamples * kPrepWaves) void prep_kernel(PrepArgs a) {
+// Coil planes of one fused item: plane c at base + c * stride (a [C][S] k-space).
+struct StridedPlanes {
+  const float2* base;
+  uint64_t stride;
+};
+__device__ inline const float2* plane_of(const CoilPlanes& p, uint32_t c) { return p.plane[c]; }
+__device__ inline const float2* plane_of(const StridedPlanes& p, uint32_t c) { return p.base + c * p.stride; }
+
+template <int CP, typename Planes>
+__device__ inline void prep_block(const Planes& planes, const float2* weight, const uint32_t* rank, float2* sorted,
+                                  uint32_t samples, uint32_t coils, uint32_t block) {
   constexpr int kTs = CP + 1;               // padded row: a coil column spreads over the banks
   constexpr int kPerWave = CP / kPrepWaves;  // coils read by one wave
   __shared__ float2 tile[kPrepSamples * kTs];
   __shared__ uint32_t rank_s[kPrepSamples];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint32_t s0 = blockIdx.x * kPrepSamples;
+  const uint32_t s0 = block * kPrepSamples;
   const uint32_t s = s0 + lane;
-  const bool live = s < a.samples;
-  const float2 w = live ? a.weight[s] : make_float2(0.f, 0.f);
-  const uint32_t r = live ? a.rank[s] : 0u;
+  const bool live = s < samples;
+  const float2 w = live ? weight[s] : make_float2(0.f, 0.f);
+  const uint32_t r = live ? rank[s] : 0u;
   float2 v[kPerWave];
 #pragma unroll
   for (int j = 0; j < kPerWave; ++j) {
     const uint32_t c = wave + kPrepWaves * j;  // wave-uniform: the plane pointer is a scalar load
-    v[j] = (c < a.coils && live) ? a.planes.plane[c][s] : make_float2(0.f, 0.f);
+    v[j] = (c < coils && live) ? plane_of(planes, c)[s] : make_float2(0.f, 0.f);
   }
   if (wave == 0) rank_s[lane] = r;  // after the sample loads are issued
 #pragma unroll
   for (int j = 0; j < kPerWave; ++j) tile[lane * kTs + wave + kPrepWaves * j] = cmul(v[j], w);
   __syncthreads();
-  const uint32_t ns = a.samples - s0 < static_cast<uint32_t>(kPrepSamples) ? a.samples - s0 : kPrepSamples;
+  const uint32_t ns = samples - s0 < static_cast<uint32_t>(kPrepSamples) ? samples - s0 : kPrepSamples;
   constexpr int kPieces = CP / 2;  // float4 pieces per sorted row
   constexpr int kIters = kPrepSamples * kPieces / (kPrepSamples * kPrepWaves);
 #pragma unroll
@@ -67,10 +78,15 @@ __global__ __launch_bounds__(kPrepSamples * kPrepWaves) void prep_kernel(PrepArg
     const uint32_t sl = i / kPieces, pc = i % kPieces;
     if (sl < ns) {
       const float2 lo = tile[sl * kTs + 2 * pc], hi = tile[sl * kTs + 2 * pc + 1];
-      float4* dst = reinterpret_cast<float4*>(a.sorted + static_cast<uint64_t>(rank_s[sl]) * CP);
+      float4* dst = reinterpret_cast<float4*>(sorted + static_cast<uint64_t>(rank_s[sl]) * CP);
       dst[pc] = make_float4(lo.x, lo.y, hi.x, hi.y);
     }
   }
+}
+
+template <int CP>
+__global__ __launch_bounds__(kPrepSamples * kPrepWaves) void prep_kernel(PrepArgs a) {
+  prep_block<CP>(a.planes, a.weight, a.rank, a.sorted, a.samples, a.coils, blockIdx.x);
 }
 
 // ----------------------------------------------------------------------------- gather
@@ -292,8 +308,7 @@ __global__ __launch_bounds__(256) void gather_item_kernel(GatherArgs a) {
 }
 
 // Blocks with several items: thread per (block, cell, coil) sums the item partials in item order.
-__global__ __launch_bounds__(256) void multi_reduce_kernel(GatherArgs a) {
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ inline void multi_reduce_one(const GatherArgs& a, uint64_t i) {
   if (i >= static_cast<uint64_t>(a.multi_count) * 4 * a.cp) return;
   const uint32_t c = static_cast<uint32_t>(i % a.cp);
   const uint32_t q = static_cast<uint32_t>((i / a.cp) % 4);
@@ -316,6 +331,10 @@ __global__ __launch_bounds__(256) void multi_reduce_kernel(GatherArgs a) {
   block_coords(b, a.block_cols, a.block_group, &by, &bx);
   const uint32_t y = 2 * by + q / 2, x = 2 * bx + q % 2;
   if (y < a.grid_rows && x < a.grid_cols) a.grid[(static_cast<uint64_t>(y) * a.grid_cols + x) * a.cp + c] = s;
+}
+
+__global__ __launch_bounds__(256) void multi_reduce_kernel(GatherArgs a) {
+  multi_reduce_one(a, static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
 }
 
 // ----------------------------------------------------------------------------- tile gather
@@ -595,6 +614,54 @@ void launch_tile_gather(const TileArgs& a, uint32_t width, uint32_t items, hipSt
   return launch_tile_gather<CP, 8>(a, width, items, stream);
 }
 
+// ----------------------------------------------------------------------------- fused items
+// One launch per stage and padded coil count over the items of an uneven batch.
+struct PrepItemArgs {
+  StridedPlanes planes;
+  const float2* weight;
+  const uint32_t* rank;
+  float2* sorted;
+  uint32_t samples, coils;
+};
+
+template <int CP>
+__global__ __launch_bounds__(kPrepSamples * kPrepWaves) void prep_fused_kernel(const FusedTable<PrepItemArgs>* t) {
+  const uint32_t item = fused_item(t, blockIdx.x);
+  const PrepItemArgs a = fused_args(t, item);
+  prep_block<CP>(a.planes, a.weight, a.rank, a.sorted, a.samples, a.coils,
+                 static_cast<uint32_t>(blockIdx.x - fused_first(t, item)));
+}
+
+template <int SW, int B, int CPL>
+__global__ __launch_bounds__(256) void gather_fused_kernel(const FusedTable<GatherArgs>* t) {
+  constexpr int NSW = 64 / SW;
+  const int lane = threadIdx.x & 63, sw = lane / SW, c = lane % SW;
+  const uint32_t it = fused_item(t, blockIdx.x);
+  const GatherArgs a = fused_args(t, it);
+  const uint32_t wg = static_cast<uint32_t>(blockIdx.x - fused_first(t, it));
+  const uint32_t item = (wg * 4 + (threadIdx.x >> 6)) * NSW + sw;
+  gather_one_item<SW, B, CPL>(a, item, item < a.item_count, sw, c);
+}
+
+__global__ __launch_bounds__(256) void reduce_fused_kernel(const FusedTable<GatherArgs>* t) {
+  const uint32_t item = fused_item(t, blockIdx.x);
+  const GatherArgs a = fused_args(t, item);
+  multi_reduce_one(a, (blockIdx.x - fused_first(t, item)) * 256ull + threadIdx.x);
+}
+
+template <int SW, int CPL>
+void launch_gather_fused(const FusedTableBuilder<GatherArgs>& b, void* d_table, hipStream_t stream) {
+  b.upload(d_table, stream);
+  const auto* t = static_cast<const FusedTable<GatherArgs>*>(d_table);
+  const dim3 grid(static_cast<uint32_t>(b.blocks())), block(256);
+  switch (tune_get(kTuneGatherBatch)) {
+    case 4: hipLaunchKernelGGL((gather_fused_kernel<SW, 4, CPL>), grid, block, 0, stream, t); break;
+    case 16: hipLaunchKernelGGL((gather_fused_kernel<SW, 16, CPL>), grid, block, 0, stream, t); break;
+    default: hipLaunchKernelGGL((gather_fused_kernel<SW, 8, CPL>), grid, block, 0, stream, t); break;
+  }
+  KSJ_LAUNCH_CHECK();
+}
+
 template <int SW, int CPL>
 void launch_gather(GatherArgs a, hipStream_t stream) {
   constexpr int NSW = 64 / SW;
@@ -716,6 +783,73 @@ void grid_gather(const Plan2& plan, const float2* d_sorted, uint32_t coils, floa
     case 32: return two ? launch_gather<16, 2>(a, stream) : launch_gather<32, 1>(a, stream);
     case 64: return two ? launch_gather<32, 2>(a, stream) : launch_gather<64, 1>(a, stream);
     default: throw Error(1, "gridding supports at most 64 coils");
+  }
+}
+
+size_t grid_fused_table_bytes() {
+  return 4 * fused_table_bytes<PrepItemArgs>() + 5 * fused_table_bytes<GatherArgs>();
+}
+
+void grid_fused(const GridFusedItem* items, uint32_t count, void* d_tables, hipStream_t stream) {
+  require(count >= 1 && count <= kMaxFusedItems, "fused gridding takes 1..64 items");
+  const bool two = tune_get(kTuneGatherCoilsPerLane) == 2;
+  FusedTableBuilder<PrepItemArgs> prep[4];
+  FusedTableBuilder<GatherArgs> gather[4], reduce;
+  for (uint32_t i = 0; i < count; ++i) {
+    const GridFusedItem& it = items[i];
+    require(it.plan != nullptr && it.plan->has_trajectory, "fused item without a ready plan");
+    require(it.plan->scheme == kSchemeBlocks, "fused gridding runs the 2x2-block schedule");
+    require(it.channels >= 1 && it.channels <= kMaxStackedCoils, "fused items take 1..64 channels");
+    require(it.plan->multi_count == 0 || it.partial != nullptr, "multi-item blocks need a partial-sum buffer");
+    const uint32_t cp = padded_coils(it.channels);
+    const int cls = cp == 8 ? 0 : cp == 16 ? 1 : cp == 32 ? 2 : 3;
+    PrepItemArgs pa{};
+    pa.planes = StridedPlanes{it.kspace, it.plan->d.samples};
+    pa.weight = it.plan->weight;
+    pa.rank = it.plan->rank;
+    pa.sorted = it.sorted;
+    pa.samples = it.plan->d.samples;
+    pa.coils = it.channels;
+    prep[cls].add(pa, ceil_div(pa.samples, kPrepSamples));
+    GatherArgs ga = gather_args(*it.plan, it.sorted, it.channels, it.grid, it.partial);
+    ga.xcd_chunk = 0;
+    const uint32_t sw = (two && cp >= 32) ? cp / 2 : cp;
+    gather[cls].add(ga, std::max<uint64_t>(1, ceil_div(ceil_div(ga.item_count, 64 / sw), 4)));
+    if (ga.multi_count > 0) reduce.add(ga, ceil_div(static_cast<uint64_t>(ga.multi_count) * 4 * cp, 256));
+  }
+  char* tab = static_cast<char*>(d_tables);
+  const dim3 prep_block(kPrepSamples * kPrepWaves);
+  for (int cls = 0; cls < 4; ++cls) {
+    void* d = tab + cls * fused_table_bytes<PrepItemArgs>();
+    if (prep[cls].t.count == 0) continue;
+    prep[cls].upload(d, stream);
+    const auto* t = static_cast<const FusedTable<PrepItemArgs>*>(d);
+    const dim3 grid(static_cast<uint32_t>(prep[cls].blocks()));
+    switch (cls) {
+      case 0: hipLaunchKernelGGL(prep_fused_kernel<8>, grid, prep_block, 0, stream, t); break;
+      case 1: hipLaunchKernelGGL(prep_fused_kernel<16>, grid, prep_block, 0, stream, t); break;
+      case 2: hipLaunchKernelGGL(prep_fused_kernel<32>, grid, prep_block, 0, stream, t); break;
+      default: hipLaunchKernelGGL(prep_fused_kernel<64>, grid, prep_block, 0, stream, t); break;
+    }
+    KSJ_LAUNCH_CHECK();
+  }
+  tab += 4 * fused_table_bytes<PrepItemArgs>();
+  for (int cls = 0; cls < 4; ++cls) {
+    void* d = tab + cls * fused_table_bytes<GatherArgs>();
+    if (gather[cls].t.count == 0) continue;
+    switch (cls) {
+      case 0: launch_gather_fused<8, 1>(gather[cls], d, stream); break;
+      case 1: launch_gather_fused<16, 1>(gather[cls], d, stream); break;
+      case 2: two ? launch_gather_fused<16, 2>(gather[cls], d, stream) : launch_gather_fused<32, 1>(gather[cls], d, stream); break;
+      default: two ? launch_gather_fused<32, 2>(gather[cls], d, stream) : launch_gather_fused<64, 1>(gather[cls], d, stream); break;
+    }
+  }
+  if (reduce.t.count > 0) {
+    void* d = tab + 4 * fused_table_bytes<GatherArgs>();
+    reduce.upload(d, stream);
+    hipLaunchKernelGGL(reduce_fused_kernel, dim3(static_cast<uint32_t>(reduce.blocks())), dim3(256), 0, stream,
+                       static_cast<const FusedTable<GatherArgs>*>(d));
+    KSJ_LAUNCH_CHECK();
   }
 }
 

@@ -37,4 +37,20 @@ void grid_gather(const Plan2& plan, const float2* d_sorted, uint32_t coils, floa
 // (item order, so the sum is deterministic); a no-op when no block was cut.
 void grid_reduce(const Plan2& plan, uint32_t coils, float2* d_grid, const float2* d_partial, hipStream_t stream);
 
+// One item of a fused uneven batch: a plan (2x2-block schedule), its [C][S] k-space
+// and the item's own sorted-row, grid and partial-sum buffers.
+struct GridFusedItem {
+  const Plan2* plan = nullptr;
+  const float2* kspace = nullptr;
+  uint32_t channels = 0;
+  float2* sorted = nullptr;
+  float2* grid = nullptr;
+  float2* partial = nullptr;
+};
+// Device bytes of the per-launch argument tables grid_fused writes.
+size_t grid_fused_table_bytes();
+// prep, gather and reduce of up to 64 items, one launch per stage and padded coil
+// count; every item's results equal grid_prep + grid_gather + grid_reduce on it.
+void grid_fused(const GridFusedItem* items, uint32_t count, void* d_tables, hipStream_t stream);
+
 }  // namespace ksj::gpu

@@ -1,18 +1,18 @@
 // Pipeline orchestration (host side; every step is enqueued on one stream).
 #include "recon.hpp"
 
+#include "combine.hpp"
+#include "fft.hpp"
+#include "fused.hpp"
+#include "grid.hpp"
+#include "prof.hpp"
+
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <tuple>
 #include <utility>
 #include <vector>
-
-#include "combine.hpp"
-#include "fft.hpp"
-#include "grid.hpp"
-#include "prof.hpp"
-
-#include <algorithm>
 
 namespace ksj::gpu {
 namespace {
@@ -118,6 +118,50 @@ Lanes& lane_pool(hipStream_t caller, int kind) {
 Lanes& lanes_for(hipStream_t caller) { return lane_pool(caller, 0); }
 Lanes& multi_lanes_for(hipStream_t caller) { return lane_pool(caller, 1); }
 
+// Row pass: lines (grid row y, coil c) of the coil-fastest grid [G_r][G_c][Cp],
+// loaded coil-fastest, stored as coil planes [C][G_r][N_c] (only the N_c kept columns).
+FftArgs row_pass_args(const Plan2& plan, uint32_t total, size_t cp, const float2* grid, float2* rowout) {
+  const auto& d = plan.d;
+  FftArgs a;
+  a.in = grid;
+  a.out = rowout;
+  a.src.planes = d.grid_rows;
+  a.src.lines = total;
+  a.src.plane_stride = static_cast<uint64_t>(d.grid_cols) * cp;
+  a.src.line_stride = 1;
+  a.src.elem_stride = cp;
+  a.dst.planes = d.grid_rows;
+  a.dst.lines = total;
+  a.dst.plane_stride = d.image_cols;
+  a.dst.line_stride = static_cast<uint64_t>(d.grid_rows) * d.image_cols;
+  a.dst.elem_stride = 1;
+  a.n = d.grid_cols;
+  a.n_out = d.image_cols;
+  a.out_offset = d.centred ? (d.grid_cols - d.image_cols / 2) % d.grid_cols : 0;
+  a.inverse = 1;
+  a.load_lines_fast = 1;
+  a.store_lines_fast = 0;
+  return a;
+}
+
+// Column pass with deapodization and crop -> coil planes [C][N_r][N_c].
+FftArgs col_pass_args(const Plan2& plan, uint32_t total, const float2* rowout, float2* out) {
+  const auto& d = plan.d;
+  FftArgs b;
+  b.in = rowout;
+  b.out = out;
+  b.src = cols_of(total, d.grid_rows, d.image_cols);
+  b.dst = cols_of(total, d.image_rows, d.image_cols);
+  b.n = d.grid_rows;
+  b.n_out = d.image_rows;
+  b.out_offset = d.centred ? (d.grid_rows - d.image_rows / 2) % d.grid_rows : 0;
+  b.out_weight = plan.deapod_rows;
+  b.line_weight = plan.deapod_cols;
+  b.inverse = 1;
+  b.load_lines_fast = b.store_lines_fast = 1;
+  return b;
+}
+
 uint32_t pass_count(uint32_t channels, uint32_t scans) {
   const uint32_t per = adjoint2_scans_per_pass(channels);
   return (std::max(scans, 1u) + per - 1) / per;
@@ -201,46 +245,14 @@ void adjoint2_execute(const Plan2& plan, const AdjointScan* scans, uint32_t coun
       grid_reduce(plan, total, grid, partial, stream);
     }
 
-    // Row pass: lines (grid row y, coil c) of the coil-fastest grid [G_r][G_c][Cp],
-    // loaded coil-fastest, stored as coil planes [C][G_r][N_c].
-    FftArgs a;
-    a.in = grid;
-    a.out = rowout;
-    a.src.planes = d.grid_rows;
-    a.src.lines = total;
-    a.src.plane_stride = static_cast<uint64_t>(d.grid_cols) * cp;
-    a.src.line_stride = 1;
-    a.src.elem_stride = cp;
-    a.dst.planes = d.grid_rows;
-    a.dst.lines = total;
-    a.dst.plane_stride = d.image_cols;
-    a.dst.line_stride = static_cast<uint64_t>(d.grid_rows) * d.image_cols;
-    a.dst.elem_stride = 1;
-    a.n = d.grid_cols;
-    a.n_out = d.image_cols;
-    a.out_offset = d.centred ? (d.grid_cols - d.image_cols / 2) % d.grid_cols : 0;
-    a.inverse = 1;
-    a.load_lines_fast = 1;
-    a.store_lines_fast = 0;
+    const FftArgs a = row_pass_args(plan, total, cp, grid, rowout);
     {
       ProfScope ps("fft_rows", stream);
       fft_launch(a, stream);
     }
 
-    // Column pass with deapodization and crop -> coil planes [C][N_r][N_c].
     const bool direct = (count == 1 && layout == kLayoutCoilMajor && scans[0].images != nullptr);
-    FftArgs b;
-    b.in = rowout;
-    b.out = direct ? scans[0].images : img;
-    b.src = cols_of(total, d.grid_rows, d.image_cols);
-    b.dst = cols_of(total, d.image_rows, d.image_cols);
-    b.n = d.grid_rows;
-    b.n_out = d.image_rows;
-    b.out_offset = d.centred ? (d.grid_rows - d.image_rows / 2) % d.grid_rows : 0;
-    b.out_weight = plan.deapod_rows;
-    b.line_weight = plan.deapod_cols;
-    b.inverse = 1;
-    b.load_lines_fast = b.store_lines_fast = 1;
+    const FftArgs b = col_pass_args(plan, total, rowout, direct ? scans[0].images : img);
     {
       ProfScope ps("fft_cols_deapod", stream);
       fft_launch(b, stream);
@@ -297,20 +309,118 @@ MultiLayout multi_layout(const MultiItem* items, uint32_t count) {
 
 }  // namespace
 
-size_t adjoint2_multi_workspace_bytes(const MultiItem* items, uint32_t count) {
+namespace {
+
+// The fused form runs every stage once for all items (one launch per stage and
+// shape class, argument tables in the workspace) when every item fits one pass of
+// the 2x2-block schedule; otherwise items go round-robin over lanes.
+bool fused_eligible(const MultiItem* items, uint32_t count) {
+  if (tune_get(kTuneMultiFused) == 0 || count > kMaxFusedItems) return false;
+  for (uint32_t i = 0; i < count; ++i)
+    if (items[i].plan->scheme != kSchemeBlocks || items[i].channels > kMaxStackedCoils) return false;
+  return true;
+}
+
+size_t fused_tables_bytes(uint32_t count) {
+  return align256(grid_fused_table_bytes()) + 2 * align256(fft_fused_table_bytes(count)) +
+         align256(coil_split_fused_table_bytes());
+}
+
+size_t fused_total_bytes(const MultiItem* items, uint32_t count) {
+  size_t total = fused_tables_bytes(count);
+  for (uint32_t i = 0; i < count; ++i)
+    total += align256(adjoint2_pass_workspace_bytes(*items[i].plan, items[i].channels, 1));
+  return total;
+}
+
+size_t lanes_total_bytes(const MultiItem* items, uint32_t count) {
   const MultiLayout m = multi_layout(items, count);
   size_t total = 0;
   for (size_t b : m.lane_ws) total += align256(b);
   return total;
 }
 
+void execute_fused(const MultiItem* items, uint32_t count, int layout, void* d_ws, hipStream_t stream) {
+  char* p = static_cast<char*>(d_ws);
+  char* grid_tab = p;
+  char* row_tab = grid_tab + align256(grid_fused_table_bytes());
+  char* col_tab = row_tab + align256(fft_fused_table_bytes(count));
+  char* split_tab = col_tab + align256(fft_fused_table_bytes(count));
+  p += fused_tables_bytes(count);
+  std::vector<GridFusedItem> gi(count);
+  std::vector<FftArgs> rows(count), cols(count);
+  std::vector<const float2*> split_in(count);
+  std::vector<float2*> split_last(count);
+  std::vector<float*> split_rss(count);
+  std::vector<uint64_t> pixels(count);
+  std::vector<uint32_t> coils(count);
+  for (uint32_t i = 0; i < count; ++i) {
+    const MultiItem& it = items[i];
+    const Plan2Desc& d = it.plan->d;
+    require(it.scan.kspace != nullptr, "null k-space");
+    const uint32_t c = it.channels;
+    const size_t cp = padded_coils(c);
+    char* q = p;
+    p += align256(adjoint2_pass_workspace_bytes(*it.plan, c, 1));
+    auto* sorted = reinterpret_cast<float2*>(q);
+    q += align256(static_cast<size_t>(d.samples) * cp * sizeof(float2));
+    auto* grid = reinterpret_cast<float2*>(q);
+    q += align256(cp * d.grid_rows * d.grid_cols * sizeof(float2));
+    auto* rowout = reinterpret_cast<float2*>(q);
+    q += align256(static_cast<size_t>(c) * d.grid_rows * d.image_cols * sizeof(float2));
+    auto* img = reinterpret_cast<float2*>(q);
+    q += align256(static_cast<size_t>(c) * d.image_rows * d.image_cols * sizeof(float2));
+    gi[i] = GridFusedItem{it.plan, it.scan.kspace, c, sorted, grid, reinterpret_cast<float2*>(q)};
+    float2* planes_out = (layout == kLayoutCoilMajor && it.scan.images) ? it.scan.images : img;
+    rows[i] = row_pass_args(*it.plan, c, cp, grid, rowout);
+    cols[i] = col_pass_args(*it.plan, c, rowout, planes_out);
+    split_in[i] = planes_out;
+    split_last[i] = layout == kLayoutCoilLast ? it.scan.images : nullptr;
+    split_rss[i] = it.scan.rss;
+    pixels[i] = static_cast<uint64_t>(d.image_rows) * d.image_cols;
+    coils[i] = c;
+  }
+  {
+    ProfScope ps("multi_grid", stream);
+    grid_fused(gi.data(), count, grid_tab, stream);
+  }
+  {
+    ProfScope ps("multi_fft_rows", stream);
+    fft_launch_fused(rows.data(), count, row_tab, stream);
+  }
+  {
+    ProfScope ps("multi_fft_cols_deapod", stream);
+    fft_launch_fused(cols.data(), count, col_tab, stream);
+  }
+  ProfScope ps("multi_coil_combine", stream);
+  coil_split_fused(split_in.data(), split_last.data(), split_rss.data(), pixels.data(), coils.data(), count,
+                   split_tab, stream);
+}
+
+}  // namespace
+
+size_t adjoint2_multi_workspace_bytes(const MultiItem* items, uint32_t count) {
+  // Enough for either form, so a tuning change between query and call stays valid.
+  size_t total = lanes_total_bytes(items, count);
+  if (count <= kMaxFusedItems) {
+    bool blocks = true;
+    for (uint32_t i = 0; i < count; ++i)
+      blocks = blocks && items[i].plan->scheme == kSchemeBlocks && items[i].channels <= kMaxStackedCoils;
+    if (blocks) total = std::max(total, fused_total_bytes(items, count));
+  }
+  return total;
+}
+
 void adjoint2_execute_multi(const MultiItem* items, uint32_t count, int layout, void* d_ws, size_t ws_bytes,
                             hipStream_t stream) {
   require(count >= 1, "multi adjoint needs at least one item");
+  require(layout == kLayoutCoilLast || layout == kLayoutCoilMajor, "unknown image layout");
   const MultiLayout m = multi_layout(items, count);
-  size_t total = 0;
-  for (size_t b : m.lane_ws) total += align256(b);
-  require(ws_bytes >= total, "multi adjoint workspace too small");
+  if (fused_eligible(items, count)) {
+    require(ws_bytes >= fused_total_bytes(items, count), "multi adjoint workspace too small");
+    return execute_fused(items, count, layout, d_ws, stream);
+  }
+  require(ws_bytes >= lanes_total_bytes(items, count), "multi adjoint workspace too small");
   if (m.lanes == 1) {
     for (uint32_t j = 0; j < count; ++j) {
       const MultiItem& it = items[m.order[j]];

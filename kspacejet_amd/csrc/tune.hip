@@ -29,6 +29,7 @@ Param g_params[kTuneKeys] = {
     {"tile_width", 4, [](int64_t v) { return v == 4 || v == 8; }},
     {"block_group", 2, [](int64_t v) { return v == 1 || v == 2 || v == 4; }},
     {"fft_schedule", 0, [](int64_t v) { return v >= 0 && v <= 2; }},
+    {"multi_fused", 0, [](int64_t v) { return v == 0 || v == 1; }},
 };
 
 }  // namespace

@@ -1,7 +1,8 @@
 """Uneven batches: items of different matrix, coil count and trajectory in one
-multi-plan call (ksj_gpu_adjoint2_execute_multi), eagerly and as a captured HIP
-graph.  Each item must equal its single-plan result bit for bit and the FP64 oracle
-to rel-L2 <= 1e-5; every graph replay must equal the eager result."""
+multi-plan call (ksj_gpu_adjoint2_execute_multi), fused (one launch per stage and
+shape class) or over lanes, eagerly and as a captured HIP graph.  Each item must
+equal its single-plan result bit for bit and the FP64 oracle to rel-L2 <= 1e-5; every
+graph replay must equal the eager result."""
 import numpy as np
 import pytest
 
@@ -67,9 +68,37 @@ def test_items_match_single_plans_and_the_oracle(ops, batch):
         assert rel_l2(rss[i].cpu().numpy(), O.rss(ref)) <= TOL, i
 
 
+def test_fused_and_lane_forms_give_identical_results(ops, batch):
+    """One launch per stage and shape class (multi_fused = 1) and items spread over
+    lanes (0) run the same per-item arithmetic: bit-identical images and RSS, in both
+    image layouts."""
+    plans, ks, _ = batch
+    saved = ops.tune_get("multi_fused")
+    results = []
+    try:
+        for fused in (1, 0):
+            ops.tune_set("multi_fused", fused)
+            for layout in ("coil_last", "coil_major"):
+                rss = [torch.zeros((p.desc.image_rows, p.desc.image_cols), dtype=torch.float32, device="cuda")
+                       for p in plans]
+                shape = (lambda p, k: (p.desc.image_rows, p.desc.image_cols, k.shape[0])) if layout == "coil_last" \
+                    else (lambda p, k: (k.shape[0], p.desc.image_rows, p.desc.image_cols))
+                imgs = [torch.zeros(shape(p, k), dtype=torch.complex64, device="cuda") for p, k in zip(plans, ks)]
+                ws = torch.empty(ops.adjoint_multi_workspace_bytes(plans, [k.shape[0] for k in ks]),
+                                 dtype=torch.uint8, device="cuda")
+                ops.adjoint_multi(plans, ks, rss=rss, images=imgs, workspace=ws, layout=layout)
+                torch.cuda.synchronize()
+                results.append([r.cpu().numpy() for r in rss] + [m.cpu().numpy() for m in imgs])
+    finally:
+        ops.tune_set("multi_fused", saved)
+    for a, b in zip(results[:2], results[2:]):
+        assert all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
 def test_item_lanes_do_not_change_results(ops, batch):
     plans, ks, _ = batch
-    saved = ops.tune_get("multi_lanes")
+    saved = ops.tune_get("multi_lanes"), ops.tune_get("multi_fused")
+    ops.tune_set("multi_fused", 0)
     results = []
     try:
         for lanes in (1, 3, 8):
@@ -81,13 +110,17 @@ def test_item_lanes_do_not_change_results(ops, batch):
             torch.cuda.synchronize()
             results.append([r.cpu().numpy() for r in rss] + [m.cpu().numpy() for m in imgs])
     finally:
-        ops.tune_set("multi_lanes", saved)
+        ops.tune_set("multi_lanes", saved[0])
+        ops.tune_set("multi_fused", saved[1])
     for res in results[1:]:
         assert all(np.array_equal(a, b) for a, b in zip(res, results[0]))
 
 
-def test_captured_graph_replays_equal_the_eager_result(ops, batch):
+@pytest.mark.parametrize("fused", [0, 1])
+def test_captured_graph_replays_equal_the_eager_result(ops, batch, fused):
     plans, ks, _ = batch
+    saved = ops.tune_get("multi_fused")
+    ops.tune_set("multi_fused", fused)
     stream = torch.cuda.Stream()
     rss, imgs = outputs(batch)
     ws = torch.empty(ops.adjoint_multi_workspace_bytes(plans, [k.shape[0] for k in ks]), dtype=torch.uint8,
@@ -109,3 +142,4 @@ def test_captured_graph_replays_equal_the_eager_result(ops, batch):
         got = rss + imgs
         assert all(torch.equal(a, b) for a, b in zip(got, eager)), replay
     graph.close()
+    ops.tune_set("multi_fused", saved)
