@@ -333,6 +333,21 @@ size_t fused_total_bytes(const MultiItem* items, uint32_t count) {
   return total;
 }
 
+// The fused form over lanes: items dealt round-robin (largest first) onto the
+// multi_lanes streams, each lane running the fused stages over its own items.
+std::vector<std::vector<MultiItem>> fused_lanes(const MultiItem* items, uint32_t count) {
+  const MultiLayout m = multi_layout(items, count);
+  std::vector<std::vector<MultiItem>> lanes(m.lanes);
+  for (uint32_t j = 0; j < count; ++j) lanes[j % m.lanes].push_back(items[m.order[j]]);
+  return lanes;
+}
+
+size_t fused_lanes_bytes(const MultiItem* items, uint32_t count) {
+  size_t total = 0;
+  for (const auto& l : fused_lanes(items, count)) total += fused_total_bytes(l.data(), static_cast<uint32_t>(l.size()));
+  return total;
+}
+
 size_t lanes_total_bytes(const MultiItem* items, uint32_t count) {
   const MultiLayout m = multi_layout(items, count);
   size_t total = 0;
@@ -406,7 +421,7 @@ size_t adjoint2_multi_workspace_bytes(const MultiItem* items, uint32_t count) {
     bool blocks = true;
     for (uint32_t i = 0; i < count; ++i)
       blocks = blocks && items[i].plan->scheme == kSchemeBlocks && items[i].channels <= kMaxStackedCoils;
-    if (blocks) total = std::max(total, fused_total_bytes(items, count));
+    if (blocks) total = std::max(total, fused_lanes_bytes(items, count));
   }
   return total;
 }
@@ -417,8 +432,23 @@ void adjoint2_execute_multi(const MultiItem* items, uint32_t count, int layout, 
   require(layout == kLayoutCoilLast || layout == kLayoutCoilMajor, "unknown image layout");
   const MultiLayout m = multi_layout(items, count);
   if (fused_eligible(items, count)) {
-    require(ws_bytes >= fused_total_bytes(items, count), "multi adjoint workspace too small");
-    return execute_fused(items, count, layout, d_ws, stream);
+    require(ws_bytes >= fused_lanes_bytes(items, count), "multi adjoint workspace too small");
+    const auto lanes = fused_lanes(items, count);
+    if (lanes.size() == 1) return execute_fused(items, count, layout, d_ws, stream);
+    Lanes& ls = multi_lanes_for(stream);
+    KSJ_HIP(hipEventRecord(ls.fork, stream));
+    char* p = static_cast<char*>(d_ws);
+    for (uint32_t l = 0; l < lanes.size(); ++l) {
+      const uint32_t n = static_cast<uint32_t>(lanes[l].size());
+      KSJ_HIP(hipStreamWaitEvent(ls.stream[l], ls.fork, 0));
+      execute_fused(lanes[l].data(), n, layout, p, ls.stream[l]);
+      p += fused_total_bytes(lanes[l].data(), n);
+    }
+    for (uint32_t l = 0; l < lanes.size(); ++l) {
+      KSJ_HIP(hipEventRecord(ls.join[l], ls.stream[l]));
+      KSJ_HIP(hipStreamWaitEvent(stream, ls.join[l], 0));
+    }
+    return;
   }
   require(ws_bytes >= lanes_total_bytes(items, count), "multi adjoint workspace too small");
   if (m.lanes == 1) {
