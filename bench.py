@@ -86,15 +86,25 @@ def algorithmic_bytes(stage: str) -> float:
     }[stage]
 
 
-def measured_traffic(scans_per_launch: int):
+def gather_kernel_label() -> str:
+    """Template instance of the 64-coil gather kernel the current tuning launches."""
+    from kspacejet_amd import ops
+    if ops.tune_get("gather_scheme") == 2:
+        return f"gather_matrix_kernel<64,{ops.tune_get('matrix_batch') // 4},{ops.tune_get('matrix_pipeline')}>"
+    two = ops.tune_get("gather_coils_per_lane") == 2
+    return f"gather_item_kernel<{32 if two else 64},{ops.tune_get('gather_batch')},{2 if two else 1}>"
+
+
+def measured_traffic(label: str, scans_per_launch: int):
     """HBM bytes per launch of the batched gather kernel, from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE passes of this workload (already scaled by the copy-kernel
-    calibration), or None when that summary is absent or was taken at another stacking."""
+    calibration), or None when that summary is absent or was taken of another kernel or
+    stacking."""
     try:
         for line in open(TRAFFIC_PROFILE):
             if line.startswith("traffic_bytes_per_launch"):
                 _, key, value = line.split()
-                if key == f"gather_item_kernel<64,8>@{scans_per_launch}scans":
+                if key == f"{label}@{scans_per_launch}scans":
                     return float(value)
     except OSError:
         pass
@@ -277,9 +287,10 @@ def main(argv=None) -> int:
     g = stages["grid_gather"]
     per_launch_bytes = algorithmic_bytes("grid_gather") * scans_per_launch
     achieved = per_launch_bytes / (g["avg_us"] * 1e-6) / 1e9
-    roofline = {"kernel": "gather_item_kernel<64,8> (grid_gather)", "bound": "hbm", "achieved": round(achieved, 1),
+    label = gather_kernel_label()
+    roofline = {"kernel": f"{label} (grid_gather)", "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(scans_per_launch), "scans_per_launch": scans_per_launch,
+                "traffic": measured_traffic(label, scans_per_launch), "scans_per_launch": scans_per_launch,
                 "algorithmic_bytes_per_launch": per_launch_bytes, "avg_us": round(g["avg_us"], 2)}
     stage_report = {k: {"avg_us": round(v["avg_us"], 2),
                         "gbs": round(algorithmic_bytes(k) * scans_per_launch / max(v["avg_us"] * 1e-6, 1e-12) / 1e9,

@@ -4,7 +4,7 @@ It times a caller-supplied workload -- a function that enqueues GPU work on torc
 current stream -- with HIP events, and walks the tuning keys one at a time
 (coordinate descent): each key is set to each of its candidate values with the
 others held at their best so far, and keeps the fastest.  The default keys leave the
-results bit-identical; the FFT radix schedule (rounding only) and item_visits
+results bit-identical; the FFT radix schedule (rounding only) and the work-item size
 (summation order of new plans) are tuned only when asked for, and every value of
 them is tested against the FP64 oracle, so tuning never trades accuracy for speed.  Chosen settings are kept
 per workload signature in a JSON file when a path is given, so a later process can
@@ -23,11 +23,15 @@ import torch
 
 from . import ops
 
-# Candidate values per key; every value gives bit-identical results.
+# Candidate values per key; every value gives bit-identical results.  The gather keys
+# are those of the default (matrix) schedule; plans of the 2x2-block schedule tune
+# gather_batch (4, 8, 16) and gather_coils_per_lane (1, 2) instead.
 EXACT_KEYS: Dict[str, tuple] = {
-    "gather_batch": (8, 4, 16),
-    "gather_map": (0, 2),
-    "gather_coils_per_lane": (1, 2),
+    "matrix_batch": (16, 8, 4),
+    "matrix_slab": (16, 1, 64),
+    "matrix_pipeline": (1, 2),
+    "matrix_run_cost": (8, 16, 4),
+    "gather_map": (0, 1),
     "fft_lines_per_block": (0, 8, 32),
     "pass_lanes": (2, 1, 3),
     "multi_lanes": (4, 2, 6, 8),
@@ -38,7 +42,9 @@ EXACT_KEYS: Dict[str, tuple] = {
 ROUNDING_KEYS: Dict[str, tuple] = {"fft_schedule": (0, 1, 2)}
 # Changes the float summation order of the gather of new plans (results stay within
 # tolerance); needs `rebuild`.
-ORDER_KEYS: Dict[str, tuple] = {"item_visits": (64, 32, 128)}
+ORDER_KEYS: Dict[str, tuple] = {"matrix_item_visits": (128, 64, 256)}
+# Keys read when a plan is built: tried only when a `rebuild` is given.
+PLAN_KEYS = {"matrix_item_visits", "item_visits", "matrix_run_cost"}
 
 
 def time_ms(run: Callable[[], None], repeats: int = 5) -> float:
@@ -63,8 +69,8 @@ def autotune(run: Callable[[], None], signature: str, keys: Optional[Dict[str, I
     """Tunes `keys` (default: every bit-exact key, plus the FFT radix schedule when
     `rounding`) for `run` and leaves the best values set.
 
-    `rebuild` is called after a key that only affects new plans changes (item_visits),
-    so the workload picks it up.  Returns {"settings": {...}, "ms": best time,
+    `rebuild` is called after a key that only affects new plans changes (PLAN_KEYS),
+    so the workload picks it up; without it those keys are skipped.  Returns {"settings": {...}, "ms": best time,
     "baseline_ms": time with the starting settings, "trials": [...]}.
     """
     if cache_path and os.path.exists(cache_path):
@@ -81,11 +87,13 @@ def autotune(run: Callable[[], None], signature: str, keys: Optional[Dict[str, I
     best_ms = baseline = time_ms(run, repeats)
     trials = [{"settings": dict(best), "ms": round(best_ms, 4)}]
     for key, values in keys.items():
+        if key in PLAN_KEYS and rebuild is None:
+            continue
         for v in values:
             if v == best[key] or time.perf_counter() > t_end:
                 continue
             ops.tune_set(key, v)
-            if rebuild is not None and key in ORDER_KEYS:
+            if key in PLAN_KEYS:
                 rebuild()
             ms = time_ms(run, repeats)
             trials.append({"settings": dict(best, **{key: v}), "ms": round(ms, 4)})
@@ -93,7 +101,7 @@ def autotune(run: Callable[[], None], signature: str, keys: Optional[Dict[str, I
                 best_ms, best[key] = ms, v
             else:
                 ops.tune_set(key, best[key])
-                if rebuild is not None and key in ORDER_KEYS:
+                if key in PLAN_KEYS:
                     rebuild()
     result = {"settings": best, "ms": round(best_ms, 4), "baseline_ms": round(baseline, 4), "trials": trials}
     if cache_path:

@@ -28,6 +28,9 @@ HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC", "-Wall", "-Wno-unused-result",
     # No packed FP32 instructions anywhere in the device code.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # Matrix-core accumulators in ordinary VGPRs: the AGPR form makes the register
+    # allocator copy accumulators between the two files on every gather batch.
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     f"-I{os.path.join(ROOT, 'include')}",
 ]
 CXX_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-Wall", "-fvisibility=hidden", "-D__HIP_PLATFORM_AMD__", f"-I{os.path.join(ROOT, 'include')}",

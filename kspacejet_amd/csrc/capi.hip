@@ -177,8 +177,7 @@ int ksj_gpu_plan2_create(const ksj_gpu_plan2_desc* desc, ksj_gpu_plan2** out_pla
     d.density = desc->density;
     d.centred = desc->centred;
     d.samples = desc->samples;
-    d.item_visits = desc->item_visits == 0 ? static_cast<uint32_t>(ksj::gpu::tune_get(ksj::gpu::kTuneItemVisits))
-                                           : desc->item_visits;
+    d.item_visits = desc->item_visits;  // 0: the schedule's tuning key
     ksj::gpu::require(d.density <= KSJ_GPU_DENSITY_GIVEN, "unknown density mode");
     auto* handle = new ksj_gpu_plan2{nullptr};
     try {
@@ -204,8 +203,10 @@ int ksj_gpu_plan2_get_info(const ksj_gpu_plan2* plan, ksj_gpu_plan2_info* out_in
     ksj::gpu::require(out_info->struct_size >= sizeof(ksj_gpu_plan2_info), "info struct too small");
     out_info->items = plan->impl->item_count;
     out_info->visits = plan->impl->visit_count;
-    out_info->device_bytes = plan->impl->device_bytes + plan->impl->visit_capacity * (sizeof(uint32_t) + sizeof(float4)) +
-                             static_cast<uint64_t>(plan->impl->item_capacity) * sizeof(uint32_t);
+    const uint64_t side = ksj::gpu::block_side(plan->impl->scheme);
+    out_info->device_bytes = plan->impl->device_bytes +
+                             plan->impl->visit_capacity * (sizeof(uint32_t) + side * side * sizeof(float)) +
+                             static_cast<uint64_t>(plan->impl->item_capacity) * (sizeof(uint4) + sizeof(uint32_t));
     out_info->beta = plan->impl->d.beta;
   });
 }

@@ -54,12 +54,19 @@ enum TuneKey : int {
   kTunePassLanes,         // streams over which the passes of a batched adjoint overlap: 1..3
   kTuneMultiLanes,        // streams over which the items of a multi-plan adjoint are spread: 1..8
   kTuneGatherCoilsPerLane,  // coils each gather lane loads per visit for 32+ coils: 1 or 2
-  kTuneGatherScheme,      // schedule of new plans: 0 = 2x2 blocks, 1 = 8x8 LDS tiles (summation order changes)
+  kTuneGatherScheme,      // schedule of new plans: 0 = 2x2 blocks, 1 = LDS tiles, 2 = 4x4 blocks on the matrix cores
+                          // (summation order changes)
   kTuneTileItemSamples,   // window samples per tile work item of new plans (summation order changes)
   kTuneTileWidth,         // tile columns of new plans: 4 (16 x 4 tiles) or 8 (8 x 8) (summation order changes)
   kTuneBlockGroup,        // 2x2-block group side of new plans: 1, 2 or 4 (work-item order; results unchanged)
   kTuneFftSchedule,       // FFT radix schedule per length: 0, 1 (half the threads) or 2 (radix-4 heavy); rounding changes
   kTuneMultiFused,        // uneven batches: 1 = one launch per stage and shape class, 0 = items over lanes
+  kTuneMatrixRunCost,     // matrix schedule of new plans: cost (visit groups of four + 1 per item) per wave run, 1..64
+  kTuneMatrixPipeline,    // matrix gather: batches of sample rows in flight per wave, 1 or 2
+  kTuneMatrixNtStore,     // matrix gather: 1 = grid cells written with non-temporal stores
+  kTuneMatrixSlab,        // matrix gather, gather_map 0: consecutive workgroups per XCD slab (1 = launch order)
+  kTuneMatrixItemVisits,  // matrix schedule: work-item size of new plans (visits; summation order changes)
+  kTuneMatrixBatch,       // matrix gather: visits per batch, 4, 8 or 16
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);

@@ -37,7 +37,7 @@ void grid_gather(const Plan2& plan, const float2* d_sorted, uint32_t coils, floa
 // (item order, so the sum is deterministic); a no-op when no block was cut.
 void grid_reduce(const Plan2& plan, uint32_t coils, float2* d_grid, const float2* d_partial, hipStream_t stream);
 
-// One item of a fused uneven batch: a plan (2x2-block schedule), its [C][S] k-space
+// One item of a fused uneven batch: a plan (2x2-block or matrix schedule), its [C][S] k-space
 // and the item's own sorted-row, grid and partial-sum buffers.
 struct GridFusedItem {
   const Plan2* plan = nullptr;

@@ -79,25 +79,29 @@ __global__ void taps_kernel(TapArgs a) {
 }
 
 // ----------------------------------------------------------------------------- gather schedule
-// Window of 2x2 block (BY, BX): bin rows 2BY - W/2 + r and columns 2BX - W/2 + c for
-// r, c in [0, W]; cell (2BY + dy, 2BX + dx) uses row tap dy - r + W - 1 and column
-// tap dx - c + W - 1 of a sample in window bin (r, c) (taps outside [0, W) weigh 0).
+// Window of s x s block (BY, BX): bin rows s BY - W/2 + r and columns s BX - W/2 + c
+// for r, c in [0, W + s - 1); cell (s BY + dy, s BX + dx) uses row tap dy - r + W - 1
+// and column tap dx - c + W - 1 of a sample in window bin (r, c) (taps outside
+// [0, W) weigh 0).
 struct ScheduleArgs {
   const uint32_t* start;
   const float* taps;  // [S][2W]
   uint32_t grid_rows, grid_cols, block_cols, block_group;
   uint64_t nblocks;
-  int width;
+  int width, side;    // side s: 2 or 4
   uint32_t* row_off;  // counts, then offsets
+  const uint32_t* chunk_off;  // matrix schedule: first group of four of each block (null otherwise)
   uint32_t* visit_k;
-  float4* visit_w;
+  float* visit_w;     // [visits][s*s]
 };
+
+__device__ inline int window_rows(const ScheduleArgs& a) { return a.width + a.side - 1; }
 
 // False for the padding blocks past the grid (no visits).
 __device__ inline bool block_in_grid(const ScheduleArgs& a, uint64_t blk) {
   uint32_t by, bx;
   block_coords(blk, a.block_cols, a.block_group, &by, &bx);
-  return 2 * by < a.grid_rows && 2 * bx < a.grid_cols;
+  return a.side * by < a.grid_rows && a.side * bx < a.grid_cols;
 }
 
 __device__ inline uint64_t window_bin(const ScheduleArgs& a, uint64_t blk, int r, int c) {
@@ -105,8 +109,8 @@ __device__ inline uint64_t window_bin(const ScheduleArgs& a, uint64_t blk, int r
   const int gr = static_cast<int>(a.grid_rows), gc = static_cast<int>(a.grid_cols);
   uint32_t byu, bxu;
   block_coords(blk, a.block_cols, a.block_group, &byu, &bxu);
-  int by = 2 * static_cast<int>(byu) - W / 2 + r;
-  int bx = 2 * static_cast<int>(bxu) - W / 2 + c;
+  int by = a.side * static_cast<int>(byu) - W / 2 + r;
+  int bx = a.side * static_cast<int>(bxu) - W / 2 + c;
   by = ((by % gr) + gr) % gr;
   bx = ((bx % gc) + gc) % gc;
   return static_cast<uint64_t>(by) * a.grid_cols + bx;
@@ -114,17 +118,18 @@ __device__ inline uint64_t window_bin(const ScheduleArgs& a, uint64_t blk, int r
 
 __global__ void visit_count_kernel(ScheduleArgs a) {
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const uint64_t rows = a.nblocks * (a.width + 1);
+  const int R = window_rows(a);
+  const uint64_t rows = a.nblocks * R;
   if (i > rows) return;
   if (i == rows) {
     a.row_off[i] = 0;
     return;
   }
-  const uint64_t blk = i / (a.width + 1);
-  const int r = static_cast<int>(i % (a.width + 1));
+  const uint64_t blk = i / R;
+  const int r = static_cast<int>(i % R);
   uint32_t n = 0;
   if (block_in_grid(a, blk)) {
-    for (int c = 0; c <= a.width; ++c) {
+    for (int c = 0; c < R; ++c) {
       const uint64_t b = window_bin(a, blk, r, c);
       n += a.start[b + 1] - a.start[b];
     }
@@ -132,27 +137,73 @@ __global__ void visit_count_kernel(ScheduleArgs a) {
   a.row_off[i] = n;
 }
 
+// Tap t of a sample's row (or column) taps, 0 outside [0, W).
+__device__ inline float tap_or_zero(const float* t, int i, int W) { return (i >= 0 && i < W) ? t[i] : 0.f; }
+
 __global__ void visit_fill_kernel(ScheduleArgs a) {
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= a.nblocks * (a.width + 1)) return;
-  const int W = a.width;
-  const uint64_t blk = i / (W + 1);
-  const int r = static_cast<int>(i % (W + 1));
+  const int W = a.width, s = a.side, R = window_rows(a);
+  if (i >= a.nblocks * R) return;
+  const uint64_t blk = i / R;
+  const int r = static_cast<int>(i % R);
   if (!block_in_grid(a, blk)) return;
-  uint32_t pos = a.row_off[i];
-  const int ty0 = 0 - r + W - 1, ty1 = 1 - r + W - 1;
-  for (int c = 0; c <= W; ++c) {
+  // Matrix schedule: blocks start on groups of four (lists padded by pad_visits_kernel).
+  uint32_t pos = a.chunk_off ? 4 * a.chunk_off[blk] + (a.row_off[i] - a.row_off[blk * R]) : a.row_off[i];
+  for (int c = 0; c < R; ++c) {
     const uint64_t b = window_bin(a, blk, r, c);
-    const int tx0 = 0 - c + W - 1, tx1 = 1 - c + W - 1;
     for (uint32_t k = a.start[b]; k < a.start[b + 1]; ++k) {
       const float* t = a.taps + static_cast<uint64_t>(k) * (2 * W);
-      const float wy0 = (ty0 >= 0 && ty0 < W) ? t[ty0] : 0.f, wy1 = (ty1 >= 0 && ty1 < W) ? t[ty1] : 0.f;
-      const float wx0 = (tx0 >= 0 && tx0 < W) ? t[W + tx0] : 0.f, wx1 = (tx1 >= 0 && tx1 < W) ? t[W + tx1] : 0.f;
+      float* w = a.visit_w + static_cast<uint64_t>(pos) * s * s;
+      for (int dy = 0; dy < s; ++dy) {
+        const float wy = tap_or_zero(t, dy - r + W - 1, W);
+        for (int dx = 0; dx < s; ++dx) w[dy * s + dx] = wy * tap_or_zero(t + W, dx - c + W - 1, W);
+      }
       a.visit_k[pos] = k;
-      a.visit_w[pos] = make_float4(wy0 * wx0, wy0 * wx1, wy1 * wx0, wy1 * wx1);
       ++pos;
     }
   }
+}
+
+// Matrix schedule: groups of four visits per block (chunks[nblocks] = 0 for the scan).
+__global__ void chunk_count_kernel(const uint32_t* row_off, uint64_t nblocks, int rows, uint32_t* chunks) {
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b > nblocks) return;
+  chunks[b] = b < nblocks ? (row_off[(b + 1) * rows] - row_off[b * rows] + 3) / 4 : 0u;
+}
+
+// Matrix schedule: the padding visits of each block repeat its first sample with zero
+// weights (so they add exact zeros and read a row the block reads anyway).
+__global__ void pad_visits_kernel(const uint32_t* row_off, const uint32_t* chunk_off, uint64_t nblocks, int rows,
+                                  uint32_t* visit_k, float* visit_w) {
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  const uint32_t base = 4 * chunk_off[b], end = 4 * chunk_off[b + 1];
+  const uint32_t n = row_off[(b + 1) * rows] - row_off[b * rows];
+  for (uint32_t p = base + n; p < end; ++p) {
+    visit_k[p] = visit_k[base];
+    for (int c = 0; c < 16; ++c) visit_w[static_cast<uint64_t>(p) * 16 + c] = 0.f;
+  }
+}
+
+// first[r] = first item whose cost prefix reaches r * run_cost; first[runs] = items.
+__global__ void run_split_kernel(const uint32_t* prefix, uint32_t items, uint32_t runs, uint32_t run_cost,
+                                 uint32_t* first) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > runs) return;
+  if (r == runs) {
+    first[r] = items;
+    return;
+  }
+  const uint64_t target = static_cast<uint64_t>(r) * run_cost;
+  uint32_t lo = 0, hi = items;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (prefix[mid] >= target)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  first[r] = lo;
 }
 
 // ----------------------------------------------------------------------------- tile schedule
@@ -207,7 +258,9 @@ __global__ void tile_item_fill_kernel(const uint32_t* first, const uint32_t* slo
   }
 }
 
-__global__ void item_count_kernel(const uint32_t* row_off, uint64_t nblocks, int width, uint32_t item_visits,
+// Block b's work is off[(b+1) stride] - off[b stride] units (visits, with stride =
+// window rows per block; or groups of four, stride 1), cut into items of `size` units.
+__global__ void item_count_kernel(const uint32_t* off, uint64_t nblocks, int stride, uint32_t size,
                                   uint32_t* first, uint32_t* multi, uint32_t* counter) {
   const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (b > nblocks) return;
@@ -215,8 +268,8 @@ __global__ void item_count_kernel(const uint32_t* row_off, uint64_t nblocks, int
     first[b] = 0;
     return;
   }
-  const uint32_t v = row_off[(b + 1) * (width + 1)] - row_off[b * (width + 1)];
-  const uint32_t n = v == 0 ? 1u : (v + item_visits - 1) / item_visits;
+  const uint32_t v = off[(b + 1) * stride] - off[b * stride];
+  const uint32_t n = v == 0 ? 1u : (v + size - 1) / size;
   first[b] = n;
   if (n > 1) multi[atomicAdd(counter, 1u)] = static_cast<uint32_t>(b);
 }
@@ -228,13 +281,13 @@ __global__ void slot_count_kernel(const uint32_t* first, uint64_t nblocks, uint3
   slots[b] = n > 1 ? n : 0u;
 }
 
-// Cost of an item: its visits plus a fixed cost for loading its header and writing
-// its cells.  cost[items] = 0 so the exclusive scan's last entry is the total.
+// Cost of an item: its visits / unit plus a fixed cost for loading its header and
+// writing its cells.  cost[items] = 0 so the exclusive scan's last entry is the total.
 constexpr uint32_t kItemFixedCost = 16;
 
-__global__ void item_cost_kernel(const uint4* info, uint32_t items, uint32_t* cost) {
+__global__ void item_cost_kernel(const uint4* info, uint32_t items, uint32_t unit, uint32_t fixed, uint32_t* cost) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < items) cost[i] = info[i].z - info[i].y + kItemFixedCost;
+  if (i < items) cost[i] = (info[i].z - info[i].y) / unit + fixed;
   if (i == items) cost[i] = 0;
 }
 
@@ -259,16 +312,17 @@ __global__ void xcd_split_kernel(const uint32_t* prefix, uint32_t items, uint32_
   first[x] = lo;
 }
 
-__global__ void item_fill_kernel(const uint32_t* first, const uint32_t* slot, const uint32_t* row_off, int width,
-                                 uint32_t item_visits, uint64_t nblocks, uint4* info) {
+// Item records {block, first visit, end visit, slot or ~0}; units of `scale` visits.
+__global__ void item_fill_kernel(const uint32_t* first, const uint32_t* slot, const uint32_t* off, int stride,
+                                 uint32_t size, uint32_t scale, uint64_t nblocks, uint4* info) {
   const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (b >= nblocks) return;
-  const uint32_t vb = row_off[b * (width + 1)], ve = row_off[(b + 1) * (width + 1)];
+  const uint32_t vb = off[b * stride], ve = off[(b + 1) * stride];
   const uint32_t n = first[b + 1] - first[b];
   for (uint32_t j = 0; j < n; ++j) {
-    const uint32_t v0 = vb + j * item_visits;
-    const uint32_t v1 = v0 + item_visits < ve ? v0 + item_visits : ve;
-    info[first[b] + j] = make_uint4(static_cast<uint32_t>(b), v0, v1, n > 1 ? slot[b] + j : 0xFFFFFFFFu);
+    const uint32_t v0 = vb + j * size;
+    const uint32_t v1 = v0 + size < ve ? v0 + size : ve;
+    info[first[b] + j] = make_uint4(static_cast<uint32_t>(b), scale * v0, scale * v1, n > 1 ? slot[b] + j : 0xFFFFFFFFu);
   }
 }
 
@@ -372,11 +426,13 @@ void reserve_items(Plan2* p, uint32_t items) {
   KSJ_HIP(hipMalloc(&p->item_scratch, std::max<size_t>(256, scan_scratch_bytes(p->item_capacity + 1ull))));
 }
 
-// Cost prefix of the items and the eight equal-cost XCD ranges.
+// Cost prefix of the items and the eight equal-cost XCD ranges.  The matrix
+// schedule counts groups of four visits plus one per item.
 void xcd_ranges(Plan2* p, hipStream_t stream) {
   const uint32_t items = p->item_count;
+  const bool matrix = p->scheme == kSchemeMatrix;
   hipLaunchKernelGGL(item_cost_kernel, dim3(ceil_div(items + 1ull, 256)), dim3(256), 0, stream, p->item_info, items,
-                     p->item_cost);
+                     matrix ? 4u : 1u, matrix ? 1u : kItemFixedCost, p->item_cost);
   KSJ_LAUNCH_CHECK();
   exclusive_scan_u32(p->item_cost, p->item_cost, items + 1ull, p->item_scratch, stream);
   hipLaunchKernelGGL(xcd_split_kernel, dim3(1), dim3(64), 0, stream, p->item_cost, items, p->xcd_first);
@@ -437,16 +493,22 @@ Plan2* plan2_create(const Plan2Desc& desc) {
   require(desc.centred == 1 || (desc.grid_rows == desc.image_rows && desc.grid_cols == desc.image_cols),
           "uncentred output requires an unoversampled grid");
   require(desc.samples >= 1, "plan needs at least one sample");
-  require(desc.item_visits >= 1 && desc.item_visits <= 4096, "item visits must be in [1, 4096]");
+  require(desc.item_visits <= 4096, "item visits must be in [1, 4096]");
   auto* p = new Plan2();
   p->d = desc;
   if (p->d.scheme == kSchemeDefault) p->d.scheme = static_cast<uint32_t>(tune_get(kTuneGatherScheme));
+  if (p->d.item_visits == 0)
+    p->d.item_visits =
+      static_cast<uint32_t>(tune_get(p->d.scheme == kSchemeMatrix ? kTuneMatrixItemVisits : kTuneItemVisits));
   if (p->d.item_samples == 0) p->d.item_samples = static_cast<uint32_t>(tune_get(kTuneTileItemSamples));
   if (p->d.tile_width == 0) p->d.tile_width = static_cast<uint32_t>(tune_get(kTuneTileWidth));
   if (p->d.block_group == 0) p->d.block_group = static_cast<uint32_t>(tune_get(kTuneBlockGroup));
+  if (p->d.run_cost == 0) p->d.run_cost = static_cast<uint32_t>(tune_get(kTuneMatrixRunCost));
+  require(p->d.run_cost >= 1 && p->d.run_cost <= 64, "run cost must be in [1, 64]");
   require(p->d.block_group == 1 || p->d.block_group == 2 || p->d.block_group == 4, "block group must be 1, 2 or 4");
   require(p->d.tile_width == 4 || p->d.tile_width == 8, "tile width must be 4 or 8");
-  require(p->d.scheme == kSchemeBlocks || p->d.scheme == kSchemeTiles, "unknown gather schedule");
+  require(p->d.scheme == kSchemeBlocks || p->d.scheme == kSchemeTiles || p->d.scheme == kSchemeMatrix,
+          "unknown gather schedule");
   if (p->d.kernel == kKernelKaiserBessel && p->d.beta <= 0.0) {
     const double sigma = static_cast<double>(desc.grid_rows) / desc.image_rows;
     const double a = static_cast<double>(desc.width) / sigma * (sigma - 0.5);
@@ -459,12 +521,15 @@ Plan2* plan2_create(const Plan2Desc& desc) {
     p->key_bits = 1;
     while ((1ull << p->key_bits) < p->nbins) ++p->key_bits;
     p->block_group = p->d.block_group;
-    p->block_rows = ceil_div(ceil_div(desc.grid_rows, 2), p->block_group) * p->block_group;
-    p->block_cols = ceil_div(ceil_div(desc.grid_cols, 2), p->block_group) * p->block_group;
+    const uint32_t side = block_side(p->d.scheme);
+    p->block_rows = ceil_div(ceil_div(desc.grid_rows, side), p->block_group) * p->block_group;
+    p->block_cols = ceil_div(ceil_div(desc.grid_cols, side), p->block_group) * p->block_group;
     p->nblocks = static_cast<uint64_t>(p->block_rows) * p->block_cols;
-    p->item_visits = desc.item_visits;
+    // The matrix gather takes visits four at a time: items start on multiples of 4.
+    p->item_visits = p->d.scheme == kSchemeMatrix ? (p->d.item_visits + 3) / 4 * 4 : p->d.item_visits;
     p->scheme = p->d.scheme;
     p->item_samples = p->d.item_samples;
+    p->run_cost = p->d.run_cost;
     p->tile_width = p->d.tile_width;
     p->tile_height = kTileCells / p->tile_width;
     p->tile_rows = ceil_div(desc.grid_rows, p->tile_height);
@@ -477,18 +542,20 @@ Plan2* plan2_create(const Plan2Desc& desc) {
     p->bins = alloc<uint32_t>(S, &total);
     p->cell_start = alloc<uint32_t>(p->nbins + 1, &total);
     p->taps = alloc<float>(S * 2 * desc.width, &total);
-    if (p->scheme == kSchemeBlocks)
-      p->visit_row_off = alloc<uint32_t>(p->nblocks * (desc.width + 1) + 1, &total);
+    const uint64_t window = desc.width + side - 1;  // window rows per block
+    if (p->scheme != kSchemeTiles)
+      p->visit_row_off = alloc<uint32_t>(p->nblocks * window + 1, &total);
     else
       p->tile_samples = alloc<uint32_t>(p->ntiles, &total);
     p->block_first_item = alloc<uint32_t>(p->nblocks + 1, &total);
     p->block_first_slot = alloc<uint32_t>(p->nblocks + 1, &total);
+    if (p->scheme == kSchemeMatrix) p->block_chunk_off = alloc<uint32_t>(p->nblocks + 1, &total);
     p->multi_blocks = alloc<uint32_t>(p->nblocks, &total);
     p->xcd_first = alloc<uint32_t>(kPlanXcds + 1, &total);
     p->frac = alloc<double>(2 * S, &total);
     p->sorted_keys = alloc<uint32_t>(S, &total);
     p->counters = alloc<uint32_t>(4, &total);
-    const uint64_t scan_n = std::max<uint64_t>(S, p->nblocks * (desc.width + 1) + 1);
+    const uint64_t scan_n = std::max<uint64_t>(S, p->nblocks * window + 1);
     p->sort_scratch = alloc<char>(std::max(radix_sort_scratch_bytes(scan_n), scan_scratch_bytes(scan_n)), &total);
     if (p->d.kernel == kKernelKaiserBessel) {
       p->deapod_rows = alloc<float>(desc.image_rows, &total);
@@ -516,7 +583,7 @@ void plan2_destroy(Plan2* p) {
                   p->taps,          p->visit_row_off, p->visit_k,    p->visit_w,      p->block_first_item,
                   p->block_first_slot, p->item_info,    p->multi_blocks, p->frac,         p->sorted_keys,
                   p->counters,      p->sort_scratch, p->deapod_rows, p->deapod_cols, p->item_cost,
-                  p->item_scratch,  p->xcd_first,   p->tile_samples};
+                  p->item_scratch,  p->xcd_first,   p->tile_samples, p->block_chunk_off, p->run_first};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -580,14 +647,30 @@ void plan2_set_trajectory(Plan2* p, const float* d_traj, const float* d_dcf, hip
   sa.block_group = p->block_group;
   sa.nblocks = p->nblocks;
   sa.width = static_cast<int>(p->d.width);
+  sa.side = static_cast<int>(block_side(p->scheme));
   sa.row_off = p->visit_row_off;
-  const uint64_t nrows = p->nblocks * (p->d.width + 1);
+  const int window = sa.width + sa.side - 1;
+  const uint64_t nrows = p->nblocks * window;
+  const uint64_t weight_vec4 = static_cast<uint64_t>(sa.side) * sa.side / 4;  // float4s of weights per visit
   hipLaunchKernelGGL(visit_count_kernel, dim3(ceil_div(nrows + 1, 256)), dim3(256), 0, stream, sa);
   KSJ_LAUNCH_CHECK();
   exclusive_scan_u32(p->visit_row_off, p->visit_row_off, nrows + 1, p->sort_scratch, stream);
+  const bool matrix = p->scheme == kSchemeMatrix;
   uint32_t visits = 0;
-  KSJ_HIP(hipMemcpyAsync(&visits, p->visit_row_off + nrows, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-  KSJ_HIP(hipStreamSynchronize(stream));
+  if (matrix) {  // lists padded to groups of four: visits = 4 * groups
+    hipLaunchKernelGGL(chunk_count_kernel, dim3(ceil_div(p->nblocks + 1, 256)), dim3(256), 0, stream,
+                       p->visit_row_off, p->nblocks, window, p->block_chunk_off);
+    KSJ_LAUNCH_CHECK();
+    exclusive_scan_u32(p->block_chunk_off, p->block_chunk_off, p->nblocks + 1, p->sort_scratch, stream);
+    uint32_t chunks = 0;
+    KSJ_HIP(hipMemcpyAsync(&chunks, p->block_chunk_off + p->nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    KSJ_HIP(hipStreamSynchronize(stream));
+    require(chunks <= 0x3FFFFFFFu, "too many gather visits for one plan");
+    visits = 4 * chunks;
+  } else {
+    KSJ_HIP(hipMemcpyAsync(&visits, p->visit_row_off + nrows, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    KSJ_HIP(hipStreamSynchronize(stream));
+  }
   if (visits > p->visit_capacity) {  // first trajectory on this plan, or a denser one
     if (p->visit_k) KSJ_HIP(hipFree(p->visit_k));
     if (p->visit_w) KSJ_HIP(hipFree(p->visit_w));
@@ -595,19 +678,27 @@ void plan2_set_trajectory(Plan2* p, const float* d_traj, const float* d_dcf, hip
     p->visit_w = nullptr;
     p->visit_capacity = visits;
     KSJ_HIP(hipMalloc(&p->visit_k, std::max<size_t>(1, visits) * sizeof(uint32_t)));
-    KSJ_HIP(hipMalloc(&p->visit_w, std::max<size_t>(1, visits) * sizeof(float4)));
+    KSJ_HIP(hipMalloc(&p->visit_w, std::max<size_t>(1, visits) * weight_vec4 * sizeof(float4)));
   }
   p->visit_count = visits;
   sa.visit_k = p->visit_k;
-  sa.visit_w = p->visit_w;
+  sa.visit_w = reinterpret_cast<float*>(p->visit_w);
+  sa.chunk_off = matrix ? p->block_chunk_off : nullptr;
   hipLaunchKernelGGL(visit_fill_kernel, dim3(ceil_div(nrows, 256)), dim3(256), 0, stream, sa);
   KSJ_LAUNCH_CHECK();
-
-  // Work items: ceil(visits / item_visits) per block (at least one, so every cell is written).
+  if (matrix) {
+    hipLaunchKernelGGL(pad_visits_kernel, dim3(ceil_div(p->nblocks, 256)), dim3(256), 0, stream, p->visit_row_off,
+                       p->block_chunk_off, p->nblocks, window, p->visit_k, sa.visit_w);
+    KSJ_LAUNCH_CHECK();
+  }
+  // Work items: ceil(visits / item_visits) per block (at least one, so every cell is
+  // written); the matrix schedule counts in groups of four.
+  const uint32_t* unit_off = matrix ? p->block_chunk_off : p->visit_row_off;
+  const int unit_stride = matrix ? 1 : window;
+  const uint32_t unit_size = matrix ? p->item_visits / 4 : p->item_visits;
   KSJ_HIP(hipMemsetAsync(p->counters, 0, sizeof(uint32_t), stream));
-  hipLaunchKernelGGL(item_count_kernel, dim3(ceil_div(p->nblocks + 1, 256)), dim3(256), 0, stream, p->visit_row_off,
-                     p->nblocks, static_cast<int>(p->d.width), p->item_visits, p->block_first_item, p->multi_blocks,
-                     p->counters);
+  hipLaunchKernelGGL(item_count_kernel, dim3(ceil_div(p->nblocks + 1, 256)), dim3(256), 0, stream, unit_off,
+                     p->nblocks, unit_stride, unit_size, p->block_first_item, p->multi_blocks, p->counters);
   KSJ_LAUNCH_CHECK();
   exclusive_scan_u32(p->block_first_item, p->block_first_item, p->nblocks + 1, p->sort_scratch, stream);
   uint32_t items = 0, multi = 0;
@@ -624,10 +715,26 @@ void plan2_set_trajectory(Plan2* p, const float* d_traj, const float* d_dcf, hip
   uint32_t slots = 0;
   KSJ_HIP(hipMemcpyAsync(&slots, p->block_first_slot + p->nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
   hipLaunchKernelGGL(item_fill_kernel, dim3(ceil_div(p->nblocks, 256)), dim3(256), 0, stream, p->block_first_item,
-                     p->block_first_slot, p->visit_row_off, static_cast<int>(p->d.width), p->item_visits, p->nblocks,
+                     p->block_first_slot, unit_off, unit_stride, unit_size, matrix ? 4u : 1u, p->nblocks,
                      p->item_info);
   KSJ_LAUNCH_CHECK();
   xcd_ranges(p, stream);
+  if (matrix) {  // wave runs over the cost prefix
+    uint32_t total = 0;
+    KSJ_HIP(hipMemcpyAsync(&total, p->item_cost + items, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    KSJ_HIP(hipStreamSynchronize(stream));
+    const uint32_t runs = ceil_div(total, p->run_cost);
+    if (runs > p->run_capacity || p->run_first == nullptr) {
+      if (p->run_first) KSJ_HIP(hipFree(p->run_first));
+      p->run_first = nullptr;
+      p->run_capacity = std::max(runs, p->run_capacity);
+      KSJ_HIP(hipMalloc(&p->run_first, (static_cast<size_t>(p->run_capacity) + 1) * sizeof(uint32_t)));
+    }
+    p->run_count = runs;
+    hipLaunchKernelGGL(run_split_kernel, dim3(ceil_div(runs + 1ull, 256)), dim3(256), 0, stream, p->item_cost, items,
+                       runs, p->run_cost, p->run_first);
+    KSJ_LAUNCH_CHECK();
+  }
   KSJ_HIP(hipStreamSynchronize(stream));
   p->slot_count = slots;
   // The multi-item list is in append order; each of its blocks is summed on its

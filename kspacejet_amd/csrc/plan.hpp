@@ -11,10 +11,10 @@
 //                s = (centred ? N/2 : 0) - origin, evaluated in FP64.
 //   taps         per sorted sample W row taps and W column taps of the kernel
 //                (Kaiser-Bessel or linear), evaluated in FP64, stored as float.
-//   schedule     either (blocks) per 2x2 cell block the list of (sample, four
-//                weights) visits, or (tiles) per 8x8 cell tile the length of its
-//                window sample list; both cut into bounded work items so dense
-//                k-space centres are split over many workgroups.
+//   schedule     either (blocks, matrix) per 2x2 or 4x4 cell block the list of
+//                (sample, cell weights) visits, or (tiles) per 64-cell tile the
+//                length of its window sample list; all cut into bounded work items
+//                so dense k-space centres are split over many workgroups.
 //   deapod       1 / phi_hat per output row / column (Kaiser-Bessel only), FP64.
 #pragma once
 
@@ -26,11 +26,14 @@ enum : uint32_t { kKernelLinear = 0, kKernelKaiserBessel = 1 };
 // XCD count of the MI355X: the gather can give each XCD a contiguous, cost-balanced item range.
 constexpr uint32_t kPlanXcds = 8;
 enum : uint32_t { kDensityNone = 0, kDensityRamp = 1, kDensityGiven = 2 };
-// Gather schedules: 2x2 blocks with per-visit weights read from HBM, or 64-cell
-// tiles (64/w rows x w columns) whose window samples are staged in LDS.
-// kSchemeDefault takes tuning key gather_scheme.
-enum : uint32_t { kSchemeBlocks = 0, kSchemeTiles = 1, kSchemeDefault = 0xFFFFFFFFu };
+// Gather schedules: 2x2 blocks with per-visit weights read from HBM, 64-cell
+// tiles (64/w rows x w columns) whose window samples are staged in LDS, or 4x4
+// blocks whose visits carry all sixteen cell weights and are summed on the matrix
+// cores.  kSchemeDefault takes tuning key gather_scheme.
+enum : uint32_t { kSchemeBlocks = 0, kSchemeTiles = 1, kSchemeMatrix = 2, kSchemeDefault = 0xFFFFFFFFu };
 constexpr uint32_t kTileCells = 64;
+// Cells per side of a schedule block: 2 (kSchemeBlocks) or 4 (kSchemeMatrix).
+inline uint32_t block_side(uint32_t scheme) { return scheme == kSchemeMatrix ? 4u : 2u; }
 
 // Block numbering: blocks come in groups of g x g (row-major inside a group,
 // groups row-major), so consecutive work items -- which one workgroup takes
@@ -56,11 +59,12 @@ struct Plan2Desc {
   uint32_t density = kDensityRamp;
   uint32_t centred = 1;                     // 1: image pixel r reads bin (r - N/2) mod G; 0: bin r
   uint32_t samples = 0;
-  uint32_t item_visits = 64;                // blocks: visits per gather work item (one wave)
+  uint32_t item_visits = 64;                // blocks, matrix: visits per gather work item (0: tuning key)
   uint32_t scheme = kSchemeDefault;
   uint32_t item_samples = 0;                // tiles: window samples per work item (0: tuning key)
   uint32_t tile_width = 0;                  // tiles: 4 or 8 columns (0: tuning key)
   uint32_t block_group = 0;                 // blocks: group side 1, 2 or 4 (0: tuning key)
+  uint32_t run_cost = 0;                    // matrix: cost per wave run, 1..64 (0: tuning key)
 };
 
 struct Plan2 {
@@ -75,18 +79,19 @@ struct Plan2 {
   uint32_t* bins = nullptr;        // [S] sample bins (original order), then sorted keys
   uint32_t* cell_start = nullptr;  // [nbins + 1]
   float* taps = nullptr;           // [S][2W] sorted: W row taps then W column taps
-  // Gather schedule.  Cells are taken in 2x2 blocks; a block's visits are the
-  // samples of its (W+1) x (W+1) contributing bins (rows then columns of that bin
-  // window, sorted order inside a bin), each with the four weights
-  // phi(row tap) * phi(column tap) of the block's cells (0 outside the footprint).
-  // Blocks are cut into work items of at most item_visits visits; a block with
-  // several items is summed item by item in item order.
+  // Gather schedule.  Cells are taken in s x s blocks (s = 2, or 4 for the matrix
+  // schedule); a block's visits are the samples of its (W+s-1) x (W+s-1)
+  // contributing bins (rows then columns of that bin window, sorted order inside a
+  // bin), each with the s*s weights phi(row tap) * phi(column tap) of the block's
+  // cells in row-major order (0 outside the footprint).  Blocks are cut into work
+  // items of at most item_visits visits (a multiple of 4 for the matrix schedule);
+  // a block with several items is summed item by item in item order.
   uint32_t block_rows = 0, block_cols = 0, block_group = 1;  // rows / columns padded to whole groups
   uint64_t nblocks = 0;
   uint32_t item_visits = 64;
-  uint32_t* visit_row_off = nullptr;  // [nblocks * (W+1) + 1] first visit of each (block, window row)
+  uint32_t* visit_row_off = nullptr;  // [nblocks * (W+s-1) + 1] first visit of each (block, window row)
   uint32_t* visit_k = nullptr;        // [visit_capacity] sorted sample position
-  float4* visit_w = nullptr;          // [visit_capacity] weights of cells (0,0) (0,1) (1,0) (1,1)
+  float4* visit_w = nullptr;          // [visit_capacity * s*s/4] cell weights, row-major per visit
   uint64_t visit_capacity = 0;
   uint64_t visit_count = 0;           // host copy
   uint32_t* block_first_item = nullptr;  // [nblocks + 1]
@@ -100,6 +105,16 @@ struct Plan2 {
   uint32_t* xcd_first = nullptr;         // [kPlanXcds + 1] first item of each XCD's contiguous range
   uint32_t* multi_blocks = nullptr;      // [nblocks] blocks with more than one item
   uint32_t multi_count = 0;              // host copy
+  // Matrix schedule (scheme == kSchemeMatrix).  Every block's visit list is padded
+  // to a multiple of four (padding visits repeat the block's first sample with zero
+  // weights), so items start on groups of four and the items of consecutive blocks
+  // are contiguous.  Items cost (visits / 4 + 1); wave run r takes the items whose
+  // cost prefix lies in [r run_cost, (r+1) run_cost), at most 64 of them.
+  uint32_t* block_chunk_off = nullptr;   // [nblocks + 1] first group of four of each block
+  uint32_t run_cost = 32;
+  uint32_t* run_first = nullptr;         // [run_capacity + 1] first item of each run
+  uint32_t run_capacity = 0;
+  uint32_t run_count = 0;                // host copy
   // Tile schedule (scheme == kSchemeTiles).  Tile (ty, tx) covers cells
   // [h ty, h ty + h) x [w tx, w tx + w) with w = tile_width, h = 64 / w; its window
   // is the (h+W-1) x (w+W-1) bins whose samples reach those cells, rows then

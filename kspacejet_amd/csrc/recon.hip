@@ -317,7 +317,7 @@ namespace {
 bool fused_eligible(const MultiItem* items, uint32_t count) {
   if (tune_get(kTuneMultiFused) == 0 || count > kMaxFusedItems) return false;
   for (uint32_t i = 0; i < count; ++i)
-    if (items[i].plan->scheme != kSchemeBlocks || items[i].channels > kMaxStackedCoils) return false;
+    if (items[i].plan->scheme == kSchemeTiles || items[i].channels > kMaxStackedCoils) return false;
   return true;
 }
 
@@ -420,7 +420,7 @@ size_t adjoint2_multi_workspace_bytes(const MultiItem* items, uint32_t count) {
   if (count <= kMaxFusedItems) {
     bool blocks = true;
     for (uint32_t i = 0; i < count; ++i)
-      blocks = blocks && items[i].plan->scheme == kSchemeBlocks && items[i].channels <= kMaxStackedCoils;
+      blocks = blocks && items[i].plan->scheme != kSchemeTiles && items[i].channels <= kMaxStackedCoils;
     if (blocks) total = std::max(total, fused_lanes_bytes(items, count));
   }
   return total;

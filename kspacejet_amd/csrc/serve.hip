@@ -271,7 +271,7 @@ Plan2* Server::plan_for(const Pending& p, hipStream_t stream) {
   d.density = r.density;
   d.centred = 1;
   d.samples = r.samples;
-  d.item_visits = static_cast<uint32_t>(tune_get(kTuneItemVisits));
+  d.item_visits = 0;  // the schedule's tuning key
   PlanPtr plan(plan2_create(d), &plan2_destroy);
   plan2_set_trajectory(plan.get(), r.d_traj, nullptr, stream);
   PlanEntry e;

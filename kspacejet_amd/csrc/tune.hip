@@ -24,12 +24,18 @@ Param g_params[kTuneKeys] = {
     {"pass_lanes", 2, [](int64_t v) { return v >= 1 && v <= 3; }},
     {"multi_lanes", 4, [](int64_t v) { return v >= 1 && v <= 8; }},
     {"gather_coils_per_lane", 1, [](int64_t v) { return v == 1 || v == 2; }},
-    {"gather_scheme", 0, [](int64_t v) { return v == 0 || v == 1; }},
+    {"gather_scheme", 2, [](int64_t v) { return v >= 0 && v <= 2; }},
     {"tile_item_samples", 256, [](int64_t v) { return v >= 1 && v <= 65536; }},
     {"tile_width", 4, [](int64_t v) { return v == 4 || v == 8; }},
     {"block_group", 2, [](int64_t v) { return v == 1 || v == 2 || v == 4; }},
     {"fft_schedule", 0, [](int64_t v) { return v >= 0 && v <= 2; }},
     {"multi_fused", 0, [](int64_t v) { return v == 0 || v == 1; }},
+    {"matrix_run_cost", 8, [](int64_t v) { return v >= 1 && v <= 64; }},
+    {"matrix_pipeline", 1, [](int64_t v) { return v == 1 || v == 2; }},
+    {"matrix_nt_store", 1, [](int64_t v) { return v == 0 || v == 1; }},
+    {"matrix_slab", 16, [](int64_t v) { return v >= 1 && v <= 4096; }},
+    {"matrix_item_visits", 128, [](int64_t v) { return v >= 1 && v <= 4096; }},
+    {"matrix_batch", 16, [](int64_t v) { return v == 4 || v == 8 || v == 16; }},
 };
 
 }  // namespace

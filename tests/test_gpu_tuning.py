@@ -1,7 +1,8 @@
 """Launch parameters the autotuner may set change speed, not results: every
-legal setting of gather_map, gather_batch and fft_lines_per_block reproduces the
-default result bit for bit (item_visits changes the float summation order and is
-checked against the oracle instead)."""
+legal setting of the gather mapping, batch, grouping and pipeline keys and of
+fft_lines_per_block reproduces the default result bit for bit, under both the
+2x2-block and the matrix gather schedule (the work-item size changes the float
+summation order and is checked against the oracle instead)."""
 import numpy as np
 import pytest
 
@@ -22,9 +23,10 @@ def ops():
     return K
 
 
-@pytest.fixture
-def tune(ops):
-    """Sets tuning keys for one test and restores the previous values afterwards."""
+@pytest.fixture(params=[0, 2], ids=["blocks", "matrix"])
+def tune(ops, request):
+    """Sets tuning keys for one test (gather schedule from the parameter) and restores
+    the previous values afterwards."""
     saved = {}
 
     def set_(key, value):
@@ -32,6 +34,7 @@ def tune(ops):
             saved[key] = ops.tune_get(key)
         ops.tune_set(key, value)
 
+    set_("gather_scheme", request.param)
     yield set_
     for key, value in saved.items():
         ops.tune_set(key, value)
@@ -56,7 +59,9 @@ def test_launch_parameters_do_not_change_results(ops, tune):
     traj, k = scan(ops)
     base_img, base_rss = recon(ops, traj, k, n)
     for key, values in (("gather_map", (0, 1, 2)), ("gather_batch", (4, 8, 16)), ("fft_lines_per_block", (0, 4, 16)),
-                        ("gather_coils_per_lane", (1, 2)), ("block_group", (1, 2, 4))):
+                        ("gather_coils_per_lane", (1, 2)), ("block_group", (1, 2, 4)), ("matrix_batch", (16, 4, 8)),
+                        ("matrix_slab", (16, 1, 5)), ("matrix_pipeline", (1, 2)), ("matrix_nt_store", (1, 0)),
+                        ("matrix_run_cost", (8, 1, 64))):
         for v in values:
             tune(key, v)
             img, rss = recon(ops, traj, k, n)
@@ -101,6 +106,7 @@ def test_item_size_keeps_oracle_parity(ops, tune, item_visits):
     n = 64
     traj, k = scan(ops, n=n, coils=4, spokes=41, readout=128, seed=5)
     tune("item_visits", item_visits)
+    tune("matrix_item_visits", item_visits)
     img, _ = recon(ops, traj, k, n)
     t = traj.cpu().numpy().astype(np.float64)
     ref = O.kb_adjoint2_coils(k, t, O.ramp_dcf2(t), n)

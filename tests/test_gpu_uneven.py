@@ -27,8 +27,11 @@ def ops():
     return K
 
 
-@pytest.fixture(scope="module")
-def batch(ops):
+@pytest.fixture(scope="module", params=[2, 0], ids=["matrix", "blocks"])
+def batch(ops, request):
+    """The items' plans, built with the gather schedule of the parameter."""
+    saved = ops.tune_get("gather_scheme")
+    ops.tune_set("gather_scheme", request.param)
     plans, ks, trajs = [], [], []
     for i, (n, c, spokes) in enumerate(ITEMS):
         traj = ops.golden_angle2(spokes, 2 * n)
@@ -38,6 +41,7 @@ def batch(ops):
         plans.append(ops.Plan2(n, n, traj.shape[0]).set_trajectory(traj))
         ks.append(k)
         trajs.append(traj)
+    ops.tune_set("gather_scheme", saved)
     return plans, ks, trajs
 
 
