@@ -151,7 +151,7 @@ def main(argv=None) -> int:
     ap.add_argument("--uneven-items", type=int, default=64, help="items of the uneven batch; 0 skips that section")
     ap.add_argument("--volume-coils", type=int, default=20,
                     help="coils of the 3-D kooshball volume (256^3, 16384 x 512 spokes); 0 skips that section")
-    ap.add_argument("--volume-coils-per-pass", type=int, default=8)
+    ap.add_argument("--volume-coils-per-pass", type=int, default=16)
     ap.add_argument("--provider-firings", type=int, default=40,
                     help="firings of the drop-in Provider operator (host-fed plain form); 0 skips it")
     ap.add_argument("--tune-seconds", type=float, default=20.0, help="autotuner budget for the uneven batch")

@@ -122,7 +122,8 @@ typedef struct ksj_gpu_plan2_desc {
   uint32_t density;                /* KSJ_GPU_DENSITY_* */
   uint32_t centred;                /* 1: crop the centre of the oversampled grid; 0: unshifted (linear route) */
   uint32_t samples;
-  uint32_t item_visits;            /* gather work-item size in visits, 0 = default (64) */
+  uint32_t item_visits;            /* gather work-item size in visits, 0 = the schedule's tuning key
+                                      ("matrix_item_visits" or "item_visits") */
   uint32_t reserved0;
   double beta;                     /* <= 0: Beatty et al. 2005 value for the oversampling */
   double row_origin, col_origin;   /* image origin; the reference uses ((R-1)/2, (C-1)/2) */
@@ -225,7 +226,8 @@ typedef struct ksj_gpu_plan3_desc {
   uint32_t kernel_width;         /* 2, 4, 6 or 8 (Kaiser-Bessel) */
   uint32_t density;              /* KSJ_GPU_DENSITY_NONE, _RAMP_SQUARED or _GIVEN */
   uint32_t samples;
-  uint32_t item_visits;          /* gather work-item size in visits, 0 = default (64) */
+  uint32_t item_visits;          /* gather work-item size in visits, 0 = the schedule's tuning key
+                                    ("matrix3_item_visits" or "item_visits") */
   uint32_t reserved0;
   double beta;                   /* <= 0: Beatty et al. 2005 value for the oversampling */
   double origin_z, origin_y, origin_x;
@@ -367,11 +369,22 @@ void ksj_gpu_serve_destroy(ksj_gpu_server* server);
  *                          cores (new plans; summation order)
  *   "matrix_run_cost"      1..64: work per gather wave of the matrix schedule, in groups of four
  *                          visits plus one per item (new plans)
+ *   "matrix_item_visits"   1..4096: work-item size of new matrix-schedule plans (summation order)
+ *   "matrix_batch"         4, 8 or 16 visits per batch of the matrix gathers
+ *   "matrix_pipeline"      1 or 2 batches of sample rows in flight per wave (2-D matrix gather)
+ *   "matrix_nt_store"      0 or 1: grid cells of the 2-D matrix gather written non-temporally
+ *   "matrix_slab"          1..4096 consecutive workgroups per XCD slab (2-D matrix gather)
+ *   "matrix_block_width"   4 or 8: 4x4 or 4x8 cell blocks of new 2-D matrix plans (summation order)
+ *   "gather3_scheme"       0 = 2x2x2 blocks, 2 = 4x4x4 blocks on the matrix cores (new 3-D plans;
+ *                          summation order)
+ *   "matrix3_item_visits"  1..4096: work-item size of new 3-D matrix plans (summation order)
+ *   "matrix3_run_cost"     1..64: work per gather wave of new 3-D matrix plans
  *   "tile_width"           4 or 8 columns per tile (new plans; summation order)
  *   "tile_item_samples"    window samples per tile work item (new plans; summation order)
- * Every setting gives bit-identical results, except item_visits, the tile keys and
- * gather_scheme (float summation order of the gather) and fft_schedule (FFT
- * rounding); each of those is tested against the FP64 oracle.  Not thread-safe
+ * Every setting gives bit-identical results, except item_visits, matrix_item_visits,
+ * matrix3_item_visits, matrix_block_width, the tile keys, gather_scheme and gather3_scheme (float summation
+ * order of the gather) and fft_schedule (FFT rounding); each of those is tested against
+ * the FP64 oracle.  Not thread-safe
  * against concurrent launches. */
 int ksj_gpu_tune_set(const char* key, int64_t value);
 int ksj_gpu_tune_get(const char* key, int64_t* value);

@@ -203,9 +203,9 @@ int ksj_gpu_plan2_get_info(const ksj_gpu_plan2* plan, ksj_gpu_plan2_info* out_in
     ksj::gpu::require(out_info->struct_size >= sizeof(ksj_gpu_plan2_info), "info struct too small");
     out_info->items = plan->impl->item_count;
     out_info->visits = plan->impl->visit_count;
-    const uint64_t side = ksj::gpu::block_side(plan->impl->scheme);
+    const uint64_t cells = static_cast<uint64_t>(plan->impl->block_h) * plan->impl->block_w;
     out_info->device_bytes = plan->impl->device_bytes +
-                             plan->impl->visit_capacity * (sizeof(uint32_t) + side * side * sizeof(float)) +
+                             plan->impl->visit_capacity * (sizeof(uint32_t) + cells * sizeof(float)) +
                              static_cast<uint64_t>(plan->impl->item_capacity) * (sizeof(uint4) + sizeof(uint32_t));
     out_info->beta = plan->impl->d.beta;
   });
@@ -263,7 +263,7 @@ int ksj_gpu_plan3_create(const ksj_gpu_plan3_desc* desc, ksj_gpu_plan3** out_pla
     d.width = desc->kernel_width;
     d.density = desc->density;
     d.samples = desc->samples;
-    d.item_visits = desc->item_visits == 0 ? 64u : desc->item_visits;
+    d.item_visits = desc->item_visits;  // 0: the schedule's tuning key
     d.beta = desc->beta;
     d.origin[0] = desc->origin_z;
     d.origin[1] = desc->origin_y;
@@ -293,8 +293,13 @@ int ksj_gpu_plan3_get_info(const ksj_gpu_plan3* plan, ksj_gpu_plan3_info* out_in
     const auto* p = plan->impl;
     out_info->items = p->item_count;
     out_info->visits = p->visit_count;
-    out_info->device_bytes = p->device_bytes + p->visit_capacity * (sizeof(uint32_t) + 2 * sizeof(float4)) +
-                             static_cast<uint64_t>(p->item_capacity) * sizeof(uint4);
+    const uint64_t per_visit = p->scheme == ksj::gpu::kSchemeMatrix ? 2 * sizeof(uint32_t)
+                                                                    : sizeof(uint32_t) + 2 * sizeof(float4);
+    const uint64_t run_tables = p->scheme == ksj::gpu::kSchemeMatrix
+                                  ? (static_cast<uint64_t>(p->item_capacity) + p->run_capacity + 2) * sizeof(uint32_t)
+                                  : 0;
+    out_info->device_bytes = p->device_bytes + p->visit_capacity * per_visit +
+                             static_cast<uint64_t>(p->item_capacity) * sizeof(uint4) + run_tables;
     out_info->beta = p->d.beta;
   });
 }

@@ -67,6 +67,10 @@ enum TuneKey : int {
   kTuneMatrixSlab,        // matrix gather, gather_map 0: consecutive workgroups per XCD slab (1 = launch order)
   kTuneMatrixItemVisits,  // matrix schedule: work-item size of new plans (visits; summation order changes)
   kTuneMatrixBatch,       // matrix gather: visits per batch, 4, 8 or 16
+  kTuneMatrixBlockWidth,  // matrix schedule of new plans: 4 x 4 or 4 x 8 cell blocks (summation order changes)
+  kTuneGather3Scheme,     // 3-D schedule of new plans: 0 = 2x2x2 blocks, 2 = 4x4x4 on the matrix cores
+  kTuneMatrix3ItemVisits, // 3-D matrix schedule: work-item size of new plans (visits; summation order changes)
+  kTuneMatrix3RunCost,    // 3-D matrix schedule of new plans: cost per wave run, 1..64
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);

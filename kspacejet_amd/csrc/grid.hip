@@ -15,6 +15,7 @@
 #include "grid.hpp"
 
 #include "fused.hpp"
+#include "matrix.hpp"
 
 #include <utility>
 
@@ -104,7 +105,7 @@ struct GatherArgs {
   const uint32_t* xcd_first;        // [9] cost-balanced XCD item ranges (gather_map 2)
   uint32_t xcd_chunk, wg_per_xcd;   // workgroup -> item range mapping (gather_map)
   uint32_t grid_rows, grid_cols, block_cols, block_group, cp, item_count, multi_count;
-  uint32_t side;                    // block side: 2, or 4 for the matrix schedule
+  uint32_t bh, bw;                  // block cells per row / column: 2 x 2, or 4 x 4 / 4 x 8 (matrix)
   const uint32_t* run_first;        // matrix schedule: [runs + 1] first item of each wave run
   uint32_t run_count;
   uint32_t nt_store;                // matrix schedule: grid cells written with non-temporal stores
@@ -314,7 +315,7 @@ __global__ __launch_bounds__(256) void gather_item_kernel(GatherArgs a) {
 
 // Blocks with several items: thread per (block, cell, coil) sums the item partials in item order.
 __device__ inline void multi_reduce_one(const GatherArgs& a, uint64_t i) {
-  const uint32_t cells = a.side * a.side;
+  const uint32_t cells = a.bh * a.bw;
   if (i >= static_cast<uint64_t>(a.multi_count) * cells * a.cp) return;
   const uint32_t c = static_cast<uint32_t>(i % a.cp);
   const uint32_t q = static_cast<uint32_t>((i / a.cp) % cells);
@@ -335,7 +336,7 @@ __device__ inline void multi_reduce_one(const GatherArgs& a, uint64_t i) {
   for (; j < n; ++j) s = cadd(s, p[j * stride]);
   uint32_t by, bx;
   block_coords(b, a.block_cols, a.block_group, &by, &bx);
-  const uint32_t y = a.side * by + q / a.side, x = a.side * bx + q % a.side;
+  const uint32_t y = a.bh * by + q / a.bw, x = a.bw * bx + q % a.bw;
   if (y < a.grid_rows && x < a.grid_cols) a.grid[(static_cast<uint64_t>(y) * a.grid_cols + x) * a.cp + c] = s;
 }
 
@@ -344,63 +345,21 @@ __global__ __launch_bounds__(256) void multi_reduce_kernel(GatherArgs a) {
 }
 
 // ----------------------------------------------------------------------------- matrix gather
-// The 4x4-block schedule as small dense products on the matrix cores: for four
-// visits at a time, the block's 16 cells x 4 visits weight matrix times the 4 visits
-// x 2 Cp floats of their sample rows (real weights act on real and imaginary parts
-// alike), accumulated by v_mfma_f32_16x16x4_f32 in FP32.  One wave per work item.
+// The 4x4 (or 4x8) block schedule as small dense products on the matrix cores: for
+// four visits at a time, each 16-cell tile's 16 cells x 4 visits weight matrix times
+// the 4 visits x 2 Cp floats of their sample rows (real weights act on real and
+// imaginary parts alike), accumulated by v_mfma_f32_16x16x4_f32 in FP32.
 //
 // Operand layout of v_mfma_f32_16x16x4_f32 (wave64, one block): lane l holds
 // A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator register r of lane l is
 // D[4 (l / 16) + r][l % 16].  So lane l takes visit q = l / 16 of the four and
-//  * A: the weight of cell l % 16 (the visit's 16 weights are 64 contiguous bytes);
+//  * A: the weight of cell l % 16 of the tile (a visit's 16 T weights are contiguous);
 //  * B: NF = Cp / 8 consecutive floats of the visit's row at column j = l % 16, one
 //    per product: product n sums float NF j + n of every row;
-//  * D: cells 4q .. 4q+3 (block row q) at floats NF j .. NF j + NF - 1, written as
-//    one vector per cell.
+//  * D: tile cells 4q .. 4q+3 at floats NF j .. NF j + NF - 1, written as one vector
+//    per cell.
 // Block visit lists are padded to whole groups of four with zero-weight visits.
 // Sums run four visits at a time in visit order, so results are deterministic.
-typedef float MfmaAcc __attribute__((ext_vector_type(4)));
-
-template <int NF>
-struct RowPiece {
-  float v[NF];
-};
-
-template <int NF>
-__device__ inline RowPiece<NF> load_piece(const float* p) {
-  RowPiece<NF> r;
-  if constexpr (NF == 1) {
-    r.v[0] = *p;
-  } else if constexpr (NF == 2) {
-    const float2 t = *reinterpret_cast<const float2*>(p);
-    r.v[0] = t.x;
-    r.v[1] = t.y;
-  } else {
-#pragma unroll
-    for (int h = 0; h < NF / 4; ++h) {
-      const float4 t = reinterpret_cast<const float4*>(p)[h];
-      r.v[4 * h] = t.x;
-      r.v[4 * h + 1] = t.y;
-      r.v[4 * h + 2] = t.z;
-      r.v[4 * h + 3] = t.w;
-    }
-  }
-  return r;
-}
-
-template <int NF>
-__device__ inline void store_piece(float* p, const float (&v)[NF]) {
-  if constexpr (NF == 1) {
-    *p = v[0];
-  } else if constexpr (NF == 2) {
-    *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
-  } else {
-#pragma unroll
-    for (int h = 0; h < NF / 4; ++h)
-      reinterpret_cast<float4*>(p)[h] = make_float4(v[4 * h], v[4 * h + 1], v[4 * h + 2], v[4 * h + 3]);
-  }
-}
-
 // One wave per run of consecutive items (at most 64): the items' groups of four are
 // contiguous, so the wave streams through them in batches of U groups with the
 // next batch's metadata (positions and weights) loaded right after this batch's
@@ -409,26 +368,10 @@ __device__ inline void store_piece(float* p, const float (&v)[NF]) {
 // cut into several items) and its accumulators cleared; items without visits
 // (blocks no sample reaches) write zeros.  Lane t holds item t's record; the wave
 // reads it with readlane.
-template <int NF>
-__device__ inline void store_piece_nt(float* p, const float (&v)[NF]) {
-  if constexpr (NF == 1) {
-    __builtin_nontemporal_store(v[0], p);
-  } else if constexpr (NF == 2) {
-    __builtin_nontemporal_store(v[0], p);
-    __builtin_nontemporal_store(v[1], p + 1);
-  } else {
-#pragma unroll
-    for (int h = 0; h < NF / 4; ++h) {
-      typedef float F4 __attribute__((ext_vector_type(4)));
-      const F4 t = {v[4 * h], v[4 * h + 1], v[4 * h + 2], v[4 * h + 3]};
-      __builtin_nontemporal_store(t, reinterpret_cast<F4*>(p) + h);
-    }
-  }
-}
-
-template <int CP, int U, int P>
+template <int CP, int U, int P, int T>
 __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
   constexpr int NF = CP / 8;
+  constexpr int kCells = 16 * T;  // 4 x 4 (T = 1) or 4 x 8 (T = 2) cells, T products of 16 cells each
   const int lane = threadIdx.x & 63, q = lane >> 4, j = lane & 15;
   const uint32_t i0 = a.run_first[run], i1 = a.run_first[run + 1];  // run is wave-uniform: scalar loads
   if (i0 >= i1) return;
@@ -441,44 +384,57 @@ __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
   const uint32_t last_visit = c1 > c0 ? 4 * c1 - 1 : 0;
   const float* wts = reinterpret_cast<const float*>(a.visit_w);
   const float* rows = reinterpret_cast<const float*>(a.sorted) + j * NF;
-  MfmaAcc acc[NF];
+  MfmaAcc acc[T][NF];
 #pragma unroll
-  for (int n = 0; n < NF; ++n) acc[n] = MfmaAcc{0.f, 0.f, 0.f, 0.f};
+  for (int h = 0; h < T; ++h)
+#pragma unroll
+    for (int n = 0; n < NF; ++n) acc[h][n] = MfmaAcc{0.f, 0.f, 0.f, 0.f};
 
+  // Cell 16 h + 4 q + r of the block (product h, lane group q, register r) is block
+  // row (16 h + 4 q + r) / bw, column (16 h + 4 q + r) % bw.
   auto flush = [&](uint32_t t) {
     const uint32_t blk = field(rec.x, t), slot = field(rec.w, t);
     uint32_t by, bx;
     block_coords(blk, a.block_cols, a.block_group, &by, &bx);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      float o[NF];
+    for (int h = 0; h < T; ++h) {
 #pragma unroll
-      for (int n = 0; n < NF; ++n) o[n] = acc[n][r];
-      if (slot == 0xFFFFFFFFu) {
-        const uint32_t y = 4 * by + q, x = 4 * bx + r;
-        if (y < a.grid_rows && x < a.grid_cols) {
-          float* dst = reinterpret_cast<float*>(a.grid) + (static_cast<uint64_t>(y) * a.grid_cols + x) * (2 * CP) + j * NF;
-          if (a.nt_store)
-            store_piece_nt<NF>(dst, o);
-          else
-            store_piece<NF>(dst, o);
+      for (int r = 0; r < 4; ++r) {
+        float o[NF];
+#pragma unroll
+        for (int n = 0; n < NF; ++n) o[n] = acc[h][n][r];
+        const uint32_t cell = 16 * h + 4 * q + r;
+        if (slot == 0xFFFFFFFFu) {
+          const uint32_t y = 4 * by + cell / (4 * T), x = 4 * T * bx + cell % (4 * T);
+          if (y < a.grid_rows && x < a.grid_cols) {
+            float* dst =
+              reinterpret_cast<float*>(a.grid) + (static_cast<uint64_t>(y) * a.grid_cols + x) * (2 * CP) + j * NF;
+            if (a.nt_store)
+              store_piece_nt<NF>(dst, o);
+            else
+              store_piece<NF>(dst, o);
+          }
+        } else {
+          store_piece<NF>(
+            reinterpret_cast<float*>(a.partial) + (static_cast<uint64_t>(slot) * kCells + cell) * (2 * CP) + j * NF,
+            o);
         }
-      } else {
-        store_piece<NF>(
-          reinterpret_cast<float*>(a.partial) + (static_cast<uint64_t>(slot) * 16 + 4 * q + r) * (2 * CP) + j * NF, o);
       }
     }
 #pragma unroll
-    for (int n = 0; n < NF; ++n) acc[n] = MfmaAcc{0.f, 0.f, 0.f, 0.f};
+    for (int h = 0; h < T; ++h)
+#pragma unroll
+      for (int n = 0; n < NF; ++n) acc[h][n] = MfmaAcc{0.f, 0.f, 0.f, 0.f};
   };
 
-  auto load_meta = [&](uint32_t c, uint32_t (&k)[U], float (&w)[U]) {  // groups c + u, clamped into the run
+  auto load_meta = [&](uint32_t c, uint32_t (&k)[U], float (&w)[U][T]) {  // groups c + u, clamped into the run
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t vi = 4 * (c + u) + q;
       const uint32_t vc = vi < last_visit ? vi : last_visit;
       k[u] = a.visit_k[vc];
-      w[u] = wts[static_cast<uint64_t>(vc) * 16 + j];
+#pragma unroll
+      for (int h = 0; h < T; ++h) w[u][h] = wts[static_cast<uint64_t>(vc) * kCells + 16 * h + j];
     }
   };
   auto load_rows = [&](const uint32_t (&k)[U], RowPiece<NF> (&b)[U]) {
@@ -487,7 +443,7 @@ __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
   };
 
   uint32_t t = 0, t_end = field(rec.z, 0) / 4;  // current item and its end group
-  auto compute = [&](uint32_t c, const RowPiece<NF> (&b)[U], const float (&wc)[U]) {
+  auto compute = [&](uint32_t c, const RowPiece<NF> (&b)[U], const float (&wc)[U][T]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (c + u < c1) {
@@ -497,23 +453,31 @@ __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
           t_end = field(rec.z, t) / 4;
         }
 #pragma unroll
-        for (int n = 0; n < NF; ++n) acc[n] = __builtin_amdgcn_mfma_f32_16x16x4f32(wc[u], b[u].v[n], acc[n], 0, 0, 0);
+        for (int h = 0; h < T; ++h)
+#pragma unroll
+          for (int n = 0; n < NF; ++n)
+            acc[h][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(wc[u][h], b[u].v[n], acc[h][n], 0, 0, 0);
       }
     }
+  };
+  auto copy_w = [](float (&dst)[U][T], const float (&src)[U][T]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int h = 0; h < T; ++h) dst[u][h] = src[u][h];
   };
 
   uint32_t c = c0;
   if constexpr (P == 1) {
     // One batch of rows in flight: rows of batch c, then metadata of batch c + U.
     uint32_t k[U];
-    float w[U];
+    float w[U][T];
     if (c < c1) load_meta(c, k, w);
     for (; c < c1; c += U) {
       RowPiece<NF> b[U];
-      float wc[U];
+      float wc[U][T];
       load_rows(k, b);
-#pragma unroll
-      for (int u = 0; u < U; ++u) wc[u] = w[u];
+      copy_w(wc, w);
       load_meta(c + U, k, w);
       // Keep every load of the batch ahead of the products: the scheduler would
       // otherwise sink each row load to its first use and expose its full latency.
@@ -525,22 +489,20 @@ __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
     // and the metadata of batch c + 2U are loading.  Two register sets, A and B,
     // alternate (written out twice so no in-flight register is ever copied).
     uint32_t kA[U], kB[U];
-    float wA[U], wB[U], wc[U];
+    float wA[U][T], wB[U][T], wc[U][T];
     RowPiece<NF> bA[U], bB[U];
     load_meta(c, kA, wA);
     load_rows(kA, bA);
     load_meta(c + U, kB, wB);
     while (true) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) wc[u] = wA[u];
+      copy_w(wc, wA);
       load_rows(kB, bB);
       load_meta(c + 2 * U, kA, wA);
       __builtin_amdgcn_sched_barrier(0);
       compute(c, bA, wc);
       c += U;
       if (c >= c1) break;
-#pragma unroll
-      for (int u = 0; u < U; ++u) wc[u] = wB[u];
+      copy_w(wc, wB);
       load_rows(kA, bA);
       load_meta(c + 2 * U, kB, wB);
       __builtin_amdgcn_sched_barrier(0);
@@ -557,7 +519,7 @@ __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
 // XCDs (slab i on XCD i % 8), so each slab's neighbouring blocks share one L2 while
 // every XCD still sweeps the whole grid; 1 or 2: workgroup w (dispatched to XCD w % 8)
 // takes the w/8-th group of an equal share of runs (runs are already of equal cost).
-template <int CP, int U, int P>
+template <int CP, int U, int P, int T>
 __global__ __launch_bounds__(256) void gather_matrix_kernel(GatherArgs a) {
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   uint32_t wg = blockIdx.x;
@@ -568,7 +530,7 @@ __global__ __launch_bounds__(256) void gather_matrix_kernel(GatherArgs a) {
     wg = ((l / a.slab) * kXcds + x) * a.slab + l % a.slab;
   }
   const uint32_t run = wg * 4 + wave;
-  if (run < a.run_count) matrix_run<CP, U, P>(a, run);
+  if (run < a.run_count) matrix_run<CP, U, P, T>(a, run);
 }
 
 // ----------------------------------------------------------------------------- tile gather
@@ -877,13 +839,13 @@ __global__ __launch_bounds__(256) void gather_fused_kernel(const FusedTable<Gath
   gather_one_item<SW, B, CPL>(a, item, item < a.item_count, sw, c);
 }
 
-template <int CP, int U, int P>
+template <int CP, int U, int P, int T>
 __global__ __launch_bounds__(256) void gather_matrix_fused_kernel(const FusedTable<GatherArgs>* t) {
   const uint32_t it = fused_item(t, blockIdx.x);
   const GatherArgs a = fused_args(t, it);
   const uint32_t wg = static_cast<uint32_t>(blockIdx.x - fused_first(t, it));
   const uint32_t run = wg * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (run < a.run_count) matrix_run<CP, U, P>(a, run);
+  if (run < a.run_count) matrix_run<CP, U, P, T>(a, run);
 }
 
 __global__ __launch_bounds__(256) void reduce_fused_kernel(const FusedTable<GatherArgs>* t) {
@@ -917,44 +879,40 @@ void launch_gather_matrix(GatherArgs a, hipStream_t stream) {
     wgs = ceil_div(wgs, kXcds * a.slab) * kXcds * a.slab;
   }
   a.nt_store = static_cast<uint32_t>(tune_get(kTuneMatrixNtStore));
-  const bool two = tune_get(kTuneMatrixPipeline) == 2;
-  switch (tune_get(kTuneMatrixBatch)) {  // visits per batch: groups of four
-    case 4:
-      if (two) hipLaunchKernelGGL((gather_matrix_kernel<CP, 1, 2>), dim3(wgs), dim3(256), 0, stream, a);
-      else hipLaunchKernelGGL((gather_matrix_kernel<CP, 1, 1>), dim3(wgs), dim3(256), 0, stream, a);
-      break;
-    case 16:
-      if (two) hipLaunchKernelGGL((gather_matrix_kernel<CP, 4, 2>), dim3(wgs), dim3(256), 0, stream, a);
-      else hipLaunchKernelGGL((gather_matrix_kernel<CP, 4, 1>), dim3(wgs), dim3(256), 0, stream, a);
-      break;
-    default:
-      if (two) hipLaunchKernelGGL((gather_matrix_kernel<CP, 2, 2>), dim3(wgs), dim3(256), 0, stream, a);
-      else hipLaunchKernelGGL((gather_matrix_kernel<CP, 2, 1>), dim3(wgs), dim3(256), 0, stream, a);
-      break;
-  }
+  with_matrix_shape(a.bw, [&]<int U, int P, int T>() {
+    hipLaunchKernelGGL((gather_matrix_kernel<CP, U, P, T>), dim3(wgs), dim3(256), 0, stream, a);
+  });
   KSJ_LAUNCH_CHECK();
 }
 
+// Calls f.template operator()<U, P, T>() for the tuned batch (U groups of four
+// visits), pipeline depth and the plans' block width (T products of 16 cells).
+template <typename F>
+void with_matrix_shape(uint32_t block_w, F&& f) {
+  const int u = static_cast<int>(tune_get(kTuneMatrixBatch)) / 4;
+  const bool two = tune_get(kTuneMatrixPipeline) == 2;
+  const bool wide = block_w == 8;
+  auto by_p = [&]<int U>() {
+    if (two) {
+      if (wide) f.template operator()<U, 2, 2>(); else f.template operator()<U, 2, 1>();
+    } else {
+      if (wide) f.template operator()<U, 1, 2>(); else f.template operator()<U, 1, 1>();
+    }
+  };
+  if (u == 1) by_p.template operator()<1>();
+  else if (u == 4) by_p.template operator()<4>();
+  else by_p.template operator()<2>();
+}
+
 template <int CP>
-void launch_gather_matrix_fused(const FusedTableBuilder<GatherArgs>& b, void* d_table, hipStream_t stream) {
+void launch_gather_matrix_fused(const FusedTableBuilder<GatherArgs>& b, uint32_t block_w, void* d_table,
+                                hipStream_t stream) {
   b.upload(d_table, stream);
   const auto* t = static_cast<const FusedTable<GatherArgs>*>(d_table);
   const dim3 grid(static_cast<uint32_t>(b.blocks())), block(256);
-  const bool two = tune_get(kTuneMatrixPipeline) == 2;
-  switch (tune_get(kTuneMatrixBatch)) {
-    case 4:
-      if (two) hipLaunchKernelGGL((gather_matrix_fused_kernel<CP, 1, 2>), grid, block, 0, stream, t);
-      else hipLaunchKernelGGL((gather_matrix_fused_kernel<CP, 1, 1>), grid, block, 0, stream, t);
-      break;
-    case 16:
-      if (two) hipLaunchKernelGGL((gather_matrix_fused_kernel<CP, 4, 2>), grid, block, 0, stream, t);
-      else hipLaunchKernelGGL((gather_matrix_fused_kernel<CP, 4, 1>), grid, block, 0, stream, t);
-      break;
-    default:
-      if (two) hipLaunchKernelGGL((gather_matrix_fused_kernel<CP, 2, 2>), grid, block, 0, stream, t);
-      else hipLaunchKernelGGL((gather_matrix_fused_kernel<CP, 2, 1>), grid, block, 0, stream, t);
-      break;
-  }
+  with_matrix_shape(block_w, [&]<int U, int P, int T>() {
+    hipLaunchKernelGGL((gather_matrix_fused_kernel<CP, U, P, T>), grid, block, 0, stream, t);
+  });
   KSJ_LAUNCH_CHECK();
 }
 
@@ -985,8 +943,7 @@ uint32_t padded_coils(uint32_t coils) {
 }
 
 size_t grid_partial_bytes(const Plan2& plan, uint32_t coils) {
-  const size_t side = block_side(plan.scheme);
-  const size_t cells = plan.scheme == kSchemeTiles ? kTileCells : side * side;
+  const size_t cells = plan.scheme == kSchemeTiles ? kTileCells : static_cast<size_t>(plan.block_h) * plan.block_w;
   return static_cast<size_t>(plan.slot_count) * cells * padded_coils(coils) * sizeof(float2);
 }
 
@@ -1036,7 +993,8 @@ GatherArgs gather_args(const Plan2& plan, const float2* d_sorted, uint32_t coils
   a.item_count = plan.item_count;
   a.multi_count = plan.multi_count;
   a.xcd_first = plan.xcd_first;
-  a.side = block_side(plan.scheme);
+  a.bh = plan.block_h;
+  a.bw = plan.block_w;
   a.run_first = plan.run_first;
   a.run_count = plan.run_count;
   return a;
@@ -1055,7 +1013,7 @@ void grid_reduce(const Plan2& plan, uint32_t coils, float2* d_grid, const float2
   }
   const GatherArgs a = gather_args(plan, nullptr, coils, d_grid, const_cast<float2*>(d_partial));
   hipLaunchKernelGGL(multi_reduce_kernel,
-                     dim3(ceil_div(static_cast<uint64_t>(a.multi_count) * a.side * a.side * a.cp, 256)), dim3(256),
+                     dim3(ceil_div(static_cast<uint64_t>(a.multi_count) * a.bh * a.bw * a.cp, 256)), dim3(256),
                      0, stream, a);
   KSJ_LAUNCH_CHECK();
 }
@@ -1107,6 +1065,7 @@ void grid_fused(const GridFusedItem* items, uint32_t count, void* d_tables, hipS
   const bool two = tune_get(kTuneGatherCoilsPerLane) == 2;
   FusedTableBuilder<PrepItemArgs> prep[4];
   FusedTableBuilder<GatherArgs> gather[4], matrix[4], reduce;
+  uint32_t matrix_w = 0;  // block width of the batch's matrix plans (one width per fused call)
   const uint32_t nt_store = static_cast<uint32_t>(tune_get(kTuneMatrixNtStore));
   for (uint32_t i = 0; i < count; ++i) {
     const GridFusedItem& it = items[i];
@@ -1128,6 +1087,8 @@ void grid_fused(const GridFusedItem* items, uint32_t count, void* d_tables, hipS
     GatherArgs ga = gather_args(*it.plan, it.sorted, it.channels, it.grid, it.partial);
     ga.xcd_chunk = 0;
     if (it.plan->scheme == kSchemeMatrix) {
+      require(matrix_w == 0 || matrix_w == it.plan->block_w, "fused matrix items share one block width");
+      matrix_w = it.plan->block_w;
       ga.nt_store = nt_store;
       matrix[cls].add(ga, std::max<uint64_t>(1, ceil_div(ga.run_count, 4)));
     } else {
@@ -1135,7 +1096,7 @@ void grid_fused(const GridFusedItem* items, uint32_t count, void* d_tables, hipS
       gather[cls].add(ga, std::max<uint64_t>(1, ceil_div(ceil_div(ga.item_count, 64 / sw), 4)));
     }
     if (ga.multi_count > 0)
-      reduce.add(ga, ceil_div(static_cast<uint64_t>(ga.multi_count) * ga.side * ga.side * cp, 256));
+      reduce.add(ga, ceil_div(static_cast<uint64_t>(ga.multi_count) * ga.bh * ga.bw * cp, 256));
   }
   char* tab = static_cast<char*>(d_tables);
   const dim3 prep_block(kPrepSamples * kPrepWaves);
@@ -1169,10 +1130,10 @@ void grid_fused(const GridFusedItem* items, uint32_t count, void* d_tables, hipS
     void* d = tab + cls * fused_table_bytes<GatherArgs>();
     if (matrix[cls].t.count == 0) continue;
     switch (cls) {
-      case 0: launch_gather_matrix_fused<8>(matrix[cls], d, stream); break;
-      case 1: launch_gather_matrix_fused<16>(matrix[cls], d, stream); break;
-      case 2: launch_gather_matrix_fused<32>(matrix[cls], d, stream); break;
-      default: launch_gather_matrix_fused<64>(matrix[cls], d, stream); break;
+      case 0: launch_gather_matrix_fused<8>(matrix[cls], matrix_w, d, stream); break;
+      case 1: launch_gather_matrix_fused<16>(matrix[cls], matrix_w, d, stream); break;
+      case 2: launch_gather_matrix_fused<32>(matrix[cls], matrix_w, d, stream); break;
+      default: launch_gather_matrix_fused<64>(matrix[cls], matrix_w, d, stream); break;
     }
   }
   if (reduce.t.count > 0) {

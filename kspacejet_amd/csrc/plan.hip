@@ -79,29 +79,30 @@ __global__ void taps_kernel(TapArgs a) {
 }
 
 // ----------------------------------------------------------------------------- gather schedule
-// Window of s x s block (BY, BX): bin rows s BY - W/2 + r and columns s BX - W/2 + c
-// for r, c in [0, W + s - 1); cell (s BY + dy, s BX + dx) uses row tap dy - r + W - 1
-// and column tap dx - c + W - 1 of a sample in window bin (r, c) (taps outside
-// [0, W) weigh 0).
+// Window of h x w block (BY, BX): bin rows h BY - W/2 + r and columns w BX - W/2 + c
+// for r in [0, W + h - 1), c in [0, W + w - 1); cell (h BY + dy, w BX + dx) uses row
+// tap dy - r + W - 1 and column tap dx - c + W - 1 of a sample in window bin (r, c)
+// (taps outside [0, W) weigh 0).
 struct ScheduleArgs {
   const uint32_t* start;
   const float* taps;  // [S][2W]
   uint32_t grid_rows, grid_cols, block_cols, block_group;
   uint64_t nblocks;
-  int width, side;    // side s: 2 or 4
+  int width, bh, bw;  // block h x w: 2 x 2, 4 x 4 or 4 x 8
   uint32_t* row_off;  // counts, then offsets
   const uint32_t* chunk_off;  // matrix schedule: first group of four of each block (null otherwise)
   uint32_t* visit_k;
-  float* visit_w;     // [visits][s*s]
+  float* visit_w;     // [visits][h*w]
 };
 
-__device__ inline int window_rows(const ScheduleArgs& a) { return a.width + a.side - 1; }
+__device__ inline int window_rows(const ScheduleArgs& a) { return a.width + a.bh - 1; }
+__device__ inline int window_cols(const ScheduleArgs& a) { return a.width + a.bw - 1; }
 
 // False for the padding blocks past the grid (no visits).
 __device__ inline bool block_in_grid(const ScheduleArgs& a, uint64_t blk) {
   uint32_t by, bx;
   block_coords(blk, a.block_cols, a.block_group, &by, &bx);
-  return a.side * by < a.grid_rows && a.side * bx < a.grid_cols;
+  return a.bh * by < a.grid_rows && a.bw * bx < a.grid_cols;
 }
 
 __device__ inline uint64_t window_bin(const ScheduleArgs& a, uint64_t blk, int r, int c) {
@@ -109,8 +110,8 @@ __device__ inline uint64_t window_bin(const ScheduleArgs& a, uint64_t blk, int r
   const int gr = static_cast<int>(a.grid_rows), gc = static_cast<int>(a.grid_cols);
   uint32_t byu, bxu;
   block_coords(blk, a.block_cols, a.block_group, &byu, &bxu);
-  int by = a.side * static_cast<int>(byu) - W / 2 + r;
-  int bx = a.side * static_cast<int>(bxu) - W / 2 + c;
+  int by = a.bh * static_cast<int>(byu) - W / 2 + r;
+  int bx = a.bw * static_cast<int>(bxu) - W / 2 + c;
   by = ((by % gr) + gr) % gr;
   bx = ((bx % gc) + gc) % gc;
   return static_cast<uint64_t>(by) * a.grid_cols + bx;
@@ -129,7 +130,7 @@ __global__ void visit_count_kernel(ScheduleArgs a) {
   const int r = static_cast<int>(i % R);
   uint32_t n = 0;
   if (block_in_grid(a, blk)) {
-    for (int c = 0; c < R; ++c) {
+    for (int c = 0; c < window_cols(a); ++c) {
       const uint64_t b = window_bin(a, blk, r, c);
       n += a.start[b + 1] - a.start[b];
     }
@@ -142,21 +143,21 @@ __device__ inline float tap_or_zero(const float* t, int i, int W) { return (i >=
 
 __global__ void visit_fill_kernel(ScheduleArgs a) {
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int W = a.width, s = a.side, R = window_rows(a);
+  const int W = a.width, h = a.bh, wd = a.bw, R = window_rows(a);
   if (i >= a.nblocks * R) return;
   const uint64_t blk = i / R;
   const int r = static_cast<int>(i % R);
   if (!block_in_grid(a, blk)) return;
   // Matrix schedule: blocks start on groups of four (lists padded by pad_visits_kernel).
   uint32_t pos = a.chunk_off ? 4 * a.chunk_off[blk] + (a.row_off[i] - a.row_off[blk * R]) : a.row_off[i];
-  for (int c = 0; c < R; ++c) {
+  for (int c = 0; c < window_cols(a); ++c) {
     const uint64_t b = window_bin(a, blk, r, c);
     for (uint32_t k = a.start[b]; k < a.start[b + 1]; ++k) {
       const float* t = a.taps + static_cast<uint64_t>(k) * (2 * W);
-      float* w = a.visit_w + static_cast<uint64_t>(pos) * s * s;
-      for (int dy = 0; dy < s; ++dy) {
+      float* w = a.visit_w + static_cast<uint64_t>(pos) * h * wd;
+      for (int dy = 0; dy < h; ++dy) {
         const float wy = tap_or_zero(t, dy - r + W - 1, W);
-        for (int dx = 0; dx < s; ++dx) w[dy * s + dx] = wy * tap_or_zero(t + W, dx - c + W - 1, W);
+        for (int dx = 0; dx < wd; ++dx) w[dy * wd + dx] = wy * tap_or_zero(t + W, dx - c + W - 1, W);
       }
       a.visit_k[pos] = k;
       ++pos;
@@ -174,14 +175,14 @@ __global__ void chunk_count_kernel(const uint32_t* row_off, uint64_t nblocks, in
 // Matrix schedule: the padding visits of each block repeat its first sample with zero
 // weights (so they add exact zeros and read a row the block reads anyway).
 __global__ void pad_visits_kernel(const uint32_t* row_off, const uint32_t* chunk_off, uint64_t nblocks, int rows,
-                                  uint32_t* visit_k, float* visit_w) {
+                                  int cells, uint32_t* visit_k, float* visit_w) {
   const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (b >= nblocks) return;
   const uint32_t base = 4 * chunk_off[b], end = 4 * chunk_off[b + 1];
   const uint32_t n = row_off[(b + 1) * rows] - row_off[b * rows];
   for (uint32_t p = base + n; p < end; ++p) {
     visit_k[p] = visit_k[base];
-    for (int c = 0; c < 16; ++c) visit_w[static_cast<uint64_t>(p) * 16 + c] = 0.f;
+    for (int c = 0; c < cells; ++c) visit_w[static_cast<uint64_t>(p) * cells + c] = 0.f;
   }
 }
 
@@ -504,6 +505,8 @@ Plan2* plan2_create(const Plan2Desc& desc) {
   if (p->d.tile_width == 0) p->d.tile_width = static_cast<uint32_t>(tune_get(kTuneTileWidth));
   if (p->d.block_group == 0) p->d.block_group = static_cast<uint32_t>(tune_get(kTuneBlockGroup));
   if (p->d.run_cost == 0) p->d.run_cost = static_cast<uint32_t>(tune_get(kTuneMatrixRunCost));
+  if (p->d.block_width == 0) p->d.block_width = static_cast<uint32_t>(tune_get(kTuneMatrixBlockWidth));
+  require(p->d.block_width == 4 || p->d.block_width == 8, "matrix block width must be 4 or 8");
   require(p->d.run_cost >= 1 && p->d.run_cost <= 64, "run cost must be in [1, 64]");
   require(p->d.block_group == 1 || p->d.block_group == 2 || p->d.block_group == 4, "block group must be 1, 2 or 4");
   require(p->d.tile_width == 4 || p->d.tile_width == 8, "tile width must be 4 or 8");
@@ -521,9 +524,11 @@ Plan2* plan2_create(const Plan2Desc& desc) {
     p->key_bits = 1;
     while ((1ull << p->key_bits) < p->nbins) ++p->key_bits;
     p->block_group = p->d.block_group;
-    const uint32_t side = block_side(p->d.scheme);
-    p->block_rows = ceil_div(ceil_div(desc.grid_rows, side), p->block_group) * p->block_group;
-    p->block_cols = ceil_div(ceil_div(desc.grid_cols, side), p->block_group) * p->block_group;
+    const bool matrix = p->d.scheme == kSchemeMatrix;
+    p->block_h = matrix ? 4u : 2u;
+    p->block_w = matrix ? p->d.block_width : 2u;
+    p->block_rows = ceil_div(ceil_div(desc.grid_rows, p->block_h), p->block_group) * p->block_group;
+    p->block_cols = ceil_div(ceil_div(desc.grid_cols, p->block_w), p->block_group) * p->block_group;
     p->nblocks = static_cast<uint64_t>(p->block_rows) * p->block_cols;
     // The matrix gather takes visits four at a time: items start on multiples of 4.
     p->item_visits = p->d.scheme == kSchemeMatrix ? (p->d.item_visits + 3) / 4 * 4 : p->d.item_visits;
@@ -542,7 +547,7 @@ Plan2* plan2_create(const Plan2Desc& desc) {
     p->bins = alloc<uint32_t>(S, &total);
     p->cell_start = alloc<uint32_t>(p->nbins + 1, &total);
     p->taps = alloc<float>(S * 2 * desc.width, &total);
-    const uint64_t window = desc.width + side - 1;  // window rows per block
+    const uint64_t window = desc.width + p->block_h - 1;  // window rows per block
     if (p->scheme != kSchemeTiles)
       p->visit_row_off = alloc<uint32_t>(p->nblocks * window + 1, &total);
     else
@@ -647,11 +652,12 @@ void plan2_set_trajectory(Plan2* p, const float* d_traj, const float* d_dcf, hip
   sa.block_group = p->block_group;
   sa.nblocks = p->nblocks;
   sa.width = static_cast<int>(p->d.width);
-  sa.side = static_cast<int>(block_side(p->scheme));
+  sa.bh = static_cast<int>(p->block_h);
+  sa.bw = static_cast<int>(p->block_w);
   sa.row_off = p->visit_row_off;
-  const int window = sa.width + sa.side - 1;
+  const int window = sa.width + sa.bh - 1;
   const uint64_t nrows = p->nblocks * window;
-  const uint64_t weight_vec4 = static_cast<uint64_t>(sa.side) * sa.side / 4;  // float4s of weights per visit
+  const uint64_t weight_vec4 = static_cast<uint64_t>(sa.bh) * sa.bw / 4;  // float4s of weights per visit
   hipLaunchKernelGGL(visit_count_kernel, dim3(ceil_div(nrows + 1, 256)), dim3(256), 0, stream, sa);
   KSJ_LAUNCH_CHECK();
   exclusive_scan_u32(p->visit_row_off, p->visit_row_off, nrows + 1, p->sort_scratch, stream);
@@ -688,7 +694,7 @@ void plan2_set_trajectory(Plan2* p, const float* d_traj, const float* d_dcf, hip
   KSJ_LAUNCH_CHECK();
   if (matrix) {
     hipLaunchKernelGGL(pad_visits_kernel, dim3(ceil_div(p->nblocks, 256)), dim3(256), 0, stream, p->visit_row_off,
-                       p->block_chunk_off, p->nblocks, window, p->visit_k, sa.visit_w);
+                       p->block_chunk_off, p->nblocks, window, sa.bh * sa.bw, p->visit_k, sa.visit_w);
     KSJ_LAUNCH_CHECK();
   }
   // Work items: ceil(visits / item_visits) per block (at least one, so every cell is

@@ -32,8 +32,7 @@ enum : uint32_t { kDensityNone = 0, kDensityRamp = 1, kDensityGiven = 2 };
 // cores.  kSchemeDefault takes tuning key gather_scheme.
 enum : uint32_t { kSchemeBlocks = 0, kSchemeTiles = 1, kSchemeMatrix = 2, kSchemeDefault = 0xFFFFFFFFu };
 constexpr uint32_t kTileCells = 64;
-// Cells per side of a schedule block: 2 (kSchemeBlocks) or 4 (kSchemeMatrix).
-inline uint32_t block_side(uint32_t scheme) { return scheme == kSchemeMatrix ? 4u : 2u; }
+
 
 // Block numbering: blocks come in groups of g x g (row-major inside a group,
 // groups row-major), so consecutive work items -- which one workgroup takes
@@ -65,6 +64,7 @@ struct Plan2Desc {
   uint32_t tile_width = 0;                  // tiles: 4 or 8 columns (0: tuning key)
   uint32_t block_group = 0;                 // blocks: group side 1, 2 or 4 (0: tuning key)
   uint32_t run_cost = 0;                    // matrix: cost per wave run, 1..64 (0: tuning key)
+  uint32_t block_width = 0;                 // matrix: block columns, 4 or 8 (0: tuning key)
 };
 
 struct Plan2 {
@@ -79,19 +79,20 @@ struct Plan2 {
   uint32_t* bins = nullptr;        // [S] sample bins (original order), then sorted keys
   uint32_t* cell_start = nullptr;  // [nbins + 1]
   float* taps = nullptr;           // [S][2W] sorted: W row taps then W column taps
-  // Gather schedule.  Cells are taken in s x s blocks (s = 2, or 4 for the matrix
-  // schedule); a block's visits are the samples of its (W+s-1) x (W+s-1)
+  // Gather schedule.  Cells are taken in h x w blocks (2 x 2; or 4 x 4 or 4 x 8 for
+  // the matrix schedule); a block's visits are the samples of its (W+h-1) x (W+w-1)
   // contributing bins (rows then columns of that bin window, sorted order inside a
-  // bin), each with the s*s weights phi(row tap) * phi(column tap) of the block's
+  // bin), each with the h*w weights phi(row tap) * phi(column tap) of the block's
   // cells in row-major order (0 outside the footprint).  Blocks are cut into work
   // items of at most item_visits visits (a multiple of 4 for the matrix schedule);
   // a block with several items is summed item by item in item order.
+  uint32_t block_h = 2, block_w = 2;     // cells per block row / column
   uint32_t block_rows = 0, block_cols = 0, block_group = 1;  // rows / columns padded to whole groups
   uint64_t nblocks = 0;
   uint32_t item_visits = 64;
-  uint32_t* visit_row_off = nullptr;  // [nblocks * (W+s-1) + 1] first visit of each (block, window row)
+  uint32_t* visit_row_off = nullptr;  // [nblocks * (W+h-1) + 1] first visit of each (block, window row)
   uint32_t* visit_k = nullptr;        // [visit_capacity] sorted sample position
-  float4* visit_w = nullptr;          // [visit_capacity * s*s/4] cell weights, row-major per visit
+  float4* visit_w = nullptr;          // [visit_capacity * h*w/4] cell weights, row-major per visit
   uint64_t visit_capacity = 0;
   uint64_t visit_count = 0;           // host copy
   uint32_t* block_first_item = nullptr;  // [nblocks + 1]

@@ -74,17 +74,20 @@ __global__ void taps3_kernel(const uint32_t* src, const double* frac, uint32_t s
 }
 
 // ----------------------------------------------------------------------------- schedule
-// Window of block (bz, by, bx): bins 2b - W/2 + r per axis (wrapped), r in [0, W];
-// cell 2b + c of an axis takes tap c - r + W - 1 of a sample in window bin r.
+// Window of block (bz, by, bx) of side s: bins s b - W/2 + r per axis (wrapped),
+// r in [0, W + s - 1); cell s b + c of an axis takes tap c - r + W - 1 of a sample in
+// window bin r.
 struct Sched3Args {
   const uint32_t* start;
   const float* taps;  // [S][3W]
   uint32_t g[3], blocks[3];
   uint64_t nblocks;
-  int width;
+  int width, side;
   uint32_t* off;      // [nblocks + 1]: counts, then offsets
+  const uint32_t* chunk_off;  // matrix schedule: first group of four of each block (null otherwise)
   uint32_t* visit_k;
-  float4* visit_w;
+  float4* visit_w;    // 2x2x2 schedule
+  uint32_t* visit_pos;  // matrix schedule
 };
 
 __device__ inline uint32_t wrap3(int v, int n) { return static_cast<uint32_t>(((v % n) + n) % n); }
@@ -93,9 +96,9 @@ __device__ inline void block_origin(const Sched3Args& a, uint64_t blk, int* o) {
   const uint32_t bx = static_cast<uint32_t>(blk % a.blocks[2]);
   const uint64_t r = blk / a.blocks[2];
   const uint32_t by = static_cast<uint32_t>(r % a.blocks[1]), bz = static_cast<uint32_t>(r / a.blocks[1]);
-  o[0] = 2 * static_cast<int>(bz) - a.width / 2;
-  o[1] = 2 * static_cast<int>(by) - a.width / 2;
-  o[2] = 2 * static_cast<int>(bx) - a.width / 2;
+  o[0] = a.side * static_cast<int>(bz) - a.width / 2;
+  o[1] = a.side * static_cast<int>(by) - a.width / 2;
+  o[2] = a.side * static_cast<int>(bx) - a.width / 2;
 }
 
 __global__ void visit3_count_kernel(Sched3Args a) {
@@ -107,13 +110,13 @@ __global__ void visit3_count_kernel(Sched3Args a) {
   }
   int o[3];
   block_origin(a, blk, o);
-  const int W = a.width;
+  const int W = a.width, R = W + a.side - 1;
   uint32_t n = 0;
-  for (int rz = 0; rz <= W; ++rz) {
+  for (int rz = 0; rz < R; ++rz) {
     const uint64_t pz = wrap3(o[0] + rz, static_cast<int>(a.g[0]));
-    for (int ry = 0; ry <= W; ++ry) {
+    for (int ry = 0; ry < R; ++ry) {
       const uint64_t row = (pz * a.g[1] + wrap3(o[1] + ry, static_cast<int>(a.g[1]))) * a.g[2];
-      for (int rx = 0; rx <= W; ++rx) {
+      for (int rx = 0; rx < R; ++rx) {
         const uint64_t b = row + wrap3(o[2] + rx, static_cast<int>(a.g[2]));
         n += a.start[b + 1] - a.start[b];
       }
@@ -123,6 +126,79 @@ __global__ void visit3_count_kernel(Sched3Args a) {
 }
 
 __device__ inline float tap_or_zero(const float* t, int i, int W) { return (i >= 0 && i < W) ? t[i] : 0.f; }
+
+// Matrix schedule: visits (sample, window position), the block's list starting on
+// its first group of four (padding written by pad3_visits_kernel).
+__global__ void visit3_fill_matrix_kernel(Sched3Args a) {
+  const uint64_t blk = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (blk >= a.nblocks) return;
+  int o[3];
+  block_origin(a, blk, o);
+  const int R = a.width + a.side - 1;
+  uint32_t pos = 4 * a.chunk_off[blk];
+  for (int rz = 0; rz < R; ++rz) {
+    const uint64_t pz = wrap3(o[0] + rz, static_cast<int>(a.g[0]));
+    for (int ry = 0; ry < R; ++ry) {
+      const uint64_t row = (pz * a.g[1] + wrap3(o[1] + ry, static_cast<int>(a.g[1]))) * a.g[2];
+      for (int rx = 0; rx < R; ++rx) {
+        const uint64_t b = row + wrap3(o[2] + rx, static_cast<int>(a.g[2]));
+        const uint32_t code = static_cast<uint32_t>(rz) | static_cast<uint32_t>(ry) << 4 | static_cast<uint32_t>(rx) << 8;
+        for (uint32_t k = a.start[b]; k < a.start[b + 1]; ++k) {
+          a.visit_k[pos] = k;
+          a.visit_pos[pos] = code;
+          ++pos;
+        }
+      }
+    }
+  }
+}
+
+// Groups of four per block (chunks[nblocks] = 0 for the scan).
+__global__ void chunk3_count_kernel(const uint32_t* off, uint64_t nblocks, uint32_t* chunks) {
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b > nblocks) return;
+  chunks[b] = b < nblocks ? (off[b + 1] - off[b] + 3) / 4 : 0u;
+}
+
+// Padding visits: the block's first sample at a position outside every window.
+__global__ void pad3_visits_kernel(const uint32_t* off, const uint32_t* chunk_off, uint64_t nblocks,
+                                   uint32_t* visit_k, uint32_t* visit_pos) {
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  const uint32_t base = 4 * chunk_off[b], end = 4 * chunk_off[b + 1];
+  for (uint32_t p = base + (off[b + 1] - off[b]); p < end; ++p) {
+    visit_k[p] = visit_k[base];
+    visit_pos[p] = kPadVisitPos;
+  }
+}
+
+// Item cost (groups of four + 1) for the run split; cost[items] = 0.
+__global__ void item3_cost_kernel(const uint4* info, uint32_t items, uint32_t* cost) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < items) cost[i] = (info[i].z - info[i].y) / 4 + 1;
+  if (i == items) cost[i] = 0;
+}
+
+// first[r] = first item whose cost prefix reaches r * run_cost; first[runs] = items.
+__global__ void run3_split_kernel(const uint32_t* prefix, uint32_t items, uint32_t runs, uint32_t run_cost,
+                                  uint32_t* first) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > runs) return;
+  if (r == runs) {
+    first[r] = items;
+    return;
+  }
+  const uint64_t target = static_cast<uint64_t>(r) * run_cost;
+  uint32_t lo = 0, hi = items;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (prefix[mid] >= target)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  first[r] = lo;
+}
 
 __global__ void visit3_fill_kernel(Sched3Args a) {
   const uint64_t blk = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -178,16 +254,18 @@ __global__ void slot3_count_kernel(const uint32_t* first, uint64_t nblocks, uint
   slots[b] = n > 1 ? n : 0u;
 }
 
-__global__ void item3_fill_kernel(const uint32_t* first, const uint32_t* slot, const uint32_t* off,
-                                  uint32_t item_visits, uint64_t nblocks, uint4* info) {
+// Item records {block, first visit, end visit, slot or ~0}; off counts units of
+// `scale` visits (1, or 4 for the matrix schedule's groups) and items hold `size` units.
+__global__ void item3_fill_kernel(const uint32_t* first, const uint32_t* slot, const uint32_t* off, uint32_t size,
+                                  uint32_t scale, uint64_t nblocks, uint4* info) {
   const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (b >= nblocks) return;
   const uint32_t vb = off[b], ve = off[b + 1];
   const uint32_t n = first[b + 1] - first[b];
   for (uint32_t j = 0; j < n; ++j) {
-    const uint32_t v0 = vb + j * item_visits;
-    const uint32_t v1 = v0 + item_visits < ve ? v0 + item_visits : ve;
-    info[first[b] + j] = make_uint4(static_cast<uint32_t>(b), v0, v1, n > 1 ? slot[b] + j : 0xFFFFFFFFu);
+    const uint32_t v0 = vb + j * size;
+    const uint32_t v1 = v0 + size < ve ? v0 + size : ve;
+    info[first[b] + j] = make_uint4(static_cast<uint32_t>(b), scale * v0, scale * v1, n > 1 ? slot[b] + j : 0xFFFFFFFFu);
   }
 }
 
@@ -229,13 +307,31 @@ Plan3* plan3_create(const Plan3Desc& desc) {
   require(desc.density == kDensityNone || desc.density == kDensityRampSquared || desc.density == kDensityGiven,
           "3-D density must be none, ramp squared or given");
   require(desc.samples >= 1, "plan needs at least one sample");
-  require(desc.item_visits >= 1 && desc.item_visits <= 4096, "item visits must be in [1, 4096]");
-  // A sample reaches at most W/2 + 1 blocks per axis; the visit offsets are 32-bit.
-  const uint64_t reach = desc.width / 2 + 1;
-  require(static_cast<uint64_t>(desc.samples) * reach * reach * reach < (1ull << 32),
-          "too many samples for one 3-D plan (visit count must stay below 2^32)");
+  require(desc.item_visits <= 4096, "item visits must be in [1, 4096]");
   auto* p = new Plan3();
   p->d = desc;
+  if (p->d.scheme == kSchemeDefault) p->d.scheme = static_cast<uint32_t>(tune_get(kTuneGather3Scheme));
+  if (p->d.item_visits == 0)
+    p->d.item_visits =
+      static_cast<uint32_t>(tune_get(p->d.scheme == kSchemeMatrix ? kTuneMatrix3ItemVisits : kTuneItemVisits));
+  if (p->d.run_cost == 0) p->d.run_cost = static_cast<uint32_t>(tune_get(kTuneMatrix3RunCost));
+  if (p->d.scheme != kSchemeBlocks && p->d.scheme != kSchemeMatrix) {
+    delete p;
+    throw Error(1, "3-D gather schedule must be 0 (2x2x2 blocks) or 2 (matrix)");
+  }
+  if (p->d.run_cost < 1 || p->d.run_cost > 64) {
+    delete p;
+    throw Error(1, "run cost must be in [1, 64]");
+  }
+  // A sample reaches at most ceil((W + s - 1) / s) blocks per axis (plus three padding
+  // visits per block for the matrix schedule); visit offsets are 32-bit.
+  const uint64_t side = p->d.scheme == kSchemeMatrix ? 4 : 2;
+  const uint64_t reach = (desc.width + 2 * side - 2) / side;
+  const uint64_t pad = p->d.scheme == kSchemeMatrix ? 3 * ((nbins + 63) / 64) : 0;
+  if (static_cast<uint64_t>(desc.samples) * reach * reach * reach + pad >= (1ull << 32)) {
+    delete p;
+    throw Error(1, "too many samples for one 3-D plan (visit count must stay below 2^32)");
+  }
   if (p->d.beta <= 0.0) {
     const double sigma = static_cast<double>(desc.grid[2]) / desc.image[2];
     const double a = static_cast<double>(desc.width) / sigma * (sigma - 0.5);
@@ -247,7 +343,11 @@ Plan3* plan3_create(const Plan3Desc& desc) {
     p->nbins = nbins;
     p->key_bits = 1;
     while ((1ull << p->key_bits) < nbins) ++p->key_bits;
-    for (int d = 0; d < 3; ++d) p->blocks[d] = (desc.grid[d] + 1) / 2;
+    p->scheme = p->d.scheme;
+    p->side = p->scheme == kSchemeMatrix ? 4u : 2u;
+    p->run_cost = p->d.run_cost;
+    p->item_visits = p->scheme == kSchemeMatrix ? (p->d.item_visits + 3) / 4 * 4 : p->d.item_visits;
+    for (int d = 0; d < 3; ++d) p->blocks[d] = (desc.grid[d] + p->side - 1) / p->side;
     p->nblocks = static_cast<uint64_t>(p->blocks[0]) * p->blocks[1] * p->blocks[2];
     size_t total = 0;
     p->weight = alloc3<float2>(S, &total);
@@ -260,6 +360,7 @@ Plan3* plan3_create(const Plan3Desc& desc) {
     p->block_first_item = alloc3<uint32_t>(p->nblocks + 1, &total);
     p->block_first_slot = alloc3<uint32_t>(p->nblocks + 1, &total);
     p->multi_blocks = alloc3<uint32_t>(p->nblocks, &total);
+    if (p->scheme == kSchemeMatrix) p->block_chunk_off = alloc3<uint32_t>(p->nblocks + 1, &total);
     p->frac = alloc3<double>(3 * S, &total);
     p->sorted_keys = alloc3<uint32_t>(S, &total);
     p->counters = alloc3<uint32_t>(4, &total);
@@ -283,7 +384,8 @@ void plan3_destroy(Plan3* p) {
   void* ptrs[] = {p->weight,     p->rank,        p->src,           p->bins,       p->cell_start,
                   p->taps,       p->block_visit_off, p->visit_k,   p->visit_w,    p->block_first_item,
                   p->block_first_slot, p->item_info, p->multi_blocks, p->frac,     p->sorted_keys,
-                  p->counters,   p->sort_scratch, p->deapod[0],    p->deapod[1],  p->deapod[2]};
+                  p->counters,   p->sort_scratch, p->deapod[0],    p->deapod[1],  p->deapod[2],
+                  p->visit_pos,  p->block_chunk_off, p->item_cost, p->item_scratch, p->run_first};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -335,32 +437,62 @@ void plan3_set_trajectory(Plan3* p, const float* d_traj, const float* d_dcf, hip
   }
   sa.nblocks = p->nblocks;
   sa.width = static_cast<int>(p->d.width);
+  sa.side = static_cast<int>(p->side);
   sa.off = p->block_visit_off;
+  const bool matrix = p->scheme == kSchemeMatrix;
   hipLaunchKernelGGL(visit3_count_kernel, dim3(ceil_div(p->nblocks + 1, 256)), dim3(256), 0, stream, sa);
   KSJ_LAUNCH_CHECK();
   exclusive_scan_u32(p->block_visit_off, p->block_visit_off, p->nblocks + 1, p->sort_scratch, stream);
   uint32_t visits = 0;
-  KSJ_HIP(hipMemcpyAsync(&visits, p->block_visit_off + p->nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-  KSJ_HIP(hipStreamSynchronize(stream));
+  if (matrix) {  // lists padded to groups of four
+    hipLaunchKernelGGL(chunk3_count_kernel, dim3(ceil_div(p->nblocks + 1, 256)), dim3(256), 0, stream,
+                       p->block_visit_off, p->nblocks, p->block_chunk_off);
+    KSJ_LAUNCH_CHECK();
+    exclusive_scan_u32(p->block_chunk_off, p->block_chunk_off, p->nblocks + 1, p->sort_scratch, stream);
+    uint32_t chunks = 0;
+    KSJ_HIP(hipMemcpyAsync(&chunks, p->block_chunk_off + p->nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    KSJ_HIP(hipStreamSynchronize(stream));
+    require(chunks <= 0x3FFFFFFFu, "too many gather visits for one 3-D plan");
+    visits = 4 * chunks;
+  } else {
+    KSJ_HIP(hipMemcpyAsync(&visits, p->block_visit_off + p->nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    KSJ_HIP(hipStreamSynchronize(stream));
+  }
   if (visits > p->visit_capacity || p->visit_k == nullptr) {
-    if (p->visit_k) KSJ_HIP(hipFree(p->visit_k));
-    if (p->visit_w) KSJ_HIP(hipFree(p->visit_w));
-    p->visit_k = nullptr;
-    p->visit_w = nullptr;
+    for (void** q : {reinterpret_cast<void**>(&p->visit_k), reinterpret_cast<void**>(&p->visit_w),
+                     reinterpret_cast<void**>(&p->visit_pos)}) {
+      if (*q) KSJ_HIP(hipFree(*q));
+      *q = nullptr;
+    }
     p->visit_capacity = std::max<uint64_t>(visits, p->visit_capacity);
     KSJ_HIP(hipMalloc(&p->visit_k, std::max<size_t>(1, p->visit_capacity) * sizeof(uint32_t)));
-    KSJ_HIP(hipMalloc(&p->visit_w, std::max<size_t>(1, p->visit_capacity) * 2 * sizeof(float4)));
+    if (matrix)
+      KSJ_HIP(hipMalloc(&p->visit_pos, std::max<size_t>(1, p->visit_capacity) * sizeof(uint32_t)));
+    else
+      KSJ_HIP(hipMalloc(&p->visit_w, std::max<size_t>(1, p->visit_capacity) * 2 * sizeof(float4)));
   }
   p->visit_count = visits;
   sa.visit_k = p->visit_k;
   sa.visit_w = p->visit_w;
-  hipLaunchKernelGGL(visit3_fill_kernel, dim3(ceil_div(p->nblocks, 256)), dim3(256), 0, stream, sa);
-  KSJ_LAUNCH_CHECK();
+  sa.visit_pos = p->visit_pos;
+  sa.chunk_off = p->block_chunk_off;
+  if (matrix) {
+    hipLaunchKernelGGL(visit3_fill_matrix_kernel, dim3(ceil_div(p->nblocks, 256)), dim3(256), 0, stream, sa);
+    KSJ_LAUNCH_CHECK();
+    hipLaunchKernelGGL(pad3_visits_kernel, dim3(ceil_div(p->nblocks, 256)), dim3(256), 0, stream,
+                       p->block_visit_off, p->block_chunk_off, p->nblocks, p->visit_k, p->visit_pos);
+    KSJ_LAUNCH_CHECK();
+  } else {
+    hipLaunchKernelGGL(visit3_fill_kernel, dim3(ceil_div(p->nblocks, 256)), dim3(256), 0, stream, sa);
+    KSJ_LAUNCH_CHECK();
+  }
 
+  // Items per block: the matrix schedule counts groups of four.
+  const uint32_t* unit_off = matrix ? p->block_chunk_off : p->block_visit_off;
+  const uint32_t unit_size = matrix ? p->item_visits / 4 : p->item_visits;
   KSJ_HIP(hipMemsetAsync(p->counters, 0, sizeof(uint32_t), stream));
-  hipLaunchKernelGGL(item3_count_kernel, dim3(ceil_div(p->nblocks + 1, 256)), dim3(256), 0, stream,
-                     p->block_visit_off, p->nblocks, p->d.item_visits, p->block_first_item, p->multi_blocks,
-                     p->counters);
+  hipLaunchKernelGGL(item3_count_kernel, dim3(ceil_div(p->nblocks + 1, 256)), dim3(256), 0, stream, unit_off,
+                     p->nblocks, unit_size, p->block_first_item, p->multi_blocks, p->counters);
   KSJ_LAUNCH_CHECK();
   exclusive_scan_u32(p->block_first_item, p->block_first_item, p->nblocks + 1, p->sort_scratch, stream);
   uint32_t items = 0, multi = 0;
@@ -368,10 +500,17 @@ void plan3_set_trajectory(Plan3* p, const float* d_traj, const float* d_dcf, hip
   KSJ_HIP(hipMemcpyAsync(&multi, p->counters, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
   KSJ_HIP(hipStreamSynchronize(stream));
   if (items > p->item_capacity || p->item_info == nullptr) {
-    if (p->item_info) KSJ_HIP(hipFree(p->item_info));
-    p->item_info = nullptr;
+    for (void** q : {reinterpret_cast<void**>(&p->item_info), reinterpret_cast<void**>(&p->item_cost),
+                     &p->item_scratch}) {
+      if (*q) KSJ_HIP(hipFree(*q));
+      *q = nullptr;
+    }
     p->item_capacity = std::max(items, p->item_capacity);
     KSJ_HIP(hipMalloc(&p->item_info, std::max<size_t>(1, p->item_capacity) * sizeof(uint4)));
+    if (matrix) {
+      KSJ_HIP(hipMalloc(&p->item_cost, (static_cast<size_t>(p->item_capacity) + 1) * sizeof(uint32_t)));
+      KSJ_HIP(hipMalloc(&p->item_scratch, std::max<size_t>(256, scan_scratch_bytes(p->item_capacity + 1ull))));
+    }
   }
   p->item_count = items;
   hipLaunchKernelGGL(slot3_count_kernel, dim3(ceil_div(p->nblocks + 1, 256)), dim3(256), 0, stream,
@@ -381,8 +520,28 @@ void plan3_set_trajectory(Plan3* p, const float* d_traj, const float* d_dcf, hip
   uint32_t slots = 0;
   KSJ_HIP(hipMemcpyAsync(&slots, p->block_first_slot + p->nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
   hipLaunchKernelGGL(item3_fill_kernel, dim3(ceil_div(p->nblocks, 256)), dim3(256), 0, stream, p->block_first_item,
-                     p->block_first_slot, p->block_visit_off, p->d.item_visits, p->nblocks, p->item_info);
+                     p->block_first_slot, unit_off, unit_size, matrix ? 4u : 1u, p->nblocks, p->item_info);
   KSJ_LAUNCH_CHECK();
+  if (matrix) {  // wave runs of equal cost over the item order
+    hipLaunchKernelGGL(item3_cost_kernel, dim3(ceil_div(items + 1ull, 256)), dim3(256), 0, stream, p->item_info,
+                       items, p->item_cost);
+    KSJ_LAUNCH_CHECK();
+    exclusive_scan_u32(p->item_cost, p->item_cost, items + 1ull, p->item_scratch, stream);
+    uint32_t total = 0;
+    KSJ_HIP(hipMemcpyAsync(&total, p->item_cost + items, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    KSJ_HIP(hipStreamSynchronize(stream));
+    const uint32_t runs = ceil_div(total, p->run_cost);
+    if (runs > p->run_capacity || p->run_first == nullptr) {
+      if (p->run_first) KSJ_HIP(hipFree(p->run_first));
+      p->run_first = nullptr;
+      p->run_capacity = std::max(runs, p->run_capacity);
+      KSJ_HIP(hipMalloc(&p->run_first, (static_cast<size_t>(p->run_capacity) + 1) * sizeof(uint32_t)));
+    }
+    p->run_count = runs;
+    hipLaunchKernelGGL(run3_split_kernel, dim3(ceil_div(runs + 1ull, 256)), dim3(256), 0, stream, p->item_cost, items,
+                       runs, p->run_cost, p->run_first);
+    KSJ_LAUNCH_CHECK();
+  }
   KSJ_HIP(hipStreamSynchronize(stream));
   p->slot_count = slots;
   p->multi_count = multi;

@@ -9,7 +9,10 @@
 //                per axis, FP64; dcf = 1, |w|^2 or given.
 //   taps         per sorted sample W taps per axis (z, y, x), FP64 rounded to float.
 //   schedule     per 2x2x2 cell block, the (sample, eight weights) visits of its
-//                (W+1)^3 window bins, cut into work items of at most item_visits.
+//                (W+1)^3 window bins; or (matrix schedule) per 4x4x4 cell block the
+//                (sample, window position) visits of its (W+3)^3 window bins, padded
+//                to groups of four, with the weights formed from the taps in the
+//                gather; cut into work items of at most item_visits.
 //   deapod       1 / phi_hat per output index of each axis, FP64.
 // Volumes are [z][y][x]; trajectories [S][3] in (z, y, x) order, radians per pixel.
 #pragma once
@@ -29,7 +32,13 @@ struct Plan3Desc {
   uint32_t density = kDensityRampSquared;  // kDensityNone, kDensityRampSquared or kDensityGiven
   uint32_t samples = 0;
   uint32_t item_visits = 64;
+  uint32_t scheme = kSchemeDefault;  // kSchemeBlocks (2x2x2) or kSchemeMatrix (4x4x4); default: tuning key
+  uint32_t run_cost = 0;             // matrix: cost per wave run, 1..64 (0: tuning key)
 };
+
+// Window position of a matrix-schedule visit: rz | ry << 4 | rx << 8; a padding visit
+// has rz = 15, outside every window, so all its weights are zero.
+constexpr uint32_t kPadVisitPos = 15;
 
 struct Plan3 {
   Plan3Desc d{};
@@ -42,12 +51,16 @@ struct Plan3 {
   uint32_t* bins = nullptr;        // [S] bins (original order)
   uint32_t* cell_start = nullptr;  // [nbins + 1]
   float* taps = nullptr;           // [S][3W] sorted: z, y, x taps
-  // Schedule over 2x2x2 blocks (block (bz, by, bx) covers cells 2b .. 2b+1 per axis).
+  // Schedule over s x s x s blocks (block (bz, by, bx) covers cells s b .. s b + s - 1
+  // per axis; s = 2, or 4 for the matrix schedule).
+  uint32_t scheme = kSchemeBlocks, side = 2;
+  uint32_t item_visits = 64;             // a multiple of four for the matrix schedule
   uint32_t blocks[3] = {0, 0, 0};
   uint64_t nblocks = 0;
   uint32_t* block_visit_off = nullptr;  // [nblocks + 1]
   uint32_t* visit_k = nullptr;          // [visit_capacity]
-  float4* visit_w = nullptr;            // [2 * visit_capacity]: cells (cz, cy, cx) in cz*4 + cy*2 + cx order
+  float4* visit_w = nullptr;            // 2x2x2: [2 * visit_capacity], cells (cz, cy, cx) in cz*4 + cy*2 + cx order
+  uint32_t* visit_pos = nullptr;        // matrix: [visit_capacity] window position (kPadVisitPos: padding)
   uint64_t visit_capacity = 0, visit_count = 0;
   uint32_t* block_first_item = nullptr;  // [nblocks + 1]
   uint32_t* block_first_slot = nullptr;  // [nblocks + 1]
@@ -55,6 +68,14 @@ struct Plan3 {
   uint32_t item_capacity = 0, item_count = 0, slot_count = 0;
   uint32_t* multi_blocks = nullptr;      // [nblocks]
   uint32_t multi_count = 0;
+  // Matrix schedule: groups of four per block, item cost prefix (groups + 1 per
+  // item) and the wave runs cut from it (at most 64 items each).
+  uint32_t* block_chunk_off = nullptr;   // [nblocks + 1]
+  uint32_t* item_cost = nullptr;         // [item_capacity + 1]
+  void* item_scratch = nullptr;
+  uint32_t run_cost = 8;
+  uint32_t* run_first = nullptr;         // [run_capacity + 1]
+  uint32_t run_capacity = 0, run_count = 0;
   float* deapod[3] = {nullptr, nullptr, nullptr};
   // build scratch
   double* frac = nullptr;          // [S][3]

@@ -314,11 +314,24 @@ namespace {
 // The fused form runs every stage once for all items (one launch per stage and
 // shape class, argument tables in the workspace) when every item fits one pass of
 // the 2x2-block schedule; otherwise items go round-robin over lanes.
-bool fused_eligible(const MultiItem* items, uint32_t count) {
-  if (tune_get(kTuneMultiFused) == 0 || count > kMaxFusedItems) return false;
-  for (uint32_t i = 0; i < count; ++i)
-    if (items[i].plan->scheme == kSchemeTiles || items[i].channels > kMaxStackedCoils) return false;
+// The fused form takes 2x2-block and matrix plans (all matrix plans of one block
+// width) of at most 64 channels.
+bool fused_shapes_ok(const MultiItem* items, uint32_t count) {
+  if (count > kMaxFusedItems) return false;
+  uint32_t matrix_w = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    const Plan2& p = *items[i].plan;
+    if (p.scheme == kSchemeTiles || items[i].channels > kMaxStackedCoils) return false;
+    if (p.scheme == kSchemeMatrix) {
+      if (matrix_w != 0 && matrix_w != p.block_w) return false;
+      matrix_w = p.block_w;
+    }
+  }
   return true;
+}
+
+bool fused_eligible(const MultiItem* items, uint32_t count) {
+  return tune_get(kTuneMultiFused) != 0 && fused_shapes_ok(items, count);
 }
 
 size_t fused_tables_bytes(uint32_t count) {
@@ -417,12 +430,7 @@ void execute_fused(const MultiItem* items, uint32_t count, int layout, void* d_w
 size_t adjoint2_multi_workspace_bytes(const MultiItem* items, uint32_t count) {
   // Enough for either form, so a tuning change between query and call stays valid.
   size_t total = lanes_total_bytes(items, count);
-  if (count <= kMaxFusedItems) {
-    bool blocks = true;
-    for (uint32_t i = 0; i < count; ++i)
-      blocks = blocks && items[i].plan->scheme != kSchemeTiles && items[i].channels <= kMaxStackedCoils;
-    if (blocks) total = std::max(total, fused_lanes_bytes(items, count));
-  }
+  if (fused_shapes_ok(items, count)) total = std::max(total, fused_lanes_bytes(items, count));
   return total;
 }
 

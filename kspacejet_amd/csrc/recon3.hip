@@ -1,10 +1,12 @@
 // 3-D KB adjoint NUFFT on one GPU: coils in passes of up to coils_per_pass, each
-// pass prep -> 2x2x2-block gather (no float atomics) -> multi-item reduce ->
+// pass prep -> gather (2x2x2 blocks on the VALU, or 4x4x4 blocks on the matrix
+// cores; no float atomics) -> multi-item reduce ->
 // pruned 3-D inverse FFT (x over the full grid, y over the kept x columns, z over
 // the kept (y, x) lines) with deapodization and crop fused into the y and z passes
 // -> RSS accumulated across passes in channel order and/or coil-last images.
 #include "fft.hpp"
 #include "grid.hpp"
+#include "matrix.hpp"
 #include "plan3.hpp"
 #include "prof.hpp"
 #include "recon3.hpp"
@@ -22,10 +24,17 @@ struct Gather3Args {
   const uint32_t* block_first_item;
   const uint32_t* block_first_slot;
   const uint32_t* multi_blocks;
-  float2* partial;             // [slots][8][Cp]
+  float2* partial;             // [slots][side^3][Cp]
   float2* grid;                // [G_z][G_y][G_x][Cp]
   uint32_t g[3], blocks[3];
   uint32_t cp, item_count, multi_count;
+  uint32_t side;               // block side: 2, or 4 for the matrix schedule
+  // matrix schedule
+  const uint32_t* visit_pos;   // window position per visit
+  const float* taps;           // [S][3W] sorted: z, y, x taps
+  const uint32_t* run_first;   // [runs + 1]
+  uint32_t run_count;
+  int width;
 };
 
 __device__ inline uint32_t last_visit(uint32_t v, uint32_t end) {
@@ -34,10 +43,10 @@ __device__ inline uint32_t last_visit(uint32_t v, uint32_t end) {
 }
 
 __device__ inline void block_cells(const Gather3Args& a, uint32_t b, uint32_t* z, uint32_t* y, uint32_t* x) {
-  *x = 2 * (b % a.blocks[2]);
+  *x = a.side * (b % a.blocks[2]);
   const uint32_t r = b / a.blocks[2];
-  *y = 2 * (r % a.blocks[1]);
-  *z = 2 * (r / a.blocks[1]);
+  *y = a.side * (r % a.blocks[1]);
+  *z = a.side * (r / a.blocks[1]);
 }
 
 // One sub-wave of CP lanes (lane = coil) per work item, 64 / CP items per wave.
@@ -111,22 +120,157 @@ __global__ __launch_bounds__(256) void gather3_kernel(Gather3Args a) {
   }
 }
 
-// Blocks with several items: thread per (block, cell, coil) sums the partials in item order.
+// Blocks with several items: thread per (block, cell, coil) sums the partials in item
+// order.  Cell q of a block is (q / s^2, (q / s) % s, q % s).
 __global__ __launch_bounds__(256) void reduce3_kernel(Gather3Args a) {
+  const uint32_t cells = a.side * a.side * a.side;
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= static_cast<uint64_t>(a.multi_count) * 8 * a.cp) return;
+  if (i >= static_cast<uint64_t>(a.multi_count) * cells * a.cp) return;
   const uint32_t c = static_cast<uint32_t>(i % a.cp);
-  const uint32_t q = static_cast<uint32_t>((i / a.cp) % 8);
-  const uint32_t b = a.multi_blocks[i / (8ull * a.cp)];
+  const uint32_t q = static_cast<uint32_t>((i / a.cp) % cells);
+  const uint32_t b = a.multi_blocks[i / (static_cast<uint64_t>(cells) * a.cp)];
   const uint32_t s0 = a.block_first_slot[b];
   const uint32_t n = a.block_first_item[b + 1] - a.block_first_item[b];
-  const float2* p = a.partial + (static_cast<uint64_t>(s0) * 8 + q) * a.cp + c;
+  const float2* p = a.partial + (static_cast<uint64_t>(s0) * cells + q) * a.cp + c;
   float2 s = make_float2(0.f, 0.f);
-  for (uint32_t j = 0; j < n; ++j) s = cadd(s, p[static_cast<uint64_t>(j) * 8 * a.cp]);
+  for (uint32_t j = 0; j < n; ++j) s = cadd(s, p[static_cast<uint64_t>(j) * cells * a.cp]);
   uint32_t z0, y0, x0;
   block_cells(a, b, &z0, &y0, &x0);
-  const uint32_t z = z0 + q / 4, y = y0 + (q / 2) % 2, x = x0 + q % 2;
+  const uint32_t z = z0 + q / (a.side * a.side), y = y0 + (q / a.side) % a.side, x = x0 + q % a.side;
   if (z < a.g[0] && y < a.g[1] && x < a.g[2]) a.grid[((static_cast<uint64_t>(z) * a.g[1] + y) * a.g[2] + x) * a.cp + c] = s;
+}
+
+// ----------------------------------------------------------------------------- matrix gather
+// 4x4x4 blocks on the matrix cores.  For four visits at a time, each z-plane of the
+// block (16 cells) is one v_mfma_f32_16x16x4_f32 per 16 columns of the sample rows:
+// A [16 cells x 4 visits] weights, B [4 visits x 16 columns] row floats.  Lane l takes
+// visit q = l / 16 of the group and cell (y, x) = (j / 4, j % 4), j = l % 16, of every
+// plane; its four weights tz[z - rz + W - 1] ty[y - ry + W - 1] tx[x - rx + W - 1]
+// (zero outside [0, W)) come from the sample's taps and the visit's window position,
+// so a visit carries 8 bytes instead of 64 weights.  Accumulator register r of lane l
+// in plane h is cell (h, q, r).  One wave per run of consecutive items, streamed in
+// batches of U groups with the next batch's visits loaded while this batch's rows and
+// taps are in flight; an item's 64 cells are written when the stream passes its end.
+// Sums run four visits at a time in visit order: deterministic.
+template <int CP, int U>
+__device__ inline void matrix3_run(const Gather3Args& a, uint32_t run) {
+  constexpr int NF = CP / 8;
+  constexpr int T = 4;  // z-planes of a block
+  const int lane = threadIdx.x & 63, q = lane >> 4, j = lane & 15;
+  const int cy = j >> 2, cx = j & 3;
+  const uint32_t i0 = a.run_first[run], i1 = a.run_first[run + 1];  // run is wave-uniform: scalar loads
+  if (i0 >= i1) return;
+  const uint32_t nitems = i1 - i0;  // <= 64 by construction of the runs
+  const uint4 rec = static_cast<uint32_t>(lane) < nitems ? a.item_info[i0 + lane] : make_uint4(0, 0, 0, 0);
+  auto field = [&](uint32_t v, uint32_t t) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(t)));
+  };
+  const uint32_t c0 = field(rec.y, 0) / 4, c1 = field(rec.z, nitems - 1) / 4;
+  const uint32_t last_v = c1 > c0 ? 4 * c1 - 1 : 0;
+  const int W = a.width;
+  const float* rows = reinterpret_cast<const float*>(a.sorted) + j * NF;
+  MfmaAcc acc[T][NF];
+#pragma unroll
+  for (int h = 0; h < T; ++h)
+#pragma unroll
+    for (int n = 0; n < NF; ++n) acc[h][n] = MfmaAcc{0.f, 0.f, 0.f, 0.f};
+
+  auto flush = [&](uint32_t t) {
+    const uint32_t blk = field(rec.x, t), slot = field(rec.w, t);
+    uint32_t z0, y0, x0;
+    block_cells(a, blk, &z0, &y0, &x0);
+#pragma unroll
+    for (int h = 0; h < T; ++h) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float o[NF];
+#pragma unroll
+        for (int n = 0; n < NF; ++n) o[n] = acc[h][n][r];
+        if (slot == 0xFFFFFFFFu) {
+          const uint32_t z = z0 + h, y = y0 + q, x = x0 + r;
+          if (z < a.g[0] && y < a.g[1] && x < a.g[2])
+            store_piece_nt<NF>(reinterpret_cast<float*>(a.grid) +
+                                 ((static_cast<uint64_t>(z) * a.g[1] + y) * a.g[2] + x) * (2 * CP) + j * NF,
+                               o);
+        } else {
+          store_piece<NF>(reinterpret_cast<float*>(a.partial) +
+                            (static_cast<uint64_t>(slot) * 64 + 16 * h + 4 * q + r) * (2 * CP) + j * NF,
+                          o);
+        }
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < T; ++h)
+#pragma unroll
+      for (int n = 0; n < NF; ++n) acc[h][n] = MfmaAcc{0.f, 0.f, 0.f, 0.f};
+  };
+
+  uint32_t k[U], pos[U];
+  auto load_meta = [&](uint32_t c) {  // groups c + u, clamped into the run
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t vi = 4 * (c + u) + q;
+      const uint32_t vc = vi < last_v ? vi : last_v;
+      k[u] = a.visit_k[vc];
+      pos[u] = a.visit_pos[vc];
+    }
+  };
+
+  uint32_t t = 0, t_end = field(rec.z, 0) / 4;  // current item and its end group
+  uint32_t c = c0;
+  if (c < c1) load_meta(c);
+  for (; c < c1; c += U) {
+    RowPiece<NF> b[U];
+    float tz[U][T], ty[U], tx[U];
+    int rzs[U];
+    bool vyx[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      b[u] = load_piece<NF>(rows + static_cast<uint64_t>(k[u]) * (2 * CP));
+      const float* tp = a.taps + static_cast<uint64_t>(k[u]) * (3 * W);
+      const int rz = static_cast<int>(pos[u] & 15), ry = static_cast<int>((pos[u] >> 4) & 15),
+                rx = static_cast<int>((pos[u] >> 8) & 15);
+      const int iy = cy - ry + W - 1, ix = cx - rx + W - 1;
+      vyx[u] = iy >= 0 && iy < W && ix >= 0 && ix < W;
+      ty[u] = tp[W + (iy >= 0 && iy < W ? iy : 0)];
+      tx[u] = tp[2 * W + (ix >= 0 && ix < W ? ix : 0)];
+#pragma unroll
+      for (int h = 0; h < T; ++h) {
+        const int iz = h - rz + W - 1;
+        tz[u][h] = tp[iz >= 0 && iz < W ? iz : 0];
+      }
+      rzs[u] = rz;
+    }
+    load_meta(c + U);
+    // Keep every load of the batch ahead of the products (see the 2-D gather).
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (c + u < c1) {
+        while (c + u >= t_end) {  // group c + u starts a later item
+          flush(t);
+          ++t;
+          t_end = field(rec.z, t) / 4;
+        }
+        const float wyx = vyx[u] ? ty[u] * tx[u] : 0.f;
+#pragma unroll
+        for (int h = 0; h < T; ++h) {
+          const int iz = h - rzs[u] + W - 1;
+          const float wv = (iz >= 0 && iz < W) ? tz[u][h] * wyx : 0.f;
+#pragma unroll
+          for (int n = 0; n < NF; ++n)
+            acc[h][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(wv, b[u].v[n], acc[h][n], 0, 0, 0);
+        }
+      }
+    }
+  }
+  for (; t < nitems; ++t) flush(t);  // the last item with visits, then any empty ones
+}
+
+template <int CP, int U>
+__global__ __launch_bounds__(256) void gather3_matrix_kernel(Gather3Args a) {
+  const uint32_t run = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (run < a.run_count) matrix3_run<CP, U>(a, run);
 }
 
 // sumsq[p] (+)= sum over the pass's coils of |img[c][p]|^2 in channel order; on the
@@ -158,6 +302,16 @@ __global__ void coil_last3_kernel(const float2* img, uint32_t coils, uint64_t pi
 
 template <int CP>
 void launch_gather3(const Gather3Args& a, hipStream_t stream) {
+  if (a.side == 4) {
+    const uint32_t wgs = std::max<uint32_t>(1, ceil_div(a.run_count, 4));
+    switch (tune_get(kTuneMatrixBatch)) {  // visits per batch: groups of four
+      case 4: hipLaunchKernelGGL((gather3_matrix_kernel<CP, 1>), dim3(wgs), dim3(256), 0, stream, a); break;
+      case 8: hipLaunchKernelGGL((gather3_matrix_kernel<CP, 2>), dim3(wgs), dim3(256), 0, stream, a); break;
+      default: hipLaunchKernelGGL((gather3_matrix_kernel<CP, 4>), dim3(wgs), dim3(256), 0, stream, a); break;
+    }
+    KSJ_LAUNCH_CHECK();
+    return;
+  }
   constexpr int NSW = 64 / CP;
   const uint32_t wgs = std::max<uint32_t>(1, ceil_div(ceil_div(a.item_count, NSW), 4));
   hipLaunchKernelGGL((gather3_kernel<CP, 4>), dim3(wgs), dim3(256), 0, stream, a);
@@ -179,7 +333,8 @@ Layout3 layout3(const Plan3& p, uint32_t channels, uint32_t coils_per_pass) {
   Layout3 l{};
   l.sorted = align256_3(static_cast<size_t>(d.samples) * cp * sizeof(float2));
   l.grid = align256_3(G * cp * sizeof(float2));
-  l.partial = align256_3(std::max<size_t>(1, static_cast<size_t>(p.slot_count) * 8 * cp) * sizeof(float2));
+  const size_t cells = static_cast<size_t>(p.side) * p.side * p.side;
+  l.partial = align256_3(std::max<size_t>(1, static_cast<size_t>(p.slot_count) * cells * cp) * sizeof(float2));
   l.xout = align256_3(cpp * static_cast<size_t>(d.grid[0]) * d.grid[1] * d.image[2] * sizeof(float2));
   l.yout = align256_3(cpp * static_cast<size_t>(d.grid[0]) * d.image[1] * d.image[2] * sizeof(float2));
   l.zout = align256_3(cpp * P * sizeof(float2));
@@ -243,6 +398,12 @@ void adjoint3_execute(const Plan3& plan, const float2* d_kspace, uint32_t channe
     a.cp = cp;
     a.item_count = plan.item_count;
     a.multi_count = plan.multi_count;
+    a.side = plan.side;
+    a.visit_pos = plan.visit_pos;
+    a.taps = plan.taps;
+    a.run_first = plan.run_first;
+    a.run_count = plan.run_count;
+    a.width = static_cast<int>(d.width);
     {
       ProfScope ps("grid3_gather", stream);
       switch (cp) {
@@ -254,7 +415,8 @@ void adjoint3_execute(const Plan3& plan, const float2* d_kspace, uint32_t channe
     }
     if (plan.multi_count > 0) {
       ProfScope ps("grid3_reduce", stream);
-      hipLaunchKernelGGL(reduce3_kernel, dim3(ceil_div(static_cast<uint64_t>(plan.multi_count) * 8 * cp, 256)),
+      const uint64_t cells = static_cast<uint64_t>(plan.side) * plan.side * plan.side;
+      hipLaunchKernelGGL(reduce3_kernel, dim3(ceil_div(static_cast<uint64_t>(plan.multi_count) * cells * cp, 256)),
                          dim3(256), 0, stream, a);
       KSJ_LAUNCH_CHECK();
     }

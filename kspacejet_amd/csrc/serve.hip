@@ -240,7 +240,7 @@ Plan3* Server::plan3_for(const Pending& p, hipStream_t stream) {
   d.width = r.kernel_width;
   d.density = r.density;
   d.samples = r.samples;
-  d.item_visits = static_cast<uint32_t>(tune_get(kTuneItemVisits));
+  d.item_visits = 0;  // the schedule's tuning key
   Plan3Ptr plan(plan3_create(d), &plan3_destroy);
   plan3_set_trajectory(plan.get(), r.d_traj, nullptr, stream);
   PlanEntry e;

@@ -35,7 +35,7 @@
 namespace ksj::gpu {
 
 enum ServeKind : uint32_t { kServeRadial2D = 0, kServeCartesian2D = 1, kServeRadial3D = 2 };
-constexpr uint32_t kServeCoilsPerPass3D = 8;  // a 512^3 grid of 8 coils is 8.6 GB
+constexpr uint32_t kServeCoilsPerPass3D = 16;  // a 512^3 grid of 16 coils is 17.2 GB
 
 struct ServeConfig {
   int device = 0;

@@ -36,6 +36,10 @@ Param g_params[kTuneKeys] = {
     {"matrix_slab", 16, [](int64_t v) { return v >= 1 && v <= 4096; }},
     {"matrix_item_visits", 128, [](int64_t v) { return v >= 1 && v <= 4096; }},
     {"matrix_batch", 16, [](int64_t v) { return v == 4 || v == 8 || v == 16; }},
+    {"matrix_block_width", 4, [](int64_t v) { return v == 4 || v == 8; }},
+    {"gather3_scheme", 2, [](int64_t v) { return v == 0 || v == 2; }},
+    {"matrix3_item_visits", 256, [](int64_t v) { return v >= 1 && v <= 4096; }},
+    {"matrix3_run_cost", 32, [](int64_t v) { return v >= 1 && v <= 64; }},
 };
 
 }  // namespace

@@ -384,10 +384,10 @@ class Plan3:
         call("ksj_gpu_plan3_export_bins", self._handle, _ptr(start), _ptr(perm), _stream())
         return start, perm
 
-    def workspace_bytes(self, channels: int, coils_per_pass: int = 8) -> int:
+    def workspace_bytes(self, channels: int, coils_per_pass: int = 16) -> int:
         return int(self._lib.ksj_gpu_adjoint3_workspace_bytes(self._handle, channels, coils_per_pass))
 
-    def adjoint(self, kspace: torch.Tensor, images: bool = True, rss: bool = False, coils_per_pass: int = 8,
+    def adjoint(self, kspace: torch.Tensor, images: bool = True, rss: bool = False, coils_per_pass: int = 16,
                 workspace: torch.Tensor | None = None):
         """[C, S] complex64 -> (coil-last [N_z, N_y, N_x, C] complex64 or None, RSS [N_z, N_y, N_x] or None)."""
         _need(kspace, torch.complex64, "kspace")

@@ -2,7 +2,8 @@
 the volume reconstruction against the FP64 oracle (rel-L2 <= 1e-5) for unequal
 axes, every kernel width, multi-pass coil batches and work items cut short so
 blocks are summed from several partial slots; coil batching and repeat runs give
-bit-identical results."""
+bit-identical results.  Every test runs under both gather schedules: 2x2x2 blocks
+on the VALU and 4x4x4 blocks on the matrix cores."""
 import numpy as np
 import pytest
 
@@ -22,6 +23,15 @@ def ops():
     from kspacejet_amd import ops as K
     K.init(0)
     return K
+
+
+@pytest.fixture(autouse=True, params=[0, 2], ids=["blocks", "matrix"])
+def scheme(ops, request):
+    """The 3-D gather schedule of the plans each test builds."""
+    saved = ops.tune_get("gather3_scheme")
+    ops.tune_set("gather3_scheme", request.param)
+    yield request.param
+    ops.tune_set("gather3_scheme", saved)
 
 
 def kspace(coils, samples, seed):
@@ -98,3 +108,22 @@ def test_random_trajectory_to_the_grid_edge(ops):
     torch.cuda.synchronize()
     assert rel_l2(img.cpu().numpy(), oracle_volume(k, traj, shape, 4, "none")) <= TOL
     plan.close()
+
+
+def test_schedules_agree(ops, scheme):
+    """The matrix schedule sums the same products as the 2x2x2 one, in groups of four."""
+    if scheme != 2:
+        pytest.skip("compared from the matrix run")
+    shape = (20, 18, 22)
+    traj = ops.golden_means3(151, 40)
+    k = torch.from_numpy(kspace(6, traj.shape[0], 21)).cuda()
+    results = []
+    for s in (2, 0):
+        ops.tune_set("gather3_scheme", s)
+        plan = ops.Plan3(shape, traj.shape[0]).set_trajectory(traj)
+        img, _ = plan.adjoint(k, rss=True)
+        torch.cuda.synchronize()
+        results.append(img.cpu().numpy())
+        plan.close()
+    ops.tune_set("gather3_scheme", 2)
+    assert rel_l2(results[0], results[1]) <= 2e-6

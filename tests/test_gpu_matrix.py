@@ -1,4 +1,4 @@
-"""The matrix gather schedule (4x4 cell blocks summed on the matrix cores,
+"""The matrix gather schedule (4x4 or 4x8 cell blocks summed on the matrix cores,
 v_mfma_f32_16x16x4_f32): every padded coil count (8..64), kernel width 2..8, grids
 that are not a multiple of 4 or smaller than a block window (so window bins wrap
 more than once), random trajectories reaching the grid edge, work items short
@@ -28,9 +28,10 @@ def ops():
     return K
 
 
-@pytest.fixture
-def tune(ops):
-    """Sets tuning keys for one test and restores them."""
+@pytest.fixture(params=[4, 8], ids=["4x4", "4x8"])
+def tune(ops, request):
+    """Sets tuning keys for one test (matrix block width from the parameter) and
+    restores them."""
     saved = {}
 
     def set_(key, value):
@@ -38,6 +39,7 @@ def tune(ops):
             saved[key] = ops.tune_get(key)
         ops.tune_set(key, value)
 
+    set_("matrix_block_width", request.param)
     yield set_
     for key, value in saved.items():
         ops.tune_set(key, value)
@@ -77,7 +79,7 @@ def oracle_images(traj, k, n, width=4, density="ramp"):
 @pytest.mark.parametrize("n,coils,spokes,width,item_visits", [
     (64, 3, 45, 4, 0),       # Cp 8
     (48, 12, 37, 4, 0),      # Cp 16
-    (41, 20, 29, 6, 0),      # Cp 32, grid 82 (20.5 blocks per side)
+    (41, 20, 29, 6, 0),      # Cp 32, grid 82 (20.5 blocks a side; 10.25 of 8 columns)
     (37, 40, 25, 8, 0),      # Cp 64, grid 74
     (64, 16, 45, 4, 4),      # nearly every block cut into several items
     (64, 16, 45, 4, 10),     # item size rounded up to 12
