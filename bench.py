@@ -90,7 +90,8 @@ def gather_kernel_label() -> str:
     """Template instance of the 64-coil gather kernel the current tuning launches."""
     from kspacejet_amd import ops
     if ops.tune_get("gather_scheme") == 2:
-        return f"gather_matrix_kernel<64,{ops.tune_get('matrix_batch') // 4},{ops.tune_get('matrix_pipeline')}>"
+        return (f"gather_matrix_kernel<64,{ops.tune_get('matrix_batch') // 4},{ops.tune_get('matrix_pipeline')},"
+                f"{ops.tune_get('matrix_block_width') // 4}>")
     two = ops.tune_get("gather_coils_per_lane") == 2
     return f"gather_item_kernel<{32 if two else 64},{ops.tune_get('gather_batch')},{2 if two else 1}>"
 
