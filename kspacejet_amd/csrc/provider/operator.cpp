@@ -4,6 +4,8 @@
 // so a host sees the same behaviour from this module as from the CPU one.
 #include "operator.hpp"
 
+#include <hip/hip_runtime_api.h>
+
 #include <new>
 #include <string>
 
@@ -213,6 +215,23 @@ bool input_payload_ok(const ksj_input_item_view& item, uint64_t bytes, TypeId ty
          host_memory_ok(p.data, p.alignment, 64) && bytes_ok(item.metadata) && type_matches(p.type, type);
 }
 
+bool device_memory_ok(const void* p, uint32_t declared_alignment, uint32_t device_ordinal) {
+  if (!host_memory_ok(p, declared_alignment, 64)) return false;  // the same pointer and alignment rules
+  hipPointerAttribute_t attr{};
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();  // an unknown pointer is a rejection, not a sticky error
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice && attr.device == static_cast<int>(device_ordinal);
+}
+
+bool input_payload_ok_device(const ksj_input_item_view& item, uint64_t bytes, TypeId type, uint32_t device_ordinal) {
+  const ksj_payload_view& p = item.payload;
+  if (!abi_ok(&p) || p.memory_domain != KSJ_PROVIDER_MEMORY_DEVICE) return input_payload_ok(item, bytes, type);
+  return p.data != nullptr && p.byte_count == bytes && device_memory_ok(p.data, p.alignment, device_ordinal) &&
+         bytes_ok(item.metadata) && type_matches(p.type, type);
+}
+
 ksj_status two_port_batches(Firing& f, ksj_input_batch_view storage[2], const ksj_input_batch_view*& port0,
                             const ksj_input_batch_view*& port1, std::string_view lease_error,
                             std::string_view port0_error, std::string_view port1_error) {
@@ -254,7 +273,7 @@ ksj_status scratch_ok(Firing& f, uint64_t bytes, uint32_t alignment, std::string
 }
 
 ksj_status output_grant(Firing& f, uint64_t bytes, TypeId type, std::string_view output_error, GrantGuard& guard,
-                        ksj_mutable_payload_view& out) {
+                        ksj_mutable_payload_view& out, bool allow_device) {
   ksj_output_grant* grant = nullptr;
   const ksj_status st = f.cb->acquire_output_grant(f.cb->host_context, f.lease, 0, &grant, f.error);
   if (st != KSJ_STATUS_OK) return st;
@@ -266,9 +285,12 @@ ksj_status output_grant(Firing& f, uint64_t bytes, TypeId type, std::string_view
   const ksj_status ms = f.cb->output_grants->map_mutable_payload(f.cb->output_grants->host_context, grant, &out, f.error);
   if (ms != KSJ_STATUS_OK) return ms;
   if (!abi_ok(&out) || out.data == nullptr || out.capacity_bytes < bytes || out.committed_bytes != 0 ||
-      out.memory_domain != KSJ_PROVIDER_MEMORY_HOST_PAGEABLE || !host_memory_ok(out.data, out.alignment, 64) ||
       !type_matches(out.type, type))
     return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, output_error);
+  const bool host = out.memory_domain == KSJ_PROVIDER_MEMORY_HOST_PAGEABLE && host_memory_ok(out.data, out.alignment, 64);
+  const bool dev = allow_device && out.memory_domain == KSJ_PROVIDER_MEMORY_DEVICE &&
+                   device_memory_ok(out.data, out.alignment, f.op.device_ordinal);
+  if (!host && !dev) return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, output_error);
   return KSJ_STATUS_OK;
 }
 

@@ -97,6 +97,11 @@ bool single_item_batch(const ksj_input_batch_view& batch, uint32_t port);
 // Payload of one input item: exact bytes, host-pageable, 64-byte aligned,
 // valid metadata view and the exact registry type.
 bool input_payload_ok(const ksj_input_item_view& item, uint64_t bytes, TypeId type);
+// As input_payload_ok, also accepting a KSJ_PROVIDER_MEMORY_DEVICE payload that lies in
+// device memory of GPU `device_ordinal` (64-byte aligned): the device edge of the
+// operators that opt in.
+bool input_payload_ok_device(const ksj_input_item_view& item, uint64_t bytes, TypeId type, uint32_t device_ordinal);
+bool device_memory_ok(const void* p, uint32_t declared_alignment, uint32_t device_ordinal);
 
 // Two input batches resolved by input_port (0 and 1), not by batch index.
 // On success `first`/`second` point into `storage`.
@@ -110,7 +115,7 @@ ksj_status scratch_ok(Firing& f, uint64_t bytes, uint32_t alignment, std::string
 // Acquires and maps the output grant of port 0 and validates the mapping.  On
 // failure a mapped-but-invalid grant is released by `guard`.
 ksj_status output_grant(Firing& f, uint64_t bytes, TypeId type, std::string_view output_error, GrantGuard& guard,
-                        ksj_mutable_payload_view& out);
+                        ksj_mutable_payload_view& out, bool allow_device = false);
 
 // Seals the output (one item, exact bytes, identity keys of `key_item`,
 // metadata either copied from `key_item` or empty) and writes DONE.

@@ -5,12 +5,37 @@
 // libksj_gpu.so on the operator's stream, and copies the result into the grant.
 #include "ops.hpp"
 
+#include <vector>
+
 #include <cmath>
 #include <numbers>
 
 namespace ksj::prov {
 
 namespace {
+
+// Device-resident payloads (KSJ_PROVIDER_MEMORY_DEVICE) of the operators that accept them.
+bool on_device(const ksj_payload_view& p) { return p.memory_domain == KSJ_PROVIDER_MEMORY_DEVICE; }
+bool on_device(const ksj_mutable_payload_view& p) { return p.memory_domain == KSJ_PROVIDER_MEMORY_DEVICE; }
+
+// Device pointer of an input payload: in place when device-resident, else staged into `slot`.
+const void* device_input(DeviceSession& d, DeviceSession::Slot slot, const ksj_payload_view& p, uint64_t bytes) {
+  if (on_device(p)) return p.data;
+  void* staged = d.buffer(slot, bytes);
+  d.upload(staged, p.data, bytes);
+  return staged;
+}
+
+// Device buffer the operator writes its output to, and the hand-over once it is done.
+void* device_output(DeviceSession& d, const ksj_mutable_payload_view& out, uint64_t bytes) {
+  return on_device(out) ? out.data : d.buffer(DeviceSession::kOutput, bytes);
+}
+void finish_output(DeviceSession& d, const ksj_mutable_payload_view& out, const void* d_out, uint64_t bytes) {
+  if (on_device(out))
+    d.synchronize();
+  else
+    d.download(out.data, d_out, bytes);
+}
 
 constexpr uint32_t kMaxDim = 512;
 constexpr uint32_t kMaxChannels = 64;
@@ -124,24 +149,23 @@ ksj_status process(Firing& f) {
   if (ksj_status st = f.cb->get_input_batch(f.cb->host_context, f.lease, 0, &batch, f.error)) return st;
   if (!single_item_batch(batch, 0)) return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, kFrame);
   const ksj_input_item_view& in = batch.items[0];
-  if (!input_payload_ok(in, in_bytes, TypeId::kKspaceFrame))
+  if (!input_payload_ok_device(in, in_bytes, TypeId::kKspaceFrame, f.op.device_ordinal))
     return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, kFrame);
   GrantGuard guard;
   ksj_mutable_payload_view out;
-  if (ksj_status st = output_grant(f, out_bytes, TypeId::kImageFrame, kOutput, guard, out)) return st;
+  if (ksj_status st = output_grant(f, out_bytes, TypeId::kImageFrame, kOutput, guard, out, true)) return st;
 
   DeviceSession& d = device(f);
-  void* d_in = d.buffer(DeviceSession::kInput, in_bytes);
+  const void* d_in = device_input(d, DeviceSession::kInput, in.payload, in_bytes);
   void* d_coils = d.buffer(DeviceSession::kInput2, in_bytes);
-  void* d_out = d.buffer(DeviceSession::kOutput, out_bytes);
+  void* d_out = device_output(d, out, out_bytes);
   const size_t ws_bytes = ksj_gpu_ifft2_workspace_bytes(s.channels, s.rows, s.cols);
   void* ws = d.buffer(DeviceSession::kWorkspace, ws_bytes);
-  d.upload(d_in, in.payload.data, in_bytes);
   check_gpu(ksj_gpu_ifft2_coil_images(d_in, d_coils, s.channels, s.rows, s.cols, static_cast<int>(s.centred),
                                       KSJ_GPU_LAYOUT_COIL_LAST, ws, ws_bytes, d.stream()),
             "ifft2");
   check_gpu(ksj_gpu_rss(d_coils, static_cast<float*>(d_out), pixels(s), s.channels, d.stream()), "rss");
-  d.download(out.data, d_out, out_bytes);
+  finish_output(d, out, d_out, out_bytes);
   return seal_and_finish(f, guard, out, out_bytes, in, false, 1, info.terminal_epoch);
 }
 }  // namespace cart_rss
@@ -216,22 +240,20 @@ ksj_status process(Firing& f) {
   if (ksj_status st = two_port_batches(f, storage, images, maps, kLease, kImages, kMaps)) return st;
   const ksj_input_item_view& img = images->items[0];
   const ksj_input_item_view& map = maps->items[0];
-  if (!input_payload_ok(img, in_bytes, TypeId::kCoilImageFrame))
+  if (!input_payload_ok_device(img, in_bytes, TypeId::kCoilImageFrame, f.op.device_ordinal))
     return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, kImages);
-  if (!input_payload_ok(map, in_bytes, TypeId::kSensitivityMap))
+  if (!input_payload_ok_device(map, in_bytes, TypeId::kSensitivityMap, f.op.device_ordinal))
     return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, kMaps);
   GrantGuard guard;
   ksj_mutable_payload_view out;
-  if (ksj_status st = output_grant(f, out_bytes, TypeId::kComplexImageFrame, kOutput, guard, out)) return st;
+  if (ksj_status st = output_grant(f, out_bytes, TypeId::kComplexImageFrame, kOutput, guard, out, true)) return st;
 
   DeviceSession& d = device(f);
-  void* d_img = d.buffer(DeviceSession::kInput, in_bytes);
-  void* d_map = d.buffer(DeviceSession::kInput2, in_bytes);
-  void* d_out = d.buffer(DeviceSession::kOutput, out_bytes);
-  d.upload(d_img, img.payload.data, in_bytes);
-  d.upload(d_map, map.payload.data, in_bytes);
+  const void* d_img = device_input(d, DeviceSession::kInput, img.payload, in_bytes);
+  const void* d_map = device_input(d, DeviceSession::kInput2, map.payload, in_bytes);
+  void* d_out = device_output(d, out, out_bytes);
   check_gpu(ksj_gpu_sense_combine(d_img, d_map, d_out, pixels(s), s.channels, d.stream()), "sensitivity combine");
-  d.download(out.data, d_out, out_bytes);
+  finish_output(d, out, d_out, out_bytes);
   return seal_and_finish(f, guard, out, out_bytes, img, false, 2, info.terminal_epoch);
 }
 }  // namespace sense
@@ -260,26 +282,44 @@ bool trajectory_ok(const float* v, uint64_t n, TrajectoryRule rule) {
 struct NoncartesianInputs {
   const ksj_input_item_view* kspace = nullptr;
   const ksj_input_item_view* trajectory = nullptr;
+  const float* trajectory_host = nullptr;  // the trajectory values on the host
+  std::vector<float> trajectory_copy;      // host copy of a device-resident trajectory
   ksj_firing_lease_info info{};
   ksj_input_batch_view storage[2];
 };
 
+// `allow_device`: the payloads may also be device-resident on the operator's GPU
+// (KSJ_PROVIDER_MEMORY_DEVICE); a device trajectory is copied to the host for its
+// value check and for the plan's trajectory key.
 ksj_status noncartesian_inputs(Firing& f, const NoncartesianTexts& t, uint64_t output_bytes, uint64_t scratch_bytes,
-                               bool check_scratch, TrajectoryRule rule, NoncartesianInputs& in) {
+                               bool check_scratch, TrajectoryRule rule, NoncartesianInputs& in,
+                               bool allow_device = false) {
   const Shape& s = f.op.shape;
   if (ksj_status st = lease_info(f, 2, output_bytes, scratch_bytes, check_scratch, t.lease, in.info)) return st;
   const ksj_input_batch_view *k = nullptr, *tr = nullptr;
   if (ksj_status st = two_port_batches(f, in.storage, k, tr, t.lease, t.kspace, t.trajectory)) return st;
   in.kspace = &k->items[0];
   in.trajectory = &tr->items[0];
-  if (!input_payload_ok(*in.kspace, kspace_bytes(s), TypeId::kNoncartesianKspaceFrame))
+  auto payload_ok = [&](const ksj_input_item_view& item, uint64_t bytes, TypeId type) {
+    return allow_device ? input_payload_ok_device(item, bytes, type, f.op.device_ordinal)
+                        : input_payload_ok(item, bytes, type);
+  };
+  if (!payload_ok(*in.kspace, kspace_bytes(s), TypeId::kNoncartesianKspaceFrame))
     return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, t.kspace);
-  if (!input_payload_ok(*in.trajectory, trajectory_bytes(s), TypeId::kTrajectoryFrame))
+  if (!payload_ok(*in.trajectory, trajectory_bytes(s), TypeId::kTrajectoryFrame))
     return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, t.trajectory);
   if (in.kspace->semantic_key_hash != in.trajectory->semantic_key_hash ||
       in.kspace->order_key != in.trajectory->order_key)
     return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, t.identity);
-  if (!trajectory_ok(static_cast<const float*>(in.trajectory->payload.data), uint64_t{s.samples} * 2, rule))
+  in.trajectory_host = static_cast<const float*>(in.trajectory->payload.data);
+  if (on_device(in.trajectory->payload)) {
+    in.trajectory_copy.resize(uint64_t{s.samples} * 2);
+    check_hip(hipMemcpy(in.trajectory_copy.data(), in.trajectory->payload.data, trajectory_bytes(s),
+                        hipMemcpyDeviceToHost),
+              "trajectory copy");
+    in.trajectory_host = in.trajectory_copy.data();
+  }
+  if (!trajectory_ok(in.trajectory_host, uint64_t{s.samples} * 2, rule))
     return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, t.trajectory);
   return KSJ_STATUS_OK;
 }
@@ -451,12 +491,12 @@ ksj_status run(Firing& f, bool rss_output) {
   const Shape& s = f.op.shape;
   const uint64_t bytes = rss_output ? pixels(s) * 4 : coil_image_bytes(s);
   NoncartesianInputs in;
-  if (ksj_status st = noncartesian_inputs(f, kTexts, bytes, 0, false, TrajectoryRule::kRadiansPerPixel, in))
+  if (ksj_status st = noncartesian_inputs(f, kTexts, bytes, 0, false, TrajectoryRule::kRadiansPerPixel, in, true))
     return st;
   GrantGuard guard;
   ksj_mutable_payload_view out;
-  if (ksj_status st =
-          output_grant(f, bytes, rss_output ? TypeId::kImageFrame : TypeId::kCoilImageFrame, kTexts.output, guard, out))
+  if (ksj_status st = output_grant(f, bytes, rss_output ? TypeId::kImageFrame : TypeId::kCoilImageFrame, kTexts.output,
+                                   guard, out, true))
     return st;
 
   DeviceSession& d = device(f);
@@ -473,16 +513,17 @@ ksj_status run(Firing& f, bool rss_output) {
   desc.samples = s.samples;
   desc.row_origin = (s.rows - 1) / 2.0;
   desc.col_origin = (s.cols - 1) / 2.0;
-  ksj_gpu_plan2* plan = d.plan(desc, static_cast<const float*>(in.trajectory->payload.data));
-  void* d_k = d.buffer(DeviceSession::kInput, kspace_bytes(s));
-  void* d_out = d.buffer(DeviceSession::kOutput, bytes);
+  ksj_gpu_plan2* plan = d.plan(desc, in.trajectory_host);
+  // Device-resident payloads are read and written in place; host ones go through the
+  // session's staging.
+  const void* d_k = device_input(d, DeviceSession::kInput, in.kspace->payload, kspace_bytes(s));
+  void* d_out = device_output(d, out, bytes);
   const size_t ws_bytes = ksj_gpu_adjoint2_workspace_bytes(plan, s.channels);
   void* ws = d.buffer(DeviceSession::kWorkspace, ws_bytes);
-  d.upload(d_k, in.kspace->payload.data, kspace_bytes(s));
   check_gpu(ksj_gpu_adjoint2_execute(plan, d_k, s.channels, rss_output ? nullptr : d_out, KSJ_GPU_LAYOUT_COIL_LAST,
                                      rss_output ? static_cast<float*>(d_out) : nullptr, ws, ws_bytes, d.stream()),
             "kaiser-bessel gridding");
-  d.download(out.data, d_out, bytes);
+  finish_output(d, out, d_out, bytes);
   return seal_and_finish(f, guard, out, bytes, *in.kspace, true, 2, in.info.terminal_epoch);
 }
 ksj_status process_images(Firing& f) { return run(f, false); }

@@ -355,10 +355,20 @@ def type_descriptor(type_ref: str) -> TypeDescriptorView:
 
 # ---- firing lease ----
 
+def _address(data) -> int:
+    """Address of a payload: a numpy array (host) or a torch tensor (host or device)."""
+    return data.data_ptr() if hasattr(data, "data_ptr") else data.ctypes.data
+
+
+def _nbytes(data) -> int:
+    return data.numel() * data.element_size() if hasattr(data, "data_ptr") else data.nbytes
+
+
 @dataclasses.dataclass
 class Item:
-    """One input item: a host payload with its registry type and identity keys."""
-    data: np.ndarray
+    """One input item: a payload (numpy array, or a device tensor with
+    memory_domain=MEMORY_DEVICE) with its registry type and identity keys."""
+    data: object
     type_ref: str
     semantic_key_hash: int = 71
     order_key: int = 73
@@ -395,7 +405,11 @@ class Lease:
                  LEASE_CAP_SCRATCH):
         self.batches = list(batches)
         self.output_spec = output
-        self.output = aligned_empty(output.nbytes, output.dtype)
+        if output.memory_domain == MEMORY_DEVICE:  # a device-resident output grant
+            import torch
+            self.output = torch.empty(output.nbytes, dtype=torch.uint8, device="cuda")
+        else:
+            self.output = aligned_empty(output.nbytes, output.dtype)
         self.scratch = aligned_empty(max(scratch_bytes, 1))
         self.scratch_bytes = scratch_bytes
         self.terminal_epoch = terminal_epoch
@@ -418,8 +432,8 @@ class Lease:
                 v = arr[i]
                 v.abi = header(InputItemView)
                 v.payload.abi = header(PayloadView)
-                v.payload.data = it.data.ctypes.data
-                v.payload.byte_count = it.data.nbytes if it.byte_count is None else it.byte_count
+                v.payload.data = _address(it.data)
+                v.payload.byte_count = _nbytes(it.data) if it.byte_count is None else it.byte_count
                 v.payload.memory_domain = it.memory_domain
                 v.payload.alignment = it.alignment
                 v.payload.type = it.type if it.type is not None else type_descriptor(it.type_ref)
@@ -440,7 +454,7 @@ class Lease:
             i.terminal_epoch = self.terminal_epoch
             i.input_batch_count = len(self.batches) if self.input_batch_count is None else self.input_batch_count
             i.output_grant_count = 1
-            i.reserved_output_bytes = self.output.nbytes if self.reserved_output_bytes is None else \
+            i.reserved_output_bytes = _nbytes(self.output) if self.reserved_output_bytes is None else \
                 self.reserved_output_bytes
             i.reserved_scratch_bytes = self.scratch_bytes if self.reserved_scratch_bytes is None else \
                 self.reserved_scratch_bytes
@@ -482,8 +496,8 @@ class Lease:
             self.map_calls += 1
             m = MutablePayloadView()
             m.abi = header(MutablePayloadView)
-            m.data = self.output.ctypes.data
-            m.capacity_bytes = self.output.nbytes
+            m.data = _address(self.output)
+            m.capacity_bytes = _nbytes(self.output)
             m.committed_bytes = 0
             m.memory_domain = self.output_spec.memory_domain
             m.alignment = self.output_spec.alignment

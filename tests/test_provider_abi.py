@@ -485,3 +485,22 @@ def test_valid_firing_without_a_device_fails_loudly_and_releases_the_grant(modul
     assert lease.acquire_calls == 1 and lease.seal_calls == 0 and lease.release_calls == 1
     assert np.all(lease.output == 0)
     op.close()
+
+
+@pytest.mark.parametrize("module,op_id,config,ports", [
+    ("coil_combine", "coil_combine_rss", '{"channels":2,"cols":2,"rows":2}', None),
+    ("gpu_recon", "cartesian_ifft2_rss", '{"centred":false,"channels":2,"cols":2,"rows":2}', None)])
+def test_device_label_on_host_memory_is_a_contract_violation(modules, module, op_id, config, ports):
+    """The reference-named modules take host-pageable payloads only; the gpu-recon
+    operators also take device-resident ones, but only memory that really is device
+    memory of their GPU.  A host array labelled KSJ_PROVIDER_MEMORY_DEVICE is rejected
+    before any device work (so on a host without a GPU too)."""
+    op = _opened(modules, module, op_id, config)
+    x = H.aligned_copy(np.ones((2, 2, 2), np.complex64))
+    type_ref = "ksj.coil-image-frame" if module == "coil_combine" else "ksj.kspace-frame"
+    lease = H.Lease([H.Batch(0, [H.Item(x, type_ref, memory_domain=H.MEMORY_DEVICE)])],
+                    H.OutputSpec(2 * 2 * 4, "ksj.image-frame", np.float32))
+    st, r, msg = op.process(lease)
+    assert st == H.CONTRACT_VIOLATION and "payload" in msg, msg
+    assert lease.seal_calls == 0
+    op.close()
