@@ -431,11 +431,18 @@ def run_volume(torch, ops, lib, coils, coils_per_pass, steps, warmup):
 
 def run_provider_plain(scan_np, traj_np, firings, warmup):
     """The drop-in Provider operator (module gpu-recon, kb_gridding_reconstruct_rss,
-    reached through ksj_provider_query): one synchronous firing per scan with
-    host-pageable, 64-byte aligned k-space and trajectory payloads, as the
-    reference's host supplies them.  The operator stages them to the device through
-    its pinned bounce buffers, reuses its resident plan (same trajectory bytes),
-    runs the KB pipeline and copies the RSS image into the host output grant."""
+    reached through ksj_provider_query): one synchronous firing per scan.  Three edge
+    forms of the same firing, same bits out:
+      * host: host-pageable, 64-byte aligned k-space and trajectory payloads and a host
+        output grant, as the reference's host supplies them; the operator stages them
+        through its pinned bounce buffers and copies the RSS image back;
+      * device: k-space payload and output grant in HBM (KSJ_PROVIDER_MEMORY_DEVICE,
+        an upstream GPU operator's output and a downstream one's input), trajectory on
+        the host, compared bytewise with the resident plan's on the host;
+      * device_trajectory: the trajectory in HBM as well, compared with the resident
+        plan's and value-checked on the GPU; nothing crosses PCIe but a flag word.
+    The resident plan is reused in every form (same trajectory bytes)."""
+    import torch
     from kspacejet_amd import provider_host as H
     S = traj_np.shape[0]
     cfg = (f'{{"channels":{COILS},"density_compensation":"radial_analytic_ramp","image_cols":{N},"image_rows":{N},'
@@ -444,26 +451,48 @@ def run_provider_plain(scan_np, traj_np, firings, warmup):
     st, op, msg = module.create("kb_gridding_reconstruct_rss", cfg)
     if st != H.OK or op.open(0) != H.OK:
         raise RuntimeError(f"provider operator: {msg}")
-    k = H.Item(H.aligned_copy(np.asarray(scan_np, np.complex64)), "ksj.noncartesian-kspace-frame")
-    t = H.Item(H.aligned_copy(np.asarray(traj_np, np.float32)), "ksj.trajectory-frame")
-    lease = H.Lease([H.Batch(0, [k]), H.Batch(1, [t])], H.OutputSpec(N * N * 4, "ksj.image-frame", np.float32))
+    k_np = np.ascontiguousarray(scan_np, np.complex64)
+    t_np = np.ascontiguousarray(traj_np, np.float32)
 
-    def fire():
-        st, r, msg = op.process(lease)
-        if st != H.OK or r.sealed_output_count != 1:
-            raise RuntimeError(f"provider firing failed: {msg}")
+    def lease(k_dev, t_dev, o_dev):
+        def item(a, type_ref, dev):
+            if dev:
+                return H.Item(torch.from_numpy(a).cuda(), type_ref, memory_domain=H.MEMORY_DEVICE)
+            return H.Item(H.aligned_copy(a), type_ref)
+        ls = H.Lease([H.Batch(0, [item(k_np, "ksj.noncartesian-kspace-frame", k_dev)]),
+                      H.Batch(1, [item(t_np, "ksj.trajectory-frame", t_dev)])],
+                     H.OutputSpec(N * N * 4, "ksj.image-frame", np.float32,
+                                  memory_domain=H.MEMORY_DEVICE if o_dev else H.MEMORY_HOST_PAGEABLE))
+        torch.cuda.synchronize()  # device payloads complete before the operator's stream reads them
+        return ls
 
-    for _ in range(max(1, warmup)):
-        fire()
-    t0 = time.perf_counter()
-    for _ in range(firings):
-        fire()
-    dt = (time.perf_counter() - t0) / firings
+    def timed(ls):
+        def fire():
+            st, r, msg = op.process(ls)
+            if st != H.OK or r.sealed_output_count != 1:
+                raise RuntimeError(f"provider firing failed: {msg}")
+        for _ in range(max(1, warmup)):
+            fire()
+        t0 = time.perf_counter()
+        for _ in range(firings):
+            fire()
+        dt = (time.perf_counter() - t0) / firings
+        out = ls.output.cpu().numpy() if hasattr(ls.output, "data_ptr") else ls.output
+        return dt, hashlib.sha256(np.ascontiguousarray(out).tobytes()).hexdigest()
+
+    host = lease(False, False, False)
+    dt, digest = timed(host)
+    forms = {}
+    for name, dev in (("device", (True, False, True)), ("device_trajectory", (True, True, True))):
+        ddt, ddigest = timed(lease(*dev))
+        forms[name] = {"images_per_s": round(1.0 / ddt, 1), "ms_per_firing": round(1e3 * ddt, 3),
+                       "same_bits_as_host": ddigest == digest}
     op.close()
     return {"images_per_s": round(1.0 / dt, 1), "ms_per_firing": round(1e3 * dt, 3), "firings": firings,
             "operator": "org.kspacejet.gpu-recon kb_gridding_reconstruct_rss via ksj_provider_query, host-pageable "
                         "payloads staged by the operator, one synchronous firing per scan",
-            "host_bytes_per_firing": int(k.data.nbytes + t.data.nbytes + N * N * 4)}
+            "host_bytes_per_firing": int(host.batches[0].items[0].data.nbytes + t_np.nbytes + N * N * 4),
+            "device_edges": forms}
 
 
 def run_uneven(torch, ops, lib, count, steps, warmup, tune_seconds):

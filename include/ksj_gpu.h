@@ -210,6 +210,21 @@ int ksj_gpu_adjoint2_execute_multi(uint32_t count, const ksj_gpu_plan2* const* p
 int ksj_gpu_normalize_trajectory2(const float* d_raw, uint64_t samples, int units, uint32_t encoded_rows,
                                   uint32_t encoded_cols, float* d_out, void* stream);
 
+/* Trajectory check on the device, for hosts that hand trajectories over in HBM.  Writes to
+ * the device word *d_flags: KSJ_GPU_TRAJ_DIFFERS when some of the `values` floats at d_traj
+ * differ bitwise from those at d_resident (always when d_resident is NULL), and
+ * KSJ_GPU_TRAJ_INVALID when some value is not finite or lies outside [-bound, bound]
+ * (bound = FLT_MAX: finite only).  The value rule is the host check of the reference's
+ * non-Cartesian operators (providers/kspacejet-noncartesian-recon/src/operators/
+ * radial_gridding_reconstruct.cpp:219-221, finite and within [-pi, pi];
+ * noncartesian_adjoint_reconstruct.cpp:300, finite), moved to where the payload lives.
+ * No reference entry point: the reference takes host payloads only.  Asynchronous on
+ * `stream`; no memset, so it may be captured. */
+#define KSJ_GPU_TRAJ_DIFFERS 1u
+#define KSJ_GPU_TRAJ_INVALID 2u
+int ksj_gpu_trajectory_check(const float* d_traj, const float* d_resident, uint64_t values, float bound,
+                             uint32_t* d_flags, void* stream);
+
 /* ------------------------------------------------------------------ 3-D (kooshball) */
 
 /* 3-D KB gridding adjoint.  No reference counterpart: the reference reconstructs 2-D

@@ -351,6 +351,15 @@ int ksj_gpu_normalize_trajectory2(const float* d_raw, uint64_t samples, int unit
   });
 }
 
+int ksj_gpu_trajectory_check(const float* d_traj, const float* d_resident, uint64_t values, float bound,
+                             uint32_t* d_flags, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(d_flags != nullptr && (values == 0 || d_traj != nullptr), "null argument");
+    ksj::gpu::require(bound >= 0.f, "bound must be non-negative");
+    ksj::gpu::trajectory_check(d_traj, d_resident, values, bound, d_flags, as_stream(stream));
+  });
+}
+
 int ksj_gpu_golden_means3(float* d_traj, uint32_t spokes, uint32_t readout, void* stream) {
   return guarded([&] {
     ksj::gpu::require(d_traj != nullptr, "null trajectory");

@@ -398,6 +398,24 @@ __global__ void normalize_kernel(NormArgs a) {
   a.out[2 * s + 1] = col;
 }
 
+__global__ void trajectory_flags_init_kernel(uint32_t* flags, uint32_t value) { *flags = value; }
+
+// Bit 0: some value's bits differ from `resident`'s; bit 1: some value is not finite or
+// lies outside [-bound, bound] (the comparison is false for NaN).  One OR per wave.
+__global__ void trajectory_check_kernel(const float* traj, const float* resident, uint64_t values, float bound,
+                                        uint32_t* flags) {
+  uint32_t mask = 0;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < values; i += stride) {
+    const float v = traj[i];
+    if (resident && __float_as_uint(v) != __float_as_uint(resident[i])) mask |= 1u;
+    if (!(fabsf(v) <= bound)) mask |= 2u;
+  }
+  const uint32_t differs = __any(mask & 1u) ? 1u : 0u;
+  const uint32_t invalid = __any(mask & 2u) ? 2u : 0u;
+  if ((differs | invalid) != 0 && (threadIdx.x % warpSize) == 0) atomicOr(flags, differs | invalid);
+}
+
 __global__ void ramp_dcf_kernel(const float* traj, float* dcf, uint64_t samples) {
   const uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= samples) return;
@@ -767,6 +785,17 @@ void golden_angle2(float* d_traj, uint32_t spokes, uint32_t readout, hipStream_t
   const uint64_t n = static_cast<uint64_t>(spokes) * readout;
   if (n == 0) return;
   hipLaunchKernelGGL(golden_angle_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, stream, d_traj, spokes, readout);
+  KSJ_LAUNCH_CHECK();
+}
+
+void trajectory_check(const float* d_traj, const float* d_resident, uint64_t values, float bound,
+                      uint32_t* d_flags, hipStream_t stream) {
+  hipLaunchKernelGGL(trajectory_flags_init_kernel, dim3(1), dim3(1), 0, stream, d_flags, d_resident ? 0u : 1u);
+  KSJ_LAUNCH_CHECK();
+  if (values == 0) return;
+  const unsigned blocks = static_cast<unsigned>(std::min<uint64_t>(ceil_div(values, 256), 1024));
+  hipLaunchKernelGGL(trajectory_check_kernel, dim3(blocks), dim3(256), 0, stream, d_traj, d_resident, values, bound,
+                     d_flags);
   KSJ_LAUNCH_CHECK();
 }
 

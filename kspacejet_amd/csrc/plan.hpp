@@ -164,6 +164,12 @@ void golden_angle2(float* d_traj, uint32_t spokes, uint32_t readout, hipStream_t
 void normalize_trajectory2(const float* d_raw, uint64_t samples, int units, uint32_t encoded_rows,
                            uint32_t encoded_cols, float* d_out, hipStream_t stream);
 
+// Device-side trajectory check: *d_flags = bit 0 when some of the `values` floats differ
+// bitwise from d_resident (always when d_resident is null), bit 1 when some value is not
+// finite or lies outside [-bound, bound].  Asynchronous on `stream`.
+void trajectory_check(const float* d_traj, const float* d_resident, uint64_t values, float bound,
+                      uint32_t* d_flags, hipStream_t stream);
+
 // Analytic ramp DCF hypot(w_r, w_c) per sample in float (eigen_radial_gridding.cpp:279-295).
 void ramp_dcf2(const float* d_traj, float* d_dcf, uint64_t samples, hipStream_t stream);
 

@@ -90,18 +90,45 @@ void DeviceSession::download(void* h_dst, const void* d_src, size_t bytes) {
 
 void DeviceSession::synchronize() { check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
 
-ksj_gpu_plan2* DeviceSession::plan(const ksj_gpu_plan2_desc& desc, const float* h_traj) {
+bool DeviceSession::resident(const ksj_gpu_plan2_desc& desc, const float* h_traj) const {
   const size_t values = size_t{desc.samples} * 2;
-  const bool same = plan_ != nullptr && std::memcmp(&plan_desc_, &desc, sizeof(desc)) == 0 &&
-                    plan_traj_.size() == values && std::memcmp(plan_traj_.data(), h_traj, values * 4) == 0;
-  if (same) return plan_;
+  return plan_ != nullptr && std::memcmp(&plan_desc_, &desc, sizeof(desc)) == 0 && plan_traj_.size() == values &&
+         std::memcmp(plan_traj_.data(), h_traj, values * 4) == 0;
+}
+
+uint32_t DeviceSession::check_device_trajectory(const ksj_gpu_plan2_desc& desc, const float* d_traj, float bound) {
+  const bool same_desc = plan_ != nullptr && std::memcmp(&plan_desc_, &desc, sizeof(desc)) == 0 &&
+                         plan_traj_.size() == size_t{desc.samples} * 2;
+  auto* flags = static_cast<uint32_t*>(buffer(kFlags, sizeof(uint32_t)));
+  check_gpu(ksj_gpu_trajectory_check(d_traj, same_desc ? trajectory() : nullptr, uint64_t{desc.samples} * 2, bound,
+                                     flags, stream_),
+            "trajectory check");
+  // The staging buffers are idle between firings' transfers.
+  check_hip(hipMemcpyAsync(stage_[0], flags, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync D2H");
+  check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  uint32_t result = 0;
+  std::memcpy(&result, stage_[0], sizeof(result));
+  return result;
+}
+
+ksj_gpu_plan2* DeviceSession::plan(const ksj_gpu_plan2_desc& desc, const float* traj, bool on_device,
+                                   bool is_resident) {
+  if (is_resident && plan_ != nullptr) return plan_;
   if (plan_) {
     ksj_gpu_plan2_destroy(plan_);
     plan_ = nullptr;
   }
   plan_traj_.clear();
+  const size_t values = size_t{desc.samples} * 2;
   void* d_traj = buffer(kTrajectory, values * 4);
-  upload(d_traj, h_traj, values * 4);
+  std::vector<float> host(values);
+  if (on_device) {
+    check_hip(hipMemcpyAsync(d_traj, traj, values * 4, hipMemcpyDeviceToDevice, stream_), "hipMemcpyAsync D2D");
+    download(host.data(), d_traj, values * 4);
+  } else {
+    std::memcpy(host.data(), traj, values * 4);
+    upload(d_traj, traj, values * 4);
+  }
   ksj_gpu_plan2* p = nullptr;
   check_gpu(ksj_gpu_plan2_create(&desc, &p), "plan create");
   const int rc = ksj_gpu_plan2_set_trajectory(p, static_cast<const float*>(d_traj), nullptr, stream_);
@@ -111,7 +138,7 @@ ksj_gpu_plan2* DeviceSession::plan(const ksj_gpu_plan2_desc& desc, const float* 
   }
   plan_ = p;
   plan_desc_ = desc;
-  plan_traj_.assign(h_traj, h_traj + values);
+  plan_traj_ = std::move(host);
   ++plan_builds_;
   return plan_;
 }

@@ -28,7 +28,7 @@ void check_hip(hipError_t e, const char* what);     // HIP runtime status
 
 class DeviceSession {
  public:
-  enum Slot : int { kInput = 0, kInput2, kOutput, kWorkspace, kTrajectory, kSlots };
+  enum Slot : int { kInput = 0, kInput2, kOutput, kWorkspace, kTrajectory, kFlags, kSlots };
 
   explicit DeviceSession(uint32_t device_ordinal);
   ~DeviceSession();
@@ -48,11 +48,19 @@ class DeviceSession {
   void download(void* h_dst, const void* d_src, size_t bytes);
   void synchronize();
 
-  // Plan for `desc` over the host trajectory [samples][2]: reused while both
-  // the descriptor and the trajectory bytes equal those of the resident plan,
-  // otherwise rebuilt on the device.  The trajectory itself stays resident in
-  // the kTrajectory slot.
-  ksj_gpu_plan2* plan(const ksj_gpu_plan2_desc& desc, const float* h_traj);
+  // True when a plan for `desc` is resident and the host trajectory [samples][2]
+  // at `h_traj` holds its trajectory's bytes.
+  bool resident(const ksj_gpu_plan2_desc& desc, const float* h_traj) const;
+  // The same question for a trajectory in this session's device memory, answered
+  // on the GPU together with the value check (every value finite and within
+  // [-bound, bound]): returns KSJ_GPU_TRAJ_DIFFERS / KSJ_GPU_TRAJ_INVALID bits.
+  uint32_t check_device_trajectory(const ksj_gpu_plan2_desc& desc, const float* d_traj, float bound);
+  // Plan for `desc` over the trajectory at `traj` (device memory when `on_device`,
+  // else host): the resident plan when `is_resident` (the caller has found the
+  // bytes equal with one of the two calls above), otherwise rebuilt on the device.
+  // The trajectory itself stays resident in the kTrajectory slot, with a host copy
+  // for later comparisons.
+  ksj_gpu_plan2* plan(const ksj_gpu_plan2_desc& desc, const float* traj, bool on_device, bool is_resident);
   // Device copy of the trajectory of the resident plan (valid after plan()).
   const float* trajectory() const { return static_cast<const float*>(buffers_[kTrajectory]); }
   uint64_t plan_builds() const { return plan_builds_; }

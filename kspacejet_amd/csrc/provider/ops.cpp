@@ -5,9 +5,8 @@
 // libksj_gpu.so on the operator's stream, and copies the result into the grant.
 #include "ops.hpp"
 
-#include <vector>
-
 #include <cmath>
+#include <limits>
 #include <numbers>
 
 namespace ksj::prov {
@@ -282,18 +281,19 @@ bool trajectory_ok(const float* v, uint64_t n, TrajectoryRule rule) {
 struct NoncartesianInputs {
   const ksj_input_item_view* kspace = nullptr;
   const ksj_input_item_view* trajectory = nullptr;
-  const float* trajectory_host = nullptr;  // the trajectory values on the host
-  std::vector<float> trajectory_copy;      // host copy of a device-resident trajectory
+  bool resident = false;  // the trajectory's bytes are those of the session's resident plan
   ksj_firing_lease_info info{};
   ksj_input_batch_view storage[2];
 };
 
-// `allow_device`: the payloads may also be device-resident on the operator's GPU
-// (KSJ_PROVIDER_MEMORY_DEVICE); a device trajectory is copied to the host for its
-// value check and for the plan's trajectory key.
+// `plan_desc`: the gridding plan the firing will run (operators with a resident plan).
+// A trajectory whose bytes equal the resident plan's passed the value check when that
+// plan was built, so the check is skipped for it.  `allow_device`: the payloads may
+// also be device-resident on the operator's GPU (KSJ_PROVIDER_MEMORY_DEVICE); a device
+// trajectory is compared with the resident plan's and value-checked on the GPU.
 ksj_status noncartesian_inputs(Firing& f, const NoncartesianTexts& t, uint64_t output_bytes, uint64_t scratch_bytes,
                                bool check_scratch, TrajectoryRule rule, NoncartesianInputs& in,
-                               bool allow_device = false) {
+                               const ksj_gpu_plan2_desc* plan_desc = nullptr, bool allow_device = false) {
   const Shape& s = f.op.shape;
   if (ksj_status st = lease_info(f, 2, output_bytes, scratch_bytes, check_scratch, t.lease, in.info)) return st;
   const ksj_input_batch_view *k = nullptr, *tr = nullptr;
@@ -311,15 +311,18 @@ ksj_status noncartesian_inputs(Firing& f, const NoncartesianTexts& t, uint64_t o
   if (in.kspace->semantic_key_hash != in.trajectory->semantic_key_hash ||
       in.kspace->order_key != in.trajectory->order_key)
     return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, t.identity);
-  in.trajectory_host = static_cast<const float*>(in.trajectory->payload.data);
+  const auto* traj = static_cast<const float*>(in.trajectory->payload.data);
   if (on_device(in.trajectory->payload)) {
-    in.trajectory_copy.resize(uint64_t{s.samples} * 2);
-    check_hip(hipMemcpy(in.trajectory_copy.data(), in.trajectory->payload.data, trajectory_bytes(s),
-                        hipMemcpyDeviceToHost),
-              "trajectory copy");
-    in.trajectory_host = in.trajectory_copy.data();
+    if (plan_desc == nullptr) return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, t.trajectory);
+    const float bound = rule == TrajectoryRule::kRadiansPerPixel ? std::numbers::pi_v<float>
+                                                                  : std::numeric_limits<float>::max();
+    const uint32_t flags = device(f).check_device_trajectory(*plan_desc, traj, bound);
+    if (flags & KSJ_GPU_TRAJ_INVALID) return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, t.trajectory);
+    in.resident = (flags & KSJ_GPU_TRAJ_DIFFERS) == 0;
+    return KSJ_STATUS_OK;
   }
-  if (!trajectory_ok(in.trajectory_host, uint64_t{s.samples} * 2, rule))
+  in.resident = plan_desc != nullptr && f.op.device && f.op.device->resident(*plan_desc, traj);
+  if (!in.resident && !trajectory_ok(traj, uint64_t{s.samples} * 2, rule))
     return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, t.trajectory);
   return KSJ_STATUS_OK;
 }
@@ -415,16 +418,6 @@ ksj_status process(Firing& f) {
   if (!callbacks_ok(f.cb, kCaps3)) return reject(f.error, KSJ_STATUS_BAD_ABI, kBadAbi);
   const Shape& s = f.op.shape;
   const uint64_t bytes = coil_image_bytes(s);
-  NoncartesianInputs in;
-  if (ksj_status st =
-          noncartesian_inputs(f, kTexts, bytes, scratch_bytes(s), true, TrajectoryRule::kRadiansPerPixel, in))
-    return st;
-  if (ksj_status st = scratch_ok(f, scratch_bytes(s), 64, kTexts.scratch)) return st;
-  GrantGuard guard;
-  ksj_mutable_payload_view out;
-  if (ksj_status st = output_grant(f, bytes, TypeId::kCoilImageFrame, kTexts.output, guard, out)) return st;
-
-  DeviceSession& d = device(f);
   ksj_gpu_plan2_desc desc{};
   desc.struct_size = sizeof(desc);
   desc.image_rows = desc.grid_rows = s.rows;
@@ -436,7 +429,18 @@ ksj_status process(Firing& f) {
   desc.samples = s.samples;
   desc.row_origin = (s.rows - 1) / 2.0;
   desc.col_origin = (s.cols - 1) / 2.0;
-  ksj_gpu_plan2* plan = d.plan(desc, static_cast<const float*>(in.trajectory->payload.data));
+  NoncartesianInputs in;
+  if (ksj_status st = noncartesian_inputs(f, kTexts, bytes, scratch_bytes(s), true, TrajectoryRule::kRadiansPerPixel,
+                                          in, &desc))
+    return st;
+  if (ksj_status st = scratch_ok(f, scratch_bytes(s), 64, kTexts.scratch)) return st;
+  GrantGuard guard;
+  ksj_mutable_payload_view out;
+  if (ksj_status st = output_grant(f, bytes, TypeId::kCoilImageFrame, kTexts.output, guard, out)) return st;
+
+  DeviceSession& d = device(f);
+  ksj_gpu_plan2* plan =
+      d.plan(desc, static_cast<const float*>(in.trajectory->payload.data), false, in.resident);
   void* d_k = d.buffer(DeviceSession::kInput, kspace_bytes(s));
   void* d_out = d.buffer(DeviceSession::kOutput, bytes);
   const size_t ws_bytes = ksj_gpu_adjoint2_workspace_bytes(plan, s.channels);
@@ -490,16 +494,6 @@ ksj_status run(Firing& f, bool rss_output) {
   if (!callbacks_ok(f.cb, kCaps2)) return reject(f.error, KSJ_STATUS_BAD_ABI, kBadAbi);
   const Shape& s = f.op.shape;
   const uint64_t bytes = rss_output ? pixels(s) * 4 : coil_image_bytes(s);
-  NoncartesianInputs in;
-  if (ksj_status st = noncartesian_inputs(f, kTexts, bytes, 0, false, TrajectoryRule::kRadiansPerPixel, in, true))
-    return st;
-  GrantGuard guard;
-  ksj_mutable_payload_view out;
-  if (ksj_status st = output_grant(f, bytes, rss_output ? TypeId::kImageFrame : TypeId::kCoilImageFrame, kTexts.output,
-                                   guard, out, true))
-    return st;
-
-  DeviceSession& d = device(f);
   ksj_gpu_plan2_desc desc{};
   desc.struct_size = sizeof(desc);
   desc.image_rows = s.rows;
@@ -513,7 +507,19 @@ ksj_status run(Firing& f, bool rss_output) {
   desc.samples = s.samples;
   desc.row_origin = (s.rows - 1) / 2.0;
   desc.col_origin = (s.cols - 1) / 2.0;
-  ksj_gpu_plan2* plan = d.plan(desc, in.trajectory_host);
+  NoncartesianInputs in;
+  if (ksj_status st =
+          noncartesian_inputs(f, kTexts, bytes, 0, false, TrajectoryRule::kRadiansPerPixel, in, &desc, true))
+    return st;
+  GrantGuard guard;
+  ksj_mutable_payload_view out;
+  if (ksj_status st = output_grant(f, bytes, rss_output ? TypeId::kImageFrame : TypeId::kCoilImageFrame, kTexts.output,
+                                   guard, out, true))
+    return st;
+
+  DeviceSession& d = device(f);
+  ksj_gpu_plan2* plan = d.plan(desc, static_cast<const float*>(in.trajectory->payload.data),
+                               on_device(in.trajectory->payload), in.resident);
   // Device-resident payloads are read and written in place; host ones go through the
   // session's staging.
   const void* d_k = device_input(d, DeviceSession::kInput, in.kspace->payload, kspace_bytes(s));

@@ -332,6 +332,24 @@ def normalize_trajectory2(raw: torch.Tensor, units: str, encoded_rows: int = 0, 
     return out
 
 
+TRAJ_DIFFERS, TRAJ_INVALID = 1, 2
+
+
+def trajectory_check(traj: torch.Tensor, resident: "torch.Tensor | None", bound: float) -> int:
+    """ksj_gpu_trajectory_check: TRAJ_DIFFERS when `traj` differs bitwise from `resident`
+    (always when it is None), TRAJ_INVALID when a value is not finite or outside
+    [-bound, bound]."""
+    _need(traj, torch.float32, "trajectory")
+    if resident is not None:
+        _need(resident, torch.float32, "resident trajectory")
+        if resident.numel() != traj.numel():
+            raise ValueError("resident trajectory size differs")
+    flags = torch.empty(1, dtype=torch.int32, device=traj.device)
+    call("ksj_gpu_trajectory_check", _ptr(traj), _ptr(resident) if resident is not None else None, traj.numel(),
+         bound, _ptr(flags), _stream())
+    return int(flags.item())
+
+
 def golden_means3(spokes: int, readout: int, device="cuda") -> torch.Tensor:
     """Golden-means kooshball [spokes * readout, 3] (z, y, x) in radians per pixel."""
     out = torch.empty((spokes * readout, 3), dtype=torch.float32, device=device)

@@ -29,9 +29,12 @@ def opened(modules, module, op_id, config):
 
 
 def item(data, type_ref, device, keys=(71, 73)):
+    """A payload on the host or in HBM.  A device payload must be complete when the
+    firing starts (the operator runs on its own stream), hence the synchronisation."""
     if device:
-        return H.Item(torch.from_numpy(np.ascontiguousarray(data)).cuda(), type_ref, *keys,
-                      memory_domain=H.MEMORY_DEVICE)
+        t = torch.from_numpy(np.ascontiguousarray(data)).cuda()
+        torch.cuda.synchronize()
+        return H.Item(t, type_ref, *keys, memory_domain=H.MEMORY_DEVICE)
     return H.Item(H.aligned_copy(data), type_ref, *keys)
 
 
@@ -130,6 +133,76 @@ def test_device_label_on_host_memory_is_rejected(modules):
     bad = H.Item(H.aligned_copy(k), "ksj.noncartesian-kspace-frame", 71, 73, memory_domain=H.MEMORY_DEVICE)
     lease = H.Lease([H.Batch(0, [bad]), H.Batch(1, [item(traj, "ksj.trajectory-frame", False)])],
                     H.OutputSpec(n * n * 4, "ksj.image-frame", np.float32))
+    st, _, _ = op.process(lease)
+    assert st == H.CONTRACT_VIOLATION
+    op.close()
+
+
+def test_device_trajectories_switch_and_reuse_the_resident_plan(modules):
+    """Two trajectories in HBM alternate, and the first comes back at another address:
+    every firing gives the bits of the host firing with that trajectory (a plan reused
+    for the wrong trajectory, or rebuilt wrong, shows as a mismatch)."""
+    n, coils, spokes, readout = 64, 4, 41, 128
+    S = spokes * readout
+    traj_a = synth.golden_angle2(spokes, readout).astype(np.float32)
+    traj_b = np.ascontiguousarray(traj_a[::-1])  # same values, another sample order
+    rng = np.random.default_rng(8)
+    k = (rng.standard_normal((coils, S)) + 1j * rng.standard_normal((coils, S))).astype(np.complex64)
+    op = opened(modules, "gpu_recon", "kb_gridding_reconstruct_rss", kb_config(coils, n, S))
+
+    def fire(traj, dev):
+        lease = H.Lease([H.Batch(0, [item(k, "ksj.noncartesian-kspace-frame", dev)]),
+                         H.Batch(1, [item(traj, "ksj.trajectory-frame", dev)])],
+                        H.OutputSpec(n * n * 4, "ksj.image-frame",
+                                     memory_domain=H.MEMORY_DEVICE if dev else H.MEMORY_HOST_PAGEABLE))
+        return result(op, lease)
+
+    host_a, host_b = fire(traj_a, False), fire(traj_b, False)
+    assert host_a != host_b
+    for traj, want in [(traj_a, host_a), (traj_b, host_b), (traj_b, host_b), (traj_a, host_a), (traj_b, host_b)]:
+        assert fire(traj, True) == want
+    assert fire(traj_a, False) == host_a
+    op.close()
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf, np.float32(np.pi) * np.float32(1.0000001)])
+@pytest.mark.parametrize("dev", [False, True])
+def test_a_changed_trajectory_is_value_checked_again(modules, bad, dev):
+    """The value check is skipped only for the resident plan's own bytes: one value
+    changed in place (same address) to NaN, inf or just past pi is rejected, and the
+    restored trajectory fires as before."""
+    n, coils, spokes, readout = 32, 2, 21, 64
+    S = spokes * readout
+    traj = synth.golden_angle2(spokes, readout).astype(np.float32)
+    k = np.ones((coils, S), np.complex64)
+    op = opened(modules, "gpu_recon", "kb_gridding_reconstruct_rss", kb_config(coils, n, S))
+    t_item = item(traj, "ksj.trajectory-frame", dev)
+
+    def lease():
+        return H.Lease([H.Batch(0, [item(k, "ksj.noncartesian-kspace-frame", False)]), H.Batch(1, [t_item])],
+                       H.OutputSpec(n * n * 4, "ksj.image-frame", np.float32))
+
+    first = result(op, lease())
+    keep = float(traj[S // 2, 1])
+    t_item.data[S // 2, 1] = float(bad)
+    torch.cuda.synchronize()
+    st, _, _ = op.process(lease())
+    assert st == H.CONTRACT_VIOLATION
+    t_item.data[S // 2, 1] = keep
+    torch.cuda.synchronize()
+    assert result(op, lease()) == first
+    op.close()
+
+
+def test_out_of_range_device_trajectory_without_a_resident_plan(modules):
+    n, coils, spokes, readout = 32, 2, 21, 64
+    S = spokes * readout
+    traj = synth.golden_angle2(spokes, readout).astype(np.float32)
+    traj[-1, 0] = -4.0
+    op = opened(modules, "gpu_recon", "kb_gridding_reconstruct", kb_config(coils, n, S))
+    lease = H.Lease([H.Batch(0, [item(np.ones((coils, S), np.complex64), "ksj.noncartesian-kspace-frame", True)]),
+                     H.Batch(1, [item(traj, "ksj.trajectory-frame", True)])],
+                    H.OutputSpec(n * n * coils * 8, "ksj.coil-image-frame", memory_domain=H.MEMORY_DEVICE))
     st, _, _ = op.process(lease)
     assert st == H.CONTRACT_VIOLATION
     op.close()
