@@ -488,11 +488,71 @@ def run_provider_plain(scan_np, traj_np, firings, warmup):
         forms[name] = {"images_per_s": round(1.0 / ddt, 1), "ms_per_firing": round(1e3 * ddt, 3),
                        "same_bits_as_host": ddigest == digest}
     op.close()
+    forms["chain"] = run_provider_chain(module, cfg, k_np, t_np, firings, warmup)
     return {"images_per_s": round(1.0 / dt, 1), "ms_per_firing": round(1e3 * dt, 3), "firings": firings,
             "operator": "org.kspacejet.gpu-recon kb_gridding_reconstruct_rss via ksj_provider_query, host-pageable "
                         "payloads staged by the operator, one synchronous firing per scan",
             "host_bytes_per_firing": int(host.batches[0].items[0].data.nbytes + t_np.nbytes + N * N * 4),
             "device_edges": forms}
+
+
+def run_provider_chain(module, cfg, k_np, t_np, firings, warmup):
+    """Two operators in a row, the coil-image edge between them the one the device domain
+    removes from PCIe: kb_gridding_reconstruct (coil images, 256 x 256 x 16 complex64 =
+    8.4 MB) then coil_combine_sensitivity with seeded Gaussian maps.  Host edges: every
+    payload host-pageable, so the coil images go D2H out of the first operator and H2D
+    into the second.  Device edges: k-space, coil images, maps and output in HBM (the
+    trajectory stays on the host).  Same bits out."""
+    import torch
+    from kspacejet_amd import provider_host as H
+    from kspacejet_amd import synth
+    st, rec, msg = module.create("kb_gridding_reconstruct", cfg)
+    if st != H.OK or rec.open(0) != H.OK:
+        raise RuntimeError(f"provider operator: {msg}")
+    st, comb, msg = module.create("coil_combine_sensitivity", f'{{"channels":{COILS},"cols":{N},"rows":{N}}}')
+    if st != H.OK or comb.open(0) != H.OK:
+        raise RuntimeError(f"provider operator: {msg}")
+    maps = np.ascontiguousarray(synth.coil_maps2(N, COILS, synth.scan_seed(1, 0)).transpose(1, 2, 0), np.complex64)
+    coil_bytes = N * N * COILS * 8
+
+    def item(a, type_ref, dev, keys=(0, 0)):
+        if dev:
+            a = a if hasattr(a, "data_ptr") else torch.from_numpy(a).cuda()
+            return H.Item(a, type_ref, *keys, memory_domain=H.MEMORY_DEVICE)
+        return H.Item(a if a.ctypes.data % 64 == 0 else H.aligned_copy(a), type_ref, *keys)
+
+    def chain(dev):
+        dom = H.MEMORY_DEVICE if dev else H.MEMORY_HOST_PAGEABLE
+        first = H.Lease([H.Batch(0, [item(k_np, "ksj.noncartesian-kspace-frame", dev)]),
+                         H.Batch(1, [item(t_np, "ksj.trajectory-frame", False)])],
+                        H.OutputSpec(coil_bytes, "ksj.coil-image-frame", np.complex64, memory_domain=dom))
+        imgs = first.output if dev else first.output.view(np.complex64)
+        second = H.Lease([H.Batch(0, [item(imgs, "ksj.coil-image-frame", dev, (5, 6))]),
+                          H.Batch(1, [item(maps, "ksj.sensitivity-map", dev, (1, 2))])],
+                         H.OutputSpec(N * N * 8, "ksj.complex-image-frame", np.complex64, memory_domain=dom))
+        torch.cuda.synchronize()
+
+        def fire():
+            for op, ls in ((rec, first), (comb, second)):
+                st, r, msg = op.process(ls)
+                if st != H.OK or r.sealed_output_count != 1:
+                    raise RuntimeError(f"provider firing failed: {msg}")
+        for _ in range(max(1, warmup)):
+            fire()
+        t0 = time.perf_counter()
+        for _ in range(firings):
+            fire()
+        dt = (time.perf_counter() - t0) / firings
+        out = second.output.cpu().numpy() if dev else second.output
+        return dt, hashlib.sha256(np.ascontiguousarray(out).tobytes()).hexdigest()
+
+    host_dt, host_digest = chain(False)
+    dev_dt, dev_digest = chain(True)
+    rec.close()
+    comb.close()
+    return {"operators": "kb_gridding_reconstruct -> coil_combine_sensitivity",
+            "host_ms_per_image": round(1e3 * host_dt, 3), "device_ms_per_image": round(1e3 * dev_dt, 3),
+            "coil_image_bytes_off_pcie": 2 * coil_bytes, "same_bits": host_digest == dev_digest}
 
 
 def run_uneven(torch, ops, lib, count, steps, warmup, tune_seconds):

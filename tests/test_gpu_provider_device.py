@@ -206,3 +206,33 @@ def test_out_of_range_device_trajectory_without_a_resident_plan(modules):
     st, _, _ = op.process(lease)
     assert st == H.CONTRACT_VIOLATION
     op.close()
+
+
+def test_reconstruct_then_combine_over_a_device_edge(modules):
+    """kb_gridding_reconstruct hands its coil images to coil_combine_sensitivity in HBM
+    (the coil-image edge never crosses PCIe): the combined image has the bits of the
+    same chain over host-pageable edges."""
+    n, coils, spokes, readout = 64, 8, 51, 128
+    S = spokes * readout
+    traj = synth.golden_angle2(spokes, readout).astype(np.float32)
+    rng = np.random.default_rng(12)
+    k = (rng.standard_normal((coils, S)) + 1j * rng.standard_normal((coils, S))).astype(np.complex64)
+    maps = synth.coil_maps2(n, coils, synth.scan_seed(1, 3)).transpose(1, 2, 0).astype(np.complex64)
+    rec = opened(modules, "gpu_recon", "kb_gridding_reconstruct", kb_config(coils, n, S))
+    comb = opened(modules, "gpu_recon", "coil_combine_sensitivity", f'{{"channels":{coils},"cols":{n},"rows":{n}}}')
+    outs = []
+    for dev in (False, True):
+        dom = H.MEMORY_DEVICE if dev else H.MEMORY_HOST_PAGEABLE
+        first = H.Lease([H.Batch(0, [item(k, "ksj.noncartesian-kspace-frame", dev)]),
+                         H.Batch(1, [item(traj, "ksj.trajectory-frame", False)])],
+                        H.OutputSpec(n * n * coils * 8, "ksj.coil-image-frame", np.complex64, memory_domain=dom))
+        result(rec, first)
+        imgs = first.output
+        img_item = (H.Item(imgs, "ksj.coil-image-frame", 5, 6, memory_domain=H.MEMORY_DEVICE) if dev else
+                    H.Item(imgs, "ksj.coil-image-frame", 5, 6))
+        second = H.Lease([H.Batch(0, [img_item]), H.Batch(1, [item(maps, "ksj.sensitivity-map", dev, (1, 2))])],
+                         H.OutputSpec(n * n * 8, "ksj.complex-image-frame", memory_domain=dom))
+        outs.append(result(comb, second))
+    assert outs[0] == outs[1]
+    rec.close()
+    comb.close()
