@@ -225,6 +225,31 @@ int ksj_gpu_normalize_trajectory2(const float* d_raw, uint64_t samples, int unit
 int ksj_gpu_trajectory_check(const float* d_traj, const float* d_resident, uint64_t values, float bound,
                              uint32_t* d_flags, void* stream);
 
+/* ------------------------------------------------------------------ k-space conditioning */
+
+/* Channel mixing of channel-major k-space: y[o][s] = sum_i m[o][i] x[i][s], complex float,
+ * m row-major [out_channels][in_channels], x [in_channels][samples], y [out_channels][samples]
+ * (samples = rows * cols for a Cartesian frame), 1 <= channels <= 64, FP32 accumulation in
+ * input-channel order; y must not overlap x.  Replaces the ksj::linalg::matmul(W, X, Y) of
+ * providers/kspacejet-kspace-conditioning/src/operators/noise_prewhiten.cpp:141 and
+ * noncartesian_noise_prewhiten.cpp:133 (m = the ksj.noise-model whitening matrix) and of
+ * coil_compress.cpp:159 and noncartesian_coil_compress.cpp:148 (m = the
+ * ksj.coil-compression-basis). */
+int ksj_gpu_channel_mix(const void* d_matrix, const void* d_kspace, void* d_output, uint32_t out_channels,
+                        uint32_t in_channels, uint64_t samples, void* stream);
+
+/* Per-channel readout phase: y[c][r][k] = x[c][r][k] * p[c][k], every product and sum
+ * rounded to float (bit-identical to std::complex<float> multiplication of finite values).
+ * Replaces the transform_indexed of phase_correct.cpp:140-142 (p = the ksj.phase-model). */
+int ksj_gpu_phase_correct(const void* d_kspace, const void* d_phase, void* d_output, uint32_t channels,
+                          uint32_t rows, uint32_t cols, void* stream);
+
+/* Readout crop: y[l][k] = x[l][offset + k] for `lines` = channels * rows lines of in_cols
+ * values, offset + out_cols <= in_cols.  Replaces the per-line std::copy_n of
+ * readout_oversampling_remove.cpp:174-180. */
+int ksj_gpu_readout_crop(const void* d_kspace, void* d_output, uint64_t lines, uint32_t in_cols, uint32_t out_cols,
+                         uint32_t offset, void* stream);
+
 /* ------------------------------------------------------------------ 3-D (kooshball) */
 
 /* 3-D KB gridding adjoint.  No reference counterpart: the reference reconstructs 2-D

@@ -2,6 +2,7 @@
 #include "../../include/ksj_gpu.h"
 
 #include "combine.hpp"
+#include "condition.hpp"
 #include "fft.hpp"
 #include "grid.hpp"
 #include "nudft.hpp"
@@ -357,6 +358,34 @@ int ksj_gpu_trajectory_check(const float* d_traj, const float* d_resident, uint6
     ksj::gpu::require(d_flags != nullptr && (values == 0 || d_traj != nullptr), "null argument");
     ksj::gpu::require(bound >= 0.f, "bound must be non-negative");
     ksj::gpu::trajectory_check(d_traj, d_resident, values, bound, d_flags, as_stream(stream));
+  });
+}
+
+int ksj_gpu_channel_mix(const void* d_matrix, const void* d_kspace, void* d_output, uint32_t out_channels,
+                        uint32_t in_channels, uint64_t samples, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(samples == 0 || (d_matrix && d_kspace && d_output), "null argument");
+    ksj::gpu::require(samples == 0 || d_kspace != d_output, "output must not overlap the input");
+    ksj::gpu::channel_mix(static_cast<const float2*>(d_matrix), static_cast<const float2*>(d_kspace),
+                          static_cast<float2*>(d_output), out_channels, in_channels, samples, as_stream(stream));
+  });
+}
+
+int ksj_gpu_phase_correct(const void* d_kspace, const void* d_phase, void* d_output, uint32_t channels,
+                          uint32_t rows, uint32_t cols, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(uint64_t{channels} * rows * cols == 0 || (d_kspace && d_phase && d_output), "null argument");
+    ksj::gpu::phase_correct(static_cast<const float2*>(d_kspace), static_cast<const float2*>(d_phase),
+                            static_cast<float2*>(d_output), channels, rows, cols, as_stream(stream));
+  });
+}
+
+int ksj_gpu_readout_crop(const void* d_kspace, void* d_output, uint64_t lines, uint32_t in_cols, uint32_t out_cols,
+                         uint32_t offset, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(lines == 0 || (d_kspace && d_output), "null argument");
+    ksj::gpu::readout_crop(static_cast<const float2*>(d_kspace), static_cast<float2*>(d_output), lines, in_cols,
+                           out_cols, offset, as_stream(stream));
   });
 }
 

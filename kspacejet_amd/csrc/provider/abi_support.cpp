@@ -23,14 +23,15 @@ struct TypeEntry {
 const TypeEntry kTypes[] = {
 #include "types.inc"
 };
-static_assert(sizeof(kTypes) / sizeof(kTypes[0]) == 7, "one entry per TypeId");
+constexpr int kTypeCount = static_cast<int>(sizeof(kTypes) / sizeof(kTypes[0]));
+static_assert(kTypeCount == static_cast<int>(TypeId::kCoilCompressionBasis) + 1, "one entry per TypeId");
 
 // Dimension-name views of each type, built once (static storage: descriptors
 // handed out by type_descriptor() borrow them for the life of the module).
 struct DimViews {
-  ksj_utf8_view v[7][8];
+  ksj_utf8_view v[kTypeCount][8];
   DimViews() {
-    for (int t = 0; t < 7; ++t)
+    for (int t = 0; t < kTypeCount; ++t)
       for (uint32_t d = 0; d < kTypes[t].rank; ++d) v[t][d] = utf8(kTypes[t].dims[d]);
   }
 };
@@ -227,10 +228,12 @@ bool parse_config(const ksj_byte_view& config, std::initializer_list<ConfigField
       if (!matched) return false;
       continue;
     }
-    // Canonical positive integer: digits only, no leading zero, fits in uint32.
+    // Canonical integer: digits only, no leading zero, fits in uint32; a lone "0"
+    // only where the field's range starts at zero.
     size_t end = pos;
     while (end < text.size() && text[end] >= '0' && text[end] <= '9') ++end;
-    if (end == pos || text[pos] == '0' || end - pos > 10) return false;
+    if (end == pos || end - pos > 10) return false;
+    if (text[pos] == '0' && (f.min != 0 || end - pos != 1)) return false;
     uint64_t value = 0;
     for (size_t i = pos; i < end; ++i) value = value * 10 + static_cast<uint64_t>(text[i] - '0');
     if (value > UINT32_MAX || value < f.min || value > f.max) return false;

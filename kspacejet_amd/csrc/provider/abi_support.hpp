@@ -51,6 +51,9 @@ enum class TypeId : int {
   kImageFrame,
   kComplexImageFrame,
   kSensitivityMap,
+  kNoiseModel,
+  kPhaseModel,
+  kCoilCompressionBasis,
 };
 // The full registry descriptor of `id` (what a host places in a payload view).
 ksj_type_descriptor_view type_descriptor(TypeId id);
@@ -60,8 +63,9 @@ ksj_type_descriptor_view type_descriptor(TypeId id);
 bool type_matches(const ksj_type_descriptor_view& t, TypeId id);
 
 // One field of a canonical config object.  Canonical means keys in a fixed
-// (sorted) order, no whitespace, integers without sign or leading zeros, and
-// literal values compared byte for byte (string literals include their quotes).
+// (sorted) order, no whitespace, integers without sign or leading zeros (zero
+// itself, written "0", only in a field whose minimum is 0), and literal values
+// compared byte for byte (string literals include their quotes).
 struct ConfigField {
   std::string_view key;
   uint32_t min = 0, max = 0;       // integer range (ignored for literals)

@@ -33,6 +33,9 @@ struct Shape {
   uint32_t kernel_width = 0;
   uint32_t oversampling = 0;
   uint32_t centred = 0;
+  uint32_t out_channels = 0;  // k-space conditioning: virtual channels of a coil compression
+  uint32_t out_cols = 0;      // k-space conditioning: readout columns kept by the crop, and
+  uint32_t offset = 0;        // the first of them
 };
 
 }  // namespace ksj::prov

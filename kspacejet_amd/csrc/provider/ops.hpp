@@ -12,6 +12,13 @@ extern const OpSpec kCoilCombineRss;            // coil_combine_rss
 // org.kspacejet.noncartesian-recon
 extern const OpSpec kNoncartesianAdjoint;       // noncartesian_adjoint_reconstruct
 extern const OpSpec kRadialGridding;            // radial_gridding_reconstruct
+// org.kspacejet.kspace-conditioning
+extern const OpSpec kNoisePrewhiten;             // noise_prewhiten
+extern const OpSpec kPhaseCorrect;               // phase_correct
+extern const OpSpec kCoilCompress;               // coil_compress
+extern const OpSpec kReadoutOversamplingRemove;  // readout_oversampling_remove
+extern const OpSpec kNoncartesianNoisePrewhiten; // noncartesian_noise_prewhiten
+extern const OpSpec kNoncartesianCoilCompress;   // noncartesian_coil_compress
 // org.kspacejet.gpu-recon (no reference implementation)
 extern const OpSpec kKbGridding;                // kb_gridding_reconstruct
 extern const OpSpec kKbGriddingRss;             // kb_gridding_reconstruct_rss

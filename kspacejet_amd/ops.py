@@ -350,6 +350,41 @@ def trajectory_check(traj: torch.Tensor, resident: "torch.Tensor | None", bound:
     return int(flags.item())
 
 
+def channel_mix(matrix: torch.Tensor, kspace: torch.Tensor) -> torch.Tensor:
+    """complex64 matrix [out, in] @ channel-major k-space [in, ...] (ksj_gpu_channel_mix):
+    noise prewhitening with a whitening matrix, coil compression with a basis."""
+    _need(matrix, torch.complex64, "matrix")
+    _need(kspace, torch.complex64, "k-space")
+    out_ch, in_ch = matrix.shape
+    if kspace.shape[0] != in_ch:
+        raise ValueError("k-space channels differ from the matrix columns")
+    out = torch.empty((out_ch,) + tuple(kspace.shape[1:]), dtype=torch.complex64, device=kspace.device)
+    call("ksj_gpu_channel_mix", _ptr(matrix), _ptr(kspace), _ptr(out), out_ch, in_ch, kspace[0].numel(), _stream())
+    return out
+
+
+def phase_correct(kspace: torch.Tensor, phase: torch.Tensor) -> torch.Tensor:
+    """complex64 k-space [C, R, K] times phase [C, K] per line (ksj_gpu_phase_correct)."""
+    _need(kspace, torch.complex64, "k-space")
+    _need(phase, torch.complex64, "phase")
+    c, r, k = kspace.shape
+    if tuple(phase.shape) != (c, k):
+        raise ValueError("phase must be [channels, cols]")
+    out = torch.empty_like(kspace)
+    call("ksj_gpu_phase_correct", _ptr(kspace), _ptr(phase), _ptr(out), c, r, k, _stream())
+    return out
+
+
+def readout_crop(kspace: torch.Tensor, out_cols: int, offset: int) -> torch.Tensor:
+    """complex64 k-space [..., in_cols] -> [..., offset:offset + out_cols] (ksj_gpu_readout_crop)."""
+    _need(kspace, torch.complex64, "k-space")
+    in_cols = kspace.shape[-1]
+    out = torch.empty(tuple(kspace.shape[:-1]) + (out_cols,), dtype=torch.complex64, device=kspace.device)
+    call("ksj_gpu_readout_crop", _ptr(kspace), _ptr(out), kspace.numel() // in_cols, in_cols, out_cols, offset,
+         _stream())
+    return out
+
+
 def golden_means3(spokes: int, readout: int, device="cuda") -> torch.Tensor:
     """Golden-means kooshball [spokes * readout, 3] (z, y, x) in radians per pixel."""
     out = torch.empty((spokes * readout, 3), dtype=torch.float32, device=device)

@@ -630,3 +630,51 @@ int ora_normalize_trajectory2(size_t samples, const float* raw, int units, uint3
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------ k-space conditioning */
+
+void ora_channel_mix(size_t out_channels, size_t in_channels, size_t samples, const double* w, const double* x,
+                     double* y) {
+  /* noncartesian_noise_prewhiten.cpp:126-133 (noise_prewhiten.cpp:141, coil_compress.cpp:159,
+   * noncartesian_coil_compress.cpp:148):
+   * ksj::linalg::matmul(W, X, Y) over row-major [out][in] and channel-major [in][samples]. */
+  for (size_t o = 0; o < out_channels; ++o) {
+    for (size_t s = 0; s < samples; ++s) {
+      double re = 0.0, im = 0.0;
+      for (size_t i = 0; i < in_channels; ++i) {
+        const double wr = w[2 * (o * in_channels + i)], wi = w[2 * (o * in_channels + i) + 1];
+        const double xr = x[2 * (i * samples + s)], xi = x[2 * (i * samples + s) + 1];
+        re += wr * xr - wi * xi;
+        im += wr * xi + wi * xr;
+      }
+      y[2 * (o * samples + s)] = re;
+      y[2 * (o * samples + s) + 1] = im;
+    }
+  }
+}
+
+void ora_phase_correct(size_t channels, size_t rows, size_t cols, const float* x, const float* p, float* y) {
+  /* phase_correct.cpp:140-142: value * phase(channel, sample % cols); built with
+   * -ffp-contract=off so each product and sum rounds to float as the reference's does. */
+  for (size_t c = 0; c < channels; ++c) {
+    for (size_t r = 0; r < rows; ++r) {
+      for (size_t k = 0; k < cols; ++k) {
+        const size_t e = (c * rows + r) * cols + k;
+        const float a = x[2 * e], b = x[2 * e + 1];
+        const float pc = p[2 * (c * cols + k)], pd = p[2 * (c * cols + k) + 1];
+        const float ac = a * pc, bd = b * pd, ad = a * pd, bc = b * pc;
+        y[2 * e] = ac - bd;
+        y[2 * e + 1] = ad + bc;
+      }
+    }
+  }
+}
+
+void ora_readout_crop(size_t lines, size_t in_cols, size_t out_cols, size_t offset, const float* x, float* y) {
+  /* readout_oversampling_remove.cpp:174-180: copy_n of out_cols values from offset per line. */
+  for (size_t l = 0; l < lines; ++l)
+    for (size_t k = 0; k < out_cols; ++k) {
+      y[2 * (l * out_cols + k)] = x[2 * (l * in_cols + offset + k)];
+      y[2 * (l * out_cols + k) + 1] = x[2 * (l * in_cols + offset + k) + 1];
+    }
+}

@@ -123,6 +123,21 @@ int ora_kb_adjoint3(size_t nz, size_t ny, size_t nx, size_t gz, size_t gy, size_
  * a = 2 pi frac(p g2), g1 = 0.4656, g2 = 0.6823; readout as ora_golden_angle2; [S][3]. */
 void ora_golden_means3(size_t spokes, size_t readout, double* traj);
 
+/* ---- k-space conditioning (providers/kspacejet-kspace-conditioning/src/operators/). */
+
+/* y[o][s] = sum_i w[o][i] x[i][s], complex, FP64 accumulation in input order: the
+ * channel-major matmul W[out x in] X[in x samples] of noise_prewhiten.cpp,
+ * coil_compress.cpp and their non-Cartesian forms (samples = rows * cols for a frame). */
+void ora_channel_mix(size_t out_channels, size_t in_channels, size_t samples, const double* w, const double* x,
+                     double* y);
+/* y[c][r][k] = x[c][r][k] * p[c][k] in float, (ac - bd, ad + bc) with every product and
+ * sum rounded to float (phase_correct.cpp:127-143; std::complex<float> multiply of finite
+ * values). */
+void ora_phase_correct(size_t channels, size_t rows, size_t cols, const float* x, const float* p, float* y);
+/* y[l][k] = x[l][offset + k] for every line l of a channel-major frame
+ * (readout_oversampling_remove.cpp:165-181). */
+void ora_readout_crop(size_t lines, size_t in_cols, size_t out_cols, size_t offset, const float* x, float* y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,9 @@ def lib():
                                       C.c_double, _sz, _dp, _dp, _dp, _dp]
         L.ora_golden_means3.argtypes = [_sz, _sz, _dp]
         L.ora_normalize_trajectory2.argtypes = [_sz, _fp, C.c_int, C.c_uint32, C.c_uint32, _fp]
+        L.ora_channel_mix.argtypes = [_sz, _sz, _sz, _dp, _dp, _dp]
+        L.ora_phase_correct.argtypes = [_sz, _sz, _sz, _fp, _fp, _fp]
+        L.ora_readout_crop.argtypes = [_sz, _sz, _sz, _sz, _fp, _fp]
         _lib = L
     return _lib
 
@@ -356,3 +359,41 @@ def normalize_trajectory2(raw_kx_ky, units: str, encoded_rows: int, encoded_cols
     rc = lib().ora_normalize_trajectory2(raw.shape[0], raw.reshape(-1), TRAJ_UNITS[units], encoded_rows,
                                          encoded_cols, out.reshape(-1))
     return (out, 0) if rc == 0 else (None, rc)
+
+
+# ---------------------------------------------------------------- k-space conditioning
+def channel_mix(w, x) -> np.ndarray:
+    """FP64 W[out, in] @ X[in, samples] (complex128 out): noise prewhitening and coil
+    compression of channel-major k-space (noncartesian_noise_prewhiten.cpp:120-136)."""
+    w = _c128(w)
+    x = _c128(x)
+    out_ch, in_ch = w.shape
+    assert x.shape[0] == in_ch
+    xs = x.reshape(in_ch, -1)
+    y = np.zeros((out_ch, xs.shape[1]), np.complex128)
+    lib().ora_channel_mix(out_ch, in_ch, xs.shape[1], w.view(np.float64).reshape(-1), xs.view(np.float64).reshape(-1),
+                          y.view(np.float64).reshape(-1))
+    return y.reshape((out_ch,) + x.shape[1:])
+
+
+def phase_correct(x, phase) -> np.ndarray:
+    """complex64 x[C, R, K] * phase[C, K] with float products and sums (phase_correct.cpp:127-143)."""
+    x = np.ascontiguousarray(np.asarray(x, np.complex64))
+    p = np.ascontiguousarray(np.asarray(phase, np.complex64))
+    C_, R, K = x.shape
+    assert p.shape == (C_, K)
+    y = np.zeros_like(x)
+    lib().ora_phase_correct(C_, R, K, x.view(np.float32).reshape(-1), p.view(np.float32).reshape(-1),
+                            y.view(np.float32).reshape(-1))
+    return y
+
+
+def readout_crop(x, out_cols: int, offset: int) -> np.ndarray:
+    """complex64 x[..., in_cols] -> x[..., offset:offset + out_cols] (readout_oversampling_remove.cpp:165-181)."""
+    x = np.ascontiguousarray(np.asarray(x, np.complex64))
+    in_cols = x.shape[-1]
+    lines = x.size // in_cols
+    y = np.zeros(x.shape[:-1] + (out_cols,), np.complex64)
+    lib().ora_readout_crop(lines, in_cols, out_cols, offset, x.view(np.float32).reshape(-1),
+                           y.view(np.float32).reshape(-1))
+    return y

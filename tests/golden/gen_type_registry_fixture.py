@@ -13,7 +13,8 @@ import re
 import sys
 
 TYPES = ["ksj.kspace-frame", "ksj.noncartesian-kspace-frame", "ksj.trajectory-frame", "ksj.coil-image-frame",
-         "ksj.image-frame", "ksj.complex-image-frame", "ksj.sensitivity-map"]
+         "ksj.image-frame", "ksj.complex-image-frame", "ksj.sensitivity-map", "ksj.noise-model", "ksj.phase-model",
+         "ksj.coil-compression-basis"]
 
 
 def main(root):

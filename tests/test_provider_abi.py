@@ -32,6 +32,10 @@ PROVIDERS = {
                         "b49ba77e1655bdbbc4839ab692eb8fb7d9551bcb7ba138c6274de836d4ab5006"),
     "coil_combine": ("org.kspacejet.coil-combine", ["coil_combine_rss"],
                      "ba4f619047391419c8ffc42220ace7d888f047c9e2fd64aaeb78d9093191aed5"),
+    "kspace_conditioning": ("org.kspacejet.kspace-conditioning",
+                            ["noise_prewhiten", "phase_correct", "coil_compress", "readout_oversampling_remove",
+                             "noncartesian_noise_prewhiten", "noncartesian_coil_compress"],
+                            "6ff48e549c4ff3ad41cfebb766e6d298360bf686dc7882f5df9e6401b7dc7325"),
     "gpu_recon": ("org.kspacejet.gpu-recon",
                   ["kb_gridding_reconstruct", "kb_gridding_reconstruct_rss", "cartesian_ifft2_rss",
                    "coil_combine_sensitivity"], None),
@@ -79,7 +83,7 @@ def test_type_identity_digests_reproduce_the_reference_header():
     the reference's generated type_registry.h and the table the modules use."""
     fixture = json.load(open(os.path.join(GOLDEN, "type_registry_reference.json")))
     table = {t["type_ref"]: t for t in json.load(open(H.TYPES_JSON))}
-    assert len(fixture) == 7
+    assert len(fixture) == 10
     for entry in fixture:
         ident = entry["identity"]
         p, m = ident["payload"], ident["memory"]
