@@ -150,6 +150,8 @@ def main(argv=None) -> int:
     ap.add_argument("--latency-requests", type=int, default=300)
     ap.add_argument("--mixed-requests", type=int, default=48, help="0 skips the mixed-stream section")
     ap.add_argument("--uneven-items", type=int, default=64, help="items of the uneven batch; 0 skips that section")
+    ap.add_argument("--conditioning-reps", type=int, default=20,
+                    help="timed launches per k-space conditioning kernel; 0 skips that section")
     ap.add_argument("--volume-coils", type=int, default=20,
                     help="coils of the 3-D kooshball volume (256^3, 16384 x 512 spokes); 0 skips that section")
     ap.add_argument("--volume-coils-per-pass", type=int, default=16)
@@ -317,6 +319,10 @@ def main(argv=None) -> int:
         log("3-D kooshball volume ...")
         volume = run_volume(torch, ops, lib, args.volume_coils, args.volume_coils_per_pass, max(1, args.steps // 4),
                             1)
+    conditioning = None
+    if args.conditioning_reps > 0:
+        log("k-space conditioning kernels ...")
+        conditioning = run_conditioning(torch, ops, lib, args.conditioning_reps)
     uneven = None
     if args.uneven_items > 0:
         log("uneven batch ...")
@@ -361,9 +367,63 @@ def main(argv=None) -> int:
         "mixed": mixed,
         "uneven": uneven,
         "volume": volume,
+        "conditioning": conditioning,
     }
     print(json.dumps(line))
     return 0
+
+
+def run_conditioning(torch, ops, lib, reps):
+    """The k-space conditioning kernels on device-resident data, each launch timed by the
+    library's stage events (a HIP event pair around the kernel on its stream): noise
+    prewhitening and coil compression (channel mix) of the headline scan's non-Cartesian
+    k-space (205,824 samples; 16x16, 32->16 and 64x64 matrices), and per-line phase
+    correction and 2x readout-oversampling removal of a 16-channel 256 x 512 Cartesian
+    frame.  Algorithmic bytes: every input value read once and every output value written
+    once; flops of the mix: 8 per complex multiply-add."""
+    g = torch.Generator(device="cuda").manual_seed(11)
+
+    def crandn(*shape):
+        return torch.complex(torch.randn(shape, generator=g, device="cuda"), torch.randn(shape, generator=g, device="cuda"))
+
+    def timed(fn, stage):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        lib.ksj_gpu_profile_enable(1)
+        lib.ksj_gpu_profile_reset()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        tot, cnt = C.c_double(0), C.c_uint64(0)
+        lib.ksj_gpu_profile_read(stage.encode(), C.byref(tot), C.byref(cnt))
+        lib.ksj_gpu_profile_enable(0)
+        return tot.value * 1e3 / max(1, cnt.value)  # us per launch
+
+    res = {}
+    S = SPOKES * READOUT
+    for name, out_ch, in_ch in (("prewhiten_16", 16, 16), ("compress_32_to_16", 16, 32), ("prewhiten_64", 64, 64)):
+        x, w = crandn(in_ch, S), crandn(out_ch, in_ch)
+        us = timed(lambda: ops.channel_mix(w, x), "channel_mix")
+        nbytes = (in_ch + out_ch) * S * 8 + out_ch * in_ch * 8
+        res[name] = {"kernel": "channel_mix_kernel", "samples": S, "us": round(us, 2),
+                     "gbs": round(nbytes / (us * 1e-6) / 1e9, 1), "frac_hbm": round(nbytes / (us * 1e-6) / 8e12, 3),
+                     "tflops": round(8.0 * out_ch * in_ch * S / (us * 1e-6) / 1e12, 2)}
+        del x, w
+    nc, nr, nk = 16, 256, 512
+    x = crandn(nc, nr, nk)
+    p = crandn(nc, nk)
+    us = timed(lambda: ops.phase_correct(x, p), "phase_correct")
+    nbytes = 2 * nc * nr * nk * 8 + nc * nk * 8
+    res["phase_correct"] = {"kernel": "phase_correct_kernel", "frame": [nc, nr, nk], "us": round(us, 2),
+                            "gbs": round(nbytes / (us * 1e-6) / 1e9, 1),
+                            "frac_hbm": round(nbytes / (us * 1e-6) / 8e12, 3)}
+    us = timed(lambda: ops.readout_crop(x, nk // 2, nk // 4), "readout_crop")
+    nbytes = 2 * nc * nr * (nk // 2) * 8
+    res["readout_crop"] = {"kernel": "readout_crop_kernel", "frame": [nc, nr, nk], "out_cols": nk // 2, "us": round(us, 2),
+                           "gbs": round(nbytes / (us * 1e-6) / 1e9, 1),
+                           "frac_hbm": round(nbytes / (us * 1e-6) / 8e12, 3)}
+    return res
 
 
 def run_volume(torch, ops, lib, coils, coils_per_pass, steps, warmup):

@@ -421,10 +421,12 @@ void ksj_gpu_serve_destroy(ksj_gpu_server* server);
  *   "matrix3_run_cost"     1..64: work per gather wave of new 3-D matrix plans
  *   "tile_width"           4 or 8 columns per tile (new plans; summation order)
  *   "tile_item_samples"    window samples per tile work item (new plans; summation order)
+ *   "mix_scheme"           1 = ksj_gpu_channel_mix on the matrix cores (default), 0 = vector FMAs
+ *                          (summation order)
  * Every setting gives bit-identical results, except item_visits, matrix_item_visits,
  * matrix3_item_visits, matrix_block_width, the tile keys, gather_scheme and gather3_scheme (float summation
- * order of the gather) and fft_schedule (FFT rounding); each of those is tested against
- * the FP64 oracle.  Not thread-safe
+ * order of the gather), fft_schedule (FFT rounding) and mix_scheme (summation order of the
+ * channel mix); each of those is tested against the FP64 oracle.  Not thread-safe
  * against concurrent launches. */
 int ksj_gpu_tune_set(const char* key, int64_t value);
 int ksj_gpu_tune_get(const char* key, int64_t* value);

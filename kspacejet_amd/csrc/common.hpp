@@ -71,6 +71,7 @@ enum TuneKey : int {
   kTuneGather3Scheme,     // 3-D schedule of new plans: 0 = 2x2x2 blocks, 2 = 4x4x4 on the matrix cores
   kTuneMatrix3ItemVisits, // 3-D matrix schedule: work-item size of new plans (visits; summation order changes)
   kTuneMatrix3RunCost,    // 3-D matrix schedule of new plans: cost per wave run, 1..64
+  kTuneMixScheme,         // channel mix: 0 = vector FMAs, 1 = matrix cores (summation order changes)
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);

@@ -40,6 +40,7 @@ Param g_params[kTuneKeys] = {
     {"gather3_scheme", 2, [](int64_t v) { return v == 0 || v == 2; }},
     {"matrix3_item_visits", 256, [](int64_t v) { return v >= 1 && v <= 4096; }},
     {"matrix3_run_cost", 32, [](int64_t v) { return v >= 1 && v <= 64; }},
+    {"mix_scheme", 1, [](int64_t v) { return v == 0 || v == 1; }},
 };
 
 }  // namespace

@@ -24,6 +24,15 @@ def ops():
     return K
 
 
+@pytest.fixture(params=[1, 0], ids=["matrix_cores", "vector"])
+def scheme(request, ops):
+    """Both channel-mix schedules (tuning key mix_scheme)."""
+    saved = ops.tune_get("mix_scheme")
+    ops.tune_set("mix_scheme", request.param)
+    yield request.param
+    ops.tune_set("mix_scheme", saved)
+
+
 def dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
@@ -32,7 +41,7 @@ def crandn(rng, shape):
     return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
 
 
-def test_reference_kats(ops):
+def test_reference_kats(ops, scheme):
     x = np.arange(1, 9, dtype=np.float32).reshape(2, 2, 2).astype(np.complex64)
     y = ops.channel_mix(dev(np.diag([2, 3]).astype(np.complex64)), dev(x)).cpu().numpy()
     assert np.array_equal(y.reshape(-1), np.array([2, 4, 6, 8, 15, 18, 21, 24], np.complex64))
@@ -54,7 +63,7 @@ def test_reference_kats(ops):
 @pytest.mark.parametrize("out_ch,in_ch,samples", [
     (1, 1, 1), (2, 3, 255), (4, 4, 257), (5, 8, 1000), (16, 16, 205824), (8, 32, 4099), (12, 17, 3000),
     (33, 40, 777), (64, 64, 65536), (1, 64, 300)])
-def test_channel_mix_matches_the_oracle(ops, out_ch, in_ch, samples):
+def test_channel_mix_matches_the_oracle(ops, scheme, out_ch, in_ch, samples):
     rng = np.random.default_rng(out_ch * 131 + in_ch)
     w = crandn(rng, (out_ch, in_ch))
     x = crandn(rng, (in_ch, samples))
@@ -67,14 +76,14 @@ def test_channel_mix_matches_the_oracle(ops, out_ch, in_ch, samples):
 
 
 @pytest.mark.parametrize("channels", [1, 7, 16, 64])
-def test_identity_matrix_copies_bits(ops, channels):
+def test_identity_matrix_copies_bits(ops, scheme, channels):
     rng = np.random.default_rng(channels)
     x = crandn(rng, (channels, 96, 80))
     got = ops.channel_mix(dev(np.eye(channels, dtype=np.complex64)), dev(x)).cpu().numpy()
     assert np.array_equal(got.view(np.uint32), x.view(np.uint32))
 
 
-def test_cartesian_frame_mix_at_the_largest_frame(ops):
+def test_cartesian_frame_mix_at_the_largest_frame(ops, scheme):
     rng = np.random.default_rng(11)
     w = crandn(rng, (16, 64))
     x = crandn(rng, (64, 512, 512))
