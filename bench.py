@@ -322,7 +322,7 @@ def main(argv=None) -> int:
     conditioning = None
     if args.conditioning_reps > 0:
         log("k-space conditioning kernels ...")
-        conditioning = run_conditioning(torch, ops, lib, args.conditioning_reps)
+        conditioning = run_conditioning(torch, ops, args.conditioning_reps)
     uneven = None
     if args.uneven_items > 0:
         log("uneven batch ...")
@@ -373,9 +373,9 @@ def main(argv=None) -> int:
     return 0
 
 
-def run_conditioning(torch, ops, lib, reps):
-    """The k-space conditioning kernels on device-resident data, each launch timed by the
-    library's stage events (a HIP event pair around the kernel on its stream): noise
+def run_conditioning(torch, ops, reps):
+    """The k-space conditioning kernels on device-resident data, timed as `reps` launches
+    replayed from one captured HIP graph (the per-launch average): noise
     prewhitening and coil compression (channel mix) of the headline scan's non-Cartesian
     k-space (205,824 samples; 16x16, 32->16 and 64x64 matrices), and per-line phase
     correction and 2x readout-oversampling removal of a 16-channel 256 x 512 Cartesian
@@ -386,39 +386,51 @@ def run_conditioning(torch, ops, lib, reps):
     def crandn(*shape):
         return torch.complex(torch.randn(shape, generator=g, device="cuda"), torch.randn(shape, generator=g, device="cuda"))
 
-    def timed(fn, stage):
-        for _ in range(3):
-            fn()
+    def timed(fn):
+        # `reps` launches captured back to back in one HIP graph (no host gaps between
+        # them), the replay bracketed by one event pair: the average launch time.  The
+        # kernels record no memsets, so they are safe to capture.
+        fn()
         torch.cuda.synchronize()
-        lib.ksj_gpu_profile_enable(1)
-        lib.ksj_gpu_profile_reset()
-        for _ in range(reps):
-            fn()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            for _ in range(reps):
+                fn()
+        graph.replay()
         torch.cuda.synchronize()
-        tot, cnt = C.c_double(0), C.c_uint64(0)
-        lib.ksj_gpu_profile_read(stage.encode(), C.byref(tot), C.byref(cnt))
-        lib.ksj_gpu_profile_enable(0)
-        return tot.value * 1e3 / max(1, cnt.value)  # us per launch
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        graph.replay()
+        stop.record()
+        torch.cuda.synchronize()
+        del graph
+        return start.elapsed_time(stop) * 1e3 / reps  # us per launch
 
     res = {}
     S = SPOKES * READOUT
     for name, out_ch, in_ch in (("prewhiten_16", 16, 16), ("compress_32_to_16", 16, 32), ("prewhiten_64", 64, 64)):
         x, w = crandn(in_ch, S), crandn(out_ch, in_ch)
-        us = timed(lambda: ops.channel_mix(w, x), "channel_mix")
+        us = timed(lambda: ops.channel_mix(w, x))
         nbytes = (in_ch + out_ch) * S * 8 + out_ch * in_ch * 8
-        res[name] = {"kernel": "channel_mix_kernel", "samples": S, "us": round(us, 2),
+        if ops.tune_get("mix_scheme") == 1:
+            rb = 2 if out_ch <= 16 else 4 if out_ch <= 32 else 8
+            pairs = 4 if in_ch <= 16 else 8 if in_ch <= 32 else 16
+            kernel = f"channel_mix_mfma_kernel<{rb}, {pairs}>"
+        else:
+            kernel = f"channel_mix_kernel<{4 if in_ch <= 4 else 8 if in_ch <= 8 else 16 if in_ch <= 16 else 32 if in_ch <= 32 else 64}>"
+        res[name] = {"kernel": kernel, "samples": S, "us": round(us, 2),
                      "gbs": round(nbytes / (us * 1e-6) / 1e9, 1), "frac_hbm": round(nbytes / (us * 1e-6) / 8e12, 3),
                      "tflops": round(8.0 * out_ch * in_ch * S / (us * 1e-6) / 1e12, 2)}
         del x, w
     nc, nr, nk = 16, 256, 512
     x = crandn(nc, nr, nk)
     p = crandn(nc, nk)
-    us = timed(lambda: ops.phase_correct(x, p), "phase_correct")
+    us = timed(lambda: ops.phase_correct(x, p))
     nbytes = 2 * nc * nr * nk * 8 + nc * nk * 8
     res["phase_correct"] = {"kernel": "phase_correct_kernel", "frame": [nc, nr, nk], "us": round(us, 2),
                             "gbs": round(nbytes / (us * 1e-6) / 1e9, 1),
                             "frac_hbm": round(nbytes / (us * 1e-6) / 8e12, 3)}
-    us = timed(lambda: ops.readout_crop(x, nk // 2, nk // 4), "readout_crop")
+    us = timed(lambda: ops.readout_crop(x, nk // 2, nk // 4))
     nbytes = 2 * nc * nr * (nk // 2) * 8
     res["readout_crop"] = {"kernel": "readout_crop_kernel", "frame": [nc, nr, nk], "out_cols": nk // 2, "us": round(us, 2),
                            "gbs": round(nbytes / (us * 1e-6) / 1e9, 1),
