@@ -162,3 +162,18 @@ def test_lease_shape_metadata_and_scratch(module):
     ls = H.Lease([H.Batch(0, [H.Item(c64(16), "ksj.kspace-frame", memory_domain=H.MEMORY_DEVICE)])],
                  H.OutputSpec(8 * CB, "ksj.kspace-frame"), scratch_bytes=0)
     rejected_before_output(crop, ls)
+
+
+def test_valid_firing_without_a_device_fails_loudly(module):
+    """No CPU fallback: on a host with no GPU a valid firing fails after the grant is
+    acquired, the grant is released and the output stays untouched."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    op = opened(module, "noncartesian_noise_prewhiten")
+    ls = lease(c64(6), "ksj.noncartesian-kspace-frame", c64(4), "ksj.noise-model", 6 * CB)
+    st, r, msg = op.process(ls)
+    assert st in (H.INTERNAL_ERROR, H.RESOURCE_EXHAUSTED), msg
+    assert ls.acquire_calls == 1 and ls.seal_calls == 0 and ls.release_calls == 1 and r.sealed_output_count == 0
+    assert not np.any(ls.output)
+    op.close()
