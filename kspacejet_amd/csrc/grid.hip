@@ -64,7 +64,13 @@ __device__ inline void prep_block(const Planes& planes, const float2* weight, co
 #pragma unroll
   for (int j = 0; j < kPerWave; ++j) {
     const uint32_t c = wave + kPrepWaves * j;  // wave-uniform: the plane pointer is a scalar load
-    v[j] = (c < coils && live) ? plane_of(planes, c)[s] : make_float2(0.f, 0.f);
+    v[j] = make_float2(0.f, 0.f);
+    if (c < coils && live) {
+      // Each k-space value is read exactly once: keep it out of the caches.
+      typedef float F2v __attribute__((ext_vector_type(2)));
+      const F2v t = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(plane_of(planes, c) + s));
+      v[j] = make_float2(t.x, t.y);
+    }
   }
   if (wave == 0) rank_s[lane] = r;  // after the sample loads are issued
 #pragma unroll
