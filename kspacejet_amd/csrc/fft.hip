@@ -232,7 +232,11 @@ __device__ inline void load_lines(const FftArgs& a, float2* buf, int ls, int lpb
     // Unconditional load, so no branch separates the loads and all of them are in
     // flight together: a line past the end transforms a copy of line 0, and its
     // outputs are never stored.
-    v[i] = a.in[(live ? base : 0ull) + static_cast<uint64_t>(src * es)];
+    // Every input element of a pass is read once: non-temporal, out of the caches' way.
+    typedef float F2v __attribute__((ext_vector_type(2)));
+    const F2v t = __builtin_nontemporal_load(
+        reinterpret_cast<const F2v*>(a.in + (live ? base : 0ull) + static_cast<uint64_t>(src * es)));
+    v[i] = make_float2(t.x, t.y);
     dst[i] = ly * ls + pad(e);
   }
 #pragma unroll
