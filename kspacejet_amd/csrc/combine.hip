@@ -41,7 +41,10 @@ __global__ __launch_bounds__(256) void major_to_last_kernel(const float2* __rest
   const uint32_t np = static_cast<uint32_t>(p0 + kTilePixels <= pixels ? kTilePixels : pixels - p0);
   for (uint32_t i = threadIdx.x; i < np * coils; i += blockDim.x) {
     const uint32_t c = i / np, pl = i % np;  // consecutive threads read consecutive pixels of one coil plane
-    tile[pl * stride + c] = in[static_cast<uint64_t>(c) * pixels + p0 + pl];
+    // Each coil-plane value is read once: a non-temporal load.
+    typedef float F2v __attribute__((ext_vector_type(2)));
+    const F2v v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(in + static_cast<uint64_t>(c) * pixels + p0 + pl));
+    tile[pl * stride + c] = make_float2(v.x, v.y);
   }
   __syncthreads();
   if (last) {
@@ -70,7 +73,10 @@ __global__ __launch_bounds__(256) void major_split_kernel(const float2* __restri
   const uint32_t np = static_cast<uint32_t>(p0 + kTilePixels <= pixels ? kTilePixels : pixels - p0);
   for (uint32_t i = threadIdx.x; i < np * total; i += blockDim.x) {
     const uint32_t c = i / np, pl = i % np;
-    tile[pl * stride + c] = in[static_cast<uint64_t>(c) * pixels + p0 + pl];
+    // Each coil-plane value is read once: a non-temporal load.
+    typedef float F2v __attribute__((ext_vector_type(2)));
+    const F2v v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(in + static_cast<uint64_t>(c) * pixels + p0 + pl));
+    tile[pl * stride + c] = make_float2(v.x, v.y);
   }
   __syncthreads();
   for (uint32_t g = 0; g < out.groups; ++g) {
