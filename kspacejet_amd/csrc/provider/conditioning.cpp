@@ -6,8 +6,10 @@
 // (providers/kspacejet-kspace-conditioning/src/provider_api.cpp:323-437): lease counts and
 // no scratch -> input ports by port number (0 = k-space, 1 = calibration) -> k-space item
 // -> calibration item -> calibration identity -> reserved output bytes -> output grant ->
-// transform -> seal with the k-space item's keys and empty metadata.  Payloads are
-// host-pageable and carry no metadata, as there.
+// transform -> seal with the k-space item's keys and empty metadata.  Payloads carry no
+// metadata, as there.  The reference-named module keeps them host-pageable; the same
+// operators registered in org.kspacejet.gpu-recon also take device-memory payloads and
+// output grants (KSJ_PROVIDER_MEMORY_DEVICE) on the operator's GPU.
 #include "ops.hpp"
 
 namespace ksj::prov {
@@ -73,12 +75,16 @@ Layout layout(Kind k, const Shape& s) {
 }
 
 // An input item as the reference accepts it: compatible headers, exact bytes, host-pageable,
-// 64-byte aligned, no metadata, the exact registry type (provider_api.cpp:165-174).
-bool item_ok(const ksj_input_item_view& item, uint64_t bytes, TypeId type) {
-  return abi_ok(&item) && input_payload_ok(item, bytes, type) && item.metadata.size == 0;
+// 64-byte aligned, no metadata, the exact registry type (provider_api.cpp:165-174).  With
+// `device_edges` (the org.kspacejet.gpu-recon registrations) the payload may instead lie in
+// device memory of the operator's GPU.
+bool item_ok(const Firing& f, const ksj_input_item_view& item, uint64_t bytes, TypeId type, bool device_edges) {
+  const bool payload = device_edges ? input_payload_ok_device(item, bytes, type, f.op.device_ordinal)
+                                    : input_payload_ok(item, bytes, type);
+  return abi_ok(&item) && payload && item.metadata.size == 0;
 }
 
-ksj_status process(Firing& f, Kind kind) {
+ksj_status process(Firing& f, Kind kind, bool device_edges) {
   if (!callbacks_ok(f.cb, kCaps)) return reject(f.error, KSJ_STATUS_BAD_ABI, kBadAbi);
   const Shape& s = f.op.shape;
   const Layout lay = layout(kind, s);
@@ -111,10 +117,10 @@ ksj_status process(Firing& f, Kind kind) {
     return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, kPorts);
 
   if (lay.dynamic_bytes > kMaxKspaceBytes || lay.output_bytes > kMaxKspaceBytes ||
-      !item_ok(*dynamic, lay.dynamic_bytes, lay.dynamic))
+      !item_ok(f, *dynamic, lay.dynamic_bytes, lay.dynamic, device_edges))
     return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, kDynamic);
   if (lay.batches == 2) {
-    if (!item_ok(*calibration, lay.calibration_bytes, lay.calibration))
+    if (!item_ok(f, *calibration, lay.calibration_bytes, lay.calibration, device_edges))
       return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, kCalibration);
     // A calibration key of zero is unkeyed and matches any k-space item (support/kspace_frame.cpp).
     if ((calibration->semantic_key_hash != 0 && calibration->semantic_key_hash != dynamic->semantic_key_hash) ||
@@ -125,17 +131,15 @@ ksj_status process(Firing& f, Kind kind) {
 
   GrantGuard guard;
   ksj_mutable_payload_view out;
-  if (ksj_status st = output_grant(f, lay.output_bytes, lay.output, kOutput, guard, out)) return st;
+  if (ksj_status st = output_grant(f, lay.output_bytes, lay.output, kOutput, guard, out, device_edges)) return st;
 
+  // Device-resident payloads are read and written in place; host ones go through the
+  // session's staging.
   DeviceSession& d = device(f);
-  void* d_in = d.buffer(DeviceSession::kInput, lay.dynamic_bytes);
-  void* d_out = d.buffer(DeviceSession::kOutput, lay.output_bytes);
-  d.upload(d_in, dynamic->payload.data, lay.dynamic_bytes);
-  void* d_cal = nullptr;
-  if (lay.batches == 2) {
-    d_cal = d.buffer(DeviceSession::kInput2, lay.calibration_bytes);
-    d.upload(d_cal, calibration->payload.data, lay.calibration_bytes);
-  }
+  const void* d_in = device_input(d, DeviceSession::kInput, dynamic->payload, lay.dynamic_bytes);
+  const void* d_cal =
+      lay.batches == 2 ? device_input(d, DeviceSession::kInput2, calibration->payload, lay.calibration_bytes) : nullptr;
+  void* d_out = device_output(d, out, lay.output_bytes);
   const uint64_t frame = noncartesian(kind) ? uint64_t{s.samples} : uint64_t{s.rows} * s.cols;
   switch (kind) {
     case Kind::kNoise:
@@ -158,7 +162,7 @@ ksj_status process(Firing& f, Kind kind) {
                 "readout crop");
       break;
   }
-  d.download(out.data, d_out, lay.output_bytes);
+  finish_output(d, out, d_out, lay.output_bytes);
   return seal_and_finish(f, guard, out, lay.output_bytes, *dynamic, false, lay.batches, info.terminal_epoch);
 }
 
@@ -218,43 +222,60 @@ bool configure_nc_coil(const ksj_byte_view& c, Shape& s) {
 }
 bool valid_nc_coil(const Shape& s) { return valid_nc(s) && s.out_channels >= 1 && s.out_channels <= s.channels; }
 
-ksj_status process_noise(Firing& f) { return process(f, Kind::kNoise); }
-ksj_status process_phase(Firing& f) { return process(f, Kind::kPhase); }
-ksj_status process_coil(Firing& f) { return process(f, Kind::kCoil); }
-ksj_status process_crop(Firing& f) { return process(f, Kind::kCrop); }
-ksj_status process_nc_noise(Firing& f) { return process(f, Kind::kNoncartesianNoise); }
-ksj_status process_nc_coil(Firing& f) { return process(f, Kind::kNoncartesianCoil); }
+template <Kind K, bool DeviceEdges> ksj_status process_as(Firing& f) { return process(f, K, DeviceEdges); }
+
+constexpr std::string_view kNoiseConfig =
+    "noise_prewhiten needs the canonical config {\"channel_count\":N,\"cols\":N,\"rows\":N} with "
+    "1 <= channel_count <= 64 and 2 <= rows, cols <= 512";
+constexpr std::string_view kPhaseConfig =
+    "phase_correct needs the canonical config {\"channel_count\":N,\"cols\":N,\"rows\":N} with "
+    "1 <= channel_count <= 64 and 2 <= rows, cols <= 512";
+constexpr std::string_view kCoilConfig =
+    "coil_compress needs the canonical config {\"cols\":N,\"physical_channel_count\":N,\"rows\":N,"
+    "\"virtual_channel_count\":N} with 1 <= virtual <= physical <= 64 and 2 <= rows, cols <= 512";
+constexpr std::string_view kCropConfig =
+    "readout_oversampling_remove needs the canonical config {\"channel_count\":N,\"input_cols\":N,"
+    "\"output_cols\":N,\"readout_offset\":N,\"rows\":N} with 1 <= channel_count <= 64, 2 <= rows, input_cols, "
+    "output_cols <= 512 and readout_offset + output_cols <= input_cols";
+constexpr std::string_view kNcNoiseConfig =
+    "noncartesian_noise_prewhiten needs the canonical config {\"channel_count\":N,\"sample_count\":N} with "
+    "1 <= channel_count <= 64 and 1 <= sample_count <= 65536";
+constexpr std::string_view kNcCoilConfig =
+    "noncartesian_coil_compress needs the canonical config {\"physical_channel_count\":N,\"sample_count\":N,"
+    "\"virtual_channel_count\":N} with 1 <= virtual <= physical <= 64 and 1 <= sample_count <= 65536";
 
 }  // namespace
 
-const OpSpec kNoisePrewhiten{"noise_prewhiten",
-                             "noise_prewhiten needs the canonical config {\"channel_count\":N,\"cols\":N,\"rows\":N} "
-                             "with 1 <= channel_count <= 64 and 2 <= rows, cols <= 512",
-                             2, kMaxKspaceBytes, 0, &configure_frame, &valid_frame, &process_noise};
-const OpSpec kPhaseCorrect{"phase_correct",
-                           "phase_correct needs the canonical config {\"channel_count\":N,\"cols\":N,\"rows\":N} "
-                           "with 1 <= channel_count <= 64 and 2 <= rows, cols <= 512",
-                           2, kMaxKspaceBytes, 0, &configure_frame, &valid_frame, &process_phase};
-const OpSpec kCoilCompress{"coil_compress",
-                           "coil_compress needs the canonical config {\"cols\":N,\"physical_channel_count\":N,"
-                           "\"rows\":N,\"virtual_channel_count\":N} with 1 <= virtual <= physical <= 64 and "
-                           "2 <= rows, cols <= 512",
-                           2, kMaxKspaceBytes, 0, &configure_coil, &valid_coil, &process_coil};
-const OpSpec kReadoutOversamplingRemove{
-    "readout_oversampling_remove",
-    "readout_oversampling_remove needs the canonical config {\"channel_count\":N,\"input_cols\":N,"
-    "\"output_cols\":N,\"readout_offset\":N,\"rows\":N} with 1 <= channel_count <= 64, 2 <= rows, input_cols, "
-    "output_cols <= 512 and readout_offset + output_cols <= input_cols",
-    1, kMaxKspaceBytes, 0, &configure_crop, &valid_crop, &process_crop};
-const OpSpec kNoncartesianNoisePrewhiten{
-    "noncartesian_noise_prewhiten",
-    "noncartesian_noise_prewhiten needs the canonical config {\"channel_count\":N,\"sample_count\":N} with "
-    "1 <= channel_count <= 64 and 1 <= sample_count <= 65536",
-    2, kMaxKspaceBytes, 0, &configure_nc_noise, &valid_nc, &process_nc_noise};
-const OpSpec kNoncartesianCoilCompress{
-    "noncartesian_coil_compress",
-    "noncartesian_coil_compress needs the canonical config {\"physical_channel_count\":N,\"sample_count\":N,"
-    "\"virtual_channel_count\":N} with 1 <= virtual <= physical <= 64 and 1 <= sample_count <= 65536",
-    2, kMaxKspaceBytes, 0, &configure_nc_coil, &valid_nc_coil, &process_nc_coil};
+// org.kspacejet.kspace-conditioning: the reference's host-pageable contract.
+const OpSpec kNoisePrewhiten{"noise_prewhiten", kNoiseConfig, 2, kMaxKspaceBytes, 0, &configure_frame, &valid_frame,
+                             &process_as<Kind::kNoise, false>};
+const OpSpec kPhaseCorrect{"phase_correct", kPhaseConfig, 2, kMaxKspaceBytes, 0, &configure_frame, &valid_frame,
+                           &process_as<Kind::kPhase, false>};
+const OpSpec kCoilCompress{"coil_compress", kCoilConfig, 2, kMaxKspaceBytes, 0, &configure_coil, &valid_coil,
+                           &process_as<Kind::kCoil, false>};
+const OpSpec kReadoutOversamplingRemove{"readout_oversampling_remove", kCropConfig, 1, kMaxKspaceBytes, 0,
+                                        &configure_crop, &valid_crop, &process_as<Kind::kCrop, false>};
+const OpSpec kNoncartesianNoisePrewhiten{"noncartesian_noise_prewhiten", kNcNoiseConfig, 2, kMaxKspaceBytes, 0,
+                                         &configure_nc_noise, &valid_nc,
+                                         &process_as<Kind::kNoncartesianNoise, false>};
+const OpSpec kNoncartesianCoilCompress{"noncartesian_coil_compress", kNcCoilConfig, 2, kMaxKspaceBytes, 0,
+                                       &configure_nc_coil, &valid_nc_coil,
+                                       &process_as<Kind::kNoncartesianCoil, false>};
+
+// org.kspacejet.gpu-recon: the same operators with device-memory edges as well.
+const OpSpec kGpuNoisePrewhiten{"noise_prewhiten", kNoiseConfig, 2, kMaxKspaceBytes, 0, &configure_frame,
+                                &valid_frame, &process_as<Kind::kNoise, true>};
+const OpSpec kGpuPhaseCorrect{"phase_correct", kPhaseConfig, 2, kMaxKspaceBytes, 0, &configure_frame, &valid_frame,
+                              &process_as<Kind::kPhase, true>};
+const OpSpec kGpuCoilCompress{"coil_compress", kCoilConfig, 2, kMaxKspaceBytes, 0, &configure_coil, &valid_coil,
+                              &process_as<Kind::kCoil, true>};
+const OpSpec kGpuReadoutOversamplingRemove{"readout_oversampling_remove", kCropConfig, 1, kMaxKspaceBytes, 0,
+                                           &configure_crop, &valid_crop, &process_as<Kind::kCrop, true>};
+const OpSpec kGpuNoncartesianNoisePrewhiten{"noncartesian_noise_prewhiten", kNcNoiseConfig, 2, kMaxKspaceBytes, 0,
+                                            &configure_nc_noise, &valid_nc,
+                                            &process_as<Kind::kNoncartesianNoise, true>};
+const OpSpec kGpuNoncartesianCoilCompress{"noncartesian_coil_compress", kNcCoilConfig, 2, kMaxKspaceBytes, 0,
+                                          &configure_nc_coil, &valid_nc_coil,
+                                          &process_as<Kind::kNoncartesianCoil, true>};
 
 }  // namespace ksj::prov

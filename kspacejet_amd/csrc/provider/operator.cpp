@@ -8,6 +8,7 @@
 
 #include <new>
 #include <string>
+#include <vector>
 
 namespace ksj::prov {
 
@@ -53,18 +54,20 @@ ksj_operator_descriptor make_operator_descriptor(const OpSpec& op) {
   return d;
 }
 
+// Descriptors of every operator of the module, built once; the host reads
+// descriptor.operators[0, operator_count).
 struct Metadata {
-  ksj_operator_descriptor ops[8]{};
+  std::vector<ksj_operator_descriptor> ops;
   ksj_provider_descriptor descriptor{};
-  Metadata() {
-    for (uint32_t i = 0; i < spec().op_count && i < 8; ++i) ops[i] = make_operator_descriptor(*spec().ops[i]);
+  Metadata() : ops(spec().op_count) {
+    for (uint32_t i = 0; i < spec().op_count; ++i) ops[i] = make_operator_descriptor(*spec().ops[i]);
     descriptor.abi = make_header(sizeof(descriptor), KSJ_PROVIDER_CAP_SYNC_PROCESS | KSJ_PROVIDER_CAP_NO_PRIVATE_THREADS |
                                                          KSJ_PROVIDER_CAP_NO_DIRECT_FILE_IO |
                                                          KSJ_PROVIDER_CAP_NO_DIRECT_NETWORK_IO);
     descriptor.provider_id = utf8(spec().provider_id);
     descriptor.bundle_digest = digest_from_hex(spec().bundle_digest_hex);
-    descriptor.operator_count = spec().op_count;
-    descriptor.operators = ops;
+    descriptor.operator_count = static_cast<uint32_t>(ops.size());
+    descriptor.operators = ops.data();
   }
 };
 
@@ -315,6 +318,24 @@ ksj_status seal_and_finish(Firing& f, GrantGuard& guard, const ksj_mutable_paylo
   guard.settled = true;
   write_done(f.result, 1, consumed, terminal_epoch);
   return KSJ_STATUS_OK;
+}
+
+const void* device_input(DeviceSession& d, DeviceSession::Slot slot, const ksj_payload_view& p, uint64_t bytes) {
+  if (on_device(p)) return p.data;
+  void* staged = d.buffer(slot, bytes);
+  d.upload(staged, p.data, bytes);
+  return staged;
+}
+
+void* device_output(DeviceSession& d, const ksj_mutable_payload_view& out, uint64_t bytes) {
+  return on_device(out) ? out.data : d.buffer(DeviceSession::kOutput, bytes);
+}
+
+void finish_output(DeviceSession& d, const ksj_mutable_payload_view& out, const void* d_out, uint64_t bytes) {
+  if (on_device(out))
+    d.synchronize();
+  else
+    d.download(out.data, d_out, bytes);
 }
 
 DeviceSession& device(Firing& f) {

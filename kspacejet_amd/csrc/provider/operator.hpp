@@ -129,4 +129,16 @@ ksj_status seal_and_finish(Firing& f, GrantGuard& guard, const ksj_mutable_paylo
 // The operator's device session, created on first use on the context's device.
 DeviceSession& device(Firing& f);
 
+// ---- device-memory edges (KSJ_PROVIDER_MEMORY_DEVICE) of the operators that accept them ----
+
+inline bool on_device(const ksj_payload_view& p) { return p.memory_domain == KSJ_PROVIDER_MEMORY_DEVICE; }
+inline bool on_device(const ksj_mutable_payload_view& p) { return p.memory_domain == KSJ_PROVIDER_MEMORY_DEVICE; }
+// Device pointer of an input payload: in place when device-resident, else staged into `slot`.
+const void* device_input(DeviceSession& d, DeviceSession::Slot slot, const ksj_payload_view& p, uint64_t bytes);
+// Device buffer the operator writes its output to (the grant itself when device-resident),
+// and the hand-over once it is computed: a stream synchronisation for a device grant, a
+// download into a host grant.
+void* device_output(DeviceSession& d, const ksj_mutable_payload_view& out, uint64_t bytes);
+void finish_output(DeviceSession& d, const ksj_mutable_payload_view& out, const void* d_out, uint64_t bytes);
+
 }  // namespace ksj::prov

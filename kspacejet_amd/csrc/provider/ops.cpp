@@ -13,29 +13,6 @@ namespace ksj::prov {
 
 namespace {
 
-// Device-resident payloads (KSJ_PROVIDER_MEMORY_DEVICE) of the operators that accept them.
-bool on_device(const ksj_payload_view& p) { return p.memory_domain == KSJ_PROVIDER_MEMORY_DEVICE; }
-bool on_device(const ksj_mutable_payload_view& p) { return p.memory_domain == KSJ_PROVIDER_MEMORY_DEVICE; }
-
-// Device pointer of an input payload: in place when device-resident, else staged into `slot`.
-const void* device_input(DeviceSession& d, DeviceSession::Slot slot, const ksj_payload_view& p, uint64_t bytes) {
-  if (on_device(p)) return p.data;
-  void* staged = d.buffer(slot, bytes);
-  d.upload(staged, p.data, bytes);
-  return staged;
-}
-
-// Device buffer the operator writes its output to, and the hand-over once it is done.
-void* device_output(DeviceSession& d, const ksj_mutable_payload_view& out, uint64_t bytes) {
-  return on_device(out) ? out.data : d.buffer(DeviceSession::kOutput, bytes);
-}
-void finish_output(DeviceSession& d, const ksj_mutable_payload_view& out, const void* d_out, uint64_t bytes) {
-  if (on_device(out))
-    d.synchronize();
-  else
-    d.download(out.data, d_out, bytes);
-}
-
 constexpr uint32_t kMaxDim = 512;
 constexpr uint32_t kMaxChannels = 64;
 constexpr uint32_t kMaxSamples = 65536;             // reference envelope of the non-Cartesian operators

@@ -19,6 +19,13 @@ extern const OpSpec kCoilCompress;               // coil_compress
 extern const OpSpec kReadoutOversamplingRemove;  // readout_oversampling_remove
 extern const OpSpec kNoncartesianNoisePrewhiten; // noncartesian_noise_prewhiten
 extern const OpSpec kNoncartesianCoilCompress;   // noncartesian_coil_compress
+// org.kspacejet.gpu-recon: the conditioning operators with device-memory edges
+extern const OpSpec kGpuNoisePrewhiten;
+extern const OpSpec kGpuPhaseCorrect;
+extern const OpSpec kGpuCoilCompress;
+extern const OpSpec kGpuReadoutOversamplingRemove;
+extern const OpSpec kGpuNoncartesianNoisePrewhiten;
+extern const OpSpec kGpuNoncartesianCoilCompress;
 // org.kspacejet.gpu-recon (no reference implementation)
 extern const OpSpec kKbGridding;                // kb_gridding_reconstruct
 extern const OpSpec kKbGriddingRss;             // kb_gridding_reconstruct_rss

@@ -236,3 +236,44 @@ def test_reconstruct_then_combine_over_a_device_edge(modules):
     assert outs[0] == outs[1]
     rec.close()
     comb.close()
+
+
+def test_prewhiten_then_reconstruct_stays_in_hbm(modules):
+    """gpu-recon's noncartesian_noise_prewhiten hands whitened k-space to
+    kb_gridding_reconstruct_rss in HBM: the image has the bits of the same chain over
+    host-pageable edges, and the whitened k-space equals the reference-named
+    conditioning module's (host edges) bit for bit."""
+    n, coils, spokes, readout = 64, 8, 31, 128
+    S = spokes * readout
+    traj = synth.golden_angle2(spokes, readout).astype(np.float32)
+    rng = np.random.default_rng(14)
+    k = (rng.standard_normal((coils, S)) + 1j * rng.standard_normal((coils, S))).astype(np.complex64)
+    w = (np.eye(coils) + 0.1 * (rng.standard_normal((coils, coils)) + 1j * rng.standard_normal((coils, coils))))
+    w = w.astype(np.complex64)
+    pw_cfg = f'{{"channel_count":{coils},"sample_count":{S}}}'
+    pw = opened(modules, "gpu_recon", "noncartesian_noise_prewhiten", pw_cfg)
+    rec = opened(modules, "gpu_recon", "kb_gridding_reconstruct_rss", kb_config(coils, n, S))
+    images, whitened = [], []
+    for dev in (False, True):
+        dom = H.MEMORY_DEVICE if dev else H.MEMORY_HOST_PAGEABLE
+        first = H.Lease([H.Batch(0, [item(k, "ksj.noncartesian-kspace-frame", dev)]),
+                         H.Batch(1, [item(w, "ksj.noise-model", dev, (0, 0))])],
+                        H.OutputSpec(coils * S * 8, "ksj.noncartesian-kspace-frame", np.complex64, memory_domain=dom))
+        whitened.append(result(pw, first))
+        kw = first.output
+        kw_item = (H.Item(kw, "ksj.noncartesian-kspace-frame", 71, 73, memory_domain=H.MEMORY_DEVICE) if dev else
+                   H.Item(kw, "ksj.noncartesian-kspace-frame", 71, 73))
+        second = H.Lease([H.Batch(0, [kw_item]), H.Batch(1, [item(traj, "ksj.trajectory-frame", False)])],
+                         H.OutputSpec(n * n * 4, "ksj.image-frame", np.float32, memory_domain=dom))
+        images.append(result(rec, second))
+    assert images[0] == images[1] and whitened[0] == whitened[1]
+    st, ref_op, msg = H.ProviderModule("kspace_conditioning").create("noncartesian_noise_prewhiten", pw_cfg)
+    assert st == H.OK, msg
+    assert ref_op.open(0) == H.OK
+    ls = H.Lease([H.Batch(0, [item(k, "ksj.noncartesian-kspace-frame", False)]),
+                  H.Batch(1, [item(w, "ksj.noise-model", False, (0, 0))])],
+                 H.OutputSpec(coils * S * 8, "ksj.noncartesian-kspace-frame", np.complex64))
+    assert result(ref_op, ls) == whitened[0]
+    ref_op.close()
+    pw.close()
+    rec.close()

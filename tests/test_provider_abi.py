@@ -38,7 +38,8 @@ PROVIDERS = {
                             "6ff48e549c4ff3ad41cfebb766e6d298360bf686dc7882f5df9e6401b7dc7325"),
     "gpu_recon": ("org.kspacejet.gpu-recon",
                   ["kb_gridding_reconstruct", "kb_gridding_reconstruct_rss", "cartesian_ifft2_rss",
-                   "coil_combine_sensitivity"], None),
+                   "coil_combine_sensitivity", "noise_prewhiten", "phase_correct", "coil_compress",
+                   "readout_oversampling_remove", "noncartesian_noise_prewhiten", "noncartesian_coil_compress"], None),
 }
 
 DIRECT_CFG = '{"channels":2,"image_cols":2,"image_rows":2,"sample_count":2}'
