@@ -157,6 +157,10 @@ def main(argv=None) -> int:
     ap.add_argument("--volume-coils-per-pass", type=int, default=16)
     ap.add_argument("--provider-firings", type=int, default=40,
                     help="firings of the drop-in Provider operator (host-fed plain form); 0 skips it")
+    ap.add_argument("--serve-fft-schedule", type=int, default=3,
+                    help="fft_schedule while the server runs the served and latency sections (3: half the threads "
+                         "per line at length 512, which overlaps better across server lanes); -1 keeps the "
+                         "process setting.  The plain, stage and other sections run with the process setting.")
     ap.add_argument("--tune-seconds", type=float, default=20.0, help="autotuner budget for the uneven batch")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -203,6 +207,9 @@ def main(argv=None) -> int:
             raise RuntimeError(lib.ksj_gpu_last_error().decode())
         return list(tickets)
 
+    fft_schedule_saved = ops.tune_get("fft_schedule")
+    if args.serve_fft_schedule >= 0:
+        ops.tune_set("fft_schedule", args.serve_fft_schedule)
     log("served bursts ...")
     for _ in range(args.warmup):
         for t in served_step():
@@ -237,6 +244,8 @@ def main(argv=None) -> int:
         stream_tickets.append(t.value)
     stream_lat = [server.wait(t) for t in stream_tickets]
     served_stats = server.stats()
+
+    ops.tune_set("fft_schedule", fft_schedule_saved)
 
     # ---------------------------------------------------------------- plain
     log("plain per-scan operator ...")
@@ -349,7 +358,9 @@ def main(argv=None) -> int:
                                "axis, resident plan)",
                    "image": N, "coils": COILS, "spokes": SPOKES, "readout": READOUT, "kernel": "kaiser_bessel",
                    "kernel_width": WIDTH, "oversampling": SIGMA, "grid": int(N * SIGMA), "batch": args.batch,
-                   "server_lanes": args.lanes, "server_max_batch": args.max_batch},
+                   "server_lanes": args.lanes, "server_max_batch": args.max_batch,
+                   "server_fft_schedule": args.serve_fft_schedule if args.serve_fft_schedule >= 0
+                   else fft_schedule_saved},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "latency": {"p50_ms": round(percentile(stream_lat, 50), 3), "p99_ms": round(percentile(stream_lat, 99), 3),

@@ -404,7 +404,8 @@ void ksj_gpu_serve_destroy(ksj_gpu_server* server);
  *                          stacked coils up
  *   "block_group"          1, 2 or 4: side of the groups of 2x2 blocks numbered together (new plans)
  *   "multi_fused"          0 or 1: form of ksj_gpu_adjoint2_execute_multi (see there)
- *   "fft_schedule"         0, 1 or 2: radix schedule of every LDS FFT length (rounding only)
+ *   "fft_schedule"         0 (default), 1 or 2: radix schedule of every LDS FFT length; 3: 1 for
+ *                          length 512, 0 for the others (for overlapped serving; rounding only)
  *   "gather_scheme"        0 = 2x2 blocks, 1 = 64-cell LDS tiles, 2 = 4x4 blocks on the matrix
  *                          cores (new plans; summation order)
  *   "matrix_run_cost"      1..64: work per gather wave of the matrix schedule, in groups of four

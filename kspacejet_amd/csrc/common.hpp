@@ -59,7 +59,8 @@ enum TuneKey : int {
   kTuneTileItemSamples,   // window samples per tile work item of new plans (summation order changes)
   kTuneTileWidth,         // tile columns of new plans: 4 (16 x 4 tiles) or 8 (8 x 8) (summation order changes)
   kTuneBlockGroup,        // 2x2-block group side of new plans: 1, 2 or 4 (work-item order; results unchanged)
-  kTuneFftSchedule,       // FFT radix schedule per length: 0, 1 (half the threads) or 2 (radix-4 heavy); rounding changes
+  kTuneFftSchedule,       // FFT radix schedule: 0 (default), 1 (half the threads), 2 (radix-4 heavy) or 3 (per length:
+                          // 1 at 512, else 0); rounding changes
   kTuneMultiFused,        // uneven batches: 1 = one launch per stage and shape class, 0 = items over lanes
   kTuneMatrixRunCost,     // matrix schedule of new plans: cost (visit groups of four + 1 per item) per wave run, 1..64
   kTuneMatrixPipeline,    // matrix gather: batches of sample rows in flight per wave, 1 or 2

@@ -28,7 +28,7 @@ Param g_params[kTuneKeys] = {
     {"tile_item_samples", 256, [](int64_t v) { return v >= 1 && v <= 65536; }},
     {"tile_width", 4, [](int64_t v) { return v == 4 || v == 8; }},
     {"block_group", 2, [](int64_t v) { return v == 1 || v == 2 || v == 4; }},
-    {"fft_schedule", 0, [](int64_t v) { return v >= 0 && v <= 2; }},
+    {"fft_schedule", 0, [](int64_t v) { return v >= 0 && v <= 3; }},
     {"multi_fused", 0, [](int64_t v) { return v == 0 || v == 1; }},
     {"matrix_run_cost", 8, [](int64_t v) { return v >= 1 && v <= 64; }},
     {"matrix_pipeline", 1, [](int64_t v) { return v == 1 || v == 2; }},

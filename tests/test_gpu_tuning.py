@@ -121,7 +121,7 @@ def test_illegal_values_are_rejected(ops):
         ops.tune_set("no_such_key", 1)
 
 
-@pytest.mark.parametrize("schedule", [0, 1, 2])
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3])
 def test_fft_schedules_match_the_oracle_at_every_length(ops, tune, schedule):
     """Every radix schedule of every LDS length (and a direct-DFT length) is an exact
     factorisation of the DFT: uncentred and centred inverse FFTs stay at the FP64
