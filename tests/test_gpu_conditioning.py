@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from kspacejet_amd import provider_host as H
 from tests.helpers import rel_l2
 
 torch = pytest.importorskip("torch")
@@ -121,8 +122,6 @@ def test_argument_checks(ops):
 
 
 # ---------------------------------------------------------------- through the Provider ABI
-
-from kspacejet_amd import provider_host as H  # noqa: E402
 
 
 @pytest.fixture(scope="module")
