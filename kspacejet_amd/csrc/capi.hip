@@ -49,8 +49,7 @@ template <typename F> int guarded(F&& f) {
     f();
     return KSJ_GPU_OK;
   } catch (const ksj::gpu::Error& e) {
-    const int code = e.code == 3 ? KSJ_GPU_DEVICE_ERROR : KSJ_GPU_INVALID_ARGUMENT;
-    return fail(code, e.what());
+    return fail(ksj::gpu::status_of(e.code), e.what());
   } catch (const std::bad_alloc&) {
     return fail(KSJ_GPU_RESOURCE_EXHAUSTED, "host allocation failed");
   } catch (const std::exception& e) {

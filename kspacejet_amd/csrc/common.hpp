@@ -25,17 +25,33 @@ __host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y
 
 // Thrown by host code on a HIP error or a violated precondition; the C ABI
 // layer turns it into a status code and a last-error message.
+enum ErrorCode : int {
+  kErrArgument = 1,   // a violated precondition (KSJ_GPU_INVALID_ARGUMENT)
+  kErrDevice = 3,     // any other HIP error (KSJ_GPU_DEVICE_ERROR)
+  kErrExhausted = 4,  // device memory or an HBM budget ran out (KSJ_GPU_RESOURCE_EXHAUSTED)
+};
+
 struct Error : std::runtime_error {
   int code;
   Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
 
+// KSJ_GPU_* status of an Error code (the values of include/ksj_gpu.h).
+inline int status_of(int error_code) {
+  switch (error_code) {
+    case kErrArgument: return 1;
+    case kErrExhausted: return 4;
+    default: return 3;
+  }
+}
+
 inline void check(hipError_t e, const char* what) {
-  if (e != hipSuccess) throw Error(3, std::string(what) + ": " + hipGetErrorString(e));
+  if (e == hipSuccess) return;
+  throw Error(e == hipErrorOutOfMemory ? kErrExhausted : kErrDevice, std::string(what) + ": " + hipGetErrorString(e));
 }
 
 inline void require(bool ok, const char* what) {
-  if (!ok) throw Error(1, what);
+  if (!ok) throw Error(kErrArgument, what);
 }
 
 #define KSJ_HIP(call) ::ksj::gpu::check((call), #call)

@@ -324,8 +324,8 @@ bool Server::launch_next(Lane& lane) {
     if (need > lane.ws_bytes) {
       // Admission against the budget: drop idle plans first, then give up on this lane's growth.
       evict_until(need - lane.ws_bytes);
-      require(used_bytes() - lane.ws_bytes + need <= cfg_.hbm_budget_bytes,
-              "request does not fit in the HBM budget of the server");  // status 4 (resource exhausted)
+      if (used_bytes() - lane.ws_bytes + need > cfg_.hbm_budget_bytes)
+        throw Error(kErrExhausted, "request does not fit in the HBM budget of the server");
       if (lane.ws) {
         KSJ_HIP(hipStreamSynchronize(lane.stream));
         KSJ_HIP(hipFree(lane.ws));
@@ -360,7 +360,7 @@ bool Server::launch_next(Lane& lane) {
     KSJ_HIP(hipEventRecord(lane.done, lane.stream));
   } catch (const Error& e) {
     lane.batch = std::move(batch);
-    complete(lane, std::strstr(e.what(), "HBM budget") ? 4 : (e.code == 3 ? 3 : 1), e.what());
+    complete(lane, status_of(e.code), e.what());
     return true;
   } catch (const std::exception& e) {
     lane.batch = std::move(batch);

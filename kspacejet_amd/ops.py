@@ -24,13 +24,35 @@ def _ptr(t: torch.Tensor) -> int:
     return t.data_ptr()
 
 
-def _need(t: torch.Tensor, dtype: torch.dtype, name: str) -> None:
-    if not t.is_cuda:
-        raise ValueError(f"{name} must be a GPU tensor")
-    if t.dtype != dtype:
-        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
-    if not t.is_contiguous():
-        raise ValueError(f"{name} must be contiguous")
+def _need_all(*specs) -> None:
+    """Checks caller tensors before any pointer of them reaches a kernel.  Each spec is
+    (tensor, dtype, name[, shape]); every dtype, exact shape (when given) and contiguity
+    is checked first, then every device.  A wrongly sized buffer would otherwise become
+    an out-of-bounds device read or write."""
+    for spec in specs:
+        t, dtype, name = spec[:3]
+        shape = spec[3] if len(spec) > 3 else None
+        if not isinstance(t, torch.Tensor):
+            raise ValueError(f"{name} must be a tensor")
+        if t.dtype != dtype:
+            raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+        if shape is not None and tuple(t.shape) != tuple(int(x) for x in shape):
+            raise ValueError(f"{name} must have shape {tuple(int(x) for x in shape)}, got {tuple(t.shape)}")
+        if not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+    for spec in specs:
+        if not spec[0].is_cuda:
+            raise ValueError(f"{spec[2]} must be a GPU tensor")
+
+
+def _need(t: torch.Tensor, dtype: torch.dtype, name: str, shape=None) -> None:
+    _need_all((t, dtype, name, shape))
+
+
+def _image_shape(rows: int, cols: int, channels: int, layout: str):
+    if layout not in LAYOUTS:
+        raise ValueError(f"unknown layout {layout!r}")
+    return (rows, cols, channels) if layout == "coil_last" else (channels, rows, cols)
 
 
 def _workspace(nbytes: int, device) -> torch.Tensor:
@@ -54,9 +76,11 @@ def tune_get(key: str) -> int:
 
 def ifft2_coil_images(kspace: torch.Tensor, centred: bool = False, layout: str = "coil_last") -> torch.Tensor:
     """[C, R, Cols] complex64 channel-major k-space -> coil images (1/R/Cols scaling)."""
+    if kspace.dim() != 3:
+        raise ValueError("kspace must be [C, R, Cols]")
     _need(kspace, torch.complex64, "kspace")
     ch, rows, cols = kspace.shape
-    shape = (rows, cols, ch) if layout == "coil_last" else (ch, rows, cols)
+    shape = _image_shape(rows, cols, ch, layout)
     out = torch.empty(shape, dtype=torch.complex64, device=kspace.device)
     ws = _workspace(load().ksj_gpu_ifft2_workspace_bytes(ch, rows, cols), kspace.device)
     call("ksj_gpu_ifft2_coil_images", _ptr(kspace), _ptr(out), ch, rows, cols, int(centred), LAYOUTS[layout],
@@ -85,6 +109,8 @@ def sense_combine(coil_images: torch.Tensor, sensitivities: torch.Tensor) -> tor
 
 
 def ramp_dcf2(traj: torch.Tensor) -> torch.Tensor:
+    if traj.dim() != 2 or traj.shape[1] != 2:
+        raise ValueError("traj must be [S, 2]")
     _need(traj, torch.float32, "traj")
     out = torch.empty(traj.shape[0], dtype=torch.float32, device=traj.device)
     call("ksj_gpu_ramp_dcf2", _ptr(traj), _ptr(out), traj.shape[0], _stream())
@@ -94,10 +120,11 @@ def ramp_dcf2(traj: torch.Tensor) -> torch.Tensor:
 def nudft2_adjoint(traj: torch.Tensor, samples: torch.Tensor, rows: int, cols: int, row_origin: float,
                    col_origin: float, layout: str = "coil_last") -> torch.Tensor:
     """Exact adjoint NUDFT (float), samples [C, S] -> coil images."""
-    _need(traj, torch.float32, "traj")
-    _need(samples, torch.complex64, "samples")
+    if samples.dim() != 2:
+        raise ValueError("samples must be [C, S]")
     ch, S = samples.shape
-    shape = (rows, cols, ch) if layout == "coil_last" else (ch, rows, cols)
+    _need_all((samples, torch.complex64, "samples"), (traj, torch.float32, "traj", (S, 2)))
+    shape = _image_shape(rows, cols, ch, layout)
     out = torch.empty(shape, dtype=torch.complex64, device=samples.device)
     ws = _workspace(load().ksj_gpu_nudft2_adjoint_workspace_bytes(ch, rows, cols), samples.device)
     call("ksj_gpu_nudft2_adjoint", _ptr(traj), _ptr(samples), _ptr(out), ch, S, rows, cols, float(row_origin),
@@ -107,6 +134,8 @@ def nudft2_adjoint(traj: torch.Tensor, samples: torch.Tensor, rows: int, cols: i
 
 def nudft2_forward_f64(traj: torch.Tensor, image: torch.Tensor, row_origin: float, col_origin: float) -> torch.Tensor:
     """Exact FP64 forward NUDFT: image [C, R, Cols] complex128 -> samples [C, S] complex128."""
+    if image.dim() != 3 or traj.dim() != 2 or traj.shape[1] != 2:
+        raise ValueError("image must be [C, R, Cols] and traj [S, 2]")
     _need(traj, torch.float32, "traj")
     _need(image, torch.complex128, "image")
     ch, rows, cols = image.shape
@@ -174,11 +203,12 @@ class Plan2:
 
     def adjoint(self, kspace: torch.Tensor, images: bool = True, rss: bool = False, layout: str = "coil_last",
                 workspace: torch.Tensor | None = None):
+        if kspace.dim() != 2 or kspace.shape[1] != self.desc.samples:
+            raise ValueError("kspace must be [C, S] with the plan's sample count")
         _need(kspace, torch.complex64, "kspace")
         ch, S = kspace.shape
-        if S != self.desc.samples:
-            raise ValueError("sample count does not match the plan")
         r, c = self.desc.image_rows, self.desc.image_cols
+        _image_shape(r, c, ch, layout)
         img = None
         if images:
             img = torch.empty((r, c, ch) if layout == "coil_last" else (ch, r, c), dtype=torch.complex64,
@@ -248,9 +278,11 @@ class Server:
     def submit_radial(self, kspace: torch.Tensor, traj: torch.Tensor, n: int, images: torch.Tensor | None = None,
                       rss: torch.Tensor | None = None, width: int = 4, density: str = "ramp",
                       trajectory_key: int = 0) -> int:
-        _need(kspace, torch.complex64, "kspace")
-        _need(traj, torch.float32, "traj")
+        if kspace.dim() != 2:
+            raise ValueError("kspace must be [C, S]")
         ch, S = kspace.shape
+        self._check_request((kspace, torch.complex64, "kspace"), (traj, torch.float32, "traj", (S, 2)),
+                            images, (n, n, ch), rss, (n, n))
         r = _lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_RADIAL2D, n, n, ch, S, width,
                               _lib.DENSITY_RAMP if density == "ramp" else _lib.DENSITY_NONE, 0, 0, trajectory_key,
                               _ptr(traj), _ptr(kspace), _ptr(images) if images is not None else None,
@@ -262,10 +294,12 @@ class Server:
                         trajectory_key: int = 0) -> int:
         """A 3-D volume: kspace [C, S], traj [S, 3] (z, y, x) rad/px, shape (N_z, N_y, N_x);
         images coil-last [N_z, N_y, N_x, C] and/or rss [N_z, N_y, N_x]."""
-        _need(kspace, torch.complex64, "kspace")
-        _need(traj, torch.float32, "traj")
+        if kspace.dim() != 2:
+            raise ValueError("kspace must be [C, S]")
         ch, S = kspace.shape
         nz, ny, nx = shape
+        self._check_request((kspace, torch.complex64, "kspace"), (traj, torch.float32, "traj", (S, 3)),
+                            images, (nz, ny, nx, ch), rss, (nz, ny, nx))
         dens = _lib.DENSITY_RAMP_SQUARED if density == "ramp_squared" else _lib.DENSITY_NONE
         r = _lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_RADIAL3D, ny, nx, ch, S, width, dens, 0, nz,
                               trajectory_key, _ptr(traj), _ptr(kspace), _ptr(images) if images is not None else None,
@@ -274,12 +308,25 @@ class Server:
 
     def submit_cartesian(self, kspace: torch.Tensor, images: torch.Tensor | None = None,
                          rss: torch.Tensor | None = None, centred: bool = False) -> int:
-        _need(kspace, torch.complex64, "kspace")
+        if kspace.dim() != 3:
+            raise ValueError("kspace must be [C, R, Cols]")
         ch, rows, cols = kspace.shape
+        self._check_request((kspace, torch.complex64, "kspace"), None, images, (rows, cols, ch), rss, (rows, cols))
         r = _lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_CARTESIAN2D, rows, cols, ch, 0, 0, 0,
                               int(centred), 0, 0, None, _ptr(kspace), _ptr(images) if images is not None else None,
                               _ptr(rss) if rss is not None else None)
         return self._submit(r, (kspace, images, rss))
+
+    @staticmethod
+    def _check_request(kspace_spec, traj_spec, images, image_shape, rss, rss_shape) -> None:
+        if images is None and rss is None:
+            raise ValueError("a request needs images and/or rss")
+        specs = [kspace_spec] + ([traj_spec] if traj_spec is not None else [])
+        if images is not None:
+            specs.append((images, torch.complex64, "images", image_shape))
+        if rss is not None:
+            specs.append((rss, torch.float32, "rss", rss_shape))
+        _need_all(*specs)
 
     def _submit(self, req, keep) -> int:
         # Work the caller queued on torch's stream (e.g. filling k-space) must land first.
@@ -353,6 +400,8 @@ def trajectory_check(traj: torch.Tensor, resident: "torch.Tensor | None", bound:
 def channel_mix(matrix: torch.Tensor, kspace: torch.Tensor) -> torch.Tensor:
     """complex64 matrix [out, in] @ channel-major k-space [in, ...] (ksj_gpu_channel_mix):
     noise prewhitening with a whitening matrix, coil compression with a basis."""
+    if matrix.dim() != 2 or kspace.dim() < 2:
+        raise ValueError("matrix must be [out, in] and k-space [in, ...]")
     _need(matrix, torch.complex64, "matrix")
     _need(kspace, torch.complex64, "k-space")
     out_ch, in_ch = matrix.shape
@@ -365,11 +414,10 @@ def channel_mix(matrix: torch.Tensor, kspace: torch.Tensor) -> torch.Tensor:
 
 def phase_correct(kspace: torch.Tensor, phase: torch.Tensor) -> torch.Tensor:
     """complex64 k-space [C, R, K] times phase [C, K] per line (ksj_gpu_phase_correct)."""
-    _need(kspace, torch.complex64, "k-space")
-    _need(phase, torch.complex64, "phase")
+    if kspace.dim() != 3:
+        raise ValueError("k-space must be [C, R, K]")
     c, r, k = kspace.shape
-    if tuple(phase.shape) != (c, k):
-        raise ValueError("phase must be [channels, cols]")
+    _need_all((kspace, torch.complex64, "k-space"), (phase, torch.complex64, "phase", (c, k)))
     out = torch.empty_like(kspace)
     call("ksj_gpu_phase_correct", _ptr(kspace), _ptr(phase), _ptr(out), c, r, k, _stream())
     return out
@@ -377,8 +425,10 @@ def phase_correct(kspace: torch.Tensor, phase: torch.Tensor) -> torch.Tensor:
 
 def readout_crop(kspace: torch.Tensor, out_cols: int, offset: int) -> torch.Tensor:
     """complex64 k-space [..., in_cols] -> [..., offset:offset + out_cols] (ksj_gpu_readout_crop)."""
-    _need(kspace, torch.complex64, "k-space")
     in_cols = kspace.shape[-1]
+    if out_cols < 0 or offset < 0 or offset + out_cols > in_cols:
+        raise ValueError("crop window must lie inside the readout")
+    _need(kspace, torch.complex64, "k-space")
     out = torch.empty(tuple(kspace.shape[:-1]) + (out_cols,), dtype=torch.complex64, device=kspace.device)
     call("ksj_gpu_readout_crop", _ptr(kspace), _ptr(out), kspace.numel() // in_cols, in_cols, out_cols, offset,
          _stream())
@@ -443,10 +493,10 @@ class Plan3:
     def adjoint(self, kspace: torch.Tensor, images: bool = True, rss: bool = False, coils_per_pass: int = 16,
                 workspace: torch.Tensor | None = None):
         """[C, S] complex64 -> (coil-last [N_z, N_y, N_x, C] complex64 or None, RSS [N_z, N_y, N_x] or None)."""
+        if kspace.dim() != 2 or kspace.shape[1] != self.desc.samples:
+            raise ValueError("kspace must be [C, S] with the plan's sample count")
         _need(kspace, torch.complex64, "kspace")
         ch, S = kspace.shape
-        if S != self.desc.samples:
-            raise ValueError("sample count does not match the plan")
         img = torch.empty((*self.shape, ch), dtype=torch.complex64, device=kspace.device) if images else None
         out_rss = torch.empty(self.shape, dtype=torch.float32, device=kspace.device) if rss else None
         need = self.workspace_bytes(ch, coils_per_pass)
@@ -485,8 +535,24 @@ def adjoint_multi(plans, kspaces, rss=None, images=None, workspace: torch.Tensor
     n = len(plans)
     if len(kspaces) != n:
         raise ValueError("one k-space per plan")
-    for k in kspaces:
-        _need(k, torch.complex64, "kspace")
+    for name, outs in (("images", images), ("rss", rss)):
+        if outs is not None and len(outs) != n:
+            raise ValueError(f"{name} needs one entry per plan")
+    specs = []
+    for i, (p, k) in enumerate(zip(plans, kspaces)):
+        if k.dim() != 2:
+            raise ValueError(f"kspace {i} must be [C, S]")
+        specs.append((k, torch.complex64, f"kspace {i}", (k.shape[0], p.desc.samples)))
+        rows, cols, ch = p.desc.image_rows, p.desc.image_cols, int(k.shape[0])
+        img = images[i] if images is not None else None
+        out = rss[i] if rss is not None else None
+        if img is None and out is None:
+            raise ValueError(f"item {i} needs images and/or rss")
+        if img is not None:
+            specs.append((img, torch.complex64, f"images {i}", _image_shape(rows, cols, ch, layout)))
+        if out is not None:
+            specs.append((out, torch.float32, f"rss {i}", (rows, cols)))
+    _need_all(*specs)
     channels = [int(k.shape[0]) for k in kspaces]
     need = adjoint_multi_workspace_bytes(plans, channels)
     if workspace is None or workspace.numel() < need:

@@ -565,6 +565,112 @@ int ora_kb_adjoint3(size_t nz, size_t ny, size_t nx, size_t gz, size_t gy, size_
   return 0;
 }
 
+/* One axis of the voxel evaluation: e^{+2 pi i m j / g} for m = 0 .. g-1, j = the crop index
+ * minus n/2 (the grid frequency ora_kb_adjoint3 reads for it), indexed exactly by (m j) mod g. */
+static double* axis_phasors(size_t g, size_t n, size_t count, const uint32_t* idx) {
+  double* e = (double*)malloc(2 * count * g * sizeof(double));
+  for (size_t i = 0; i < count; ++i) {
+    const long j = (long)idx[i] - (long)(n / 2);
+    const size_t jm = (size_t)((j % (long)g + (long)g) % (long)g);
+    for (size_t m = 0; m < g; ++m) {
+      const double a = ORA_TWO_PI * (double)((m * jm) % g) / (double)g;
+      e[2 * (i * g + m)] = cos(a);
+      e[2 * (i * g + m) + 1] = sin(a);
+    }
+  }
+  return e;
+}
+
+/* sum_a t[a] e^{2 pi i (c + a - lead) j / g} for each requested index of one axis. */
+static void axis_sums(const double* e, size_t g, size_t count, int width, int lead, uint32_t c, const double* t,
+                      double* out) {
+  for (size_t i = 0; i < count; ++i) {
+    double sr = 0.0, si = 0.0;
+    for (int a = 0; a < width; ++a) {
+      const size_t m = ((size_t)c + g + (size_t)a - (size_t)lead) % g;
+      sr += t[a] * e[2 * (i * g + m)];
+      si += t[a] * e[2 * (i * g + m) + 1];
+    }
+    out[2 * i] = sr;
+    out[2 * i + 1] = si;
+  }
+}
+
+int ora_kb_adjoint3_voxels(size_t nz, size_t ny, size_t nx, size_t gz, size_t gy, size_t gx, int width, double beta,
+                           double oz, double oy, double ox, size_t s_begin, size_t s_end, const double* traj,
+                           const double* values, const double* dcf, size_t nzs, const uint32_t* zs, size_t nys,
+                           const uint32_t* ys, size_t nxs, const uint32_t* xs, double* out) {
+  const double sz = (double)(nz / 2) - oz, sy = (double)(ny / 2) - oy, sx = (double)(nx / 2) - ox;
+  const int lead = width / 2 - 1;
+  double* ez = axis_phasors(gz, nz, nzs, zs);
+  double* ey = axis_phasors(gy, ny, nys, ys);
+  double* ex = axis_phasors(gx, nx, nxs, xs);
+  double* az = (double*)malloc(2 * nzs * sizeof(double));
+  double* ay = (double*)malloc(2 * nys * sizeof(double));
+  double* ax = (double*)malloc(2 * nxs * sizeof(double));
+  double* acc = (double*)calloc(2 * nzs * nys * nxs, sizeof(double));
+  double tz[16], ty[16], tx[16];
+  int rc = 0;
+  for (size_t s = s_begin; s < s_end; ++s) {
+    const double wz = traj[3 * s], wy = traj[3 * s + 1], wx = traj[3 * s + 2];
+    const double weight = dcf ? dcf[s] : 1.0;
+    const double phase = wz * sz + wy * sy + wx * sx;
+    const double pr = cos(phase), pi = sin(phase);
+    const double yr = values[2 * s] * weight, yi = values[2 * s + 1] * weight;
+    const double zr = yr * pr - yi * pi, zi = yr * pi + yi * pr;
+    uint32_t cz, cy, cx;
+    float f;
+    if (ora_grid_coordinate((float)wz, (uint32_t)gz, &cz, &f) || ora_grid_coordinate((float)wy, (uint32_t)gy, &cy, &f) ||
+        ora_grid_coordinate((float)wx, (uint32_t)gx, &cx, &f)) {
+      rc = -1;
+      break;
+    }
+    const double dz = wrapped_coordinate(wz, gz) - (double)cz, dy = wrapped_coordinate(wy, gy) - (double)cy;
+    const double dx = wrapped_coordinate(wx, gx) - (double)cx;
+    for (int t = 0; t < width; ++t) {
+      tz[t] = ora_kb_kernel((double)(t - lead) - dz, width, beta);
+      ty[t] = ora_kb_kernel((double)(t - lead) - dy, width, beta);
+      tx[t] = ora_kb_kernel((double)(t - lead) - dx, width, beta);
+    }
+    axis_sums(ez, gz, nzs, width, lead, cz, tz, az);
+    axis_sums(ey, gy, nys, width, lead, cy, ty, ay);
+    axis_sums(ex, gx, nxs, width, lead, cx, tx, ax);
+    for (size_t a = 0; a < nzs; ++a) {
+      const double qr = zr * az[2 * a] - zi * az[2 * a + 1], qi = zr * az[2 * a + 1] + zi * az[2 * a];
+      for (size_t b = 0; b < nys; ++b) {
+        const double rr = qr * ay[2 * b] - qi * ay[2 * b + 1], ri = qr * ay[2 * b + 1] + qi * ay[2 * b];
+        double* o = acc + 2 * (a * nys + b) * nxs;
+        for (size_t c = 0; c < nxs; ++c) {
+          o[2 * c] += rr * ax[2 * c] - ri * ax[2 * c + 1];
+          o[2 * c + 1] += rr * ax[2 * c + 1] + ri * ax[2 * c];
+        }
+      }
+    }
+  }
+  if (rc == 0) {
+    for (size_t a = 0; a < nzs; ++a) {
+      const double dza = ora_kb_ft((double)((long)zs[a] - (long)(nz / 2)), gz, width, beta);
+      for (size_t b = 0; b < nys; ++b) {
+        const double dzy = dza * ora_kb_ft((double)((long)ys[b] - (long)(ny / 2)), gy, width, beta);
+        for (size_t c = 0; c < nxs; ++c) {
+          const double d = dzy * ora_kb_ft((double)((long)xs[c] - (long)(nx / 2)), gx, width, beta);
+          const size_t i = (a * nys + b) * nxs + c;
+          out[2 * i] += acc[2 * i] / d;
+          out[2 * i + 1] += acc[2 * i + 1] / d;
+        }
+      }
+    }
+  }
+  free(ez);
+  free(ey);
+  free(ex);
+  free(az);
+  free(ay);
+  free(ax);
+  free(acc);
+  return rc;
+}
+
 void ora_golden_means3(size_t spokes, size_t readout, double* traj) {
   /* 3-D radial "kooshball" with multidimensional golden means (Chan et al., MRM 61
    * 2009): spoke p has polar cosine frac(p g1) and azimuth 2 pi frac(p g2), with

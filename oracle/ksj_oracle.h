@@ -119,6 +119,16 @@ int ora_bin3(size_t samples, const float* traj, uint32_t gz, uint32_t gy, uint32
 int ora_kb_adjoint3(size_t nz, size_t ny, size_t nx, size_t gz, size_t gy, size_t gx, int width, double beta,
                     double oz, double oy, double ox, size_t samples, const double* traj, const double* values,
                     const double* dcf, double* image);
+/* ora_kb_adjoint3 evaluated only at the output voxels of the lattice zs x ys x xs (indices
+ * into the cropped volume), from samples [s_begin, s_end): the DFT of the KB grid is summed
+ * directly at those voxels, per sample as a product of three one-axis sums over its taps,
+ * and divided by the deapodization.  `out` [nzs][nys][nxs] (interleaved) is ADDED to, so
+ * sample ranges can be split across threads.  The grid-cell rule, taps, weight and origin
+ * phase are those of ora_kb_adjoint3.  Returns -1 on an unmappable coordinate. */
+int ora_kb_adjoint3_voxels(size_t nz, size_t ny, size_t nx, size_t gz, size_t gy, size_t gx, int width, double beta,
+                           double oz, double oy, double ox, size_t s_begin, size_t s_end, const double* traj,
+                           const double* values, const double* dcf, size_t nzs, const uint32_t* zs, size_t nys,
+                           const uint32_t* ys, size_t nxs, const uint32_t* xs, double* out);
 /* Golden-means kooshball: spoke p along (frac(p g1), sqrt(1-c^2) sin a, sqrt(1-c^2) cos a),
  * a = 2 pi frac(p g2), g1 = 0.4656, g2 = 0.6823; readout as ora_golden_angle2; [S][3]. */
 void ora_golden_means3(size_t spokes, size_t readout, double* traj);

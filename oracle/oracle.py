@@ -64,6 +64,9 @@ def lib():
         L.ora_bin3.argtypes = [_sz, _fp, C.c_uint32, C.c_uint32, C.c_uint32, _up, _up, _up]
         L.ora_kb_adjoint3.argtypes = [_sz, _sz, _sz, _sz, _sz, _sz, C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_double, _sz, _dp, _dp, _dp, _dp]
+        L.ora_kb_adjoint3_voxels.argtypes = [_sz, _sz, _sz, _sz, _sz, _sz, C.c_int, C.c_double, C.c_double,
+                                             C.c_double, C.c_double, _sz, _sz, _dp, _dp, _dp, _sz, _up, _sz, _up,
+                                             _sz, _up, _dp]
         L.ora_golden_means3.argtypes = [_sz, _sz, _dp]
         L.ora_normalize_trajectory2.argtypes = [_sz, _fp, C.c_int, C.c_uint32, C.c_uint32, _fp]
         L.ora_channel_mix.argtypes = [_sz, _sz, _sz, _dp, _dp, _dp]
@@ -338,6 +341,39 @@ def kb_adjoint3(values, traj, dcf, shape, oversampling=2.0, width=4, beta=None, 
     if rc != 0:
         raise ValueError("trajectory cannot be mapped to the grid")
     return out
+
+
+def kb_adjoint3_voxels(values, traj, dcf, shape, zs, ys, xs, oversampling=2.0, width=4, beta=None, origin=None,
+                       threads: int = 8) -> np.ndarray:
+    """kb_adjoint3 at the voxel lattice zs x ys x xs only ([len(zs), len(ys), len(xs)] complex128),
+    for volumes whose full FP64 oracle would take minutes.  The samples are split over
+    `threads` host threads (the C call releases the GIL) and the partial sums added."""
+    import concurrent.futures as cf
+    nz, ny, nx = shape
+    gz, gy, gx = (int(round(n * oversampling)) for n in shape)
+    if beta is None:
+        beta = kb_beta(width, oversampling)
+    if origin is None:
+        origin = ((nz - 1) / 2.0, (ny - 1) / 2.0, (nx - 1) / 2.0)
+    v = _c128(values).reshape(-1)
+    t = _traj3(traj)
+    S = t.shape[0]
+    d = np.ones(S, np.float64) if dcf is None else np.ascontiguousarray(np.asarray(dcf, np.float64))
+    idx = [np.ascontiguousarray(np.asarray(a, np.uint32)) for a in (zs, ys, xs)]
+    bounds = np.linspace(0, S, max(1, threads) + 1).astype(np.int64)
+
+    def part(i):
+        out = np.zeros((len(idx[0]), len(idx[1]), len(idx[2])), np.complex128)
+        rc = lib().ora_kb_adjoint3_voxels(nz, ny, nx, gz, gy, gx, int(width), float(beta), *map(float, origin),
+                                          int(bounds[i]), int(bounds[i + 1]), t.reshape(-1), v.view(np.float64), d,
+                                          len(idx[0]), idx[0], len(idx[1]), idx[1], len(idx[2]), idx[2],
+                                          out.view(np.float64).reshape(-1))
+        if rc != 0:
+            raise ValueError("trajectory cannot be mapped to the grid")
+        return out
+
+    with cf.ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        return sum(ex.map(part, range(len(bounds) - 1)))
 
 
 def golden_means3(spokes: int, readout: int) -> np.ndarray:

@@ -82,3 +82,33 @@ def test_golden_means3_directions():
     assert np.allclose(d[:, 0], np.mod(p * g1, 1.0), atol=1e-14)
     assert np.allclose(np.mod(np.arctan2(d[:, 1], d[:, 2]), 2 * np.pi)[1:], 2 * np.pi * np.mod(p * g2, 1.0)[1:],
                        atol=1e-12)
+
+
+def test_ramp_sq_dcf3_is_the_squared_norm():
+    """|k|^2 in (rad/px)^2: a KAT, numpy's sum of squares, and not |k| (a |k|-for-|k|^2 slip
+    would pass every test that applies the same weights on both sides)."""
+    kat = O.ramp_sq_dcf3(np.array([[1.0, 2.0, 2.0], [0.0, 0.0, 0.0], [-np.pi, 0.0, 0.0], [0.5, -0.5, 0.5]]))
+    assert np.array_equal(kat, [9.0, 0.0, np.pi ** 2, 0.75])
+    t = ball_trajectory(5000, 8)
+    d = O.ramp_sq_dcf3(t)
+    assert np.allclose(d, (t ** 2).sum(axis=1), rtol=1e-15, atol=0)
+    assert not np.allclose(d, np.linalg.norm(t, axis=1))
+
+
+@pytest.mark.parametrize("shape,width", [((8, 8, 8), 4), ((6, 10, 12), 6), ((9, 7, 5), 2)])
+def test_kb_adjoint3_voxels_equal_the_full_volume(shape, width):
+    """The lattice evaluation (used at sizes where the full FP64 volume takes minutes) is
+    the same sum as kb_adjoint3 at those voxels, also with the samples split over threads."""
+    t = ball_trajectory(700, 9)
+    rng = np.random.default_rng(10)
+    y = rng.standard_normal(t.shape[0]) + 1j * rng.standard_normal(t.shape[0])
+    dcf = O.ramp_sq_dcf3(t)
+    full = O.kb_adjoint3(y, t, dcf, shape, width=width)
+    zs, ys, xs = [0, shape[0] // 2, shape[0] - 1], [1, shape[1] - 2], list(range(shape[2]))
+    ref = full[np.ix_(zs, ys, xs)]
+    for threads in (1, 3):
+        got = O.kb_adjoint3_voxels(y, t, dcf, shape, zs, ys, xs, width=width, threads=threads)
+        assert rel_l2(got, ref) <= 1e-12
+    # a wrong crop index (the voxel next to it) is caught
+    other = O.kb_adjoint3_voxels(y, t, dcf, shape, zs, ys, [x + 1 for x in xs[:-1]] + [0], width=width)
+    assert rel_l2(other, ref) > 1e-3
