@@ -117,8 +117,14 @@ def test_argument_checks(ops):
     x = dev(np.zeros((65, 4), np.complex64))
     with pytest.raises(RuntimeError, match="channel counts"):
         ops.channel_mix(dev(np.zeros((1, 65), np.complex64)), x)
-    with pytest.raises(RuntimeError, match="crop range"):
+    with pytest.raises(ValueError, match="inside the readout"):  # the wrapper checks first
         ops.readout_crop(dev(np.zeros((2, 2, 4), np.complex64)), 3, 2)
+    # the library checks too, for callers of the C ABI
+    from kspacejet_amd import _lib
+    src, dst = dev(np.zeros((2, 2, 4), np.complex64)), dev(np.zeros((2, 2, 3), np.complex64))
+    with pytest.raises(RuntimeError, match="crop range"):
+        _lib.call("ksj_gpu_readout_crop", src.data_ptr(), dst.data_ptr(), 4, 4, 3, 2,
+                  torch.cuda.current_stream().cuda_stream)
 
 
 # ---------------------------------------------------------------- through the Provider ABI
