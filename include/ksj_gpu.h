@@ -165,6 +165,23 @@ size_t ksj_gpu_adjoint2_workspace_bytes(const ksj_gpu_plan2* plan, uint32_t chan
 int ksj_gpu_adjoint2_execute(const ksj_gpu_plan2* plan, const void* d_kspace, uint32_t channels, void* d_images,
                              int layout, float* d_rss, void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* The same adjoint for callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, hosts that forbid hipMalloc.  No counterpart in the
+ * reference (its operators run on host memory); it exists for those callers only.
+ * The workspace is a file-scope device array of KSJ_GPU_STATIC_SCRATCH_BYTES (128 MiB) per
+ * GPU, held until the process ends; the call makes no hipMalloc/hipFree and no other
+ * allocating call (the plan's FFT tables are made when its trajectory is set).  It fails
+ * with KSJ_GPU_RESOURCE_EXHAUSTED, before enqueuing anything, when
+ * ksj_gpu_adjoint2_workspace_bytes(plan, channels) exceeds the array (the 256x256,
+ * 16-channel, 205,824-sample scan needs about 90 MB; 64 channels at that shape do not fit).
+ * Every call on one GPU uses that GPU's one array: calls that overlap on one GPU (another
+ * stream or thread before the previous call's work has finished) race and must be
+ * serialised by the caller.  The array keeps the last call's contents; no kernel relies on
+ * it starting at zero.  Results are bit-identical to ksj_gpu_adjoint2_execute. */
+#define KSJ_GPU_STATIC_SCRATCH_BYTES (128u * 1024u * 1024u)
+int ksj_gpu_adjoint2_execute_static(const ksj_gpu_plan2* plan, const void* d_kspace, uint32_t channels,
+                                    void* d_images, int layout, float* d_rss, void* stream);
+
 /* Batched adjoint of `count` scans that share the plan's trajectory: the scans are
  * stacked along the coil axis (up to 64 coils per gridding pass), so one pass of
  * gather/FFT/deapodize kernels serves several scans.  d_kspace[i] is scan i's

@@ -20,6 +20,7 @@ DENSITY_NONE = 0
 DENSITY_RAMP = 1
 DENSITY_GIVEN = 2
 DENSITY_RAMP_SQUARED = 3
+STATIC_SCRATCH_BYTES = 128 * 1024 * 1024
 
 
 class KsjGpuError(RuntimeError):
@@ -113,6 +114,7 @@ SIGNATURES = {
     "ksj_gpu_plan2_destroy": (None, [_vp]),
     "ksj_gpu_adjoint2_workspace_bytes": (_sz, [_vp, _u32]),
     "ksj_gpu_adjoint2_execute": (_int, [_vp, _vp, _u32, _vp, _int, _vp, _vp, _sz, _vp]),
+    "ksj_gpu_adjoint2_execute_static": (_int, [_vp, _vp, _u32, _vp, _int, _vp, _vp]),
     "ksj_gpu_plan3_create": (_int, [C.POINTER(Plan3Desc), C.POINTER(_vp)]),
     "ksj_gpu_plan3_set_trajectory": (_int, [_vp, _vp, _vp, _vp]),
     "ksj_gpu_plan3_get_info": (_int, [_vp, C.POINTER(Plan3Info)]),

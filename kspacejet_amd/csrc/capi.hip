@@ -14,6 +14,7 @@
 #include "serve.hpp"
 
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -61,6 +62,31 @@ template <typename F> int guarded(F&& f) {
 
 hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
+// Workspace of ksj_gpu_adjoint2_execute_static: a file-scope device array, one copy per
+// GPU the library runs on, held until the process ends.  In host code `scratch` names a
+// host-side stand-in, never device memory: its device address on each GPU comes from
+// hipGetSymbolAddress (at ksj_gpu_init, or at the entry point's first call on a device)
+// and only that address is handed to kernels.
+constexpr int kMaxScratchDevices = 64;
+__device__ __align__(256) unsigned char scratch[KSJ_GPU_STATIC_SCRATCH_BYTES];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once per device).
+void* scratch_for_current_device() {
+  int device = 0;
+  KSJ_HIP(hipGetDevice(&device));
+  ksj::gpu::require(device >= 0 && device < kMaxScratchDevices, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device] == nullptr) {
+    void* ptr = nullptr;
+    KSJ_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    ksj::gpu::require(ptr != nullptr, "scratch array has no device address");
+    g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
+
 }  // namespace
 
 extern "C" {
@@ -72,6 +98,7 @@ int ksj_gpu_init(int device_ordinal) {
     ksj::gpu::require(device_ordinal >= 0 && device_ordinal < count, "device ordinal out of range");
     KSJ_HIP(hipSetDevice(device_ordinal));
     KSJ_HIP(hipFree(nullptr));
+    scratch_for_current_device();
   });
 }
 
@@ -194,6 +221,10 @@ int ksj_gpu_plan2_set_trajectory(ksj_gpu_plan2* plan, const float* d_traj, const
   return guarded([&] {
     ksj::gpu::require(plan && plan->impl && d_traj, "null argument");
     ksj::gpu::plan2_set_trajectory(plan->impl, d_traj, d_dcf, as_stream(stream));
+    // The FFT tables of the plan's grid lengths exist from here on, so executing the
+    // plan allocates nothing (ksj_gpu_adjoint2_execute_static relies on it).
+    ksj::gpu::fft_twiddles(plan->impl->d.grid_rows, as_stream(stream));
+    ksj::gpu::fft_twiddles(plan->impl->d.grid_cols, as_stream(stream));
   });
 }
 
@@ -245,6 +276,27 @@ int ksj_gpu_adjoint2_execute(const ksj_gpu_plan2* plan, const void* d_kspace, ui
     scan.images = static_cast<float2*>(d_images);
     scan.rss = d_rss;
     ksj::gpu::adjoint2_execute(*plan->impl, &scan, 1, channels, layout, d_workspace, workspace_bytes,
+                               as_stream(stream));
+  });
+}
+
+int ksj_gpu_adjoint2_execute_static(const ksj_gpu_plan2* plan, const void* d_kspace, uint32_t channels,
+                                    void* d_images, int layout, float* d_rss, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(plan && plan->impl && d_kspace, "null argument");
+    ksj::gpu::require(d_images || d_rss, "adjoint needs an output");
+    ksj::gpu::require(channels >= 1 && channels <= 64, "adjoint supports 1..64 channels");
+    const size_t need = ksj::gpu::adjoint2_workspace_bytes(*plan->impl, channels, 1);
+    if (need > KSJ_GPU_STATIC_SCRATCH_BYTES)
+      throw ksj::gpu::Error(ksj::gpu::kErrExhausted,
+                            "workspace of " + std::to_string(need) + " bytes exceeds the static scratch of " +
+                              std::to_string(static_cast<size_t>(KSJ_GPU_STATIC_SCRATCH_BYTES)) + " bytes");
+    void* ws = scratch_for_current_device();
+    ksj::gpu::AdjointScan scan;
+    scan.kspace = static_cast<const float2*>(d_kspace);
+    scan.images = static_cast<float2*>(d_images);
+    scan.rss = d_rss;
+    ksj::gpu::adjoint2_execute(*plan->impl, &scan, 1, channels, layout, ws, KSJ_GPU_STATIC_SCRATCH_BYTES,
                                as_stream(stream));
   });
 }
