@@ -90,8 +90,11 @@ def gather_kernel_label() -> str:
     """Template instance of the 64-coil gather kernel the current tuning launches."""
     from kspacejet_amd import ops
     if ops.tune_get("gather_scheme") == 2:
-        return (f"gather_matrix_kernel<64,{ops.tune_get('matrix_batch') // 4},{ops.tune_get('matrix_pipeline')},"
-                f"{ops.tune_get('matrix_block_width') // 4}>")
+        taps = ops.tune_get("matrix_weights") == 1
+        share = taps and ops.tune_get("matrix_tap_share") == 1
+        pipeline = 1 if taps else ops.tune_get("matrix_pipeline")  # the packed-visit loop keeps one batch in flight
+        return (f"gather_matrix_kernel<64,{ops.tune_get('matrix_batch') // 4},{pipeline},"
+                f"{ops.tune_get('matrix_block_width') // 4},{str(taps).lower()},{str(share).lower()}>")
     two = ops.tune_get("gather_coils_per_lane") == 2
     return f"gather_item_kernel<{32 if two else 64},{ops.tune_get('gather_batch')},{2 if two else 1}>"
 

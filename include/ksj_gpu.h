@@ -426,7 +426,7 @@ void ksj_gpu_serve_destroy(ksj_gpu_server* server);
  *   "gather_scheme"        0 = 2x2 blocks, 1 = 64-cell LDS tiles, 2 = 4x4 blocks on the matrix
  *                          cores (new plans; summation order)
  *   "matrix_run_cost"      1..64: work per gather wave of the matrix schedule, in groups of four
- *                          visits plus one per item (new plans)
+ *                          visits plus one per item (new plans; default 12)
  *   "matrix_item_visits"   1..4096: work-item size of new matrix-schedule plans (summation order)
  *   "matrix_batch"         4, 8 or 16 visits per batch of the matrix gathers
  *   "matrix_pipeline"      1 or 2 batches of sample rows in flight per wave (2-D matrix gather)
@@ -441,6 +441,14 @@ void ksj_gpu_serve_destroy(ksj_gpu_server* server);
  *   "tile_item_samples"    window samples per tile work item (new plans; summation order)
  *   "mix_scheme"           1 = ksj_gpu_channel_mix on the matrix cores (default), 0 = vector FMAs
  *                          (summation order)
+ *   "matrix_persistent"    0 = one gather wave per run (default); 1..8 = that many resident workgroups per CU,
+ *                          each XCD's waves looping over a contiguous eighth of the runs (same bits)
+ *   "matrix_tap_share"     1 = each gather lane loads one word of a visit's taps and takes its two taps from
+ *                          the visit's lanes by bpermute, the packed visits of a batch likewise (default);
+ *                          0 = each lane loads its own taps and visits (same bits)
+ *   "matrix_weights"       1 = new 2-D matrix plans store a 4-byte (sample, window bin) visit and the
+ *                          gather forms the cell weights from the sample's taps (default), 0 = every
+ *                          visit stores its 16 or 32 cell weights (same bits; plan size and speed only)
  * Every setting gives bit-identical results, except item_visits, matrix_item_visits,
  * matrix3_item_visits, matrix_block_width, the tile keys, gather_scheme and gather3_scheme (float summation
  * order of the gather), fft_schedule (FFT rounding) and mix_scheme (summation order of the

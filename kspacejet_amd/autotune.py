@@ -30,7 +30,9 @@ EXACT_KEYS: Dict[str, tuple] = {
     "matrix_batch": (16, 8, 4),
     "matrix_slab": (16, 1, 64),
     "matrix_pipeline": (1, 2),
-    "matrix_run_cost": (8, 16, 4),
+    "matrix_run_cost": (12, 8, 16),
+    "matrix_tap_share": (1, 0),
+    "matrix_persistent": (0, 8),
     "gather_map": (0, 1),
     "fft_lines_per_block": (0, 8, 32),
     "pass_lanes": (2, 1, 3),

@@ -234,7 +234,8 @@ int ksj_gpu_plan2_get_info(const ksj_gpu_plan2* plan, ksj_gpu_plan2_info* out_in
     ksj::gpu::require(out_info->struct_size >= sizeof(ksj_gpu_plan2_info), "info struct too small");
     out_info->items = plan->impl->item_count;
     out_info->visits = plan->impl->visit_count;
-    const uint64_t cells = static_cast<uint64_t>(plan->impl->block_h) * plan->impl->block_w;
+    const uint64_t cells =
+      plan->impl->packed_visits ? 0 : static_cast<uint64_t>(plan->impl->block_h) * plan->impl->block_w;
     out_info->device_bytes = plan->impl->device_bytes +
                              plan->impl->visit_capacity * (sizeof(uint32_t) + cells * sizeof(float)) +
                              static_cast<uint64_t>(plan->impl->item_capacity) * (sizeof(uint4) + sizeof(uint32_t));

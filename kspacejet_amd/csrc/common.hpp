@@ -89,9 +89,15 @@ enum TuneKey : int {
   kTuneMatrix3ItemVisits, // 3-D matrix schedule: work-item size of new plans (visits; summation order changes)
   kTuneMatrix3RunCost,    // 3-D matrix schedule of new plans: cost per wave run, 1..64
   kTuneMixScheme,         // channel mix: 0 = vector FMAs, 1 = matrix cores (summation order changes)
+  kTuneMatrixWeights,     // 2-D matrix schedule of new plans: 0 = 16 (or 32) cell weights stored per visit,
+                          // 1 = packed (sample, window position) visits, weights formed from the taps (same bits)
+  kTuneMatrixPersistent,  // 2-D matrix gather: 0 = one wave per run; n = n resident workgroups per CU looping over
+                          // their XCD's runs (same bits)
+  kTuneMatrixTapShare,    // 2-D matrix gather, packed visits: 1 = one tap load per lane shared by bpermute (same bits)
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);
+uint32_t device_cu_count();  // compute units of the current device (cached per device)
 void tune_set(TuneKey key, int64_t value);  // throws Error on an illegal value
 int tune_key(const char* name);             // -1 for an unknown name
 

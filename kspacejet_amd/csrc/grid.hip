@@ -116,6 +116,10 @@ struct GatherArgs {
   uint32_t run_count;
   uint32_t nt_store;                // matrix schedule: grid cells written with non-temporal stores
   uint32_t slab;                    // matrix schedule: workgroups per XCD slab (gather_map 0)
+  uint32_t persist;                 // matrix schedule: 1 = resident waves loop over their XCD's runs
+  const float* taps;                // [S][2W] sorted taps (matrix schedule with packed visits)
+  uint32_t width;                   // W
+  uint32_t from_taps;               // 1: visit_k packs (sample, window position), weights formed from taps
 };
 
 __device__ inline void fma4(float2 (&acc)[4], float4 w, float2 d) {
@@ -374,10 +378,18 @@ __global__ __launch_bounds__(256) void multi_reduce_kernel(GatherArgs a) {
 // cut into several items) and its accumulators cleared; items without visits
 // (blocks no sample reaches) write zeros.  Lane t holds item t's record; the wave
 // reads it with readlane.
-template <int CP, int U, int P, int T>
+//
+// WT (weights from taps): a visit is one packed word, sorted sample k (bits 0-23) and the
+// sample's bin in the block's window, row r (bits 24-27) and column c (bits 28-31).  The
+// lane forms its cell's weight as row tap (dy - r + W - 1) times column tap
+// (dx - c + W - 1) of the sample (0 outside [0, W)), the same float product the weight
+// table holds, so both forms give the same bits.  Padding visits carry r = c = 15, which
+// no cell reaches.  4 bytes of metadata per visit instead of 4 + 4 T x 16.
+template <int CP, int U, int P, int T, bool WT, bool TS>
 __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
   constexpr int NF = CP / 8;
   constexpr int kCells = 16 * T;  // 4 x 4 (T = 1) or 4 x 8 (T = 2) cells, T products of 16 cells each
+  constexpr uint32_t kKeyMask = WT ? 0x00FFFFFFu : 0xFFFFFFFFu;
   const int lane = threadIdx.x & 63, q = lane >> 4, j = lane & 15;
   const uint32_t i0 = a.run_first[run], i1 = a.run_first[run + 1];  // run is wave-uniform: scalar loads
   if (i0 >= i1) return;
@@ -389,7 +401,7 @@ __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
   const uint32_t c0 = field(rec.y, 0) / 4, c1 = field(rec.z, nitems - 1) / 4;
   const uint32_t last_visit = c1 > c0 ? 4 * c1 - 1 : 0;
   const float* wts = reinterpret_cast<const float*>(a.visit_w);
-  const float* rows = reinterpret_cast<const float*>(a.sorted) + j * NF;
+  const float* rows = reinterpret_cast<const float*>(a.sorted) + piece_offset<NF>(j);
   MfmaAcc acc[T][NF];
 #pragma unroll
   for (int h = 0; h < T; ++h)
@@ -414,7 +426,7 @@ __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
           const uint32_t y = 4 * by + cell / (4 * T), x = 4 * T * bx + cell % (4 * T);
           if (y < a.grid_rows && x < a.grid_cols) {
             float* dst =
-              reinterpret_cast<float*>(a.grid) + (static_cast<uint64_t>(y) * a.grid_cols + x) * (2 * CP) + j * NF;
+              reinterpret_cast<float*>(a.grid) + (static_cast<uint64_t>(y) * a.grid_cols + x) * (2 * CP) + piece_offset<NF>(j);
             if (a.nt_store)
               store_piece_nt<NF>(dst, o);
             else
@@ -422,7 +434,7 @@ __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
           }
         } else {
           store_piece<NF>(
-            reinterpret_cast<float*>(a.partial) + (static_cast<uint64_t>(slot) * kCells + cell) * (2 * CP) + j * NF,
+            reinterpret_cast<float*>(a.partial) + (static_cast<uint64_t>(slot) * kCells + cell) * (2 * CP) + piece_offset<NF>(j),
             o);
         }
       }
@@ -439,13 +451,35 @@ __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
       const uint32_t vi = 4 * (c + u) + q;
       const uint32_t vc = vi < last_visit ? vi : last_visit;
       k[u] = a.visit_k[vc];
+      if constexpr (!WT) {
 #pragma unroll
-      for (int h = 0; h < T; ++h) w[u][h] = wts[static_cast<uint64_t>(vc) * kCells + 16 * h + j];
+        for (int h = 0; h < T; ++h) w[u][h] = wts[static_cast<uint64_t>(vc) * kCells + 16 * h + j];
+      }
     }
   };
   auto load_rows = [&](const uint32_t (&k)[U], RowPiece<NF> (&b)[U]) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) b[u] = load_piece<NF>(rows + static_cast<uint64_t>(k[u]) * (2 * CP));
+    for (int u = 0; u < U; ++u) b[u] = load_piece<NF>(rows + static_cast<uint64_t>(k[u] & kKeyMask) * (2 * CP));
+  };
+  // WT: the lane's row and column tap of each visit (a clamped, always in-bounds index;
+  // `live` marks the taps inside [0, W)).  Cell 16 h + j is block row (16 h + j) / (4 T),
+  // column (16 h + j) % (4 T).
+  const int W = static_cast<int>(a.width);
+  auto load_taps = [&](const uint32_t (&k)[U], float (&tr)[U][T], float (&tc)[U][T], bool (&live)[U][T]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const float* t = a.taps + static_cast<uint64_t>(k[u] & 0x00FFFFFFu) * (2 * W);
+      const int r = static_cast<int>((k[u] >> 24) & 15u), cw = static_cast<int>(k[u] >> 28);
+#pragma unroll
+      for (int h = 0; h < T; ++h) {
+        const int cell = 16 * h + j;
+        const int ir = cell / (4 * T) - r + W - 1, ic = cell % (4 * T) - cw + W - 1;
+        live[u][h] = static_cast<unsigned>(ir) < static_cast<unsigned>(W) &&
+                     static_cast<unsigned>(ic) < static_cast<unsigned>(W);
+        tr[u][h] = t[live[u][h] ? ir : 0];
+        tc[u][h] = t[W + (live[u][h] ? ic : 0)];
+      }
+    }
   };
 
   uint32_t t = 0, t_end = field(rec.z, 0) / 4;  // current item and its end group
@@ -474,7 +508,77 @@ __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
   };
 
   uint32_t c = c0;
-  if constexpr (P == 1) {
+  if constexpr (WT) {
+    // One batch of rows and taps in flight, then the positions of batch c + U.  With TS
+    // the batch's 4 U packed visits are loaded one per lane (lane l: visit l mod 4 U) and
+    // spread to the lanes of each group by bpermute.
+    uint32_t k[U];
+    float w[U][T];
+    uint32_t kw = 0;
+    auto load_keys = [&](uint32_t cc) {
+      const uint32_t vi = 4 * cc + (static_cast<uint32_t>(lane) & (4 * U - 1));
+      kw = a.visit_k[vi < last_visit ? vi : last_visit];
+    };
+    if (c < c1) {
+      if constexpr (TS)
+        load_keys(c);
+      else
+        load_meta(c, k, w);
+    }
+    for (; c < c1; c += U) {
+      RowPiece<NF> b[U];
+      float tr[U][T], tc[U][T], wc[U][T];
+      bool live[U][T];
+      if constexpr (TS) {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          k[u] = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(4 * (4 * u + q), static_cast<int>(kw)));
+      }
+      load_rows(k, b);
+      if constexpr (TS) {
+        // One tap word per lane (lane j of visit q's 16 holds tap min(j, 2W - 1)); each
+        // lane then takes its row and column tap from the visit's lanes by bpermute.
+        uint32_t word[U];
+        int src_r[U][T], src_c[U][T];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const float* t = a.taps + static_cast<uint64_t>(k[u] & 0x00FFFFFFu) * (2 * W);
+          word[u] = __float_as_uint(t[j < 2 * W ? j : 2 * W - 1]);
+          const int r = static_cast<int>((k[u] >> 24) & 15u), cw = static_cast<int>(k[u] >> 28);
+#pragma unroll
+          for (int h = 0; h < T; ++h) {
+            const int cell = 16 * h + j;
+            const int ir = cell / (4 * T) - r + W - 1, ic = cell % (4 * T) - cw + W - 1;
+            live[u][h] = static_cast<unsigned>(ir) < static_cast<unsigned>(W) &&
+                         static_cast<unsigned>(ic) < static_cast<unsigned>(W);
+            src_r[u][h] = 4 * (16 * q + (live[u][h] ? ir : 0));
+            src_c[u][h] = 4 * (16 * q + W + (live[u][h] ? ic : 0));
+          }
+        }
+        load_keys(c + U);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int h = 0; h < T; ++h) {
+            const float pr = __uint_as_float(static_cast<uint32_t>(
+              __builtin_amdgcn_ds_bpermute(src_r[u][h], static_cast<int>(word[u]))));
+            const float pc = __uint_as_float(static_cast<uint32_t>(
+              __builtin_amdgcn_ds_bpermute(src_c[u][h], static_cast<int>(word[u]))));
+            wc[u][h] = live[u][h] ? pr * pc : 0.f;
+          }
+      } else {
+        load_taps(k, tr, tc, live);
+        load_meta(c + U, k, w);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int h = 0; h < T; ++h) wc[u][h] = live[u][h] ? tr[u][h] * tc[u][h] : 0.f;
+      }
+      compute(c, b, wc);
+    }
+  } else if constexpr (P == 1) {
     // One batch of rows in flight: rows of batch c, then metadata of batch c + U.
     uint32_t k[U];
     float w[U][T];
@@ -525,9 +629,22 @@ __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
 // XCDs (slab i on XCD i % 8), so each slab's neighbouring blocks share one L2 while
 // every XCD still sweeps the whole grid; 1 or 2: workgroup w (dispatched to XCD w % 8)
 // takes the w/8-th group of an equal share of runs (runs are already of equal cost).
-template <int CP, int U, int P, int T>
+//
+// Persistent form (matrix_persistent > 0): only as many workgroups as the CUs hold at
+// once are launched, wg_per_xcd per XCD; XCD x takes the x-th eighth of the runs (a
+// contiguous stretch of blocks, whose shared sample rows stay in that XCD's L2) and its
+// waves stride through it.  Each run is computed exactly as in the one-wave-per-run
+// form, so the results are the same bits.
+template <int CP, int U, int P, int T, bool WT, bool TS>
 __global__ __launch_bounds__(256) void gather_matrix_kernel(GatherArgs a) {
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (a.persist) {
+    const uint32_t x = blockIdx.x % kXcds, l = blockIdx.x / kXcds;
+    const uint32_t per = (a.run_count + kXcds - 1) / kXcds;
+    const uint32_t r1 = min(a.run_count, (x + 1) * per);
+    for (uint32_t run = x * per + 4 * l + wave; run < r1; run += 4 * a.wg_per_xcd) matrix_run<CP, U, P, T, WT, TS>(a, run);
+    return;
+  }
   uint32_t wg = blockIdx.x;
   if (a.xcd_chunk) {
     wg = (blockIdx.x % kXcds) * a.wg_per_xcd + blockIdx.x / kXcds;
@@ -536,7 +653,7 @@ __global__ __launch_bounds__(256) void gather_matrix_kernel(GatherArgs a) {
     wg = ((l / a.slab) * kXcds + x) * a.slab + l % a.slab;
   }
   const uint32_t run = wg * 4 + wave;
-  if (run < a.run_count) matrix_run<CP, U, P, T>(a, run);
+  if (run < a.run_count) matrix_run<CP, U, P, T, WT, TS>(a, run);
 }
 
 // ----------------------------------------------------------------------------- tile gather
@@ -845,13 +962,13 @@ __global__ __launch_bounds__(256) void gather_fused_kernel(const FusedTable<Gath
   gather_one_item<SW, B, CPL>(a, item, item < a.item_count, sw, c);
 }
 
-template <int CP, int U, int P, int T>
+template <int CP, int U, int P, int T, bool WT, bool TS>
 __global__ __launch_bounds__(256) void gather_matrix_fused_kernel(const FusedTable<GatherArgs>* t) {
   const uint32_t it = fused_item(t, blockIdx.x);
   const GatherArgs a = fused_args(t, it);
   const uint32_t wg = static_cast<uint32_t>(blockIdx.x - fused_first(t, it));
   const uint32_t run = wg * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (run < a.run_count) matrix_run<CP, U, P, T>(a, run);
+  if (run < a.run_count) matrix_run<CP, U, P, T, WT, TS>(a, run);
 }
 
 __global__ __launch_bounds__(256) void reduce_fused_kernel(const FusedTable<GatherArgs>* t) {
@@ -878,31 +995,44 @@ void launch_gather_matrix(GatherArgs a, hipStream_t stream) {
   uint32_t wgs = ceil_div(a.run_count, 4);
   a.xcd_chunk = static_cast<uint32_t>(tune_get(kTuneGatherMap));
   a.slab = static_cast<uint32_t>(tune_get(kTuneMatrixSlab));
-  if (a.xcd_chunk) {
+  a.persist = 0;
+  const uint32_t per_cu = static_cast<uint32_t>(tune_get(kTuneMatrixPersistent));
+  if (per_cu > 0 && wgs > per_cu * device_cu_count()) {
+    a.persist = 1;
+    a.wg_per_xcd = per_cu * ceil_div(device_cu_count(), kXcds);
+    wgs = a.wg_per_xcd * kXcds;
+  } else if (a.xcd_chunk) {
     a.wg_per_xcd = ceil_div(wgs, kXcds);
     wgs = a.wg_per_xcd * kXcds;  // the padding workgroups find no run and exit
   } else if (a.slab > 1) {
     wgs = ceil_div(wgs, kXcds * a.slab) * kXcds * a.slab;
   }
   a.nt_store = static_cast<uint32_t>(tune_get(kTuneMatrixNtStore));
-  with_matrix_shape(a.bw, [&]<int U, int P, int T>() {
-    hipLaunchKernelGGL((gather_matrix_kernel<CP, U, P, T>), dim3(wgs), dim3(256), 0, stream, a);
+  with_matrix_shape(a.bw, a.from_taps != 0, [&]<int U, int P, int T, bool WT, bool TS>() {
+    hipLaunchKernelGGL((gather_matrix_kernel<CP, U, P, T, WT, TS>), dim3(wgs), dim3(256), 0, stream, a);
   });
   KSJ_LAUNCH_CHECK();
 }
 
-// Calls f.template operator()<U, P, T>() for the tuned batch (U groups of four
-// visits), pipeline depth and the plans' block width (T products of 16 cells).
+// Calls f.template operator()<U, P, T, WT, TS>() for the tuned batch (U groups of four
+// visits), pipeline depth, the plans' block width (T products of 16 cells) and visit
+// form (WT: packed positions, weights from taps; its loop keeps one batch in flight; TS:
+// each lane loads one tap word and takes its two taps from the visit's lanes).
 template <typename F>
-void with_matrix_shape(uint32_t block_w, F&& f) {
+void with_matrix_shape(uint32_t block_w, bool from_taps, F&& f) {
   const int u = static_cast<int>(tune_get(kTuneMatrixBatch)) / 4;
   const bool two = tune_get(kTuneMatrixPipeline) == 2;
   const bool wide = block_w == 8;
+  const bool share = tune_get(kTuneMatrixTapShare) == 1;
   auto by_p = [&]<int U>() {
-    if (two) {
-      if (wide) f.template operator()<U, 2, 2>(); else f.template operator()<U, 2, 1>();
+    if (from_taps && share) {
+      if (wide) f.template operator()<U, 1, 2, true, true>(); else f.template operator()<U, 1, 1, true, true>();
+    } else if (from_taps) {
+      if (wide) f.template operator()<U, 1, 2, true, false>(); else f.template operator()<U, 1, 1, true, false>();
+    } else if (two) {
+      if (wide) f.template operator()<U, 2, 2, false, false>(); else f.template operator()<U, 2, 1, false, false>();
     } else {
-      if (wide) f.template operator()<U, 1, 2>(); else f.template operator()<U, 1, 1>();
+      if (wide) f.template operator()<U, 1, 2, false, false>(); else f.template operator()<U, 1, 1, false, false>();
     }
   };
   if (u == 1) by_p.template operator()<1>();
@@ -911,13 +1041,13 @@ void with_matrix_shape(uint32_t block_w, F&& f) {
 }
 
 template <int CP>
-void launch_gather_matrix_fused(const FusedTableBuilder<GatherArgs>& b, uint32_t block_w, void* d_table,
-                                hipStream_t stream) {
+void launch_gather_matrix_fused(const FusedTableBuilder<GatherArgs>& b, uint32_t block_w, bool from_taps,
+                                void* d_table, hipStream_t stream) {
   b.upload(d_table, stream);
   const auto* t = static_cast<const FusedTable<GatherArgs>*>(d_table);
   const dim3 grid(static_cast<uint32_t>(b.blocks())), block(256);
-  with_matrix_shape(block_w, [&]<int U, int P, int T>() {
-    hipLaunchKernelGGL((gather_matrix_fused_kernel<CP, U, P, T>), grid, block, 0, stream, t);
+  with_matrix_shape(block_w, from_taps, [&]<int U, int P, int T, bool WT, bool TS>() {
+    hipLaunchKernelGGL((gather_matrix_fused_kernel<CP, U, P, T, WT, TS>), grid, block, 0, stream, t);
   });
   KSJ_LAUNCH_CHECK();
 }
@@ -1003,6 +1133,9 @@ GatherArgs gather_args(const Plan2& plan, const float2* d_sorted, uint32_t coils
   a.bw = plan.block_w;
   a.run_first = plan.run_first;
   a.run_count = plan.run_count;
+  a.taps = plan.taps;
+  a.width = plan.d.width;
+  a.from_taps = plan.packed_visits ? 1u : 0u;
   return a;
 }
 }  // namespace
@@ -1072,6 +1205,7 @@ void grid_fused(const GridFusedItem* items, uint32_t count, void* d_tables, hipS
   FusedTableBuilder<PrepItemArgs> prep[4];
   FusedTableBuilder<GatherArgs> gather[4], matrix[4], reduce;
   uint32_t matrix_w = 0;  // block width of the batch's matrix plans (one width per fused call)
+  int matrix_taps = -1;   // visit form of the batch's matrix plans (one form per fused call)
   const uint32_t nt_store = static_cast<uint32_t>(tune_get(kTuneMatrixNtStore));
   for (uint32_t i = 0; i < count; ++i) {
     const GridFusedItem& it = items[i];
@@ -1094,7 +1228,9 @@ void grid_fused(const GridFusedItem* items, uint32_t count, void* d_tables, hipS
     ga.xcd_chunk = 0;
     if (it.plan->scheme == kSchemeMatrix) {
       require(matrix_w == 0 || matrix_w == it.plan->block_w, "fused matrix items share one block width");
+      require(matrix_taps < 0 || matrix_taps == static_cast<int>(ga.from_taps), "fused matrix items share one visit form");
       matrix_w = it.plan->block_w;
+      matrix_taps = static_cast<int>(ga.from_taps);
       ga.nt_store = nt_store;
       matrix[cls].add(ga, std::max<uint64_t>(1, ceil_div(ga.run_count, 4)));
     } else {
@@ -1136,10 +1272,10 @@ void grid_fused(const GridFusedItem* items, uint32_t count, void* d_tables, hipS
     void* d = tab + cls * fused_table_bytes<GatherArgs>();
     if (matrix[cls].t.count == 0) continue;
     switch (cls) {
-      case 0: launch_gather_matrix_fused<8>(matrix[cls], matrix_w, d, stream); break;
-      case 1: launch_gather_matrix_fused<16>(matrix[cls], matrix_w, d, stream); break;
-      case 2: launch_gather_matrix_fused<32>(matrix[cls], matrix_w, d, stream); break;
-      default: launch_gather_matrix_fused<64>(matrix[cls], matrix_w, d, stream); break;
+      case 0: launch_gather_matrix_fused<8>(matrix[cls], matrix_w, matrix_taps == 1, d, stream); break;
+      case 1: launch_gather_matrix_fused<16>(matrix[cls], matrix_w, matrix_taps == 1, d, stream); break;
+      case 2: launch_gather_matrix_fused<32>(matrix[cls], matrix_w, matrix_taps == 1, d, stream); break;
+      default: launch_gather_matrix_fused<64>(matrix[cls], matrix_w, matrix_taps == 1, d, stream); break;
     }
   }
   if (reduce.t.count > 0) {

@@ -92,7 +92,8 @@ struct ScheduleArgs {
   uint32_t* row_off;  // counts, then offsets
   const uint32_t* chunk_off;  // matrix schedule: first group of four of each block (null otherwise)
   uint32_t* visit_k;
-  float* visit_w;     // [visits][h*w]
+  float* visit_w;     // [visits][h*w], or null for packed visits
+  int packed;         // visit_k = k | r << 24 | c << 28 (window bin (r, c)); no weights
 };
 
 __device__ inline int window_rows(const ScheduleArgs& a) { return a.width + a.bh - 1; }
@@ -153,6 +154,10 @@ __global__ void visit_fill_kernel(ScheduleArgs a) {
   for (int c = 0; c < window_cols(a); ++c) {
     const uint64_t b = window_bin(a, blk, r, c);
     for (uint32_t k = a.start[b]; k < a.start[b + 1]; ++k) {
+      if (a.packed) {
+        a.visit_k[pos++] = k | static_cast<uint32_t>(r) << 24 | static_cast<uint32_t>(c) << 28;
+        continue;
+      }
       const float* t = a.taps + static_cast<uint64_t>(k) * (2 * W);
       float* w = a.visit_w + static_cast<uint64_t>(pos) * h * wd;
       for (int dy = 0; dy < h; ++dy) {
@@ -173,7 +178,8 @@ __global__ void chunk_count_kernel(const uint32_t* row_off, uint64_t nblocks, in
 }
 
 // Matrix schedule: the padding visits of each block repeat its first sample with zero
-// weights (so they add exact zeros and read a row the block reads anyway).
+// weights (so they add exact zeros and read a row the block reads anyway); packed
+// visits carry window bin (15, 15), which no cell of a block reaches.
 __global__ void pad_visits_kernel(const uint32_t* row_off, const uint32_t* chunk_off, uint64_t nblocks, int rows,
                                   int cells, uint32_t* visit_k, float* visit_w) {
   const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -181,6 +187,10 @@ __global__ void pad_visits_kernel(const uint32_t* row_off, const uint32_t* chunk
   const uint32_t base = 4 * chunk_off[b], end = 4 * chunk_off[b + 1];
   const uint32_t n = row_off[(b + 1) * rows] - row_off[b * rows];
   for (uint32_t p = base + n; p < end; ++p) {
+    if (visit_w == nullptr) {
+      visit_k[p] = (visit_k[base] & 0x00FFFFFFu) | 0xFF000000u;
+      continue;
+    }
     visit_k[p] = visit_k[base];
     for (int c = 0; c < cells; ++c) visit_w[static_cast<uint64_t>(p) * cells + c] = 0.f;
   }
@@ -551,6 +561,9 @@ Plan2* plan2_create(const Plan2Desc& desc) {
     // The matrix gather takes visits four at a time: items start on multiples of 4.
     p->item_visits = p->d.scheme == kSchemeMatrix ? (p->d.item_visits + 3) / 4 * 4 : p->d.item_visits;
     p->scheme = p->d.scheme;
+    // Packed visits need the sorted position in 24 bits and window bins below 15
+    // (W + 8 - 1 <= 15 for every legal width and block).
+    p->packed_visits = matrix && tune_get(kTuneMatrixWeights) == 1 && S < (1ull << 24);
     p->item_samples = p->d.item_samples;
     p->run_cost = p->d.run_cost;
     p->tile_width = p->d.tile_width;
@@ -702,11 +715,13 @@ void plan2_set_trajectory(Plan2* p, const float* d_traj, const float* d_dcf, hip
     p->visit_w = nullptr;
     p->visit_capacity = visits;
     KSJ_HIP(hipMalloc(&p->visit_k, std::max<size_t>(1, visits) * sizeof(uint32_t)));
-    KSJ_HIP(hipMalloc(&p->visit_w, std::max<size_t>(1, visits) * weight_vec4 * sizeof(float4)));
+    if (!p->packed_visits)
+      KSJ_HIP(hipMalloc(&p->visit_w, std::max<size_t>(1, visits) * weight_vec4 * sizeof(float4)));
   }
   p->visit_count = visits;
   sa.visit_k = p->visit_k;
   sa.visit_w = reinterpret_cast<float*>(p->visit_w);
+  sa.packed = p->packed_visits ? 1 : 0;
   sa.chunk_off = matrix ? p->block_chunk_off : nullptr;
   hipLaunchKernelGGL(visit_fill_kernel, dim3(ceil_div(nrows, 256)), dim3(256), 0, stream, sa);
   KSJ_LAUNCH_CHECK();

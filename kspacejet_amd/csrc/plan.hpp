@@ -91,8 +91,11 @@ struct Plan2 {
   uint64_t nblocks = 0;
   uint32_t item_visits = 64;
   uint32_t* visit_row_off = nullptr;  // [nblocks * (W+h-1) + 1] first visit of each (block, window row)
-  uint32_t* visit_k = nullptr;        // [visit_capacity] sorted sample position
-  float4* visit_w = nullptr;          // [visit_capacity * h*w/4] cell weights, row-major per visit
+  uint32_t* visit_k = nullptr;        // [visit_capacity] sorted sample position (packed_visits: position in
+                                      // bits 0-23, window row in bits 24-27, window column in bits 28-31)
+  float4* visit_w = nullptr;          // [visit_capacity * h*w/4] cell weights, row-major per visit (null when
+                                      // packed_visits: the gather forms them from the taps)
+  bool packed_visits = false;         // matrix schedule with weights formed from the taps
   uint64_t visit_capacity = 0;
   uint64_t visit_count = 0;           // host copy
   uint32_t* block_first_item = nullptr;  // [nblocks + 1]

@@ -168,7 +168,7 @@ __device__ inline void matrix3_run(const Gather3Args& a, uint32_t run) {
   const uint32_t c0 = field(rec.y, 0) / 4, c1 = field(rec.z, nitems - 1) / 4;
   const uint32_t last_v = c1 > c0 ? 4 * c1 - 1 : 0;
   const int W = a.width;
-  const float* rows = reinterpret_cast<const float*>(a.sorted) + j * NF;
+  const float* rows = reinterpret_cast<const float*>(a.sorted) + piece_offset<NF>(j);
   MfmaAcc acc[T][NF];
 #pragma unroll
   for (int h = 0; h < T; ++h)
@@ -190,11 +190,11 @@ __device__ inline void matrix3_run(const Gather3Args& a, uint32_t run) {
           const uint32_t z = z0 + h, y = y0 + q, x = x0 + r;
           if (z < a.g[0] && y < a.g[1] && x < a.g[2])
             store_piece_nt<NF>(reinterpret_cast<float*>(a.grid) +
-                                 ((static_cast<uint64_t>(z) * a.g[1] + y) * a.g[2] + x) * (2 * CP) + j * NF,
+                                 ((static_cast<uint64_t>(z) * a.g[1] + y) * a.g[2] + x) * (2 * CP) + piece_offset<NF>(j),
                                o);
         } else {
           store_piece<NF>(reinterpret_cast<float*>(a.partial) +
-                            (static_cast<uint64_t>(slot) * 64 + 16 * h + 4 * q + r) * (2 * CP) + j * NF,
+                            (static_cast<uint64_t>(slot) * 64 + 16 * h + 4 * q + r) * (2 * CP) + piece_offset<NF>(j),
                           o);
         }
       }

@@ -275,7 +275,9 @@ Plan2* Server::plan_for(const Pending& p, hipStream_t stream) {
   PlanPtr plan(plan2_create(d), &plan2_destroy);
   plan2_set_trajectory(plan.get(), r.d_traj, nullptr, stream);
   PlanEntry e;
-  e.bytes = plan->device_bytes;
+  const uint64_t cells = plan->packed_visits ? 0 : static_cast<uint64_t>(plan->block_h) * plan->block_w;
+  e.bytes = plan->device_bytes + plan->visit_capacity * (sizeof(uint32_t) + cells * sizeof(float)) +
+            static_cast<uint64_t>(plan->item_capacity) * (sizeof(uint4) + sizeof(uint32_t));
   e.plan = std::move(plan);
   Plan2* raw = e.plan.get();
   plans_.emplace(p.plan_key, std::move(e));

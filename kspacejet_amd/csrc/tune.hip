@@ -30,7 +30,7 @@ Param g_params[kTuneKeys] = {
     {"block_group", 2, [](int64_t v) { return v == 1 || v == 2 || v == 4; }},
     {"fft_schedule", 0, [](int64_t v) { return v >= 0 && v <= 3; }},
     {"multi_fused", 0, [](int64_t v) { return v == 0 || v == 1; }},
-    {"matrix_run_cost", 8, [](int64_t v) { return v >= 1 && v <= 64; }},
+    {"matrix_run_cost", 12, [](int64_t v) { return v >= 1 && v <= 64; }},
     {"matrix_pipeline", 1, [](int64_t v) { return v == 1 || v == 2; }},
     {"matrix_nt_store", 1, [](int64_t v) { return v == 0 || v == 1; }},
     {"matrix_slab", 16, [](int64_t v) { return v >= 1 && v <= 4096; }},
@@ -41,6 +41,9 @@ Param g_params[kTuneKeys] = {
     {"matrix3_item_visits", 256, [](int64_t v) { return v >= 1 && v <= 4096; }},
     {"matrix3_run_cost", 32, [](int64_t v) { return v >= 1 && v <= 64; }},
     {"mix_scheme", 1, [](int64_t v) { return v == 0 || v == 1; }},
+    {"matrix_weights", 1, [](int64_t v) { return v == 0 || v == 1; }},
+    {"matrix_persistent", 0, [](int64_t v) { return v >= 0 && v <= 8; }},
+    {"matrix_tap_share", 1, [](int64_t v) { return v == 0 || v == 1; }},
 };
 
 }  // namespace
@@ -51,6 +54,22 @@ void tune_set(TuneKey key, int64_t value) {
   require(key >= 0 && key < kTuneKeys, "unknown tuning key");
   require(g_params[key].legal(value), "illegal tuning value");
   g_params[key].value = value;
+}
+
+uint32_t device_cu_count() {
+  constexpr int kMaxDevices = 64;
+  static std::atomic<uint32_t> cached[kMaxDevices] = {};
+  int device = 0;
+  KSJ_HIP(hipGetDevice(&device));
+  require(device >= 0 && device < kMaxDevices, "device ordinal beyond the CU-count table");
+  uint32_t n = cached[device].load(std::memory_order_relaxed);
+  if (n == 0) {
+    int cus = 0;
+    KSJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    n = static_cast<uint32_t>(cus > 0 ? cus : 1);
+    cached[device].store(n, std::memory_order_relaxed);
+  }
+  return n;
 }
 
 int tune_key(const char* name) {

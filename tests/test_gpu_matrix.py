@@ -151,3 +151,34 @@ def test_linear_gridding_on_the_matrix_schedule(ops, tune):
     a = run(ops, tune, 2, traj, k, 30, width=2, **kw)
     b = run(ops, tune, 0, traj, k, 30, width=2, **kw)
     assert rel_l2(a[0], b[0]) <= 2e-6
+
+
+@pytest.mark.parametrize("n,coils,spokes,width,item_visits", [
+    (64, 3, 45, 4, 0), (48, 12, 37, 2, 0), (41, 20, 29, 6, 8), (37, 40, 25, 8, 0), (30, 64, 21, 4, 12)])
+def test_weights_from_taps_give_the_table_bits(ops, tune, n, coils, spokes, width, item_visits):
+    """Packed visits (sample and window bin, weights formed in the gather from the
+    sample's taps, loaded per lane or shared across the visit's lanes) and visits
+    carrying the stored cell weights sum the same float products in the same order:
+    bit-identical images and RSS at every width, padded coil count and block width,
+    with items cut short."""
+    traj, k = radial_scan(ops, n, coils, spokes, synth.scan_seed(12, n + coils))
+    results = []
+    for weights, share in ((1, 0), (1, 1), (0, 0)):
+        tune("matrix_weights", weights)
+        tune("matrix_tap_share", share)
+        results.append(run(ops, tune, 2, traj, k, n, width, item_visits=item_visits))
+    for other in results[1:]:
+        assert np.array_equal(results[0][0], other[0]) and np.array_equal(results[0][1], other[1])
+
+
+def test_packed_visit_plans_are_smaller(ops, tune):
+    traj, _ = radial_scan(ops, 64, 2, 45, 1)
+    sizes = []
+    for weights in (1, 0):
+        tune("matrix_weights", weights)
+        tune("gather_scheme", 2)
+        plan = ops.Plan2(64, 64, traj.shape[0]).set_trajectory(traj)
+        info = plan.info()
+        sizes.append(info["device_bytes"])
+        plan.close()
+    assert sizes[0] < sizes[1]
