@@ -267,6 +267,59 @@ int ksj_gpu_phase_correct(const void* d_kspace, const void* d_phase, void* d_out
 int ksj_gpu_readout_crop(const void* d_kspace, void* d_output, uint64_t lines, uint32_t in_cols, uint32_t out_cols,
                          uint32_t offset, void* stream);
 
+/* ------------------------------------------------------------------ calibration */
+
+/* Estimates of providers/kspacejet-calibration/src/operators/ from channel-major complex
+ * float samples x[C][S] (C <= 64).  The covariance is summed on the GPU (FP32 products per
+ * chunk of 64 samples, FP64 sums in a fixed order) and decomposed in FP64 by a cyclic
+ * Jacobi solver on one workgroup; outputs are rounded to float once.  The estimate calls
+ * synchronise `stream` and fail with KSJ_GPU_INVALID_ARGUMENT when a sample is not finite
+ * or the decomposition does not converge (the reference reports the same cases as "could
+ * not estimate").  Workspace: ksj_gpu_calibration_workspace_bytes(C, S). */
+size_t ksj_gpu_calibration_workspace_bytes(uint32_t channels, uint64_t samples);
+
+/* Hermitian covariance [C][C] (complex128, row-major).  centred = 1: the mean-centred
+ * population covariance sum_s conj(x_r - m_r)(x_c - m_c) / S of
+ * ksj::stats::covariance_channel_major_with_workspace
+ * (libs/numerics/kspacejet-stats/include/kspacejet/stats/moments.hpp:99-140, as called with
+ * VarianceNormalization::population by noise_model_estimate.cpp:182); centred = 0: the
+ * uncentred sum_s x_r conj(x_c) / S of coil_compression_basis_estimate.cpp:207-227.
+ * Asynchronous. */
+int ksj_gpu_channel_covariance(const void* d_samples, uint32_t channels, uint64_t samples, int centred,
+                               void* d_covariance, void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* Eigen-decomposition of a Hermitian n x n complex128 matrix (n <= 64): eigenvalues
+ * ascending (float64) and eigenvector k in column k of d_vectors (complex128 [n][n]).
+ * *d_status: 0 = converged, 1 = non-finite input, 2 = no convergence.  Replaces
+ * ksj::linalg::self_adjoint_eigen_decomposition_with_workspace
+ * (libs/numerics/kspacejet-linalg/include/kspacejet/linalg/decompositions.hpp:503-626): the
+ * same rotation per pivot, applied in the cyclic round-robin order rather than largest
+ * pivot first, so each eigenvector agrees with the reference's up to a unit phase.
+ * Asynchronous. */
+int ksj_gpu_hermitian_eigen(const void* d_matrix, uint32_t n, double* d_values, void* d_vectors, uint32_t* d_status,
+                            void* stream);
+
+/* Noise model: W = V diag(1 / sqrt(max(lambda, floor))) V^H of the centred covariance,
+ * complex float [C][C] row-major, applied as y = W x.  Replaces noise_model_estimate.cpp:158-203
+ * (floor 1e-6 there) over ksj::linalg::whitening_matrix_from_self_adjoint_eigen_with_workspace
+ * (libs/numerics/kspacejet-linalg/include/kspacejet/linalg/whitening.hpp:24-50).  S >= 2. */
+int ksj_gpu_noise_model_estimate(const void* d_samples, uint32_t channels, uint64_t samples, float eigenvalue_floor,
+                                 void* d_whitening, void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* Coil-compression basis B[v][p] = conj(V[p][P-1-v]) of the uncentred covariance (the
+ * conjugated eigenvectors of the largest eigenvalues), complex float [V][P] row-major,
+ * applied as y = B x.  Replaces coil_compression_basis_estimate.cpp:177-244. */
+int ksj_gpu_coil_compression_basis_estimate(const void* d_samples, uint32_t physical_channels,
+                                            uint32_t virtual_channels, uint64_t samples, void* d_basis,
+                                            void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* Readout phase model p[c][k] = conj(s) / |s|, s = sum over reference lines of
+ * ref[c][line][k] in float in line order (1 + 0i where s = 0), |s| = the FP64 square root
+ * of re^2 + im^2 rounded to float.  Replaces phase_correction_estimate.cpp:141-163.
+ * Asynchronous. */
+int ksj_gpu_phase_correction_estimate(const void* d_reference, uint32_t channels, uint32_t lines, uint32_t readout,
+                                      void* d_phase, void* stream);
+
 /* ------------------------------------------------------------------ 3-D (kooshball) */
 
 /* 3-D KB gridding adjoint.  No reference counterpart: the reference reconstructs 2-D

@@ -1,6 +1,7 @@
 // extern "C" boundary of libksj_gpu.so: argument checks, exception -> status mapping.
 #include "../../include/ksj_gpu.h"
 
+#include "calib.hpp"
 #include "combine.hpp"
 #include "condition.hpp"
 #include "fft.hpp"
@@ -299,6 +300,55 @@ int ksj_gpu_adjoint2_execute_static(const ksj_gpu_plan2* plan, const void* d_ksp
     scan.rss = d_rss;
     ksj::gpu::adjoint2_execute(*plan->impl, &scan, 1, channels, layout, ws, KSJ_GPU_STATIC_SCRATCH_BYTES,
                                as_stream(stream));
+  });
+}
+
+size_t ksj_gpu_calibration_workspace_bytes(uint32_t channels, uint64_t samples) {
+  return ksj::gpu::calibration_workspace_bytes(channels, samples);
+}
+
+int ksj_gpu_channel_covariance(const void* d_samples, uint32_t channels, uint64_t samples, int centred,
+                               void* d_covariance, void* d_workspace, size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(d_covariance != nullptr, "null argument");
+    ksj::gpu::require(centred == 0 || centred == 1, "centred must be 0 or 1");
+    ksj::gpu::channel_covariance(static_cast<const float2*>(d_samples), channels, samples,
+                                 centred ? ksj::gpu::kCovarianceNoise : ksj::gpu::kCovarianceSecondMoment,
+                                 static_cast<double2*>(d_covariance), d_workspace, workspace_bytes, as_stream(stream));
+  });
+}
+
+int ksj_gpu_hermitian_eigen(const void* d_matrix, uint32_t n, double* d_values, void* d_vectors, uint32_t* d_status,
+                            void* stream) {
+  return guarded([&] {
+    ksj::gpu::hermitian_eigen(static_cast<const double2*>(d_matrix), n, d_values, static_cast<double2*>(d_vectors),
+                              d_status, as_stream(stream));
+  });
+}
+
+int ksj_gpu_noise_model_estimate(const void* d_samples, uint32_t channels, uint64_t samples, float eigenvalue_floor,
+                                 void* d_whitening, void* d_workspace, size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    ksj::gpu::noise_model_estimate(static_cast<const float2*>(d_samples), channels, samples, eigenvalue_floor,
+                                   static_cast<float2*>(d_whitening), d_workspace, workspace_bytes, as_stream(stream));
+  });
+}
+
+int ksj_gpu_coil_compression_basis_estimate(const void* d_samples, uint32_t physical_channels,
+                                            uint32_t virtual_channels, uint64_t samples, void* d_basis,
+                                            void* d_workspace, size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    ksj::gpu::coil_compression_basis_estimate(static_cast<const float2*>(d_samples), physical_channels,
+                                              virtual_channels, samples, static_cast<float2*>(d_basis), d_workspace,
+                                              workspace_bytes, as_stream(stream));
+  });
+}
+
+int ksj_gpu_phase_correction_estimate(const void* d_reference, uint32_t channels, uint32_t lines, uint32_t readout,
+                                      void* d_phase, void* stream) {
+  return guarded([&] {
+    ksj::gpu::phase_correction_estimate(static_cast<const float2*>(d_reference), channels, lines, readout,
+                                        static_cast<float2*>(d_phase), as_stream(stream));
   });
 }
 

@@ -24,7 +24,7 @@ const TypeEntry kTypes[] = {
 #include "types.inc"
 };
 constexpr int kTypeCount = static_cast<int>(sizeof(kTypes) / sizeof(kTypes[0]));
-static_assert(kTypeCount == static_cast<int>(TypeId::kCoilCompressionBasis) + 1, "one entry per TypeId");
+static_assert(kTypeCount == static_cast<int>(TypeId::kPhaseReferenceFrame) + 1, "one entry per TypeId");
 
 // Dimension-name views of each type, built once (static storage: descriptors
 // handed out by type_descriptor() borrow them for the life of the module).

@@ -54,6 +54,8 @@ enum class TypeId : int {
   kNoiseModel,
   kPhaseModel,
   kCoilCompressionBasis,
+  kNoiseCalibrationFrame,
+  kPhaseReferenceFrame,
 };
 // The full registry descriptor of `id` (what a host places in a payload view).
 ksj_type_descriptor_view type_descriptor(TypeId id);

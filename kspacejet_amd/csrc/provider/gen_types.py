@@ -18,7 +18,7 @@ import sys
 
 TYPES = ["ksj.kspace-frame", "ksj.noncartesian-kspace-frame", "ksj.trajectory-frame", "ksj.coil-image-frame",
          "ksj.image-frame", "ksj.complex-image-frame", "ksj.sensitivity-map", "ksj.noise-model", "ksj.phase-model",
-         "ksj.coil-compression-basis"]
+         "ksj.coil-compression-basis", "ksj.noise-calibration-frame", "ksj.phase-reference-frame"]
 ELEMENT = {"float32": "KSJ_ELEMENT_TYPE_FLOAT32", "complex_float32": "KSJ_ELEMENT_TYPE_COMPLEX_FLOAT32"}
 KIND = {"buffer_handle": "KSJ_PAYLOAD_KIND_BUFFER_HANDLE"}
 LAYOUT = {"channel_major_contiguous": "KSJ_TYPE_LAYOUT_CHANNEL_MAJOR_CONTIGUOUS",

@@ -19,6 +19,10 @@ extern const OpSpec kCoilCompress;               // coil_compress
 extern const OpSpec kReadoutOversamplingRemove;  // readout_oversampling_remove
 extern const OpSpec kNoncartesianNoisePrewhiten; // noncartesian_noise_prewhiten
 extern const OpSpec kNoncartesianCoilCompress;   // noncartesian_coil_compress
+// org.kspacejet.calibration
+extern const OpSpec kNoiseModelEstimate;            // noise_model_estimate
+extern const OpSpec kPhaseCorrectionEstimate;       // phase_correction_estimate
+extern const OpSpec kCoilCompressionBasisEstimate;  // coil_compression_basis_estimate
 // org.kspacejet.gpu-recon: the conditioning operators with device-memory edges
 extern const OpSpec kGpuNoisePrewhiten;
 extern const OpSpec kGpuPhaseCorrect;

@@ -601,3 +601,81 @@ class Graph:
 
     def __del__(self):
         self.close()
+
+
+# ---------------------------------------------------------------- calibration
+
+def _calibration_workspace(channels: int, samples: int, device) -> torch.Tensor:
+    return _workspace(load().ksj_gpu_calibration_workspace_bytes(channels, samples), device)
+
+
+def channel_covariance(samples: torch.Tensor, centred: bool) -> torch.Tensor:
+    """[C, S] complex64 -> [C, C] complex128 covariance (ksj_gpu_channel_covariance)."""
+    if samples.dim() != 2:
+        raise ValueError("samples must be [C, S]")
+    _need(samples, torch.complex64, "samples")
+    C, S = samples.shape
+    out = torch.empty((C, C), dtype=torch.complex128, device=samples.device)
+    ws = _calibration_workspace(C, S, samples.device)
+    call("ksj_gpu_channel_covariance", _ptr(samples), C, S, int(bool(centred)), _ptr(out), _ptr(ws), ws.numel(),
+         _stream())
+    return out
+
+
+def hermitian_eigen(matrix: torch.Tensor):
+    """Hermitian [n, n] complex128 -> (eigenvalues ascending float64 [n], eigenvectors in columns
+    complex128 [n, n]); raises when the decomposition fails (ksj_gpu_hermitian_eigen)."""
+    if matrix.dim() != 2 or matrix.shape[0] != matrix.shape[1]:
+        raise ValueError("matrix must be square")
+    _need(matrix, torch.complex128, "matrix")
+    n = matrix.shape[0]
+    values = torch.empty(n, dtype=torch.float64, device=matrix.device)
+    vectors = torch.empty((n, n), dtype=torch.complex128, device=matrix.device)
+    status = torch.empty(1, dtype=torch.int32, device=matrix.device)
+    call("ksj_gpu_hermitian_eigen", _ptr(matrix), n, _ptr(values), _ptr(vectors), _ptr(status), _stream())
+    code = int(status.item())
+    if code != 0:
+        raise RuntimeError("eigen-decomposition failed: " + ("non-finite input" if code == 1 else "no convergence"))
+    return values, vectors
+
+
+def noise_model_estimate(samples: torch.Tensor, eigenvalue_floor: float = 1.0e-6) -> torch.Tensor:
+    """Noise-only samples [C, S] complex64 -> whitening matrix [C, C] complex64 (ksj_gpu_noise_model_estimate)."""
+    if samples.dim() != 2:
+        raise ValueError("samples must be [C, S]")
+    _need(samples, torch.complex64, "samples")
+    C, S = samples.shape
+    out = torch.empty((C, C), dtype=torch.complex64, device=samples.device)
+    ws = _calibration_workspace(C, S, samples.device)
+    call("ksj_gpu_noise_model_estimate", _ptr(samples), C, S, float(eigenvalue_floor), _ptr(out), _ptr(ws),
+         ws.numel(), _stream())
+    return out
+
+
+def coil_compression_basis_estimate(kspace: torch.Tensor, virtual_channels: int) -> torch.Tensor:
+    """Channel-major k-space [P, ...] complex64 -> basis [V, P] complex64
+    (ksj_gpu_coil_compression_basis_estimate)."""
+    if kspace.dim() < 2:
+        raise ValueError("k-space must be [P, ...]")
+    _need(kspace, torch.complex64, "k-space")
+    P = kspace.shape[0]
+    S = kspace[0].numel()
+    if not 1 <= virtual_channels <= P:
+        raise ValueError("virtual channels must be in [1, physical]")
+    out = torch.empty((virtual_channels, P), dtype=torch.complex64, device=kspace.device)
+    ws = _calibration_workspace(P, S, kspace.device)
+    call("ksj_gpu_coil_compression_basis_estimate", _ptr(kspace), P, virtual_channels, S, _ptr(out), _ptr(ws),
+         ws.numel(), _stream())
+    return out
+
+
+def phase_correction_estimate(reference: torch.Tensor) -> torch.Tensor:
+    """Phase-reference frame [C, lines, K] complex64 -> phase model [C, K] complex64
+    (ksj_gpu_phase_correction_estimate)."""
+    if reference.dim() != 3:
+        raise ValueError("reference must be [C, lines, K]")
+    _need(reference, torch.complex64, "reference")
+    C, L, K = reference.shape
+    out = torch.empty((C, K), dtype=torch.complex64, device=reference.device)
+    call("ksj_gpu_phase_correction_estimate", _ptr(reference), C, L, K, _ptr(out), _stream())
+    return out

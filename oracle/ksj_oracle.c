@@ -784,3 +784,205 @@ void ora_readout_crop(size_t lines, size_t in_cols, size_t out_cols, size_t offs
       y[2 * (l * out_cols + k) + 1] = x[2 * (l * in_cols + offset + k) + 1];
     }
 }
+
+/* ---------------------------------------------------------------- calibration */
+
+void ora_channel_covariance(size_t channels, size_t samples, const float* x, int centred, double* cov) {
+  double* mr = (double*)calloc(channels, sizeof(double));
+  double* mi = (double*)calloc(channels, sizeof(double));
+  if (centred) {
+    for (size_t c = 0; c < channels; ++c) {
+      for (size_t s = 0; s < samples; ++s) {
+        mr[c] += x[2 * (c * samples + s)];
+        mi[c] += x[2 * (c * samples + s) + 1];
+      }
+      mr[c] /= (double)samples;
+      mi[c] /= (double)samples;
+    }
+  }
+  for (size_t r = 0; r < channels; ++r)
+    for (size_t c = 0; c < channels; ++c) {
+      double sr = 0.0, si = 0.0;
+      for (size_t s = 0; s < samples; ++s) {
+        const double ar = x[2 * (r * samples + s)] - mr[r], ai = x[2 * (r * samples + s) + 1] - mi[r];
+        const double br = x[2 * (c * samples + s)] - mr[c], bi = x[2 * (c * samples + s) + 1] - mi[c];
+        if (centred) { /* conj(a) b */
+          sr += ar * br + ai * bi;
+          si += ar * bi - ai * br;
+        } else { /* a conj(b) */
+          sr += ar * br + ai * bi;
+          si += ai * br - ar * bi;
+        }
+      }
+      cov[2 * (r * channels + c)] = sr / (double)samples;
+      cov[2 * (r * channels + c) + 1] = si / (double)samples;
+    }
+  free(mr);
+  free(mi);
+}
+
+int ora_hermitian_eigen(size_t n, const double* matrix, double* values, double* vectors) {
+  /* decompositions.hpp:533-626: copy, identity vectors, tolerance from the diagonal scale,
+   * largest off-diagonal pivot, rotation (tau, tangent, cosine, sine, pivot phase), sort. */
+  double* a = (double*)malloc(2 * n * n * sizeof(double));
+  double* v = vectors;
+  double scale = 0.0;
+  for (size_t r = 0; r < n; ++r) {
+    for (size_t c = 0; c < n; ++c) {
+      a[2 * (r * n + c)] = matrix[2 * (r * n + c)];
+      a[2 * (r * n + c) + 1] = matrix[2 * (r * n + c) + 1];
+      v[2 * (r * n + c)] = r == c ? 1.0 : 0.0;
+      v[2 * (r * n + c) + 1] = 0.0;
+    }
+    if (!isfinite(a[2 * (r * n + r)])) {
+      free(a);
+      return -1;
+    }
+    scale += fabs(a[2 * (r * n + r)]);
+  }
+  const double tol = 2.220446049250313e-16 * (scale > 1.0 ? scale : 1.0) * (double)n * 8.0;
+  const size_t max_rot = n * n * 64;
+  for (size_t rot = 0; rot < max_rot; ++rot) {
+    size_t p = 0, q = 0;
+    double mag = 0.0;
+    for (size_t r = 0; r < n; ++r)
+      for (size_t c = r + 1; c < n; ++c) {
+        const double m = hypot(a[2 * (r * n + c)], a[2 * (r * n + c) + 1]);
+        if (!isfinite(m)) {
+          free(a);
+          return -1;
+        }
+        if (m > mag) {
+          mag = m;
+          p = r;
+          q = c;
+        }
+      }
+    if (mag <= tol) {
+      for (size_t i = 0; i < n; ++i) values[i] = a[2 * (i * n + i)];
+      for (size_t first = 0; first < n; ++first) {
+        size_t mn = first;
+        for (size_t cand = first + 1; cand < n; ++cand)
+          if (values[cand] < values[mn]) mn = cand;
+        if (mn == first) continue;
+        const double t = values[first];
+        values[first] = values[mn];
+        values[mn] = t;
+        for (size_t r = 0; r < n; ++r)
+          for (int h = 0; h < 2; ++h) {
+            const double u = v[2 * (r * n + first) + h];
+            v[2 * (r * n + first) + h] = v[2 * (r * n + mn) + h];
+            v[2 * (r * n + mn) + h] = u;
+          }
+      }
+      free(a);
+      return 0;
+    }
+    const double app = a[2 * (p * n + p)], aqq = a[2 * (q * n + q)];
+    const double tau = (aqq - app) / (2.0 * mag);
+    const double tg = tau >= 0.0 ? 1.0 / (tau + sqrt(1.0 + tau * tau)) : -1.0 / (-tau + sqrt(1.0 + tau * tau));
+    const double cs = 1.0 / sqrt(1.0 + tg * tg), sn = tg * cs;
+    const double phr = a[2 * (p * n + q)] / mag, phi = a[2 * (p * n + q) + 1] / mag; /* phase; conj = (phr, -phi) */
+    for (size_t i = 0; i < n; ++i) {
+      if (i == p || i == q) continue;
+      const double rr = a[2 * (i * n + p)], ri = a[2 * (i * n + p) + 1];
+      const double qr0 = a[2 * (i * n + q)], qi0 = a[2 * (i * n + q) + 1];
+      const double cr = qr0 * phr + qi0 * phi, ci = qi0 * phr - qr0 * phi; /* times conj(phase) */
+      const double ur = cs * rr - sn * cr, ui = cs * ri - sn * ci;
+      const double wr = sn * rr + cs * cr, wi = sn * ri + cs * ci;
+      a[2 * (i * n + p)] = ur;
+      a[2 * (i * n + p) + 1] = ui;
+      a[2 * (p * n + i)] = ur;
+      a[2 * (p * n + i) + 1] = -ui;
+      a[2 * (i * n + q)] = wr;
+      a[2 * (i * n + q) + 1] = wi;
+      a[2 * (q * n + i)] = wr;
+      a[2 * (q * n + i) + 1] = -wi;
+    }
+    a[2 * (p * n + p)] = cs * cs * app - 2.0 * cs * sn * mag + sn * sn * aqq;
+    a[2 * (p * n + p) + 1] = 0.0;
+    a[2 * (q * n + q)] = sn * sn * app + 2.0 * cs * sn * mag + cs * cs * aqq;
+    a[2 * (q * n + q) + 1] = 0.0;
+    a[2 * (p * n + q)] = a[2 * (p * n + q) + 1] = 0.0;
+    a[2 * (q * n + p)] = a[2 * (q * n + p) + 1] = 0.0;
+    for (size_t r = 0; r < n; ++r) {
+      const double xr = v[2 * (r * n + p)], xi = v[2 * (r * n + p) + 1];
+      const double yr0 = v[2 * (r * n + q)], yi0 = v[2 * (r * n + q) + 1];
+      const double yr = yr0 * phr + yi0 * phi, yi = yi0 * phr - yr0 * phi;
+      v[2 * (r * n + p)] = cs * xr - sn * yr;
+      v[2 * (r * n + p) + 1] = cs * xi - sn * yi;
+      v[2 * (r * n + q)] = sn * xr + cs * yr;
+      v[2 * (r * n + q) + 1] = sn * xi + cs * yi;
+    }
+  }
+  free(a);
+  return -1;
+}
+
+int ora_noise_model_estimate(size_t channels, size_t samples, const float* x, double floor_value, double* whitening) {
+  double* cov = (double*)malloc(2 * channels * channels * sizeof(double));
+  double* vec = (double*)malloc(2 * channels * channels * sizeof(double));
+  double* val = (double*)malloc(channels * sizeof(double));
+  ora_channel_covariance(channels, samples, x, 1, cov);
+  const int rc = ora_hermitian_eigen(channels, cov, val, vec);
+  if (rc == 0) {
+    for (size_t r = 0; r < channels; ++r)
+      for (size_t c = 0; c < channels; ++c) {
+        double sr = 0.0, si = 0.0;
+        for (size_t k = 0; k < channels; ++k) {
+          const double inv = 1.0 / sqrt(val[k] > floor_value ? val[k] : floor_value);
+          const double ar = vec[2 * (r * channels + k)], ai = vec[2 * (r * channels + k) + 1];
+          const double br = vec[2 * (c * channels + k)], bi = vec[2 * (c * channels + k) + 1];
+          sr += (ar * br + ai * bi) * inv;
+          si += (ai * br - ar * bi) * inv;
+        }
+        whitening[2 * (r * channels + c)] = sr;
+        whitening[2 * (r * channels + c) + 1] = si;
+      }
+  }
+  free(cov);
+  free(vec);
+  free(val);
+  return rc;
+}
+
+int ora_coil_compression_basis_estimate(size_t physical, size_t virtual_channels, size_t samples, const float* x,
+                                        double* basis) {
+  double* cov = (double*)malloc(2 * physical * physical * sizeof(double));
+  double* vec = (double*)malloc(2 * physical * physical * sizeof(double));
+  double* val = (double*)malloc(physical * sizeof(double));
+  ora_channel_covariance(physical, samples, x, 0, cov);
+  const int rc = ora_hermitian_eigen(physical, cov, val, vec);
+  if (rc == 0) {
+    for (size_t vc = 0; vc < virtual_channels; ++vc)
+      for (size_t p = 0; p < physical; ++p) {
+        const size_t col = physical - 1 - vc;
+        basis[2 * (vc * physical + p)] = vec[2 * (p * physical + col)];
+        basis[2 * (vc * physical + p) + 1] = -vec[2 * (p * physical + col) + 1];
+      }
+  }
+  free(cov);
+  free(vec);
+  free(val);
+  return rc;
+}
+
+void ora_phase_correction_estimate(size_t channels, size_t lines, size_t readout, const float* ref, float* phase) {
+  for (size_t c = 0; c < channels; ++c)
+    for (size_t k = 0; k < readout; ++k) {
+      float sr = 0.f, si = 0.f;
+      for (size_t l = 0; l < lines; ++l) {
+        sr += ref[2 * ((c * lines + l) * readout + k)];
+        si += ref[2 * ((c * lines + l) * readout + k) + 1];
+      }
+      const float mag = (float)sqrt((double)sr * (double)sr + (double)si * (double)si);
+      float* out = phase + 2 * (c * readout + k);
+      if (mag == 0.f) {
+        out[0] = 1.f;
+        out[1] = 0.f;
+      } else {
+        out[0] = sr / mag;
+        out[1] = -si / mag;
+      }
+    }
+}

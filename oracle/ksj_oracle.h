@@ -148,6 +148,30 @@ void ora_phase_correct(size_t channels, size_t rows, size_t cols, const float* x
  * (readout_oversampling_remove.cpp:165-181). */
 void ora_readout_crop(size_t lines, size_t in_cols, size_t out_cols, size_t offset, const float* x, float* y);
 
+/* ---- calibration estimators (providers/kspacejet-calibration/src/operators/). */
+
+/* Hermitian covariance [C][C] (interleaved complex128) of float samples x[C][S] in FP64.
+ * centred = 1: sum_s conj(x_r - m_r)(x_c - m_c) / S with per-channel means m
+ * (kspacejet-stats moments.hpp:99-140, population normalisation as noise_model_estimate.cpp:182);
+ * centred = 0: sum_s x_r conj(x_c) / S (coil_compression_basis_estimate.cpp:207-227). */
+void ora_channel_covariance(size_t channels, size_t samples, const float* x, int centred, double* cov);
+/* The reference's largest-pivot Jacobi eigensolver (kspacejet-linalg decompositions.hpp:533-626)
+ * restated in FP64: eigenvalues ascending, eigenvector k in column k of vectors [n][n]
+ * (interleaved).  Returns 0, or -1 on a non-finite entry or no convergence. */
+int ora_hermitian_eigen(size_t n, const double* matrix, double* values, double* vectors);
+/* Whitening W = V diag(1 / sqrt(max(lambda, floor))) V^H of the centred covariance, [C][C]
+ * (whitening.hpp:24-50 over noise_model_estimate.cpp:182-190).  Returns 0 or -1. */
+int ora_noise_model_estimate(size_t channels, size_t samples, const float* x, double floor_value, double* whitening);
+/* B[v][p] = conj(V[p][P-1-v]) of the uncentred covariance, [V][P]
+ * (coil_compression_basis_estimate.cpp:207-238).  Returns 0 or -1. */
+int ora_coil_compression_basis_estimate(size_t physical, size_t virtual_channels, size_t samples, const float* x,
+                                        double* basis);
+/* p[c][k] = conj(s) / |s|, s = float sum over lines of ref[c][line][k] in line order,
+ * |s| = (float) sqrt((double) re^2 + (double) im^2) (the computation of glibc's hypotf
+ * behind std::abs of std::complex<float>); 1 + 0i where |s| = 0
+ * (phase_correction_estimate.cpp:141-163). */
+void ora_phase_correction_estimate(size_t channels, size_t lines, size_t readout, const float* ref, float* phase);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,11 @@ def lib():
                                              C.c_double, C.c_double, _sz, _sz, _dp, _dp, _dp, _sz, _up, _sz, _up,
                                              _sz, _up, _dp]
         L.ora_golden_means3.argtypes = [_sz, _sz, _dp]
+        L.ora_channel_covariance.argtypes = [_sz, _sz, _fp, C.c_int, _dp]
+        L.ora_hermitian_eigen.argtypes = [_sz, _dp, _dp, _dp]
+        L.ora_noise_model_estimate.argtypes = [_sz, _sz, _fp, C.c_double, _dp]
+        L.ora_coil_compression_basis_estimate.argtypes = [_sz, _sz, _sz, _fp, _dp]
+        L.ora_phase_correction_estimate.argtypes = [_sz, _sz, _sz, _fp, _fp]
         L.ora_normalize_trajectory2.argtypes = [_sz, _fp, C.c_int, C.c_uint32, C.c_uint32, _fp]
         L.ora_channel_mix.argtypes = [_sz, _sz, _sz, _dp, _dp, _dp]
         L.ora_phase_correct.argtypes = [_sz, _sz, _sz, _fp, _fp, _fp]
@@ -433,3 +438,60 @@ def readout_crop(x, out_cols: int, offset: int) -> np.ndarray:
     lib().ora_readout_crop(lines, in_cols, out_cols, offset, x.view(np.float32).reshape(-1),
                            y.view(np.float32).reshape(-1))
     return y
+
+
+# ---------------------------------------------------------------- calibration
+# providers/kspacejet-calibration/src/operators/*.cpp restated: complex arrays in, complex128
+# (or complex64 for the phase model) out.
+
+def _c64_pairs(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, np.complex64)).view(np.float32).reshape(-1)
+
+
+def channel_covariance(x, centred: bool) -> np.ndarray:
+    """[C, S] complex -> [C, C] complex128 (centred: conj(x_r - m_r)(x_c - m_c); else x_r conj(x_c))."""
+    x = np.ascontiguousarray(np.asarray(x, np.complex64))
+    C, S = x.shape
+    out = np.zeros((C, C), np.complex128)
+    lib().ora_channel_covariance(C, S, _c64_pairs(x), int(centred), out.view(np.float64).reshape(-1))
+    return out
+
+
+def hermitian_eigen(a):
+    """The reference's largest-pivot Jacobi in FP64: (eigenvalues ascending, vectors in columns)."""
+    a = np.ascontiguousarray(np.asarray(a, np.complex128))
+    n = a.shape[0]
+    vals = np.zeros(n, np.float64)
+    vecs = np.zeros((n, n), np.complex128)
+    if lib().ora_hermitian_eigen(n, a.view(np.float64).reshape(-1), vals, vecs.view(np.float64).reshape(-1)) != 0:
+        raise ValueError("eigen-decomposition failed")
+    return vals, vecs
+
+
+def noise_model_estimate(x, floor: float = float(np.float32(1e-6))) -> np.ndarray:
+    x = np.ascontiguousarray(np.asarray(x, np.complex64))
+    C, S = x.shape
+    out = np.zeros((C, C), np.complex128)
+    if lib().ora_noise_model_estimate(C, S, _c64_pairs(x), float(floor), out.view(np.float64).reshape(-1)) != 0:
+        raise ValueError("noise model estimate failed")
+    return out
+
+
+def coil_compression_basis_estimate(x, virtual_channels: int) -> np.ndarray:
+    x = np.ascontiguousarray(np.asarray(x, np.complex64))
+    P = x.shape[0]
+    S = x.size // P
+    out = np.zeros((virtual_channels, P), np.complex128)
+    if lib().ora_coil_compression_basis_estimate(P, virtual_channels, S, _c64_pairs(x),
+                                                 out.view(np.float64).reshape(-1)) != 0:
+        raise ValueError("coil compression basis estimate failed")
+    return out
+
+
+def phase_correction_estimate(ref) -> np.ndarray:
+    """[C, lines, K] complex64 -> [C, K] complex64."""
+    ref = np.ascontiguousarray(np.asarray(ref, np.complex64))
+    C, L, K = ref.shape
+    out = np.zeros((C, K), np.complex64)
+    lib().ora_phase_correction_estimate(C, L, K, _c64_pairs(ref), out.view(np.float32).reshape(-1))
+    return out

@@ -14,7 +14,7 @@ import sys
 
 TYPES = ["ksj.kspace-frame", "ksj.noncartesian-kspace-frame", "ksj.trajectory-frame", "ksj.coil-image-frame",
          "ksj.image-frame", "ksj.complex-image-frame", "ksj.sensitivity-map", "ksj.noise-model", "ksj.phase-model",
-         "ksj.coil-compression-basis"]
+         "ksj.coil-compression-basis", "ksj.noise-calibration-frame", "ksj.phase-reference-frame"]
 
 
 def main(root):

@@ -36,6 +36,9 @@ PROVIDERS = {
                             ["noise_prewhiten", "phase_correct", "coil_compress", "readout_oversampling_remove",
                              "noncartesian_noise_prewhiten", "noncartesian_coil_compress"],
                             "6ff48e549c4ff3ad41cfebb766e6d298360bf686dc7882f5df9e6401b7dc7325"),
+    "calibration": ("org.kspacejet.calibration",
+                    ["noise_model_estimate", "phase_correction_estimate", "coil_compression_basis_estimate"],
+                    "abd611a1428074691393e7057e21266246647acd9c8ec7d1d4ad65a971d40f25"),
     "gpu_recon": ("org.kspacejet.gpu-recon",
                   ["kb_gridding_reconstruct", "kb_gridding_reconstruct_rss", "cartesian_ifft2_rss",
                    "coil_combine_sensitivity", "noise_prewhiten", "phase_correct", "coil_compress",
@@ -84,7 +87,7 @@ def test_type_identity_digests_reproduce_the_reference_header():
     the reference's generated type_registry.h and the table the modules use."""
     fixture = json.load(open(os.path.join(GOLDEN, "type_registry_reference.json")))
     table = {t["type_ref"]: t for t in json.load(open(H.TYPES_JSON))}
-    assert len(fixture) == 10
+    assert len(fixture) == 12
     for entry in fixture:
         ident = entry["identity"]
         p, m = ident["payload"], ident["memory"]
