@@ -17,15 +17,23 @@ image.  `value` is served images per second over the timed steps.  Beside it the
     one synchronous call per scan, data resident in HBM), images/s and ms per scan;
   * plain_provider: the drop-in Provider operator (gpu-recon kb_gridding_reconstruct_rss
     through ksj_provider_query) fed host-pageable payloads as the reference's host
-    feeds them, one synchronous firing per scan (PCIe staging included);
+    feeds them, one synchronous firing per scan (PCIe staging included), and over
+    device edges (payloads in HBM);
+  * host_fed: host-memory requests through the server (pageable and pinned caller
+    buffers, PCIe staging included), the measured PCIe rates, and the like-for-like
+    ratio to the host-fed Provider;
+  * end_to_end: k-space in + RSS out per image at the served rate, against HBM peak;
   * uneven: 64 seeded items of mixed matrix / coil / spoke counts, plain per item vs
     one autotuned multi-plan call vs that call replayed as a captured HIP graph;
   * stages / roofline: per-kernel HIP-event timings of the batched pass (one pass
     lane, so kernels do not overlap) and the gather kernel against HBM bandwidth;
-  * mixed: the reference's mixed stream (60 % radial scans of the headline shape, 30 %
-    Cartesian scans of 128/256/384 matrix with 8-32 coils, 10 % 3-D kooshball volumes at
-    half the 3-D spoke count), served vs plain;
-  * cpu_baseline: the reference's own radial route compiled from its sources.
+  * mixed / mixed_poisson: the reference's mixed stream (60 % radial scans of the
+    headline shape, 30 % Cartesian scans of 128/256/384 matrix with 8-32 coils, 10 % 3-D
+    kooshball volumes at half the 3-D spoke count), served vs plain, as a burst and
+    under open-loop Poisson arrivals at several rates (p50/p99 per rate);
+  * cpu_baseline: the reference's own radial route compiled from its sources and the
+    FP32 restatement of this route, on every core of the affinity mask and on one.
+Every ratio compares like with like: both sides in HBM, or both fed from host memory.
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1]
 
@@ -53,6 +61,7 @@ HBM_PEAK_GBS = 8000.0
 N, COILS, SPOKES, READOUT, WIDTH, SIGMA = 256, 16, 402, 512, 4, 2.0
 STAGES = ("grid_prep", "grid_gather", "grid_reduce", "fft_rows", "fft_cols_deapod", "coil_combine")
 TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "radial256_c16_kb4_hbm_traffic.txt")
+COMPULSORY_BYTES = COILS * SPOKES * READOUT * 8 + N * N * 4  # k-space in, RSS out
 
 
 def log(*a):
@@ -69,7 +78,7 @@ def make_scans(ops, torch, traj, count, distinct):
         k = ops.nudft2_forward_f64(traj, img, (N - 1) / 2, (N - 1) / 2).cpu().numpy()
         base.append(synth.add_noise(k, seed).astype(np.complex64))
     scans = [torch.from_numpy(base[i % distinct]).cuda() for i in range(count)]
-    return scans, base[0]
+    return scans, base
 
 
 def algorithmic_bytes(stage: str) -> float:
@@ -115,25 +124,65 @@ def measured_traffic(label: str, scans_per_launch: int):
     return None
 
 
+def host_cores():
+    """Cores this process may run on (its affinity mask) and the cgroup CPU quota, if any."""
+    cores = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = round(int(q) / int(period), 2)
+    except (OSError, ValueError):
+        pass
+    return cores, quota
+
+
 def cpu_baseline(scan_np, traj_np, seconds_hint: float):
-    """The reference's own radial route compiled from its sources (oracle/_ref), all host cores."""
+    """Two CPU routes of the headline scan, each on every core the process may run on and
+    on one core: the reference's own radial route compiled from its sources (oracle/_ref:
+    ramp DCF + bilinear gridding on the 256 x 256 image grid + radix-2 inverse FFT + RSS),
+    and the like-for-like FP32 restatement of the GPU route (oracle/ksj_kb_route.c: KB
+    W = 4 on the 512 x 512 grid, sigma 2, deapodize, crop, RSS).  Each leg times whole
+    scans over about a quarter of `seconds_hint`."""
     import oracle as O
+    cores, quota = host_cores()
+    leg = max(1.0, seconds_hint / 4)
+
+    def timed(run, threads):
+        """images/s of `run(scans, threads) -> seconds` over about `leg` seconds."""
+        probe = run(threads, threads)
+        scans = threads * max(1, int(leg / max(probe, 1e-3)))
+        return scans / run(scans, threads), scans
+
+    def ref_run(scans, threads):
+        t0 = time.perf_counter()
+        O.ref_radial_route_rss(scan_np, traj_np, N, N, scans=scans, threads=threads)
+        return time.perf_counter() - t0
+
+    def port_run(scans, threads):
+        return O.kb_route_rss_f32(scan_np, traj_np, N, width=WIDTH, scans=scans, threads=threads, timed=True)[1]
+
+    port_all, port_scans = timed(port_run, cores)
+    port_one, _ = timed(port_run, 1)
+    like = {"value": round(port_all, 2), "unit": "images/s", "cores": cores, "kind": "port",
+            "one_core_value": round(port_one, 2),
+            "sample": f"{port_scans} scans through the FP32 restatement of the GPU route (ramp DCF, KB W={WIDTH} "
+                      f"beta Beatty on the {int(N * SIGMA)}^2 grid, radix-2 inverse FFT of the rows and of the "
+                      f"{N} kept columns, deapodize, crop, RSS), scan-parallel on {cores} threads; per-trajectory "
+                      f"tables made once, outside the timing, as the GPU's resident plan"}
     if not O.ref_available():
-        return None
-    threads = min(16, os.cpu_count() or 1)
-    t0 = time.perf_counter()
-    O.ref_radial_route_rss(scan_np, traj_np, N, N, scans=threads, threads=threads)
-    one_round = time.perf_counter() - t0
-    rounds = max(1, int(seconds_hint / max(one_round, 1e-3)))
-    scans = rounds * threads
-    t0 = time.perf_counter()
-    O.ref_radial_route_rss(scan_np, traj_np, N, N, scans=scans, threads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": scans / dt, "unit": "images/s", "cores": threads, "kind": "reference",
-            "sample": f"{scans} scans of the 256x256x16-coil 402x512 radial shape through the reference route "
+        out = dict(like)
+        out["cpu_quota_cores"] = quota
+        return out
+    ref_all, ref_scans = timed(ref_run, cores)
+    ref_one, _ = timed(ref_run, 1)
+    return {"value": round(ref_all, 2), "unit": "images/s", "cores": cores, "kind": "reference",
+            "one_core_value": round(ref_one, 2), "cpu_quota_cores": quota,
+            "sample": f"{ref_scans} scans of the 256x256x16-coil 402x512 radial shape through the reference route "
                       f"(ramp DCF + bilinear gridding on 256x256 + radix-2 inverse FFT per coil, then RSS) "
-                      f"compiled from the reference's eigen_radial_gridding.cpp, scan-parallel on {threads} "
-                      f"threads; sample-count envelope lifted; {dt:.1f} s wall"}
+                      f"compiled from the reference's eigen_radial_gridding.cpp, scan-parallel on {cores} threads "
+                      f"(every core of the affinity mask); sample-count envelope lifted",
+            "like_for_like": like}
 
 
 def percentile(xs, q):
@@ -151,7 +200,12 @@ def main(argv=None) -> int:
     ap.add_argument("--max-batch", type=int, default=16, help="scans stacked into one server launch")
     ap.add_argument("--load", type=float, default=0.7, help="Poisson arrival rate as a fraction of served rate")
     ap.add_argument("--latency-requests", type=int, default=300)
-    ap.add_argument("--mixed-requests", type=int, default=48, help="0 skips the mixed-stream section")
+    ap.add_argument("--mixed-requests", type=int, default=48, help="0 skips the mixed-stream burst section")
+    ap.add_argument("--poisson-requests", type=int, default=300,
+                    help="requests per arrival rate of the mixed-stream Poisson sweep; 0 skips it")
+    ap.add_argument("--poisson-fractions", default="0.3,0.5,0.7,0.9",
+                    help="arrival rates of the Poisson sweep as fractions of the served burst capacity")
+    ap.add_argument("--host-fed-steps", type=int, default=10, help="bursts of host-memory requests; 0 skips them")
     ap.add_argument("--uneven-items", type=int, default=64, help="items of the uneven batch; 0 skips that section")
     ap.add_argument("--conditioning-reps", type=int, default=20,
                     help="timed launches per k-space conditioning kernel; 0 skips that section")
@@ -191,7 +245,8 @@ def main(argv=None) -> int:
     S = SPOKES * READOUT
     traj_key = int.from_bytes(hashlib.sha256(traj.cpu().numpy().tobytes()).digest()[:8], "little") | 1
     log("generating synthetic scans ...")
-    scans, scan0 = make_scans(ops, torch, traj, args.batch, min(args.distinct, args.batch))
+    scans, base = make_scans(ops, torch, traj, args.batch, min(args.distinct, args.batch))
+    scan0 = base[0]
     outs = [torch.empty((N, N), dtype=torch.float32, device="cuda") for _ in range(args.batch)]
     stream = lambda: torch.cuda.current_stream().cuda_stream
 
@@ -320,12 +375,26 @@ def main(argv=None) -> int:
                                            stream, 0.1),
                  "2d_only": run_mixed(torch, lib, _lib, server, plan, scans, traj, traj_key, args.mixed_requests,
                                       stream, 0.0)}
+    poisson = None
+    if args.poisson_requests > 0:
+        log("mixed stream under Poisson arrivals ...")
+        poisson = run_mixed_poisson(torch, lib, _lib, server, plan, scans, traj, traj_key, args.poisson_requests,
+                                    stream, [float(x) for x in args.poisson_fractions.split(",")])
     server.close()
+    host_fed = None
+    if args.host_fed_steps > 0:
+        log("host-memory requests ...")
+        host_fed = run_host_fed(torch, ops, lib, _lib, base, traj, traj_key, args.batch, args.host_fed_steps,
+                                args.warmup, args.lanes, args.max_batch)
     del scans, outs, ws
     provider = None
     if args.provider_firings > 0:
         log("drop-in Provider operator ...")
         provider = run_provider_plain(scan0, traj.cpu().numpy(), args.provider_firings, args.warmup)
+        if host_fed is not None:  # like for like: both fed from host-pageable memory
+            host_fed["plain_provider_host_images_per_s"] = provider["images_per_s"]
+            host_fed["served_over_plain_provider_host"] = round(
+                host_fed["forms"]["pageable"]["images_per_s"] / provider["images_per_s"], 2)
     volume = None
     if args.volume_coils > 0:
         log("3-D kooshball volume ...")
@@ -374,11 +443,18 @@ def main(argv=None) -> int:
         "plain": {"images_per_s": round(plain_value, 1), "ms_per_scan": round(1e3 / plain_value, 3),
                   "operator": "ksj_gpu_adjoint2_execute, one synchronous call per scan, plan resident"},
         "served_over_plain": round(value / plain_value, 2),
+        "end_to_end": {"compulsory_bytes_per_image": COMPULSORY_BYTES,
+                       "achieved_GBps": round(value * COMPULSORY_BYTES / 1e9, 1),
+                       "frac": round(value * COMPULSORY_BYTES / 1e9 / HBM_PEAK_GBS, 4),
+                       "note": "k-space in + RSS out per image over the served rate, against 8 TB/s"},
         "plain_provider": provider,
-        "served_over_plain_provider": round(value / provider["images_per_s"], 2) if provider else None,
+        "served_over_plain_provider_device": round(
+            value / provider["device_edges"]["device_trajectory"]["images_per_s"], 2) if provider else None,
+        "host_fed": host_fed,
         "stages": stage_report,
         "served_stats": served_stats,
         "mixed": mixed,
+        "mixed_poisson": poisson,
         "uneven": uneven,
         "volume": volume,
         "conditioning": conditioning,
@@ -738,100 +814,281 @@ def run_uneven(torch, ops, lib, count, steps, warmup, tune_seconds):
     return res
 
 
-def run_mixed(torch, lib, _lib, server, plan, radial_scans, traj, traj_key, count, stream, volume_fraction=0.1):
-    """Mixed request stream in the reference's proportions: 60 % radial scans of the
+class MixedStream:
+    """Request stream in the reference's mixed proportions: 60 % radial scans of the
     headline shape, 30 % Cartesian scans (matrix 128/256/384, 8-32 coils, centred), 10 %
     3-D kooshball volumes at half the 3-D spoke count (256^3, 20 coils, 8192 x 512
-    golden-means spokes, |k|^2 density).  Served (queued at once on the server) vs one
-    synchronous plain call per request with resident plans; images/s counts every
-    request, 2-D image or 3-D volume."""
-    from kspacejet_amd import ops, synth
-    rng = np.random.default_rng(11)
-    S = SPOKES * READOUT
-    cart_shapes = [(int(rng.choice([128, 256, 384])), int(rng.integers(8, 33))) for _ in range(6)]
-    cart = [torch.from_numpy(synth.cartesian_kspace2(n, c, synth.scan_seed(5, i)).astype(np.complex64)).cuda()
-            for i, (n, c) in enumerate(cart_shapes)]
-    n3 = round(volume_fraction * count)
-    nc = round(0.3 * count)
-    kinds = rng.permutation(["radial"] * (count - n3 - nc) + ["cartesian"] * nc + ["volume"] * n3)
-    vol_n, vol_coils, vol_spokes = 256, 20, 8192
-    traj3 = ops.golden_means3(vol_spokes, READOUT)
-    S3 = traj3.shape[0]
-    gen = torch.Generator(device="cuda").manual_seed(0x4B53504A ^ (5 << 32))
-    k3 = torch.randn((vol_coils, S3), dtype=torch.complex64, device="cuda", generator=gen)
-    traj3_key = int.from_bytes(hashlib.sha256(traj3.cpu().numpy().tobytes()).digest()[:8], "little") | 1
-    reqs, jobs = [], []
-    for i, kind in enumerate(kinds):
-        if kind == "radial":
-            k = radial_scans[i % len(radial_scans)]
-            o = torch.empty((N, N), dtype=torch.float32, device="cuda")
-            reqs.append(_lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_RADIAL2D, N, N, COILS, S, WIDTH,
-                                          _lib.DENSITY_RAMP, 0, 0, traj_key, traj.data_ptr(), k.data_ptr(), None,
-                                          o.data_ptr()))
-        elif kind == "cartesian":
-            k = cart[i % len(cart)]
-            c, n, _ = k.shape
-            o = torch.empty((n, n), dtype=torch.float32, device="cuda")
-            reqs.append(_lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_CARTESIAN2D, n, n, c, 0, 0, 0, 1, 0,
-                                          0, None, k.data_ptr(), None, o.data_ptr()))
-        else:
-            k = k3
-            o = torch.empty((vol_n, vol_n, vol_n), dtype=torch.float32, device="cuda")
-            reqs.append(_lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_RADIAL3D, vol_n, vol_n, vol_coils,
-                                          S3, WIDTH, _lib.DENSITY_RAMP_SQUARED, 0, vol_n, traj3_key,
-                                          traj3.data_ptr(), k.data_ptr(), None, o.data_ptr()))
-        jobs.append((kind, k, o))
-    arr = (_lib.ServeRequest * count)(*reqs)
-    tickets = (C.c_uint64 * count)()
+    golden-means spokes, |k|^2 density), in a seeded order.  `served` queues requests on
+    the server; `plain` runs one request as a synchronous plain call with resident plans.
+    A 2-D image and a 3-D volume each count as one request."""
 
-    def served():
+    def __init__(self, torch, lib, _lib, plan, radial_scans, traj, traj_key, count, stream, volume_fraction,
+                 seed=11):
+        from kspacejet_amd import ops, synth
+        self.torch, self.lib, self.stream = torch, lib, stream
+        rng = np.random.default_rng(seed)
+        S = SPOKES * READOUT
+        cart_shapes = [(int(rng.choice([128, 256, 384])), int(rng.integers(8, 33))) for _ in range(6)]
+        cart = [torch.from_numpy(synth.cartesian_kspace2(n, c, synth.scan_seed(5, i)).astype(np.complex64)).cuda()
+                for i, (n, c) in enumerate(cart_shapes)]
+        n3 = round(volume_fraction * count)
+        nc = round(0.3 * count)
+        self.kinds = rng.permutation(["radial"] * (count - n3 - nc) + ["cartesian"] * nc + ["volume"] * n3)
+        self.vol_n, self.vol_coils, vol_spokes = 256, 20, 8192
+        vol_n = self.vol_n
+        self.plan3 = None
+        if n3:
+            traj3 = ops.golden_means3(vol_spokes, READOUT)
+            S3 = traj3.shape[0]
+            gen = torch.Generator(device="cuda").manual_seed(0x4B53504A ^ (5 << 32))
+            self.k3 = torch.randn((self.vol_coils, S3), dtype=torch.complex64, device="cuda", generator=gen)
+            traj3_key = int.from_bytes(hashlib.sha256(traj3.cpu().numpy().tobytes()).digest()[:8], "little") | 1
+            self.plan3 = ops.Plan3((vol_n, vol_n, vol_n), S3, width=WIDTH).set_trajectory(traj3)
+        reqs, self.jobs = [], []
+        for i, kind in enumerate(self.kinds):
+            if kind == "radial":
+                k = radial_scans[i % len(radial_scans)]
+                o = torch.empty((N, N), dtype=torch.float32, device="cuda")
+                reqs.append(_lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_RADIAL2D, N, N, COILS, S, WIDTH,
+                                              _lib.DENSITY_RAMP, 0, 0, traj_key, traj.data_ptr(), k.data_ptr(), None,
+                                              o.data_ptr()))
+            elif kind == "cartesian":
+                k = cart[i % len(cart)]
+                c, n, _ = k.shape
+                o = torch.empty((n, n), dtype=torch.float32, device="cuda")
+                reqs.append(_lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_CARTESIAN2D, n, n, c, 0, 0, 0, 1,
+                                              0, 0, None, k.data_ptr(), None, o.data_ptr()))
+            else:
+                k = self.k3
+                o = torch.empty((vol_n, vol_n, vol_n), dtype=torch.float32, device="cuda")
+                reqs.append(_lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_RADIAL3D, vol_n, vol_n,
+                                              self.vol_coils, S3, WIDTH, _lib.DENSITY_RAMP_SQUARED, 0, vol_n, traj3_key,
+                                              traj3.data_ptr(), k.data_ptr(), None, o.data_ptr()))
+            self.jobs.append((kind, k, o))
+        self.count = count
+        self.reqs = (_lib.ServeRequest * count)(*reqs)
+        self.ws3 = torch.empty(self.plan3.workspace_bytes(self.vol_coils, 8) if n3 else 1, dtype=torch.uint8,
+                               device="cuda")
+        self.plan = plan
+        self.ws = torch.empty(max(plan.workspace_bytes(COILS),
+                                  max(lib.ksj_gpu_ifft2_workspace_bytes(c, n, n) for n, c in cart_shapes)),
+                              dtype=torch.uint8, device="cuda")
+        self.coil_buf = torch.empty(max(n * n * c for n, c in cart_shapes), dtype=torch.complex64, device="cuda")
+        self.cart_shapes = cart_shapes
+        self.vol_spokes = vol_spokes
+
+    def describe(self):
+        return {"requests": self.count, "radial": int((self.kinds == "radial").sum()),
+                "cartesian": int((self.kinds == "cartesian").sum()), "volumes": int((self.kinds == "volume").sum()),
+                "cartesian_shapes": [f"{n}x{n}x{c}" for n, c in self.cart_shapes],
+                "volume_shape": f"{self.vol_n}^3 x {self.vol_coils} coils, {self.vol_spokes}x{READOUT} spokes"}
+
+    def plain(self, i):
+        kind, k, o = self.jobs[i]
+        lib, stream = self.lib, self.stream
+        if kind == "radial":
+            lib.ksj_gpu_adjoint2_execute(self.plan._handle, k.data_ptr(), COILS, None, 0, o.data_ptr(),
+                                         self.ws.data_ptr(), self.ws.numel(), stream())
+        elif kind == "cartesian":
+            c, n, _ = k.shape
+            lib.ksj_gpu_ifft2_coil_images(k.data_ptr(), self.coil_buf.data_ptr(), c, n, n, 1, 0, self.ws.data_ptr(),
+                                          self.ws.numel(), stream())
+            lib.ksj_gpu_rss(self.coil_buf.data_ptr(), o.data_ptr(), n * n, c, stream())
+        else:
+            lib.ksj_gpu_adjoint3_execute(self.plan3._handle, k.data_ptr(), self.vol_coils, 8, None, o.data_ptr(),
+                                         self.ws3.data_ptr(), self.ws3.numel(), stream())
+        self.torch.cuda.synchronize()
+
+    def burst(self, server):
+        """Every request queued at once; returns (seconds, latencies)."""
+        lib = self.lib
+        tickets = (C.c_uint64 * self.count)()
+        t0 = time.perf_counter()
         server.hold(True)
-        if lib.ksj_gpu_serve_submit_batch(server._handle, count, arr, tickets) != 0:
+        if lib.ksj_gpu_serve_submit_batch(server._handle, self.count, self.reqs, tickets) != 0:
             raise RuntimeError(lib.ksj_gpu_last_error().decode())
         server.hold(False)
         server.drain()
-        return [server.wait(t) for t in tickets]
+        dt = time.perf_counter() - t0
+        return dt, [server.wait(t) for t in tickets]
 
-    plan3 = ops.Plan3((vol_n, vol_n, vol_n), S3, width=WIDTH).set_trajectory(traj3) if n3 else None
-    ws3 = torch.empty(plan3.workspace_bytes(vol_coils, 8) if n3 else 1, dtype=torch.uint8, device="cuda")
-    ws = torch.empty(max(plan.workspace_bytes(COILS),
-                         max(lib.ksj_gpu_ifft2_workspace_bytes(c, n, n) for n, c in cart_shapes)),
-                     dtype=torch.uint8, device="cuda")
-    coil_buf = torch.empty(max(n * n * c for n, c in cart_shapes), dtype=torch.complex64, device="cuda")
+    def plain_all(self):
+        t0 = time.perf_counter()
+        for i in range(self.count):
+            self.plain(i)
+        return time.perf_counter() - t0
 
-    def plain():
-        for kind, k, o in jobs:
-            if kind == "radial":
-                lib.ksj_gpu_adjoint2_execute(plan._handle, k.data_ptr(), COILS, None, 0, o.data_ptr(), ws.data_ptr(),
-                                             ws.numel(), stream())
-            elif kind == "cartesian":
-                c, n, _ = k.shape
-                lib.ksj_gpu_ifft2_coil_images(k.data_ptr(), coil_buf.data_ptr(), c, n, n, 1, 0, ws.data_ptr(),
-                                              ws.numel(), stream())
-                lib.ksj_gpu_rss(coil_buf.data_ptr(), o.data_ptr(), n * n, c, stream())
-            else:
-                lib.ksj_gpu_adjoint3_execute(plan3._handle, k.data_ptr(), vol_coils, 8, None, o.data_ptr(),
-                                             ws3.data_ptr(), ws3.numel(), stream())
-            torch.cuda.synchronize()
+    def poisson_served(self, server, rate, seed):
+        """Open-loop Poisson arrivals at `rate` requests/s, each submitted at its arrival;
+        latency = arrival to completion."""
+        lib = self.lib
+        arrivals = np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, self.count))
+        one = _one_request_array()
+        tickets = []
+        t_start = time.perf_counter()
+        for i, t_arr in enumerate(arrivals):
+            while time.perf_counter() - t_start < t_arr:
+                pass
+            r = one(self.reqs[i])
+            t = C.c_uint64(0)
+            if lib.ksj_gpu_serve_submit_batch(server._handle, 1, r, C.byref(t)) != 0:
+                raise RuntimeError(lib.ksj_gpu_last_error().decode())
+            tickets.append(t.value)
+        lat = [server.wait(t) for t in tickets]
+        return time.perf_counter() - t_start, lat
 
-    served()
-    plain()
-    t0 = time.perf_counter()
-    lat = served()
-    t_served = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    plain()
-    t_plain = time.perf_counter() - t0
-    if plan3 is not None:
-        plan3.close()
-    del ws3, k3
-    return {"requests": count, "radial": int((kinds == "radial").sum()), "cartesian": int((kinds == "cartesian").sum()),
-            "volumes": int((kinds == "volume").sum()),
-            "cartesian_shapes": [f"{n}x{n}x{c}" for n, c in cart_shapes],
-            "volume_shape": f"{vol_n}^3 x {vol_coils} coils, {vol_spokes}x{READOUT} spokes",
-            "served_requests_per_s": round(count / t_served, 1), "plain_requests_per_s": round(count / t_plain, 1),
-            "served_over_plain": round(t_plain / t_served, 2), "burst_p50_ms": round(percentile(lat, 50), 3),
-            "burst_p99_ms": round(percentile(lat, 99), 3)}
+    def poisson_plain(self, rate, seed):
+        """The same arrivals served one at a time, in order, by synchronous plain calls
+        (a plain per-request operator loop); latency = arrival to completion."""
+        arrivals = np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, self.count))
+        lat = []
+        t_start = time.perf_counter()
+        for i, t_arr in enumerate(arrivals):
+            while time.perf_counter() - t_start < t_arr:
+                pass
+            self.plain(i)
+            lat.append(1e3 * (time.perf_counter() - t_start - t_arr))
+        return time.perf_counter() - t_start, lat
+
+    def close(self):
+        if self.plan3 is not None:
+            self.plan3.close()
+        self.jobs = None
+
+
+def _one_request_array():
+    from kspacejet_amd import _lib
+    return _lib.ServeRequest * 1
+
+
+def run_mixed(torch, lib, _lib, server, plan, radial_scans, traj, traj_key, count, stream, volume_fraction=0.1):
+    """The mixed stream queued at once (a burst) on the server vs one synchronous plain
+    call per request: requests/s and burst latencies."""
+    ms = MixedStream(torch, lib, _lib, plan, radial_scans, traj, traj_key, count, stream, volume_fraction)
+    ms.burst(server)
+    ms.plain_all()
+    t_served, lat = ms.burst(server)
+    t_plain = ms.plain_all()
+    out = ms.describe()
+    out.update({"served_requests_per_s": round(count / t_served, 1), "plain_requests_per_s": round(count / t_plain, 1),
+                "served_over_plain": round(t_plain / t_served, 2), "burst_p50_ms": round(percentile(lat, 50), 3),
+                "burst_p99_ms": round(percentile(lat, 99), 3)})
+    ms.close()
+    return out
+
+
+def run_mixed_poisson(torch, lib, _lib, server, plan, radial_scans, traj, traj_key, count, stream, fractions):
+    """The mixed stream (10 % volumes) under open-loop Poisson arrivals at several rates,
+    each a fraction of the served burst capacity of the stream: per rate, p50/p99
+    latency and achieved requests/s, served (the server) vs plain (one synchronous plain
+    call per request in arrival order)."""
+    ms = MixedStream(torch, lib, _lib, plan, radial_scans, traj, traj_key, count, stream, 0.1, seed=13)
+    ms.burst(server)
+    t_burst, _ = ms.burst(server)
+    t_plain = ms.plain_all()
+    capacity = count / t_burst
+    rates = []
+    for j, f in enumerate(fractions):
+        lam = f * capacity
+        ts, ls = ms.poisson_served(server, lam, 100 + j)
+        tp, lp = ms.poisson_plain(lam, 100 + j)
+        entry = {"fraction_of_served_capacity": f, "arrival_rate_per_s": round(lam, 1),
+                 "served_p50_ms": round(percentile(ls, 50), 3), "served_p99_ms": round(percentile(ls, 99), 3),
+                 "plain_p50_ms": round(percentile(lp, 50), 3), "plain_p99_ms": round(percentile(lp, 99), 3),
+                 "served_requests_per_s": round(count / ts, 1), "plain_requests_per_s": round(count / tp, 1),
+                 "p99_plain_over_served": round(percentile(lp, 99) / percentile(ls, 99), 2)}
+        for cls, sel in (("2d", ms.kinds != "volume"), ("3d", ms.kinds == "volume")):
+            if sel.any():
+                a, b = np.asarray(ls)[sel], np.asarray(lp)[sel]
+                entry[f"by_class_{cls}"] = {"served_p50_ms": round(percentile(a, 50), 3),
+                                            "served_p99_ms": round(percentile(a, 99), 3),
+                                            "plain_p50_ms": round(percentile(b, 50), 3),
+                                            "plain_p99_ms": round(percentile(b, 99), 3)}
+        rates.append(entry)
+        log(f"  poisson {f:.2f} x capacity: served p99 {rates[-1]['served_p99_ms']} ms, "
+            f"plain p99 {rates[-1]['plain_p99_ms']} ms")
+    out = ms.describe()
+    out.update({"served_capacity_requests_per_s": round(capacity, 1),
+                "plain_capacity_requests_per_s": round(count / t_plain, 1),
+                "capacity_served_over_plain": round(t_plain / t_burst, 2), "rates": rates,
+                "process": "open-loop Poisson arrivals of the seeded mixed order; served: each request submitted at "
+                           "its arrival; plain: one synchronous call per request in arrival order"})
+    ms.close()
+    return out
+
+
+def run_host_fed(torch, ops, lib, _lib, base_np, traj, traj_key, batch, steps, warmup, lanes, max_batch):
+    """Host-memory requests of the headline scan through the server: k-space in host
+    memory, RSS back to host memory, bursts of `batch` distinct scans (host buffers of
+    26.3 MB each, so nothing is cache resident).  Two forms: pageable caller buffers
+    (copied into the server's pinned staging by its copy threads) and pinned caller
+    buffers (DMA from and to them).  Beside them the measured PCIe rates (256 MiB copies)
+    and the rate the H2D link alone would allow."""
+    S = SPOKES * READOUT
+    traj_h = traj.cpu()
+    in_bytes = COILS * S * 8
+    out_bytes = N * N * 4
+    server = ops.Server(lanes=lanes, max_batch_scans=max_batch)
+    forms = {}
+    for form in ("pageable", "pinned"):
+        pin = form == "pinned"
+        ks = [torch.from_numpy(base_np[i % len(base_np)].copy()) for i in range(batch)]
+        if pin:
+            ks = [k.pin_memory() for k in ks]
+        outs = [torch.empty((N, N), dtype=torch.float32, pin_memory=pin) for _ in range(batch)]
+        reqs = (_lib.ServeRequest * batch)(*[
+            _lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_RADIAL2D, N, N, COILS, S, WIDTH,
+                              _lib.DENSITY_RAMP, 0, 0, traj_key, traj_h.data_ptr(), k.data_ptr(), None, o.data_ptr(),
+                              _lib.SERVE_HOST_MEMORY, 0) for k, o in zip(ks, outs)])
+        tickets = (C.c_uint64 * batch)()
+
+        def step():
+            if lib.ksj_gpu_serve_submit_batch(server._handle, batch, reqs, tickets) != 0:
+                raise RuntimeError(lib.ksj_gpu_last_error().decode())
+            return list(tickets)
+
+        for _ in range(max(1, warmup)):
+            for t in step():
+                server.wait(t)
+        all_t = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            all_t += step()
+        server.drain()
+        dt = time.perf_counter() - t0
+        lat = [server.wait(t) for t in all_t]
+        forms[form] = {"images_per_s": round(batch * steps / dt, 1), "saturated_p50_ms": round(percentile(lat, 50), 3),
+                       "saturated_p99_ms": round(percentile(lat, 99), 3), "images": batch * steps}
+        del ks, outs, reqs
+    server.close()
+    # PCIe rates of plain copies, timed with events on torch's stream (where torch copies).
+    big = 256 << 20
+    dev = torch.empty(big, dtype=torch.uint8, device="cuda")
+    rates = {}
+    for name, host, h2d in (("h2d_pinned", torch.empty(big, dtype=torch.uint8, pin_memory=True), True),
+                            ("d2h_pinned", torch.empty(big, dtype=torch.uint8, pin_memory=True), False),
+                            ("h2d_pageable", torch.empty(big, dtype=torch.uint8), True)):
+        host.fill_(1)
+        for _ in range(2):
+            (dev.copy_(host, non_blocking=True) if h2d else host.copy_(dev, non_blocking=True))
+        torch.cuda.synchronize()
+        reps = 5
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            (dev.copy_(host, non_blocking=True) if h2d else host.copy_(dev, non_blocking=True))
+        e1.record()
+        e1.synchronize()
+        rates[name + "_GBps"] = round(reps * big / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+        del host
+    del dev
+    link_bound = rates["h2d_pinned_GBps"] * 1e9 / (in_bytes + 0.0)
+    for f in forms.values():
+        f["fraction_of_h2d_link_bound"] = round(f["images_per_s"] / link_bound, 3)
+    return {"forms": forms, "pcie": rates, "host_bytes_per_image": in_bytes + out_bytes,
+            "h2d_link_bound_images_per_s": round(link_bound, 1), "batch": batch, "steps": steps,
+            "server": f"{lanes} lanes, up to {max_batch} scans per launch, 1024 MiB pinned staging, "
+                      f"{ops.tune_get('serve_copy_threads')} copy threads"}
 
 
 if __name__ == "__main__":

@@ -413,8 +413,12 @@ typedef struct ksj_gpu_serve_config {
   uint32_t lanes;              /* batches in flight, 1..8 (0 = 2) */
   uint32_t max_batch_scans;    /* scans per radial launch, 1..64 (0 = 8) */
   uint32_t plan_cache_entries; /* 0 = 16 */
-  uint32_t reserved0;
+  uint32_t host_staging_mib;   /* pinned staging of host-memory requests in MiB, allocated at the first
+                                  such request (0 = 1024) */
 } ksj_gpu_serve_config;
+
+#define KSJ_GPU_SERVE_DEVICE_MEMORY 0 /* request buffers are device memory of the server's GPU */
+#define KSJ_GPU_SERVE_HOST_MEMORY 1   /* request buffers are host memory (pageable or pinned) */
 
 typedef struct ksj_gpu_serve_request {
   uint32_t struct_size;
@@ -430,6 +434,15 @@ typedef struct ksj_gpu_serve_request {
   const void* d_kspace;          /* [C][S] (radial) or [C][R][Cols] (Cartesian) cf32 */
   void* d_images;                /* coil-last [R][Cols][C] cf32, or NULL */
   float* d_rss;                  /* [R][Cols] float, or NULL */
+  uint32_t memory;               /* KSJ_GPU_SERVE_*_MEMORY: where the four buffers above lie (2-D kinds
+                                    only may be host memory).  A host-memory request is copied into the
+                                    server's pinned staging when it is submitted (the call blocks while
+                                    the staging is full), sent to the GPU on its lane's stream ahead of
+                                    its kernels (so the transfer overlaps other lanes' work), and its
+                                    outputs are copied back to the caller's buffers when it completes.
+                                    A request struct without this field (an older, shorter struct_size)
+                                    is device memory. */
+  uint32_t reserved1;
 } ksj_gpu_serve_request;
 
 typedef struct ksj_gpu_serve_stats {
@@ -442,7 +455,7 @@ typedef struct ksj_gpu_serve_stats {
 typedef struct ksj_gpu_server ksj_gpu_server;
 
 int ksj_gpu_serve_create(const ksj_gpu_serve_config* config, ksj_gpu_server** out_server);
-/* Queues a request (device buffers must stay valid until it completes); *ticket > 0. */
+/* Queues a request (its buffers must stay valid until it completes); *ticket > 0. */
 int ksj_gpu_serve_submit(ksj_gpu_server* server, const ksj_gpu_serve_request* request, uint64_t* ticket);
 /* Queues `count` requests at once (all validated first; none is queued on an error). */
 int ksj_gpu_serve_submit_batch(ksj_gpu_server* server, uint32_t count, const ksj_gpu_serve_request* requests,
@@ -502,6 +515,9 @@ void ksj_gpu_serve_destroy(ksj_gpu_server* server);
  *   "matrix_weights"       1 = new 2-D matrix plans store a 4-byte (sample, window bin) visit and the
  *                          gather forms the cell weights from the sample's taps (default), 0 = every
  *                          visit stores its 16 or 32 cell weights (same bits; plan size and speed only)
+ *   "serve_copy_threads"   worker threads (0..15, default 7) a server created afterwards adds to the
+ *                          submitting thread for memcpy of host-memory requests into and out of its pinned
+ *                          staging (copies of 4 MiB or more; same bits)
  * Every setting gives bit-identical results, except item_visits, matrix_item_visits,
  * matrix3_item_visits, matrix_block_width, the tile keys, gather_scheme and gather3_scheme (float summation
  * order of the gather), fft_schedule (FFT rounding) and mix_scheme (summation order of the

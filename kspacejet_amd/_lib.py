@@ -65,7 +65,7 @@ class Plan3Info(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("device", C.c_int32), ("hbm_budget_bytes", C.c_uint64),
                 ("lanes", C.c_uint32), ("max_batch_scans", C.c_uint32), ("plan_cache_entries", C.c_uint32),
-                ("reserved0", C.c_uint32)]
+                ("host_staging_mib", C.c_uint32)]
 
 
 class ServeRequest(C.Structure):
@@ -73,7 +73,8 @@ class ServeRequest(C.Structure):
                 ("image_cols", C.c_uint32), ("channels", C.c_uint32), ("samples", C.c_uint32),
                 ("kernel_width", C.c_uint32), ("density", C.c_uint32), ("centred", C.c_uint32),
                 ("image_depth", C.c_uint32), ("trajectory_key", C.c_uint64), ("d_traj", C.c_void_p),
-                ("d_kspace", C.c_void_p), ("d_images", C.c_void_p), ("d_rss", C.c_void_p)]
+                ("d_kspace", C.c_void_p), ("d_images", C.c_void_p), ("d_rss", C.c_void_p),
+                ("memory", C.c_uint32), ("reserved1", C.c_uint32)]
 
 
 class ServeStats(C.Structure):
@@ -85,6 +86,8 @@ class ServeStats(C.Structure):
 SERVE_RADIAL2D = 0
 SERVE_CARTESIAN2D = 1
 SERVE_RADIAL3D = 2
+SERVE_DEVICE_MEMORY = 0
+SERVE_HOST_MEMORY = 1
 
 _vp = C.c_void_p
 _u32 = C.c_uint32

@@ -614,6 +614,7 @@ int ksj_gpu_serve_create(const ksj_gpu_serve_config* config, ksj_gpu_server** ou
     c.lanes = config->lanes ? config->lanes : 2;
     c.max_batch_scans = config->max_batch_scans ? config->max_batch_scans : 8;
     c.plan_cache_entries = config->plan_cache_entries ? config->plan_cache_entries : 16;
+    c.host_staging_bytes = uint64_t{config->host_staging_mib ? config->host_staging_mib : 1024u} << 20;
     auto* h = new ksj_gpu_server{nullptr};
     try {
       h->impl = new ksj::gpu::Server(c);
@@ -629,7 +630,14 @@ int ksj_gpu_serve_create(const ksj_gpu_serve_config* config, ksj_gpu_server** ou
 
 namespace {
 ksj::gpu::ServeRequest to_request(const ksj_gpu_serve_request* request) {
-  ksj::gpu::require(request->struct_size >= sizeof(ksj_gpu_serve_request), "request struct too small");
+  // A struct that ends before `memory` (the layout without host-memory requests) is a
+  // device-memory request.
+  ksj::gpu::require(request->struct_size >= offsetof(ksj_gpu_serve_request, memory), "request struct too small");
+  const uint32_t memory = request->struct_size >= offsetof(ksj_gpu_serve_request, reserved1)
+                              ? request->memory
+                              : KSJ_GPU_SERVE_DEVICE_MEMORY;
+  ksj::gpu::require(memory == KSJ_GPU_SERVE_DEVICE_MEMORY || memory == KSJ_GPU_SERVE_HOST_MEMORY,
+                    "request memory must be device or host");
   ksj::gpu::ServeRequest r;
   r.kind = request->kind;
   r.image_rows = request->image_rows;
@@ -645,6 +653,7 @@ ksj::gpu::ServeRequest to_request(const ksj_gpu_serve_request* request) {
   r.d_kspace = static_cast<const float2*>(request->d_kspace);
   r.d_images = static_cast<float2*>(request->d_images);
   r.d_rss = request->d_rss;
+  r.host = memory == KSJ_GPU_SERVE_HOST_MEMORY;
   return r;
 }
 }  // namespace

@@ -94,6 +94,7 @@ enum TuneKey : int {
   kTuneMatrixPersistent,  // 2-D matrix gather: 0 = one wave per run; n = n resident workgroups per CU looping over
                           // their XCD's runs (same bits)
   kTuneMatrixTapShare,    // 2-D matrix gather, packed visits: 1 = one tap load per lane shared by bpermute (same bits)
+  kTuneServeCopyThreads,  // servers created from now on: memcpy worker threads (beside the caller) for host requests
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);

@@ -36,6 +36,28 @@ uint64_t mix(uint64_t h, uint64_t v) {
   return h;
 }
 
+uint64_t round_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
+
+// True when [p, p + len) lies in one pinned (page-locked) host allocation, which the
+// copy engines read and write directly.
+bool pinned_host(const void* p, uint64_t len) {
+  hipPointerAttribute_t a{}, b{};
+  const char* first = static_cast<const char*>(p);
+  const char* last = first + len - 1;
+  for (auto [attr, q] : {std::pair<hipPointerAttribute_t*, const char*>{&a, first}, {&b, last}}) {
+    if (hipPointerGetAttributes(attr, q) != hipSuccess) {
+      (void)hipGetLastError();  // pageable memory: clear the error this thread now holds
+      return false;
+    }
+    if (attr->type != hipMemoryTypeHost || attr->hostPointer == nullptr) return false;
+  }
+  // Both ends in the same allocation: the same offset from the start of the range.
+  return static_cast<const char*>(b.hostPointer) - static_cast<const char*>(a.hostPointer) == last - first;
+}
+
+constexpr size_t kCopyChunk = 2u << 20;  // bytes per memcpy task
+constexpr size_t kCopyParallelMin = 4u << 20;
+
 double ms_since(std::chrono::steady_clock::time_point t) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
 }
@@ -53,6 +75,84 @@ uint64_t trajectory_hash(const float* d_traj, uint64_t values, hipStream_t strea
   KSJ_HIP(hipFreeAsync(d_out, stream));
   KSJ_HIP(hipStreamSynchronize(stream));
   return h;
+}
+
+uint64_t trajectory_hash_host(const float* h_traj, uint64_t values) {
+  // The same chunking and fold as hash_kernel, so a trajectory keys one plan whether it
+  // arrives in host or device memory.
+  const auto* words = reinterpret_cast<const uint32_t*>(h_traj);
+  const uint64_t per = (values + kHashThreads - 1) / kHashThreads;
+  uint64_t f = 1469598103934665603ull ^ values;
+  for (uint64_t t = 0; t < kHashThreads; ++t) {
+    const uint64_t b = t * per, e = std::min(b + per, values);
+    uint64_t h = 1469598103934665603ull;
+    for (uint64_t i = b; i < e; ++i) h = (h ^ words[i]) * 1099511628211ull;
+    f = (f ^ h) * 1099511628211ull;
+  }
+  return f;
+}
+
+CopyPool::CopyPool(unsigned workers) {
+  for (unsigned i = 0; i < workers; ++i)
+    workers_.emplace_back([this] {
+      uint64_t seen = 0;
+      while (true) {
+        std::shared_ptr<Job> j;
+        {
+          std::unique_lock<std::mutex> lock(mu_);
+          cv_.wait(lock, [&] { return stop_ || generation_ != seen; });
+          if (stop_) return;
+          seen = generation_;
+          j = job_;
+        }
+        if (j) run(*j);
+      }
+    });
+}
+
+CopyPool::~CopyPool() {
+  {
+    std::lock_guard<std::mutex> lock(mu_);
+    stop_ = true;
+  }
+  cv_.notify_all();
+  for (std::thread& t : workers_) t.join();
+}
+
+void CopyPool::run(Job& j) {
+  while (true) {
+    const size_t c = j.next.fetch_add(1);
+    if (c >= j.chunks) return;
+    const size_t b = c * kCopyChunk, n = std::min(kCopyChunk, j.bytes - b);
+    std::memcpy(j.dst + b, j.src + b, n);
+    if (j.done.fetch_add(1) + 1 == j.chunks) {
+      std::lock_guard<std::mutex> lock(mu_);
+      cv_done_.notify_all();
+    }
+  }
+}
+
+void CopyPool::copy(void* dst, const void* src, size_t bytes) {
+  if (bytes < kCopyParallelMin || workers_.empty()) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  std::lock_guard<std::mutex> one(job_mu_);  // one parallel copy at a time
+  auto j = std::make_shared<Job>();
+  j->dst = static_cast<char*>(dst);
+  j->src = static_cast<const char*>(src);
+  j->bytes = bytes;
+  j->chunks = (bytes + kCopyChunk - 1) / kCopyChunk;
+  {
+    std::lock_guard<std::mutex> lock(mu_);
+    job_ = j;
+    ++generation_;
+  }
+  cv_.notify_all();
+  run(*j);
+  std::unique_lock<std::mutex> lock(mu_);
+  cv_done_.wait(lock, [&] { return j->done.load() == j->chunks; });
+  job_.reset();
 }
 
 uint64_t Server::key_of(const ServeRequest& r) {
@@ -78,6 +178,7 @@ Server::Server(const ServeConfig& cfg) : cfg_(cfg) {
     KSJ_HIP(hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking));
     KSJ_HIP(hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
   }
+  copies_ = std::make_unique<CopyPool>(static_cast<unsigned>(tune_get(kTuneServeCopyThreads)));
   thread_ = std::thread([this] { run(); });
 }
 
@@ -86,16 +187,23 @@ Server::~Server() {
     std::lock_guard<std::mutex> lock(mu_);
     stop_ = true;
   }
+  {
+    std::lock_guard<std::mutex> lock(staging_.mu);
+    staging_.stop = true;
+  }
+  staging_.cv.notify_all();
   cv_work_.notify_all();
   if (thread_.joinable()) thread_.join();
   (void)hipSetDevice(cfg_.device);
   for (Lane& l : lanes_) {
     if (l.stream) (void)hipStreamSynchronize(l.stream);
     if (l.ws) (void)hipFree(l.ws);
+    if (l.stage) (void)hipFree(l.stage);
     if (l.done) (void)hipEventDestroy(l.done);
     if (l.stream) (void)hipStreamDestroy(l.stream);
   }
   plans_.clear();
+  if (staging_.base) (void)hipHostFree(staging_.base);
 }
 
 Server::Pending Server::prepare(const ServeRequest& r) {
@@ -106,6 +214,7 @@ Server::Pending Server::prepare(const ServeRequest& r) {
   require(r.image_rows >= 2 && r.image_cols >= 2 && r.image_rows <= 4096 && r.image_cols <= 4096,
           "request image dimensions out of range");
   require(r.d_kspace != nullptr && (r.d_images != nullptr || r.d_rss != nullptr), "request needs k-space and an output");
+  require(!r.host || r.kind != kServeRadial3D, "host-memory requests are 2-D (3-D volumes are served from HBM)");
   Pending p{};
   p.req = r;
   if (r.kind == kServeRadial2D || r.kind == kServeRadial3D) {
@@ -117,7 +226,9 @@ Server::Pending Server::prepare(const ServeRequest& r) {
             "density must be none or ramp (2-D) / ramp squared (3-D)");
     require(!three || (r.image_depth >= 2 && r.image_depth <= 1024), "3-D request needs an image depth in [2, 1024]");
     if (!three) p.req.image_depth = 0;
-    if (p.req.trajectory_key == 0) {
+    if (p.req.trajectory_key == 0 && r.host) {
+      p.req.trajectory_key = trajectory_hash_host(r.d_traj, static_cast<uint64_t>(r.samples) * 2) | 1ull;
+    } else if (p.req.trajectory_key == 0) {
       KSJ_HIP(hipSetDevice(cfg_.device));
       hipStream_t s = nullptr;
       KSJ_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -130,7 +241,59 @@ Server::Pending Server::prepare(const ServeRequest& r) {
     p.req.trajectory_key = 0;
   }
   p.plan_key = key_of(p.req);
+  if (r.host) {
+    const uint64_t pixels = static_cast<uint64_t>(r.image_rows) * r.image_cols;
+    const uint64_t values = r.kind == kServeRadial2D ? uint64_t{r.samples} : pixels;
+    auto set = [](Pending::HostBuf& h, const void* user, uint64_t len) {
+      if (!user) return;
+      h.user = const_cast<void*>(user);
+      h.len = len;
+      h.dev = round_up(len, 256);
+      h.direct = pinned_host(user, len);
+    };
+    set(p.in, r.d_kspace, values * r.channels * sizeof(float2));
+    set(p.rss, r.d_rss, pixels * sizeof(float));
+    set(p.img, r.d_images, pixels * r.channels * sizeof(float2));
+  }
   return p;
+}
+
+// Reserves one contiguous region of the pinned staging for `bytes` (first fit, blocking
+// while the staging is full) and returns its offset; the region carries `refs` requests.
+size_t Server::reserve_slot(size_t bytes, uint32_t refs) {
+  std::unique_lock<std::mutex> lock(staging_.mu);
+  if (!staging_.base) {
+    KSJ_HIP(hipSetDevice(cfg_.device));
+    void* base = nullptr;
+    KSJ_HIP(hipHostMalloc(&base, cfg_.host_staging_bytes, hipHostMallocDefault));
+    staging_.base = static_cast<char*>(base);
+    staging_.bytes = cfg_.host_staging_bytes;
+  }
+  require(bytes <= staging_.bytes, "host-memory request larger than the server's pinned staging");
+  while (true) {
+    require(!staging_.stop, "server is shutting down");
+    size_t at = 0;  // first gap in offset order that holds the region
+    for (const auto& kv : staging_.used) {
+      if (kv.first - at >= bytes) break;
+      at = kv.first + kv.second.bytes;
+    }
+    if (at + bytes <= staging_.bytes) {
+      staging_.used[at] = Slot{bytes, refs};
+      return at;
+    }
+    staging_.cv.wait(lock);
+  }
+}
+
+void Server::release_slot(size_t slot) {
+  {
+    std::lock_guard<std::mutex> lock(staging_.mu);
+    auto it = staging_.used.find(slot);
+    if (it == staging_.used.end()) return;
+    if (--it->second.refs > 0) return;
+    staging_.used.erase(it);
+  }
+  staging_.cv.notify_all();
 }
 
 uint64_t Server::submit(const ServeRequest& r) {
@@ -143,19 +306,51 @@ void Server::submit_many(const ServeRequest* r, uint32_t count, uint64_t* ticket
   std::vector<Pending> ps;
   ps.reserve(count);
   for (uint32_t i = 0; i < count; ++i) ps.push_back(prepare(r[i]));  // all validated before any is queued
-  {
-    std::lock_guard<std::mutex> lock(mu_);
-    const auto now = std::chrono::steady_clock::now();
-    for (uint32_t i = 0; i < count; ++i) {
-      ps[i].ticket = next_ticket_++;
-      ps[i].submitted = now;
-      tickets[i] = ps[i].ticket;
-      pending_.push_back(ps[i]);
+  for (const Pending& p : ps)
+    require(p.staged_bytes() <= cfg_.host_staging_bytes, "host-memory request larger than the server's pinned staging");
+  // Host-memory requests are staged in groups that fit the staging at once; each group
+  // is queued as soon as its inputs are copied, so a call larger than the staging cannot
+  // wait on itself.  A call that fits is queued atomically.
+  uint32_t first = 0;
+  while (first < count) {
+    uint32_t last = first;
+    size_t group = 0, staged = 0;
+    for (; last < count; ++last) {
+      const size_t b = ps[last].staged_bytes();
+      if (group + b > cfg_.host_staging_bytes) break;
+      group += b;
+      staged += b ? 1 : 0;
     }
-    stats_.submitted += count;
-    inflight_ += count;
+    if (staged > 0) {
+      const size_t base = reserve_slot(group, static_cast<uint32_t>(staged));
+      size_t at = base;
+      for (uint32_t i = first; i < last; ++i) {
+        Pending& p = ps[i];
+        if (p.staged_bytes() == 0) continue;
+        p.slot = base;
+        for (Pending::HostBuf* h : {&p.in, &p.rss, &p.img}) {
+          if (h->direct || h->dev == 0) continue;
+          h->at = at;
+          at += h->dev;
+        }
+        if (!p.in.direct) copies_->copy(staging_.base + p.in.at, p.in.user, p.in.len);
+      }
+    }
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      const auto now = std::chrono::steady_clock::now();
+      for (uint32_t i = first; i < last; ++i) {
+        ps[i].ticket = next_ticket_++;
+        ps[i].submitted = now;
+        tickets[i] = ps[i].ticket;
+        pending_.push_back(ps[i]);
+      }
+      stats_.submitted += last - first;
+      inflight_ += last - first;
+    }
+    cv_work_.notify_one();
+    first = last;
   }
-  cv_work_.notify_one();
 }
 
 int Server::wait(uint64_t ticket, double* latency_ms, std::string* error) {
@@ -192,7 +387,7 @@ ServeStats Server::stats() {
 
 uint64_t Server::used_bytes() const {
   uint64_t b = 0;
-  for (const Lane& l : lanes_) b += l.ws_bytes;
+  for (const Lane& l : lanes_) b += l.ws_bytes + l.stage_bytes;
   for (const auto& kv : plans_) b += kv.second.bytes;
   return b;
 }
@@ -273,7 +468,22 @@ Plan2* Server::plan_for(const Pending& p, hipStream_t stream) {
   d.samples = r.samples;
   d.item_visits = 0;  // the schedule's tuning key
   PlanPtr plan(plan2_create(d), &plan2_destroy);
-  plan2_set_trajectory(plan.get(), r.d_traj, nullptr, stream);
+  if (r.host) {  // the plan reads its trajectory once, while it is built
+    const size_t bytes = static_cast<size_t>(r.samples) * 2 * sizeof(float);
+    float* d_traj = nullptr;
+    KSJ_HIP(hipMalloc(&d_traj, bytes));
+    try {
+      KSJ_HIP(hipMemcpyAsync(d_traj, r.d_traj, bytes, hipMemcpyHostToDevice, stream));
+      plan2_set_trajectory(plan.get(), d_traj, nullptr, stream);
+    } catch (...) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipFree(d_traj);
+      throw;
+    }
+    KSJ_HIP(hipFree(d_traj));
+  } else {
+    plan2_set_trajectory(plan.get(), r.d_traj, nullptr, stream);
+  }
   PlanEntry e;
   const uint64_t cells = plan->packed_visits ? 0 : static_cast<uint64_t>(plan->block_h) * plan->block_w;
   e.bytes = plan->device_bytes + plan->visit_capacity * (sizeof(uint32_t) + cells * sizeof(float)) +
@@ -323,20 +533,38 @@ bool Server::launch_next(Lane& lane) {
       need = ifft2_workspace_bytes(r.channels, r.image_rows, r.image_cols) +
              256 + static_cast<size_t>(r.image_rows) * r.image_cols * r.channels * sizeof(float2);
     }
-    if (need > lane.ws_bytes) {
-      // Admission against the budget: drop idle plans first, then give up on this lane's growth.
-      evict_until(need - lane.ws_bytes);
-      if (used_bytes() - lane.ws_bytes + need > cfg_.hbm_budget_bytes)
+    // Admission against the budget: drop idle plans first, then give up on this lane's growth.
+    auto grow = [&](void*& buf, size_t& have, size_t want) {
+      if (want <= have) return;
+      evict_until(want - have);
+      if (used_bytes() - have + want > cfg_.hbm_budget_bytes)
         throw Error(kErrExhausted, "request does not fit in the HBM budget of the server");
-      if (lane.ws) {
+      if (buf) {
         KSJ_HIP(hipStreamSynchronize(lane.stream));
-        KSJ_HIP(hipFree(lane.ws));
-        lane.ws = nullptr;
-        lane.ws_bytes = 0;
+        KSJ_HIP(hipFree(buf));
+        buf = nullptr;
+        have = 0;
       }
-      KSJ_HIP(hipMalloc(&lane.ws, need));
-      lane.ws_bytes = need;
+      KSJ_HIP(hipMalloc(&buf, want));
+      have = want;
       stats_.peak_workspace_bytes = std::max<uint64_t>(stats_.peak_workspace_bytes, used_bytes());
+    };
+    grow(lane.ws, lane.ws_bytes, need);
+    // Host-memory requests: their k-space goes to the lane's device staging ahead of the
+    // kernels, which then read and write that staging.
+    size_t stage = 0;
+    for (const Pending& p : batch) stage += p.device_bytes();
+    grow(lane.stage, lane.stage_bytes, stage);
+    size_t off = 0;
+    for (Pending& p : batch) {
+      if (!p.req.host) continue;
+      char* d = static_cast<char*>(lane.stage) + off;
+      const void* src = p.in.direct ? p.in.user : staging_.base + p.in.at;
+      KSJ_HIP(hipMemcpyAsync(d, src, p.in.len, hipMemcpyHostToDevice, lane.stream));
+      p.req.d_kspace = reinterpret_cast<const float2*>(d);
+      p.req.d_rss = p.rss.dev ? reinterpret_cast<float*>(d + p.in.dev) : nullptr;
+      p.req.d_images = p.img.dev ? reinterpret_cast<float2*>(d + p.in.dev + p.rss.dev) : nullptr;
+      off += p.device_bytes();
     }
     if (r.kind == kServeRadial3D) {
       adjoint3_execute(*plan3, r.d_kspace, r.channels, kServeCoilsPerPass3D, r.d_images, r.d_rss, lane.ws,
@@ -359,12 +587,23 @@ bool Server::launch_next(Lane& lane) {
                         lane.stream);
       }
     }
+    for (const Pending& p : batch) {  // outputs of host-memory requests back to the host
+      if (!p.req.host) continue;
+      const std::pair<const Pending::HostBuf*, const void*> outs[2] = {{&p.rss, p.req.d_rss}, {&p.img, p.req.d_images}};
+      for (const auto& [h, d] : outs) {
+        if (h->dev == 0) continue;
+        void* dst = h->direct ? h->user : staging_.base + h->at;
+        KSJ_HIP(hipMemcpyAsync(dst, d, h->len, hipMemcpyDeviceToHost, lane.stream));
+      }
+    }
     KSJ_HIP(hipEventRecord(lane.done, lane.stream));
   } catch (const Error& e) {
+    (void)hipStreamSynchronize(lane.stream);  // copies already queued end before the staging is reused
     lane.batch = std::move(batch);
     complete(lane, status_of(e.code), e.what());
     return true;
   } catch (const std::exception& e) {
+    (void)hipStreamSynchronize(lane.stream);
     lane.batch = std::move(batch);
     complete(lane, 3, e.what());
     return true;
@@ -378,6 +617,12 @@ bool Server::launch_next(Lane& lane) {
 
 void Server::complete(Lane& lane, int status, const std::string& error) {
   for (const Pending& p : lane.batch) {
+    if (p.req.host) {
+      if (status == 0)
+        for (const Pending::HostBuf* h : {&p.rss, &p.img})
+          if (h->dev && !h->direct) copies_->copy(h->user, staging_.base + h->at, h->len);
+      if (p.slot != SIZE_MAX) release_slot(p.slot);
+    }
     done_[p.ticket] = Done{status, ms_since(p.submitted), error};
     if (status == 0)
       ++stats_.completed;

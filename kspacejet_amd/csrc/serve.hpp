@@ -1,8 +1,8 @@
 // Continuous-batching reconstruction server on one GPU.
 //
 // Requests (2-D radial KB adjoint, 3-D radial KB adjoint of a volume, or 2-D
-// Cartesian inverse FFT, inputs and outputs in HBM) are queued by submit() and run by
-// one scheduler thread:
+// Cartesian inverse FFT; inputs and outputs in HBM, or for 2-D kinds in host memory)
+// are queued by submit() and run by one scheduler thread:
 //  * admission: a request's batch is launched only when a lane (stream) is free and
 //    the HBM budget covers its workspace next to the resident plans;
 //  * plans (FP64 coordinates, bin sort, visit schedule, KB and deapodization tables)
@@ -11,7 +11,11 @@
 //    axis into one launch (up to max_batch_scans), Cartesian requests of one shape
 //    share a launch; requests arriving while batches run join the next free lane;
 //  * lanes: up to `lanes` batches are in flight on separate default-priority streams,
-//    so their kernels overlap on the device.
+//    so their kernels overlap on the device;
+//  * host memory: submit() copies a host request's k-space into a pinned staging region
+//    (blocking while the staging is full); its lane sends it to HBM ahead of the batch's
+//    kernels and its outputs back after them, so one lane's copies overlap another lane's
+//    kernels; the scheduler copies the outputs to the caller's buffers on completion.
 // Every request's result equals its result when run alone (each coil lane's
 // arithmetic is independent of what it is stacked with).
 #pragma once
@@ -19,6 +23,7 @@
 #include "plan.hpp"
 #include "plan3.hpp"
 
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstdint>
@@ -43,6 +48,7 @@ struct ServeConfig {
   uint32_t lanes = 2;             // batches in flight at once (streams)
   uint32_t max_batch_scans = 8;   // scans stacked into one radial launch
   uint32_t plan_cache_entries = 16;
+  uint64_t host_staging_bytes = 1ull << 30;  // pinned staging of host-memory requests
 };
 
 struct ServeRequest {
@@ -57,6 +63,7 @@ struct ServeRequest {
   const float2* d_kspace = nullptr;
   float2* d_images = nullptr;  // coil-last, may be null
   float* d_rss = nullptr;      // may be null
+  bool host = false;           // the four buffers above are host memory (2-D kinds)
 };
 
 struct ServeStats {
@@ -64,6 +71,30 @@ struct ServeStats {
   uint64_t batches = 0, plan_builds = 0, plan_hits = 0, plan_evictions = 0;
   uint64_t peak_workspace_bytes = 0, plan_bytes = 0;
   uint64_t max_batch_seen = 0;
+};
+
+// Parallel memcpy on a few worker threads plus the caller (copies of host requests into
+// and out of pinned staging, where one thread reaches only part of the host's copy rate).
+class CopyPool {
+ public:
+  explicit CopyPool(unsigned workers);
+  ~CopyPool();
+  void copy(void* dst, const void* src, size_t bytes);  // returns when every byte is copied
+
+ private:
+  struct Job {
+    char* dst;
+    const char* src;
+    size_t bytes, chunks;
+    std::atomic<size_t> next{0}, done{0};
+  };
+  void run(Job& j);
+  std::mutex mu_, job_mu_;
+  std::condition_variable cv_, cv_done_;
+  std::shared_ptr<Job> job_;
+  uint64_t generation_ = 0;
+  bool stop_ = false;
+  std::vector<std::thread> workers_;
 };
 
 class Server {
@@ -92,6 +123,36 @@ class Server {
     ServeRequest req;
     uint64_t plan_key;
     std::chrono::steady_clock::time_point submitted;
+    // Host-memory requests: k-space, RSS and coil images.  Each buffer has `len` bytes,
+    // `dev` bytes (len rounded to 256) in the lane's device staging, and either goes by
+    // DMA to or from the caller's buffer itself (`direct`: the caller's memory is pinned)
+    // or through offset `at` of the server's pinned staging (region `slot`).
+    struct HostBuf {
+      void* user = nullptr;
+      size_t len = 0, dev = 0, at = 0;
+      bool direct = false;
+    };
+    HostBuf in, rss, img;
+    size_t slot = SIZE_MAX;
+    size_t staged_bytes() const {
+      size_t b = 0;
+      for (const HostBuf* h : {&in, &rss, &img}) b += h->direct ? 0 : h->dev;
+      return b;
+    }
+    size_t device_bytes() const { return in.dev + rss.dev + img.dev; }
+  };
+  // First-fit allocator over one pinned host buffer (the staging of host-memory requests).
+  struct Slot {
+    size_t bytes;
+    uint32_t refs;  // requests staged in the region that have not completed
+  };
+  struct Staging {
+    std::mutex mu;  // taken after mu_ when both are held
+    std::condition_variable cv;
+    char* base = nullptr;
+    size_t bytes = 0;
+    std::map<size_t, Slot> used;  // offset -> region
+    bool stop = false;
   };
   struct Done {
     int status;
@@ -114,9 +175,13 @@ class Server {
     bool busy = false;
     std::vector<Pending> batch;
     uint64_t plan_key = 0;
+    void* stage = nullptr;  // device copies of the batch's host-memory inputs and outputs
+    size_t stage_bytes = 0;
   };
 
   Pending prepare(const ServeRequest& r);  // validates and keys a request (hashes a trajectory if needed)
+  size_t reserve_slot(size_t bytes, uint32_t refs);  // pinned staging region (blocks while full)
+  void release_slot(size_t slot);
   void run();
   bool launch_next(Lane& lane);  // under mu_; returns false when nothing could be launched
   void complete(Lane& lane, int status, const std::string& error);
@@ -137,6 +202,8 @@ class Server {
   std::map<uint64_t, PlanEntry> plans_;
   std::list<uint64_t> lru_;  // front = most recently used
   ServeStats stats_;
+  Staging staging_;
+  std::unique_ptr<CopyPool> copies_;  // parallel memcpy into and out of the staging
   uint64_t next_ticket_ = 1;
   uint64_t inflight_ = 0;
   bool stop_ = false;
@@ -147,5 +214,7 @@ class Server {
 // 64-bit hash of a device trajectory of `values` floats (FNV-1a over its bytes,
 // computed on the device and read back); synchronises `stream`.
 uint64_t trajectory_hash(const float* d_traj, uint64_t values, hipStream_t stream);
+// The same hash of a trajectory in host memory.
+uint64_t trajectory_hash_host(const float* h_traj, uint64_t values);
 
 }  // namespace ksj::gpu

@@ -24,11 +24,11 @@ def _ptr(t: torch.Tensor) -> int:
     return t.data_ptr()
 
 
-def _need_all(*specs) -> None:
+def _need_all(*specs, host: bool = False) -> None:
     """Checks caller tensors before any pointer of them reaches a kernel.  Each spec is
     (tensor, dtype, name[, shape]); every dtype, exact shape (when given) and contiguity
-    is checked first, then every device.  A wrongly sized buffer would otherwise become
-    an out-of-bounds device read or write."""
+    is checked first, then every device (GPU, or CPU when `host`).  A wrongly sized
+    buffer would otherwise become an out-of-bounds device read or write."""
     for spec in specs:
         t, dtype, name = spec[:3]
         shape = spec[3] if len(spec) > 3 else None
@@ -41,7 +41,9 @@ def _need_all(*specs) -> None:
         if not t.is_contiguous():
             raise ValueError(f"{name} must be contiguous")
     for spec in specs:
-        if not spec[0].is_cuda:
+        if host and spec[0].is_cuda:
+            raise ValueError(f"{spec[2]} must be a CPU tensor (a host-memory request keeps every buffer on the host)")
+        if not host and not spec[0].is_cuda:
             raise ValueError(f"{spec[2]} must be a GPU tensor")
 
 
@@ -262,15 +264,18 @@ class Plan2:
 
 
 class Server:
-    """Continuous-batching server (ksj_gpu_serve_*): submit device-resident scans,
-    wait for tickets.  Buffers passed to submit() must stay alive until their
-    ticket completes; the server keeps references to them until then."""
+    """Continuous-batching server (ksj_gpu_serve_*): submit scans, wait for tickets.
+    A 2-D request whose tensors are all on the CPU is a host-memory request (staged
+    through the server's pinned staging of `host_staging_mib` MiB, 0 = 1024; submit
+    blocks while it is full); otherwise every tensor must be on the GPU.  Buffers passed
+    to submit() must stay alive until their ticket completes; the server keeps
+    references to them until then."""
 
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, lanes: int = 2, max_batch_scans: int = 8,
-                 plan_cache_entries: int = 16):
+                 plan_cache_entries: int = 16, host_staging_mib: int = 0):
         self._lib = load()
         cfg = _lib.ServeConfig(C.sizeof(_lib.ServeConfig), device, hbm_budget_bytes, lanes, max_batch_scans,
-                               plan_cache_entries, 0)
+                               plan_cache_entries, host_staging_mib)
         self._handle = C.c_void_p()
         call("ksj_gpu_serve_create", C.byref(cfg), C.byref(self._handle))
         self._live = {}
@@ -281,13 +286,15 @@ class Server:
         if kspace.dim() != 2:
             raise ValueError("kspace must be [C, S]")
         ch, S = kspace.shape
+        host = not kspace.is_cuda
         self._check_request((kspace, torch.complex64, "kspace"), (traj, torch.float32, "traj", (S, 2)),
-                            images, (n, n, ch), rss, (n, n))
+                            images, (n, n, ch), rss, (n, n), host=host)
         r = _lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_RADIAL2D, n, n, ch, S, width,
                               _lib.DENSITY_RAMP if density == "ramp" else _lib.DENSITY_NONE, 0, 0, trajectory_key,
                               _ptr(traj), _ptr(kspace), _ptr(images) if images is not None else None,
-                              _ptr(rss) if rss is not None else None)
-        return self._submit(r, (kspace, traj, images, rss))
+                              _ptr(rss) if rss is not None else None,
+                              _lib.SERVE_HOST_MEMORY if host else _lib.SERVE_DEVICE_MEMORY, 0)
+        return self._submit(r, (kspace, traj, images, rss), host)
 
     def submit_radial3d(self, kspace: torch.Tensor, traj: torch.Tensor, shape, images: torch.Tensor | None = None,
                         rss: torch.Tensor | None = None, width: int = 4, density: str = "ramp_squared",
@@ -304,21 +311,24 @@ class Server:
         r = _lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_RADIAL3D, ny, nx, ch, S, width, dens, 0, nz,
                               trajectory_key, _ptr(traj), _ptr(kspace), _ptr(images) if images is not None else None,
                               _ptr(rss) if rss is not None else None)
-        return self._submit(r, (kspace, traj, images, rss))
+        return self._submit(r, (kspace, traj, images, rss), False)
 
     def submit_cartesian(self, kspace: torch.Tensor, images: torch.Tensor | None = None,
                          rss: torch.Tensor | None = None, centred: bool = False) -> int:
         if kspace.dim() != 3:
             raise ValueError("kspace must be [C, R, Cols]")
         ch, rows, cols = kspace.shape
-        self._check_request((kspace, torch.complex64, "kspace"), None, images, (rows, cols, ch), rss, (rows, cols))
+        host = not kspace.is_cuda
+        self._check_request((kspace, torch.complex64, "kspace"), None, images, (rows, cols, ch), rss, (rows, cols),
+                            host=host)
         r = _lib.ServeRequest(C.sizeof(_lib.ServeRequest), _lib.SERVE_CARTESIAN2D, rows, cols, ch, 0, 0, 0,
                               int(centred), 0, 0, None, _ptr(kspace), _ptr(images) if images is not None else None,
-                              _ptr(rss) if rss is not None else None)
-        return self._submit(r, (kspace, images, rss))
+                              _ptr(rss) if rss is not None else None,
+                              _lib.SERVE_HOST_MEMORY if host else _lib.SERVE_DEVICE_MEMORY, 0)
+        return self._submit(r, (kspace, images, rss), host)
 
     @staticmethod
-    def _check_request(kspace_spec, traj_spec, images, image_shape, rss, rss_shape) -> None:
+    def _check_request(kspace_spec, traj_spec, images, image_shape, rss, rss_shape, host: bool = False) -> None:
         if images is None and rss is None:
             raise ValueError("a request needs images and/or rss")
         specs = [kspace_spec] + ([traj_spec] if traj_spec is not None else [])
@@ -326,11 +336,12 @@ class Server:
             specs.append((images, torch.complex64, "images", image_shape))
         if rss is not None:
             specs.append((rss, torch.float32, "rss", rss_shape))
-        _need_all(*specs)
+        _need_all(*specs, host=host)
 
-    def _submit(self, req, keep) -> int:
+    def _submit(self, req, keep, host: bool) -> int:
         # Work the caller queued on torch's stream (e.g. filling k-space) must land first.
-        torch.cuda.current_stream().synchronize()
+        if not host:
+            torch.cuda.current_stream().synchronize()
         t = C.c_uint64(0)
         call("ksj_gpu_serve_submit", self._handle, C.byref(req), C.byref(t))
         self._live[t.value] = keep

@@ -15,9 +15,11 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _ORACLE_SO = os.path.join(_HERE, "libksj_oracle.so")
 _REF_SO = os.path.join(_HERE, "_ref", "libksj_ref.so")
+_ROUTE_SO = os.path.join(_HERE, "libksj_kb_route.so")
 
 _lib = None
 _ref = None
+_route = None
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
@@ -282,6 +284,33 @@ def ref_radial_route_rss(values_f32, traj_f32, rows: int, cols: int, scans: int 
     if rc != 0:
         raise ValueError("reference route rejected its input")
     return out
+
+
+def kb_route_rss_f32(values_f32, traj_f32, n: int, width: int = 4, scans: int = 1, threads: int = 1,
+                     timed: bool = False):
+    """FP32 CPU restatement of the GPU radial route (ramp DCF, KB width `width` with the
+    Beatty beta on the 2n x 2n grid, radix-2 inverse FFT, deapodization, crop, RSS), the
+    like-for-like CPU baseline; the scan is reconstructed `scans` times on `threads`
+    threads.  values [C, S] complex64, traj [S, 2] float32 rad/px -> rss [n, n] float32
+    (and, when `timed`, the wall seconds of the scans after the per-trajectory tables)."""
+    global _route
+    if _route is None:
+        if not os.path.exists(_ROUTE_SO):
+            subprocess.run(["make", "-s", "-C", _HERE, "all"], check=True)
+        R = C.CDLL(_ROUTE_SO)
+        R.ora_kb_route_rss_f32.argtypes = [_sz, _sz, _sz, _fp, _fp, C.c_int, _sz, C.c_int, _fp,
+                                           C.POINTER(C.c_double)]
+        R.ora_kb_route_rss_f32.restype = C.c_int
+        _route = R
+    v = np.ascontiguousarray(values_f32, np.complex64)
+    t = np.ascontiguousarray(traj_f32, np.float32).reshape(-1, 2)
+    out = np.zeros((n, n), np.float32)
+    secs = C.c_double(0)
+    rc = _route.ora_kb_route_rss_f32(n, t.shape[0], v.shape[0], t.reshape(-1), v.view(np.float32).reshape(-1), width,
+                                     scans, threads, out.reshape(-1), C.byref(secs))
+    if rc != 0:
+        raise ValueError("CPU route rejected its input")
+    return (out, secs.value) if timed else out
 
 
 # ---------------------------------------------------------------- 3-D (kooshball)

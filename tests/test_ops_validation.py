@@ -1,7 +1,9 @@
 """Caller tensors are checked before any pointer reaches a kernel (CPU only, no GPU):
 a wrongly shaped trajectory, image or RSS buffer raises ValueError naming the shape,
-and a correctly shaped CPU tensor still fails as "must be a GPU tensor" -- so the
-library is never called with a buffer that does not match the launch."""
+and a correctly shaped CPU tensor still fails as "must be a GPU tensor" where only
+device memory is served -- so the library is never called with a buffer that does not
+match the launch.  A 2-D server request whose buffers are all on the CPU is a
+host-memory request; one that mixes the two sides is refused."""
 import types
 
 import pytest
@@ -47,10 +49,22 @@ def test_submit_radial_checks_every_buffer(bad):
         server().submit_radial(**args)
 
 
-def test_submit_radial_with_matching_cpu_buffers_fails_on_the_device_only():
+def captured(monkeypatch):
+    got = {}
+    monkeypatch.setattr(ops.Server, "_submit", lambda self, req, keep, host: got.update(req=req, host=host) or 1)
+    return got
+
+
+def test_matching_cpu_buffers_form_a_host_memory_request(monkeypatch):
+    from kspacejet_amd import _lib
     n, ch, S = 16, 3, 200
-    with pytest.raises(ValueError, match="GPU tensor"):
-        server().submit_radial(z((ch, S), c64), z((S, 2), f32), n, images=z((n, n, ch), c64), rss=z((n, n), f32))
+    got = captured(monkeypatch)
+    server().submit_radial(z((ch, S), c64), z((S, 2), f32), n, images=z((n, n, ch), c64), rss=z((n, n), f32))
+    assert got["host"] and got["req"].memory == _lib.SERVE_HOST_MEMORY
+    server().submit_cartesian(z((ch, 8, 6), c64), rss=z((8, 6), f32))
+    assert got["host"] and got["req"].memory == _lib.SERVE_HOST_MEMORY and got["req"].kind == _lib.SERVE_CARTESIAN2D
+    with pytest.raises(ValueError, match="GPU tensor"):  # 3-D volumes are served from HBM only
+        server().submit_radial3d(z((2, 50), c64), z((50, 3), f32), (4, 5, 6), rss=z((4, 5, 6), f32))
 
 
 @pytest.mark.parametrize("bad", ["traj", "images", "rss"])
@@ -98,3 +112,28 @@ def test_phase_correct_and_crop_check_their_shapes():
         ops.phase_correct(z((2, 3, 4), c64), z((2, 5), c64))
     with pytest.raises(ValueError, match="inside the readout"):
         ops.readout_crop(z((2, 3, 8), c64), 6, 3)
+
+
+class _OnGpu(torch.Tensor):
+    """A CPU tensor that reports itself as a GPU tensor (only the device check reads it)."""
+    @property
+    def is_cuda(self):
+        return True
+
+
+def on_gpu(t):
+    return torch.Tensor._make_subclass(_OnGpu, t)
+
+
+def test_a_host_memory_request_keeps_every_buffer_on_the_host(monkeypatch):
+    """A 2-D request with CPU k-space is a host-memory request: a GPU buffer in it is
+    refused before the library sees a pointer, and a GPU request refuses CPU buffers."""
+    captured(monkeypatch)
+    with pytest.raises(ValueError, match="CPU tensor"):
+        server().submit_radial(z((4, 100), c64), z((100, 2), f32), 8, rss=on_gpu(z((8, 8), f32)))
+    with pytest.raises(ValueError, match="CPU tensor"):
+        server().submit_cartesian(z((4, 8, 8), c64), images=on_gpu(z((8, 8, 4), c64)), rss=z((8, 8), f32))
+    with pytest.raises(ValueError, match="GPU tensor"):
+        server().submit_cartesian(on_gpu(z((4, 8, 8), c64)), rss=z((8, 8), f32))
+    with pytest.raises(ValueError, match="shape"):
+        server().submit_cartesian(z((4, 8, 8), c64), rss=z((8, 9), f32))
