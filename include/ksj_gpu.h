@@ -515,6 +515,10 @@ void ksj_gpu_serve_destroy(ksj_gpu_server* server);
  *   "matrix_weights"       1 = new 2-D matrix plans store a 4-byte (sample, window bin) visit and the
  *                          gather forms the cell weights from the sample's taps (default), 0 = every
  *                          visit stores its 16 or 32 cell weights (same bits; plan size and speed only)
+ *   "fft_register"         1 = FFT passes whose loads and stores both run across contiguous lines keep
+ *                          the first and last radix pass in registers, and 2-D radial coil-last
+ *                          outputs take one column pass that writes the coil images and RSS itself
+ *                          (default); 0 = every pass staged through LDS (same bits)
  *   "serve_copy_threads"   worker threads (0..15, default 7) a server created afterwards adds to the
  *                          submitting thread for memcpy of host-memory requests into and out of its pinned
  *                          staging (copies of 4 MiB or more; same bits)

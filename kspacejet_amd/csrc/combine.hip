@@ -28,7 +28,7 @@ __global__ __launch_bounds__(256) void rss_kernel(const float2* __restrict__ in,
   float acc = 0.f;
   for (uint32_t c = 0; c < coils; ++c) {
     const float2 v = tile[threadIdx.x * stride + c];
-    acc += v.x * v.x + v.y * v.y;
+    acc += norm2(v);
   }
   out[p] = sqrtf(acc);
 }
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void major_to_last_kernel(const float2* __rest
     float acc = 0.f;
     for (uint32_t c = 0; c < coils; ++c) {
       const float2 v = tile[threadIdx.x * stride + c];
-      acc += v.x * v.x + v.y * v.y;
+      acc += norm2(v);
     }
     rss[p0 + threadIdx.x] = sqrtf(acc);
   }
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void major_split_kernel(const float2* __restri
     float acc = 0.f;
     for (uint32_t c = 0; c < out.coils; ++c) {
       const float2 v = tile[pl * stride + g * out.coils + c];
-      acc += v.x * v.x + v.y * v.y;
+      acc += norm2(v);
     }
     out.rss[g][p0 + pl] = sqrtf(acc);
   }
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void split_fused_kernel(const FusedTable<Split
       float acc = 0.f;
       for (uint32_t c = 0; c < a.coils; ++c) {
         const float2 v = tile[pl * stride + c];
-        acc += v.x * v.x + v.y * v.y;
+        acc += norm2(v);
       }
       a.rss[p0 + pl] = sqrtf(acc);
     }

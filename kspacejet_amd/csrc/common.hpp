@@ -21,6 +21,8 @@ __host__ __device__ inline float2 cmul(float2 a, float2 b) { return make_float2(
 __host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// |a|^2 with one explicit fma, so every kernel that sums it rounds the same way.
+__host__ __device__ inline float norm2(float2 a) { return fmaf(a.x, a.x, a.y * a.y); }
 __host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 
 // Thrown by host code on a HIP error or a violated precondition; the C ABI
@@ -95,6 +97,8 @@ enum TuneKey : int {
                           // their XCD's runs (same bits)
   kTuneMatrixTapShare,    // 2-D matrix gather, packed visits: 1 = one tap load per lane shared by bpermute (same bits)
   kTuneServeCopyThreads,  // servers created from now on: memcpy worker threads (beside the caller) for host requests
+  kTuneFftRegister,       // 1 = register-staged FFT where loads and stores both run across lines (same bits),
+                          // 0 = the LDS-staged form everywhere
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);

@@ -12,6 +12,7 @@
 #include "fused.hpp"
 
 #include <map>
+#include <type_traits>
 #include <vector>
 #include <mutex>
 #include <utility>
@@ -84,6 +85,13 @@ constexpr int line_stride() {
   return N + N / 16 + 1;
 }
 
+// Products of the form a b +- c d are written as one explicit fma and one multiply, so
+// the compiler's contraction of the remaining adds has one choice only and every form
+// of a pass (staged or register) rounds identically.
+__device__ inline float2 tmul(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
+}
+
 __device__ inline float2 mul_i(float2 z, bool inv) {  // z * (+i) for inverse, z * (-i) for forward
   return inv ? make_float2(-z.y, z.x) : make_float2(z.y, -z.x);
 }
@@ -118,7 +126,7 @@ template <bool INV> struct Dft<8, INV> {
     Dft<4, INV>::run(o);
     const float s = INV ? 1.0f : -1.0f;
     const float2 w1 = make_float2(h, s * h), w3 = make_float2(-h, s * h);
-    const float2 t0 = o[0], t1 = cmul(o[1], w1), t2 = mul_i(o[2], INV), t3 = cmul(o[3], w3);
+    const float2 t0 = o[0], t1 = tmul(o[1], w1), t2 = mul_i(o[2], INV), t3 = tmul(o[3], w3);
     v[0] = cadd(e[0], t0);
     v[4] = csub(e[0], t0);
     v[1] = cadd(e[1], t1);
@@ -134,11 +142,13 @@ template <bool INV> struct Dft<3, INV> {
   __device__ static inline void run(float2* v) {
     constexpr float r3 = 0.86602540378443864676f;
     const float2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-    const float2 m = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
-    const float2 n = mul_i(cscale(t2, r3), INV);
+    const float2 m = make_float2(fmaf(-0.5f, t1.x, v[0].x), fmaf(-0.5f, t1.y, v[0].y));
+    // m +- i r3 t2 (inverse) or m -+ i r3 t2 (forward), each component one fma: the
+    // product r3 t2 is not shared between the two outputs.
+    const float s = INV ? r3 : -r3;
     v[0] = cadd(v[0], t1);
-    v[1] = cadd(m, n);
-    v[2] = csub(m, n);
+    v[1] = make_float2(fmaf(-s, t2.y, m.x), fmaf(s, t2.x, m.y));
+    v[2] = make_float2(fmaf(s, t2.y, m.x), fmaf(-s, t2.x, m.y));
   }
 };
 
@@ -148,10 +158,10 @@ template <bool INV> struct Dft<5, INV> {
     constexpr float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
     const float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
     const float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-    const float2 a1 = make_float2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
-    const float2 a2 = make_float2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
-    const float2 b1 = mul_i(make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y), INV);
-    const float2 b2 = mul_i(make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y), INV);
+    const float2 a1 = make_float2(fmaf(c2, t2.x, fmaf(c1, t1.x, v[0].x)), fmaf(c2, t2.y, fmaf(c1, t1.y, v[0].y)));
+    const float2 a2 = make_float2(fmaf(c1, t2.x, fmaf(c2, t1.x, v[0].x)), fmaf(c1, t2.y, fmaf(c2, t1.y, v[0].y)));
+    const float2 b1 = mul_i(make_float2(fmaf(s1, t3.x, s2 * t4.x), fmaf(s1, t3.y, s2 * t4.y)), INV);
+    const float2 b2 = mul_i(make_float2(fmaf(s2, t3.x, -(s1 * t4.x)), fmaf(s2, t3.y, -(s1 * t4.y))), INV);
     v[0] = cadd(v[0], cadd(t1, t2));
     v[1] = cadd(a1, b1);
     v[4] = csub(a1, b1);
@@ -179,7 +189,7 @@ __device__ inline void passes(float2* line, const float2* tw, int t) {
         for (int r = 1; r < R; ++r) {
           float2 w = tw[k * r * (N / (Ns * R))];
           if (!INV) w.y = -w.y;
-          v[b][r] = cmul(v[b][r], w);
+          v[b][r] = tmul(v[b][r], w);
         }
       }
       Dft<R, INV>::run(v[b]);
@@ -307,6 +317,218 @@ __device__ inline void fft_lds_block(const FftArgs& a, const float2* __restrict_
   store_lines<N, V>(a, buf, LS, lpb_log2, ow, lw, dst_base);
 }
 
+// ----------------------------------------------------------------------------- register-staged form
+// For launches whose loads and stores both take the same element of consecutive lines
+// (lines contiguous in memory), thread (ly, t) = (tid % lpb, tid / lpb) holds elements
+// t + i T (i < N / T) of line ly.  Those are exactly the inputs of its butterflies in
+// the first Stockham pass and the outputs of its butterflies in the last, so the first
+// pass runs on the loaded registers and the last pass's results go straight to the
+// stores: two LDS round trips fewer than the staged form, and the same arithmetic
+// (same butterflies, twiddles and order), so the same bits.  LDS holds the lines
+// element-major, element e of line ly at e * (lpb + 1) + ly, so a wave's lanes (lines)
+// touch consecutive words.
+
+template <int N, int V, int P> struct StageNs {
+  static constexpr int value = StageNs<N, V, P - 1>::value * Shape<N, V>::R[P - 1];
+};
+template <int N, int V> struct StageNs<N, V, 0> {
+  static constexpr int value = 1;
+};
+
+// The middle passes (1 .. P-2) between the register first and last passes.
+template <int N, int V, bool INV, int P>
+__device__ inline void reg_mid_passes(float2* buf, int S, int ly, const float2* tw, int t) {
+  if constexpr (P < Shape<N, V>::P - 1) {
+    constexpr int T = Shape<N, V>::T;
+    constexpr int R = Shape<N, V>::R[P];
+    constexpr int B = N / R / T;
+    constexpr int Ns = StageNs<N, V, P>::value;
+    float2 v[B][R];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int j = t + b * T;
+      const int k = j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = buf[(j + r * (N / R)) * S + ly];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[k * r * (N / (Ns * R))];
+        if (!INV) w.y = -w.y;
+        v[b][r] = tmul(v[b][r], w);
+      }
+      Dft<R, INV>::run(v[b]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int j = t + b * T;
+      const int k = j % Ns;
+      const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(base + r * Ns) * S + ly] = v[b][r];
+    }
+    __syncthreads();
+    reg_mid_passes<N, V, INV, P + 1>(buf, S, ly, tw, t);
+  }
+}
+
+// A thread's elements of its line in the register form.
+template <int N, int V> struct RegLine {
+  float2 v[N / Shape<N, V>::T];
+};
+
+// x.v[i] = element t + i T of the line on entry and bin t + i T of its transform on exit.
+template <int N, int V, bool INV>
+__device__ inline void reg_transform(RegLine<N, V>& x, float2* buf, int S, int ly, const float2* tw, int t) {
+  constexpr int T = Shape<N, V>::T;
+  constexpr int PL = Shape<N, V>::P - 1;
+  static_assert(PL >= 1, "the register form needs at least two passes");
+  {  // first pass: inputs j + r N / R0 = element t + T (b + r B0), no twiddles
+    constexpr int R = Shape<N, V>::R[0];
+    constexpr int B = N / R / T;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      float2 v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = x.v[b + r * B];
+      Dft<R, INV>::run(v);
+      const int j = t + b * T;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(j * R + r) * S + ly] = v[r];
+    }
+    __syncthreads();
+  }
+  reg_mid_passes<N, V, INV, 1>(buf, S, ly, tw, t);
+  {  // last pass: outputs j + r Ns (j < Ns) = bin t + T (b + r B)
+    constexpr int R = Shape<N, V>::R[PL];
+    constexpr int B = N / R / T;
+    constexpr int Ns = StageNs<N, V, PL>::value;
+    static_assert(Ns * R == N, "the last pass must close the transform");
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int j = t + b * T;
+      float2 v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = buf[(j + r * (N / R)) * S + ly];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[j * r * (N / (Ns * R))];
+        if (!INV) w.y = -w.y;
+        v[r] = tmul(v[r], w);
+      }
+      Dft<R, INV>::run(v);
+#pragma unroll
+      for (int r = 0; r < R; ++r) x.v[b + r * B] = v[r];
+    }
+  }
+}
+
+template <int N>
+__host__ __device__ constexpr int reg_lds_elems(int lpb) {
+  return N * (lpb + 1);
+}
+
+// Logical block of a launch of `blocks` blocks: consecutive logical blocks on one XCD
+// (hardware places block b on XCD b % 8), so lines that share output cache lines meet
+// in one L2.
+__device__ inline uint64_t xcd_block(uint64_t blocks) {
+  const uint64_t b = blockIdx.x;
+  if (blocks % 8 != 0) return b;
+  return (b % 8) * (blocks / 8) + b / 8;
+}
+
+template <int N, int V, bool INV, bool EPI>
+__global__ __launch_bounds__(1024) void fft_reg_kernel(FftArgs a, ColumnOutputs co, const float2* __restrict__ tw_global,
+                                                       int lpb_log2) {
+  static_assert(shape_ok<N, V>(), "radix schedule must multiply to N and split evenly over the line's threads");
+  constexpr int T = Shape<N, V>::T;
+  constexpr int E = N / T;
+  extern __shared__ float2 smem[];
+  __shared__ uint64_t src_base[64], dst_base[64];
+  __shared__ float ow[N], lw[64];
+  const int lpb = 1 << lpb_log2;
+  const int S = lpb + 1;
+  float2* tw = smem;
+  float2* buf = smem + N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    tw[i] = tw_global[i];
+    ow[i] = (i < static_cast<int>(a.n_out) && a.out_weight) ? a.scale * a.out_weight[i] : a.scale;
+  }
+  const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
+  const uint64_t first = xcd_block(gridDim.x) * lpb;
+  for (int l = threadIdx.x; l < lpb; l += blockDim.x) {
+    const uint64_t line = first + l;
+    src_base[l] = line < total ? line_base(a.src, line) : ~0ull;
+    dst_base[l] = line < total ? line_base(a.dst, line) : ~0ull;
+    const uint64_t wi = a.line_weight_by_plane ? line / a.src.lines : line % a.src.lines;
+    lw[l] = (a.line_weight && line < total) ? a.line_weight[wi] : 1.0f;
+  }
+  __syncthreads();
+  const int ly = threadIdx.x & (lpb - 1), t = threadIdx.x >> lpb_log2;
+  RegLine<N, V> x;
+  {
+    const uint64_t base = src_base[ly];
+    const uint64_t b0 = base != ~0ull ? base : 0ull;  // a line past the end transforms line 0; never stored
+    const uint32_t es = static_cast<uint32_t>(a.src.elem_stride);
+    typedef float F2v __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      uint32_t src = static_cast<uint32_t>(t + i * T) + a.in_offset;
+      if (src >= static_cast<uint32_t>(N)) src -= N;
+      const F2v v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(a.in + b0 + static_cast<uint64_t>(src * es)));
+      x.v[i] = make_float2(v.x, v.y);
+    }
+  }
+  reg_transform<N, V, INV>(x, buf, S, ly, tw, t);
+  const uint32_t nout = a.n_out;
+  if constexpr (!EPI) {
+    const uint64_t base = dst_base[ly];
+    if (base == ~0ull) return;
+    const uint32_t es = static_cast<uint32_t>(a.dst.elem_stride);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const uint32_t bin = static_cast<uint32_t>(t + i * T);
+      const uint32_t e = bin >= a.out_offset ? bin - a.out_offset : bin + N - a.out_offset;
+      if (e < nout) a.out[base + static_cast<uint64_t>(e * es)] = cscale(x.v[i], ow[e] * lw[ly]);
+    }
+  } else {
+    // Column epilogue: line = (column c = plane, stacked coil k); group g = k / channels
+    // writes coil-last images [N_r][cols][channels] and/or RSS [N_r][cols], the RSS
+    // summed in coil order from |x|^2 staged in LDS (every group lies in one block).
+    const uint64_t line = first + ly;
+    const bool live = line < total;
+    const uint32_t c = live ? static_cast<uint32_t>(line / a.src.lines) : 0u;
+    const uint32_t k = live ? static_cast<uint32_t>(line % a.src.lines) : 0u;
+    const uint32_t g = k / co.channels, kk = k - g * co.channels;
+    float2* img = live ? co.coil_last[g] : nullptr;
+    float* pw = reinterpret_cast<float*>(buf);
+    __syncthreads();  // every thread's last-pass reads are done before buf is reused
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const uint32_t bin = static_cast<uint32_t>(t + i * T);
+      const uint32_t e = bin >= a.out_offset ? bin - a.out_offset : bin + N - a.out_offset;
+      if (e >= nout) continue;
+      const float2 v = cscale(x.v[i], ow[e] * lw[ly]);
+      if (img) img[(static_cast<uint64_t>(e) * co.cols + c) * co.channels + kk] = v;
+      pw[e * lpb + ly] = norm2(v);
+    }
+    __syncthreads();
+    const uint32_t gpb = static_cast<uint32_t>(lpb) / co.channels;  // groups per block
+    for (uint32_t q = threadIdx.x; q < nout * gpb; q += blockDim.x) {
+      const uint32_t e = q / gpb, gb = q - e * gpb;
+      const uint64_t l0 = first + gb * co.channels;
+      if (l0 >= total) continue;
+      const uint32_t col = static_cast<uint32_t>(l0 / a.src.lines);
+      const uint32_t grp = static_cast<uint32_t>(l0 % a.src.lines) / co.channels;
+      float* rss = co.rss[grp];
+      if (!rss) continue;
+      float acc = 0.f;
+      for (uint32_t m = 0; m < co.channels; ++m) acc += pw[e * lpb + gb * co.channels + m];
+      rss[static_cast<uint64_t>(e) * co.cols + col] = sqrtf(acc);
+    }
+  }
+}
+
 template <int N, int V, bool INV>
 __global__ __launch_bounds__(1024) void fft_lds_kernel(FftArgs a, const float2* __restrict__ tw_global, int lpb_log2) {
   fft_lds_block<N, V, INV>(a, tw_global, lpb_log2, blockIdx.x);
@@ -415,22 +637,68 @@ size_t fft_shmem(int lpb_log2) {
 }
 
 template <int N, int V>
+size_t reg_shmem(int lpb_log2) {
+  return (static_cast<size_t>(N) + reg_lds_elems<N>(1 << lpb_log2)) * sizeof(float2);
+}
+
+template <typename K>
+void raise_lds(K kernel, size_t shmem) {
+  if (shmem > 65536)
+    KSJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(shmem)));
+}
+
+// The register-staged form: both mappings across lines, a schedule of two or more passes.
+template <int N, int V>
+constexpr bool reg_form_compiled() {
+  return Shape<N, V>::P >= 2;
+}
+
+template <int N, int V>
+bool reg_form(const FftArgs& a) {
+  return reg_form_compiled<N, V>() && a.load_lines_fast && a.store_lines_fast && tune_get(kTuneFftRegister) == 1;
+}
+
+template <int N, int V, bool EPI>
+void launch_reg(const FftArgs& a, const ColumnOutputs& co, const float2* tw, hipStream_t stream, int lpb) {
+  constexpr int T = Shape<N, V>::T;
+  int lpb_log2 = lines_per_block_log2<N, V>(a, lpb);
+  // LDS of the element-major layout (N (lpb + 1) next to the twiddles and tables).
+  constexpr int kStatic = 64 * 2 * 8 + N * 4 + 64 * 4;
+  while (lpb_log2 > 0 && static_cast<int>(reg_shmem<N, V>(lpb_log2)) + kStatic > max_dynamic_lds()) --lpb_log2;
+  const int lines = 1 << lpb_log2;
+  if constexpr (EPI) require(lines % co.channels == 0, "column epilogue needs whole coil groups per block");
+  const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
+  const size_t shmem = reg_shmem<N, V>(lpb_log2);
+  const unsigned grid = ceil_div(total, static_cast<uint64_t>(lines));
+  if (a.inverse) {
+    raise_lds(&fft_reg_kernel<N, V, true, EPI>, shmem);
+    hipLaunchKernelGGL((fft_reg_kernel<N, V, true, EPI>), dim3(grid), dim3(lines * T), shmem, stream, a, co, tw,
+                       lpb_log2);
+  } else {
+    raise_lds(&fft_reg_kernel<N, V, false, EPI>, shmem);
+    hipLaunchKernelGGL((fft_reg_kernel<N, V, false, EPI>), dim3(grid), dim3(lines * T), shmem, stream, a, co, tw,
+                       lpb_log2);
+  }
+  KSJ_LAUNCH_CHECK();
+}
+
+template <int N, int V>
 void launch_fast(const FftArgs& a, const float2* tw, hipStream_t stream, int lpb) {
   constexpr int T = Shape<N, V>::T;
+  if constexpr (reg_form_compiled<N, V>()) {
+    if (reg_form<N, V>(a)) return launch_reg<N, V, false>(a, ColumnOutputs{}, tw, stream, lpb);
+  }
   const int lpb_log2 = lines_per_block_log2<N, V>(a, lpb);
   lpb = 1 << lpb_log2;
   const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
   const size_t shmem = fft_shmem<N, V>(lpb_log2);
   const unsigned grid = ceil_div(total, static_cast<uint64_t>(lpb));
   if (a.inverse) {
-    if (shmem > 65536)
-      KSJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_lds_kernel<N, V, true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+    raise_lds(&fft_lds_kernel<N, V, true>, shmem);
     hipLaunchKernelGGL((fft_lds_kernel<N, V, true>), dim3(grid), dim3(lpb * T), shmem, stream, a, tw, lpb_log2);
   } else {
-    if (shmem > 65536)
-      KSJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_lds_kernel<N, V, false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+    raise_lds(&fft_lds_kernel<N, V, false>, shmem);
     hipLaunchKernelGGL((fft_lds_kernel<N, V, false>), dim3(grid), dim3(lpb * T), shmem, stream, a, tw, lpb_log2);
   }
   KSJ_LAUNCH_CHECK();
@@ -554,6 +822,72 @@ void fft_launch(const FftArgs& a, hipStream_t stream, int lpb) {
   require(shmem <= 160 * 1024, "direct DFT line does not fit in LDS");
   hipLaunchKernelGGL(dft_direct_kernel<0>, dim3(ceil_div(total, l)), dim3(256), shmem, stream, a, tw, l);
   KSJ_LAUNCH_CHECK();
+}
+
+namespace {
+
+template <int N>
+int columns_lines_per_block(const FftArgs& a) {
+  const int v = fft_schedule<N>();
+  auto lines = [&](auto vv) {
+    constexpr int V = decltype(vv)::value;
+    if constexpr (!reg_form_compiled<N, V>()) {
+      return 0;
+    } else {
+      int lpb_log2 = lines_per_block_log2<N, V>(a, 0);
+      constexpr int kStatic = 64 * 2 * 8 + N * 4 + 64 * 4;
+      while (lpb_log2 > 0 && static_cast<int>(reg_shmem<N, V>(lpb_log2)) + kStatic > max_dynamic_lds()) --lpb_log2;
+      return 1 << lpb_log2;
+    }
+  };
+  switch (v) {
+    case 1: return lines(std::integral_constant<int, 1>{});
+    case 2: return lines(std::integral_constant<int, 2>{});
+    default: return lines(std::integral_constant<int, 0>{});
+  }
+}
+
+template <int N>
+void launch_columns(const FftArgs& a, const ColumnOutputs& co, const float2* tw, hipStream_t stream) {
+  switch (fft_schedule<N>()) {
+    case 1: return launch_reg<N, 1, true>(a, co, tw, stream, 0);
+    case 2: return launch_reg<N, 2, true>(a, co, tw, stream, 0);
+    default: return launch_reg<N, 0, true>(a, co, tw, stream, 0);
+  }
+}
+
+}  // namespace
+
+bool fft_columns_supported(const FftArgs& a, uint32_t channels) {
+  if (!a.load_lines_fast || !a.store_lines_fast || channels == 0 || tune_get(kTuneFftRegister) != 1) return false;
+  if (a.src.lines % channels != 0) return false;
+  int lines = 0;
+  switch (a.n) {
+    case 256: lines = columns_lines_per_block<256>(a); break;
+    case 320: lines = columns_lines_per_block<320>(a); break;
+    case 384: lines = columns_lines_per_block<384>(a); break;
+    case 512: lines = columns_lines_per_block<512>(a); break;
+    case 640: lines = columns_lines_per_block<640>(a); break;
+    case 768: lines = columns_lines_per_block<768>(a); break;
+    case 1024: lines = columns_lines_per_block<1024>(a); break;
+    default: return false;
+  }
+  return lines > 0 && lines % channels == 0;
+}
+
+void fft_launch_columns(const FftArgs& a, const ColumnOutputs& out, hipStream_t stream) {
+  require(fft_columns_supported(a, out.channels), "column epilogue not available for this pass");
+  require(a.n_out >= 1 && a.n_out <= a.n && a.in_offset < a.n && a.out_offset < a.n, "invalid column pass");
+  const float2* tw = fft_twiddles(a.n, stream);
+  switch (a.n) {
+    case 256: return launch_columns<256>(a, out, tw, stream);
+    case 320: return launch_columns<320>(a, out, tw, stream);
+    case 384: return launch_columns<384>(a, out, tw, stream);
+    case 512: return launch_columns<512>(a, out, tw, stream);
+    case 640: return launch_columns<640>(a, out, tw, stream);
+    case 768: return launch_columns<768>(a, out, tw, stream);
+    default: return launch_columns<1024>(a, out, tw, stream);
+  }
 }
 
 size_t fft_fused_table_bytes(uint32_t count) { return static_cast<size_t>(count) * fused_table_bytes<FftArgs>(); }

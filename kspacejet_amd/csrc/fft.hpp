@@ -36,6 +36,19 @@ struct FftArgs {
   // (1).  Pick the one whose addresses are contiguous in memory.
   int load_lines_fast = 0;
   int store_lines_fast = 0;
+  // line_weight index: line % lines (0) or the line's plane (1).
+  int line_weight_by_plane = 0;
+};
+
+// Outputs of a column pass whose lines are (plane = image column c, line = stacked coil
+// k), k in [0, groups * channels): group g = k / channels writes coil-last images
+// coil_last[g] [n_out][cols][channels] and/or its RSS image rss[g] [n_out][cols]
+// (sqrt of the sum of |x|^2 in coil order); null pointers skip.
+struct ColumnOutputs {
+  float2* coil_last[64] = {};
+  float* rss[64] = {};
+  uint32_t channels = 1;
+  uint32_t cols = 0;
 };
 
 // Returns true when an LDS Stockham kernel is compiled for length n; other
@@ -47,6 +60,12 @@ bool fft_fast_length(uint32_t n);
 const float2* fft_twiddles(uint32_t n, hipStream_t stream);
 
 void fft_launch(const FftArgs& args, hipStream_t stream, int lines_per_block = 0);
+
+// True when fft_launch_columns can take this pass: an LDS length, both mappings across
+// lines, and coil groups that fit whole in the launch's blocks.
+bool fft_columns_supported(const FftArgs& args, uint32_t channels);
+// The pass of `args` (dst unused) with the ColumnOutputs epilogue in place of its stores.
+void fft_launch_columns(const FftArgs& args, const ColumnOutputs& out, hipStream_t stream);
 
 // Transforms of up to 64 items of an uneven batch (same direction and thread
 // mapping): one launch per LDS length, each item's blocks running that length's

@@ -162,6 +162,43 @@ FftArgs col_pass_args(const Plan2& plan, uint32_t total, const float2* rowout, f
   return b;
 }
 
+// Row pass of the column-epilogue pipeline: the same lines, stored coil-fastest
+// [G_r][N_c][Cp] (only the N_c kept columns), so both mappings run across coils.
+FftArgs row_pass_coil_last_args(const Plan2& plan, uint32_t total, size_t cp, const float2* grid, float2* rowout) {
+  FftArgs a = row_pass_args(plan, total, cp, grid, rowout);
+  a.dst.planes = plan.d.grid_rows;
+  a.dst.lines = total;
+  a.dst.plane_stride = static_cast<uint64_t>(plan.d.image_cols) * cp;
+  a.dst.line_stride = 1;
+  a.dst.elem_stride = cp;
+  a.store_lines_fast = 1;
+  return a;
+}
+
+// Column pass over the coil-fastest row output: lines (plane = column c, coil k) along
+// the grid rows, deapodized (rows by output element, columns by plane) and cropped; its
+// epilogue writes each scan's coil-last images and RSS.
+FftArgs col_pass_coil_last_args(const Plan2& plan, uint32_t total, size_t cp, const float2* rowout) {
+  const auto& d = plan.d;
+  FftArgs b;
+  b.in = rowout;
+  b.src.planes = d.image_cols;
+  b.src.lines = total;
+  b.src.plane_stride = cp;
+  b.src.line_stride = 1;
+  b.src.elem_stride = static_cast<uint64_t>(d.image_cols) * cp;
+  b.dst = b.src;
+  b.n = d.grid_rows;
+  b.n_out = d.image_rows;
+  b.out_offset = d.centred ? (d.grid_rows - d.image_rows / 2) % d.grid_rows : 0;
+  b.out_weight = plan.deapod_rows;
+  b.line_weight = plan.deapod_cols;
+  b.line_weight_by_plane = 1;
+  b.inverse = 1;
+  b.load_lines_fast = b.store_lines_fast = 1;
+  return b;
+}
+
 uint32_t pass_count(uint32_t channels, uint32_t scans) {
   const uint32_t per = adjoint2_scans_per_pass(channels);
   return (std::max(scans, 1u) + per - 1) / per;
@@ -184,7 +221,7 @@ size_t adjoint2_pass_workspace_bytes(const Plan2& plan, uint32_t channels, uint3
   const size_t cp = padded_coils(total);
   const size_t sorted = static_cast<size_t>(d.samples) * cp * sizeof(float2);
   const size_t grid = cp * d.grid_rows * d.grid_cols * sizeof(float2);
-  const size_t rowout = static_cast<size_t>(total) * d.grid_rows * d.image_cols * sizeof(float2);
+  const size_t rowout = cp * d.grid_rows * d.image_cols * sizeof(float2);
   const size_t img = static_cast<size_t>(total) * d.image_rows * d.image_cols * sizeof(float2);
   return align256(sorted) + align256(grid) + align256(rowout) + align256(img) +
          align256(grid_partial_bytes(plan, total));
@@ -220,7 +257,7 @@ void adjoint2_execute(const Plan2& plan, const AdjointScan* scans, uint32_t coun
     auto* grid = reinterpret_cast<float2*>(p);
     p += align256(cp * d.grid_rows * d.grid_cols * sizeof(float2));
     auto* rowout = reinterpret_cast<float2*>(p);
-    p += align256(static_cast<size_t>(total) * d.grid_rows * d.image_cols * sizeof(float2));
+    p += align256(cp * d.grid_rows * d.image_cols * sizeof(float2));
     auto* img = reinterpret_cast<float2*>(p);
     p += align256(static_cast<size_t>(total) * d.image_rows * d.image_cols * sizeof(float2));
     auto* partial = reinterpret_cast<float2*>(p);
@@ -243,6 +280,26 @@ void adjoint2_execute(const Plan2& plan, const AdjointScan* scans, uint32_t coun
     if (plan.multi_count > 0) {
       ProfScope ps("grid_reduce", stream);
       grid_reduce(plan, total, grid, partial, stream);
+    }
+
+    // Coil-last outputs: rows stored coil-fastest, then one column pass that writes
+    // every scan's coil-last images and RSS itself (no coil planes, no split pass).
+    const FftArgs b2 = col_pass_coil_last_args(plan, total, cp, rowout);
+    if (layout == kLayoutCoilLast && fft_columns_supported(b2, channels)) {
+      {
+        ProfScope ps("fft_rows", stream);
+        fft_launch(row_pass_coil_last_args(plan, total, cp, grid, rowout), stream);
+      }
+      ColumnOutputs co;
+      co.channels = channels;
+      co.cols = d.image_cols;
+      for (uint32_t i = 0; i < n; ++i) {
+        co.coil_last[i] = scans[first + i].images;
+        co.rss[i] = scans[first + i].rss;
+      }
+      ProfScope ps("fft_cols_deapod", stream);
+      fft_launch_columns(b2, co, stream);
+      continue;
     }
 
     const FftArgs a = row_pass_args(plan, total, cp, grid, rowout);

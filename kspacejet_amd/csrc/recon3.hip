@@ -282,7 +282,7 @@ __global__ void rss3_kernel(const float2* img, uint32_t coils, uint64_t pixels, 
   float s = first ? 0.f : sumsq[p];
   for (uint32_t c = 0; c < coils; ++c) {
     const float2 v = img[static_cast<uint64_t>(c) * pixels + p];
-    s += v.x * v.x + v.y * v.y;
+    s += norm2(v);
   }
   if (last)
     rss[p] = sqrtf(s);
