@@ -99,6 +99,7 @@ enum TuneKey : int {
   kTuneServeCopyThreads,  // servers created from now on: memcpy worker threads (beside the caller) for host requests
   kTuneFftRegister,       // 1 = register-staged FFT where loads and stores both run across lines (same bits),
                           // 0 = the LDS-staged form everywhere
+  kTuneFftLdsPad,         // register-form FFT: LDS element stride = lines per block + this (1..4; same bits)
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);

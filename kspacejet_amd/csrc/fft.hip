@@ -423,9 +423,11 @@ __device__ inline void reg_transform(RegLine<N, V>& x, float2* buf, int S, int l
   }
 }
 
+// Element-major LDS of the register form: element stride lpb + pad (tuning key
+// fft_lds_pad, >= 1, so the epilogue's stride lpb + 1 staging of |x|^2 fits too).
 template <int N>
-__host__ __device__ constexpr int reg_lds_elems(int lpb) {
-  return N * (lpb + 1);
+__host__ __device__ constexpr int reg_lds_elems(int lpb, int pad) {
+  return N * (lpb + pad);
 }
 
 // Logical block of a launch of `blocks` blocks: consecutive logical blocks on one XCD
@@ -439,7 +441,7 @@ __device__ inline uint64_t xcd_block(uint64_t blocks) {
 
 template <int N, int V, bool INV, bool EPI>
 __global__ __launch_bounds__(1024) void fft_reg_kernel(FftArgs a, ColumnOutputs co, const float2* __restrict__ tw_global,
-                                                       int lpb_log2) {
+                                                       int lpb_log2, int pad) {
   static_assert(shape_ok<N, V>(), "radix schedule must multiply to N and split evenly over the line's threads");
   constexpr int T = Shape<N, V>::T;
   constexpr int E = N / T;
@@ -447,7 +449,7 @@ __global__ __launch_bounds__(1024) void fft_reg_kernel(FftArgs a, ColumnOutputs 
   __shared__ uint64_t src_base[64], dst_base[64];
   __shared__ float ow[N], lw[64];
   const int lpb = 1 << lpb_log2;
-  const int S = lpb + 1;
+  const int S = lpb + pad;
   float2* tw = smem;
   float2* buf = smem + N;
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
@@ -510,7 +512,7 @@ __global__ __launch_bounds__(1024) void fft_reg_kernel(FftArgs a, ColumnOutputs 
       if (e >= nout) continue;
       const float2 v = cscale(x.v[i], ow[e] * lw[ly]);
       if (img) img[(static_cast<uint64_t>(e) * co.cols + c) * co.channels + kk] = v;
-      pw[e * lpb + ly] = norm2(v);
+      pw[e * (lpb + 1) + ly] = norm2(v);
     }
     __syncthreads();
     const uint32_t gpb = static_cast<uint32_t>(lpb) / co.channels;  // groups per block
@@ -523,7 +525,7 @@ __global__ __launch_bounds__(1024) void fft_reg_kernel(FftArgs a, ColumnOutputs 
       float* rss = co.rss[grp];
       if (!rss) continue;
       float acc = 0.f;
-      for (uint32_t m = 0; m < co.channels; ++m) acc += pw[e * lpb + gb * co.channels + m];
+      for (uint32_t m = 0; m < co.channels; ++m) acc += pw[e * (lpb + 1) + gb * co.channels + m];
       rss[static_cast<uint64_t>(e) * co.cols + col] = sqrtf(acc);
     }
   }
@@ -638,7 +640,8 @@ size_t fft_shmem(int lpb_log2) {
 
 template <int N, int V>
 size_t reg_shmem(int lpb_log2) {
-  return (static_cast<size_t>(N) + reg_lds_elems<N>(1 << lpb_log2)) * sizeof(float2);
+  return (static_cast<size_t>(N) + reg_lds_elems<N>(1 << lpb_log2, static_cast<int>(tune_get(kTuneFftLdsPad)))) *
+         sizeof(float2);
 }
 
 template <typename K>
@@ -669,16 +672,17 @@ void launch_reg(const FftArgs& a, const ColumnOutputs& co, const float2* tw, hip
   const int lines = 1 << lpb_log2;
   if constexpr (EPI) require(lines % co.channels == 0, "column epilogue needs whole coil groups per block");
   const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
+  const int pad = static_cast<int>(tune_get(kTuneFftLdsPad));
   const size_t shmem = reg_shmem<N, V>(lpb_log2);
   const unsigned grid = ceil_div(total, static_cast<uint64_t>(lines));
   if (a.inverse) {
     raise_lds(&fft_reg_kernel<N, V, true, EPI>, shmem);
     hipLaunchKernelGGL((fft_reg_kernel<N, V, true, EPI>), dim3(grid), dim3(lines * T), shmem, stream, a, co, tw,
-                       lpb_log2);
+                       lpb_log2, pad);
   } else {
     raise_lds(&fft_reg_kernel<N, V, false, EPI>, shmem);
     hipLaunchKernelGGL((fft_reg_kernel<N, V, false, EPI>), dim3(grid), dim3(lines * T), shmem, stream, a, co, tw,
-                       lpb_log2);
+                       lpb_log2, pad);
   }
   KSJ_LAUNCH_CHECK();
 }

@@ -46,6 +46,7 @@ Param g_params[kTuneKeys] = {
     {"matrix_tap_share", 1, [](int64_t v) { return v == 0 || v == 1; }},
     {"serve_copy_threads", 7, [](int64_t v) { return v >= 0 && v <= 15; }},
     {"fft_register", 1, [](int64_t v) { return v == 0 || v == 1; }},
+    {"fft_lds_pad", 1, [](int64_t v) { return v >= 1 && v <= 4; }},
 };
 
 }  // namespace
