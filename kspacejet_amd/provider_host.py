@@ -610,10 +610,12 @@ class Operator:
 
 
 class ProviderModule:
-    """A loaded provider module: libksj_provider_<name>.so beside this file."""
+    """A loaded provider module: libksj_provider_<name>.so beside this file, or in the
+    directory KSJ_PROVIDER_DIR names (another build of the same modules, e.g. one with
+    the host sanitizers)."""
 
     def __init__(self, name: str, path: Optional[str] = None):
-        self.path = path or os.path.join(PKG, f"libksj_provider_{name}.so")
+        self.path = path or os.path.join(os.environ.get("KSJ_PROVIDER_DIR") or PKG, f"libksj_provider_{name}.so")
         if not os.path.exists(self.path):
             raise ImportError(f"{self.path} is not built; run python -m kspacejet_amd.build")
         self.lib = C.CDLL(self.path)
@@ -642,12 +644,14 @@ class ProviderModule:
     def operators(self) -> List[OperatorDescriptor]:
         return [self.descriptor.operators[i] for i in range(self.descriptor.operator_count)]
 
-    def create(self, operator_id: str, config: str):
-        """operator_create -> (status, Operator or None, error message)."""
+    def create(self, operator_id: str, config):
+        """operator_create -> (status, Operator or None, error message); config is text
+        (sent as UTF-8) or raw bytes."""
         req = OperatorCreateRequest()
         req.abi = header(OperatorCreateRequest)
         req.operator_id = utf8(operator_id)
-        req.canonical_config = byte_view(config.encode())
+        raw = config if isinstance(config, (bytes, bytearray)) else config.encode()
+        req.canonical_config = byte_view(bytes(raw))
         out = vp()
         e = empty_error()
         st = self.api.operator_create(C.byref(req), C.byref(out), C.byref(e))
