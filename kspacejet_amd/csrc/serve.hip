@@ -258,10 +258,9 @@ Server::Pending Server::prepare(const ServeRequest& r) {
   return p;
 }
 
-// Reserves one contiguous region of the pinned staging for `bytes` (first fit, blocking
-// while the staging is full) and returns its offset; the region carries `refs` requests.
-size_t Server::reserve_slot(size_t bytes, uint32_t refs) {
-  std::unique_lock<std::mutex> lock(staging_.mu);
+// First fit of `bytes` in the pinned staging (allocated at the first use), under
+// staging_.mu; returns false when no gap holds it.
+bool Server::fit_slot(size_t bytes, size_t* offset) {
   if (!staging_.base) {
     KSJ_HIP(hipSetDevice(cfg_.device));
     void* base = nullptr;
@@ -270,17 +269,29 @@ size_t Server::reserve_slot(size_t bytes, uint32_t refs) {
     staging_.bytes = cfg_.host_staging_bytes;
   }
   require(bytes <= staging_.bytes, "host-memory request larger than the server's pinned staging");
+  size_t at = 0;  // first gap in offset order that holds the region
+  for (const auto& kv : staging_.used) {
+    if (kv.first - at >= bytes) break;
+    at = kv.first + kv.second;
+  }
+  if (at + bytes > staging_.bytes) return false;
+  staging_.used[at] = bytes;
+  *offset = at;
+  return true;
+}
+
+bool Server::try_reserve_slot(size_t bytes, size_t* offset) {
+  std::lock_guard<std::mutex> lock(staging_.mu);
+  return fit_slot(bytes, offset);
+}
+
+// Reserves one region of the pinned staging, blocking while the staging is full.
+size_t Server::reserve_slot(size_t bytes) {
+  std::unique_lock<std::mutex> lock(staging_.mu);
+  size_t at = 0;
   while (true) {
     require(!staging_.stop, "server is shutting down");
-    size_t at = 0;  // first gap in offset order that holds the region
-    for (const auto& kv : staging_.used) {
-      if (kv.first - at >= bytes) break;
-      at = kv.first + kv.second.bytes;
-    }
-    if (at + bytes <= staging_.bytes) {
-      staging_.used[at] = Slot{bytes, refs};
-      return at;
-    }
+    if (fit_slot(bytes, &at)) return at;
     staging_.cv.wait(lock);
   }
 }
@@ -288,10 +299,7 @@ size_t Server::reserve_slot(size_t bytes, uint32_t refs) {
 void Server::release_slot(size_t slot) {
   {
     std::lock_guard<std::mutex> lock(staging_.mu);
-    auto it = staging_.used.find(slot);
-    if (it == staging_.used.end()) return;
-    if (--it->second.refs > 0) return;
-    staging_.used.erase(it);
+    staging_.used.erase(slot);
   }
   staging_.cv.notify_all();
 }
@@ -308,49 +316,57 @@ void Server::submit_many(const ServeRequest* r, uint32_t count, uint64_t* ticket
   for (uint32_t i = 0; i < count; ++i) ps.push_back(prepare(r[i]));  // all validated before any is queued
   for (const Pending& p : ps)
     require(p.staged_bytes() <= cfg_.host_staging_bytes, "host-memory request larger than the server's pinned staging");
-  // Host-memory requests are staged in groups that fit the staging at once; each group
-  // is queued as soon as its inputs are copied, so a call larger than the staging cannot
-  // wait on itself.  A call that fits is queued atomically.
-  uint32_t first = 0;
-  while (first < count) {
-    uint32_t last = first;
-    size_t group = 0, staged = 0;
-    for (; last < count; ++last) {
-      const size_t b = ps[last].staged_bytes();
-      if (group + b > cfg_.host_staging_bytes) break;
-      group += b;
-      staged += b ? 1 : 0;
-    }
-    if (staged > 0) {
-      const size_t base = reserve_slot(group, static_cast<uint32_t>(staged));
-      size_t at = base;
-      for (uint32_t i = first; i < last; ++i) {
-        Pending& p = ps[i];
-        if (p.staged_bytes() == 0) continue;
-        p.slot = base;
-        for (Pending::HostBuf* h : {&p.in, &p.rss, &p.img}) {
-          if (h->direct || h->dev == 0) continue;
-          h->at = at;
-          at += h->dev;
-        }
-        if (!p.in.direct) copies_->copy(staging_.base + p.in.at, p.in.user, p.in.len);
-      }
-    }
+  // A call without host-memory requests is queued atomically (the scheduler sees all of
+  // it at once).  Host-memory requests are staged one by one, each in its own region of
+  // the pinned staging, and queued in flushes of up to max_batch_scans requests or
+  // kFlushBytes staged bytes, so the copies of later requests overlap the transfers and
+  // kernels of earlier ones.  Before waiting for staging space the call queues what it
+  // has staged, so it never waits on its own requests.
+  constexpr size_t kFlushBytes = 64u << 20;
+  bool any_host = false;
+  for (const Pending& p : ps) any_host = any_host || p.staged_bytes() > 0;
+  std::vector<uint32_t> ready;
+  size_t ready_bytes = 0;
+  auto flush = [&] {
+    if (ready.empty()) return;
     {
       std::lock_guard<std::mutex> lock(mu_);
       const auto now = std::chrono::steady_clock::now();
-      for (uint32_t i = first; i < last; ++i) {
+      for (uint32_t i : ready) {
         ps[i].ticket = next_ticket_++;
         ps[i].submitted = now;
         tickets[i] = ps[i].ticket;
         pending_.push_back(ps[i]);
       }
-      stats_.submitted += last - first;
-      inflight_ += last - first;
+      stats_.submitted += ready.size();
+      inflight_ += ready.size();
     }
     cv_work_.notify_one();
-    first = last;
+    ready.clear();
+    ready_bytes = 0;
+  };
+  for (uint32_t i = 0; i < count; ++i) {
+    Pending& p = ps[i];
+    const size_t b = p.staged_bytes();
+    if (b > 0) {
+      size_t at = 0;
+      if (!try_reserve_slot(b, &at)) {
+        flush();
+        at = reserve_slot(b);
+      }
+      p.slot = at;
+      for (Pending::HostBuf* h : {&p.in, &p.rss, &p.img}) {
+        if (h->direct || h->dev == 0) continue;
+        h->at = at;
+        at += h->dev;
+      }
+      if (!p.in.direct) copies_->copy(staging_.base + p.in.at, p.in.user, p.in.len);
+      ready_bytes += b;
+    }
+    ready.push_back(i);
+    if (any_host && (ready.size() >= cfg_.max_batch_scans || ready_bytes >= kFlushBytes)) flush();
   }
+  flush();
 }
 
 int Server::wait(uint64_t ticket, double* latency_ms, std::string* error) {

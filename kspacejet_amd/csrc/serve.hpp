@@ -142,16 +142,12 @@ class Server {
     size_t device_bytes() const { return in.dev + rss.dev + img.dev; }
   };
   // First-fit allocator over one pinned host buffer (the staging of host-memory requests).
-  struct Slot {
-    size_t bytes;
-    uint32_t refs;  // requests staged in the region that have not completed
-  };
   struct Staging {
     std::mutex mu;  // taken after mu_ when both are held
     std::condition_variable cv;
     char* base = nullptr;
     size_t bytes = 0;
-    std::map<size_t, Slot> used;  // offset -> region
+    std::map<size_t, size_t> used;  // offset -> bytes of a staged request's region
     bool stop = false;
   };
   struct Done {
@@ -180,7 +176,9 @@ class Server {
   };
 
   Pending prepare(const ServeRequest& r);  // validates and keys a request (hashes a trajectory if needed)
-  size_t reserve_slot(size_t bytes, uint32_t refs);  // pinned staging region (blocks while full)
+  bool fit_slot(size_t bytes, size_t* offset);        // under staging_.mu
+  bool try_reserve_slot(size_t bytes, size_t* offset);
+  size_t reserve_slot(size_t bytes);                  // blocks while the staging is full
   void release_slot(size_t slot);
   void run();
   bool launch_next(Lane& lane);  // under mu_; returns false when nothing could be launched
