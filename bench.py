@@ -800,6 +800,17 @@ def run_uneven(torch, ops, lib, count, steps, warmup, tune_seconds):
         multi()
     graph_s = timed(graph.replay)
     graph.close()
+    # both multi-plan forms with the other tuned keys, each captured and replayed
+    by_form = {}
+    tuned_fused = ops.tune_get("multi_fused")
+    for name, form in (("lanes", 0), ("fused", 1)):
+        ops.tune_set("multi_fused", form)
+        g = ops.Graph(stream)
+        with g:
+            multi()
+        by_form[name] = round(1e3 * timed(g.replay), 3)
+        g.close()
+    ops.tune_set("multi_fused", tuned_fused)
     res = {
         "items": count, "shapes": "matrix {128,192,256,320} x coils 8-32 x spokes 101-603, readout 2 x matrix",
         "matrix_counts": {str(m): sum(1 for s in shapes if s[0] == m) for m in synth.UNEVEN_MATRICES},
@@ -811,6 +822,7 @@ def run_uneven(torch, ops, lib, count, steps, warmup, tune_seconds):
         "graph_over_plain": round(plain_s / graph_s, 2),
         "ms_per_batch": {"plain": round(1e3 * plain_s, 3), "multi": round(1e3 * multi_s, 3),
                          "graph": round(1e3 * graph_s, 3)},
+        "graph_ms_by_form": by_form,
         "tuned": tuned["settings"], "tune_trials": len(tuned["trials"]),
         "workspace_bytes": ws.numel(),
     }

@@ -440,8 +440,8 @@ __device__ inline uint64_t xcd_block(uint64_t blocks) {
 }
 
 template <int N, int V, bool INV, bool EPI>
-__global__ __launch_bounds__(1024) void fft_reg_kernel(FftArgs a, ColumnOutputs co, const float2* __restrict__ tw_global,
-                                                       int lpb_log2, int pad) {
+__device__ inline void fft_reg_block(const FftArgs& a, const ColumnOutputs& co, const float2* __restrict__ tw_global,
+                                     int lpb_log2, int pad, uint64_t block) {
   static_assert(shape_ok<N, V>(), "radix schedule must multiply to N and split evenly over the line's threads");
   constexpr int T = Shape<N, V>::T;
   constexpr int E = N / T;
@@ -457,7 +457,7 @@ __global__ __launch_bounds__(1024) void fft_reg_kernel(FftArgs a, ColumnOutputs 
     ow[i] = (i < static_cast<int>(a.n_out) && a.out_weight) ? a.scale * a.out_weight[i] : a.scale;
   }
   const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
-  const uint64_t first = xcd_block(gridDim.x) * lpb;
+  const uint64_t first = block * lpb;
   for (int l = threadIdx.x; l < lpb; l += blockDim.x) {
     const uint64_t line = first + l;
     src_base[l] = line < total ? line_base(a.src, line) : ~0ull;
@@ -529,6 +529,22 @@ __global__ __launch_bounds__(1024) void fft_reg_kernel(FftArgs a, ColumnOutputs 
       rss[static_cast<uint64_t>(e) * co.cols + col] = sqrtf(acc);
     }
   }
+}
+
+template <int N, int V, bool INV, bool EPI>
+__global__ __launch_bounds__(1024) void fft_reg_kernel(FftArgs a, ColumnOutputs co, const float2* __restrict__ tw_global,
+                                                       int lpb_log2, int pad) {
+  fft_reg_block<N, V, INV, EPI>(a, co, tw_global, lpb_log2, pad, xcd_block(gridDim.x));
+}
+
+// Fused items of one length in the register form: each block runs its item's lines.
+template <int N, int V, bool INV>
+__global__ __launch_bounds__(1024) void fft_fused_reg_kernel(const FusedTable<FftArgs>* t,
+                                                             const float2* __restrict__ tw_global, int lpb_log2,
+                                                             int pad) {
+  const uint32_t item = fused_item(t, blockIdx.x);
+  const FftArgs a = fused_args(t, item);
+  fft_reg_block<N, V, INV, false>(a, ColumnOutputs{}, tw_global, lpb_log2, pad, blockIdx.x - fused_first(t, item));
 }
 
 template <int N, int V, bool INV>
@@ -662,13 +678,20 @@ bool reg_form(const FftArgs& a) {
   return reg_form_compiled<N, V>() && a.load_lines_fast && a.store_lines_fast && tune_get(kTuneFftRegister) == 1;
 }
 
+// Lines per block (log2) of a register-form launch: the staged form's choice, reduced
+// until the element-major LDS (N (lpb + pad) next to the twiddles and tables) fits.
+template <int N, int V>
+int reg_lines_log2(const FftArgs& a, int lpb) {
+  int lpb_log2 = lines_per_block_log2<N, V>(a, lpb);
+  constexpr int kStatic = 64 * 2 * 8 + N * 4 + 64 * 4;
+  while (lpb_log2 > 0 && static_cast<int>(reg_shmem<N, V>(lpb_log2)) + kStatic > max_dynamic_lds()) --lpb_log2;
+  return lpb_log2;
+}
+
 template <int N, int V, bool EPI>
 void launch_reg(const FftArgs& a, const ColumnOutputs& co, const float2* tw, hipStream_t stream, int lpb) {
   constexpr int T = Shape<N, V>::T;
-  int lpb_log2 = lines_per_block_log2<N, V>(a, lpb);
-  // LDS of the element-major layout (N (lpb + 1) next to the twiddles and tables).
-  constexpr int kStatic = 64 * 2 * 8 + N * 4 + 64 * 4;
-  while (lpb_log2 > 0 && static_cast<int>(reg_shmem<N, V>(lpb_log2)) + kStatic > max_dynamic_lds()) --lpb_log2;
+  const int lpb_log2 = reg_lines_log2<N, V>(a, lpb);
   const int lines = 1 << lpb_log2;
   if constexpr (EPI) require(lines % co.channels == 0, "column epilogue needs whole coil groups per block");
   const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
@@ -711,7 +734,9 @@ void launch_fast(const FftArgs& a, const float2* tw, hipStream_t stream, int lpb
 template <int N, int V>
 void launch_fused(const FftArgs* const* items, uint32_t count, const float2* tw, void* d_table, hipStream_t stream) {
   constexpr int T = Shape<N, V>::T;
-  const int lpb_log2 = lines_per_block_log2<N, V>(*items[0], 0);
+  bool reg = false;
+  if constexpr (reg_form_compiled<N, V>()) reg = reg_form<N, V>(*items[0]);
+  const int lpb_log2 = reg ? reg_lines_log2<N, V>(*items[0], 0) : lines_per_block_log2<N, V>(*items[0], 0);
   const uint64_t lpb = 1ull << lpb_log2;
   FusedTableBuilder<FftArgs> b;
   for (uint32_t i = 0; i < count; ++i) {
@@ -722,8 +747,23 @@ void launch_fused(const FftArgs* const* items, uint32_t count, const float2* tw,
   }
   b.upload(d_table, stream);
   const auto* t = static_cast<const FusedTable<FftArgs>*>(d_table);
-  const size_t shmem = fft_shmem<N, V>(lpb_log2);
   const dim3 grid(static_cast<uint32_t>(b.blocks())), block(static_cast<uint32_t>(lpb * T));
+  if constexpr (reg_form_compiled<N, V>()) {
+    if (reg) {
+      const size_t rshmem = reg_shmem<N, V>(lpb_log2);
+      const int pad = static_cast<int>(tune_get(kTuneFftLdsPad));
+      if (items[0]->inverse) {
+        raise_lds(&fft_fused_reg_kernel<N, V, true>, rshmem);
+        hipLaunchKernelGGL((fft_fused_reg_kernel<N, V, true>), grid, block, rshmem, stream, t, tw, lpb_log2, pad);
+      } else {
+        raise_lds(&fft_fused_reg_kernel<N, V, false>, rshmem);
+        hipLaunchKernelGGL((fft_fused_reg_kernel<N, V, false>), grid, block, rshmem, stream, t, tw, lpb_log2, pad);
+      }
+      KSJ_LAUNCH_CHECK();
+      return;
+    }
+  }
+  const size_t shmem = fft_shmem<N, V>(lpb_log2);
   if (items[0]->inverse) {
     if (shmem > 65536)
       KSJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_fused_kernel<N, V, true>),
@@ -838,10 +878,7 @@ int columns_lines_per_block(const FftArgs& a) {
     if constexpr (!reg_form_compiled<N, V>()) {
       return 0;
     } else {
-      int lpb_log2 = lines_per_block_log2<N, V>(a, 0);
-      constexpr int kStatic = 64 * 2 * 8 + N * 4 + 64 * 4;
-      while (lpb_log2 > 0 && static_cast<int>(reg_shmem<N, V>(lpb_log2)) + kStatic > max_dynamic_lds()) --lpb_log2;
-      return 1 << lpb_log2;
+      return 1 << reg_lines_log2<N, V>(a, 0);
     }
   };
   switch (v) {
