@@ -319,12 +319,12 @@ void Server::submit_many(const ServeRequest* r, uint32_t count, uint64_t* ticket
   // A call without host-memory requests is queued atomically (the scheduler sees all of
   // it at once).  Host-memory requests are staged one by one, each in its own region of
   // the pinned staging, and queued in flushes of up to max_batch_scans requests or
-  // kFlushBytes staged bytes, so the copies of later requests overlap the transfers and
+  // kFlushBytes of transfers, so the copies of later requests overlap the transfers and
   // kernels of earlier ones.  Before waiting for staging space the call queues what it
   // has staged, so it never waits on its own requests.
   constexpr size_t kFlushBytes = 64u << 20;
   bool any_host = false;
-  for (const Pending& p : ps) any_host = any_host || p.staged_bytes() > 0;
+  for (const Pending& p : ps) any_host = any_host || p.req.host;
   std::vector<uint32_t> ready;
   size_t ready_bytes = 0;
   auto flush = [&] {
@@ -361,8 +361,8 @@ void Server::submit_many(const ServeRequest* r, uint32_t count, uint64_t* ticket
         at += h->dev;
       }
       if (!p.in.direct) copies_->copy(staging_.base + p.in.at, p.in.user, p.in.len);
-      ready_bytes += b;
     }
+    ready_bytes += p.req.host ? p.device_bytes() : 0;
     ready.push_back(i);
     if (any_host && (ready.size() >= cfg_.max_batch_scans || ready_bytes >= kFlushBytes)) flush();
   }
@@ -520,10 +520,16 @@ bool Server::launch_next(Lane& lane) {
   const ServeRequest& r = head.req;
   std::vector<Pending> batch;
   const uint32_t limit = r.kind == kServeRadial3D ? 1u : cfg_.max_batch_scans;
+  // Host-memory requests are bound by their transfers, not the kernels: a launch takes at
+  // most kHostBatchBytes of them, so one lane's transfers overlap another's kernels.
+  constexpr size_t kHostBatchBytes = 64u << 20;
+  size_t host_bytes = 0;
   for (auto it = pending_.begin(); it != pending_.end() && batch.size() < limit;) {
     const ServeRequest& q = it->req;
     const bool same = it->plan_key == head.plan_key && q.channels == r.channels;
+    if (same && q.host && !batch.empty() && host_bytes + it->device_bytes() > kHostBatchBytes) break;
     if (same) {
+      host_bytes += q.host ? it->device_bytes() : 0;
       batch.push_back(*it);
       it = pending_.erase(it);
     } else {
