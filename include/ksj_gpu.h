@@ -512,9 +512,16 @@ void ksj_gpu_serve_destroy(ksj_gpu_server* server);
  *   "matrix_tap_share"     1 = each gather lane loads one word of a visit's taps and takes its two taps from
  *                          the visit's lanes by bpermute, the packed visits of a batch likewise (default);
  *                          0 = each lane loads its own taps and visits (same bits)
+ *   "matrix3_tap_share"    3-D matrix gather: 1 = one or two words of a visit's 3W taps per lane, handed round
+ *                          by bpermute; 0 = each lane loads its own six taps (default; same bits)
+ *   "matrix3_map"          3-D matrix gather: 0 = runs in launch order (default), 1 = each XCD the contiguous
+ *                          eighth of the runs, s > 1 = slabs of s workgroups dealt round-robin to the XCDs
+ *                          (same bits)
  *   "matrix_weights"       1 = new 2-D matrix plans store a 4-byte (sample, window bin) visit and the
  *                          gather forms the cell weights from the sample's taps (default), 0 = every
  *                          visit stores its 16 or 32 cell weights (same bits; plan size and speed only)
+ *   "fft_lds_pad"          register-form FFT: LDS element stride = lines per block + pad (1..4, default 1;
+ *                          same bits)
  *   "fft_register"         1 = FFT passes whose loads and stores both run across contiguous lines keep
  *                          the first and last radix pass in registers, and 2-D radial coil-last
  *                          outputs take one column pass that writes the coil images and RSS itself

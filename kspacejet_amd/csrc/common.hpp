@@ -100,6 +100,8 @@ enum TuneKey : int {
   kTuneFftRegister,       // 1 = register-staged FFT where loads and stores both run across lines (same bits),
                           // 0 = the LDS-staged form everywhere
   kTuneFftLdsPad,         // register-form FFT: LDS element stride = lines per block + this (1..4; same bits)
+  kTuneMatrix3Map,        // 3-D matrix gather: 0 launch order, 1 XCD-contiguous eighths, s > 1 XCD slabs (same bits)
+  kTuneMatrix3TapShare,   // 3-D matrix gather: 1 = one or two tap words per lane shared by bpermute (same bits)
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);

@@ -35,6 +35,9 @@ struct Gather3Args {
   const uint32_t* run_first;   // [runs + 1]
   uint32_t run_count;
   int width;
+  // workgroup -> XCD mapping (matrix3_map): 0 launch order; 1 XCD x takes the x-th eighth
+  // of the workgroups (contiguous blocks); s > 1 slabs of s workgroups dealt round-robin
+  uint32_t map, wg_per_xcd;
 };
 
 __device__ inline uint32_t last_visit(uint32_t v, uint32_t end) {
@@ -152,7 +155,7 @@ __global__ __launch_bounds__(256) void reduce3_kernel(Gather3Args a) {
 // batches of U groups with the next batch's visits loaded while this batch's rows and
 // taps are in flight; an item's 64 cells are written when the stream passes its end.
 // Sums run four visits at a time in visit order: deterministic.
-template <int CP, int U>
+template <int CP, int U, bool TS>
 __device__ inline void matrix3_run(const Gather3Args& a, uint32_t run) {
   constexpr int NF = CP / 8;
   constexpr int T = 4;  // z-planes of a block
@@ -222,8 +225,9 @@ __device__ inline void matrix3_run(const Gather3Args& a, uint32_t run) {
   for (; c < c1; c += U) {
     RowPiece<NF> b[U];
     float tz[U][T], ty[U], tx[U];
-    int rzs[U];
+    int rzs[U], iys[U], ixs[U];
     bool vyx[U];
+    float tw0[U], tw1[U];  // tap-sharing form: words j and j + 16 of the visit's 3W taps
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       b[u] = load_piece<NF>(rows + static_cast<uint64_t>(k[u]) * (2 * CP));
@@ -232,14 +236,23 @@ __device__ inline void matrix3_run(const Gather3Args& a, uint32_t run) {
                 rx = static_cast<int>((pos[u] >> 8) & 15);
       const int iy = cy - ry + W - 1, ix = cx - rx + W - 1;
       vyx[u] = iy >= 0 && iy < W && ix >= 0 && ix < W;
-      ty[u] = tp[W + (iy >= 0 && iy < W ? iy : 0)];
-      tx[u] = tp[2 * W + (ix >= 0 && ix < W ? ix : 0)];
-#pragma unroll
-      for (int h = 0; h < T; ++h) {
-        const int iz = h - rz + W - 1;
-        tz[u][h] = tp[iz >= 0 && iz < W ? iz : 0];
-      }
       rzs[u] = rz;
+      iys[u] = iy >= 0 && iy < W ? iy : 0;
+      ixs[u] = ix >= 0 && ix < W ? ix : 0;
+      if constexpr (TS) {
+        // The 16 lanes of a visit need 3W distinct taps between them: each loads one or
+        // two words and takes its own from the visit's lanes by bpermute after the wait.
+        tw0[u] = tp[j < 3 * W ? j : 3 * W - 1];
+        tw1[u] = 3 * W > 16 ? tp[j + 16 < 3 * W ? j + 16 : 3 * W - 1] : 0.f;
+      } else {
+        ty[u] = tp[W + iys[u]];
+        tx[u] = tp[2 * W + ixs[u]];
+#pragma unroll
+        for (int h = 0; h < T; ++h) {
+          const int iz = h - rz + W - 1;
+          tz[u][h] = tp[iz >= 0 && iz < W ? iz : 0];
+        }
+      }
     }
     load_meta(c + U);
     // Keep every load of the batch ahead of the products (see the 2-D gather).
@@ -251,6 +264,24 @@ __device__ inline void matrix3_run(const Gather3Args& a, uint32_t run) {
           flush(t);
           ++t;
           t_end = field(rec.z, t) / 4;
+        }
+        if constexpr (TS) {
+          // word w of visit q lies in lane 16 q + w (w < 16) of tw0, or 16 q + w - 16 of tw1
+          auto tap = [&](int w) {
+            const int w0 = w < 16 ? w : w - 16;
+            const int addr = 4 * (16 * q + w0);
+            const float lo = __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(tw0[u])));
+            if (3 * W <= 16) return lo;
+            const float hi = __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(tw1[u])));
+            return w < 16 ? lo : hi;
+          };
+          ty[u] = tap(W + iys[u]);
+          tx[u] = tap(2 * W + ixs[u]);
+#pragma unroll
+          for (int h = 0; h < T; ++h) {
+            const int iz = h - rzs[u] + W - 1;
+            tz[u][h] = tap(iz >= 0 && iz < W ? iz : 0);
+          }
         }
         const float wyx = vyx[u] ? ty[u] * tx[u] : 0.f;
 #pragma unroll
@@ -267,10 +298,21 @@ __device__ inline void matrix3_run(const Gather3Args& a, uint32_t run) {
   for (; t < nitems; ++t) flush(t);  // the last item with visits, then any empty ones
 }
 
-template <int CP, int U>
+// Four waves per workgroup, one run per wave.  Hardware places workgroup b on XCD b % 8;
+// matrix3_map decides which runs it takes: in launch order (0), the XCD's contiguous
+// eighth of the runs (1), or slabs of `map` consecutive workgroups dealt round-robin to
+// the XCDs (> 1), so neighbouring blocks, which share sample rows, meet in one L2.
+template <int CP, int U, bool TS>
 __global__ __launch_bounds__(256) void gather3_matrix_kernel(Gather3Args a) {
-  const uint32_t run = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (run < a.run_count) matrix3_run<CP, U>(a, run);
+  constexpr uint32_t kXcd = 8;
+  uint32_t wg = blockIdx.x;
+  const uint32_t x = blockIdx.x % kXcd, l = blockIdx.x / kXcd;
+  if (a.map == 1)
+    wg = x * a.wg_per_xcd + l;
+  else if (a.map > 1)
+    wg = ((l / a.map) * kXcd + x) * a.map + l % a.map;
+  const uint32_t run = wg * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (run < a.run_count) matrix3_run<CP, U, TS>(a, run);
 }
 
 // sumsq[p] (+)= sum over the pass's coils of |img[c][p]|^2 in channel order; on the
@@ -303,12 +345,25 @@ __global__ void coil_last3_kernel(const float2* img, uint32_t coils, uint64_t pi
 template <int CP>
 void launch_gather3(const Gather3Args& a, hipStream_t stream) {
   if (a.side == 4) {
-    const uint32_t wgs = std::max<uint32_t>(1, ceil_div(a.run_count, 4));
-    switch (tune_get(kTuneMatrixBatch)) {  // visits per batch: groups of four
-      case 4: hipLaunchKernelGGL((gather3_matrix_kernel<CP, 1>), dim3(wgs), dim3(256), 0, stream, a); break;
-      case 8: hipLaunchKernelGGL((gather3_matrix_kernel<CP, 2>), dim3(wgs), dim3(256), 0, stream, a); break;
-      default: hipLaunchKernelGGL((gather3_matrix_kernel<CP, 4>), dim3(wgs), dim3(256), 0, stream, a); break;
-    }
+    Gather3Args m = a;
+    uint32_t wgs = std::max<uint32_t>(1, ceil_div(a.run_count, 4));
+    m.map = static_cast<uint32_t>(tune_get(kTuneMatrix3Map));
+    m.wg_per_xcd = ceil_div(wgs, 8u);
+    // padding workgroups find no run and exit
+    if (m.map == 1) wgs = m.wg_per_xcd * 8;
+    else if (m.map > 1) wgs = ceil_div(wgs, 8 * m.map) * 8 * m.map;
+    auto go = [&]<bool TS>() {
+      switch (tune_get(kTuneMatrixBatch)) {  // visits per batch: groups of four
+        case 4: hipLaunchKernelGGL((gather3_matrix_kernel<CP, 1, TS>), dim3(wgs), dim3(256), 0, stream, m); break;
+        case 8: hipLaunchKernelGGL((gather3_matrix_kernel<CP, 2, TS>), dim3(wgs), dim3(256), 0, stream, m); break;
+        default: hipLaunchKernelGGL((gather3_matrix_kernel<CP, 4, TS>), dim3(wgs), dim3(256), 0, stream, m); break;
+      }
+    };
+    // 3W taps over 16 lanes and at most two words each: widths up to 10
+    if (tune_get(kTuneMatrix3TapShare) == 1 && 3 * a.width <= 32)
+      go.template operator()<true>();
+    else
+      go.template operator()<false>();
     KSJ_LAUNCH_CHECK();
     return;
   }

@@ -127,3 +127,26 @@ def test_schedules_agree(ops, scheme):
         plan.close()
     ops.tune_set("gather3_scheme", 2)
     assert rel_l2(results[0], results[1]) <= 2e-6
+
+
+@pytest.mark.parametrize("width", [4, 6, 8])
+def test_shared_tap_loads_give_the_same_bits(ops, scheme, width):
+    """The matrix gather's tap-sharing form (each lane loads one or two words of a visit's
+    3W taps and takes its own by bpermute: one word at width 4, two at 6 and 8) against
+    every lane loading its own taps, bit for bit, with the oracle beside."""
+    shape = (10, 13, 11)
+    traj = ops.golden_means3(60, 24)
+    k = kspace(5, traj.shape[0], 31 + width)
+    saved = ops.tune_get("matrix3_tap_share")
+    outs = []
+    try:
+        for share in (1, 0):
+            ops.tune_set("matrix3_tap_share", share)
+            plan = ops.Plan3(shape, traj.shape[0], width=width).set_trajectory(traj)
+            img, _ = plan.adjoint(torch.from_numpy(k).cuda(), coils_per_pass=8)
+            outs.append(img.cpu().numpy())
+            plan.close()
+    finally:
+        ops.tune_set("matrix3_tap_share", saved)
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    assert rel_l2(outs[0], oracle_volume(k, traj, shape, width, "ramp_squared")) <= TOL
