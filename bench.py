@@ -803,7 +803,7 @@ def run_uneven(torch, ops, lib, count, steps, warmup, tune_seconds):
     # both multi-plan forms with the other tuned keys, each captured and replayed
     by_form = {}
     tuned_fused = ops.tune_get("multi_fused")
-    for name, form in (("lanes", 0), ("fused", 1)):
+    for name, form in (("lanes", 0), ("fused", 1), ("chain", 2)):
         ops.tune_set("multi_fused", form)
         g = ops.Graph(stream)
         with g:
@@ -811,6 +811,9 @@ def run_uneven(torch, ops, lib, count, steps, warmup, tune_seconds):
         by_form[name] = round(1e3 * timed(g.replay), 3)
         g.close()
     ops.tune_set("multi_fused", tuned_fused)
+    chain_faults = ops.chain_faults()
+    if chain_faults:
+        raise RuntimeError(f"the uneven batch's work queue reported {chain_faults} timed-out waits")
     res = {
         "items": count, "shapes": "matrix {128,192,256,320} x coils 8-32 x spokes 101-603, readout 2 x matrix",
         "matrix_counts": {str(m): sum(1 for s in shapes if s[0] == m) for m in synth.UNEVEN_MATRICES},

@@ -199,10 +199,13 @@ int ksj_gpu_adjoint2_execute_batch(const ksj_gpu_plan2* plan, uint32_t count, co
  * and channel count.  Two forms with bit-identical results (tuning key "multi_fused"):
  *   0: every item's pipeline runs on one of "multi_lanes" streams (largest items first,
  *      round-robin), joined back to `stream`;
- *   1: fused, for up to 64 items of the 2x2-block schedule with <= 64 channels: each
- *      stage is one launch per shape class (padded coil count for gridding, length for
- *      each FFT pass) over all items, with per-item argument tables written into the
- *      workspace by small argument-carrying kernels.
+ *   1: fused, for up to 64 items of the 2x2-block or matrix schedule with <= 64 channels:
+ *      each stage is one launch per shape class (padded coil count for gridding, length
+ *      for each FFT pass) over all items, with per-item argument tables written into the
+ *      workspace by small argument-carrying kernels;
+ *   2: as 1, with prep, gather and reduce of every item (all padded coil counts) in one
+ *      persistent launch fed by a device work queue per XCD (matrix-schedule plans with
+ *      packed visits and matrix_batch 16; other batches take form 1).  Same bits as 0 and 1.
  * The call only enqueues work (no host synchronisation, no allocation once every grid
  * length has been run once), so it can be captured with ksj_gpu_graph_begin/end and
  * replayed.  The workspace size covers both forms.
@@ -213,6 +216,10 @@ size_t ksj_gpu_adjoint2_multi_workspace_bytes(uint32_t count, const ksj_gpu_plan
 int ksj_gpu_adjoint2_execute_multi(uint32_t count, const ksj_gpu_plan2* const* plans, const void* const* d_kspace,
                                    const uint32_t* channels, void* const* d_images, int layout, float* const* d_rss,
                                    void* d_workspace, size_t workspace_bytes, void* stream);
+/* Launches of the multi_fused = 2 work queue on the current device whose bounded waits
+ * ran out (each such launch stops early and its results are undefined); 0 in correct
+ * operation.  The waits are bounded so that a broken queue cannot hang the GPU. */
+int ksj_gpu_chain_faults(uint64_t* out_count);
 
 /* Route trajectory normalisation: raw ISMRMRD-order [S][kx, ky] -> the Provider payload
  * order [S][ky, kx] in rad/px, each axis normalised with its own encoded dimension, float

@@ -143,6 +143,7 @@ SIGNATURES = {
     "ksj_gpu_adjoint2_multi_workspace_bytes": (_sz, [_u32, C.POINTER(_vp), C.POINTER(_u32)]),
     "ksj_gpu_adjoint2_execute_multi": (_int, [_u32, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_u32), C.POINTER(_vp),
                                               _int, C.POINTER(_vp), _vp, _sz, _vp]),
+    "ksj_gpu_chain_faults": (_int, [C.POINTER(_u64)]),
     "ksj_gpu_graph_begin": (_int, [_vp]),
     "ksj_gpu_graph_end": (_int, [_vp, C.POINTER(_vp)]),
     "ksj_gpu_graph_launch": (_int, [_vp, _vp]),

@@ -37,7 +37,7 @@ EXACT_KEYS: Dict[str, tuple] = {
     "fft_lines_per_block": (0, 8, 32),
     "pass_lanes": (2, 1, 3),
     "multi_lanes": (4, 2, 6, 8),
-    "multi_fused": (0, 1),
+    "multi_fused": (0, 1, 2),
 }
 # Change rounding only: every FFT radix schedule is an exact factorisation of the DFT
 # (tested against the FP64 oracle at every length); tuned when `rounding=True`.

@@ -100,6 +100,7 @@ int ksj_gpu_init(int device_ordinal) {
     KSJ_HIP(hipSetDevice(device_ordinal));
     KSJ_HIP(hipFree(nullptr));
     scratch_for_current_device();
+    ksj::gpu::grid_chain_prepare(nullptr);
   });
 }
 
@@ -564,6 +565,13 @@ int ksj_gpu_adjoint2_execute_multi(uint32_t count, const ksj_gpu_plan2* const* p
     ksj::gpu::require(d_kspace && d_workspace, "null argument");
     const auto items = multi_items(count, plans, channels, d_kspace, d_images, d_rss);
     ksj::gpu::adjoint2_execute_multi(items.data(), count, layout, d_workspace, workspace_bytes, as_stream(stream));
+  });
+}
+
+int ksj_gpu_chain_faults(uint64_t* out_count) {
+  return guarded([&] {
+    ksj::gpu::require(out_count != nullptr, "null argument");
+    *out_count = ksj::gpu::grid_chain_faults();
   });
 }
 

@@ -79,7 +79,8 @@ enum TuneKey : int {
   kTuneBlockGroup,        // 2x2-block group side of new plans: 1, 2 or 4 (work-item order; results unchanged)
   kTuneFftSchedule,       // FFT radix schedule: 0 (default), 1 (half the threads), 2 (radix-4 heavy) or 3 (per length:
                           // 1 at 512, else 0); rounding changes
-  kTuneMultiFused,        // uneven batches: 1 = one launch per stage and shape class, 0 = items over lanes
+  kTuneMultiFused,        // uneven batches: 0 = items over lanes, 1 = one launch per stage and shape class,
+                          // 2 = as 1 with prep, gather and reduce in one work-queue launch (same bits)
   kTuneMatrixRunCost,     // matrix schedule of new plans: cost (visit groups of four + 1 per item) per wave run, 1..64
   kTuneMatrixPipeline,    // matrix gather: batches of sample rows in flight per wave, 1 or 2
   kTuneMatrixNtStore,     // matrix gather: 1 = grid cells written with non-temporal stores

@@ -17,7 +17,13 @@
 #include "fused.hpp"
 #include "matrix.hpp"
 
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <memory>
+#include <mutex>
 #include <utility>
+#include <vector>
 
 namespace ksj::gpu {
 namespace {
@@ -25,6 +31,15 @@ namespace {
 // ----------------------------------------------------------------------------- prep
 constexpr int kPrepSamples = 64;  // one sample per lane of a wave
 constexpr int kPrepWaves = 4;
+
+// threadIdx.x behind an empty asm: values derived from it are then recomputed where they
+// are used instead of being hoisted out of a persistent kernel's task loop, where the
+// hoisted values of every inlined stage body would stay live at once.
+__device__ inline uint32_t opaque_tid() {
+  uint32_t t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
 
 struct PrepArgs {
   CoilPlanes planes;     // coil c -> [S] samples
@@ -46,15 +61,21 @@ struct StridedPlanes {
 __device__ inline const float2* plane_of(const CoilPlanes& p, uint32_t c) { return p.plane[c]; }
 __device__ inline const float2* plane_of(const StridedPlanes& p, uint32_t c) { return p.base + c * p.stride; }
 
+// LDS of one prep block: the padded tile, then the block's ranks.
+template <int CP>
+constexpr size_t prep_lds_bytes() {
+  return sizeof(float2) * kPrepSamples * (CP + 1) + sizeof(uint32_t) * kPrepSamples;
+}
+
 template <int CP, typename Planes>
-__device__ inline void prep_block(const Planes& planes, const float2* weight, const uint32_t* rank, float2* sorted,
-                                  uint32_t samples, uint32_t coils, uint32_t block) {
+__device__ inline void prep_block_in(const Planes& planes, const float2* weight, const uint32_t* rank, float2* sorted,
+                                     uint32_t samples, uint32_t coils, uint32_t block, float2* tile,
+                                     uint32_t* rank_s) {
   constexpr int kTs = CP + 1;               // padded row: a coil column spreads over the banks
   constexpr int kPerWave = CP / kPrepWaves;  // coils read by one wave
-  __shared__ float2 tile[kPrepSamples * kTs];
-  __shared__ uint32_t rank_s[kPrepSamples];
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t tid = opaque_tid();
+  const uint32_t lane = tid & 63;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const uint32_t s0 = block * kPrepSamples;
   const uint32_t s = s0 + lane;
   const bool live = s < samples;
@@ -81,7 +102,7 @@ __device__ inline void prep_block(const Planes& planes, const float2* weight, co
   constexpr int kIters = kPrepSamples * kPieces / (kPrepSamples * kPrepWaves);
 #pragma unroll
   for (int it = 0; it < kIters; ++it) {
-    const uint32_t i = threadIdx.x + it * kPrepSamples * kPrepWaves;
+    const uint32_t i = tid + it * kPrepSamples * kPrepWaves;
     const uint32_t sl = i / kPieces, pc = i % kPieces;
     if (sl < ns) {
       const float2 lo = tile[sl * kTs + 2 * pc], hi = tile[sl * kTs + 2 * pc + 1];
@@ -89,6 +110,14 @@ __device__ inline void prep_block(const Planes& planes, const float2* weight, co
       dst[pc] = make_float4(lo.x, lo.y, hi.x, hi.y);
     }
   }
+}
+
+template <int CP, typename Planes>
+__device__ inline void prep_block(const Planes& planes, const float2* weight, const uint32_t* rank, float2* sorted,
+                                  uint32_t samples, uint32_t coils, uint32_t block) {
+  __shared__ float2 tile[kPrepSamples * (CP + 1)];
+  __shared__ uint32_t rank_s[kPrepSamples];
+  prep_block_in<CP>(planes, weight, rank, sorted, samples, coils, block, tile, rank_s);
 }
 
 template <int CP>
@@ -390,7 +419,7 @@ __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
   constexpr int NF = CP / 8;
   constexpr int kCells = 16 * T;  // 4 x 4 (T = 1) or 4 x 8 (T = 2) cells, T products of 16 cells each
   constexpr uint32_t kKeyMask = WT ? 0x00FFFFFFu : 0xFFFFFFFFu;
-  const int lane = threadIdx.x & 63, q = lane >> 4, j = lane & 15;
+  const int lane = static_cast<int>(opaque_tid() & 63), q = lane >> 4, j = lane & 15;
   const uint32_t i0 = a.run_first[run], i1 = a.run_first[run + 1];  // run is wave-uniform: scalar loads
   if (i0 >= i1) return;
   const uint32_t nitems = i1 - i0;  // <= 64 by construction of the runs
@@ -1070,6 +1099,194 @@ void launch_gather(GatherArgs a, hipStream_t stream) {
   KSJ_LAUNCH_CHECK();
 }
 
+// ----------------------------------------------------------------------------- chain (work queue)
+// The gridding stages of an uneven batch -- prep, gather and reduce of every item, all
+// padded coil counts -- as one persistent launch fed by a device work queue.
+//  * Items are assigned to XCDs, largest first onto the least loaded one, and each XCD
+//    has its own queue of tasks: prep (kChainPrepBlocks blocks of 64 samples), gather
+//    (4 kChainRunsPerWave runs, kChainRunsPerWave per wave) and reduce
+//    (kChainReduceIters x 256 partial sums).  Queues interleave items as prep k,
+//    gather k - 1, reduce k - 2, so most dependencies are met before a task is taken.
+//  * A workgroup reads its XCD from the hardware (XCC_ID) and takes the next task of
+//    that XCD's queue with one atomic.  A gather task waits until every prep task of its
+//    item has finished, a reduce task until every gather task has.
+//  * All tasks of an item run on one XCD, so its sorted rows and partial sums pass
+//    through that XCD's L2 only: a finished task waits for its own stores (s_waitcnt)
+//    and then counts itself done with an atomic; no L2 write-back or invalidate.
+//  * Tasks are taken in queue order and wait only on earlier tasks of the same queue,
+//    which running workgroups hold, so every wait ends.  Each wait is still bounded:
+//    past `spin_limit` sleeps the workgroup marks the launch faulty (every workgroup then
+//    stops taking tasks) and raises the fault counter, so a broken queue ends the kernel
+//    with wrong results instead of hanging it.
+// Every task computes exactly what the per-stage kernels compute (same blocks, runs and
+// reduce order), so results are the same bits as the fused and per-item forms.
+constexpr uint32_t kChainPrepBlocks = 8;
+constexpr uint32_t kChainRunsPerWave = 2;
+constexpr uint32_t kChainReduceIters = 4;
+constexpr uint32_t kChainMaxSegs = 3 * kMaxFusedItems;
+constexpr uint32_t kChainSpinLimit = 1u << 22;  // sleeps of 8 x 64 cycles: about a second
+enum : uint32_t { kTaskPrep = 0, kTaskGather = 1, kTaskReduce = 2 };
+
+struct ChainItem {
+  PrepItemArgs prep;
+  GatherArgs gather;
+  uint32_t cp, prep_tasks, gather_tasks, reduce_tasks;
+};
+
+struct ChainTable {
+  uint32_t queues;                         // XCD queues in use (1..8)
+  uint32_t pad;
+  uint32_t seg_begin[kXcds + 1];           // queue x owns segments [seg_begin[x], seg_begin[x + 1])
+  uint32_t task_count[kXcds];              // tasks of each queue
+  uint32_t seg_first[kChainMaxSegs];       // first task of each segment (within its queue)
+  uint32_t seg_info[kChainMaxSegs];        // type << 16 | item
+  ChainItem items[kMaxFusedItems];         // last: only the used entries are uploaded
+};
+
+struct ChainState {
+  uint32_t head[kXcds * 32];  // next task of each queue, one 128-byte line apiece
+  uint32_t prep_done[kMaxFusedItems];
+  uint32_t gather_done[kMaxFusedItems];
+  uint32_t fault;
+};
+
+// Launches whose waits ran out (read by ksj_gpu_chain_faults).
+__device__ unsigned int g_chain_faults;
+
+__device__ inline uint32_t hw_xcc_id() {
+  return static_cast<uint32_t>(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20));  // HW_REG_XCC_ID, bits 0-3
+}
+
+__device__ inline uint32_t cword(const uint32_t* p) { return *(ConstWord)(p); }
+
+template <typename A>
+__device__ inline A const_load(const A* p) {
+  static_assert(sizeof(A) % 4 == 0, "table entries are read as 32-bit words");
+  const ConstWord src = (ConstWord)(p);
+  uint32_t w[sizeof(A) / 4];
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(A) / 4; ++i) w[i] = src[i];
+  A a;
+  __builtin_memcpy(&a, w, sizeof(A));
+  return a;
+}
+
+__global__ void chain_reset_kernel(ChainState* st) {
+  uint32_t* w = reinterpret_cast<uint32_t*>(st);
+  for (uint32_t i = threadIdx.x; i < sizeof(ChainState) / 4; i += blockDim.x) w[i] = 0;
+}
+
+__global__ void xcc_probe_kernel(uint32_t* out) {
+  if (threadIdx.x == 0) out[blockIdx.x] = hw_xcc_id();
+}
+
+template <int CPMAX, int U, int T, bool TS>
+__global__ __launch_bounds__(256) void grid_chain_kernel(const ChainTable* tab, ChainState* st, uint32_t spin_limit) {
+  extern __shared__ float2 chain_lds[];
+  __shared__ uint32_t s_task, s_ok;
+  const uint32_t nq = cword(&tab->queues);
+  const uint32_t q = hw_xcc_id() % nq;
+  const uint32_t sb = cword(&tab->seg_begin[q]), se = cword(&tab->seg_begin[q + 1]);
+  const uint32_t ntasks = cword(&tab->task_count[q]);
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (;;) {
+    if (threadIdx.x == 0) {
+      uint32_t t = ~0u;
+      if (__hip_atomic_load(&st->fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        t = __hip_atomic_fetch_add(&st->head[32 * q], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_task = t;
+    }
+    __syncthreads();
+    const uint32_t task = __builtin_amdgcn_readfirstlane(s_task);
+    if (task >= ntasks) break;
+    uint32_t lo = sb, hi = se;  // the segment holding the task: seg_first[lo] <= task < seg_first[lo + 1]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (cword(&tab->seg_first[mid]) <= task)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint32_t info = cword(&tab->seg_info[lo]);
+    const uint32_t type = info >> 16, item = info & 0xFFFFu;
+    const uint32_t idx = task - cword(&tab->seg_first[lo]);
+    const ChainItem* it = &tab->items[item];
+    if (type != kTaskPrep) {
+      if (threadIdx.x == 0) {
+        const uint32_t* ctr = type == kTaskGather ? &st->prep_done[item] : &st->gather_done[item];
+        const uint32_t need = type == kTaskGather ? cword(&it->prep_tasks) : cword(&it->gather_tasks);
+        uint32_t n = 0;
+        bool ok = true, timed_out = false;
+        while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+          if (__hip_atomic_load(&st->fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            ok = false;  // another workgroup's wait ran out: stop
+            break;
+          }
+          if (++n > spin_limit) {
+            ok = false;
+            timed_out = true;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(8);
+        }
+        if (timed_out && __hip_atomic_exchange(&st->fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+          __hip_atomic_fetch_add(&g_chain_faults, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_ok = ok ? 1u : 0u;
+      }
+      __syncthreads();
+      if (__builtin_amdgcn_readfirstlane(s_ok) == 0) break;
+    }
+    const uint32_t cp = cword(&it->cp);
+    if (type == kTaskPrep) {
+      const PrepItemArgs a = const_load(&it->prep);
+      const uint32_t nb = (a.samples + kPrepSamples - 1) / kPrepSamples;
+      const uint32_t b0 = idx * kChainPrepBlocks, b1 = min(b0 + kChainPrepBlocks, nb);
+      auto* rank_s = reinterpret_cast<uint32_t*>(chain_lds + kPrepSamples * (cp + 1));
+      for (uint32_t b = b0; b < b1; ++b) {
+        if (cp == 8)
+          prep_block_in<8>(a.planes, a.weight, a.rank, a.sorted, a.samples, a.coils, b, chain_lds, rank_s);
+        else if (cp == 16)
+          prep_block_in<16>(a.planes, a.weight, a.rank, a.sorted, a.samples, a.coils, b, chain_lds, rank_s);
+        else if constexpr (CPMAX >= 32) {
+          if (cp == 32)
+            prep_block_in<32>(a.planes, a.weight, a.rank, a.sorted, a.samples, a.coils, b, chain_lds, rank_s);
+          else if constexpr (CPMAX >= 64)
+            prep_block_in<64>(a.planes, a.weight, a.rank, a.sorted, a.samples, a.coils, b, chain_lds, rank_s);
+        }
+        __syncthreads();  // the tile is read out before the next block writes it
+      }
+    } else if (type == kTaskGather) {
+      const GatherArgs a = const_load(&it->gather);
+#pragma unroll 1
+      for (uint32_t r = 0; r < kChainRunsPerWave; ++r) {
+        const uint32_t run = (idx * kChainRunsPerWave + r) * 4 + wave;
+        if (run >= a.run_count) break;
+        if (cp == 8)
+          matrix_run<8, U, 1, T, true, TS>(a, run);
+        else if (cp == 16)
+          matrix_run<16, U, 1, T, true, TS>(a, run);
+        else if constexpr (CPMAX >= 32) {
+          if (cp == 32)
+            matrix_run<32, U, 1, T, true, TS>(a, run);
+          else if constexpr (CPMAX >= 64)
+            matrix_run<64, U, 1, T, true, TS>(a, run);
+        }
+      }
+    } else {
+      const GatherArgs a = const_load(&it->gather);
+#pragma unroll 1
+      for (uint32_t i = 0; i < kChainReduceIters; ++i)
+        multi_reduce_one(a, (static_cast<uint64_t>(idx) * kChainReduceIters + i) * 256u + threadIdx.x);
+    }
+    // Done: this workgroup's stores have reached the XCD's L2, then one count.
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0 && type != kTaskReduce)
+      __hip_atomic_fetch_add(type == kTaskPrep ? &st->prep_done[item] : &st->gather_done[item], 1u, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 }  // namespace
 
 uint32_t padded_coils(uint32_t coils) {
@@ -1285,6 +1502,194 @@ void grid_fused(const GridFusedItem* items, uint32_t count, void* d_tables, hipS
                        static_cast<const FusedTable<GatherArgs>*>(d));
     KSJ_LAUNCH_CHECK();
   }
+}
+
+// ----------------------------------------------------------------------------- chain form (host)
+namespace {
+
+constexpr int kMaxChainDevices = 64;
+std::mutex g_chain_mu;
+int g_chain_queues[kMaxChainDevices] = {};  // 0 not probed, -1 unusable, else XCD queues
+
+// XCC ids seen by 2048 one-wave blocks: the chain needs them to be exactly 0 .. n - 1
+// for n in {1, 2, 4, 8} (every queue then has workgroups).
+int probe_chain_queues() {
+  constexpr uint32_t kBlocks = 2048;
+  uint32_t* d = nullptr;
+  hipStream_t s = nullptr;
+  KSJ_HIP(hipMalloc(&d, kBlocks * sizeof(uint32_t)));
+  std::vector<uint32_t> h(kBlocks, 0xFFFFFFFFu);
+  try {
+    KSJ_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipLaunchKernelGGL(xcc_probe_kernel, dim3(kBlocks), dim3(64), 0, s, d);
+    KSJ_LAUNCH_CHECK();
+    KSJ_HIP(hipMemcpyAsync(h.data(), d, kBlocks * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    KSJ_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    if (s) (void)hipStreamDestroy(s);
+    (void)hipFree(d);
+    throw;
+  }
+  (void)hipStreamDestroy(s);
+  (void)hipFree(d);
+  uint32_t seen = 0;
+  for (uint32_t v : h) {
+    if (v >= 16) return -1;
+    seen |= 1u << v;
+  }
+  for (int n : {8, 4, 2, 1})
+    if (seen == (1u << n) - 1) return n;
+  return -1;
+}
+
+// Queues of the chain on the current device (0: the chain is not usable).  The probe
+// synchronises, so it is not run while `stream` is being captured.
+uint32_t chain_queues(hipStream_t stream) {
+  int dev = 0;
+  KSJ_HIP(hipGetDevice(&dev));
+  require(dev >= 0 && dev < kMaxChainDevices, "device ordinal beyond the chain table");
+  std::lock_guard<std::mutex> lock(g_chain_mu);
+  if (g_chain_queues[dev] == 0) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (stream != nullptr) KSJ_HIP(hipStreamIsCapturing(stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) return 0;
+    g_chain_queues[dev] = probe_chain_queues();
+  }
+  return g_chain_queues[dev] > 0 ? static_cast<uint32_t>(g_chain_queues[dev]) : 0u;
+}
+
+constexpr size_t chain_table_bytes() { return (sizeof(ChainTable) + 255) & ~static_cast<size_t>(255); }
+constexpr size_t chain_state_bytes() { return (sizeof(ChainState) + 255) & ~static_cast<size_t>(255); }
+
+template <int CPMAX, int T, bool TS>
+void launch_chain(const ChainTable* t, ChainState* st, hipStream_t stream) {
+  constexpr size_t lds = prep_lds_bytes<CPMAX>();
+  static std::atomic<int> per_cu{0};
+  int n = per_cu.load(std::memory_order_relaxed);
+  if (n == 0) {
+    KSJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grid_chain_kernel<CPMAX, 4, T, TS>, 256, lds));
+    n = std::max(n, 1);
+    per_cu.store(n, std::memory_order_relaxed);
+  }
+  const dim3 grid(static_cast<uint32_t>(n) * device_cu_count());
+  hipLaunchKernelGGL((grid_chain_kernel<CPMAX, 4, T, TS>), grid, dim3(256), lds, stream, t, st, kChainSpinLimit);
+  KSJ_LAUNCH_CHECK();
+}
+
+template <int CPMAX>
+void launch_chain_cp(uint32_t block_w, bool share, const ChainTable* t, ChainState* st, hipStream_t stream) {
+  if (block_w == 8) {
+    if (share) launch_chain<CPMAX, 2, true>(t, st, stream); else launch_chain<CPMAX, 2, false>(t, st, stream);
+  } else {
+    if (share) launch_chain<CPMAX, 1, true>(t, st, stream); else launch_chain<CPMAX, 1, false>(t, st, stream);
+  }
+}
+
+}  // namespace
+
+void grid_chain_prepare(hipStream_t stream) { (void)chain_queues(stream); }
+
+uint64_t grid_chain_faults() {
+  unsigned int v = 0;
+  KSJ_HIP(hipMemcpyFromSymbol(&v, HIP_SYMBOL(g_chain_faults), sizeof(v), 0, hipMemcpyDeviceToHost));
+  return v;
+}
+
+size_t grid_chain_table_bytes() { return chain_table_bytes() + chain_state_bytes(); }
+
+bool grid_chain_supported(const GridFusedItem* items, uint32_t count, hipStream_t stream) {
+  if (count < 1 || count > kMaxFusedItems) return false;
+  // The compiled chain shapes: packed visits, 16 visits per batch.
+  if (tune_get(kTuneMatrixBatch) != 16) return false;
+  uint32_t bw = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    const Plan2* p = items[i].plan;
+    if (p == nullptr || !p->has_trajectory || p->scheme != kSchemeMatrix || !p->packed_visits) return false;
+    if (items[i].channels < 1 || items[i].channels > kMaxStackedCoils) return false;
+    if (bw != 0 && bw != p->block_w) return false;
+    bw = p->block_w;
+  }
+  return chain_queues(stream) > 0;
+}
+
+void grid_chain(const GridFusedItem* items, uint32_t count, void* d_tables, hipStream_t stream) {
+  require(grid_chain_supported(items, count, stream), "the gridding chain takes 1..64 packed matrix-schedule items");
+  const uint32_t nq = chain_queues(stream);
+  auto t = std::make_unique<ChainTable>();
+  std::memset(t.get(), 0, sizeof(ChainTable));
+  t->queues = nq;
+  const uint32_t nt_store = static_cast<uint32_t>(tune_get(kTuneMatrixNtStore));
+  std::vector<double> cost(count);
+  uint32_t cpmax = 8;
+  for (uint32_t i = 0; i < count; ++i) {
+    const GridFusedItem& it = items[i];
+    require(it.plan->multi_count == 0 || it.partial != nullptr, "multi-item blocks need a partial-sum buffer");
+    const uint32_t cp = padded_coils(it.channels);
+    cpmax = std::max(cpmax, cp);
+    ChainItem& c = t->items[i];
+    c.prep.planes = StridedPlanes{it.kspace, it.plan->d.samples};
+    c.prep.weight = it.plan->weight;
+    c.prep.rank = it.plan->rank;
+    c.prep.sorted = it.sorted;
+    c.prep.samples = it.plan->d.samples;
+    c.prep.coils = it.channels;
+    c.gather = gather_args(*it.plan, it.sorted, it.channels, it.grid, it.partial);
+    c.gather.xcd_chunk = 0;
+    c.gather.nt_store = nt_store;
+    c.cp = cp;
+    c.prep_tasks = ceil_div(ceil_div(c.prep.samples, kPrepSamples), kChainPrepBlocks);
+    c.gather_tasks = ceil_div(c.gather.run_count, 4 * kChainRunsPerWave);
+    c.reduce_tasks = ceil_div(static_cast<uint64_t>(c.gather.multi_count) * c.gather.bh * c.gather.bw * cp,
+                              256ull * kChainReduceIters);
+    const double g2 = static_cast<double>(it.plan->d.grid_rows) * it.plan->d.grid_cols;
+    cost[i] = (24.0 * it.plan->d.samples + 8.0 * g2) * cp;
+  }
+  // Largest first onto the least loaded XCD.
+  std::vector<uint32_t> order(count);
+  for (uint32_t i = 0; i < count; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+  std::vector<double> load(nq, 0.0);
+  std::vector<std::vector<uint32_t>> on(nq);
+  for (uint32_t i : order) {
+    const uint32_t x = static_cast<uint32_t>(std::min_element(load.begin(), load.end()) - load.begin());
+    load[x] += cost[i];
+    on[x].push_back(i);
+  }
+  uint32_t seg = 0;
+  for (uint32_t x = 0; x < nq; ++x) {
+    t->seg_begin[x] = seg;
+    uint32_t task = 0;
+    auto add = [&](uint32_t type, uint32_t item, uint32_t n) {
+      if (n == 0) return;
+      t->seg_first[seg] = task;
+      t->seg_info[seg] = type << 16 | item;
+      ++seg;
+      task += n;
+    };
+    const auto& L = on[x];
+    const size_t n = L.size();
+    for (size_t k = 0; k < n + 2; ++k) {  // prep k, gather k - 1, reduce k - 2
+      if (k < n) add(kTaskPrep, L[k], t->items[L[k]].prep_tasks);
+      if (k >= 1 && k - 1 < n) add(kTaskGather, L[k - 1], t->items[L[k - 1]].gather_tasks);
+      if (k >= 2 && k - 2 < n) add(kTaskReduce, L[k - 2], t->items[L[k - 2]].reduce_tasks);
+    }
+    t->task_count[x] = task;
+  }
+  for (uint32_t x = nq; x <= kXcds; ++x) t->seg_begin[x] = seg;
+  const size_t used = offsetof(ChainTable, items) + count * sizeof(ChainItem);
+  upload_table(d_tables, t.get(), (used + 3) & ~static_cast<size_t>(3), stream);
+  auto* tab = static_cast<const ChainTable*>(d_tables);
+  auto* st = reinterpret_cast<ChainState*>(static_cast<char*>(d_tables) + chain_table_bytes());
+  hipLaunchKernelGGL(chain_reset_kernel, dim3(1), dim3(256), 0, stream, st);
+  KSJ_LAUNCH_CHECK();
+  const uint32_t bw = items[0].plan->block_w;
+  const bool share = tune_get(kTuneMatrixTapShare) == 1;
+  if (cpmax <= 16)
+    launch_chain_cp<16>(bw, share, tab, st, stream);
+  else if (cpmax == 32)
+    launch_chain_cp<32>(bw, share, tab, st, stream);
+  else
+    launch_chain_cp<64>(bw, share, tab, st, stream);
 }
 
 }  // namespace ksj::gpu

@@ -53,4 +53,16 @@ size_t grid_fused_table_bytes();
 // count; every item's results equal grid_prep + grid_gather + grid_reduce on it.
 void grid_fused(const GridFusedItem* items, uint32_t count, void* d_tables, hipStream_t stream);
 
+// The same three stages as one persistent launch fed by a per-XCD device work queue
+// (every item's tasks on one XCD; waits bounded).  Same bits as grid_fused.  Takes
+// matrix-schedule plans with packed visits of one block width, with 16 visits per batch;
+// grid_chain_supported says whether `items` qualify and the device's XCD ids allow it.
+size_t grid_chain_table_bytes();
+bool grid_chain_supported(const GridFusedItem* items, uint32_t count, hipStream_t stream);
+void grid_chain(const GridFusedItem* items, uint32_t count, void* d_tables, hipStream_t stream);
+// Probes the current device's XCD ids once (it synchronises; not while `stream` is captured).
+void grid_chain_prepare(hipStream_t stream);
+// Chain launches on the current device whose bounded waits ran out (0 in correct runs).
+uint64_t grid_chain_faults();
+
 }  // namespace ksj::gpu

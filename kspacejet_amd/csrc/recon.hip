@@ -392,8 +392,8 @@ bool fused_eligible(const MultiItem* items, uint32_t count) {
 }
 
 size_t fused_tables_bytes(uint32_t count) {
-  return align256(grid_fused_table_bytes()) + 2 * align256(fft_fused_table_bytes(count)) +
-         align256(coil_split_fused_table_bytes());
+  return align256(std::max(grid_fused_table_bytes(), grid_chain_table_bytes())) +
+         2 * align256(fft_fused_table_bytes(count)) + align256(coil_split_fused_table_bytes());
 }
 
 size_t fused_total_bytes(const MultiItem* items, uint32_t count) {
@@ -428,7 +428,7 @@ size_t lanes_total_bytes(const MultiItem* items, uint32_t count) {
 void execute_fused(const MultiItem* items, uint32_t count, int layout, void* d_ws, hipStream_t stream) {
   char* p = static_cast<char*>(d_ws);
   char* grid_tab = p;
-  char* row_tab = grid_tab + align256(grid_fused_table_bytes());
+  char* row_tab = grid_tab + align256(std::max(grid_fused_table_bytes(), grid_chain_table_bytes()));
   char* col_tab = row_tab + align256(fft_fused_table_bytes(count));
   char* split_tab = col_tab + align256(fft_fused_table_bytes(count));
   p += fused_tables_bytes(count);
@@ -465,7 +465,10 @@ void execute_fused(const MultiItem* items, uint32_t count, int layout, void* d_w
     pixels[i] = static_cast<uint64_t>(d.image_rows) * d.image_cols;
     coils[i] = c;
   }
-  {
+  if (tune_get(kTuneMultiFused) == 2 && grid_chain_supported(gi.data(), count, stream)) {
+    ProfScope ps("multi_grid_chain", stream);
+    grid_chain(gi.data(), count, grid_tab, stream);
+  } else {
     ProfScope ps("multi_grid", stream);
     grid_fused(gi.data(), count, grid_tab, stream);
   }

@@ -29,7 +29,7 @@ Param g_params[kTuneKeys] = {
     {"tile_width", 4, [](int64_t v) { return v == 4 || v == 8; }},
     {"block_group", 2, [](int64_t v) { return v == 1 || v == 2 || v == 4; }},
     {"fft_schedule", 0, [](int64_t v) { return v >= 0 && v <= 3; }},
-    {"multi_fused", 0, [](int64_t v) { return v == 0 || v == 1; }},
+    {"multi_fused", 0, [](int64_t v) { return v >= 0 && v <= 2; }},
     {"matrix_run_cost", 12, [](int64_t v) { return v >= 1 && v <= 64; }},
     {"matrix_pipeline", 1, [](int64_t v) { return v == 1 || v == 2; }},
     {"matrix_nt_store", 1, [](int64_t v) { return v == 0 || v == 1; }},

@@ -76,6 +76,29 @@ def tune_get(key: str) -> int:
     return int(v.value)
 
 
+def chain_faults() -> int:
+    """Launches of the uneven batch's work-queue form (multi_fused = 2) on the current
+    device whose bounded waits ran out; 0 in correct operation."""
+    v = C.c_uint64(0)
+    call("ksj_gpu_chain_faults", C.byref(v))
+    return int(v.value)
+
+
+def profile_enable(on: bool) -> None:
+    call("ksj_gpu_profile_enable", 1 if on else 0)
+
+
+def profile_reset() -> None:
+    call("ksj_gpu_profile_reset")
+
+
+def profile_read(stage: str):
+    """(total ms, launches) recorded for a stage since the last reset."""
+    tot, cnt = C.c_double(0.0), C.c_uint64(0)
+    call("ksj_gpu_profile_read", stage.encode(), C.byref(tot), C.byref(cnt))
+    return float(tot.value), int(cnt.value)
+
+
 def ifft2_coil_images(kspace: torch.Tensor, centred: bool = False, layout: str = "coil_last") -> torch.Tensor:
     """[C, R, Cols] complex64 channel-major k-space -> coil images (1/R/Cols scaling)."""
     if kspace.dim() != 3:

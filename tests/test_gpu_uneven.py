@@ -1,6 +1,7 @@
 """Uneven batches: items of different matrix, coil count and trajectory in one
 multi-plan call (ksj_gpu_adjoint2_execute_multi), fused (one launch per stage and
-shape class) or over lanes, eagerly and as a captured HIP graph.  Each item must
+shape class, with or without the gridding work queue) or over lanes, eagerly and as a
+captured HIP graph.  Each item must
 equal its single-plan result bit for bit and the FP64 oracle to rel-L2 <= 1e-5; every
 graph replay must equal the eager result."""
 import numpy as np
@@ -73,14 +74,14 @@ def test_items_match_single_plans_and_the_oracle(ops, batch):
 
 
 def test_fused_and_lane_forms_give_identical_results(ops, batch):
-    """One launch per stage and shape class (multi_fused = 1) and items spread over
-    lanes (0) run the same per-item arithmetic: bit-identical images and RSS, in both
-    image layouts."""
+    """The work-queue chain (multi_fused = 2), one launch per stage and shape class (1)
+    and items spread over lanes (0) run the same per-item arithmetic: bit-identical
+    images and RSS, in both image layouts."""
     plans, ks, _ = batch
     saved = ops.tune_get("multi_fused")
     results = []
     try:
-        for fused in (1, 0):
+        for fused in (2, 1, 0):
             ops.tune_set("multi_fused", fused)
             for layout in ("coil_last", "coil_major"):
                 rss = [torch.zeros((p.desc.image_rows, p.desc.image_cols), dtype=torch.float32, device="cuda")
@@ -95,8 +96,50 @@ def test_fused_and_lane_forms_give_identical_results(ops, batch):
                 results.append([r.cpu().numpy() for r in rss] + [m.cpu().numpy() for m in imgs])
     finally:
         ops.tune_set("multi_fused", saved)
-    for a, b in zip(results[:2], results[2:]):
-        assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    for form in (1, 2):
+        for a, b in zip(results[:2], results[2 * form:2 * form + 2]):
+            assert all(np.array_equal(x, y) for x, y in zip(a, b)), form
+    assert ops.chain_faults() == 0
+
+
+def test_chain_form_runs_its_work_queue(ops, batch, request):
+    """multi_fused = 2 grids matrix-schedule items in the work-queue launch (its stage is
+    recorded as multi_grid_chain) and falls back to the per-class launches for the 2x2-block
+    schedule; no bounded wait runs out.  A single item goes through the chain too."""
+    plans, ks, _ = batch
+    matrix = request.node.callspec.id == "matrix"
+    saved = ops.tune_get("multi_fused"), ops.tune_get("multi_lanes")
+    ops.tune_set("multi_fused", 2)
+    try:
+        for lanes in (1, 3):
+            ops.tune_set("multi_lanes", lanes)
+            ops.profile_reset()
+            ops.profile_enable(True)
+            rss, imgs = outputs(batch)
+            ws = torch.empty(ops.adjoint_multi_workspace_bytes(plans, [k.shape[0] for k in ks]), dtype=torch.uint8,
+                             device="cuda")
+            ops.adjoint_multi(plans, ks, rss=rss, images=imgs, workspace=ws)
+            torch.cuda.synchronize()
+            ops.profile_enable(False)
+            chain = ops.profile_read("multi_grid_chain")[1]
+            staged = ops.profile_read("multi_grid")[1]
+            assert (chain, staged) == ((lanes, 0) if matrix else (0, lanes)), (lanes, chain, staged)
+            for i, (plan, k) in enumerate(zip(plans, ks)):
+                img1, rss1 = plan.adjoint(k, rss=True)
+                assert torch.equal(imgs[i], img1) and torch.equal(rss[i], rss1), (lanes, i)
+        one_rss = [torch.zeros_like(rss[2])]
+        one_img = [torch.zeros_like(imgs[2])]
+        ws = torch.empty(ops.adjoint_multi_workspace_bytes(plans[2:3], [ks[2].shape[0]]), dtype=torch.uint8,
+                         device="cuda")
+        ops.adjoint_multi(plans[2:3], ks[2:3], rss=one_rss, images=one_img, workspace=ws)
+        torch.cuda.synchronize()
+        img1, rss1 = plans[2].adjoint(ks[2], rss=True)
+        assert torch.equal(one_img[0], img1) and torch.equal(one_rss[0], rss1)
+    finally:
+        ops.profile_enable(False)
+        ops.tune_set("multi_fused", saved[0])
+        ops.tune_set("multi_lanes", saved[1])
+    assert ops.chain_faults() == 0
 
 
 def test_item_lanes_do_not_change_results(ops, batch):
@@ -120,7 +163,7 @@ def test_item_lanes_do_not_change_results(ops, batch):
         assert all(np.array_equal(a, b) for a, b in zip(res, results[0]))
 
 
-@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("fused", [0, 1, 2])
 def test_captured_graph_replays_equal_the_eager_result(ops, batch, fused):
     plans, ks, _ = batch
     saved = ops.tune_get("multi_fused")

@@ -2,8 +2,9 @@
 128/192/256/320, so grids of 256/384/512/640; 8-32 coils, so every padded coil class
 8/16/32; 101-603 golden-angle spokes, readout 2 x matrix), simulated as the benchmark
 simulates them (ellipse phantom x Gaussian coils, exact FP64 forward DFT, noise
-1e-3 max|k|).  One multi-plan call runs them in each form, fused (one launch per stage
-and shape class) and over lanes, eagerly and as replays of a captured HIP graph:
+1e-3 max|k|).  One multi-plan call runs them in each form -- the gridding work queue
+(one persistent launch for every item's prep, gather and reduce), fused (one launch per
+stage and shape class) and over lanes -- eagerly and as replays of a captured HIP graph:
 
 * every item equals its single-plan result bit for bit, in both forms;
 * two coils of every item (the first and the last) are within rel-L2 1e-5 of the FP64
@@ -103,12 +104,14 @@ def test_every_shape_class_is_present():
 def test_forms_match_single_plans_and_the_oracle(ops, data, plans):
     import concurrent.futures as cf
     shapes, trajs, ks = data
+    chain = run_multi(ops, plans, ks, shapes, 2)
     fused = run_multi(ops, plans, ks, shapes, 1)
     lanes = run_multi(ops, plans, ks, shapes, 0)
     for i, (plan, k) in enumerate(zip(plans, ks)):
         img1, rss1 = plan.adjoint(k, rss=True)
-        for rss, imgs in (fused, lanes):
+        for rss, imgs in (chain, fused, lanes):
             assert torch.equal(imgs[i], img1) and torch.equal(rss[i], rss1), i
+    assert ops.chain_faults() == 0
     torch.cuda.synchronize()
     jobs = [(i, c) for i, (_, cc, _) in enumerate(shapes) for c in (0, cc - 1)]
     with cf.ThreadPoolExecutor(max_workers=16) as ex:
@@ -123,7 +126,7 @@ def test_forms_match_single_plans_and_the_oracle(ops, data, plans):
     print(f"worst rel-L2 over {len(jobs)} coil images: {worst:.2e}")
 
 
-@pytest.mark.parametrize("fused", [1, 0])
+@pytest.mark.parametrize("fused", [2, 1, 0])
 def test_graph_replays_equal_the_eager_result(ops, data, plans, fused):
     shapes, _, ks = data
     saved = ops.tune_get("multi_fused")
