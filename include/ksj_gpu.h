@@ -320,6 +320,17 @@ int ksj_gpu_coil_compression_basis_estimate(const void* d_samples, uint32_t phys
                                             uint32_t virtual_channels, uint64_t samples, void* d_basis,
                                             void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* Sensitivity maps [rows][cols][channels] (ksj.sensitivity-map) from coil-last coil images
+ * (ksj.coil-image-frame) by Walsh's adaptive method: at each pixel the dominant eigenvector
+ * of the coil covariance summed over a window x window neighbourhood (odd, <= 31, clipped
+ * at the image edges), by `iterations` (<= 64) normalised power-iteration steps from
+ * R e_0 in FP32, rotated so that coil 0 is real and >= 0; zero where the neighbourhood is
+ * zero.  For the planned interface providers/interfaces/org.kspacejet.calibration/
+ * sensitivity_map_estimate.json (no reference implementation); its maps feed
+ * ksj_gpu_sense_combine (coil_combine_adaptive.json).  Asynchronous. */
+int ksj_gpu_sensitivity_maps_walsh(const void* d_coil_images, uint32_t rows, uint32_t cols, uint32_t channels,
+                                   uint32_t window, uint32_t iterations, void* d_maps, void* stream);
+
 /* Readout phase model p[c][k] = conj(s) / |s|, s = sum over reference lines of
  * ref[c][line][k] in float in line order (1 + 0i where s = 0), |s| = the FP64 square root
  * of re^2 + im^2 rounded to float.  Replaces phase_correction_estimate.cpp:141-163.

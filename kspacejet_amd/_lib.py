@@ -124,6 +124,7 @@ SIGNATURES = {
     "ksj_gpu_noise_model_estimate": (_int, [_vp, _u32, _u64, C.c_float, _vp, _vp, _sz, _vp]),
     "ksj_gpu_coil_compression_basis_estimate": (_int, [_vp, _u32, _u32, _u64, _vp, _vp, _sz, _vp]),
     "ksj_gpu_phase_correction_estimate": (_int, [_vp, _u32, _u32, _u32, _vp, _vp]),
+    "ksj_gpu_sensitivity_maps_walsh": (_int, [_vp, _u32, _u32, _u32, _u32, _u32, _vp, _vp]),
     "ksj_gpu_plan3_create": (_int, [C.POINTER(Plan3Desc), C.POINTER(_vp)]),
     "ksj_gpu_plan3_set_trajectory": (_int, [_vp, _vp, _vp, _vp]),
     "ksj_gpu_plan3_get_info": (_int, [_vp, C.POINTER(Plan3Info)]),

@@ -416,6 +416,72 @@ uint32_t read_status(const uint32_t* d_status, hipStream_t stream) {
   return s;
 }
 
+// Walsh's adaptive sensitivity maps: CP lanes per pixel (lane c holds coil c; lanes past
+// the coil count hold zeros), 64 / CP pixels per wave.  The local covariance is never
+// formed: R v = sum over the window of I (I^H v), with I^H v summed across the pixel's
+// lanes by an xor butterfly (every lane ends with the same bits: each step adds the
+// same two values in either order).  Window offsets run in the same order for every
+// pixel, with out-of-image neighbours read as zero, so the shuffles run on full waves.
+template <int CP>
+__device__ inline float group_sum(float s) {
+#pragma unroll
+  for (int m = 1; m < CP; m <<= 1) s += __shfl_xor(s, m, CP);
+  return s;
+}
+
+template <int CP>
+__device__ inline float2 group_normalise(float2 v) {
+  const float n2 = group_sum<CP>(fmaf(v.x, v.x, v.y * v.y));
+  const float inv = n2 > 0.f ? 1.0f / sqrtf(n2) : 0.f;
+  return make_float2(v.x * inv, v.y * inv);
+}
+
+template <int CP>
+__global__ __launch_bounds__(256) void walsh_kernel(const float2* __restrict__ img, float2* __restrict__ maps,
+                                                    uint32_t rows, uint32_t cols, uint32_t coils, int r,
+                                                    uint32_t iterations) {
+  constexpr uint32_t kPerWave = 64 / CP;
+  const uint32_t lane = threadIdx.x & 63, c = lane % CP, g = lane / CP;
+  const uint64_t p = (static_cast<uint64_t>(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64) * kPerWave + g;
+  const uint64_t npix = static_cast<uint64_t>(rows) * cols;
+  const bool live = p < npix;
+  const int y = live ? static_cast<int>(p / cols) : 0, x = live ? static_cast<int>(p % cols) : 0;
+  auto load = [&](int dy, int dx) {
+    const int yy = y + dy, xx = x + dx;
+    const bool ok = live && c < coils && yy >= 0 && yy < static_cast<int>(rows) && xx >= 0 &&
+                    xx < static_cast<int>(cols);
+    return ok ? img[(static_cast<uint64_t>(yy) * cols + xx) * coils + c] : make_float2(0.f, 0.f);
+  };
+  // v = R e_0 = sum I conj(I_0)
+  float2 v = make_float2(0.f, 0.f);
+  for (int dy = -r; dy <= r; ++dy)
+    for (int dx = -r; dx <= r; ++dx) {
+      const float2 a = load(dy, dx);
+      const float2 a0 = make_float2(__shfl(a.x, 0, CP), __shfl(a.y, 0, CP));
+      v.x = fmaf(a.x, a0.x, fmaf(a.y, a0.y, v.x));
+      v.y = fmaf(a.y, a0.x, fmaf(-a.x, a0.y, v.y));
+    }
+  v = group_normalise<CP>(v);
+  for (uint32_t it = 0; it < iterations; ++it) {
+    float2 u = make_float2(0.f, 0.f);
+    for (int dy = -r; dy <= r; ++dy)
+      for (int dx = -r; dx <= r; ++dx) {
+        const float2 a = load(dy, dx);
+        // d = I^H v, then u += I d
+        const float dr = group_sum<CP>(fmaf(a.x, v.x, a.y * v.y));
+        const float di = group_sum<CP>(fmaf(a.x, v.y, -(a.y * v.x)));
+        u.x = fmaf(a.x, dr, fmaf(-a.y, di, u.x));
+        u.y = fmaf(a.x, di, fmaf(a.y, dr, u.y));
+      }
+    v = group_normalise<CP>(u);
+  }
+  // coil 0 real and >= 0
+  const float2 v0 = make_float2(__shfl(v.x, 0, CP), __shfl(v.y, 0, CP));
+  const float m = sqrtf(fmaf(v0.x, v0.x, v0.y * v0.y));
+  if (m > 0.f) v = make_float2(fmaf(v.x, v0.x, v.y * v0.y) / m, fmaf(v.y, v0.x, -(v.x * v0.y)) / m);
+  if (live && c < coils) maps[p * coils + c] = v;
+}
+
 void eigen_of_covariance(const float2* d_x, uint32_t channels, uint64_t samples, int kind, void* d_ws,
                          size_t ws_bytes, hipStream_t stream, Workspace& w) {
   channel_covariance(d_x, channels, samples, kind, nullptr, d_ws, ws_bytes, stream);
@@ -503,6 +569,31 @@ void phase_correction_estimate(const float2* d_ref, uint32_t channels, uint32_t 
   require(d_ref != nullptr && d_phase != nullptr, "null argument");
   hipLaunchKernelGGL(phase_estimate_kernel, dim3(ceil_div(readout, 256), channels), dim3(256), 0, stream, d_ref,
                      channels, lines, readout, d_phase);
+  KSJ_LAUNCH_CHECK();
+}
+
+void sensitivity_maps_walsh(const float2* d_images, uint32_t rows, uint32_t cols, uint32_t coils, uint32_t window,
+                            uint32_t iterations, float2* d_maps, hipStream_t stream) {
+  require(d_images != nullptr && d_maps != nullptr, "null argument");
+  require(rows >= 1 && cols >= 1 && static_cast<uint64_t>(rows) * cols <= (uint64_t{1} << 26),
+          "sensitivity maps take 1 .. 2^26 pixels");
+  require(coils >= 1 && coils <= kCalibMaxChannels, "sensitivity maps take 1..64 coils");
+  require(window % 2 == 1 && window <= 31, "the sensitivity window is an odd side of at most 31 pixels");
+  require(iterations <= 64, "at most 64 power iterations");
+  uint32_t cp = 1;
+  while (cp < coils) cp *= 2;
+  const uint64_t npix = static_cast<uint64_t>(rows) * cols;
+  const dim3 block(256), grid(static_cast<uint32_t>((npix + 4 * (64 / cp) - 1) / (4 * (64 / cp))));
+  const int r = static_cast<int>(window / 2);
+  switch (cp) {
+    case 1: hipLaunchKernelGGL(walsh_kernel<1>, grid, block, 0, stream, d_images, d_maps, rows, cols, coils, r, iterations); break;
+    case 2: hipLaunchKernelGGL(walsh_kernel<2>, grid, block, 0, stream, d_images, d_maps, rows, cols, coils, r, iterations); break;
+    case 4: hipLaunchKernelGGL(walsh_kernel<4>, grid, block, 0, stream, d_images, d_maps, rows, cols, coils, r, iterations); break;
+    case 8: hipLaunchKernelGGL(walsh_kernel<8>, grid, block, 0, stream, d_images, d_maps, rows, cols, coils, r, iterations); break;
+    case 16: hipLaunchKernelGGL(walsh_kernel<16>, grid, block, 0, stream, d_images, d_maps, rows, cols, coils, r, iterations); break;
+    case 32: hipLaunchKernelGGL(walsh_kernel<32>, grid, block, 0, stream, d_images, d_maps, rows, cols, coils, r, iterations); break;
+    default: hipLaunchKernelGGL(walsh_kernel<64>, grid, block, 0, stream, d_images, d_maps, rows, cols, coils, r, iterations); break;
+  }
   KSJ_LAUNCH_CHECK();
 }
 

@@ -46,4 +46,12 @@ void coil_compression_basis_estimate(const float2* d_x, uint32_t physical, uint3
 void phase_correction_estimate(const float2* d_ref, uint32_t channels, uint32_t lines, uint32_t readout,
                                float2* d_phase, hipStream_t stream);
 
+// Sensitivity maps by Walsh's adaptive method (Walsh, Gmitro & Marcellin, MRM 2000): at
+// each pixel the dominant eigenvector of the coil covariance summed over a window x window
+// neighbourhood (clipped at the edges), by power iteration from R e_0 (`iterations`
+// normalised steps) in FP32, rotated so that coil 0 is real and >= 0; zero where the
+// neighbourhood is zero.  Coil-last images and maps [rows][cols][coils].  Asynchronous.
+void sensitivity_maps_walsh(const float2* d_images, uint32_t rows, uint32_t cols, uint32_t coils, uint32_t window,
+                            uint32_t iterations, float2* d_maps, hipStream_t stream);
+
 }  // namespace ksj::gpu

@@ -345,6 +345,14 @@ int ksj_gpu_coil_compression_basis_estimate(const void* d_samples, uint32_t phys
   });
 }
 
+int ksj_gpu_sensitivity_maps_walsh(const void* d_coil_images, uint32_t rows, uint32_t cols, uint32_t channels,
+                                   uint32_t window, uint32_t iterations, void* d_maps, void* stream) {
+  return guarded([&] {
+    ksj::gpu::sensitivity_maps_walsh(static_cast<const float2*>(d_coil_images), rows, cols, channels, window,
+                                     iterations, static_cast<float2*>(d_maps), as_stream(stream));
+  });
+}
+
 int ksj_gpu_phase_correction_estimate(const void* d_reference, uint32_t channels, uint32_t lines, uint32_t readout,
                                       void* d_phase, void* stream) {
   return guarded([&] {

@@ -307,7 +307,8 @@ __device__ inline void fft_lds_block(const FftArgs& a, const float2* __restrict_
     const uint64_t line = first + ly;
     src_base[ly] = line < total ? line_base(a.src, line) : ~0ull;
     dst_base[ly] = line < total ? line_base(a.dst, line) : ~0ull;
-    lw[ly] = a.line_weight ? a.line_weight[line % a.src.lines] : 1.0f;
+    const uint64_t wi = a.line_weight_by_plane ? line / a.src.lines : line % a.src.lines;
+    lw[ly] = (a.line_weight && line < total) ? a.line_weight[wi] : 1.0f;
   }
   __syncthreads();
   load_lines<N, V>(a, buf, LS, lpb_log2, src_base);
@@ -605,7 +606,7 @@ __global__ __launch_bounds__(256) void dft_direct_kernel(FftArgs a, const float2
     if (bin >= static_cast<uint32_t>(n)) bin -= n;
     float w = a.scale;
     if (a.out_weight) w *= a.out_weight[e];
-    if (a.line_weight) w *= a.line_weight[line % a.src.lines];
+    if (a.line_weight) w *= a.line_weight[a.line_weight_by_plane ? line / a.src.lines : line % a.src.lines];
     a.out[line_base(a.dst, line) + static_cast<uint64_t>(e) * a.dst.elem_stride] = cscale(res[ly * ls + bin], w);
   }
 }

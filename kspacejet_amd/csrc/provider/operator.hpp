@@ -36,6 +36,9 @@ struct Shape {
   uint32_t out_channels = 0;  // k-space conditioning: virtual channels of a coil compression
   uint32_t out_cols = 0;      // k-space conditioning: readout columns kept by the crop, and
   uint32_t offset = 0;        // the first of them
+  uint32_t window = 0;        // sensitivity maps: neighbourhood side (odd)
+  uint32_t iterations = 0;    // sensitivity maps: power-iteration steps
+  uint32_t method = 0;        // sensitivity maps: index of the method literal
 };
 
 }  // namespace ksj::prov

@@ -234,6 +234,104 @@ ksj_status process(Firing& f) {
 }
 }  // namespace sense
 
+// The same combine under the planned interface's own id
+// (providers/interfaces/org.kspacejet.coil-combine/coil_combine_adaptive.json: coil images
+// and the maps of sensitivity_map_estimate in, one complex image out).  No reference
+// implementation; the interface's optional "normalization" key is not taken.
+namespace adaptive {
+constexpr std::string_view kConfig = "coil_combine_adaptive needs the canonical config "
+                                     "{\"channels\":N,\"cols\":N,\"rows\":N} with channels in [1,64] and rows, cols "
+                                     "in [2,512]";
+constexpr std::string_view kLease = "coil_combine_adaptive needs two input batches and one output grant";
+constexpr std::string_view kImages = "coil_combine_adaptive needs one exact ksj.coil-image-frame on input port 0";
+constexpr std::string_view kMaps = "coil_combine_adaptive needs one exact ksj.sensitivity-map on input port 1";
+constexpr std::string_view kOutput = "coil_combine_adaptive needs one exact ksj.complex-image-frame output grant";
+
+ksj_status process(Firing& f) {
+  if (!callbacks_ok(f.cb, kCaps2)) return reject(f.error, KSJ_STATUS_BAD_ABI, kBadAbi);
+  const Shape& s = f.op.shape;
+  const uint64_t in_bytes = coil_image_bytes(s), out_bytes = pixels(s) * kCf32;
+  ksj_firing_lease_info info;
+  if (ksj_status st = lease_info(f, 2, out_bytes, 0, false, kLease, info)) return st;
+  ksj_input_batch_view storage[2];
+  const ksj_input_batch_view *images = nullptr, *maps = nullptr;
+  if (ksj_status st = two_port_batches(f, storage, images, maps, kLease, kImages, kMaps)) return st;
+  const ksj_input_item_view& img = images->items[0];
+  const ksj_input_item_view& map = maps->items[0];
+  if (!input_payload_ok_device(img, in_bytes, TypeId::kCoilImageFrame, f.op.device_ordinal))
+    return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, kImages);
+  if (!input_payload_ok_device(map, in_bytes, TypeId::kSensitivityMap, f.op.device_ordinal))
+    return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, kMaps);
+  GrantGuard guard;
+  ksj_mutable_payload_view out;
+  if (ksj_status st = output_grant(f, out_bytes, TypeId::kComplexImageFrame, kOutput, guard, out, true)) return st;
+
+  DeviceSession& d = device(f);
+  const void* d_img = device_input(d, DeviceSession::kInput, img.payload, in_bytes);
+  const void* d_map = device_input(d, DeviceSession::kInput2, map.payload, in_bytes);
+  void* d_out = device_output(d, out, out_bytes);
+  check_gpu(ksj_gpu_sense_combine(d_img, d_map, d_out, pixels(s), s.channels, d.stream()), "adaptive combine");
+  finish_output(d, out, d_out, out_bytes);
+  return seal_and_finish(f, guard, out, out_bytes, img, false, 2, info.terminal_epoch);
+}
+}  // namespace adaptive
+
+// =================================================================== sensitivity maps
+
+// sensitivity_map_estimate of the planned interface
+// providers/interfaces/org.kspacejet.calibration/sensitivity_map_estimate.json: coil images
+// in, ksj.sensitivity-map out, method required.  Method "walsh" (the adaptive combine's
+// estimate): the dominant eigenvector of the windowed coil covariance at each pixel.  No
+// reference implementation; the optional calibration_region and regularization keys are not
+// taken.
+namespace smap {
+constexpr std::string_view kConfig = "sensitivity_map_estimate needs the canonical config "
+                                     "{\"channels\":N,\"cols\":N,\"iterations\":N,\"method\":\"walsh\",\"rows\":N,"
+                                     "\"window\":N} with channels in [1,64], rows, cols in [2,512], iterations in "
+                                     "[1,64] and an odd window in [1,31]";
+constexpr std::string_view kLease = "sensitivity_map_estimate needs one input frame and one output grant";
+constexpr std::string_view kImages = "sensitivity_map_estimate needs one exact ksj.coil-image-frame payload";
+constexpr std::string_view kOutput = "sensitivity_map_estimate needs one exact ksj.sensitivity-map output grant";
+
+bool configure(const ksj_byte_view& c, Shape& s) {
+  return parse_config(c, {u32_field("channels", 1, kMaxChannels, &s.channels), u32_field("cols", 2, kMaxDim, &s.cols),
+                          u32_field("iterations", 1, 64, &s.iterations), literal_field("method", "\"walsh\"", &s.method),
+                          u32_field("rows", 2, kMaxDim, &s.rows), u32_field("window", 1, 31, &s.window)}) &&
+         s.window % 2 == 1;  // a centred neighbourhood
+}
+bool valid(const Shape& s) {
+  return dims_ok(s, 2) && s.iterations >= 1 && s.iterations <= 64 && s.window % 2 == 1 && s.window <= 31 &&
+         s.method == 0;
+}
+
+ksj_status process(Firing& f) {
+  if (!callbacks_ok(f.cb, kCaps2)) return reject(f.error, KSJ_STATUS_BAD_ABI, kBadAbi);
+  const Shape& s = f.op.shape;
+  const uint64_t bytes = coil_image_bytes(s);
+  ksj_firing_lease_info info;
+  if (ksj_status st = lease_info(f, 1, bytes, 0, false, kLease, info)) return st;
+  ksj_input_batch_view batch{};
+  batch.abi = make_header(sizeof(batch));
+  if (ksj_status st = f.cb->get_input_batch(f.cb->host_context, f.lease, 0, &batch, f.error)) return st;
+  if (!single_item_batch(batch, 0)) return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, kImages);
+  const ksj_input_item_view& in = batch.items[0];
+  if (!input_payload_ok_device(in, bytes, TypeId::kCoilImageFrame, f.op.device_ordinal))
+    return reject(f.error, KSJ_STATUS_CONTRACT_VIOLATION, kImages);
+  GrantGuard guard;
+  ksj_mutable_payload_view out;
+  if (ksj_status st = output_grant(f, bytes, TypeId::kSensitivityMap, kOutput, guard, out, true)) return st;
+
+  DeviceSession& d = device(f);
+  const void* d_in = device_input(d, DeviceSession::kInput, in.payload, bytes);
+  void* d_out = device_output(d, out, bytes);
+  check_gpu(ksj_gpu_sensitivity_maps_walsh(d_in, s.rows, s.cols, s.channels, s.window, s.iterations, d_out,
+                                           d.stream()),
+            "sensitivity maps");
+  finish_output(d, out, d_out, bytes);
+  return seal_and_finish(f, guard, out, bytes, in, false, 1, info.terminal_epoch);
+}
+}  // namespace smap
+
 // =================================================================== non-Cartesian
 
 // The two-input firing shared by the non-Cartesian operators
@@ -536,5 +634,9 @@ const OpSpec kCartesianIfft2Rss{"cartesian_ifft2_rss", cart_rss::kConfig, 1, kMa
                                 &cart_rss::valid, &cart_rss::process};
 const OpSpec kCoilCombineSensitivity{"coil_combine_sensitivity", sense::kConfig, 2, kMaxComplexImageBytes, 0,
                                      &sense::configure, &sense::valid, &sense::process};
+const OpSpec kCoilCombineAdaptive{"coil_combine_adaptive", adaptive::kConfig, 2, kMaxComplexImageBytes, 0,
+                                  &sense::configure, &sense::valid, &adaptive::process};
+const OpSpec kSensitivityMapEstimate{"sensitivity_map_estimate", smap::kConfig, 1, kMaxCoilImageBytes, 0,
+                                     &smap::configure, &smap::valid, &smap::process};
 
 }  // namespace ksj::prov

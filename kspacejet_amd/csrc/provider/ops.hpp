@@ -35,5 +35,7 @@ extern const OpSpec kKbGridding;                // kb_gridding_reconstruct
 extern const OpSpec kKbGriddingRss;             // kb_gridding_reconstruct_rss
 extern const OpSpec kCartesianIfft2Rss;         // cartesian_ifft2_rss
 extern const OpSpec kCoilCombineSensitivity;    // coil_combine_sensitivity
+extern const OpSpec kCoilCombineAdaptive;       // coil_combine_adaptive
+extern const OpSpec kSensitivityMapEstimate;    // sensitivity_map_estimate
 
 }  // namespace ksj::prov

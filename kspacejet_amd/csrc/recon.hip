@@ -452,7 +452,7 @@ void execute_fused(const MultiItem* items, uint32_t count, int layout, void* d_w
     auto* grid = reinterpret_cast<float2*>(q);
     q += align256(cp * d.grid_rows * d.grid_cols * sizeof(float2));
     auto* rowout = reinterpret_cast<float2*>(q);
-    q += align256(static_cast<size_t>(c) * d.grid_rows * d.image_cols * sizeof(float2));
+    q += align256(cp * d.grid_rows * d.image_cols * sizeof(float2));
     auto* img = reinterpret_cast<float2*>(q);
     q += align256(static_cast<size_t>(c) * d.image_rows * d.image_cols * sizeof(float2));
     gi[i] = GridFusedItem{it.plan, it.scan.kspace, c, sorted, grid, reinterpret_cast<float2*>(q)};

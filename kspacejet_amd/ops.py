@@ -713,3 +713,16 @@ def phase_correction_estimate(reference: torch.Tensor) -> torch.Tensor:
     out = torch.empty((C, K), dtype=torch.complex64, device=reference.device)
     call("ksj_gpu_phase_correction_estimate", _ptr(reference), C, L, K, _ptr(out), _stream())
     return out
+
+
+def sensitivity_maps_walsh(coil_images: torch.Tensor, window: int = 5, iterations: int = 8) -> torch.Tensor:
+    """Coil-last coil images [rows, cols, C] complex64 -> sensitivity maps of the same shape
+    (Walsh's adaptive method; see ksj_gpu_sensitivity_maps_walsh)."""
+    _need(coil_images, torch.complex64, "coil_images")
+    if coil_images.dim() != 3:
+        raise ValueError("coil_images must be [rows, cols, channels]")
+    rows, cols, c = coil_images.shape
+    maps = torch.empty_like(coil_images)
+    call("ksj_gpu_sensitivity_maps_walsh", _ptr(coil_images), rows, cols, c, int(window), int(iterations), _ptr(maps),
+         _stream())
+    return maps

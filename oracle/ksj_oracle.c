@@ -986,3 +986,72 @@ void ora_phase_correction_estimate(size_t channels, size_t lines, size_t readout
       }
     }
 }
+
+/* ---------------------------------------------------------------- sensitivity maps */
+
+/* Walsh, Gmitro & Marcellin, "Adaptive reconstruction of phased array MR imagery",
+ * MRM 43:682-690 (2000): at each pixel the coil sensitivity vector is the dominant
+ * eigenvector of the local coil covariance R = sum over the window of I I^H.  Restated
+ * with the local covariance formed explicitly and the power iteration written out. */
+static void walsh_normalise(size_t coils, double* v) {
+  double n2 = 0.0;
+  for (size_t c = 0; c < coils; ++c) n2 += v[2 * c] * v[2 * c] + v[2 * c + 1] * v[2 * c + 1];
+  const double inv = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
+  for (size_t c = 0; c < 2 * coils; ++c) v[c] *= inv;
+}
+
+void ora_walsh_maps(size_t rows, size_t cols, size_t coils, size_t window, size_t iterations, const double* images,
+                    double* maps) {
+  const long r = (long)(window / 2);
+  double* R = (double*)calloc(2 * coils * coils, sizeof(double));
+  double* v = (double*)calloc(2 * coils, sizeof(double));
+  double* u = (double*)calloc(2 * coils, sizeof(double));
+  for (size_t y = 0; y < rows; ++y)
+    for (size_t x = 0; x < cols; ++x) {
+      memset(R, 0, 2 * coils * coils * sizeof(double));
+      for (long dy = -r; dy <= r; ++dy)
+        for (long dx = -r; dx <= r; ++dx) {
+          const long yy = (long)y + dy, xx = (long)x + dx;
+          if (yy < 0 || yy >= (long)rows || xx < 0 || xx >= (long)cols) continue;
+          const double* a = images + 2 * (((size_t)yy * cols + (size_t)xx) * coils);
+          for (size_t i = 0; i < coils; ++i)
+            for (size_t j = 0; j < coils; ++j) { /* R[i][j] += a_i conj(a_j) */
+              R[2 * (i * coils + j)] += a[2 * i] * a[2 * j] + a[2 * i + 1] * a[2 * j + 1];
+              R[2 * (i * coils + j) + 1] += a[2 * i + 1] * a[2 * j] - a[2 * i] * a[2 * j + 1];
+            }
+        }
+      for (size_t i = 0; i < coils; ++i) { /* v = R e_0 */
+        v[2 * i] = R[2 * (i * coils)];
+        v[2 * i + 1] = R[2 * (i * coils) + 1];
+      }
+      walsh_normalise(coils, v);
+      for (size_t it = 0; it < iterations; ++it) {
+        for (size_t i = 0; i < coils; ++i) {
+          double ur = 0.0, ui = 0.0;
+          for (size_t j = 0; j < coils; ++j) {
+            const double rr = R[2 * (i * coils + j)], ri = R[2 * (i * coils + j) + 1];
+            ur += rr * v[2 * j] - ri * v[2 * j + 1];
+            ui += rr * v[2 * j + 1] + ri * v[2 * j];
+          }
+          u[2 * i] = ur;
+          u[2 * i + 1] = ui;
+        }
+        memcpy(v, u, 2 * coils * sizeof(double));
+        walsh_normalise(coils, v);
+      }
+      const double m = sqrt(v[0] * v[0] + v[1] * v[1]); /* coil 0 real and >= 0 */
+      double* out = maps + 2 * ((y * cols + x) * coils);
+      for (size_t c = 0; c < coils; ++c) {
+        if (m > 0.0) {
+          out[2 * c] = (v[2 * c] * v[0] + v[2 * c + 1] * v[1]) / m;
+          out[2 * c + 1] = (v[2 * c + 1] * v[0] - v[2 * c] * v[1]) / m;
+        } else {
+          out[2 * c] = v[2 * c];
+          out[2 * c + 1] = v[2 * c + 1];
+        }
+      }
+    }
+  free(R);
+  free(v);
+  free(u);
+}

@@ -172,6 +172,17 @@ int ora_coil_compression_basis_estimate(size_t physical, size_t virtual_channels
  * (phase_correction_estimate.cpp:141-163). */
 void ora_phase_correction_estimate(size_t channels, size_t lines, size_t readout, const float* ref, float* phase);
 
+/* ---- sensitivity maps (planned interface providers/interfaces/org.kspacejet.calibration/
+ * sensitivity_map_estimate.json; no reference implementation). */
+
+/* Walsh's adaptive sensitivity estimate: at each pixel, the dominant eigenvector of
+ * R = sum over a window x window neighbourhood (clipped at the image edges) of I I^H, by
+ * power iteration from R e_0 (normalised after every step, `iterations` steps), rotated
+ * so that coil 0 is real and >= 0; unit norm, or zero where R e_0 vanishes.
+ * images and maps [rows][cols][coils], interleaved complex128. */
+void ora_walsh_maps(size_t rows, size_t cols, size_t coils, size_t window, size_t iterations, const double* images,
+                    double* maps);
+
 #ifdef __cplusplus
 }
 #endif

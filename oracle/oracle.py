@@ -75,6 +75,7 @@ def lib():
         L.ora_noise_model_estimate.argtypes = [_sz, _sz, _fp, C.c_double, _dp]
         L.ora_coil_compression_basis_estimate.argtypes = [_sz, _sz, _sz, _fp, _dp]
         L.ora_phase_correction_estimate.argtypes = [_sz, _sz, _sz, _fp, _fp]
+        L.ora_walsh_maps.argtypes = [_sz, _sz, _sz, _sz, _sz, _dp, _dp]
         L.ora_normalize_trajectory2.argtypes = [_sz, _fp, C.c_int, C.c_uint32, C.c_uint32, _fp]
         L.ora_channel_mix.argtypes = [_sz, _sz, _sz, _dp, _dp, _dp]
         L.ora_phase_correct.argtypes = [_sz, _sz, _sz, _fp, _fp, _fp]
@@ -523,4 +524,17 @@ def phase_correction_estimate(ref) -> np.ndarray:
     C, L, K = ref.shape
     out = np.zeros((C, K), np.complex64)
     lib().ora_phase_correction_estimate(C, L, K, _c64_pairs(ref), out.view(np.float32).reshape(-1))
+    return out
+
+
+# ---------------------------------------------------------------- sensitivity maps
+
+def walsh_maps(coil_last, window: int, iterations: int) -> np.ndarray:
+    """[rows, cols, C] complex -> [rows, cols, C] complex128 maps (Walsh's adaptive method:
+    dominant eigenvector of the windowed coil covariance, coil 0 real and >= 0)."""
+    x = _c128(coil_last)
+    rows, cols, coils = x.shape
+    out = np.zeros_like(x)
+    lib().ora_walsh_maps(rows, cols, coils, window, iterations, x.view(np.float64).reshape(-1),
+                         out.view(np.float64).reshape(-1))
     return out

@@ -102,6 +102,28 @@ def test_fused_and_lane_forms_give_identical_results(ops, batch):
     assert ops.chain_faults() == 0
 
 
+def test_fused_form_with_staged_ffts_matches_single_plans(ops, batch):
+    """With the LDS-staged FFT everywhere (fft_register = 0) the fused form's coil-last
+    column pass (coil-fastest lines, column weights by plane) still gives every item's
+    single-plan bits, at LDS and direct-DFT lengths alike."""
+    plans, ks, _ = batch
+    saved = ops.tune_get("multi_fused"), ops.tune_get("fft_register")
+    try:
+        ops.tune_set("fft_register", 0)
+        ops.tune_set("multi_fused", 1)
+        rss, imgs = outputs(batch)
+        ws = torch.empty(ops.adjoint_multi_workspace_bytes(plans, [k.shape[0] for k in ks]), dtype=torch.uint8,
+                         device="cuda")
+        ops.adjoint_multi(plans, ks, rss=rss, images=imgs, workspace=ws)
+        torch.cuda.synchronize()
+        for i, (plan, k) in enumerate(zip(plans, ks)):
+            img1, rss1 = plan.adjoint(k, rss=True)
+            assert torch.equal(imgs[i], img1) and torch.equal(rss[i], rss1), i
+    finally:
+        ops.tune_set("multi_fused", saved[0])
+        ops.tune_set("fft_register", saved[1])
+
+
 def test_chain_form_runs_its_work_queue(ops, batch, request):
     """multi_fused = 2 grids matrix-schedule items in the work-queue launch (its stage is
     recorded as multi_grid_chain) and falls back to the per-class launches for the 2x2-block

@@ -42,7 +42,8 @@ PROVIDERS = {
     "gpu_recon": ("org.kspacejet.gpu-recon",
                   ["kb_gridding_reconstruct", "kb_gridding_reconstruct_rss", "cartesian_ifft2_rss",
                    "coil_combine_sensitivity", "noise_prewhiten", "phase_correct", "coil_compress",
-                   "readout_oversampling_remove", "noncartesian_noise_prewhiten", "noncartesian_coil_compress"], None),
+                   "readout_oversampling_remove", "noncartesian_noise_prewhiten", "noncartesian_coil_compress",
+                   "sensitivity_map_estimate", "coil_combine_adaptive"], None),
 }
 
 DIRECT_CFG = '{"channels":2,"image_cols":2,"image_rows":2,"sample_count":2}'
@@ -182,6 +183,11 @@ def test_query_rejects_incompatible_abi_storage(modules):
      '"kernel_width":8,"oversampling":2,"sample_count":4194304,"trajectory_units":"radians_per_pixel"}'),
     ("gpu_recon", "cartesian_ifft2_rss", '{"centred":true,"channels":8,"cols":256,"rows":256}'),
     ("gpu_recon", "coil_combine_sensitivity", '{"channels":8,"cols":256,"rows":256}'),
+    ("gpu_recon", "coil_combine_adaptive", '{"channels":8,"cols":256,"rows":256}'),
+    ("gpu_recon", "sensitivity_map_estimate",
+     '{"channels":16,"cols":256,"iterations":8,"method":"walsh","rows":256,"window":5}'),
+    ("gpu_recon", "sensitivity_map_estimate",
+     '{"channels":64,"cols":512,"iterations":64,"method":"walsh","rows":512,"window":31}'),
 ])
 def test_canonical_configs_create_operators(modules, module, op, config):
     st, h, msg = modules[module].create(op, config)
@@ -227,6 +233,18 @@ def test_canonical_configs_create_operators(modules, module, op, config):
      '{"channels":16,"density_compensation":"radial_analytic_rampx","image_cols":256,"image_rows":256,'
      '"kernel_width":4,"oversampling":2,"sample_count":205824,"trajectory_units":"radians_per_pixel"}'),
     ("gpu_recon", "cartesian_ifft2_rss", '{"centred":1,"channels":8,"cols":256,"rows":256}'),
+    ("gpu_recon", "sensitivity_map_estimate",
+     '{"channels":16,"cols":256,"iterations":8,"method":"walsh","rows":256,"window":4}'),
+    ("gpu_recon", "sensitivity_map_estimate",
+     '{"channels":16,"cols":256,"iterations":8,"method":"walsh","rows":256,"window":33}'),
+    ("gpu_recon", "sensitivity_map_estimate",
+     '{"channels":16,"cols":256,"iterations":0,"method":"walsh","rows":256,"window":5}'),
+    ("gpu_recon", "sensitivity_map_estimate",
+     '{"channels":16,"cols":256,"iterations":8,"method":"espirit","rows":256,"window":5}'),
+    ("gpu_recon", "sensitivity_map_estimate",
+     '{"channels":16,"cols":256,"iterations":8,"rows":256,"window":5}'),
+    ("gpu_recon", "sensitivity_map_estimate",
+     '{"channels":16,"cols":256,"method":"walsh","iterations":8,"rows":256,"window":5}'),
 ])
 def test_noncanonical_or_out_of_envelope_configs_are_rejected(modules, module, op, config):
     st, h, _ = modules[module].create(op, config)
