@@ -103,6 +103,8 @@ enum TuneKey : int {
   kTuneFftLdsPad,         // register-form FFT: LDS element stride = lines per block + this (1..4; same bits)
   kTuneMatrix3Map,        // 3-D matrix gather: 0 launch order, 1 XCD-contiguous eighths, s > 1 XCD slabs (same bits)
   kTuneMatrix3TapShare,   // 3-D matrix gather: 1 = one or two tap words per lane shared by bpermute (same bits)
+  kTuneGridPrune,         // 2-D matrix schedule: 1 = cells no sample reaches (around column G/2) are neither written
+                          // by the gather nor read by the row FFT, which takes them as zero (same bits)
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);

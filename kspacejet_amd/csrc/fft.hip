@@ -217,8 +217,12 @@ __device__ inline uint64_t line_base(const FftLines& L, uint64_t line) {
 // of two and the block has lpb * T threads, so every thread moves exactly N / T
 // elements: all of its global loads are issued before the first LDS write, so the
 // loads of one thread overlap instead of paying one memory latency each.
+// Element e of a line is read as zero when e lies in the line's zero span [zs.x, zs.y).
+__device__ inline bool in_span(uint32_t e, uint2 zs) { return e - zs.x < zs.y - zs.x; }
+
 template <int N, int V>
-__device__ inline void load_lines(const FftArgs& a, float2* buf, int ls, int lpb_log2, const uint64_t* src_base) {
+__device__ inline void load_lines(const FftArgs& a, float2* buf, int ls, int lpb_log2, const uint64_t* src_base,
+                                  const uint2* zspan) {
   constexpr int E = N / Shape<N, V>::T;
   const int lpb = 1 << lpb_log2;
   const uint32_t es = static_cast<uint32_t>(a.src.elem_stride);
@@ -244,8 +248,10 @@ __device__ inline void load_lines(const FftArgs& a, float2* buf, int ls, int lpb
     // outputs are never stored.
     // Every input element of a pass is read once: non-temporal, out of the caches' way.
     typedef float F2v __attribute__((ext_vector_type(2)));
-    const F2v t = __builtin_nontemporal_load(
-        reinterpret_cast<const F2v*>(a.in + (live ? base : 0ull) + static_cast<uint64_t>(src * es)));
+    F2v t = {0.f, 0.f};
+    if (!in_span(src, zspan[ly]))
+      t = __builtin_nontemporal_load(
+          reinterpret_cast<const F2v*>(a.in + (live ? base : 0ull) + static_cast<uint64_t>(src * es)));
     v[i] = make_float2(t.x, t.y);
     dst[i] = ly * ls + pad(e);
   }
@@ -293,6 +299,7 @@ __device__ inline void fft_lds_block(const FftArgs& a, const float2* __restrict_
   constexpr int LS = line_stride<N>();
   extern __shared__ float2 smem[];
   __shared__ uint64_t src_base[64], dst_base[64];
+  __shared__ uint2 zspan[64];
   __shared__ float ow[N], lw[64];  // output weights (scale x out_weight) and per-line weights
   const int lpb = 1 << lpb_log2;
   float2* tw = smem;
@@ -307,11 +314,12 @@ __device__ inline void fft_lds_block(const FftArgs& a, const float2* __restrict_
     const uint64_t line = first + ly;
     src_base[ly] = line < total ? line_base(a.src, line) : ~0ull;
     dst_base[ly] = line < total ? line_base(a.dst, line) : ~0ull;
+    zspan[ly] = (a.zero_span && line < total) ? a.zero_span[line / a.src.lines] : make_uint2(0, 0);
     const uint64_t wi = a.line_weight_by_plane ? line / a.src.lines : line % a.src.lines;
     lw[ly] = (a.line_weight && line < total) ? a.line_weight[wi] : 1.0f;
   }
   __syncthreads();
-  load_lines<N, V>(a, buf, LS, lpb_log2, src_base);
+  load_lines<N, V>(a, buf, LS, lpb_log2, src_base, zspan);
   __syncthreads();
   const int ly = threadIdx.x / T, t = threadIdx.x % T;
   passes<N, V, INV, 0, 1>(buf + ly * LS, tw, t);
@@ -448,6 +456,7 @@ __device__ inline void fft_reg_block(const FftArgs& a, const ColumnOutputs& co, 
   constexpr int E = N / T;
   extern __shared__ float2 smem[];
   __shared__ uint64_t src_base[64], dst_base[64];
+  __shared__ uint2 zspan[64];
   __shared__ float ow[N], lw[64];
   const int lpb = 1 << lpb_log2;
   const int S = lpb + pad;
@@ -463,6 +472,7 @@ __device__ inline void fft_reg_block(const FftArgs& a, const ColumnOutputs& co, 
     const uint64_t line = first + l;
     src_base[l] = line < total ? line_base(a.src, line) : ~0ull;
     dst_base[l] = line < total ? line_base(a.dst, line) : ~0ull;
+    zspan[l] = (a.zero_span && line < total) ? a.zero_span[line / a.src.lines] : make_uint2(0, 0);
     const uint64_t wi = a.line_weight_by_plane ? line / a.src.lines : line % a.src.lines;
     lw[l] = (a.line_weight && line < total) ? a.line_weight[wi] : 1.0f;
   }
@@ -473,12 +483,15 @@ __device__ inline void fft_reg_block(const FftArgs& a, const ColumnOutputs& co, 
     const uint64_t base = src_base[ly];
     const uint64_t b0 = base != ~0ull ? base : 0ull;  // a line past the end transforms line 0; never stored
     const uint32_t es = static_cast<uint32_t>(a.src.elem_stride);
+    const uint2 zs = zspan[ly];
     typedef float F2v __attribute__((ext_vector_type(2)));
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       uint32_t src = static_cast<uint32_t>(t + i * T) + a.in_offset;
       if (src >= static_cast<uint32_t>(N)) src -= N;
-      const F2v v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(a.in + b0 + static_cast<uint64_t>(src * es)));
+      F2v v = {0.f, 0.f};
+      if (!in_span(src, zs))
+        v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(a.in + b0 + static_cast<uint64_t>(src * es)));
       x.v[i] = make_float2(v.x, v.y);
     }
   }
@@ -582,7 +595,8 @@ __global__ __launch_bounds__(256) void dft_direct_kernel(FftArgs a, const float2
     if (line < total) {
       uint32_t src = static_cast<uint32_t>(e) + a.in_offset;
       if (src >= static_cast<uint32_t>(n)) src -= n;
-      v = a.in[line_base(a.src, line) + src * a.src.elem_stride];
+      const uint2 zs = a.zero_span ? a.zero_span[line / a.src.lines] : make_uint2(0, 0);
+      if (!in_span(src, zs)) v = a.in[line_base(a.src, line) + src * a.src.elem_stride];
     }
     buf[ly * ls + e] = v;
   }

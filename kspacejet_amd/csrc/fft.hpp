@@ -38,6 +38,9 @@ struct FftArgs {
   int store_lines_fast = 0;
   // line_weight index: line % lines (0) or the line's plane (1).
   int line_weight_by_plane = 0;
+  // Per source plane: elements [x, y) of each of its lines are zero and are not read
+  // (null: every element is read).
+  const uint2* zero_span = nullptr;
 };
 
 // Outputs of a column pass whose lines are (plane = image column c, line = stacked coil

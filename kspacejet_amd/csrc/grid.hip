@@ -149,6 +149,7 @@ struct GatherArgs {
   const float* taps;                // [S][2W] sorted taps (matrix schedule with packed visits)
   uint32_t width;                   // W
   uint32_t from_taps;               // 1: visit_k packs (sample, window position), weights formed from taps
+  const uint2* zero_cols;           // matrix schedule: per grid row, columns no sample reaches (not written), or null
 };
 
 __device__ inline void fma4(float2 (&acc)[4], float4 w, float2 d) {
@@ -443,6 +444,18 @@ __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
     const uint32_t blk = field(rec.x, t), slot = field(rec.w, t);
     uint32_t by, bx;
     block_coords(blk, a.block_cols, a.block_group, &by, &bx);
+    if (a.zero_cols != nullptr && slot == 0xFFFFFFFFu && 4 * by < a.grid_rows) {
+      // A block inside its rows' zero span (no sample reaches it): the row FFT reads
+      // those cells as zero, so they are not written.
+      const uint2 zs = a.zero_cols[4 * by];
+      if (4 * T * bx >= zs.x && 4 * T * (bx + 1) <= zs.y) {
+#pragma unroll
+        for (int h = 0; h < T; ++h)
+#pragma unroll
+          for (int n = 0; n < NF; ++n) acc[h][n] = MfmaAcc{0.f, 0.f, 0.f, 0.f};
+        return;
+      }
+    }
 #pragma unroll
     for (int h = 0; h < T; ++h) {
 #pragma unroll
@@ -1353,6 +1366,7 @@ GatherArgs gather_args(const Plan2& plan, const float2* d_sorted, uint32_t coils
   a.taps = plan.taps;
   a.width = plan.d.width;
   a.from_taps = plan.packed_visits ? 1u : 0u;
+  a.zero_cols = tune_get(kTuneGridPrune) == 1 ? plan.zero_cols : nullptr;
   return a;
 }
 }  // namespace

@@ -337,6 +337,35 @@ __global__ void item_fill_kernel(const uint32_t* first, const uint32_t* slot, co
   }
 }
 
+// Block (by, bx) has visits: flag in row-major block order.
+__global__ void block_visited_kernel(const uint32_t* chunk_off, uint64_t nblocks, uint32_t block_cols, uint32_t group,
+                                     uint8_t* visited) {
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  uint32_t by, bx;
+  block_coords(b, block_cols, group, &by, &bx);
+  visited[static_cast<uint64_t>(by) * block_cols + bx] = chunk_off[b + 1] > chunk_off[b] ? 1 : 0;
+}
+
+// Per grid row: the run of visitless blocks of its block row that holds column G_c / 2,
+// as cell columns [x, y) (empty when that block has visits).
+__global__ void zero_span_kernel(const uint8_t* visited, uint32_t grid_rows, uint32_t grid_cols, uint32_t block_cols,
+                                 uint32_t bh, uint32_t bw, uint2* span) {
+  const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= grid_rows) return;
+  const uint8_t* row = visited + static_cast<uint64_t>(y / bh) * block_cols;
+  const uint32_t nb = (grid_cols + bw - 1) / bw;  // blocks holding grid columns
+  const uint32_t mid = (grid_cols / 2) / bw;
+  uint2 s = make_uint2(0, 0);
+  if (mid < nb && row[mid] == 0) {
+    uint32_t lo = mid, hi = mid + 1;
+    while (lo > 0 && row[lo - 1] == 0) --lo;
+    while (hi < nb && row[hi] == 0) ++hi;
+    s = make_uint2(lo * bw, min(hi * bw, grid_cols));
+  }
+  span[y] = s;
+}
+
 __global__ void deapod_kernel(float* table, uint32_t n, uint32_t grid, int width, double beta, double i0_beta) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
@@ -585,7 +614,11 @@ Plan2* plan2_create(const Plan2Desc& desc) {
       p->tile_samples = alloc<uint32_t>(p->ntiles, &total);
     p->block_first_item = alloc<uint32_t>(p->nblocks + 1, &total);
     p->block_first_slot = alloc<uint32_t>(p->nblocks + 1, &total);
-    if (p->scheme == kSchemeMatrix) p->block_chunk_off = alloc<uint32_t>(p->nblocks + 1, &total);
+    if (p->scheme == kSchemeMatrix) {
+      p->block_chunk_off = alloc<uint32_t>(p->nblocks + 1, &total);
+      p->zero_cols = alloc<uint2>(desc.grid_rows, &total);
+      p->block_visited = alloc<uint8_t>(p->nblocks, &total);
+    }
     p->multi_blocks = alloc<uint32_t>(p->nblocks, &total);
     p->xcd_first = alloc<uint32_t>(kPlanXcds + 1, &total);
     p->frac = alloc<double>(2 * S, &total);
@@ -619,7 +652,8 @@ void plan2_destroy(Plan2* p) {
                   p->taps,          p->visit_row_off, p->visit_k,    p->visit_w,      p->block_first_item,
                   p->block_first_slot, p->item_info,    p->multi_blocks, p->frac,         p->sorted_keys,
                   p->counters,      p->sort_scratch, p->deapod_rows, p->deapod_cols, p->item_cost,
-                  p->item_scratch,  p->xcd_first,   p->tile_samples, p->block_chunk_off, p->run_first};
+                  p->item_scratch,  p->xcd_first,   p->tile_samples, p->block_chunk_off, p->run_first,
+                  p->zero_cols,     p->block_visited};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -704,6 +738,13 @@ void plan2_set_trajectory(Plan2* p, const float* d_traj, const float* d_dcf, hip
     KSJ_HIP(hipStreamSynchronize(stream));
     require(chunks <= 0x3FFFFFFFu, "too many gather visits for one plan");
     visits = 4 * chunks;
+    hipLaunchKernelGGL(block_visited_kernel, dim3(ceil_div(p->nblocks, 256)), dim3(256), 0, stream,
+                       p->block_chunk_off, p->nblocks, p->block_cols, p->block_group, p->block_visited);
+    KSJ_LAUNCH_CHECK();
+    hipLaunchKernelGGL(zero_span_kernel, dim3(ceil_div(p->d.grid_rows, 256)), dim3(256), 0, stream,
+                       p->block_visited, p->d.grid_rows, p->d.grid_cols, p->block_cols, p->block_h, p->block_w,
+                       p->zero_cols);
+    KSJ_LAUNCH_CHECK();
   } else {
     KSJ_HIP(hipMemcpyAsync(&visits, p->visit_row_off + nrows, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     KSJ_HIP(hipStreamSynchronize(stream));

@@ -131,6 +131,12 @@ struct Plan2 {
   uint32_t* tile_samples = nullptr;      // [ntiles] window-list length
   float* deapod_rows = nullptr;    // [N_r] or null
   float* deapod_cols = nullptr;    // [N_c] or null
+  // Matrix schedule: per grid row, the columns [x, y) that no sample reaches -- the run of
+  // blocks without visits around column G_c / 2 (the corners of k-space outside the
+  // sampled disk, in the grid's uncentred layout).  The gather does not write those
+  // cells and the row FFT reads them as zero (tuning key grid_prune).
+  uint2* zero_cols = nullptr;      // [G_r] or null
+  uint8_t* block_visited = nullptr;  // [block_rows * block_cols] scratch of zero_cols
   // scratch used only while building
   double* frac = nullptr;          // [S][2]
   uint32_t* sorted_keys = nullptr; // [S]

@@ -141,6 +141,8 @@ FftArgs row_pass_args(const Plan2& plan, uint32_t total, size_t cp, const float2
   a.inverse = 1;
   a.load_lines_fast = 1;
   a.store_lines_fast = 0;
+  // Columns the gather leaves unwritten (no sample reaches them) are read as zero.
+  a.zero_span = tune_get(kTuneGridPrune) == 1 ? plan.zero_cols : nullptr;
   return a;
 }
 

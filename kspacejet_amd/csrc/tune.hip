@@ -49,6 +49,7 @@ Param g_params[kTuneKeys] = {
     {"fft_lds_pad", 1, [](int64_t v) { return v >= 1 && v <= 4; }},
     {"matrix3_map", 0, [](int64_t v) { return v >= 0 && v <= 4096; }},
     {"matrix3_tap_share", 0, [](int64_t v) { return v == 0 || v == 1; }},
+    {"grid_prune", 1, [](int64_t v) { return v == 0 || v == 1; }},
 };
 
 }  // namespace
