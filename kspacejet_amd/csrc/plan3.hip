@@ -295,6 +295,35 @@ template <typename T> T* alloc3(size_t count, size_t* total) {
 
 }  // namespace
 
+namespace {
+
+// Block b (row-major z, y, x) has visits.
+__global__ void block_visited3_kernel(const uint32_t* chunk_off, uint64_t nblocks, uint8_t* visited) {
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b < nblocks) visited[b] = chunk_off[b + 1] > chunk_off[b] ? 1 : 0;
+}
+
+// Per grid row (z, y): the run of visitless blocks of its block row holding column G_x / 2,
+// as cell columns [a, b) (empty when that block has visits).
+__global__ void zero_span3_kernel(const uint8_t* visited, uint32_t gz, uint32_t gy, uint32_t gx, uint32_t by_count,
+                                  uint32_t bx_count, uint32_t side, uint2* span) {
+  const uint64_t r = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= static_cast<uint64_t>(gz) * gy) return;
+  const uint32_t z = static_cast<uint32_t>(r / gy), y = static_cast<uint32_t>(r % gy);
+  const uint8_t* row = visited + (static_cast<uint64_t>(z / side) * by_count + y / side) * bx_count;
+  const uint32_t mid = (gx / 2) / side;
+  uint2 s = make_uint2(0, 0);
+  if (mid < bx_count && row[mid] == 0) {
+    uint32_t lo = mid, hi = mid + 1;
+    while (lo > 0 && row[lo - 1] == 0) --lo;
+    while (hi < bx_count && row[hi] == 0) ++hi;
+    s = make_uint2(lo * side, min(hi * side, gx));
+  }
+  span[r] = s;
+}
+
+}  // namespace
+
 Plan3* plan3_create(const Plan3Desc& desc) {
   for (int d = 0; d < 3; ++d) {
     require(desc.image[d] >= 1, "plan image dimensions must be positive");
@@ -360,7 +389,11 @@ Plan3* plan3_create(const Plan3Desc& desc) {
     p->block_first_item = alloc3<uint32_t>(p->nblocks + 1, &total);
     p->block_first_slot = alloc3<uint32_t>(p->nblocks + 1, &total);
     p->multi_blocks = alloc3<uint32_t>(p->nblocks, &total);
-    if (p->scheme == kSchemeMatrix) p->block_chunk_off = alloc3<uint32_t>(p->nblocks + 1, &total);
+    if (p->scheme == kSchemeMatrix) {
+      p->block_chunk_off = alloc3<uint32_t>(p->nblocks + 1, &total);
+      p->zero_cols = alloc3<uint2>(static_cast<uint64_t>(desc.grid[0]) * desc.grid[1], &total);
+      p->block_visited = alloc3<uint8_t>(p->nblocks, &total);
+    }
     p->frac = alloc3<double>(3 * S, &total);
     p->sorted_keys = alloc3<uint32_t>(S, &total);
     p->counters = alloc3<uint32_t>(4, &total);
@@ -385,7 +418,8 @@ void plan3_destroy(Plan3* p) {
                   p->taps,       p->block_visit_off, p->visit_k,   p->visit_w,    p->block_first_item,
                   p->block_first_slot, p->item_info, p->multi_blocks, p->frac,     p->sorted_keys,
                   p->counters,   p->sort_scratch, p->deapod[0],    p->deapod[1],  p->deapod[2],
-                  p->visit_pos,  p->block_chunk_off, p->item_cost, p->item_scratch, p->run_first};
+                  p->visit_pos,  p->block_chunk_off, p->item_cost, p->item_scratch, p->run_first,
+                  p->zero_cols,  p->block_visited};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -454,6 +488,13 @@ void plan3_set_trajectory(Plan3* p, const float* d_traj, const float* d_dcf, hip
     KSJ_HIP(hipStreamSynchronize(stream));
     require(chunks <= 0x3FFFFFFFu, "too many gather visits for one 3-D plan");
     visits = 4 * chunks;
+    hipLaunchKernelGGL(block_visited3_kernel, dim3(ceil_div(p->nblocks, 256)), dim3(256), 0, stream,
+                       p->block_chunk_off, p->nblocks, p->block_visited);
+    KSJ_LAUNCH_CHECK();
+    const uint64_t grid_rows = static_cast<uint64_t>(p->d.grid[0]) * p->d.grid[1];
+    hipLaunchKernelGGL(zero_span3_kernel, dim3(ceil_div(grid_rows, 256)), dim3(256), 0, stream, p->block_visited,
+                       p->d.grid[0], p->d.grid[1], p->d.grid[2], p->blocks[1], p->blocks[2], p->side, p->zero_cols);
+    KSJ_LAUNCH_CHECK();
   } else {
     KSJ_HIP(hipMemcpyAsync(&visits, p->block_visit_off + p->nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     KSJ_HIP(hipStreamSynchronize(stream));

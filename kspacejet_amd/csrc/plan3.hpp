@@ -77,6 +77,11 @@ struct Plan3 {
   uint32_t* run_first = nullptr;         // [run_capacity + 1]
   uint32_t run_capacity = 0, run_count = 0;
   float* deapod[3] = {nullptr, nullptr, nullptr};
+  // Matrix schedule: per grid row (z, y), the x columns [a, b) no sample reaches -- the run
+  // of visitless blocks around column G_x / 2 (every column for rows outside the sampled
+  // ball); the gather does not write them and the x FFT reads them as zero (grid_prune).
+  uint2* zero_cols = nullptr;        // [G_z * G_y] or null
+  uint8_t* block_visited = nullptr;  // [nblocks] scratch of zero_cols
   // build scratch
   double* frac = nullptr;          // [S][3]
   uint32_t* sorted_keys = nullptr; // [S]

@@ -38,6 +38,8 @@ struct Gather3Args {
   // workgroup -> XCD mapping (matrix3_map): 0 launch order; 1 XCD x takes the x-th eighth
   // of the workgroups (contiguous blocks); s > 1 slabs of s workgroups dealt round-robin
   uint32_t map, wg_per_xcd;
+  // matrix schedule: per grid row (z, y), the x columns no sample reaches (not written), or null
+  const uint2* zero_cols;
 };
 
 __device__ inline uint32_t last_visit(uint32_t v, uint32_t end) {
@@ -182,6 +184,18 @@ __device__ inline void matrix3_run(const Gather3Args& a, uint32_t run) {
     const uint32_t blk = field(rec.x, t), slot = field(rec.w, t);
     uint32_t z0, y0, x0;
     block_cells(a, blk, &z0, &y0, &x0);
+    if (a.zero_cols != nullptr && slot == 0xFFFFFFFFu && z0 < a.g[0] && y0 < a.g[1]) {
+      // A block inside its rows' zero span (no sample reaches it): the x FFT reads those
+      // cells as zero, so they are not written.
+      const uint2 zs = a.zero_cols[static_cast<uint64_t>(z0) * a.g[1] + y0];
+      if (x0 >= zs.x && x0 + 4 <= zs.y) {
+#pragma unroll
+        for (int h = 0; h < T; ++h)
+#pragma unroll
+          for (int n = 0; n < NF; ++n) acc[h][n] = MfmaAcc{0.f, 0.f, 0.f, 0.f};
+        return;
+      }
+    }
 #pragma unroll
     for (int h = 0; h < T; ++h) {
 #pragma unroll
@@ -459,6 +473,8 @@ void adjoint3_execute(const Plan3& plan, const float2* d_kspace, uint32_t channe
     a.run_first = plan.run_first;
     a.run_count = plan.run_count;
     a.width = static_cast<int>(d.width);
+    const uint2* zero_cols = tune_get(kTuneGridPrune) == 1 ? plan.zero_cols : nullptr;
+    a.zero_cols = zero_cols;
     {
       ProfScope ps("grid3_gather", stream);
       switch (cp) {
@@ -497,6 +513,7 @@ void adjoint3_execute(const Plan3& plan, const float2* d_kspace, uint32_t channe
     fx.inverse = 1;
     fx.load_lines_fast = 1;
     fx.store_lines_fast = 0;
+    fx.zero_span = zero_cols;  // per plane (z, y): the columns the gather left unwritten
     {
       ProfScope ps("fft3_x", stream);
       fft_launch(fx, stream);

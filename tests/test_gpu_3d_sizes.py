@@ -130,3 +130,39 @@ def test_full_volume_is_linear(ops, full):
     ref = img + 2 * b
     del b
     assert float(torch.linalg.vector_norm(ab - ref) / torch.linalg.vector_norm(ref)) <= 1e-6
+
+
+def test_full_volume_pruning_keeps_every_bit(ops, full):
+    """Grid pruning (the cells outside the sampled ball are neither written by the gather
+    nor read by the x pass) gives the same bits as writing and reading the whole grid."""
+    _, k, plan, img, rss = full
+    saved = ops.tune_get("grid_prune")
+    ops.tune_set("grid_prune", 0)
+    try:
+        img2, rss2 = plan.adjoint(k, rss=True, coils_per_pass=16)
+        torch.cuda.synchronize()
+    finally:
+        ops.tune_set("grid_prune", saved)
+    assert torch.equal(img2, img) and torch.equal(rss2, rss)
+
+
+@pytest.mark.parametrize("shape,spokes", [((22, 22, 22), 300), ((32, 32, 32), 700), ((32, 64, 96), 1200)])
+def test_pruning_keeps_small_volumes(ops, shape, spokes):
+    """The same at grid lengths that take the direct DFT (44) and the LDS passes."""
+    traj = ops.golden_means3(spokes, 2 * max(shape))
+    g = torch.Generator(device="cuda").manual_seed(spokes)
+    k = torch.complex(torch.randn((3, traj.shape[0]), generator=g, device="cuda"),
+                      torch.randn((3, traj.shape[0]), generator=g, device="cuda"))
+    plan = with_scheme(ops, 2, lambda: ops.Plan3(shape, traj.shape[0]).set_trajectory(traj))
+    saved = ops.tune_get("grid_prune")
+    out = []
+    try:
+        for prune in (1, 0):
+            ops.tune_set("grid_prune", prune)
+            img, rss = plan.adjoint(k, rss=True, coils_per_pass=2)
+            torch.cuda.synchronize()
+            out.append((img, rss))
+    finally:
+        ops.tune_set("grid_prune", saved)
+        plan.close()
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
