@@ -214,6 +214,8 @@ def main(argv=None) -> int:
     ap.add_argument("--volume-coils", type=int, default=20,
                     help="coils of the 3-D kooshball volume (256^3, 16384 x 512 spokes); 0 skips that section")
     ap.add_argument("--volume-coils-per-pass", type=int, default=16)
+    ap.add_argument("--volume-random", action="store_true",
+                    help="3-D section on seeded complex Gaussian k-space instead of the simulated phantom")
     ap.add_argument("--provider-firings", type=int, default=40,
                     help="firings of the drop-in Provider operator (host-fed plain form); 0 skips it")
     ap.add_argument("--serve-fft-schedule", type=int, default=3,
@@ -403,7 +405,7 @@ def main(argv=None) -> int:
     if args.volume_coils > 0:
         log("3-D kooshball volume ...")
         volume = run_volume(torch, ops, lib, args.volume_coils, args.volume_coils_per_pass, max(1, args.steps // 4),
-                            1)
+                            1, simulate=not args.volume_random)
     conditioning = None
     if args.conditioning_reps > 0:
         log("k-space conditioning kernels ...")
@@ -533,18 +535,27 @@ def run_conditioning(torch, ops, reps):
     return res
 
 
-def run_volume(torch, ops, lib, coils, coils_per_pass, steps, warmup):
+def run_volume(torch, ops, lib, coils, coils_per_pass, steps, warmup, simulate=True):
     """3-D radial (kooshball) adjoint of one 256^3 volume: 16384 golden-means spokes x
     512 samples, KB width 4 on a 512^3 grid (sigma 2), |k|^2 density, coils in passes
-    of `coils_per_pass`, RSS volume out.  The k-space is seeded complex Gaussian noise
-    of the workload's shape (the adjoint's cost does not depend on the values; parity
-    is tested on small volumes against the FP64 oracle)."""
+    of `coils_per_pass`, RSS volume out.  The k-space is simulated as the workload
+    defines it: the 10-ellipsoid phantom times 3-D Gaussian coil sensitivities (sigma 90
+    voxels, centres on a sphere of radius 140), through the FP64 Kaiser-Bessel forward
+    NUFFT of width 6 on the 2x grid, plus complex Gaussian noise of sigma 1e-3 max|k|,
+    stored complex64 (`simulate` False: seeded complex Gaussian data of the same shape)."""
     n, spokes, readout = 256, 16384, 512
     traj = ops.golden_means3(spokes, readout)
     S = traj.shape[0]
-    gen = torch.Generator(device="cuda").manual_seed(0x4B53504A ^ (4 << 32))
-    k = torch.randn((coils, S), dtype=torch.complex64, device="cuda", generator=gen)
+    t_sim = time.perf_counter()
+    if simulate:
+        from kspacejet_amd import synth
+        k = synth.kooshball_kspace3(ops, torch, n, coils, traj, synth.scan_seed(4), width=6)
+    else:
+        gen = torch.Generator(device="cuda").manual_seed(0x4B53504A ^ (4 << 32))
+        k = torch.randn((coils, S), dtype=torch.complex64, device="cuda", generator=gen)
     torch.cuda.synchronize()
+    sim_s = time.perf_counter() - t_sim
+    log(f"  3-D k-space {'simulated (FP64 KB W=6 forward NUFFT)' if simulate else 'random'} in {sim_s:.1f} s")
     t0 = time.perf_counter()
     plan = ops.Plan3((n, n, n), S, width=WIDTH).set_trajectory(traj)
     torch.cuda.synchronize()
@@ -590,7 +601,10 @@ def run_volume(torch, ops, lib, coils, coils_per_pass, steps, warmup):
            "plan_items": info["items"], "plan_visits": info["visits"], "plan_device_bytes": info["device_bytes"],
            "workspace_bytes": ws.numel(), "stages_ms_per_volume": stages,
            "gather_gbs": round(gather_bytes / (gather_ms * 1e-3) / 1e9, 1) if gather_ms else None,
-           "data": "seeded complex Gaussian k-space of the workload's shape"}
+           "data": ("10-ellipsoid phantom x 20 3-D Gaussian coil sensitivities, FP64 Kaiser-Bessel W=6 forward "
+                    "NUFFT on the 2x grid, complex Gaussian noise 1e-3 max|k|, stored complex64"
+                    if simulate else "seeded complex Gaussian k-space of the workload's shape"),
+           "simulation_s": round(sim_s, 2)}
     plan.close()
     del ws, rss, k
     return res

@@ -393,6 +393,19 @@ int ksj_gpu_adjoint3_execute(const ksj_gpu_plan3* plan, const void* d_kspace, ui
  * k_j = (j - M/2)/M cycles per pixel; FP64 on the device, rounded to float. */
 int ksj_gpu_golden_means3(float* d_traj, uint32_t spokes, uint32_t readout, void* stream);
 
+/* FP64 forward NUFFT of a volume, for simulating k-space: y_s ~ sum_v x_v exp(-i w_s . (v - o))
+ * for x [Nz][Ny][Nx] complex128 (image = {Nz, Ny, Nx}, each side 1..512 with twice the side a
+ * power of two), origin o = {oz, oy, ox} and a trajectory [S][3] (z, y, x) float rad/px;
+ * y [S] complex128.  Kaiser-Bessel width 4, 6 or 8 on the 2x grid (Beatty beta), the same
+ * kernel and deapodization as the adjoint, FP64 throughout (scatter of x / phi_hat, a radix-2
+ * FFT per axis, W^3-tap interpolation).  No reference counterpart: the reference has no 3-D
+ * path; the 3-D workload's data are defined as an FP64 width-6 NUFFT of a phantom.
+ * Workspace: ksj_gpu_nufft3_forward_f64_workspace_bytes (the complex128 grid).  Asynchronous. */
+size_t ksj_gpu_nufft3_forward_f64_workspace_bytes(const uint32_t* image);
+int ksj_gpu_nufft3_forward_f64(const void* d_image, const uint32_t* image, const double* origin, const float* d_traj,
+                               uint64_t samples, uint32_t width, void* d_out, void* d_workspace,
+                               size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------ HIP graphs */
 
 /* Captures the work enqueued on `stream` (and on streams it forks to through events)

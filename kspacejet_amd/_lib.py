@@ -133,6 +133,9 @@ SIGNATURES = {
     "ksj_gpu_adjoint3_workspace_bytes": (_sz, [_vp, _u32, _u32]),
     "ksj_gpu_adjoint3_execute": (_int, [_vp, _vp, _u32, _u32, _vp, _vp, _vp, _sz, _vp]),
     "ksj_gpu_golden_means3": (_int, [_vp, _u32, _u32, _vp]),
+    "ksj_gpu_nufft3_forward_f64_workspace_bytes": (_sz, [C.POINTER(_u32)]),
+    "ksj_gpu_nufft3_forward_f64": (_int, [_vp, C.POINTER(_u32), C.POINTER(_dbl), _vp, _u64, _u32, _vp, _vp, _sz,
+                                          _vp]),
     "ksj_gpu_normalize_trajectory2": (_int, [_vp, _u64, _int, _u32, _u32, _vp, _vp]),
     "ksj_gpu_trajectory_check": (_int, [_vp, _vp, _u64, C.c_float, _vp, _vp]),
     "ksj_gpu_channel_mix": (_int, [_vp, _vp, _vp, _u32, _u32, _u64, _vp]),

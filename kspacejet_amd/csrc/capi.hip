@@ -10,6 +10,7 @@
 #include "plan.hpp"
 #include "plan3.hpp"
 #include "prof.hpp"
+#include "sim3.hpp"
 #include "recon.hpp"
 #include "recon3.hpp"
 #include "serve.hpp"
@@ -580,6 +581,22 @@ int ksj_gpu_chain_faults(uint64_t* out_count) {
   return guarded([&] {
     ksj::gpu::require(out_count != nullptr, "null argument");
     *out_count = ksj::gpu::grid_chain_faults();
+  });
+}
+
+size_t ksj_gpu_nufft3_forward_f64_workspace_bytes(const uint32_t* image) {
+  if (image == nullptr) return 0;
+  return ksj::gpu::nufft3_forward_f64_workspace_bytes(image);
+}
+
+int ksj_gpu_nufft3_forward_f64(const void* d_image, const uint32_t* image, const double* origin, const float* d_traj,
+                               uint64_t samples, uint32_t width, void* d_out, void* d_workspace,
+                               size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    ksj::gpu::require(image != nullptr && origin != nullptr, "null argument");
+    ksj::gpu::nufft3_forward_f64(static_cast<const double2*>(d_image), image, origin, d_traj, samples,
+                                 static_cast<int>(width), static_cast<double2*>(d_out), d_workspace, workspace_bytes,
+                                 as_stream(stream));
   });
 }
 

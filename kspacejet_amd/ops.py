@@ -726,3 +726,25 @@ def sensitivity_maps_walsh(coil_images: torch.Tensor, window: int = 5, iteration
     call("ksj_gpu_sensitivity_maps_walsh", _ptr(coil_images), rows, cols, c, int(window), int(iterations), _ptr(maps),
          _stream())
     return maps
+
+
+def nufft3_forward_f64(image: torch.Tensor, traj: torch.Tensor, origin=None, width: int = 6) -> torch.Tensor:
+    """FP64 Kaiser-Bessel forward NUFFT of a volume [Nz, Ny, Nx] complex128 at a trajectory
+    [S, 3] float32 (z, y, x) rad/px: [S] complex128 ~ sum_v x_v exp(-i w . (v - origin)),
+    origin (N - 1) / 2 per axis by default (see ksj_gpu_nufft3_forward_f64)."""
+    if image.dim() != 3:
+        raise ValueError("image must be [Nz, Ny, Nx]")
+    _need(image, torch.complex128, "image")
+    if traj.dim() != 2 or traj.shape[1] != 3:
+        raise ValueError("traj must be [S, 3]")
+    _need(traj, torch.float32, "traj")
+    nz, ny, nx = image.shape
+    if origin is None:
+        origin = ((nz - 1) / 2, (ny - 1) / 2, (nx - 1) / 2)
+    dims = (C.c_uint32 * 3)(nz, ny, nx)
+    org = (C.c_double * 3)(*[float(o) for o in origin])
+    out = torch.empty(traj.shape[0], dtype=torch.complex128, device=image.device)
+    ws = _workspace(load().ksj_gpu_nufft3_forward_f64_workspace_bytes(dims), image.device)
+    call("ksj_gpu_nufft3_forward_f64", _ptr(image), dims, org, _ptr(traj), traj.shape[0], int(width), _ptr(out),
+         _ptr(ws), ws.numel(), _stream())
+    return out

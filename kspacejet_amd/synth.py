@@ -138,3 +138,78 @@ def uneven_items2(count: int, config: int = 3) -> list[tuple[int, int, int]]:
         u = uniform(scan_seed(config, i), 3, offset=1 << 24)
         out.append((UNEVEN_MATRICES[int(u[0] * 4)], 8 + int(u[1] * 25), 101 + int(u[2] * 503)))
     return out
+
+
+# 3-D ellipsoid phantom: intensity, semi-axes a (x), b (y), c (z), centre x, y, z, angle about
+# z (deg); the 2-D table above extended along z (outer shells, ventricles, small features).
+_ELLIPSOIDS = np.array([
+    [1.0, 0.6900, 0.9200, 0.8100, 0.00, 0.0000, 0.00, 0.0],
+    [-0.8, 0.6624, 0.8740, 0.7800, 0.00, -0.0184, 0.00, 0.0],
+    [-0.2, 0.1100, 0.3100, 0.2200, 0.22, 0.0000, 0.00, -18.0],
+    [-0.2, 0.1600, 0.4100, 0.2800, -0.22, 0.0000, 0.00, 18.0],
+    [0.1, 0.2100, 0.2500, 0.4100, 0.00, 0.3500, -0.15, 0.0],
+    [0.1, 0.0460, 0.0460, 0.0500, 0.00, 0.1000, 0.25, 0.0],
+    [0.1, 0.0460, 0.0460, 0.0500, 0.00, -0.1000, 0.25, 0.0],
+    [0.1, 0.0460, 0.0230, 0.0500, -0.08, -0.6050, 0.00, 0.0],
+    [0.1, 0.0230, 0.0230, 0.0200, 0.00, -0.6060, 0.00, 0.0],
+    [0.1, 0.0230, 0.0460, 0.0200, 0.06, -0.6050, 0.00, 0.0],
+])
+
+
+def phantom3(n: int, seed: int, torch, device="cuda"):
+    """[n, n, n] (z, y, x) float64 torch tensor: the 10-ellipsoid phantom (values clipped to
+    [0, 1]), each centre offset by U(-2, 2) voxels per axis."""
+    off = (uniform(seed, 30, offset=3000) * 4.0 - 2.0).reshape(10, 3)
+    half = n / 2.0
+    c = (torch.arange(n, dtype=torch.float64, device=device) - (n - 1) / 2.0) / half
+    z = c.view(n, 1, 1)
+    y = -c.view(1, n, 1)
+    x = c.view(1, 1, n)
+    img = torch.zeros((n, n, n), dtype=torch.float64, device=device)
+    for k, (amp, a, b, cc, x0, y0, z0, ang) in enumerate(_ELLIPSOIDS):
+        x0 = x0 + off[k, 0] / half
+        y0 = y0 + off[k, 1] / half
+        z0 = z0 + off[k, 2] / half
+        t = np.deg2rad(ang)
+        xr = (x - x0) * np.cos(t) + (y - y0) * np.sin(t)
+        yr = -(x - x0) * np.sin(t) + (y - y0) * np.cos(t)
+        img += amp * (((xr / a) ** 2 + (yr / b) ** 2 + ((z - z0) / cc) ** 2) <= 1.0).to(torch.float64)
+    return img.clamp_(0.0, 1.0)
+
+
+def coil_map3(n: int, coil: int, coils: int, seed: int, torch, device="cuda", sigma: float | None = None,
+              radius: float | None = None):
+    """[n, n, n] complex128 torch tensor: coil `coil` of `coils` 3-D Gaussian sensitivities
+    (sigma = 90 voxels, centres on a sphere of radius 140 voxels at n = 256, both scaled with
+    n; centres on a Fibonacci sphere) with a seeded random phase."""
+    sigma = 90.0 * n / 256.0 if sigma is None else sigma
+    radius = 140.0 * n / 256.0 if radius is None else radius
+    phase = float(uniform(seed, coils, offset=4000)[coil] * 2.0 * np.pi)
+    golden = np.pi * (3.0 - np.sqrt(5.0))
+    cz = 1.0 - 2.0 * (coil + 0.5) / coils
+    r = np.sqrt(max(0.0, 1.0 - cz * cz))
+    cy, cx = r * np.sin(golden * coil), r * np.cos(golden * coil)
+    centre = (n - 1) / 2.0
+    g = torch.arange(n, dtype=torch.float64, device=device)
+    dz = (g - (centre + radius * cz)).view(n, 1, 1) ** 2
+    dy = (g - (centre + radius * cy)).view(1, n, 1) ** 2
+    dx = (g - (centre + radius * cx)).view(1, 1, n) ** 2
+    mag = torch.exp(-(dz + dy + dx) / (2.0 * sigma ** 2))
+    return torch.polar(mag, torch.full_like(mag, phase))
+
+
+def kooshball_kspace3(ops, torch, n: int, coils: int, traj, seed: int, width: int = 6, noise: float = 1e-3):
+    """[coils, S] complex64 k-space of the 3-D phantom times each coil's sensitivity,
+    simulated by the FP64 Kaiser-Bessel forward NUFFT of `width` on the 2x grid, plus complex
+    Gaussian noise of per-component sigma = noise x max|k| (torch's generator, seeded)."""
+    obj = phantom3(n, seed, torch)
+    out = torch.empty((coils, traj.shape[0]), dtype=torch.complex128, device=traj.device)
+    for c in range(coils):
+        img = obj * coil_map3(n, c, coils, seed, torch)
+        out[c] = ops.nufft3_forward_f64(img.contiguous(), traj, width=width)
+        del img
+    s = noise * float(out.abs().max())
+    g = torch.Generator(device=traj.device).manual_seed(seed & 0x7FFFFFFFFFFFFFFF)
+    out += torch.complex(torch.randn(out.shape, dtype=torch.float64, device=traj.device, generator=g),
+                         torch.randn(out.shape, dtype=torch.float64, device=traj.device, generator=g)) * s
+    return out.to(torch.complex64)

@@ -428,6 +428,26 @@ void ora_nudft3_adjoint(size_t nz, size_t ny, size_t nx, double oz, double oy, d
       }
 }
 
+void ora_nudft3_forward(size_t nz, size_t ny, size_t nx, double oz, double oy, double ox, size_t samples,
+                        const double* traj, const double* image, double* values) {
+  /* y_s = sum_v x_v exp(-i w_s . (v - o)): the exact forward model the 3-D data follow */
+  for (size_t s = 0; s < samples; ++s) {
+    double acc_re = 0.0, acc_im = 0.0;
+    for (size_t z = 0; z < nz; ++z)
+      for (size_t y = 0; y < ny; ++y)
+        for (size_t x = 0; x < nx; ++x) {
+          const double phase = -(traj[3 * s] * ((double)z - oz) + traj[3 * s + 1] * ((double)y - oy) +
+                                 traj[3 * s + 2] * ((double)x - ox));
+          const double cs = cos(phase), sn = sin(phase);
+          const double xr = image[2 * ((z * ny + y) * nx + x)], xi = image[2 * ((z * ny + y) * nx + x) + 1];
+          acc_re += xr * cs - xi * sn;
+          acc_im += xr * sn + xi * cs;
+        }
+    values[2 * s] = acc_re;
+    values[2 * s + 1] = acc_im;
+  }
+}
+
 void ora_ramp_sq_dcf3(size_t samples, const double* traj, double* dcf) {
   /* |k|^2 in (radians per pixel)^2: the density of a 3-D radial trajectory falls as 1/|k|^2 */
   for (size_t s = 0; s < samples; ++s)

@@ -111,6 +111,10 @@ int ora_normalize_trajectory2(size_t samples, const float* raw, int units, uint3
 void ora_nudft3_adjoint(size_t nz, size_t ny, size_t nx, double oz, double oy, double ox, size_t samples,
                         const double* traj, const double* values, double* image);
 /* |k|^2 per sample. */
+/* Exact 3-D forward model: y_s = sum_v x_v exp(-i w_s . (v - o)), the transpose of
+ * ora_nudft3_adjoint; image [nz][ny][nx] and values [S] interleaved complex128. */
+void ora_nudft3_forward(size_t nz, size_t ny, size_t nx, double oz, double oy, double ox, size_t samples,
+                        const double* traj, const double* image, double* values);
 void ora_ramp_sq_dcf3(size_t samples, const double* traj, double* dcf);
 /* Bin (cz * gy + cy) * gx + cx per sample from the float coordinates, stable counting sort. */
 int ora_bin3(size_t samples, const float* traj, uint32_t gz, uint32_t gy, uint32_t gx, uint32_t* bins,

@@ -76,6 +76,7 @@ def lib():
         L.ora_coil_compression_basis_estimate.argtypes = [_sz, _sz, _sz, _fp, _dp]
         L.ora_phase_correction_estimate.argtypes = [_sz, _sz, _sz, _fp, _fp]
         L.ora_walsh_maps.argtypes = [_sz, _sz, _sz, _sz, _sz, _dp, _dp]
+        L.ora_nudft3_forward.argtypes = [_sz, _sz, _sz, C.c_double, C.c_double, C.c_double, _sz, _dp, _dp, _dp]
         L.ora_normalize_trajectory2.argtypes = [_sz, _fp, C.c_int, C.c_uint32, C.c_uint32, _fp]
         L.ora_channel_mix.argtypes = [_sz, _sz, _sz, _dp, _dp, _dp]
         L.ora_phase_correct.argtypes = [_sz, _sz, _sz, _fp, _fp, _fp]
@@ -336,6 +337,19 @@ def nudft3_adjoint(values, traj, shape, origin=None) -> np.ndarray:
     out = np.zeros(shape, np.complex128)
     lib().ora_nudft3_adjoint(nz, ny, nx, *map(float, origin), t.shape[0], t.reshape(-1), v.view(np.float64),
                              out.view(np.float64).reshape(-1))
+    return out
+
+
+def nudft3_forward(image, traj, origin=None) -> np.ndarray:
+    """Exact FP64 forward model: [S] complex128 = sum_v x_v exp(-i w_s . (v - origin))."""
+    x = _c128(image)
+    nz, ny, nx = x.shape
+    t = _traj3(traj)
+    if origin is None:
+        origin = ((nz - 1) / 2, (ny - 1) / 2, (nx - 1) / 2)
+    out = np.zeros(t.shape[0], np.complex128)
+    lib().ora_nudft3_forward(nz, ny, nx, float(origin[0]), float(origin[1]), float(origin[2]), t.shape[0],
+                             t.reshape(-1), x.view(np.float64).reshape(-1), out.view(np.float64))
     return out
 
 

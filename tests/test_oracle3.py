@@ -112,3 +112,22 @@ def test_kb_adjoint3_voxels_equal_the_full_volume(shape, width):
     # a wrong crop index (the voxel next to it) is caught
     other = O.kb_adjoint3_voxels(y, t, dcf, shape, zs, ys, [x + 1 for x in xs[:-1]] + [0], width=width)
     assert rel_l2(other, ref) > 1e-3
+
+
+def test_nudft3_forward_is_the_brute_force_sum_and_the_adjoint_transpose():
+    """ora_nudft3_forward against a numpy sum, and <A x, y> = <x, A^H y> with the adjoint."""
+    rng = np.random.default_rng(31)
+    shape, S = (3, 4, 5), 17
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    t = rng.uniform(-np.pi, np.pi, (S, 3))
+    o = ((shape[0] - 1) / 2, (shape[1] - 1) / 2, (shape[2] - 1) / 2)
+    zz, yy, xx = np.meshgrid(*[np.arange(n) - oo for n, oo in zip(shape, o)], indexing="ij")
+    want = np.array([np.sum(x * np.exp(-1j * (w[0] * zz + w[1] * yy + w[2] * xx))) for w in t])
+    got = O.nudft3_forward(x, t)
+    assert np.max(np.abs(got - want)) < 1e-12 * np.max(np.abs(want))
+    y = rng.standard_normal(S) + 1j * rng.standard_normal(S)
+    lhs = np.vdot(y, got)
+    rhs = np.vdot(O.nudft3_adjoint(y, t, shape), x)
+    assert abs(lhs - rhs) < 1e-11 * abs(lhs)
+    shifted = O.nudft3_forward(x, t, origin=(o[0] + 1, o[1], o[2]))
+    assert np.max(np.abs(shifted - want)) > 1e-3  # the origin is honoured
