@@ -220,6 +220,16 @@ __device__ inline uint64_t line_base(const FftLines& L, uint64_t line) {
 // Element e of a line is read as zero when e lies in the line's zero span [zs.x, zs.y).
 __device__ inline bool in_span(uint32_t e, uint2 zs) { return e - zs.x < zs.y - zs.x; }
 
+// The zero span of a source plane, and whether a line's outputs are skipped.
+__device__ inline uint2 plane_zero_span(const FftArgs& a, uint64_t plane) {
+  if (!a.zero_span) return make_uint2(0, 0);
+  return a.zero_span[a.zero_span_period ? plane % a.zero_span_period : plane];
+}
+__device__ inline bool plane_skipped(const FftArgs& a, uint64_t plane) {
+  if (!a.skip_span) return false;
+  return in_span(static_cast<uint32_t>(plane % a.skip_div), a.skip_span[plane / a.skip_div]);
+}
+
 template <int N, int V>
 __device__ inline void load_lines(const FftArgs& a, float2* buf, int ls, int lpb_log2, const uint64_t* src_base,
                                   const uint2* zspan) {
@@ -313,8 +323,8 @@ __device__ inline void fft_lds_block(const FftArgs& a, const float2* __restrict_
   for (int ly = threadIdx.x; ly < lpb; ly += blockDim.x) {
     const uint64_t line = first + ly;
     src_base[ly] = line < total ? line_base(a.src, line) : ~0ull;
-    dst_base[ly] = line < total ? line_base(a.dst, line) : ~0ull;
-    zspan[ly] = (a.zero_span && line < total) ? a.zero_span[line / a.src.lines] : make_uint2(0, 0);
+    dst_base[ly] = (line < total && !plane_skipped(a, line / a.src.lines)) ? line_base(a.dst, line) : ~0ull;
+    zspan[ly] = line < total ? plane_zero_span(a, line / a.src.lines) : make_uint2(0, 0);
     const uint64_t wi = a.line_weight_by_plane ? line / a.src.lines : line % a.src.lines;
     lw[ly] = (a.line_weight && line < total) ? a.line_weight[wi] : 1.0f;
   }
@@ -471,8 +481,8 @@ __device__ inline void fft_reg_block(const FftArgs& a, const ColumnOutputs& co, 
   for (int l = threadIdx.x; l < lpb; l += blockDim.x) {
     const uint64_t line = first + l;
     src_base[l] = line < total ? line_base(a.src, line) : ~0ull;
-    dst_base[l] = line < total ? line_base(a.dst, line) : ~0ull;
-    zspan[l] = (a.zero_span && line < total) ? a.zero_span[line / a.src.lines] : make_uint2(0, 0);
+    dst_base[l] = (line < total && !plane_skipped(a, line / a.src.lines)) ? line_base(a.dst, line) : ~0ull;
+    zspan[l] = line < total ? plane_zero_span(a, line / a.src.lines) : make_uint2(0, 0);
     const uint64_t wi = a.line_weight_by_plane ? line / a.src.lines : line % a.src.lines;
     lw[l] = (a.line_weight && line < total) ? a.line_weight[wi] : 1.0f;
   }
@@ -595,7 +605,7 @@ __global__ __launch_bounds__(256) void dft_direct_kernel(FftArgs a, const float2
     if (line < total) {
       uint32_t src = static_cast<uint32_t>(e) + a.in_offset;
       if (src >= static_cast<uint32_t>(n)) src -= n;
-      const uint2 zs = a.zero_span ? a.zero_span[line / a.src.lines] : make_uint2(0, 0);
+      const uint2 zs = plane_zero_span(a, line / a.src.lines);
       if (!in_span(src, zs)) v = a.in[line_base(a.src, line) + src * a.src.elem_stride];
     }
     buf[ly * ls + e] = v;
@@ -615,7 +625,7 @@ __global__ __launch_bounds__(256) void dft_direct_kernel(FftArgs a, const float2
   for (int idx = threadIdx.x; idx < lpb * static_cast<int>(a.n_out); idx += blockDim.x) {
     const int ly = idx / a.n_out, e = idx % a.n_out;
     const uint64_t line = first + ly;
-    if (line >= total) continue;
+    if (line >= total || plane_skipped(a, line / a.src.lines)) continue;
     uint32_t bin = static_cast<uint32_t>(e) + a.out_offset;
     if (bin >= static_cast<uint32_t>(n)) bin -= n;
     float w = a.scale;

@@ -39,8 +39,14 @@ struct FftArgs {
   // line_weight index: line % lines (0) or the line's plane (1).
   int line_weight_by_plane = 0;
   // Per source plane: elements [x, y) of each of its lines are zero and are not read
-  // (null: every element is read).
+  // (null: every element is read).  With zero_span_period > 0 the span of plane p is
+  // zero_span[p % zero_span_period].
   const uint2* zero_span = nullptr;
+  uint32_t zero_span_period = 0;
+  // Lines whose outputs are not stored (all zero; the next pass reads them as zero): plane
+  // p is skipped when p % skip_div lies in skip_span[p / skip_div] (null: every line stored).
+  const uint2* skip_span = nullptr;
+  uint32_t skip_div = 1;
 };
 
 // Outputs of a column pass whose lines are (plane = image column c, line = stacked coil

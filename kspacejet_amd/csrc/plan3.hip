@@ -322,6 +322,23 @@ __global__ void zero_span3_kernel(const uint8_t* visited, uint32_t gz, uint32_t 
   span[r] = s;
 }
 
+// Per z-plane: the run of rows around G_y / 2 whose zero span covers the whole row.
+__global__ void zero_rows3_kernel(const uint2* zero_cols, uint32_t gz, uint32_t gy, uint32_t gx, uint2* rows) {
+  const uint32_t z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= gz) return;
+  const uint2* row = zero_cols + static_cast<uint64_t>(z) * gy;
+  auto empty = [&](uint32_t y) { return row[y].x == 0 && row[y].y >= gx; };
+  const uint32_t mid = gy / 2;
+  uint2 s = make_uint2(0, 0);
+  if (empty(mid)) {
+    uint32_t lo = mid, hi = mid + 1;
+    while (lo > 0 && empty(lo - 1)) --lo;
+    while (hi < gy && empty(hi)) ++hi;
+    s = make_uint2(lo, hi);
+  }
+  rows[z] = s;
+}
+
 }  // namespace
 
 Plan3* plan3_create(const Plan3Desc& desc) {
@@ -392,6 +409,7 @@ Plan3* plan3_create(const Plan3Desc& desc) {
     if (p->scheme == kSchemeMatrix) {
       p->block_chunk_off = alloc3<uint32_t>(p->nblocks + 1, &total);
       p->zero_cols = alloc3<uint2>(static_cast<uint64_t>(desc.grid[0]) * desc.grid[1], &total);
+      p->zero_rows = alloc3<uint2>(desc.grid[0], &total);
       p->block_visited = alloc3<uint8_t>(p->nblocks, &total);
     }
     p->frac = alloc3<double>(3 * S, &total);
@@ -419,7 +437,7 @@ void plan3_destroy(Plan3* p) {
                   p->block_first_slot, p->item_info, p->multi_blocks, p->frac,     p->sorted_keys,
                   p->counters,   p->sort_scratch, p->deapod[0],    p->deapod[1],  p->deapod[2],
                   p->visit_pos,  p->block_chunk_off, p->item_cost, p->item_scratch, p->run_first,
-                  p->zero_cols,  p->block_visited};
+                  p->zero_cols,  p->block_visited, p->zero_rows};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -494,6 +512,9 @@ void plan3_set_trajectory(Plan3* p, const float* d_traj, const float* d_dcf, hip
     const uint64_t grid_rows = static_cast<uint64_t>(p->d.grid[0]) * p->d.grid[1];
     hipLaunchKernelGGL(zero_span3_kernel, dim3(ceil_div(grid_rows, 256)), dim3(256), 0, stream, p->block_visited,
                        p->d.grid[0], p->d.grid[1], p->d.grid[2], p->blocks[1], p->blocks[2], p->side, p->zero_cols);
+    KSJ_LAUNCH_CHECK();
+    hipLaunchKernelGGL(zero_rows3_kernel, dim3(ceil_div(p->d.grid[0], 256)), dim3(256), 0, stream, p->zero_cols,
+                       p->d.grid[0], p->d.grid[1], p->d.grid[2], p->zero_rows);
     KSJ_LAUNCH_CHECK();
   } else {
     KSJ_HIP(hipMemcpyAsync(&visits, p->block_visit_off + p->nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));

@@ -81,6 +81,10 @@ struct Plan3 {
   // of visitless blocks around column G_x / 2 (every column for rows outside the sampled
   // ball); the gather does not write them and the x FFT reads them as zero (grid_prune).
   uint2* zero_cols = nullptr;        // [G_z * G_y] or null
+  // Per z-plane: the run of rows y around G_y / 2 whose zero span is the whole row (rows
+  // outside the sampled ball); the x pass does not store them and the y pass reads them
+  // as zero.
+  uint2* zero_rows = nullptr;        // [G_z] or null
   uint8_t* block_visited = nullptr;  // [nblocks] scratch of zero_cols
   // build scratch
   double* frac = nullptr;          // [S][3]

@@ -514,6 +514,9 @@ void adjoint3_execute(const Plan3& plan, const float2* d_kspace, uint32_t channe
     fx.load_lines_fast = 1;
     fx.store_lines_fast = 0;
     fx.zero_span = zero_cols;  // per plane (z, y): the columns the gather left unwritten
+    const uint2* zero_rows = zero_cols ? plan.zero_rows : nullptr;
+    fx.skip_span = zero_rows;  // rows outside the sampled ball: all zero, not stored
+    fx.skip_div = Gy;
     {
       ProfScope ps("fft3_x", stream);
       fft_launch(fx, stream);
@@ -537,6 +540,8 @@ void adjoint3_execute(const Plan3& plan, const float2* d_kspace, uint32_t channe
     fy.out_offset = (Gy - Ny / 2) % Gy;
     fy.out_weight = plan.deapod[1];
     fy.line_weight = plan.deapod[2];
+    fy.zero_span = zero_rows;  // per plane (coil, z): the rows the x pass did not store
+    fy.zero_span_period = Gz;
     fy.inverse = 1;
     fy.load_lines_fast = fy.store_lines_fast = 1;
     {
