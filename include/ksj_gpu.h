@@ -517,7 +517,7 @@ void ksj_gpu_serve_destroy(ksj_gpu_server* server);
  *   "gather_coils_per_lane" 1 or 2 coils per gather lane (one 8- or 16-byte load per visit) from 32
  *                          stacked coils up
  *   "block_group"          1, 2 or 4: side of the groups of 2x2 blocks numbered together (new plans)
- *   "multi_fused"          0 or 1: form of ksj_gpu_adjoint2_execute_multi (see there)
+ *   "multi_fused"          0, 1 or 2: form of ksj_gpu_adjoint2_execute_multi (see there)
  *   "fft_schedule"         0 (default), 1 or 2: radix schedule of every LDS FFT length; 3: 1 for
  *                          length 512, 0 for the others (for overlapped serving; rounding only)
  *   "gather_scheme"        0 = 2x2 blocks, 1 = 64-cell LDS tiles, 2 = 4x4 blocks on the matrix
@@ -557,6 +557,11 @@ void ksj_gpu_serve_destroy(ksj_gpu_server* server);
  *                          the first and last radix pass in registers, and 2-D radial coil-last
  *                          outputs take one column pass that writes the coil images and RSS itself
  *                          (default); 0 = every pass staged through LDS (same bits)
+ *   "row_order"            coil-fastest row-pass output of the 2-D column-epilogue pipeline: 0 = grid-row
+ *                          major (default), 1 = column major (same bits)
+ *   "grid_prune"           1 = cells of the oversampled grid that no sample reaches are neither written by
+ *                          the 2-D / 3-D matrix gathers nor read by the first FFT pass, which takes them as
+ *                          zero (default); 0 = every cell written and read (same bits)
  *   "serve_copy_threads"   worker threads (0..15, default 7) a server created afterwards adds to the
  *                          submitting thread for memcpy of host-memory requests into and out of its pinned
  *                          staging (copies of 4 MiB or more; same bits)

@@ -105,6 +105,8 @@ enum TuneKey : int {
   kTuneMatrix3TapShare,   // 3-D matrix gather: 1 = one or two tap words per lane shared by bpermute (same bits)
   kTuneGridPrune,         // 2-D matrix schedule: 1 = cells no sample reaches (around column G/2) are neither written
                           // by the gather nor read by the row FFT, which takes them as zero (same bits)
+  kTuneRowOrder,          // coil-fastest row-pass output of the column-epilogue pipeline: 0 = grid-row major
+                          // [G_r][N_c][Cp], 1 = column major [N_c][G_r][Cp] (same bits)
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);

@@ -449,25 +449,78 @@ __host__ __device__ constexpr int reg_lds_elems(int lpb, int pad) {
   return N * (lpb + pad);
 }
 
-// Logical block of a launch of `blocks` blocks: consecutive logical blocks on one XCD
-// (hardware places block b on XCD b % 8), so lines that share output cache lines meet
-// in one L2.
-__device__ inline uint64_t xcd_block(uint64_t blocks) {
-  const uint64_t b = blockIdx.x;
-  if (blocks % 8 != 0) return b;
-  return (b % 8) * (blocks / 8) + b / 8;
+// One thread's line of a register-form group: where it is read and stored, the
+// span read as zero and its weight.  A line past the end reads line 0 and is never
+// stored (its transform keeps every wave's loads unconditional).
+struct RegLineView {
+  uint64_t src, dst;  // element 0 of the source / destination line; dst ~0: not stored
+  uint2 zs;
+  float lw;
+  uint32_t plane, line;  // the line's plane and index within the plane
+  bool live;
+};
+
+// l / a.src.lines: a multiply-high by the launch's magic number where it is exact.
+__device__ inline uint32_t line_plane(const FftArgs& a, uint32_t l) {
+  if (a.lines_magic) return __umulhi(l, a.lines_magic);
+  if (a.src.lines == 1) return l;
+  return l / a.src.lines;
 }
 
+__device__ inline RegLineView reg_line_view(const FftArgs& a, uint32_t line, uint32_t total) {
+  RegLineView r;
+  r.live = line < total;
+  const uint32_t l = r.live ? line : 0u;
+  r.plane = line_plane(a, l);
+  r.line = l - r.plane * a.src.lines;
+  r.src = r.plane * a.src.plane_stride + r.line * a.src.line_stride;
+  r.dst = (r.live && !plane_skipped(a, r.plane)) ? r.plane * a.dst.plane_stride + r.line * a.dst.line_stride : ~0ull;
+  r.zs = r.live ? plane_zero_span(a, r.plane) : make_uint2(0, 0);
+  r.lw = (a.line_weight && r.live) ? a.line_weight[a.line_weight_by_plane ? r.plane : r.line] : 1.0f;
+  return r;
+}
+
+// Element t + i T of the line into x.v[i] (rotated by in_offset; the zero span reads 0).
+template <int N, int V>
+__device__ inline void reg_load(const FftArgs& a, const RegLineView& w, RegLine<N, V>& x, int t) {
+  constexpr int T = Shape<N, V>::T;
+  constexpr int E = N / T;
+  const uint32_t es = static_cast<uint32_t>(a.src.elem_stride);
+  typedef float F2v __attribute__((ext_vector_type(2)));
+  // Every input element of a pass is read once: non-temporal, out of the caches' way.
+  if (a.in_offset == 0 && !a.zero_span) {  // no rotation, no zero span: plain strided loads
+    const float2* p = a.in + w.src + static_cast<uint64_t>(static_cast<uint32_t>(t) * es);
+    const uint64_t step = static_cast<uint64_t>(T) * es;
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const F2v v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(p + i * step));
+      x.v[i] = make_float2(v.x, v.y);
+    }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    uint32_t src = static_cast<uint32_t>(t + i * T) + a.in_offset;
+    if (src >= static_cast<uint32_t>(N)) src -= N;
+    F2v v = {0.f, 0.f};
+    if (!in_span(src, w.zs))
+      v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(a.in + w.src + static_cast<uint64_t>(src * es)));
+    x.v[i] = make_float2(v.x, v.y);
+  }
+}
+
+// The register form of line group g, lines [g lpb, (g + 1) lpb) of the launch: thread
+// (ly, t) = (tid % lpb, tid / lpb) holds elements t + i T of line g lpb + ly.  Each
+// thread finds its own line's addresses (no per-block line tables, no barrier before
+// the loads).
 template <int N, int V, bool INV, bool EPI>
-__device__ inline void fft_reg_block(const FftArgs& a, const ColumnOutputs& co, const float2* __restrict__ tw_global,
-                                     int lpb_log2, int pad, uint64_t block) {
+__device__ inline void fft_reg_group(const FftArgs& a, const ColumnOutputs& co, const float2* __restrict__ tw_global,
+                                     int lpb_log2, int pad, uint32_t g) {
   static_assert(shape_ok<N, V>(), "radix schedule must multiply to N and split evenly over the line's threads");
   constexpr int T = Shape<N, V>::T;
   constexpr int E = N / T;
   extern __shared__ float2 smem[];
-  __shared__ uint64_t src_base[64], dst_base[64];
-  __shared__ uint2 zspan[64];
-  __shared__ float ow[N], lw[64];
+  __shared__ float ow[N];
   const int lpb = 1 << lpb_log2;
   const int S = lpb + pad;
   float2* tw = smem;
@@ -476,89 +529,76 @@ __device__ inline void fft_reg_block(const FftArgs& a, const ColumnOutputs& co, 
     tw[i] = tw_global[i];
     ow[i] = (i < static_cast<int>(a.n_out) && a.out_weight) ? a.scale * a.out_weight[i] : a.scale;
   }
-  const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
-  const uint64_t first = block * lpb;
-  for (int l = threadIdx.x; l < lpb; l += blockDim.x) {
-    const uint64_t line = first + l;
-    src_base[l] = line < total ? line_base(a.src, line) : ~0ull;
-    dst_base[l] = (line < total && !plane_skipped(a, line / a.src.lines)) ? line_base(a.dst, line) : ~0ull;
-    zspan[l] = line < total ? plane_zero_span(a, line / a.src.lines) : make_uint2(0, 0);
-    const uint64_t wi = a.line_weight_by_plane ? line / a.src.lines : line % a.src.lines;
-    lw[l] = (a.line_weight && line < total) ? a.line_weight[wi] : 1.0f;
-  }
-  __syncthreads();
+  const uint32_t total = a.src.planes * a.src.lines;  // below 2^32 (checked at launch)
+  const uint32_t nout = a.n_out;
   const int ly = threadIdx.x & (lpb - 1), t = threadIdx.x >> lpb_log2;
   RegLine<N, V> x;
-  {
-    const uint64_t base = src_base[ly];
-    const uint64_t b0 = base != ~0ull ? base : 0ull;  // a line past the end transforms line 0; never stored
-    const uint32_t es = static_cast<uint32_t>(a.src.elem_stride);
-    const uint2 zs = zspan[ly];
-    typedef float F2v __attribute__((ext_vector_type(2)));
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      uint32_t src = static_cast<uint32_t>(t + i * T) + a.in_offset;
-      if (src >= static_cast<uint32_t>(N)) src -= N;
-      F2v v = {0.f, 0.f};
-      if (!in_span(src, zs))
-        v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(a.in + b0 + static_cast<uint64_t>(src * es)));
-      x.v[i] = make_float2(v.x, v.y);
-    }
-  }
+  reg_load<N, V>(a, reg_line_view(a, g * lpb + ly, total), x, t);
+  __syncthreads();  // twiddles and output weights staged
   reg_transform<N, V, INV>(x, buf, S, ly, tw, t);
-  const uint32_t nout = a.n_out;
-  if constexpr (!EPI) {
-    const uint64_t base = dst_base[ly];
-    if (base == ~0ull) return;
-    const uint32_t es = static_cast<uint32_t>(a.dst.elem_stride);
+  // The view again rather than kept across the transform: fewer live registers.
+  const RegLineView cur = reg_line_view(a, g * lpb + ly, total);
+  {
+    if constexpr (!EPI) {
+      if (cur.dst != ~0ull) {
+        const uint32_t es = static_cast<uint32_t>(a.dst.elem_stride);
 #pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const uint32_t bin = static_cast<uint32_t>(t + i * T);
-      const uint32_t e = bin >= a.out_offset ? bin - a.out_offset : bin + N - a.out_offset;
-      if (e < nout) a.out[base + static_cast<uint64_t>(e * es)] = cscale(x.v[i], ow[e] * lw[ly]);
-    }
-  } else {
-    // Column epilogue: line = (column c = plane, stacked coil k); group g = k / channels
-    // writes coil-last images [N_r][cols][channels] and/or RSS [N_r][cols], the RSS
-    // summed in coil order from |x|^2 staged in LDS (every group lies in one block).
-    const uint64_t line = first + ly;
-    const bool live = line < total;
-    const uint32_t c = live ? static_cast<uint32_t>(line / a.src.lines) : 0u;
-    const uint32_t k = live ? static_cast<uint32_t>(line % a.src.lines) : 0u;
-    const uint32_t g = k / co.channels, kk = k - g * co.channels;
-    float2* img = live ? co.coil_last[g] : nullptr;
-    float* pw = reinterpret_cast<float*>(buf);
-    __syncthreads();  // every thread's last-pass reads are done before buf is reused
+        for (int i = 0; i < E; ++i) {
+          const uint32_t bin = static_cast<uint32_t>(t + i * T);
+          const uint32_t e = bin >= a.out_offset ? bin - a.out_offset : bin + N - a.out_offset;
+          if (e < nout) a.out[cur.dst + static_cast<uint64_t>(e * es)] = cscale(x.v[i], ow[e] * cur.lw);
+        }
+      }
+    } else {
+      // Column epilogue: line = (column c = plane, stacked coil k); group q = k / channels
+      // writes coil-last images [N_r][cols][channels] and/or RSS [N_r][cols], the RSS
+      // summed in coil order from |x|^2 staged in LDS (every group lies in one block).
+      // Channels per group divide the (power of two) lines per block: a power of two.
+      const uint32_t c = cur.plane, k = cur.line, ch = co.channels;
+      const uint32_t ch_log2 = __builtin_ctz(ch);
+      const uint32_t q = k >> ch_log2, kk = k & (ch - 1);
+      float2* img = cur.live ? co.coil_last[q] : nullptr;
+      float* pw = reinterpret_cast<float*>(buf);
+      __syncthreads();  // every thread's last-pass reads are done before buf is reused
 #pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const uint32_t bin = static_cast<uint32_t>(t + i * T);
-      const uint32_t e = bin >= a.out_offset ? bin - a.out_offset : bin + N - a.out_offset;
-      if (e >= nout) continue;
-      const float2 v = cscale(x.v[i], ow[e] * lw[ly]);
-      if (img) img[(static_cast<uint64_t>(e) * co.cols + c) * co.channels + kk] = v;
-      pw[e * (lpb + 1) + ly] = norm2(v);
-    }
-    __syncthreads();
-    const uint32_t gpb = static_cast<uint32_t>(lpb) / co.channels;  // groups per block
-    for (uint32_t q = threadIdx.x; q < nout * gpb; q += blockDim.x) {
-      const uint32_t e = q / gpb, gb = q - e * gpb;
-      const uint64_t l0 = first + gb * co.channels;
-      if (l0 >= total) continue;
-      const uint32_t col = static_cast<uint32_t>(l0 / a.src.lines);
-      const uint32_t grp = static_cast<uint32_t>(l0 % a.src.lines) / co.channels;
-      float* rss = co.rss[grp];
-      if (!rss) continue;
-      float acc = 0.f;
-      for (uint32_t m = 0; m < co.channels; ++m) acc += pw[e * (lpb + 1) + gb * co.channels + m];
-      rss[static_cast<uint64_t>(e) * co.cols + col] = sqrtf(acc);
+      for (int i = 0; i < E; ++i) {
+        const uint32_t bin = static_cast<uint32_t>(t + i * T);
+        const uint32_t e = bin >= a.out_offset ? bin - a.out_offset : bin + N - a.out_offset;
+        if (e >= nout) continue;
+        const float2 v = cscale(x.v[i], ow[e] * cur.lw);
+        if (img) img[(static_cast<uint64_t>(e) * co.cols + c) * ch + kk] = v;
+        pw[e * (lpb + 1) + ly] = norm2(v);
+      }
+      __syncthreads();
+      const uint32_t gpb_log2 = lpb_log2 - ch_log2;  // groups per block (log2)
+      const uint32_t l0g = g * lpb;
+      for (uint32_t r = threadIdx.x; r < (nout << gpb_log2); r += blockDim.x) {
+        const uint32_t e = r >> gpb_log2, gb = r & ((1u << gpb_log2) - 1);
+        const uint32_t l0 = l0g + (gb << ch_log2);
+        if (l0 >= total) continue;
+        const uint32_t col = line_plane(a, l0);
+        float* out = co.rss[(l0 - col * a.src.lines) >> ch_log2];
+        if (!out) continue;
+        float acc = 0.f;
+        for (uint32_t m = 0; m < ch; ++m) acc += pw[e * (lpb + 1) + gb * ch + m];
+        out[static_cast<uint64_t>(e) * co.cols + col] = sqrtf(acc);
+      }
     }
   }
+}
+
+// Group of block b when the groups split evenly over the 8 XCDs (block b runs on XCD
+// b % 8): consecutive groups on one XCD, so lines that share cache lines meet in one L2.
+__device__ inline uint32_t xcd_group(uint32_t groups) {
+  const uint32_t b = blockIdx.x;
+  if (groups % 8 != 0) return b;
+  return (b % 8) * (groups / 8) + b / 8;
 }
 
 template <int N, int V, bool INV, bool EPI>
 __global__ __launch_bounds__(1024) void fft_reg_kernel(FftArgs a, ColumnOutputs co, const float2* __restrict__ tw_global,
                                                        int lpb_log2, int pad) {
-  fft_reg_block<N, V, INV, EPI>(a, co, tw_global, lpb_log2, pad, xcd_block(gridDim.x));
+  fft_reg_group<N, V, INV, EPI>(a, co, tw_global, lpb_log2, pad, xcd_group(gridDim.x));
 }
 
 // Fused items of one length in the register form: each block runs its item's lines.
@@ -568,7 +608,8 @@ __global__ __launch_bounds__(1024) void fft_fused_reg_kernel(const FusedTable<Ff
                                                              int pad) {
   const uint32_t item = fused_item(t, blockIdx.x);
   const FftArgs a = fused_args(t, item);
-  fft_reg_block<N, V, INV, false>(a, ColumnOutputs{}, tw_global, lpb_log2, pad, blockIdx.x - fused_first(t, item));
+  const uint32_t g = static_cast<uint32_t>(blockIdx.x - fused_first(t, item));
+  fft_reg_group<N, V, INV, false>(a, ColumnOutputs{}, tw_global, lpb_log2, pad, g);
 }
 
 template <int N, int V, bool INV>
@@ -713,6 +754,16 @@ int reg_lines_log2(const FftArgs& a, int lpb) {
   return lpb_log2;
 }
 
+// a with lines_magic set for line indices below `span` (the launch's groups x lines per
+// block): ceil(2^32 / lines) gives l / lines exactly by a multiply-high whenever
+// l * lines < 2^32 (the rounding error of the magic, below l / 2^32, stays under 1 / lines).
+inline FftArgs with_lines_magic(const FftArgs& a, uint64_t span) {
+  FftArgs b = a;
+  const uint64_t d = a.src.lines;
+  b.lines_magic = (d >= 2 && span * d < (1ull << 32)) ? static_cast<uint32_t>(((1ull << 32) + d - 1) / d) : 0u;
+  return b;
+}
+
 template <int N, int V, bool EPI>
 void launch_reg(const FftArgs& a, const ColumnOutputs& co, const float2* tw, hipStream_t stream, int lpb) {
   constexpr int T = Shape<N, V>::T;
@@ -720,18 +771,19 @@ void launch_reg(const FftArgs& a, const ColumnOutputs& co, const float2* tw, hip
   const int lines = 1 << lpb_log2;
   if constexpr (EPI) require(lines % co.channels == 0, "column epilogue needs whole coil groups per block");
   const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
+  require(total < (1ull << 32), "register-form FFT launches take fewer than 2^32 lines");
   const int pad = static_cast<int>(tune_get(kTuneFftLdsPad));
   const size_t shmem = reg_shmem<N, V>(lpb_log2);
-  const unsigned grid = ceil_div(total, static_cast<uint64_t>(lines));
-  if (a.inverse) {
-    raise_lds(&fft_reg_kernel<N, V, true, EPI>, shmem);
-    hipLaunchKernelGGL((fft_reg_kernel<N, V, true, EPI>), dim3(grid), dim3(lines * T), shmem, stream, a, co, tw,
-                       lpb_log2, pad);
-  } else {
-    raise_lds(&fft_reg_kernel<N, V, false, EPI>, shmem);
-    hipLaunchKernelGGL((fft_reg_kernel<N, V, false, EPI>), dim3(grid), dim3(lines * T), shmem, stream, a, co, tw,
-                       lpb_log2, pad);
-  }
+  const uint32_t groups = static_cast<uint32_t>(ceil_div(total, static_cast<uint64_t>(lines)));
+  const FftArgs b = with_lines_magic(a, static_cast<uint64_t>(groups) * lines);
+  auto go = [&](auto kernel) {
+    raise_lds(kernel, shmem);
+    hipLaunchKernelGGL(kernel, dim3(groups), dim3(lines * T), shmem, stream, b, co, tw, lpb_log2, pad);
+  };
+  if (a.inverse)
+    go(&fft_reg_kernel<N, V, true, EPI>);
+  else
+    go(&fft_reg_kernel<N, V, false, EPI>);
   KSJ_LAUNCH_CHECK();
 }
 
@@ -768,7 +820,10 @@ void launch_fused(const FftArgs* const* items, uint32_t count, const float2* tw,
     require(items[i]->inverse == items[0]->inverse && items[i]->load_lines_fast == items[0]->load_lines_fast &&
               items[i]->store_lines_fast == items[0]->store_lines_fast,
             "fused FFT items must share direction and thread mapping");
-    b.add(*items[i], ceil_div(static_cast<uint64_t>(items[i]->src.planes) * items[i]->src.lines, lpb));
+    const uint64_t total = static_cast<uint64_t>(items[i]->src.planes) * items[i]->src.lines;
+    require(!reg || total < (1ull << 32), "register-form FFT launches take fewer than 2^32 lines");
+    const uint64_t groups = ceil_div(total, lpb);
+    b.add(reg ? with_lines_magic(*items[i], groups * lpb) : *items[i], groups);
   }
   b.upload(d_table, stream);
   const auto* t = static_cast<const FusedTable<FftArgs>*>(d_table);

@@ -47,6 +47,9 @@ struct FftArgs {
   // p is skipped when p % skip_div lies in skip_span[p / skip_div] (null: every line stored).
   const uint2* skip_span = nullptr;
   uint32_t skip_div = 1;
+  // Set by the launch (callers leave 0): ceil(2^32 / src.lines) when every line index l
+  // satisfies l * src.lines < 2^32, so l / src.lines = mulhi(l, lines_magic); 0: divide.
+  uint32_t lines_magic = 0;
 };
 
 // Outputs of a column pass whose lines are (plane = image column c, line = stacked coil

@@ -164,15 +164,18 @@ FftArgs col_pass_args(const Plan2& plan, uint32_t total, const float2* rowout, f
   return b;
 }
 
-// Row pass of the column-epilogue pipeline: the same lines, stored coil-fastest
-// [G_r][N_c][Cp] (only the N_c kept columns), so both mappings run across coils.
+// Row pass of the column-epilogue pipeline: the same lines, stored coil-fastest (only
+// the N_c kept columns), so both mappings run across coils.  Grid-row major
+// [G_r][N_c][Cp] (row_order 0) or column major [N_c][G_r][Cp] (row_order 1, where
+// each column-pass block reads one contiguous run per column).
 FftArgs row_pass_coil_last_args(const Plan2& plan, uint32_t total, size_t cp, const float2* grid, float2* rowout) {
   FftArgs a = row_pass_args(plan, total, cp, grid, rowout);
+  const bool by_column = tune_get(kTuneRowOrder) == 1;
   a.dst.planes = plan.d.grid_rows;
   a.dst.lines = total;
-  a.dst.plane_stride = static_cast<uint64_t>(plan.d.image_cols) * cp;
+  a.dst.plane_stride = by_column ? cp : static_cast<uint64_t>(plan.d.image_cols) * cp;
   a.dst.line_stride = 1;
-  a.dst.elem_stride = cp;
+  a.dst.elem_stride = by_column ? static_cast<uint64_t>(plan.d.grid_rows) * cp : cp;
   a.store_lines_fast = 1;
   return a;
 }
@@ -184,11 +187,12 @@ FftArgs col_pass_coil_last_args(const Plan2& plan, uint32_t total, size_t cp, co
   const auto& d = plan.d;
   FftArgs b;
   b.in = rowout;
+  const bool by_column = tune_get(kTuneRowOrder) == 1;
   b.src.planes = d.image_cols;
   b.src.lines = total;
-  b.src.plane_stride = cp;
+  b.src.plane_stride = by_column ? static_cast<uint64_t>(d.grid_rows) * cp : cp;
   b.src.line_stride = 1;
-  b.src.elem_stride = static_cast<uint64_t>(d.image_cols) * cp;
+  b.src.elem_stride = by_column ? cp : static_cast<uint64_t>(d.image_cols) * cp;
   b.dst = b.src;
   b.n = d.grid_rows;
   b.n_out = d.image_rows;

@@ -50,6 +50,7 @@ Param g_params[kTuneKeys] = {
     {"matrix3_map", 0, [](int64_t v) { return v >= 0 && v <= 4096; }},
     {"matrix3_tap_share", 0, [](int64_t v) { return v == 0 || v == 1; }},
     {"grid_prune", 1, [](int64_t v) { return v == 0 || v == 1; }},
+    {"row_order", 0, [](int64_t v) { return v == 0 || v == 1; }},
 };
 
 }  // namespace

@@ -9,7 +9,7 @@ coil-fastest plus one column pass that writes each scan's coil-last images and R
 itself; its images equal the staged pipeline's bit for bit and its RSS (the same float
 sum in coil order) too, at 1..32 channels, several scans per pass, uneven matrices
 (grids 384 and 640) and channel counts whose groups do not fit a block (which keep the
-staged pipeline)."""
+staged pipeline), and the headline shape (256, four 16-coil scans stacked in one pass)."""
 import numpy as np
 import pytest
 
@@ -84,7 +84,8 @@ def radial(ops, n, coils, spokes, readout, seed):
 
 
 @pytest.mark.parametrize("n,coils,scans", [(64, 1, 3), (64, 2, 5), (96, 4, 4), (64, 8, 9), (96, 16, 5), (128, 32, 2),
-                                           (192, 16, 2), (320, 8, 2), (64, 12, 3), (50, 16, 2)])
+                                           (192, 16, 2), (320, 8, 2), (64, 12, 3), (50, 16, 2),
+                                           (256, 16, 4)])
 def test_radial_column_epilogue_matches_the_staged_pipeline(ops, tune, n, coils, scans):
     traj, _ = radial(ops, n, coils, 2 * n // 3 + 1, 2 * n, n + coils)
     ks = [torch.from_numpy(radial(ops, n, coils, 2 * n // 3 + 1, 2 * n, 100 + i)[1]).cuda() for i in range(scans)]

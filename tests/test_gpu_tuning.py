@@ -62,7 +62,7 @@ def test_launch_parameters_do_not_change_results(ops, tune):
                         ("gather_coils_per_lane", (1, 2)), ("block_group", (1, 2, 4)), ("matrix_batch", (16, 4, 8)),
                         ("matrix_slab", (16, 1, 5)), ("matrix_pipeline", (1, 2)), ("matrix_nt_store", (1, 0)),
                         ("matrix_run_cost", (12, 1, 64)), ("matrix_tap_share", (1, 0)),
-                        ("matrix_persistent", (0, 8, 1)), ("grid_prune", (1, 0))):
+                        ("matrix_persistent", (0, 8, 1)), ("grid_prune", (1, 0)), ("row_order", (0, 1))):
         # (matrix_block_width changes the summation order: covered by test_gpu_matrix.py)
         for v in values:
             tune(key, v)
