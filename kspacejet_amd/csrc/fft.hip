@@ -325,7 +325,7 @@ __device__ inline void fft_lds_block(const FftArgs& a, const float2* __restrict_
     src_base[ly] = line < total ? line_base(a.src, line) : ~0ull;
     dst_base[ly] = (line < total && !plane_skipped(a, line / a.src.lines)) ? line_base(a.dst, line) : ~0ull;
     zspan[ly] = line < total ? plane_zero_span(a, line / a.src.lines) : make_uint2(0, 0);
-    const uint64_t wi = a.line_weight_by_plane ? line / a.src.lines : line % a.src.lines;
+    const uint64_t wi = (a.line_weight_by_plane ? line / a.src.lines : line % a.src.lines) / a.line_weight_div;
     lw[ly] = (a.line_weight && line < total) ? a.line_weight[wi] : 1.0f;
   }
   __syncthreads();
@@ -476,7 +476,9 @@ __device__ inline RegLineView reg_line_view(const FftArgs& a, uint32_t line, uin
   r.src = r.plane * a.src.plane_stride + r.line * a.src.line_stride;
   r.dst = (r.live && !plane_skipped(a, r.plane)) ? r.plane * a.dst.plane_stride + r.line * a.dst.line_stride : ~0ull;
   r.zs = r.live ? plane_zero_span(a, r.plane) : make_uint2(0, 0);
-  r.lw = (a.line_weight && r.live) ? a.line_weight[a.line_weight_by_plane ? r.plane : r.line] : 1.0f;
+  r.lw = (a.line_weight && r.live)
+             ? a.line_weight[(a.line_weight_by_plane ? r.plane : r.line) / a.line_weight_div]
+             : 1.0f;
   return r;
 }
 
@@ -557,6 +559,7 @@ __device__ inline void fft_reg_group(const FftArgs& a, const ColumnOutputs& co, 
       const uint32_t c = cur.plane, k = cur.line, ch = co.channels;
       const uint32_t ch_log2 = __builtin_ctz(ch);
       const uint32_t q = k >> ch_log2, kk = k & (ch - 1);
+      const uint32_t istride = co.img_stride ? co.img_stride : ch;
       float2* img = cur.live ? co.coil_last[q] : nullptr;
       float* pw = reinterpret_cast<float*>(buf);
       __syncthreads();  // every thread's last-pass reads are done before buf is reused
@@ -566,7 +569,7 @@ __device__ inline void fft_reg_group(const FftArgs& a, const ColumnOutputs& co, 
         const uint32_t e = bin >= a.out_offset ? bin - a.out_offset : bin + N - a.out_offset;
         if (e >= nout) continue;
         const float2 v = cscale(x.v[i], ow[e] * cur.lw);
-        if (img) img[(static_cast<uint64_t>(e) * co.cols + c) * ch + kk] = v;
+        if (img) img[(static_cast<uint64_t>(e) * co.cols + c) * istride + kk] = v;
         pw[e * (lpb + 1) + ly] = norm2(v);
       }
       __syncthreads();
@@ -578,10 +581,20 @@ __device__ inline void fft_reg_group(const FftArgs& a, const ColumnOutputs& co, 
         if (l0 >= total) continue;
         const uint32_t col = line_plane(a, l0);
         float* out = co.rss[(l0 - col * a.src.lines) >> ch_log2];
+        const uint64_t o = static_cast<uint64_t>(e) * co.cols + col;
+        if (co.sumsq) {  // carried across launches (one group per position)
+          float acc = co.first ? 0.f : co.sumsq[o];
+          for (uint32_t m = 0; m < ch; ++m) acc += pw[e * (lpb + 1) + gb * ch + m];
+          if (!co.last)
+            co.sumsq[o] = acc;
+          else if (out)
+            out[o] = sqrtf(acc);
+          continue;
+        }
         if (!out) continue;
         float acc = 0.f;
         for (uint32_t m = 0; m < ch; ++m) acc += pw[e * (lpb + 1) + gb * ch + m];
-        out[static_cast<uint64_t>(e) * co.cols + col] = sqrtf(acc);
+        out[o] = sqrtf(acc);
       }
     }
   }
@@ -671,7 +684,8 @@ __global__ __launch_bounds__(256) void dft_direct_kernel(FftArgs a, const float2
     if (bin >= static_cast<uint32_t>(n)) bin -= n;
     float w = a.scale;
     if (a.out_weight) w *= a.out_weight[e];
-    if (a.line_weight) w *= a.line_weight[a.line_weight_by_plane ? line / a.src.lines : line % a.src.lines];
+    if (a.line_weight)
+      w *= a.line_weight[(a.line_weight_by_plane ? line / a.src.lines : line % a.src.lines) / a.line_weight_div];
     a.out[line_base(a.dst, line) + static_cast<uint64_t>(e) * a.dst.elem_stride] = cscale(res[ly * ls + bin], w);
   }
 }

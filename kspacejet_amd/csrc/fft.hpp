@@ -36,8 +36,10 @@ struct FftArgs {
   // (1).  Pick the one whose addresses are contiguous in memory.
   int load_lines_fast = 0;
   int store_lines_fast = 0;
-  // line_weight index: line % lines (0) or the line's plane (1).
+  // line_weight index: line % lines (0) or the line's plane (1), divided by
+  // line_weight_div (lines that share one weight, e.g. the coils of one column).
   int line_weight_by_plane = 0;
+  uint32_t line_weight_div = 1;
   // Per source plane: elements [x, y) of each of its lines are zero and are not read
   // (null: every element is read).  With zero_span_period > 0 the span of plane p is
   // zero_span[p % zero_span_period].
@@ -61,6 +63,14 @@ struct ColumnOutputs {
   float* rss[64] = {};
   uint32_t channels = 1;
   uint32_t cols = 0;
+  // Coil stride of the coil-last images (0: channels), for images holding more coils
+  // than one group (coil_last[g] then points at the group's first coil).
+  uint32_t img_stride = 0;
+  // Sum of squares carried across launches (one group per block position): the RSS sum
+  // starts from sumsq[e cols + c] unless `first`, and is stored back there unless
+  // `last`, when rss gets its square root (null: every launch is first and last).
+  float* sumsq = nullptr;
+  int first = 1, last = 1;
 };
 
 // Returns true when an LDS Stockham kernel is compiled for length n; other

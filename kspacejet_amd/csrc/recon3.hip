@@ -329,31 +329,27 @@ __global__ __launch_bounds__(256) void gather3_matrix_kernel(Gather3Args a) {
   if (run < a.run_count) matrix3_run<CP, U, TS>(a, run);
 }
 
-// sumsq[p] (+)= sum over the pass's coils of |img[c][p]|^2 in channel order; on the
-// last pass rss[p] = sqrt(sumsq[p]).
+// RSS over the coil-fastest z output [P][coils], carried across coil passes in channel
+// order: s = (first ? 0 : sumsq) + |x_0|^2 + |x_1|^2 + ...
 __global__ void rss3_kernel(const float2* img, uint32_t coils, uint64_t pixels, float* sumsq, float* rss, int first,
                             int last) {
   const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= pixels) return;
   float s = first ? 0.f : sumsq[p];
-  for (uint32_t c = 0; c < coils; ++c) {
-    const float2 v = img[static_cast<uint64_t>(c) * pixels + p];
-    s += norm2(v);
-  }
+  for (uint32_t c = 0; c < coils; ++c) s += norm2(img[p * coils + c]);
   if (last)
     rss[p] = sqrtf(s);
   else
     sumsq[p] = s;
 }
-
-// Coil-major pass images [coils][P] -> channels c0 .. c0 + coils of coil-last [P][C].
+// Coil-fastest z output [P][coils] into coil-last images [P][channels] at channel c0.
 __global__ void coil_last3_kernel(const float2* img, uint32_t coils, uint64_t pixels, float2* out, uint32_t c0,
                                   uint32_t channels) {
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= pixels * coils) return;
   const uint64_t p = i / coils;
-  const uint32_t c = static_cast<uint32_t>(i % coils);
-  out[p * channels + c0 + c] = img[static_cast<uint64_t>(c) * pixels + p];
+  const uint32_t c = static_cast<uint32_t>(i - p * coils);
+  out[p * channels + c0 + c] = img[i];
 }
 
 template <int CP>
@@ -493,7 +489,9 @@ void adjoint3_execute(const Plan3& plan, const float2* d_kspace, uint32_t channe
     }
     const uint32_t Gz = d.grid[0], Gy = d.grid[1], Gx = d.grid[2];
     const uint32_t Nz = d.image[0], Ny = d.image[1], Nx = d.image[2];
-    // x: lines (z, y, coil) of the coil-fastest grid -> [pc][Gz][Gy][Nx], crop to the centred Nx.
+    // Every pass output is coil-fastest, so each pass loads and stores the same element
+    // of consecutive lines (the register form where the length has one).
+    // x: lines (z, y, coil) of the coil-fastest grid -> [Gz][Gy][Nx][pc], crop to the centred Nx.
     FftArgs fx;
     fx.in = grid;
     fx.out = xout;
@@ -504,15 +502,14 @@ void adjoint3_execute(const Plan3& plan, const float2* d_kspace, uint32_t channe
     fx.src.elem_stride = cp;
     fx.dst.planes = Gz * Gy;
     fx.dst.lines = pc;
-    fx.dst.plane_stride = Nx;
-    fx.dst.line_stride = static_cast<uint64_t>(Gz) * Gy * Nx;
-    fx.dst.elem_stride = 1;
+    fx.dst.plane_stride = static_cast<uint64_t>(Nx) * pc;
+    fx.dst.line_stride = 1;
+    fx.dst.elem_stride = pc;
     fx.n = Gx;
     fx.n_out = Nx;
     fx.out_offset = (Gx - Nx / 2) % Gx;
     fx.inverse = 1;
-    fx.load_lines_fast = 1;
-    fx.store_lines_fast = 0;
+    fx.load_lines_fast = fx.store_lines_fast = 1;
     fx.zero_span = zero_cols;  // per plane (z, y): the columns the gather left unwritten
     const uint2* zero_rows = zero_cols ? plan.zero_rows : nullptr;
     fx.skip_span = zero_rows;  // rows outside the sampled ball: all zero, not stored
@@ -521,53 +518,67 @@ void adjoint3_execute(const Plan3& plan, const float2* d_kspace, uint32_t channe
       ProfScope ps("fft3_x", stream);
       fft_launch(fx, stream);
     }
-    // y: lines (coil, z, x) -> [pc][Gz][Ny][Nx]; deapodization of y (per output) and x (per line).
+    // y: lines (z; x, coil) -> [Gz][Ny][Nx][pc]; deapodization of y (per output) and x (per
+    // line, shared by a column's coils).
     FftArgs fy;
     fy.in = xout;
     fy.out = yout;
-    fy.src.planes = pc * Gz;
-    fy.src.lines = Nx;
-    fy.src.plane_stride = static_cast<uint64_t>(Gy) * Nx;
+    fy.src.planes = Gz;
+    fy.src.lines = Nx * pc;
+    fy.src.plane_stride = static_cast<uint64_t>(Gy) * Nx * pc;
     fy.src.line_stride = 1;
-    fy.src.elem_stride = Nx;
-    fy.dst.planes = pc * Gz;
-    fy.dst.lines = Nx;
-    fy.dst.plane_stride = static_cast<uint64_t>(Ny) * Nx;
+    fy.src.elem_stride = static_cast<uint64_t>(Nx) * pc;
+    fy.dst.planes = Gz;
+    fy.dst.lines = Nx * pc;
+    fy.dst.plane_stride = static_cast<uint64_t>(Ny) * Nx * pc;
     fy.dst.line_stride = 1;
-    fy.dst.elem_stride = Nx;
+    fy.dst.elem_stride = static_cast<uint64_t>(Nx) * pc;
     fy.n = Gy;
     fy.n_out = Ny;
     fy.out_offset = (Gy - Ny / 2) % Gy;
     fy.out_weight = plan.deapod[1];
     fy.line_weight = plan.deapod[2];
-    fy.zero_span = zero_rows;  // per plane (coil, z): the rows the x pass did not store
-    fy.zero_span_period = Gz;
+    fy.line_weight_div = pc;
+    fy.zero_span = zero_rows;  // per plane z: the rows the x pass did not store
     fy.inverse = 1;
     fy.load_lines_fast = fy.store_lines_fast = 1;
     {
       ProfScope ps("fft3_y_deapod", stream);
       fft_launch(fy, stream);
     }
-    // z: lines (coil, y, x) -> [pc][Nz][Ny][Nx]; deapodization of z.
+    // z: lines (pixel (y, x); coil) -> deapodization of z; the pass itself writes the coil-last
+    // images and carries the RSS sum across coil passes where its epilogue applies, else
+    // it stores [Nz][Ny][Nx][pc] for the two combine kernels.
     FftArgs fz;
     fz.in = yout;
     fz.out = zout;
-    fz.src.planes = pc;
-    fz.src.lines = Ny * Nx;
-    fz.src.plane_stride = static_cast<uint64_t>(Gz) * Ny * Nx;
+    fz.src.planes = Ny * Nx;
+    fz.src.lines = pc;
+    fz.src.plane_stride = pc;
     fz.src.line_stride = 1;
-    fz.src.elem_stride = static_cast<uint64_t>(Ny) * Nx;
-    fz.dst.planes = pc;
-    fz.dst.lines = Ny * Nx;
-    fz.dst.plane_stride = P;
-    fz.dst.line_stride = 1;
-    fz.dst.elem_stride = static_cast<uint64_t>(Ny) * Nx;
+    fz.src.elem_stride = static_cast<uint64_t>(Ny) * Nx * pc;
+    fz.dst = fz.src;
     fz.n = Gz;
     fz.n_out = Nz;
     fz.out_offset = (Gz - Nz / 2) % Gz;
     fz.out_weight = plan.deapod[0];
     fz.inverse = 1;
     fz.load_lines_fast = fz.store_lines_fast = 1;
+    const bool first = c0 == 0, last = c0 + pc >= channels;
+    if (fft_columns_supported(fz, pc)) {
+      ColumnOutputs co;
+      co.channels = pc;
+      co.cols = Ny * Nx;
+      co.img_stride = channels;
+      co.coil_last[0] = d_images ? d_images + c0 : nullptr;
+      co.rss[0] = d_rss;
+      co.sumsq = d_rss ? sumsq : nullptr;
+      co.first = first;
+      co.last = last;
+      ProfScope ps("fft3_z_deapod", stream);
+      fft_launch_columns(fz, co, stream);
+      continue;
+    }
     {
       ProfScope ps("fft3_z_deapod", stream);
       fft_launch(fz, stream);
@@ -575,7 +586,7 @@ void adjoint3_execute(const Plan3& plan, const float2* d_kspace, uint32_t channe
     ProfScope ps("coil_combine3", stream);
     if (d_rss) {
       hipLaunchKernelGGL(rss3_kernel, dim3(ceil_div(P, 256)), dim3(256), 0, stream, zout, pc, P, sumsq, d_rss,
-                         c0 == 0 ? 1 : 0, c0 + pc >= channels ? 1 : 0);
+                         first ? 1 : 0, last ? 1 : 0);
       KSJ_LAUNCH_CHECK();
     }
     if (d_images) {

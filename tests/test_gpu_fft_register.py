@@ -9,7 +9,10 @@ coil-fastest plus one column pass that writes each scan's coil-last images and R
 itself; its images equal the staged pipeline's bit for bit and its RSS (the same float
 sum in coil order) too, at 1..32 channels, several scans per pass, uneven matrices
 (grids 384 and 640) and channel counts whose groups do not fit a block (which keep the
-staged pipeline), and the headline shape (256, four 16-coil scans stacked in one pass)."""
+staged pipeline), and the headline shape (256, four 16-coil scans stacked in one pass).
+Volumes keep every pass output coil-fastest; from grid 256 up the z pass takes the column
+epilogue (coil-last images and the RSS carried across coil passes), whose bits equal the
+staged pipeline's separate combine kernels."""
 import numpy as np
 import pytest
 
@@ -106,9 +109,12 @@ def test_radial_column_epilogue_matches_the_staged_pipeline(ops, tune, n, coils,
     plan.close()
 
 
-def test_volumes_give_the_staged_bits(ops, tune):
-    shape = (32, 48, 40)  # grids 64 / 96 / 80: the LDS 64 and direct-DFT passes
-    traj3 = ops.golden_means3(120, 64)
+@pytest.mark.parametrize("shape,spokes,readout", [
+    ((32, 48, 40), 120, 64),      # grids 64 / 96 / 80: the LDS 64 and direct-DFT passes
+    ((128, 128, 128), 600, 256),  # grid 256: the z pass writes images and RSS itself
+])
+def test_volumes_give_the_staged_bits(ops, tune, shape, spokes, readout):
+    traj3 = ops.golden_means3(spokes, readout)
     rng = np.random.default_rng(9)
     k3 = torch.from_numpy((rng.standard_normal((5, traj3.shape[0]))
                            + 1j * rng.standard_normal((5, traj3.shape[0]))).astype(np.complex64)).cuda()
