@@ -467,7 +467,9 @@ __device__ inline uint32_t line_plane(const FftArgs& a, uint32_t l) {
   return l / a.src.lines;
 }
 
-__device__ inline RegLineView reg_line_view(const FftArgs& a, uint32_t line, uint32_t total) {
+// `one_plane`: every line of the block lies in one plane, so its zero span is read once
+// through the scalar cache (no per-lane load ahead of the data loads).
+__device__ inline RegLineView reg_line_view(const FftArgs& a, uint32_t line, uint32_t total, bool one_plane) {
   RegLineView r;
   r.live = line < total;
   const uint32_t l = r.live ? line : 0u;
@@ -475,7 +477,12 @@ __device__ inline RegLineView reg_line_view(const FftArgs& a, uint32_t line, uin
   r.line = l - r.plane * a.src.lines;
   r.src = r.plane * a.src.plane_stride + r.line * a.src.line_stride;
   r.dst = (r.live && !plane_skipped(a, r.plane)) ? r.plane * a.dst.plane_stride + r.line * a.dst.line_stride : ~0ull;
-  r.zs = r.live ? plane_zero_span(a, r.plane) : make_uint2(0, 0);
+  if (one_plane && a.zero_span) {
+    const uint32_t p = __builtin_amdgcn_readfirstlane(r.plane);
+    r.zs = a.zero_span[a.zero_span_period ? p % a.zero_span_period : p];
+  } else {
+    r.zs = r.live ? plane_zero_span(a, r.plane) : make_uint2(0, 0);
+  }
   r.lw = (a.line_weight && r.live)
              ? a.line_weight[(a.line_weight_by_plane ? r.plane : r.line) / a.line_weight_div]
              : 1.0f;
@@ -490,13 +497,25 @@ __device__ inline void reg_load(const FftArgs& a, const RegLineView& w, RegLine<
   const uint32_t es = static_cast<uint32_t>(a.src.elem_stride);
   typedef float F2v __attribute__((ext_vector_type(2)));
   // Every input element of a pass is read once: non-temporal, out of the caches' way.
-  if (a.in_offset == 0 && !a.zero_span) {  // no rotation, no zero span: plain strided loads
+  if (a.in_offset == 0) {  // no rotation: strided loads, the zero span (if any) skipped
     const float2* p = a.in + w.src + static_cast<uint64_t>(static_cast<uint32_t>(t) * es);
     const uint64_t step = static_cast<uint64_t>(T) * es;
+    if (!a.zero_span) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const F2v v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(p + i * step));
+        x.v[i] = make_float2(v.x, v.y);
+      }
+      return;
+    }
+    // Branch-free, so every load is in flight at once: an element of the zero span reads
+    // the thread's first element instead (a cache hit; the cell itself may never have been
+    // written) and is replaced by zero.
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      const F2v v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(p + i * step));
-      x.v[i] = make_float2(v.x, v.y);
+      const bool zero = in_span(static_cast<uint32_t>(t + i * T), w.zs);
+      const F2v v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(zero ? p : p + i * step));
+      x.v[i] = zero ? make_float2(0.f, 0.f) : make_float2(v.x, v.y);
     }
     return;
   }
@@ -527,19 +546,24 @@ __device__ inline void fft_reg_group(const FftArgs& a, const ColumnOutputs& co, 
   const int S = lpb + pad;
   float2* tw = smem;
   float2* buf = smem + N;
+  const uint32_t total = a.src.planes * a.src.lines;  // below 2^32 (checked at launch)
+  const uint32_t nout = a.n_out;
+  const int ly = threadIdx.x & (lpb - 1), t = threadIdx.x >> lpb_log2;
+  // A block lies in one plane when lines per plane is a multiple of the block's lines
+  // and the block holds no line past the end.
+  const bool one_plane = a.src.lines % lpb == 0 && (g + 1) * lpb <= total;
+  RegLine<N, V> x;
+  // The line loads go first: the twiddle and weight staging below waits on its own loads
+  // and so overlaps them.
+  reg_load<N, V>(a, reg_line_view(a, g * lpb + ly, total, one_plane), x, t);
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
     tw[i] = tw_global[i];
     ow[i] = (i < static_cast<int>(a.n_out) && a.out_weight) ? a.scale * a.out_weight[i] : a.scale;
   }
-  const uint32_t total = a.src.planes * a.src.lines;  // below 2^32 (checked at launch)
-  const uint32_t nout = a.n_out;
-  const int ly = threadIdx.x & (lpb - 1), t = threadIdx.x >> lpb_log2;
-  RegLine<N, V> x;
-  reg_load<N, V>(a, reg_line_view(a, g * lpb + ly, total), x, t);
   __syncthreads();  // twiddles and output weights staged
   reg_transform<N, V, INV>(x, buf, S, ly, tw, t);
   // The view again rather than kept across the transform: fewer live registers.
-  const RegLineView cur = reg_line_view(a, g * lpb + ly, total);
+  const RegLineView cur = reg_line_view(a, g * lpb + ly, total, one_plane);
   {
     if constexpr (!EPI) {
       if (cur.dst != ~0ull) {
