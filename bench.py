@@ -191,6 +191,11 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
 
 
+def ms3(x):
+    """Milliseconds to three decimals; None (no samples) stays None."""
+    return None if x is None else round(x, 3)
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,10 +223,10 @@ def main(argv=None) -> int:
                     help="3-D section on seeded complex Gaussian k-space instead of the simulated phantom")
     ap.add_argument("--provider-firings", type=int, default=40,
                     help="firings of the drop-in Provider operator (host-fed plain form); 0 skips it")
-    ap.add_argument("--serve-fft-schedule", type=int, default=3,
-                    help="fft_schedule while the server runs the served and latency sections (3: half the threads "
-                         "per line at length 512, which overlaps better across server lanes); -1 keeps the "
-                         "process setting.  The plain, stage and other sections run with the process setting.")
+    ap.add_argument("--serve-fft-schedule", type=int, default=-1,
+                    help="fft_schedule while the server runs the served and latency sections (-1, the default, "
+                         "keeps the process setting; 3 takes half the threads per line at length 512).  The plain, "
+                         "stage and other sections run with the process setting.")
     ap.add_argument("--tune-seconds", type=float, default=20.0, help="autotuner budget for the uneven batch")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -441,11 +446,11 @@ def main(argv=None) -> int:
                    else fft_schedule_saved},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "latency": {"p50_ms": round(percentile(stream_lat, 50), 3), "p99_ms": round(percentile(stream_lat, 99), 3),
+        "latency": {"p50_ms": ms3(percentile(stream_lat, 50)), "p99_ms": ms3(percentile(stream_lat, 99)),
                     "arrival_rate_per_s": round(rate, 1), "requests": len(stream_lat),
                     "process": "open-loop Poisson arrivals, one scan per request",
-                    "saturated_p50_ms": round(percentile(burst_lat, 50), 3),
-                    "saturated_p99_ms": round(percentile(burst_lat, 99), 3)},
+                    "saturated_p50_ms": ms3(percentile(burst_lat, 50)),
+                    "saturated_p99_ms": ms3(percentile(burst_lat, 99))},
         "plain": {"images_per_s": round(plain_value, 1), "ms_per_scan": round(1e3 / plain_value, 3),
                   "operator": "ksj_gpu_adjoint2_execute, one synchronous call per scan, plan resident"},
         "served_over_plain": round(value / plain_value, 2),

@@ -897,9 +897,9 @@ void launch_fused(const FftArgs* const* items, uint32_t count, const float2* tw,
 }
 
 // The radix schedule of length N: the tuning key, where 3 picks per length -- half the
-// threads per line (1) at 512, which overlaps better with the other lanes of a served
-// stream (+1.4 % served images/s on the 256x256 radial stream; a serial per-scan call is
-// about 2.5 % slower with it), and schedule 0 at every other length, where 1 is slower.
+// threads per line (1) at 512 and schedule 0 at every other length.  (With per-block line
+// tables, 3 served the 256x256 radial stream 1.4 % faster; with each thread finding its own
+// line, schedule 0 serves it 4 % faster than 3.)
 template <int N> int fft_schedule() {
   const int v = static_cast<int>(tune_get(kTuneFftSchedule));
   return v == 3 ? (N == 512 ? 1 : 0) : v;
