@@ -82,16 +82,26 @@ __device__ inline void prep_block_in(const Planes& planes, const float2* weight,
   const float2 w = live ? weight[s] : make_float2(0.f, 0.f);
   const uint32_t r = live ? rank[s] : 0u;
   float2 v[kPerWave];
+  // Branch-free, so all of a wave's plane loads are in flight together (a load under a
+  // branch waits for its plane pointer first): a coil past the end reads plane 0 and a
+  // sample past the end the last sample, and both are multiplied by 0 (a live value by
+  // 1, exactly itself).  Such values only reach padded coil columns, which no output
+  // reads, and rows past the end, which are not stored.
+  const uint32_t sc = live ? s : samples - 1;
+  const float2* base[kPerWave];
 #pragma unroll
   for (int j = 0; j < kPerWave; ++j) {
     const uint32_t c = wave + kPrepWaves * j;  // wave-uniform: the plane pointer is a scalar load
-    v[j] = make_float2(0.f, 0.f);
-    if (c < coils && live) {
-      // Each k-space value is read exactly once: keep it out of the caches.
-      typedef float F2v __attribute__((ext_vector_type(2)));
-      const F2v t = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(plane_of(planes, c) + s));
-      v[j] = make_float2(t.x, t.y);
-    }
+    base[j] = plane_of(planes, c < coils ? c : 0u);
+  }
+#pragma unroll
+  for (int j = 0; j < kPerWave; ++j) {
+    const uint32_t c = wave + kPrepWaves * j;
+    const float keep = (c < coils && live) ? 1.f : 0.f;
+    // Each k-space value is read exactly once: keep it out of the caches.
+    typedef float F2v __attribute__((ext_vector_type(2)));
+    const F2v t = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(base[j] + sc));
+    v[j] = make_float2(t.x * keep, t.y * keep);
   }
   if (wave == 0) rank_s[lane] = r;  // after the sample loads are issued
 #pragma unroll
