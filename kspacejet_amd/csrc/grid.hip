@@ -431,14 +431,19 @@ __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
   constexpr int kCells = 16 * T;  // 4 x 4 (T = 1) or 4 x 8 (T = 2) cells, T products of 16 cells each
   constexpr uint32_t kKeyMask = WT ? 0x00FFFFFFu : 0xFFFFFFFFu;
   const int lane = static_cast<int>(opaque_tid() & 63), q = lane >> 4, j = lane & 15;
-  const uint32_t i0 = a.run_first[run], i1 = a.run_first[run + 1];  // run is wave-uniform: scalar loads
+  // The run's bounds and its first and last visit through the scalar cache (run is
+  // wave-uniform; plan tables are read-only here), so the first batch's visit loads do
+  // not wait for the per-lane item records.
+  const ConstWord run_first = (ConstWord)(a.run_first);
+  const uint32_t i0 = run_first[run], i1 = run_first[run + 1];
   if (i0 >= i1) return;
   const uint32_t nitems = i1 - i0;  // <= 64 by construction of the runs
+  const ConstWord info = (ConstWord)(a.item_info);
+  const uint32_t c0 = info[4 * static_cast<uint64_t>(i0) + 1] / 4, c1 = info[4 * static_cast<uint64_t>(i1 - 1) + 2] / 4;
   const uint4 rec = static_cast<uint32_t>(lane) < nitems ? a.item_info[i0 + lane] : make_uint4(0, 0, 0, 0);
   auto field = [&](uint32_t v, uint32_t t) {
     return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(t)));
   };
-  const uint32_t c0 = field(rec.y, 0) / 4, c1 = field(rec.z, nitems - 1) / 4;
   const uint32_t last_visit = c1 > c0 ? 4 * c1 - 1 : 0;
   const float* wts = reinterpret_cast<const float*>(a.visit_w);
   const float* rows = reinterpret_cast<const float*>(a.sorted) + piece_offset<NF>(j);
@@ -534,7 +539,7 @@ __device__ inline void matrix_run(const GatherArgs& a, uint32_t run) {
     }
   };
 
-  uint32_t t = 0, t_end = field(rec.z, 0) / 4;  // current item and its end group
+  uint32_t t = 0, t_end = info[4 * static_cast<uint64_t>(i0) + 2] / 4;  // current item and its end group
   auto compute = [&](uint32_t c, const RowPiece<NF> (&b)[U], const float (&wc)[U][T]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
