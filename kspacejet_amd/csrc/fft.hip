@@ -220,6 +220,13 @@ __device__ inline uint64_t line_base(const FftLines& L, uint64_t line) {
 // Element e of a line is read as zero when e lies in the line's zero span [zs.x, zs.y).
 __device__ inline bool in_span(uint32_t e, uint2 zs) { return e - zs.x < zs.y - zs.x; }
 
+// A loaded element kept (keep = ~0) or replaced by +0 (keep = 0) through its bit
+// pattern: integer masks, which the compiler cannot turn back into a branch around the
+// load (a branch per load serialises the loads).
+__device__ inline float2 masked(float x, float y, uint32_t keep) {
+  return make_float2(__uint_as_float(__float_as_uint(x) & keep), __uint_as_float(__float_as_uint(y) & keep));
+}
+
 // The zero span of a source plane, and whether a line's outputs are skipped.
 __device__ inline uint2 plane_zero_span(const FftArgs& a, uint64_t plane) {
   if (!a.zero_span) return make_uint2(0, 0);
@@ -230,9 +237,9 @@ __device__ inline bool plane_skipped(const FftArgs& a, uint64_t plane) {
   return in_span(static_cast<uint32_t>(plane % a.skip_div), a.skip_span[plane / a.skip_div]);
 }
 
-template <int N, int V>
+template <int N, int V, typename Mid>
 __device__ inline void load_lines(const FftArgs& a, float2* buf, int ls, int lpb_log2, const uint64_t* src_base,
-                                  const uint2* zspan) {
+                                  const uint2* zspan, Mid mid) {
   constexpr int E = N / Shape<N, V>::T;
   const int lpb = 1 << lpb_log2;
   const uint32_t es = static_cast<uint32_t>(a.src.elem_stride);
@@ -258,13 +265,13 @@ __device__ inline void load_lines(const FftArgs& a, float2* buf, int ls, int lpb
     // outputs are never stored.
     // Every input element of a pass is read once: non-temporal, out of the caches' way.
     typedef float F2v __attribute__((ext_vector_type(2)));
-    F2v t = {0.f, 0.f};
-    if (!in_span(src, zspan[ly]))
-      t = __builtin_nontemporal_load(
-          reinterpret_cast<const F2v*>(a.in + (live ? base : 0ull) + static_cast<uint64_t>(src * es)));
-    v[i] = make_float2(t.x, t.y);
+    const uint32_t keep = in_span(src, zspan[ly]) ? 0u : ~0u;  // the zero span reads element 0, then zero
+    const F2v t = __builtin_nontemporal_load(
+        reinterpret_cast<const F2v*>(a.in + (live ? base : 0ull) + static_cast<uint64_t>((src & keep) * es)));
+    v[i] = masked(t.x, t.y, keep);
     dst[i] = ly * ls + pad(e);
   }
+  mid();  // work that overlaps the loads in flight
 #pragma unroll
   for (int i = 0; i < E; ++i) buf[dst[i]] = v[i];
 }
@@ -314,10 +321,6 @@ __device__ inline void fft_lds_block(const FftArgs& a, const float2* __restrict_
   const int lpb = 1 << lpb_log2;
   float2* tw = smem;
   float2* buf = smem + N;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) {
-    tw[i] = tw_global[i];
-    ow[i] = (i < static_cast<int>(a.n_out) && a.out_weight) ? a.scale * a.out_weight[i] : a.scale;
-  }
   const uint64_t total = static_cast<uint64_t>(a.src.planes) * a.src.lines;
   const uint64_t first = block * lpb;
   for (int ly = threadIdx.x; ly < lpb; ly += blockDim.x) {
@@ -329,7 +332,13 @@ __device__ inline void fft_lds_block(const FftArgs& a, const float2* __restrict_
     lw[ly] = (a.line_weight && line < total) ? a.line_weight[wi] : 1.0f;
   }
   __syncthreads();
-  load_lines<N, V>(a, buf, LS, lpb_log2, src_base, zspan);
+  // The line loads go first; the twiddle and weight staging runs while they are in flight.
+  load_lines<N, V>(a, buf, LS, lpb_log2, src_base, zspan, [&] {
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+      tw[i] = tw_global[i];
+      ow[i] = (i < static_cast<int>(a.n_out) && a.out_weight) ? a.scale * a.out_weight[i] : a.scale;
+    }
+  });
   __syncthreads();
   const int ly = threadIdx.x / T, t = threadIdx.x % T;
   passes<N, V, INV, 0, 1>(buf + ly * LS, tw, t);
@@ -513,9 +522,10 @@ __device__ inline void reg_load(const FftArgs& a, const RegLineView& w, RegLine<
     // written) and is replaced by zero.
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      const bool zero = in_span(static_cast<uint32_t>(t + i * T), w.zs);
-      const F2v v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(zero ? p : p + i * step));
-      x.v[i] = zero ? make_float2(0.f, 0.f) : make_float2(v.x, v.y);
+      const uint32_t keep = in_span(static_cast<uint32_t>(t + i * T), w.zs) ? 0u : ~0u;
+      const uint64_t off = (static_cast<uint64_t>(i) * step) & static_cast<uint64_t>(static_cast<int64_t>(static_cast<int32_t>(keep)));
+      const F2v v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(p + off));
+      x.v[i] = masked(v.x, v.y, keep);
     }
     return;
   }
@@ -523,10 +533,9 @@ __device__ inline void reg_load(const FftArgs& a, const RegLineView& w, RegLine<
   for (int i = 0; i < E; ++i) {
     uint32_t src = static_cast<uint32_t>(t + i * T) + a.in_offset;
     if (src >= static_cast<uint32_t>(N)) src -= N;
-    F2v v = {0.f, 0.f};
-    if (!in_span(src, w.zs))
-      v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(a.in + w.src + static_cast<uint64_t>(src * es)));
-    x.v[i] = make_float2(v.x, v.y);
+    const uint32_t keep = in_span(src, w.zs) ? 0u : ~0u;  // the zero span reads element 0, then zero
+    const F2v v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(a.in + w.src + static_cast<uint64_t>((src & keep) * es)));
+    x.v[i] = masked(v.x, v.y, keep);
   }
 }
 
