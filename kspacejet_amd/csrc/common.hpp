@@ -23,6 +23,12 @@ __host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(
 __host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 // |a|^2 with one explicit fma, so every kernel that sums it rounds the same way.
 __host__ __device__ inline float norm2(float2 a) { return fmaf(a.x, a.x, a.y * a.y); }
+// a kept (keep = ~0) or replaced by +0 (keep = 0) through its bit pattern: an integer mask,
+// which the compiler cannot turn back into a branch around the load that produced a (a
+// branch per load serialises a thread's loads).
+__device__ inline float2 keep_bits(float2 a, uint32_t keep) {
+  return make_float2(__uint_as_float(__float_as_uint(a.x) & keep), __uint_as_float(__float_as_uint(a.y) & keep));
+}
 __host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 
 // Thrown by host code on a HIP error or a violated precondition; the C ABI

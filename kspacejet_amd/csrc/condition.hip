@@ -112,12 +112,17 @@ __global__ __launch_bounds__(kMixMfmaBlock) void channel_mix_mfma_kernel(const f
   const uint64_t tiles = (samples + 15) / 16;
   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kMixMfmaBlock / 64);
   // Lane's inputs of tile t: x[4p + j][16 t + n], zero past the samples and the channels.
+  // Branch-free, so all of a lane's loads are in flight together: out-of-range positions
+  // read an in-range element and are cleared.
   auto load_tile = [&](uint64_t t, float2 (&b)[PAIRS]) {
     const uint64_t s = t * 16 + n;
+    const bool in_s = s < samples;
+    const uint64_t sc = in_s ? s : samples - 1;
 #pragma unroll
     for (int p = 0; p < PAIRS; ++p) {
       const uint32_t c = 4 * p + j;
-      b[p] = s < samples && c < in_ch ? x[static_cast<uint64_t>(c) * samples + s] : make_float2(0.f, 0.f);
+      const uint32_t keep = in_s && c < in_ch ? ~0u : 0u;
+      b[p] = keep_bits(x[static_cast<uint64_t>(c < in_ch ? c : 0u) * samples + sc], keep);
     }
   };
   // With up to 16 input channels the next tile's inputs are loaded while this tile's

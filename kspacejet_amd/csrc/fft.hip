@@ -220,13 +220,6 @@ __device__ inline uint64_t line_base(const FftLines& L, uint64_t line) {
 // Element e of a line is read as zero when e lies in the line's zero span [zs.x, zs.y).
 __device__ inline bool in_span(uint32_t e, uint2 zs) { return e - zs.x < zs.y - zs.x; }
 
-// A loaded element kept (keep = ~0) or replaced by +0 (keep = 0) through its bit
-// pattern: integer masks, which the compiler cannot turn back into a branch around the
-// load (a branch per load serialises the loads).
-__device__ inline float2 masked(float x, float y, uint32_t keep) {
-  return make_float2(__uint_as_float(__float_as_uint(x) & keep), __uint_as_float(__float_as_uint(y) & keep));
-}
-
 // The zero span of a source plane, and whether a line's outputs are skipped.
 __device__ inline uint2 plane_zero_span(const FftArgs& a, uint64_t plane) {
   if (!a.zero_span) return make_uint2(0, 0);
@@ -268,7 +261,7 @@ __device__ inline void load_lines(const FftArgs& a, float2* buf, int ls, int lpb
     const uint32_t keep = in_span(src, zspan[ly]) ? 0u : ~0u;  // the zero span reads element 0, then zero
     const F2v t = __builtin_nontemporal_load(
         reinterpret_cast<const F2v*>(a.in + (live ? base : 0ull) + static_cast<uint64_t>((src & keep) * es)));
-    v[i] = masked(t.x, t.y, keep);
+    v[i] = keep_bits(make_float2(t.x, t.y), keep);
     dst[i] = ly * ls + pad(e);
   }
   mid();  // work that overlaps the loads in flight
@@ -525,7 +518,7 @@ __device__ inline void reg_load(const FftArgs& a, const RegLineView& w, RegLine<
       const uint32_t keep = in_span(static_cast<uint32_t>(t + i * T), w.zs) ? 0u : ~0u;
       const uint64_t off = (static_cast<uint64_t>(i) * step) & static_cast<uint64_t>(static_cast<int64_t>(static_cast<int32_t>(keep)));
       const F2v v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(p + off));
-      x.v[i] = masked(v.x, v.y, keep);
+      x.v[i] = keep_bits(make_float2(v.x, v.y), keep);
     }
     return;
   }
@@ -535,7 +528,7 @@ __device__ inline void reg_load(const FftArgs& a, const RegLineView& w, RegLine<
     if (src >= static_cast<uint32_t>(N)) src -= N;
     const uint32_t keep = in_span(src, w.zs) ? 0u : ~0u;  // the zero span reads element 0, then zero
     const F2v v = __builtin_nontemporal_load(reinterpret_cast<const F2v*>(a.in + w.src + static_cast<uint64_t>((src & keep) * es)));
-    x.v[i] = masked(v.x, v.y, keep);
+    x.v[i] = keep_bits(make_float2(v.x, v.y), keep);
   }
 }
 
