@@ -375,15 +375,18 @@ __device__ inline void multi_reduce_one(const GatherArgs& a, uint64_t i) {
   const float2* p = a.partial + (static_cast<uint64_t>(s0) * cells + q) * a.cp + c;
   const uint64_t stride = static_cast<uint64_t>(cells) * a.cp;
   float2 s = make_float2(0.f, 0.f);
-  uint32_t j = 0;
-  for (; j + 8 <= n; j += 8) {  // eight loads in flight, summed in item order
+  // Eight loads in flight, summed in item order; the last group reads clamped indices and
+  // adds +0 for items past the end (s starts at +0 and never becomes -0, so s + 0 = s).
+  for (uint32_t j = 0; j < n; j += 8) {
     float2 t[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) t[u] = p[(j + u) * stride];
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t i = j + u < n ? j + u : n - 1;
+      t[u] = keep_bits(p[i * stride], j + u < n ? ~0u : 0u);
+    }
 #pragma unroll
     for (int u = 0; u < 8; ++u) s = cadd(s, t[u]);
   }
-  for (; j < n; ++j) s = cadd(s, p[j * stride]);
   uint32_t by, bx;
   block_coords(b, a.block_cols, a.block_group, &by, &bx);
   const uint32_t y = a.bh * by + q / a.bw, x = a.bw * bx + q % a.bw;

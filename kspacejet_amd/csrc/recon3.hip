@@ -138,7 +138,19 @@ __global__ __launch_bounds__(256) void reduce3_kernel(Gather3Args a) {
   const uint32_t n = a.block_first_item[b + 1] - a.block_first_item[b];
   const float2* p = a.partial + (static_cast<uint64_t>(s0) * cells + q) * a.cp + c;
   float2 s = make_float2(0.f, 0.f);
-  for (uint32_t j = 0; j < n; ++j) s = cadd(s, p[static_cast<uint64_t>(j) * cells * a.cp]);
+  // Eight loads in flight, summed in item order; the last group reads clamped indices and
+  // adds +0 for items past the end (s starts at +0 and never becomes -0, so s + 0 = s).
+  const uint64_t stride = static_cast<uint64_t>(cells) * a.cp;
+  for (uint32_t j = 0; j < n; j += 8) {
+    float2 t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t k = j + u < n ? j + u : n - 1;
+      t[u] = keep_bits(p[k * stride], j + u < n ? ~0u : 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s = cadd(s, t[u]);
+  }
   uint32_t z0, y0, x0;
   block_cells(a, b, &z0, &y0, &x0);
   const uint32_t z = z0 + q / (a.side * a.side), y = y0 + (q / a.side) % a.side, x = x0 + q % a.side;
