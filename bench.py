@@ -32,7 +32,11 @@ image.  `value` is served images per second over the timed steps.  Beside it the
     kooshball volumes at half the 3-D spoke count), served vs plain, as a burst and
     under open-loop Poisson arrivals at several rates (p50/p99 per rate);
   * cpu_baseline: the reference's own radial route compiled from its sources and the
-    FP32 restatement of this route, on every core of the affinity mask and on one.
+    FP32 restatement of this route, on every core of the affinity mask and on one;
+    volume.cpu_baseline and cartesian.cpu_baseline: FP32 restatements of the 3-D route
+    and of the Cartesian inverse FFT + RSS, timed the same way;
+  * cartesian: the reference's CPU-runnable Cartesian case (256 x 256, 8 coils, centred
+    inverse FFT per coil + RSS) through the plain GPU operator.
 Every ratio compares like with like: both sides in HBM, or both fed from host memory.
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1]
@@ -221,6 +225,9 @@ def main(argv=None) -> int:
     ap.add_argument("--volume-coils-per-pass", type=int, default=16)
     ap.add_argument("--volume-random", action="store_true",
                     help="3-D section on seeded complex Gaussian k-space instead of the simulated phantom")
+    ap.add_argument("--cartesian-steps", type=int, default=10,
+                    help="steps of 32 scans of the Cartesian 256x256 8-coil case (plain GPU operator and CPU "
+                         "restatement); 0 skips it")
     ap.add_argument("--provider-firings", type=int, default=40,
                     help="firings of the drop-in Provider operator (host-fed plain form); 0 skips it")
     ap.add_argument("--serve-fft-schedule", type=int, default=-1,
@@ -410,7 +417,11 @@ def main(argv=None) -> int:
     if args.volume_coils > 0:
         log("3-D kooshball volume ...")
         volume = run_volume(torch, ops, lib, args.volume_coils, args.volume_coils_per_pass, max(1, args.steps // 4),
-                            1, simulate=not args.volume_random)
+                            1, simulate=not args.volume_random, cpu=not args.no_cpu_baseline)
+    cartesian = None
+    if args.cartesian_steps > 0:
+        log("Cartesian 256x256 8-coil case ...")
+        cartesian = run_cartesian(torch, ops, lib, args.cartesian_steps, cpu=not args.no_cpu_baseline)
     conditioning = None
     if args.conditioning_reps > 0:
         log("k-space conditioning kernels ...")
@@ -469,6 +480,7 @@ def main(argv=None) -> int:
         "mixed_poisson": poisson,
         "uneven": uneven,
         "volume": volume,
+        "cartesian": cartesian,
         "conditioning": conditioning,
     }
     print(json.dumps(line))
@@ -540,7 +552,7 @@ def run_conditioning(torch, ops, reps):
     return res
 
 
-def run_volume(torch, ops, lib, coils, coils_per_pass, steps, warmup, simulate=True):
+def run_volume(torch, ops, lib, coils, coils_per_pass, steps, warmup, simulate=True, cpu=True):
     """3-D radial (kooshball) adjoint of one 256^3 volume: 16384 golden-means spokes x
     512 samples, KB width 4 on a 512^3 grid (sigma 2), |k|^2 density, coils in passes
     of `coils_per_pass`, RSS volume out.  The k-space is simulated as the workload
@@ -611,7 +623,82 @@ def run_volume(torch, ops, lib, coils, coils_per_pass, steps, warmup, simulate=T
                     if simulate else "seeded complex Gaussian k-space of the workload's shape"),
            "simulation_s": round(sim_s, 2)}
     plan.close()
-    del ws, rss, k
+    del ws, rss
+    if cpu:
+        res["cpu_baseline"] = volume_cpu_baseline(k, traj, n, coils)
+    del k
+    return res
+
+
+def volume_cpu_baseline(k, traj, n, coils):
+    """The FP32 CPU restatement of the 3-D route (oracle/ksj_kb_route.c: |k|^2 DCF, KB W=4,
+    beta Beatty, on the (2n)^3 grid, inverse FFT pruned axis by axis, deapodization, RSS)
+    on a bounded sample of the same volume: one coil per core of the affinity mask at once
+    (each core its own 1 GiB grid), then one coil on one core; volumes/s = coils/s divided
+    by the volume's coil count.  Per-trajectory tables are made outside the timing, as the
+    GPU's resident plan."""
+    import oracle as O
+    cores, _ = host_cores()
+    t = min(cores, coils)
+    traj_np = traj.cpu().numpy().astype(np.float32)
+    k_np = k[:t].cpu().numpy()
+    _, secs = O.kb_route3_rss_f32(k_np, traj_np, n, width=WIDTH, threads=t, timed=True)
+    _, one = O.kb_route3_rss_f32(k_np[:1], traj_np, n, width=WIDTH, threads=1, timed=True)
+    return {"value": round(t / secs / coils, 4), "unit": "volumes/s", "cores": t, "kind": "port",
+            "one_core_value": round(1.0 / one / coils, 5),
+            "sample": f"{t} coils of the {n}^3 {coils}-coil volume at once on {t} threads ({secs:.1f} s), and one "
+                      f"coil on one thread ({one:.1f} s), through the FP32 restatement of the GPU's 3-D route; "
+                      f"volumes/s = coils/s / {coils}"}
+
+
+def run_cartesian(torch, ops, lib, steps, cpu=True):
+    """The reference's CPU-runnable Cartesian case: 256 x 256, 8 coils, k-space the FP64
+    centred FFT of the 10-ellipse phantom times 2-D Gaussian coils plus complex Gaussian
+    noise 1e-3 max|k|, stored complex64; recon = centred inverse 2-D FFT per coil, then
+    RSS.  Plain per-scan GPU operator (ksj_gpu_ifft2_coil_images + ksj_gpu_rss, one
+    synchronous call pair per scan, data in HBM) and, on the host, the FP32 CPU
+    restatement (oracle/ksj_kb_route.c) scan-parallel on every core and on one core."""
+    from kspacejet_amd import synth
+    n, coils, batch = 256, 8, 32
+    k_np = synth.cartesian_kspace2(n, coils, synth.scan_seed(1), centred=True).astype(np.complex64)
+    k = torch.from_numpy(k_np).cuda()
+    img = torch.empty((n, n, coils), dtype=torch.complex64, device="cuda")
+    rss = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    ws = torch.empty(int(lib.ksj_gpu_ifft2_workspace_bytes(coils, n, n)), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def one():
+        lib.ksj_gpu_ifft2_coil_images(k.data_ptr(), img.data_ptr(), coils, n, n, 1, 0, ws.data_ptr(), ws.numel(),
+                                      stream)
+        lib.ksj_gpu_rss(img.data_ptr(), rss.data_ptr(), n * n, coils, stream)
+        torch.cuda.synchronize()
+
+    for _ in range(5):
+        one()
+    scans = max(1, steps) * batch
+    t0 = time.perf_counter()
+    for _ in range(scans):
+        one()
+    gpu = scans / (time.perf_counter() - t0)
+    res = {"images_per_s": round(gpu, 1), "shape": [n, n], "coils": coils, "scans": scans,
+           "operator": "ksj_gpu_ifft2_coil_images (centred, coil-last) + ksj_gpu_rss, one synchronous call pair per "
+                       "scan, data in HBM",
+           "data": "10-ellipse phantom x 8 2-D Gaussian coils, FP64 centred FFT, complex Gaussian noise 1e-3 max|k|, "
+                   "stored complex64"}
+    if cpu:
+        import oracle as O
+        cores, _ = host_cores()
+        probe = O.cart_route_rss_f32(k_np, centred=True, scans=cores, threads=cores, timed=True)[1]
+        reps = cores * max(1, int(3.0 / max(probe, 1e-3)))
+        secs = O.cart_route_rss_f32(k_np, centred=True, scans=reps, threads=cores, timed=True)[1]
+        reps1 = max(1, int(2.0 / max(probe, 1e-3)))
+        secs1 = O.cart_route_rss_f32(k_np, centred=True, scans=reps1, threads=1, timed=True)[1]
+        res["cpu_baseline"] = {"value": round(reps / secs, 1), "unit": "images/s", "cores": cores, "kind": "port",
+                               "one_core_value": round(reps1 / secs1, 1),
+                               "sample": f"{reps} scans on {cores} threads and {reps1} on one, through the FP32 "
+                                         f"restatement (radix-2 inverse FFT per coil with Normalization::inverse, "
+                                         f"centred, RSS); the reference's Eigen FFT is not in the image"}
+    del k, img, rss, ws
     return res
 
 

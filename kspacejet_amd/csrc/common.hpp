@@ -27,7 +27,8 @@ __host__ __device__ inline float norm2(float2 a) { return fmaf(a.x, a.x, a.y * a
 // which the compiler cannot turn back into a branch around the load that produced a (a
 // branch per load serialises a thread's loads).
 __device__ inline float2 keep_bits(float2 a, uint32_t keep) {
-  return make_float2(__uint_as_float(__float_as_uint(a.x) & keep), __uint_as_float(__float_as_uint(a.y) & keep));
+  return make_float2(__builtin_bit_cast(float, __builtin_bit_cast(uint32_t, a.x) & keep),
+                     __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, a.y) & keep));
 }
 __host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 

@@ -295,6 +295,18 @@ def kb_route_rss_f32(values_f32, traj_f32, n: int, width: int = 4, scans: int = 
     like-for-like CPU baseline; the scan is reconstructed `scans` times on `threads`
     threads.  values [C, S] complex64, traj [S, 2] float32 rad/px -> rss [n, n] float32
     (and, when `timed`, the wall seconds of the scans after the per-trajectory tables)."""
+    v = np.ascontiguousarray(values_f32, np.complex64)
+    t = np.ascontiguousarray(traj_f32, np.float32).reshape(-1, 2)
+    out = np.zeros((n, n), np.float32)
+    secs = C.c_double(0)
+    rc = _route_lib().ora_kb_route_rss_f32(n, t.shape[0], v.shape[0], t.reshape(-1), v.view(np.float32).reshape(-1), width,
+                                     scans, threads, out.reshape(-1), C.byref(secs))
+    if rc != 0:
+        raise ValueError("CPU route rejected its input")
+    return (out, secs.value) if timed else out
+
+
+def _route_lib():
     global _route
     if _route is None:
         if not os.path.exists(_ROUTE_SO):
@@ -303,15 +315,44 @@ def kb_route_rss_f32(values_f32, traj_f32, n: int, width: int = 4, scans: int = 
         R.ora_kb_route_rss_f32.argtypes = [_sz, _sz, _sz, _fp, _fp, C.c_int, _sz, C.c_int, _fp,
                                            C.POINTER(C.c_double)]
         R.ora_kb_route_rss_f32.restype = C.c_int
+        R.ora_cart_route_rss_f32.argtypes = [_sz, _sz, _sz, _fp, C.c_int, _sz, C.c_int, _fp, C.POINTER(C.c_double)]
+        R.ora_cart_route_rss_f32.restype = C.c_int
+        R.ora_kb_route3_rss_f32.argtypes = [_sz, _sz, _sz, _fp, _fp, C.c_int, C.c_int, _fp, C.POINTER(C.c_double)]
+        R.ora_kb_route3_rss_f32.restype = C.c_int
         _route = R
-    v = np.ascontiguousarray(values_f32, np.complex64)
-    t = np.ascontiguousarray(traj_f32, np.float32).reshape(-1, 2)
-    out = np.zeros((n, n), np.float32)
+    return _route
+
+
+def cart_route_rss_f32(kspace_f32, centred: bool = True, scans: int = 1, threads: int = 1, timed: bool = False):
+    """FP32 CPU restatement of the Cartesian route: per coil the inverse 2-D FFT with
+    Normalization::inverse (centred or not), then RSS in coil order.  kspace [C, rows, cols]
+    complex64 (powers of two) -> rss [rows, cols] float32 (and, when `timed`, the wall
+    seconds of `scans` repeats on `threads` threads)."""
+    k = np.ascontiguousarray(kspace_f32, np.complex64)
+    c, r, w = k.shape
+    out = np.zeros((r, w), np.float32)
     secs = C.c_double(0)
-    rc = _route.ora_kb_route_rss_f32(n, t.shape[0], v.shape[0], t.reshape(-1), v.view(np.float32).reshape(-1), width,
-                                     scans, threads, out.reshape(-1), C.byref(secs))
+    rc = _route_lib().ora_cart_route_rss_f32(r, w, c, k.view(np.float32).reshape(-1), 1 if centred else 0, scans,
+                                             threads, out.reshape(-1), C.byref(secs))
     if rc != 0:
-        raise ValueError("CPU route rejected its input")
+        raise ValueError("Cartesian CPU route rejected its input")
+    return (out, secs.value) if timed else out
+
+
+def kb_route3_rss_f32(values_f32, traj_f32, n: int, width: int = 4, threads: int = 1, timed: bool = False):
+    """FP32 CPU restatement of the 3-D kooshball route (|k|^2 DCF, KB `width` with the
+    Beatty beta on the (2n)^3 grid, radix-2 inverse FFT pruned axis by axis,
+    deapodization, RSS); threads take whole coils.  values [C, S] complex64, traj [S, 3]
+    float32 (z, y, x) rad/px -> rss [n, n, n] float32 (and, when `timed`, the wall
+    seconds of the coils after the per-trajectory tables)."""
+    v = np.ascontiguousarray(values_f32, np.complex64)
+    t = np.ascontiguousarray(traj_f32, np.float32).reshape(-1, 3)
+    out = np.zeros((n, n, n), np.float32)
+    secs = C.c_double(0)
+    rc = _route_lib().ora_kb_route3_rss_f32(n, t.shape[0], v.shape[0], t.reshape(-1), v.view(np.float32).reshape(-1),
+                                            width, threads, out.reshape(-1), C.byref(secs))
+    if rc != 0:
+        raise ValueError("3-D CPU route rejected its input")
     return (out, secs.value) if timed else out
 
 
