@@ -342,13 +342,22 @@ __global__ __launch_bounds__(256) void gather3_matrix_kernel(Gather3Args a) {
 }
 
 // RSS over the coil-fastest z output [P][coils], carried across coil passes in channel
-// order: s = (first ? 0 : sumsq) + |x_0|^2 + |x_1|^2 + ...
-__global__ void rss3_kernel(const float2* img, uint32_t coils, uint64_t pixels, float* sumsq, float* rss, int first,
-                            int last) {
-  const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= pixels) return;
+// order: s = (first ? 0 : sumsq) + |x_0|^2 + |x_1|^2 + ...  A block's kRss3Pixels pixels
+// hold one contiguous run of values: their |x|^2 are staged in LDS by coalesced loads,
+// then each thread sums its pixel's coils.
+constexpr uint32_t kRss3Pixels = 256;
+__global__ __launch_bounds__(kRss3Pixels) void rss3_kernel(const float2* img, uint32_t coils, uint64_t pixels,
+                                                           float* sumsq, float* rss, int first, int last) {
+  extern __shared__ float norms[];  // [kRss3Pixels][coils]
+  const uint64_t p0 = static_cast<uint64_t>(blockIdx.x) * kRss3Pixels;
+  const uint32_t np = pixels - p0 < kRss3Pixels ? static_cast<uint32_t>(pixels - p0) : kRss3Pixels;
+  const float2* src = img + p0 * coils;
+  for (uint32_t i = threadIdx.x; i < np * coils; i += kRss3Pixels) norms[i] = norm2(src[i]);
+  __syncthreads();
+  if (threadIdx.x >= np) return;
+  const uint64_t p = p0 + threadIdx.x;
   float s = first ? 0.f : sumsq[p];
-  for (uint32_t c = 0; c < coils; ++c) s += norm2(img[p * coils + c]);
+  for (uint32_t c = 0; c < coils; ++c) s += norms[threadIdx.x * coils + c];
   if (last)
     rss[p] = sqrtf(s);
   else
@@ -597,8 +606,9 @@ void adjoint3_execute(const Plan3& plan, const float2* d_kspace, uint32_t channe
     }
     ProfScope ps("coil_combine3", stream);
     if (d_rss) {
-      hipLaunchKernelGGL(rss3_kernel, dim3(ceil_div(P, 256)), dim3(256), 0, stream, zout, pc, P, sumsq, d_rss,
-                         first ? 1 : 0, last ? 1 : 0);
+      const size_t lds = static_cast<size_t>(kRss3Pixels) * pc * sizeof(float);  // <= 64 KiB at 64 coils
+      hipLaunchKernelGGL(rss3_kernel, dim3(ceil_div(P, static_cast<uint64_t>(kRss3Pixels))), dim3(kRss3Pixels), lds,
+                         stream, zout, pc, P, sumsq, d_rss, first ? 1 : 0, last ? 1 : 0);
       KSJ_LAUNCH_CHECK();
     }
     if (d_images) {
