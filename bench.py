@@ -228,6 +228,9 @@ def main(argv=None) -> int:
     ap.add_argument("--cartesian-steps", type=int, default=10,
                     help="steps of 32 scans of the Cartesian 256x256 8-coil case (plain GPU operator and CPU "
                          "restatement); 0 skips it")
+    ap.add_argument("--ingest-reps", type=int, default=50,
+                    help="launches of the acquisition -> k-space frame assembly on the headline scan's 402 "
+                         "acquisitions; 0 skips it")
     ap.add_argument("--provider-firings", type=int, default=40,
                     help="firings of the drop-in Provider operator (host-fed plain form); 0 skips it")
     ap.add_argument("--serve-fft-schedule", type=int, default=-1,
@@ -422,6 +425,10 @@ def main(argv=None) -> int:
     if args.cartesian_steps > 0:
         log("Cartesian 256x256 8-coil case ...")
         cartesian = run_cartesian(torch, ops, lib, args.cartesian_steps, cpu=not args.no_cpu_baseline)
+    ingest = None
+    if args.ingest_reps > 0:
+        log("acquisition -> k-space frame assembly ...")
+        ingest = run_ingest(torch, ops, lib, args.ingest_reps, cpu=not args.no_cpu_baseline)
     conditioning = None
     if args.conditioning_reps > 0:
         log("k-space conditioning kernels ...")
@@ -482,6 +489,7 @@ def main(argv=None) -> int:
         "volume": volume,
         "cartesian": cartesian,
         "conditioning": conditioning,
+        "ingest": ingest,
     }
     print(json.dumps(line))
     return 0
@@ -699,6 +707,104 @@ def run_cartesian(torch, ops, lib, steps, cpu=True):
                                          f"restatement (radix-2 inverse FFT per coil with Normalization::inverse, "
                                          f"centred, RSS); the reference's Eigen FFT is not in the image"}
     del k, img, rss, ws
+    return res
+
+
+def run_ingest(torch, ops, lib, reps, cpu=True):
+    """ISMRMRD acquisitions -> the route's k-space frame (ksj_gpu_assemble_acquisitions) for the
+    headline scan: 402 spokes, one acquisition each, 16 channels x 512 samples, packed in HBM
+    in the ISMRMRD layout.  The uniform form (every spoke 512 samples) is timed as `reps`
+    launches replayed from one captured HIP graph; the ragged form (random 384-640 samples per
+    spoke) per call on the host clock, its offset upload and stream synchronisation included.
+    Algorithmic bytes per frame: the frame read once and written once, 2 x 8 x S x C.
+    cpu_baseline: the oracle's copy loop (numpy slice copies per acquisition and channel, as
+    the route's replay does with memcpy) on one core."""
+    S = SPOKES * READOUT
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    packed = torch.complex(torch.randn(COILS * S, generator=g, device="cuda"),
+                           torch.randn(COILS * S, generator=g, device="cuda"))
+    frame = torch.empty((COILS, S), dtype=torch.complex64, device="cuda")
+    ws = torch.empty(int(lib.ksj_gpu_assemble_workspace_bytes(SPOKES)), dtype=torch.uint8, device="cuda")
+    counts = (C.c_uint32 * SPOKES)(*([READOUT] * SPOKES))
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def uniform():
+        rc = lib.ksj_gpu_assemble_acquisitions(packed.data_ptr(), counts, SPOKES, COILS, frame.data_ptr(),
+                                               ws.data_ptr(), ws.numel(), torch.cuda.current_stream().cuda_stream)
+        assert rc == 0
+
+    uniform()
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        for _ in range(reps):
+            uniform()
+    graph.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    graph.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    del graph
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    nbytes = 2 * 8 * S * COILS
+    gbs = nbytes / (us * 1e-6) / 1e9
+    # The same bytes through a plain device-to-device copy (torch copy_), for scale.
+    flat = frame.view(-1)
+    flat.copy_(packed)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        for _ in range(reps):
+            flat.copy_(packed)
+    graph.replay()
+    torch.cuda.synchronize()
+    e0.record()
+    graph.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    del graph
+    copy_us = e0.elapsed_time(e1) * 1e3 / reps
+    rng = np.random.default_rng(9)
+    ragged_counts = rng.integers(384, 641, SPOKES)
+    rtotal = int(ragged_counts.sum())
+    rpacked = torch.complex(torch.randn(COILS * rtotal, generator=g, device="cuda"),
+                            torch.randn(COILS * rtotal, generator=g, device="cuda"))
+    rframe = torch.empty((COILS, rtotal), dtype=torch.complex64, device="cuda")
+    rc_counts = (C.c_uint32 * SPOKES)(*[int(x) for x in ragged_counts])
+
+    def ragged():
+        rc = lib.ksj_gpu_assemble_acquisitions(rpacked.data_ptr(), rc_counts, SPOKES, COILS, rframe.data_ptr(),
+                                               ws.data_ptr(), ws.numel(), stream)
+        assert rc == 0
+
+    ragged()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ragged()
+    ragged_us = (time.perf_counter() - t0) * 1e6 / reps
+    res = {"us_per_frame": round(us, 2), "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+           "plain_copy_us": round(copy_us, 2), "plain_copy_GBps": round(nbytes / (copy_us * 1e-6) / 1e9, 1),
+           "bytes_per_frame": nbytes, "ragged_us_per_call": round(ragged_us, 2), "ragged_samples": rtotal,
+           "shape": {"acquisitions": SPOKES, "channels": COILS, "samples_per_acquisition": READOUT},
+           "operator": "ksj_gpu_assemble_acquisitions: ISMRMRD acquisitions [C][ns] packed in HBM -> frame [C][S]"}
+    if cpu:
+        import oracle as O
+        host = packed.cpu().numpy()
+        acqs = [host[a * COILS * READOUT:(a + 1) * COILS * READOUT] for a in range(SPOKES)]
+        O.assemble_acquisitions(acqs, COILS)
+        frames, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 3.0:
+            O.assemble_acquisitions(acqs, COILS)
+            frames += 1
+        secs = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(frames / secs, 1), "unit": "frames/s", "cores": 1, "kind": "port",
+                               "sample": f"{frames} frames through the oracle's copy loop (numpy slice copies, one "
+                                         f"per acquisition and channel) on one core"}
+        res["gpu_frames_per_s"] = round(1e6 / us, 1)
+    del packed, frame, ws, rpacked, rframe
     return res
 
 

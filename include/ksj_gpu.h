@@ -234,6 +234,26 @@ int ksj_gpu_chain_faults(uint64_t* out_count);
 int ksj_gpu_normalize_trajectory2(const float* d_raw, uint64_t samples, int units, uint32_t encoded_rows,
                                   uint32_t encoded_cols, float* d_out, void* stream);
 
+/* ISMRMRD acquisitions -> the k-space frame of a non-Cartesian route.  d_acquisitions
+ * holds the acquisitions one after another, each in the ISMRMRD sample layout
+ * samples[sample + channel * ns_a] (complex float; libs/io/kspacejet-ismrmrd/include/
+ * kspacejet/ismrmrd/dataset_reader.hpp:57) with ns_a = samples_per_acquisition[a] (host
+ * array); d_kspace receives the channel-major frame [channels][S], S = sum of ns_a, acquisition
+ * a at columns [o_a, o_a + ns_a) in acquisition order.  Replaces the per-channel copy loop of
+ * the route's replay (libs/recon/kspacejet-recon-runtime/src/noncartesian_rss_hdf5.cpp:
+ * 1077-1083) with its checks (:880-884): every acquisition has samples, channels in [1, 64],
+ * S below 2^32.  The trajectory needs no reordering: ISMRMRD's [sample][dims] rows of the
+ * acquisitions, concatenated, are the raw [S][kx, ky] input of
+ * ksj_gpu_normalize_trajectory2.  When every acquisition has the same count the call is
+ * asynchronous on `stream` and needs no workspace; otherwise the acquisition offsets go to
+ * d_workspace (at least ksj_gpu_assemble_workspace_bytes, 4-byte aligned) and `stream` is
+ * synchronised.  No reference entry point takes device memory: the reference assembles the
+ * frame in host memory while the file streams. */
+size_t ksj_gpu_assemble_workspace_bytes(uint32_t acquisitions);
+int ksj_gpu_assemble_acquisitions(const void* d_acquisitions, const uint32_t* samples_per_acquisition,
+                                  uint32_t acquisitions, uint32_t channels, void* d_kspace, void* d_workspace,
+                                  size_t workspace_bytes, void* stream);
+
 /* Trajectory check on the device, for hosts that hand trajectories over in HBM.  Writes to
  * the device word *d_flags: KSJ_GPU_TRAJ_DIFFERS when some of the `values` floats at d_traj
  * differ bitwise from those at d_resident (always when d_resident is NULL), and
@@ -559,6 +579,8 @@ void ksj_gpu_serve_destroy(ksj_gpu_server* server);
  *                          (default); 0 = every pass staged through LDS (same bits)
  *   "row_order"            coil-fastest row-pass output of the 2-D column-epilogue pipeline: 0 = grid-row
  *                          major (default), 1 = column major (same bits)
+ *   "assemble_rows"        ksj_gpu_assemble_acquisitions: channel rows of one acquisition per workgroup, 1,
+ *                          2 or 4 (default; the largest of them dividing the channels is taken; same bits)
  *   "grid_prune"           1 = cells of the oversampled grid that no sample reaches are neither written by
  *                          the 2-D / 3-D matrix gathers nor read by the first FFT pass, which takes them as
  *                          zero (default); 0 = every cell written and read (same bits)

@@ -137,6 +137,8 @@ SIGNATURES = {
     "ksj_gpu_nufft3_forward_f64": (_int, [_vp, C.POINTER(_u32), C.POINTER(_dbl), _vp, _u64, _u32, _vp, _vp, _sz,
                                           _vp]),
     "ksj_gpu_normalize_trajectory2": (_int, [_vp, _u64, _int, _u32, _u32, _vp, _vp]),
+    "ksj_gpu_assemble_workspace_bytes": (_sz, [_u32]),
+    "ksj_gpu_assemble_acquisitions": (_int, [_vp, _vp, _u32, _u32, _vp, _vp, _sz, _vp]),
     "ksj_gpu_trajectory_check": (_int, [_vp, _vp, _u64, C.c_float, _vp, _vp]),
     "ksj_gpu_channel_mix": (_int, [_vp, _vp, _vp, _u32, _u32, _u64, _vp]),
     "ksj_gpu_phase_correct": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _vp]),

@@ -464,6 +464,20 @@ int ksj_gpu_normalize_trajectory2(const float* d_raw, uint64_t samples, int unit
   });
 }
 
+size_t ksj_gpu_assemble_workspace_bytes(uint32_t acquisitions) {
+  return ksj::gpu::assemble_workspace_bytes(acquisitions);
+}
+
+int ksj_gpu_assemble_acquisitions(const void* d_acquisitions, const uint32_t* samples_per_acquisition,
+                                  uint32_t acquisitions, uint32_t channels, void* d_kspace, void* d_workspace,
+                                  size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    ksj::gpu::assemble_acquisitions(static_cast<const float2*>(d_acquisitions), samples_per_acquisition,
+                                    acquisitions, channels, static_cast<float2*>(d_kspace), d_workspace,
+                                    workspace_bytes, as_stream(stream));
+  });
+}
+
 int ksj_gpu_trajectory_check(const float* d_traj, const float* d_resident, uint64_t values, float bound,
                              uint32_t* d_flags, void* stream) {
   return guarded([&] {

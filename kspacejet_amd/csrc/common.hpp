@@ -114,6 +114,7 @@ enum TuneKey : int {
                           // by the gather nor read by the row FFT, which takes them as zero (same bits)
   kTuneRowOrder,          // coil-fastest row-pass output of the column-epilogue pipeline: 0 = grid-row major
                           // [G_r][N_c][Cp], 1 = column major [N_c][G_r][Cp] (same bits)
+  kTuneAssembleRows,      // acquisition assembly: channel rows of one acquisition per workgroup (1, 2, 4; same bits)
   kTuneKeys
 };
 int64_t tune_get(TuneKey key);

@@ -182,4 +182,13 @@ void trajectory_check(const float* d_traj, const float* d_resident, uint64_t val
 // Analytic ramp DCF hypot(w_r, w_c) per sample in float (eigen_radial_gridding.cpp:279-295).
 void ramp_dcf2(const float* d_traj, float* d_dcf, uint64_t samples, hipStream_t stream);
 
+// ISMRMRD acquisitions, packed one after another each as [C][ns_a], -> the route's k-space
+// frame [C][S] (the replay's copy loop, noncartesian_rss_hdf5.cpp:1077-1083).  Counts are
+// host values.  Every acquisition with the same count: asynchronous, no workspace; otherwise
+// the offsets go to d_workspace (assemble_workspace_bytes) and `stream` is synchronised.
+size_t assemble_workspace_bytes(uint32_t acquisitions);
+void assemble_acquisitions(const float2* d_packed, const uint32_t* samples_per_acquisition, uint32_t acquisitions,
+                           uint32_t channels, float2* d_frame, void* d_workspace, size_t workspace_bytes,
+                           hipStream_t stream);
+
 }  // namespace ksj::gpu

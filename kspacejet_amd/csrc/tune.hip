@@ -51,6 +51,7 @@ Param g_params[kTuneKeys] = {
     {"matrix3_tap_share", 0, [](int64_t v) { return v == 0 || v == 1; }},
     {"grid_prune", 1, [](int64_t v) { return v == 0 || v == 1; }},
     {"row_order", 0, [](int64_t v) { return v == 0 || v == 1; }},
+    {"assemble_rows", 4, [](int64_t v) { return v == 1 || v == 2 || v == 4; }},
 };
 
 }  // namespace

@@ -413,6 +413,24 @@ def normalize_trajectory2(raw: torch.Tensor, units: str, encoded_rows: int = 0, 
     return out
 
 
+def assemble_acquisitions(packed: torch.Tensor, samples_per_acquisition, channels: int) -> torch.Tensor:
+    """ISMRMRD acquisitions packed one after another, each [channels][ns_a] complex64 (flat), -> the
+    route's k-space frame [channels][S] (ksj_gpu_assemble_acquisitions)."""
+    counts = [int(n) for n in samples_per_acquisition]
+    if not counts or min(counts) < 1 or max(counts) > 0xFFFFFFFF:
+        raise ValueError("every acquisition needs between 1 and 2^32 - 1 samples")
+    if not 1 <= channels <= 64:
+        raise ValueError("acquisitions must have [1, 64] active channels")
+    total = sum(counts)
+    _need(packed, torch.complex64, "acquisitions", (channels * total,))
+    out = torch.empty((channels, total), dtype=torch.complex64, device=packed.device)
+    ws = _workspace(load().ksj_gpu_assemble_workspace_bytes(len(counts)), packed.device)
+    host_counts = (C.c_uint32 * len(counts))(*counts)
+    call("ksj_gpu_assemble_acquisitions", _ptr(packed), C.addressof(host_counts), len(counts), channels, _ptr(out),
+         _ptr(ws), ws.numel(), _stream())
+    return out
+
+
 TRAJ_DIFFERS, TRAJ_INVALID = 1, 2
 
 

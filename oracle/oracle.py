@@ -487,6 +487,27 @@ def normalize_trajectory2(raw_kx_ky, units: str, encoded_rows: int, encoded_cols
     return (out, 0) if rc == 0 else (None, rc)
 
 
+def assemble_acquisitions(acquisitions, channels: int):
+    """The route's replay copy loop (noncartesian_rss_hdf5.cpp:1077-1083): acquisitions, each a flat
+    complex64 array in the ISMRMRD layout samples[sample + channel * ns] (dataset_reader.hpp:57), ->
+    (frame [channels][S] complex64, 0), acquisition a's channel-c row at columns [o_a, o_a + ns_a)
+    with o_a the samples written before it (:1104); or (None, 1) on the route's rejections
+    (:880-884): no acquisition, one without samples, channels outside [1, 64]."""
+    if not acquisitions or not 1 <= channels <= 64:
+        return None, 1
+    rows = [np.asarray(a, np.complex64).reshape(-1) for a in acquisitions]
+    if any(r.size == 0 or r.size % channels for r in rows):
+        return None, 1
+    frame = np.zeros((channels, sum(r.size // channels for r in rows)), np.complex64)
+    written = 0
+    for r in rows:
+        n = r.size // channels
+        for c in range(channels):
+            frame[c, written:written + n] = r[c * n:(c + 1) * n]
+        written += n
+    return frame, 0
+
+
 # ---------------------------------------------------------------- k-space conditioning
 def channel_mix(w, x) -> np.ndarray:
     """FP64 W[out, in] @ X[in, samples] (complex128 out): noise prewhitening and coil
