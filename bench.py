@@ -37,6 +37,9 @@ image.  `value` is served images per second over the timed steps.  Beside it the
     and of the Cartesian inverse FFT + RSS, timed the same way;
   * cartesian: the reference's CPU-runnable Cartesian case (256 x 256, 8 coils, centred
     inverse FFT per coil + RSS) through the plain GPU operator.
+  * ingest: the headline scan's 402 ISMRMRD acquisitions assembled into the route's
+    k-space frame in HBM (ksj_gpu_assemble_acquisitions), beside a plain device copy of
+    the same bytes; ingest.cpu_baseline: the oracle's copy loop on one core.
 Every ratio compares like with like: both sides in HBM, or both fed from host memory.
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1]
