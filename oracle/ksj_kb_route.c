@@ -611,3 +611,27 @@ int ora_kb_route3_rss_f32(size_t n, size_t samples, size_t coils, const float* t
   route3_free(&p);
   return rc;
 }
+
+int ora_assemble_acquisitions_f32(const float* packed, const unsigned* counts, size_t acquisitions, size_t channels,
+                                  size_t frames, float* frame, double* seconds) {
+  if (acquisitions == 0 || channels == 0 || channels > 64 || !packed || !counts || !frame) return -1;
+  size_t total = 0;
+  for (size_t a = 0; a < acquisitions; ++a) {
+    if (counts[a] == 0) return -1;
+    total += counts[a];
+  }
+  const double t0 = now_seconds();
+  for (size_t f = 0; f < frames; ++f) {
+    size_t written = 0;        /* samples of the acquisitions before this one */
+    const float* src = packed; /* this acquisition's block */
+    for (size_t a = 0; a < acquisitions; ++a) {
+      const size_t n = counts[a];
+      for (size_t c = 0; c < channels; ++c)
+        memcpy(frame + 2 * (c * total + written), src + 2 * c * n, n * 2 * sizeof(float));
+      src += 2 * channels * n;
+      written += n;
+    }
+  }
+  if (seconds) *seconds = now_seconds() - t0;
+  return 0;
+}

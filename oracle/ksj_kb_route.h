@@ -35,6 +35,15 @@ int ora_cart_route_rss_f32(size_t rows, size_t cols, size_t coils, const float* 
 int ora_kb_route3_rss_f32(size_t n, size_t samples, size_t coils, const float* traj, const float* values, int width,
                           int threads, float* rss, double* coil_seconds);
 
+/* The route's replay copy loop (noncartesian_rss_hdf5.cpp:1077-1083) in C, for the CPU
+ * baseline of the acquisition assembly: acquisitions packed one after another, each
+ * [channels][counts[a]] complex float (ISMRMRD samples[sample + channel * ns]), -> frame
+ * [channels][S] with one memcpy per acquisition and channel, repeated `frames` times on one
+ * thread.  `seconds` (may be NULL) receives the wall time of the repeats.  Returns 0, or -1
+ * on the route's rejections (no acquisition, one without samples, channels outside [1, 64]). */
+int ora_assemble_acquisitions_f32(const float* packed, const unsigned* counts, size_t acquisitions, size_t channels,
+                                  size_t frames, float* frame, double* seconds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -319,8 +319,23 @@ def _route_lib():
         R.ora_cart_route_rss_f32.restype = C.c_int
         R.ora_kb_route3_rss_f32.argtypes = [_sz, _sz, _sz, _fp, _fp, C.c_int, C.c_int, _fp, C.POINTER(C.c_double)]
         R.ora_kb_route3_rss_f32.restype = C.c_int
+        R.ora_assemble_acquisitions_f32.argtypes = [_fp, _up, _sz, _sz, _sz, _fp, C.POINTER(C.c_double)]
+        R.ora_assemble_acquisitions_f32.restype = C.c_int
         _route = R
     return _route
+
+
+def assemble_acquisitions_c(packed, counts, channels: int, frames: int = 1):
+    """The replay's copy loop in C (memcpy per acquisition and channel; the CPU baseline of the
+    acquisition assembly): packed complex64 acquisitions -> (frame [channels][S] complex64, wall
+    seconds of `frames` repeats on one thread), or (None, 0.0) on the route's rejections."""
+    p = np.ascontiguousarray(np.asarray(packed, np.complex64).reshape(-1))
+    cnt = np.ascontiguousarray(np.asarray(counts, np.uint32))
+    out = np.zeros((channels, int(cnt.sum(dtype=np.uint64))), np.complex64)
+    secs = C.c_double(0)
+    rc = _route_lib().ora_assemble_acquisitions_f32(p.view(np.float32), cnt, cnt.size, channels, frames,
+                                                    out.view(np.float32).reshape(-1), C.byref(secs))
+    return (out, secs.value) if rc == 0 else (None, 0.0)
 
 
 def cart_route_rss_f32(kspace_f32, centred: bool = True, scans: int = 1, threads: int = 1, timed: bool = False):

@@ -96,3 +96,13 @@ def test_rows_per_workgroup_key_accepts_1_2_4_only():
     assert lib.ksj_gpu_tune_set(b"assemble_rows", 3) != 0
     assert lib.ksj_gpu_tune_set(b"assemble_rows", 1) == 0
     assert lib.ksj_gpu_tune_set(b"assemble_rows", 4) == 0
+
+
+def test_c_copy_loop_of_the_cpu_baseline_equals_the_oracle():
+    counts, channels = [512, 1, 300, 4096, 7], 6
+    acqs = [(np.arange(n * channels) * (1 + 1j) + a).astype(np.complex64) for a, n in enumerate(counts)]
+    want, _ = O.assemble_acquisitions(acqs, channels)
+    got, secs = O.assemble_acquisitions_c(np.concatenate(acqs), counts, channels, frames=3)
+    assert secs >= 0 and np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert O.assemble_acquisitions_c(np.zeros(8, np.complex64), [4, 0], 1)[0] is None
+    assert O.assemble_acquisitions_c(np.zeros(65, np.complex64), [1], 65)[0] is None
