@@ -211,7 +211,7 @@ def main(argv=None) -> int:
     ap.add_argument("--batch", type=int, default=32, help="scans per step")
     ap.add_argument("--distinct", type=int, default=4, help="distinct seeded scans behind the batch")
     ap.add_argument("--lanes", type=int, default=3, help="server lanes (batches in flight)")
-    ap.add_argument("--max-batch", type=int, default=32, help="scans stacked into one server launch")
+    ap.add_argument("--max-batch", type=int, default=64, help="scans stacked into one server launch")
     ap.add_argument("--load", type=float, default=0.7, help="Poisson arrival rate as a fraction of served rate")
     ap.add_argument("--latency-requests", type=int, default=300)
     ap.add_argument("--mixed-requests", type=int, default=48, help="0 skips the mixed-stream burst section")
